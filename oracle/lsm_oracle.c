@@ -1,0 +1,794 @@
+/*
+ * lsm_oracle.c -- CPU restatement of the reference level-set transport path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP library is
+ * compared against; it is never linked into, loaded by or called from the
+ * product library.  Every routine restates one reference routine, in the same
+ * floating-point operation order (compile with -ffp-contract=off, no
+ * fast-math), and cites the reference file:line it follows.  Its results are
+ * pinned against golden vectors written by the compiled reference Fortran
+ * (tests/golden) and against closed forms (tests/test_oracle_*.py).
+ *
+ * Layout: one block, every face a physical boundary (one MPI rank in the
+ * reference: igp = jgp = kgp = 0), arrays i-fastest with 0-based indices,
+ * node (i,j,k) of the reference = (i-1, j-1, k-1) here.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct {
+    int32_t ni, nj, nk;
+    int32_t btype[6];          /* btype_lsm(1:6): i1, im, j1, jm, k1, km */
+    double dc, de, dz;
+    double big_phi;
+    const double *csi[3], *eta[3], *zet[3];   /* grad xi, grad eta, grad zeta */
+    const double *aj;
+    const double *x, *y, *z;
+} OBlock;
+
+typedef struct {
+    double epslsm;
+    double eps_reinit;          /* epsReinitialisation */
+    double rn_timestep;         /* RNtimestep */
+    int32_t rn_itermax;         /* RNitermax */
+    int32_t order_bc;           /* OrderReinitialisationBoundaries */
+    int32_t eno_bc;             /* ENOBCReinitialisation */
+} OReinit;
+
+#define EPS_SIMS 2.220446049250313e-16   /* epsilon(1._rdf), precision.F90:62 */
+#define PI_REF 3.141592653589793         /* precision.F90:51 */
+
+static inline size_t IX(const OBlock *b, int i, int j, int k)
+{
+    return (size_t)i + (size_t)b->ni * ((size_t)j + (size_t)b->nj * (size_t)k);
+}
+
+static inline double fsign1(double x) { return copysign(1.0, x); }
+
+/* ------------------------------------------------------------------------- */
+/* scalar helpers                                                            */
+/* ------------------------------------------------------------------------- */
+
+/* GetWENO3Reconstruction, AdvectionMethods.F90:195-245 */
+double oracle_weno3(double f0, double f1, double f2, double f3, double ws)
+{
+    const double d0 = 2.0 / 3.0, d1 = 1.0 / 3.0, e = 0.000001;
+    double fL, fC, fR;
+    if (ws > 0.0) { fL = f0; fC = f1; fR = f2; }
+    else          { fL = f3; fC = f2; fR = f1; }
+    double b0 = (fC - fR) * (fC - fR);
+    double b1 = (fL - fC) * (fL - fC);
+    double a0 = d0 / ((b0 + e) * (b0 + e));
+    double a1 = d1 / ((b1 + e) * (b1 + e));
+    double w0 = a0 / (a0 + a1);
+    double w1 = a1 / (a0 + a1);
+    double u0 = 0.5 * fC + 0.5 * fR;
+    double u1 = (-0.5) * fL + 1.5 * fC;
+    return u0 * w0 + u1 * w1;
+}
+
+/* GetENO2Reconstruction, AdvectionMethods.F90:64-191 */
+double oracle_eno2(double f0, double fLL, double fL, double fC, double fR, double fRR,
+                   int left_out, int right_out)
+{
+    double a, b, Mf, dp, dm, s;
+    if (left_out) {
+        a = (fR - 2.0 * fC) + fL;
+        b = (fRR - 2.0 * fR) + fC;
+        Mf = a; if (fabs(b) < fabs(a)) Mf = b;
+        dp = (fR - fC) - 0.5 * Mf;
+        a = (fR - 2.0 * fC) + fL;
+        Mf = a;
+        dm = (fC - fL) + 0.5 * Mf;
+    } else if (right_out) {
+        a = (fR - 2.0 * fC) + fL;
+        Mf = a;
+        dp = (fR - fC) - 0.5 * Mf;
+        a = (fR - 2.0 * fC) + fL;
+        b = (fC - 2.0 * fL) + fLL;
+        Mf = a; if (fabs(b) < fabs(a)) Mf = b;
+        dm = (fC - fL) + 0.5 * Mf;
+    } else {
+        a = (fR - 2.0 * fC) + fL;
+        b = (fRR - 2.0 * fR) + fC;
+        Mf = a; if (fabs(b) < fabs(a)) Mf = b;
+        dp = (fR - fC) - 0.5 * Mf;
+        a = (fR - 2.0 * fC) + fL;
+        b = (fC - 2.0 * fL) + fLL;
+        Mf = a; if (fabs(b) < fabs(a)) Mf = b;
+        dm = (fC - fL) + 0.5 * Mf;
+    }
+    s = (fabs(f0) > EPS_SIMS) ? fsign1(f0) : 0.0;
+    if (s * (fR - fC) < 0.0 && s * (fC - fL) < -(s * (fR - fC))) return dp;
+    if (s * (fC - fL) > 0.0 && s * (fR - fC) > -(s * (fC - fL))) return dm;
+    return 0.5 * (dp + dm);
+}
+
+/* ENO2Boundary, AdvectionMethods.F90:654-707 */
+static double eno2_boundary(double f0, double f1, double f2, int dir)
+{
+    double as = (double)dir;
+    double D1 = as * (f1 - f0);
+    double D2 = as * (f2 - f1);
+    double D3 = as * (f2 - f0) / 2.0;
+    if (fabs(D1 - D2) < EPS_SIMS) return D1;
+    if (fabs(D1) < fabs(D2)) return (2.0 * D1 + D2) / 3.0;
+    if (fabs(D2) < fabs(D1)) return (D1 + 2.0 * D2) / 3.0;
+    if (fabs(D1) == fabs(D2)) return (4.0 * D1 + D2) / 5.0;
+    if (fabs(D1) < fabs(D3) && fabs(D2) < fabs(D3)) return ((3.0 * D1 + 6.0 * D2) - D3) / 7.0;
+    if (fabs(D1) >= fabs(D3) && fabs(D2) >= fabs(D3)) return D3;
+    return D1;
+}
+
+/* BiasedDerivative, AdvectionMethods.F90:248-302 */
+double oracle_biased(double f0, double f1, double f2, double f3, int order, int dir, int enobc)
+{
+    double as = (double)dir, r = 0.0;
+    if (order == 1) r = as * (-(1.0 * f0) + 1.0 * f1);
+    if (order == 2) {
+        r = as * ((-(3.0 * f0) + 4.0 * f1) - 1.0 * f2) / 2.0;
+        if (enobc) r = eno2_boundary(f0, f1, f2, dir);
+    }
+    if (order == 3) r = as * (((-(11.0 * f0) + 18.0 * f1) - 9.0 * f2) + 2.0 * f3) / 6.0;
+    return r;
+}
+
+/* GetSignPhi0, AdvectionMethods.F90:576-587 */
+static double sign_phi0(double p, double eps)
+{
+    if (fabs(p) > eps) return fsign1(p);
+    return p / eps - (1.0 / PI_REF) * sin((PI_REF * p) / eps);
+}
+
+/* deltaf, global_lsm.F90:120-133 */
+static double deltaf(double v, double eps)
+{
+    if (fabs(v) > eps) return 0.0;
+    return (1.0 / (2.0 * eps)) * (1.0 + cos((PI_REF * v) / eps));
+}
+
+/* GetLambdaMassCorrection, global_lsm.F90:200-219 */
+static double lambda_mass(double p, double s, double g, double eps)
+{
+    double num = -((deltaf(p, eps) * s) * (1.0 - g));
+    double del = deltaf(p, eps);
+    double den = (del * del) * g;
+    if (fabs(den) < EPS_SIMS) return 0.0;
+    return num / den;
+}
+
+/* ------------------------------------------------------------------------- */
+/* R2: rhs_contra_j, rhs_contra_j.F90:17-48                                   */
+/* ------------------------------------------------------------------------- */
+void oracle_contra(const OBlock *b, const double *u, const double *v, const double *w,
+                   double *U1, double *U2, double *U3)
+{
+    for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+    for (int i = 0; i < b->ni; ++i) {
+        size_t l = IX(b, i, j, k);
+        double tmp = 1.0 / b->aj[l];
+        U1[l] = tmp * ((b->csi[0][l] * u[l] + b->csi[1][l] * v[l]) + b->csi[2][l] * w[l]);
+        U2[l] = tmp * ((b->eta[0][l] * u[l] + b->eta[1][l] * v[l]) + b->eta[2][l] * w[l]);
+        U3[l] = tmp * ((b->zet[0][l] * u[l] + b->zet[1][l] * v[l]) + b->zet[2][l] * w[l]);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* R4: calc_RH_AD_WENO3, calc_RH_AD_WENO3.F90:86-403 (no blanking)            */
+/* writes interior nodes only; the caller zeroes rh (ad_RKTVD3.F90:55)        */
+/* ------------------------------------------------------------------------- */
+void oracle_weno3_rhs(const OBlock *b, const double *phi, const double *U1,
+                      const double *U2, const double *U3, const int32_t *mask, double *rh)
+{
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    const size_t si = 1, sj = (size_t)ni, sk = (size_t)ni * nj;
+    for (int i = 1; i <= ni - 2; ++i)
+    for (int j = 1; j <= nj - 2; ++j)
+    for (int k = 1; k <= nk - 2; ++k) {
+        size_t l = IX(b, i, j, k);
+        if (mask[l] == 0) continue;
+        const int mb = (i == 1), mf = (i == ni - 2);
+        const int ml = (j == 1), mr = (j == nj - 2);
+        const int md = (k == 1), mu = (k == nk - 2);
+        const int bnd[3][2] = {{mb, mf}, {ml, mr}, {md, mu}};
+        const size_t st[3] = {si, sj, sk};
+        const double *Uc[3] = {U1, U2, U3};
+        const double dd[3] = {b->dc, b->de, b->dz};
+        double L[3];
+        for (int d = 0; d < 3; ++d) {
+            const size_t s = st[d];
+            double ws = Uc[d][l];
+            double p0 = phi[l - s], p1 = phi[l], p2 = phi[l + s], p3;
+            if (bnd[d][1]) { ws = 1.0; p3 = 0.0; } else p3 = phi[l + 2 * s];
+            double fp = oracle_weno3(p0, p1, p2, p3, ws);
+            ws = Uc[d][l];
+            double q0;
+            if (bnd[d][0]) { ws = -1.0; q0 = 0.0; } else q0 = phi[l - 2 * s];
+            double fm = oracle_weno3(q0, phi[l - s], phi[l], phi[l + s], ws);
+            ws = Uc[d][l];
+            L[d] = ws * (dd[d] * (fp - fm));
+        }
+        rh[l] = (L[0] + L[1]) + L[2];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* R6: bcond_lsm, bcond_lsm.F90:26-287 (faces in order i1, im, j1, jm, k1, km) */
+/* ------------------------------------------------------------------------- */
+static void bc_face_pair(double *phi, size_t lb, size_t l1, size_t l2, int bt, double big, int is_im)
+{
+    const double a2 = 4.0 / 3.0, b2 = -1.0 / 3.0, a1 = 1.0, b1 = 0.0;
+    if (bt == 0) {
+        phi[lb] = phi[l1];
+    } else if (bt == 1) {
+        double ex = (1.0 + fsign1(fabs(phi[l2]) - big)) / 2.0;
+        double ta = a2 + ex * (a1 - a2);
+        double tb = b2 + ex * (b1 - b2);
+        phi[lb] = ta * phi[l1] + tb * phi[l2];
+    } else if (bt == 2 && is_im) {
+        phi[lb] = 2.0 * phi[l1] - 1.0 * phi[l2];
+    }
+}
+
+void oracle_bcond(const OBlock *b, double *phi)
+{
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    for (int k = 0; k < nk; ++k) for (int j = 0; j < nj; ++j)
+        bc_face_pair(phi, IX(b, 0, j, k), IX(b, 1, j, k), IX(b, 2, j, k), b->btype[0], b->big_phi, 0);
+    for (int k = 0; k < nk; ++k) for (int j = 0; j < nj; ++j)
+        bc_face_pair(phi, IX(b, ni - 1, j, k), IX(b, ni - 2, j, k), IX(b, ni - 3, j, k), b->btype[1], b->big_phi, 1);
+    for (int k = 0; k < nk; ++k) for (int i = 0; i < ni; ++i)
+        bc_face_pair(phi, IX(b, i, 0, k), IX(b, i, 1, k), IX(b, i, 2, k), b->btype[2], b->big_phi, 0);
+    for (int k = 0; k < nk; ++k) for (int i = 0; i < ni; ++i)
+        bc_face_pair(phi, IX(b, i, nj - 1, k), IX(b, i, nj - 2, k), IX(b, i, nj - 3, k), b->btype[3], b->big_phi, 0);
+    for (int j = 0; j < nj; ++j) for (int i = 0; i < ni; ++i)
+        bc_face_pair(phi, IX(b, i, j, 0), IX(b, i, j, 1), IX(b, i, j, 2), b->btype[4], b->big_phi, 0);
+    for (int j = 0; j < nj; ++j) for (int i = 0; i < ni; ++i)
+        bc_face_pair(phi, IX(b, i, j, nk - 1), IX(b, i, j, nk - 2), IX(b, i, j, nk - 3), b->btype[5], b->big_phi, 0);
+}
+
+/* ------------------------------------------------------------------------- */
+/* R3: ad_RKTVD3, ad_RKTVD3.F90:35-86                                         */
+/* ------------------------------------------------------------------------- */
+void oracle_rk3(const OBlock *b, double *phi_np1, const double *phi_n, const int32_t *mask,
+                const double *U1, const double *U2, const double *U3, double dt)
+{
+    const size_t n = (size_t)b->ni * b->nj * b->nk;
+    const double c[3][3] = {{1.0, 0.0, 1.0},
+                            {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
+                            {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    double *rh = (double *)malloc(n * sizeof(double));
+    memcpy(phi_np1, phi_n, n * sizeof(double));
+    for (int s = 0; s < 3; ++s) {
+        memset(rh, 0, n * sizeof(double));
+        oracle_weno3_rhs(b, phi_np1, U1, U2, U3, mask, rh);
+        for (size_t l = 0; l < n; ++l)
+            if (mask[l] > 0)
+                phi_np1[l] = (c[s][0] * phi_n[l] + c[s][1] * phi_np1[l])
+                           - ((c[s][2] * b->aj[l]) * dt) * rh[l];
+        oracle_bcond(b, phi_np1);
+    }
+    free(rh);
+}
+
+/* ------------------------------------------------------------------------- */
+/* R10: calc_NormGrad_d, calc_NormGrad_d.F90:39-2446                          */
+/* ------------------------------------------------------------------------- */
+void oracle_normgrad(const OBlock *b, const double *d, const double *phi0, double *ng,
+                     int order_bc, int eno_bc)
+{
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    const int n[3] = {ni, nj, nk};
+    const size_t st[3] = {1, (size_t)ni, (size_t)ni * nj};
+    const double dd[3] = {b->dc, b->de, b->dz};
+    /* i = 1 / i = im face blocks never assign phi0, so they use the value left
+       by the last interior node of the interior loop (calc_NormGrad_d.F90:91,
+       187-367): node (im-1, jm-1, km-1). */
+    const double phi0_stale = phi0[IX(b, ni - 2, nj - 2, nk - 2)];
+    for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+    for (int i = 0; i < ni; ++i) {
+        const int idx[3] = {i, j, k};
+        const size_t l = IX(b, i, j, k);
+        int on[3];
+        for (int a = 0; a < 3; ++a) on[a] = (idx[a] == 0 || idx[a] == n[a] - 1);
+        const double p0 = (on[0] && !on[1] && !on[2]) ? phi0_stale : phi0[l];
+        double g[3];
+        for (int a = 0; a < 3; ++a) {
+            const size_t s = st[a];
+            if (on[a]) {
+                const int dir = (idx[a] == 0) ? 1 : -1;
+                if (a == 0 && i == 0 && k == 0 && !on[1]) {
+                    /* edge i = 1, k = 1 (calc_NormGrad_d.F90:1281-1283) */
+                    const double dc2 = 0.5 * b->dc;
+                    g[a] = dc2 * ((-(1.0 * d[l + 2]) + 4.0 * d[l + 1]) - 3.0 * d[l]);
+                } else {
+                    const long ss = (long)s * dir;
+                    g[a] = dd[a] * oracle_biased(d[l], d[l + ss], d[l + 2 * ss], d[l + 3 * ss],
+                                                 order_bc, dir, eno_bc);
+                }
+            } else {
+                const int lo = (idx[a] == 1), hi = (idx[a] == n[a] - 2);
+                const double fLL = lo ? 0.0 : d[l - 2 * s];
+                const double fRR = hi ? 0.0 : d[l + 2 * s];
+                g[a] = dd[a] * oracle_eno2(p0, fLL, d[l - s], d[l], d[l + s], fRR, lo, hi);
+            }
+        }
+        const double gx = (b->csi[0][l] * g[0] + b->eta[0][l] * g[1]) + b->zet[0][l] * g[2];
+        const double gy = (b->csi[1][l] * g[0] + b->eta[1][l] * g[1]) + b->zet[1][l] * g[2];
+        const double gz = (b->csi[2][l] * g[0] + b->eta[2][l] * g[1]) + b->zet[2][l] * g[2];
+        ng[l] = sqrt((gx * gx + gy * gy) + gz * gz);
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* R8: reinitialisation_test2, reinitialisation_test2.F90:42-210 (non-hybrid) */
+/* returns the number of pseudo-time iterations that ran                      */
+/* ------------------------------------------------------------------------- */
+int oracle_reinit_eno2(const OBlock *b, const OReinit *p, const double *phi0, double *out)
+{
+    const size_t n = (size_t)b->ni * b->nj * b->nk;
+    const double c[3][3] = {{1.0, 0.0, 1.0},
+                            {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
+                            {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    double *s = (double *)malloc(n * sizeof(double));
+    double *dn = (double *)malloc(n * sizeof(double));
+    double *ng = (double *)malloc(n * sizeof(double));
+    double *ngc = (double *)malloc(n * sizeof(double));
+    for (size_t l = 0; l < n; ++l) s[l] = sign_phi0(phi0[l], p->epslsm);
+    memcpy(out, phi0, n * sizeof(double));
+    oracle_normgrad(b, out, phi0, ng, p->order_bc, p->eno_bc);
+    int it;
+    for (it = 0; it < p->rn_itermax; ++it) {
+        double mx = ng[0], mn = ng[0];
+        for (size_t l = 1; l < n; ++l) { if (ng[l] > mx) mx = ng[l]; if (ng[l] < mn) mn = ng[l]; }
+        if (fabs(mx - 1.0) < p->eps_reinit && fabs(mn - 1.0) < p->eps_reinit) break;
+        memcpy(dn, out, n * sizeof(double));
+        oracle_normgrad(b, dn, phi0, ngc, p->order_bc, p->eno_bc);
+        for (int r = 0; r < 3; ++r) {
+            oracle_normgrad(b, out, phi0, ng, p->order_bc, p->eno_bc);
+            for (size_t l = 0; l < n; ++l) {
+                if (ngc[l] - 1.0 > p->eps_reinit) {
+                    double rhs = s[l] * (ng[l] - 1.0);
+                    double lam = lambda_mass(out[l], s[l], ng[l], p->epslsm);
+                    double aux = deltaf(out[l], p->epslsm) * ng[l];
+                    rhs = rhs - lam * aux;
+                    out[l] = (c[r][0] * dn[l] + c[r][1] * out[l]) - (c[r][2] * p->rn_timestep) * rhs;
+                }
+            }
+        }
+    }
+    free(s); free(dn); free(ng); free(ngc);
+    return it;
+}
+
+/* ------------------------------------------------------------------------- */
+/* one-sided differences with the boundary copies of calc_sign_ini.F90:41-176 */
+/* and calc_RH_RN.F90:35-175; Godunov |grad| of calc_sign_ini.F90:199-250     */
+/* ------------------------------------------------------------------------- */
+static void godunov_norm(const OBlock *b, const double *f, size_t l, int i, int j, int k,
+                         double *ng_plus, double *ng_minus)
+{
+    const int n[3] = {b->ni, b->nj, b->nk};
+    const int idx[3] = {i, j, k};
+    const size_t st[3] = {1, (size_t)b->ni, (size_t)b->ni * b->nj};
+    const double dd[3] = {b->dc, b->de, b->dz};
+    double mas[3], men[3];
+    for (int a = 0; a < 3; ++a) {
+        const size_t s = st[a];
+        if (idx[a] == 0) {
+            mas[a] = dd[a] * (f[l + s] - f[l]);
+            men[a] = mas[a];
+        } else if (idx[a] == n[a] - 1) {
+            men[a] = dd[a] * (f[l] - f[l - s]);
+            mas[a] = men[a];
+        } else {
+            mas[a] = dd[a] * (f[l + s] - f[l]);
+            men[a] = dd[a] * (f[l] - f[l - s]);
+        }
+    }
+    double db[3], df[3];
+    for (int c = 0; c < 3; ++c) {
+        db[c] = (b->csi[c][l] * men[0] + b->eta[c][l] * men[1]) + b->zet[c][l] * men[2];
+        df[c] = (b->csi[c][l] * mas[0] + b->eta[c][l] * mas[1]) + b->zet[c][l] * mas[2];
+    }
+    double bm[3], bn[3], fm[3], fn[3];
+    for (int c = 0; c < 3; ++c) {
+        double a = db[c] / 2.0; bm[c] = a + fabs(a); bn[c] = a - fabs(a);
+        a = df[c] / 2.0;        fm[c] = a + fabs(a); fn[c] = a - fabs(a);
+    }
+#define MAX2(p, q) ((p) > (q) ? (p) : (q))
+    *ng_plus = sqrt((MAX2(bm[0] * bm[0], fn[0] * fn[0]) + MAX2(bm[1] * bm[1], fn[1] * fn[1]))
+                    + MAX2(bm[2] * bm[2], fn[2] * fn[2]));
+    *ng_minus = sqrt((MAX2(bn[0] * bn[0], fm[0] * fm[0]) + MAX2(bn[1] * bn[1], fm[1] * fm[1]))
+                     + MAX2(bn[2] * bn[2], fm[2] * fm[2]));
+#undef MAX2
+}
+
+/* R13: calc_sign_ini, calc_sign_ini.F90:1-280 */
+void oracle_sign_ini(const OBlock *b, const double *phi0, double *sgn, double *ng,
+                     double epslsm, int m_sphi0)
+{
+    for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+    for (int i = 0; i < b->ni; ++i) {
+        size_t l = IX(b, i, j, k);
+        double gp, gm, g;
+        godunov_norm(b, phi0, l, i, j, k, &gp, &gm);
+        const double p = phi0[l];
+        if (p > 0.0) g = gp; else if (p < 0.0) g = gm; else g = 1.0;
+        if (m_sphi0 == 1) {
+            if (p >= epslsm) sgn[l] = 1.0;
+            else if (p <= (-1.0) * epslsm) sgn[l] = -1.0;
+            else sgn[l] = p / epslsm - (1.0 / PI_REF) * sin((PI_REF * p) / epslsm);
+        } else {
+            sgn[l] = p / sqrt(p * p + (g * epslsm) * (g * epslsm));
+        }
+        ng[l] = g;
+    }
+}
+
+/* R14: calc_RH_RN, calc_RH_RN.F90:1-240 */
+void oracle_rh_rn(const OBlock *b, const double *d, const double *sgn, double *rh)
+{
+    for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+    for (int i = 0; i < b->ni; ++i) {
+        size_t l = IX(b, i, j, k);
+        double gp, gm, g;
+        godunov_norm(b, d, l, i, j, k, &gp, &gm);
+        if (sgn[l] > 0.0) g = gp; else if (sgn[l] < 0.0) g = gm; else g = 1.0;
+        rh[l] = sgn[l] * (g - 1.0);
+    }
+}
+
+/* R15: calc_lambda_RN, method 0, calc_lambda_RN.F90:42-63 */
+void oracle_lambda_rn(const OBlock *b, const double *phi0, const double *ng0, const double *d,
+                      double *lam, double dt, double epslsm)
+{
+    const size_t n = (size_t)b->ni * b->nj * b->nk;
+    for (size_t l = 0; l < n; ++l) {
+        double vol = ((1.0 / b->dc) * (1.0 / b->de)) * (1.0 / b->dz);
+        double Lop = deltaf(phi0[l], epslsm) * ((1.0 / dt) * (d[l] - phi0[l]));
+        double vn = vol * Lop;
+        double del = deltaf(phi0[l], epslsm);
+        double vd = (vol * (del * del)) * ng0[l];
+        lam[l] = (vd == 0.0) ? 0.0 : -(vn / vd);
+    }
+}
+
+/* R12: reinitialization_benchmark, reinitialization_benchmark.F90:45-222    */
+/* (M_Grad = 0, no obstacles, non-hybrid)                                    */
+void oracle_reinit_godunov(const OBlock *b, const OReinit *p, int m_sphi0, int m_ti,
+                           const double *phi0, double *out)
+{
+    const size_t n = (size_t)b->ni * b->nj * b->nk;
+    const double c[3][3] = {{1.0, 0.0, 1.0},
+                            {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
+                            {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    const double dt = p->rn_timestep;
+    double *sn = (double *)malloc(n * sizeof(double));
+    double *sg = (double *)malloc(n * sizeof(double));
+    double *ng0 = (double *)malloc(n * sizeof(double));
+    double *rh = (double *)malloc(n * sizeof(double));
+    double *lam = (double *)malloc(n * sizeof(double));
+    memcpy(sn, phi0, n * sizeof(double));
+    memcpy(out, phi0, n * sizeof(double));
+    oracle_sign_ini(b, phi0, sg, ng0, p->epslsm, m_sphi0);
+    for (int it = 0; it < p->rn_itermax; ++it) {
+        if (m_ti == 1) {
+            for (int r = 0; r < 3; ++r) {
+                oracle_rh_rn(b, out, sg, rh);
+                for (size_t l = 0; l < n; ++l)
+                    out[l] = (c[r][0] * sn[l] + c[r][1] * out[l]) - (c[r][2] * dt) * rh[l];
+            }
+        } else {
+            oracle_rh_rn(b, out, sg, rh);
+            for (size_t l = 0; l < n; ++l) out[l] = sn[l] - dt * rh[l];
+        }
+        oracle_lambda_rn(b, phi0, ng0, out, lam, dt, p->epslsm);
+        for (size_t l = 0; l < n; ++l)
+            out[l] = out[l] + ((dt * lam[l]) * deltaf(phi0[l], p->epslsm)) * ng0[l];
+        memcpy(sn, out, n * sizeof(double));
+    }
+    free(sn); free(sg); free(ng0); free(rh); free(lam);
+}
+
+/* ------------------------------------------------------------------------- */
+/* R16: TotalWaterVolume + TetrahedronMethods                                */
+/* ------------------------------------------------------------------------- */
+
+/* M44Det, TetrahedronMethods.F90:1698-1720 (A is row-major 4x4, A[r][c]) */
+static double m44det(const double A[4][4])
+{
+#define a(r, c) A[(r) - 1][(c) - 1]
+    return ((a(1,1)*((a(2,2)*(a(3,3)*a(4,4)-a(3,4)*a(4,3))+a(2,3)*(a(3,4)*a(4,2)-a(3,2)*a(4,4)))+a(2,4)*(a(3,2)*a(4,3)-
+            a(3,3)*a(4,2)))-a(1,2)*((a(2,1)*(a(3,3)*a(4,4)-a(3,4)*a(4,3))+a(2,3)*(a(3,4)*a(4,1)-a(3,1)*a(4,4)))+
+            a(2,4)*(a(3,1)*a(4,3)-a(3,3)*a(4,1))))+a(1,3)*((a(2,1)*(a(3,2)*a(4,4)-a(3,4)*a(4,2))+a(2,2)*(a(3,4)*a(4,1)-
+            a(3,1)*a(4,4)))+a(2,4)*(a(3,1)*a(4,2)-a(3,2)*a(4,1))))-a(1,4)*((a(2,1)*(a(3,2)*a(4,3)-a(3,3)*a(4,2))+
+            a(2,2)*(a(3,3)*a(4,1)-a(3,1)*a(4,3)))+a(2,3)*(a(3,1)*a(4,2)-a(3,2)*a(4,1)));
+#undef a
+}
+
+/* GetTetrahedronVolume, TetrahedronMethods.F90:666-688 */
+static double tet_volume(const double v[4][3])
+{
+    double A[4][4];
+    for (int r = 0; r < 4; ++r) { A[r][0] = v[r][0]; A[r][1] = v[r][1]; A[r][2] = v[r][2]; A[r][3] = 1.0; }
+    return (1.0 / 6.0) * fabs(m44det(A));
+}
+
+static void cross3(const double *u, const double *v, double *o)
+{
+    o[0] = u[1] * v[2] - u[2] * v[1];
+    o[1] = u[2] * v[0] - u[0] * v[2];
+    o[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+static double dot3(const double *u, const double *v) { return (u[0] * v[0] + u[1] * v[1]) + u[2] * v[2]; }
+
+/* HexahedronVolume (Davies & Salmond), TetrahedronMethods.F90:1724-1793 */
+static double hex_volume(const double V[8][3])
+{
+    const double *A = V[0], *B = V[1], *C = V[2], *D = V[3], *E = V[4], *F = V[5], *G = V[6], *H = V[7];
+    double AC[3], AF[3], AH[3], BE[3], CH[3], DB[3], ED[3], FC[3];
+    double GA[3], GB[3], GC[3], GD[3], GE[3], GF[3], GH[3], HF[3];
+    for (int c = 0; c < 3; ++c) {
+        AC[c] = C[c] - A[c]; AF[c] = F[c] - A[c]; AH[c] = H[c] - A[c];
+        BE[c] = E[c] - B[c]; CH[c] = H[c] - C[c]; DB[c] = B[c] - D[c];
+        ED[c] = D[c] - E[c]; FC[c] = C[c] - F[c];
+        GA[c] = A[c] - G[c]; GB[c] = B[c] - G[c]; GC[c] = C[c] - G[c]; GD[c] = D[c] - G[c];
+        GE[c] = E[c] - G[c]; GF[c] = F[c] - G[c]; GH[c] = H[c] - G[c]; HF[c] = F[c] - H[c];
+    }
+    double t[3], s[3], acc;
+    for (int c = 0; c < 3; ++c) s[c] = GA[c] + GB[c];
+    cross3(DB, AC, t); acc = dot3(s, t);
+    for (int c = 0; c < 3; ++c) s[c] = GA[c] + GE[c];
+    cross3(BE, AF, t); acc = acc + dot3(s, t);
+    for (int c = 0; c < 3; ++c) s[c] = GA[c] + GD[c];
+    cross3(ED, AH, t); acc = acc + dot3(s, t);
+    cross3(HF, GE, t); acc = acc + dot3(GF, t);
+    cross3(CH, GD, t); acc = acc + dot3(GH, t);
+    cross3(FC, GB, t); acc = acc + dot3(GC, t);
+    return -((1.0 / 12.0) * acc);
+}
+
+/* edges of a tetrahedron (0-based vertices), TetrahedronMethods.F90:71-79 */
+static const int TET_EDGE[6][2] = {{0, 1}, {1, 2}, {2, 0}, {0, 3}, {1, 3}, {2, 3}};
+/* edge-intersection flags per case, TetrahedronMethods.F90:130-135 */
+static const int TET_EDGE_FLAGS[16] = {0x00, 0x0d, 0x13, 0x1e, 0x26, 0x2b, 0x35, 0x38,
+                                      0x38, 0x35, 0x2b, 0x26, 0x1e, 0x13, 0x0d, 0x00};
+
+/* GetWaterVolume, TetrahedronMethods.F90:710-954 */
+static double water_volume_cut(const double v[4][3], const double E[6][3], int flag)
+{
+    const int cs = flag + 1;
+    const double T = tet_volume(v);
+    double t1[4][3], t2[4][3], t3[4][3];
+#define CP(dst, src) do { (dst)[0] = (src)[0]; (dst)[1] = (src)[1]; (dst)[2] = (src)[2]; } while (0)
+    switch (cs) {
+    case 1: return T;
+    case 16: return 0.0;
+    case 2: case 15:
+        CP(t1[0], v[0]); CP(t1[1], E[0]); CP(t1[2], E[2]); CP(t1[3], E[3]);
+        return cs == 2 ? T - tet_volume(t1) : tet_volume(t1);
+    case 3: case 14:
+        CP(t1[0], v[1]); CP(t1[1], E[0]); CP(t1[2], E[1]); CP(t1[3], E[4]);
+        return cs == 3 ? T - tet_volume(t1) : tet_volume(t1);
+    case 5: case 12:
+        CP(t1[0], v[2]); CP(t1[1], E[1]); CP(t1[2], E[2]); CP(t1[3], E[5]);
+        return cs == 5 ? T - tet_volume(t1) : tet_volume(t1);
+    case 8: case 9:
+        CP(t1[0], v[3]); CP(t1[1], E[3]); CP(t1[2], E[4]); CP(t1[3], E[5]);
+        return cs == 8 ? tet_volume(t1) : T - tet_volume(t1);
+    case 4: case 13: {
+        CP(t1[0], v[0]); CP(t1[1], E[2]); CP(t1[2], E[3]); CP(t1[3], E[4]);
+        CP(t2[0], v[0]); CP(t2[1], E[1]); CP(t2[2], E[2]); CP(t2[3], E[4]);
+        CP(t3[0], v[0]); CP(t3[1], v[1]); CP(t3[2], E[1]); CP(t3[3], E[4]);
+        double aux = (tet_volume(t1) + tet_volume(t2)) + tet_volume(t3);
+        return cs == 4 ? T - aux : aux;
+    }
+    case 6: case 11: {
+        CP(t1[0], v[0]); CP(t1[1], E[0]); CP(t1[2], E[1]); CP(t1[3], E[5]);
+        CP(t2[0], v[0]); CP(t2[1], E[0]); CP(t2[2], E[3]); CP(t2[3], E[5]);
+        CP(t3[0], v[0]); CP(t3[1], v[2]); CP(t3[2], E[1]); CP(t3[3], E[5]);
+        double aux = (tet_volume(t1) + tet_volume(t2)) + tet_volume(t3);
+        return cs == 6 ? T - aux : aux;
+    }
+    case 7: case 10: {
+        CP(t1[0], v[1]); CP(t1[1], E[0]); CP(t1[2], E[2]); CP(t1[3], E[5]);
+        CP(t2[0], v[1]); CP(t2[1], E[0]); CP(t2[2], E[4]); CP(t2[3], E[5]);
+        CP(t3[0], v[1]); CP(t3[1], v[2]); CP(t3[2], E[2]); CP(t3[3], E[5]);
+        double aux = (tet_volume(t1) + tet_volume(t2)) + tet_volume(t3);
+        return cs == 7 ? T - aux : aux;
+    }
+    }
+#undef CP
+    return 0.0;
+}
+
+/* water volume of MarchingTetrahedron, TetrahedronMethods.F90:26-664 */
+static double marching_tet_water(const double v[4][3], const double ph[4])
+{
+    const double tiny = 2.2250738585072014e-308;   /* tiny(eps_sims) */
+    int sg[4], npos = 0, nneg = 0, nzero = 0;
+    for (int q = 0; q < 4; ++q) {
+        sg[q] = 0;
+        if (ph[q] > tiny) sg[q] = 1;
+        if (ph[q] < -tiny) sg[q] = -1;
+        npos += sg[q] == 1; nneg += sg[q] == -1; nzero += sg[q] == 0;
+    }
+    if (npos == 4) return tet_volume(v);
+    if (nneg == 4) return 0.0;
+    int flag = 0;
+    if (nzero > 0 && (nneg == 0 || npos == 0)) {
+        /* isosurface through 1, 2 or 3 vertices: the tetrahedron is full when
+           the first listed non-zero vertex of the matching case is positive */
+        if (nzero == 1) {
+            if (sg[0] == 0) return sg[1] == 1 ? tet_volume(v) : 0.0;
+            if (sg[1] == 0) return sg[0] == 1 ? tet_volume(v) : 0.0;
+            if (sg[2] == 0) return sg[0] == 1 ? tet_volume(v) : 0.0;
+            return sg[0] == 1 ? tet_volume(v) : 0.0;
+        }
+        if (nzero == 2) {
+            if (sg[0] == 0 && sg[1] == 0) return sg[2] == 1 ? tet_volume(v) : 0.0;
+            if (sg[1] == 0 && sg[2] == 0) return sg[0] == 1 ? tet_volume(v) : 0.0;
+            if (sg[2] == 0 && sg[0] == 0) return sg[1] == 1 ? tet_volume(v) : 0.0;
+            if (sg[0] == 0 && sg[3] == 0) return sg[1] == 1 ? tet_volume(v) : 0.0;
+            if (sg[1] == 0 && sg[3] == 0) return sg[0] == 1 ? tet_volume(v) : 0.0;
+            return sg[0] == 1 ? tet_volume(v) : 0.0;
+        }
+        if (nzero == 3) {
+            for (int q = 0; q < 4; ++q)
+                if (sg[q] != 0) return sg[q] == 1 ? tet_volume(v) : 0.0;
+        }
+        /* four zero vertices: no case matches, FlagIndex stays 0 */
+        return tet_volume(v);
+    }
+    for (int q = 0; q < 4; ++q) if (sg[q] == -1) flag |= 1 << q;
+    if (flag == 0) return tet_volume(v);
+    if (flag == 15) return 0.0;
+    const int ef = TET_EDGE_FLAGS[flag];
+    double E[6][3];
+    for (int e = 0; e < 6; ++e) { E[e][0] = E[e][1] = E[e][2] = -9.9; }
+    for (int e = 0; e < 6; ++e) {
+        if (!(ef & (1 << e))) continue;
+        const int s0 = TET_EDGE[e][0], s1 = TET_EDGE[e][1];
+        const double den = ph[s1] - ph[s0];
+        double s = 0.0;
+        if (!(fabs(den) < EPS_SIMS * EPS_SIMS * EPS_SIMS)) s = ph[s1] / den;
+        for (int c = 0; c < 3; ++c) E[e][c] = v[s1][c] + s * (v[s0][c] - v[s1][c]);
+    }
+    return water_volume_cut(v, E, flag);
+}
+
+/* tetrahedra of a cell, TriangulationCase 1, TetrahedronMethods.F90:1929-1936 */
+static const int CELL_TETS[6][4] = {{0, 5, 1, 6}, {0, 1, 2, 6}, {0, 2, 3, 6},
+                                    {0, 3, 7, 6}, {0, 7, 4, 6}, {0, 4, 5, 6}};
+
+/* CellAnalysis, TetrahedronMethods.F90:1823-2121 (volumes only) */
+static void cell_analysis(const double ph[8], const double V[8][3], double *pc_vol, double *st_vol)
+{
+    int pcell = 0;
+    *pc_vol = 0.0; *st_vol = 0.0;
+    for (int q = 0; q < 8; ++q) if (fabs(ph[q]) < EPS_SIMS) pcell = 1;
+    if (!pcell) {
+        double sgn = fsign1(ph[0]);
+        for (int q = 1; q < 8; ++q) if (sgn * ph[q] < 0.0) { pcell = 1; break; }
+    }
+    if (!pcell) return;
+    for (int t = 0; t < 6; ++t) {
+        int pct = 0, second = 1;
+        double pa = ph[CELL_TETS[t][0]];
+        if (pa <= EPS_SIMS) second = 0;
+        if (fabs(pa) < EPS_SIMS) pct = 1;
+        double sgn = fsign1(pa);
+        for (int q = 1; q < 4; ++q) {
+            pa = ph[CELL_TETS[t][q]];
+            if (sgn * pa < 0.0 || fabs(pa) < EPS_SIMS) {
+                if (pa <= EPS_SIMS) second = 0;
+                pct = 1;
+            }
+        }
+        double tv[4][3], tp[4];
+        for (int q = 0; q < 4; ++q) {
+            const int vq = CELL_TETS[t][q];
+            tv[q][0] = V[vq][0]; tv[q][1] = V[vq][1]; tv[q][2] = V[vq][2];
+            tp[q] = ph[vq];
+        }
+        if (pct) *pc_vol = *pc_vol + marching_tet_water(tv, tp);
+        if (second && !pct) *st_vol = *st_vol + tet_volume(tv);
+    }
+}
+
+/* TotalWaterVolume, TotalWaterVolume.F90:69-294 (no blanking).  vol[0..2] =
+   (K-tetrahedra, second tetrahedra, bulk cells); mask = AdvectionNodes. */
+void oracle_water_volume(const OBlock *b, const double *phi, double vol[3], int32_t *mask)
+{
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    const size_t n = (size_t)ni * nj * nk;
+    if (mask) memset(mask, 0, n * sizeof(int32_t));
+    double vk = 0.0, vs = 0.0, vb = 0.0;
+    static const int OFF[8][3] = {{0,0,0},{1,0,0},{1,1,0},{0,1,0},{0,0,1},{1,0,1},{1,1,1},{0,1,1}};
+    for (int k = 0; k <= nk - 2; ++k)
+    for (int j = 0; j <= nj - 2; ++j)
+    for (int i = 0; i <= ni - 2; ++i) {
+        const int km = k - 1 < 0 ? 0 : k - 1, kp = k + 1 > nk - 1 ? nk - 1 : k + 1;
+        const int jm = j - 1 < 0 ? 0 : j - 1, jp = j + 1 > nj - 1 ? nj - 1 : j + 1;
+        const int im = i - 1 < 0 ? 0 : i - 1, ip = i + 1 > ni - 1 ? ni - 1 : i + 1;
+        const double p = phi[IX(b, i, j, k)];
+        const double sgn = fsign1(p);
+        int change = fabs(p) < EPS_SIMS;
+        if (!change) {
+            for (int kk = km; kk <= kp && !change; ++kk)
+            for (int jj = jm; jj <= jp && !change; ++jj)
+            for (int ii = im; ii <= ip; ++ii) {
+                const double q = phi[IX(b, ii, jj, kk)];
+                if (sgn * q < 0.0 || fabs(q) < EPS_SIMS) { change = 1; break; }
+            }
+        }
+        if (change && mask) {
+            for (int kk = km; kk <= kp; ++kk)
+            for (int jj = jm; jj <= jp; ++jj)
+            for (int ii = im; ii <= ip; ++ii) mask[IX(b, ii, jj, kk)] = 1;
+        }
+        double V[8][3], ph[8];
+        int allp = 1, alln = 1;
+        for (int q = 0; q < 8; ++q) {
+            const size_t l = IX(b, i + OFF[q][0], j + OFF[q][1], k + OFF[q][2]);
+            V[q][0] = b->x[l]; V[q][1] = b->y[l]; V[q][2] = b->z[l];
+            ph[q] = phi[l];
+            if (!(ph[q] > EPS_SIMS)) allp = 0;
+            if (!(ph[q] < -EPS_SIMS)) alln = 0;
+        }
+        double cw = 0.0, st = 0.0, pc = 0.0;
+        if (allp) cw = hex_volume(V);
+        else if (alln) continue;
+        else cell_analysis(ph, V, &pc, &st);
+        vk = vk + pc;
+        vs = vs + st;
+        vb = vb + cw;
+    }
+    vol[0] = vk; vol[1] = vs; vol[2] = vb;
+}
+
+/* ------------------------------------------------------------------------- */
+/* R1: levelsetmethod, levelsetmethod.F90:135-326 (non-hybrid Sussman path,   */
+/* no global mass correction).  phi: in = phi^n, out = phi^(n+1).  Returns    */
+/* the pre-advection water volume (the value WriteTotalWaterVolume appends).  */
+/* ------------------------------------------------------------------------- */
+double oracle_levelset_step(const OBlock *b, const OReinit *p, double *phi,
+                            const double *u, const double *v, const double *w,
+                            int numiter_lsm, double delti_lsm, int call_levelset,
+                            int call_reinit, int rn_freq, int ntime)
+{
+    const size_t n = (size_t)b->ni * b->nj * b->nk;
+    double vol[3];
+    int32_t *mask = (int32_t *)malloc(n * sizeof(int32_t));
+    double *pc = (double *)calloc(n, sizeof(double));
+    oracle_water_volume(b, phi, vol, mask);
+    const double total = (vol[0] + vol[1]) + vol[2];
+    for (size_t l = 0; l < n; ++l) mask[l] = 1;
+    if (call_levelset) {
+        double *U1 = (double *)malloc(n * sizeof(double));
+        double *U2 = (double *)malloc(n * sizeof(double));
+        double *U3 = (double *)malloc(n * sizeof(double));
+        oracle_contra(b, u, v, w, U1, U2, U3);
+        memcpy(pc, phi, n * sizeof(double));
+        for (int it = 0; it < numiter_lsm; ++it) {
+            oracle_rk3(b, phi, pc, mask, U1, U2, U3, delti_lsm);
+            memcpy(pc, phi, n * sizeof(double));
+        }
+        free(U1); free(U2); free(U3);
+    }
+    if (call_reinit && rn_freq > 0 && ntime % rn_freq == 0) {
+        oracle_reinit_eno2(b, p, phi, pc);
+        oracle_bcond(b, pc);
+        memcpy(phi, pc, n * sizeof(double));
+    }
+    free(mask); free(pc);
+    return total;
+}

@@ -1,0 +1,177 @@
+"""Writes the golden vectors under tests/golden/*.npz by running the compiled
+reference Fortran (oracle/_ref/lsm_ref_harness, built by `make -C oracle/ref`)
+on small seeded inputs.
+
+Each fixture stores its inputs (grid coordinates, metrics, velocity, level-set
+fields, parameters) and the reference outputs, so the tests need neither the
+reference tree nor the input generators to check the oracle.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+sys.path.insert(0, os.path.join(ROOT, "ns-lsm-single-phase_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle", "ref"))
+
+from lsm_amd import inputs  # noqa: E402
+import refio  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def grid_arrays(g):
+    d = {"x": g.x, "y": g.y, "z": g.z, "aj": g.aj}
+    for name in ("csi", "eta", "zet"):
+        for c in range(3):
+            d[f"{name}{c}"] = getattr(g, name)[c]
+    return d
+
+
+def case(name, g, phi, phi0, u, v, w, params, runs, extra_inputs=None):
+    """runs: list of (key, mode, kwargs, input overrides)."""
+    data = {f"in_{k}": np.asarray(a) for k, a in grid_arrays(g).items()}
+    data.update({"in_u": u, "in_v": v, "in_w": w, "in_phi": phi, "in_phi0": phi0})
+    for k, a in (extra_inputs or {}).items():
+        data[f"in_{k}"] = np.asarray(a)
+    for k, val in params.items():
+        data[f"p_{k}"] = np.asarray(val)
+    for key, mode, kw, ov in runs:
+        args = dict(params)
+        args.update(kw)
+        p = ov.get("phi", phi)
+        p0 = ov.get("phi0", phi0)
+        aux = ov.get("aux", None)
+        mask = ov.get("mask", None)
+        outs, _, extra = refio.run(mode, g, p, phi0=p0, aux=aux, mask=mask, u=u, v=v, w=w,
+                                   **{k2: v2 for k2, v2 in args.items() if k2 in RUN_KEYS})
+        for n, o in enumerate(outs):
+            data[f"out_{key}_{n}"] = o
+        if "global_volume" in extra:
+            data[f"out_{key}_volume"] = extra["global_volume"]
+        for k2, v2 in kw.items():
+            data[f"kw_{key}_{k2}"] = np.asarray(v2)
+    path = os.path.join(OUT, f"{name}.npz")
+    np.savez_compressed(path, **data)
+    print(f"wrote {path} ({os.path.getsize(path) / 1024:.0f} KiB)")
+
+
+RUN_KEYS = {"nsteps", "btype", "order_ad", "numiter_lsm", "rn_itermax", "order_reinit_bc",
+            "eno_bc", "m_sphi0", "m_ti", "hybrid", "call_reinit", "rn_freq", "ntime0",
+            "sussman_method", "call_levelset", "delti_lsm", "epslsm", "rn_timestep",
+            "eps_reinit", "big_phi", "lambda_dt"}
+
+
+def curvilinear_case():
+    """Distorted unit cube, spheres, random Fourier velocity: every routine."""
+    g = inputs.sinusoidal_cube_grid(11, amp=0.04, seed=2, nj=12, nk=13)
+    phi = inputs.spheres_sdf(g, nspheres=3, seed=3, rmin=0.2, rmax=0.3)
+    u, v, w = inputs.fourier_velocity(g, seed=4)
+    dt = inputs.cfl_dt(g, u, v, w)
+    hmin, hmax = inputs.spacing_bounds(g)
+    eps = 1.5 * hmax
+    dtau = 0.5 * hmin
+    phi0 = phi * (0.1 + (g.x - 0.5) ** 2)
+    params = dict(delti_lsm=dt, epslsm=eps, rn_timestep=dtau, eps_reinit=1e-5,
+                  big_phi=5.0, btype=(1, 1, 1, 1, 1, 1))
+    (sg, ng0), _, _ = refio.run("sign_ini", g, phi, phi0=phi0, m_sphi0=1, epslsm=eps)
+    runs = [("contra", "contra", {}, {}),
+            ("weno_rhs", "weno_rhs", {}, {}),
+            ("bcond", "bcond", {}, {}),
+            ("advect", "advect", {"nsteps": 3}, {}),
+            ("advect_iter2", "advect", {"nsteps": 2, "numiter_lsm": 2}, {})]
+    for ob in (1, 2, 3):
+        for eb in (0, 1):
+            runs.append((f"normgrad_o{ob}_e{eb}", "normgrad",
+                         {"order_reinit_bc": ob, "eno_bc": eb}, {}))
+    for ob in (1, 2):
+        runs.append((f"reinit_eno2_o{ob}", "reinit_eno2",
+                     {"rn_itermax": 5, "order_reinit_bc": ob}, {"phi": phi0}))
+    for ms in (0, 1):
+        runs.append((f"sign_ini_m{ms}", "sign_ini", {"m_sphi0": ms}, {}))
+    runs.append(("rh_rn", "rh_rn", {}, {"aux": sg}))
+    runs.append(("lambda_rn", "lambda_rn", {"lambda_dt": dtau}, {"aux": ng0}))
+    for ms in (0, 1):
+        for mt in (0, 1):
+            runs.append((f"reinit_godunov_m{ms}_t{mt}", "reinit_godunov",
+                         {"m_sphi0": ms, "m_ti": mt, "rn_itermax": 4}, {"phi": phi0}))
+    runs.append(("volume", "volume", {}, {}))
+    runs.append(("levelset", "levelset", {"nsteps": 3, "rn_itermax": 4}, {}))
+    case("curvilinear", g, phi, phi0, u, v, w, params, runs,
+         extra_inputs={"sign": sg, "ng0": ng0})
+
+
+def channel_case():
+    """Solitary-wave channel (the 96x24x48 configuration, coarsened)."""
+    g = inputs.channel_grid(32, 6, 16)
+    phi = inputs.solitary_phi(g, H=0.3)
+    u, v, w = inputs.solitary_velocity(g, H=0.3)
+    dt = inputs.cfl_dt(g, u, v, w)
+    hmin, hmax = inputs.spacing_bounds(g)
+    params = dict(delti_lsm=dt, epslsm=1.5 * hmax, rn_timestep=0.5 * hmin,
+                  eps_reinit=1e-5, big_phi=5.0, btype=(1, 1, 1, 1, 1, 1))
+    runs = [("advect", "advect", {"nsteps": 5}, {}),
+            ("volume", "volume", {}, {}),
+            ("levelset", "levelset", {"nsteps": 2, "rn_itermax": 4}, {}),
+            ("reinit_eno2_o1", "reinit_eno2", {"rn_itermax": 4, "order_reinit_bc": 1}, {})]
+    case("channel", g, phi, phi, u, v, w, params, runs)
+
+
+def boundary_case():
+    """Boundary-condition types 0/1/2, the BigPhi switch, a partial advection
+    mask and zero / negative / exactly-zero contravariant speeds."""
+    g = inputs.sinusoidal_cube_grid(9, amp=0.03, seed=11, nj=10, nk=8)
+    phi = 3.0 * (inputs.spheres_sdf(g, nspheres=2, seed=12, rmin=0.25, rmax=0.35))
+    u, v, w = inputs.fourier_velocity(g, nmodes=6, kmax=2, seed=13)
+    u[:, :, 4] = 0.0
+    v[3, :, :] = 0.0
+    w[:, 5, :] = 0.0
+    rng = np.random.default_rng(14)
+    mask = (rng.uniform(size=g.shape) > 0.3).astype(np.int32)
+    dt = inputs.cfl_dt(g, u, v, w)
+    params = dict(delti_lsm=dt, epslsm=0.15, rn_timestep=0.05, eps_reinit=1e-5,
+                  big_phi=0.4, btype=(0, 2, 1, 2, 1, 0))
+    runs = [("bcond", "bcond", {}, {}),
+            ("weno_rhs_mask", "weno_rhs", {}, {"mask": mask}),
+            ("advect_mask", "advect", {"nsteps": 2}, {"mask": mask}),
+            ("advect", "advect", {"nsteps": 2}, {}),
+            ("bcond_b1", "bcond", {"btype": (1, 1, 1, 1, 1, 1)}, {}),
+            ("bcond_b2", "bcond", {"btype": (2, 2, 2, 2, 2, 2), "big_phi": 0.2}, {})]
+    case("boundary", g, phi, phi, u, v, w, params, runs, extra_inputs={"mask": mask})
+
+
+def volume_case():
+    """Water volume with level-set values exactly zero at nodes (planes through
+    grid nodes), tilted planes and a sphere on a uniform grid."""
+    g = inputs.uniform_grid(9, 8, 7, h=0.125)
+    planes = {
+        "flat": g.z - 0.375,                       # zero on a node layer
+        "tilt": 0.3 * g.x + 0.2 * g.y - g.z + 0.21,
+        "diag": g.x - g.y,                         # zero on a diagonal node set
+        "sphere": 0.31 - np.sqrt((g.x - 0.5) ** 2 + (g.y - 0.45) ** 2 + (g.z - 0.4) ** 2),
+    }
+    zeros = np.zeros(g.shape)
+    params = dict(delti_lsm=1e-3, epslsm=0.2, rn_timestep=0.05, eps_reinit=1e-5,
+                  big_phi=5.0, btype=(1, 1, 1, 1, 1, 1))
+    runs = []
+    extra = {}
+    for name, p in planes.items():
+        extra[f"plane_{name}"] = p
+        runs.append((f"volume_{name}", "volume", {}, {"phi": p}))
+        runs.append((f"normgrad_{name}", "normgrad", {}, {"phi": p, "phi0": p}))
+    case("volume", g, planes["tilt"], planes["tilt"], zeros, zeros, zeros, params, runs,
+         extra_inputs=extra)
+
+
+if __name__ == "__main__":
+    if not refio.available():
+        sys.exit("reference harness not built: make -C oracle/ref")
+    curvilinear_case()
+    channel_case()
+    boundary_case()
+    volume_case()

@@ -1,0 +1,154 @@
+"""The CPU oracle (oracle/lsm_oracle.c) against golden vectors written by the
+compiled reference Fortran.  Every comparison is bit-exact: the oracle keeps
+the reference's operation order."""
+import numpy as np
+import pytest
+
+import oracle as O
+from fixtures import load
+
+
+def _blk(fx, btype=None, big_phi=None):
+    return O.Block(fx.grid, btype=btype or fx.par("btype"),
+                   big_phi=fx.par("big_phi") if big_phi is None else big_phi)
+
+
+def _eq(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape
+    diff = int(np.sum(a != b))
+    assert diff == 0, f"{diff} of {a.size} values differ, max |d| = {np.max(np.abs(a - b))}"
+
+
+@pytest.fixture(scope="module")
+def curv():
+    return load("curvilinear")
+
+
+def test_contra(curv):
+    blk = _blk(curv)
+    U = O.contra(blk, *curv.velocity)
+    for c in range(3):
+        _eq(U[c], curv.out("contra", c))
+
+
+def test_weno3_rhs(curv):
+    blk = _blk(curv)
+    U = O.contra(blk, *curv.velocity)
+    _eq(O.weno3_rhs(blk, curv.inp("phi"), U), curv.out("weno_rhs"))
+
+
+def test_bcond(curv):
+    _eq(O.bcond(_blk(curv), curv.inp("phi")), curv.out("bcond"))
+
+
+@pytest.mark.parametrize("key", ["advect", "advect_iter2"])
+def test_rk3_steps(curv, key):
+    blk = _blk(curv)
+    U = O.contra(blk, *curv.velocity)
+    dt = curv.par("delti_lsm")
+    nsteps = curv.kw(key, "nsteps")
+    numiter = curv.kw(key, "numiter_lsm", 1)
+    _eq(O.rk3(blk, curv.inp("phi"), U, dt, nsteps=1, numiter=numiter), curv.out(key, 0))
+    _eq(O.rk3(blk, curv.inp("phi"), U, dt, nsteps=nsteps, numiter=numiter), curv.out(key, 1))
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("eno", [0, 1])
+def test_normgrad(curv, order, eno):
+    ng = O.normgrad(_blk(curv), curv.inp("phi"), curv.inp("phi0"), order, eno)
+    _eq(ng, curv.out(f"normgrad_o{order}_e{eno}"))
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_reinit_eno2(curv, order):
+    out, _ = O.reinit_eno2(_blk(curv), curv.inp("phi0"), curv.par("epslsm"),
+                           curv.par("eps_reinit"), curv.par("rn_timestep"), 5, order)
+    _eq(out, curv.out(f"reinit_eno2_o{order}"))
+
+
+@pytest.mark.parametrize("m", [0, 1])
+def test_sign_ini(curv, m):
+    s, g = O.sign_ini(_blk(curv), curv.inp("phi0"), curv.par("epslsm"), m)
+    _eq(s, curv.out(f"sign_ini_m{m}", 0))
+    _eq(g, curv.out(f"sign_ini_m{m}", 1))
+
+
+def test_rh_rn(curv):
+    _eq(O.rh_rn(_blk(curv), curv.inp("phi"), curv.inp("sign")), curv.out("rh_rn"))
+
+
+def test_lambda_rn(curv):
+    lam = O.lambda_rn(_blk(curv), curv.inp("phi0"), curv.inp("ng0"), curv.inp("phi"),
+                      curv.par("rn_timestep"), curv.par("epslsm"))
+    _eq(lam, curv.out("lambda_rn"))
+
+
+@pytest.mark.parametrize("m", [0, 1])
+@pytest.mark.parametrize("t", [0, 1])
+def test_reinit_godunov(curv, m, t):
+    out = O.reinit_godunov(_blk(curv), curv.inp("phi0"), curv.par("epslsm"),
+                           curv.par("rn_timestep"), 4, m, t)
+    _eq(out, curv.out(f"reinit_godunov_m{m}_t{t}"))
+
+
+def test_volume(curv):
+    vol, mask = O.water_volume(_blk(curv), curv.inp("phi"))
+    _eq(vol, curv.out("volume", 0))
+    _eq(mask, curv.out("volume", 1))
+
+
+@pytest.mark.parametrize("name", ["curvilinear", "channel"])
+def test_levelset_steps(name):
+    fx = load(name)
+    blk = _blk(fx)
+    nsteps = fx.kw("levelset", "nsteps")
+    p = fx.inp("phi").copy()
+    vols = []
+    for s in range(nsteps):
+        p, v = O.levelset_step(blk, p, *fx.velocity, fx.par("delti_lsm"), fx.par("epslsm"),
+                               fx.par("eps_reinit"), fx.par("rn_timestep"),
+                               fx.kw("levelset", "rn_itermax"), ntime=1 + s)
+        vols.append(v)
+    _eq(p, fx.out("levelset"))
+    _eq(np.array(vols), fx.d["out_levelset_volume"])
+
+
+def test_channel_advect_and_volume():
+    fx = load("channel")
+    blk = _blk(fx)
+    U = O.contra(blk, *fx.velocity)
+    _eq(O.rk3(blk, fx.inp("phi"), U, fx.par("delti_lsm"), nsteps=5), fx.out("advect", 1))
+    vol, mask = O.water_volume(blk, fx.inp("phi"))
+    _eq(vol, fx.out("volume", 0))
+    _eq(mask, fx.out("volume", 1))
+    out, _ = O.reinit_eno2(blk, fx.inp("phi"), fx.par("epslsm"), fx.par("eps_reinit"),
+                           fx.par("rn_timestep"), 4, 1)
+    _eq(out, fx.out("reinit_eno2_o1"))
+
+
+def test_boundary_types_and_mask():
+    fx = load("boundary")
+    blk = _blk(fx)
+    phi = fx.inp("phi")
+    _eq(O.bcond(blk, phi), fx.out("bcond"))
+    _eq(O.bcond(_blk(fx, btype=(1,) * 6), phi), fx.out("bcond_b1"))
+    _eq(O.bcond(_blk(fx, btype=(2,) * 6, big_phi=0.2), phi), fx.out("bcond_b2"))
+    U = O.contra(blk, *fx.velocity)
+    mask = fx.inp("mask")
+    _eq(O.weno3_rhs(blk, phi, U, mask), fx.out("weno_rhs_mask"))
+    dt = fx.par("delti_lsm")
+    _eq(O.rk3(blk, phi, U, dt, mask=mask, nsteps=2), fx.out("advect_mask", 1))
+    _eq(O.rk3(blk, phi, U, dt, nsteps=2), fx.out("advect", 1))
+
+
+@pytest.mark.parametrize("plane", ["flat", "tilt", "diag", "sphere"])
+def test_volume_cases(plane):
+    fx = load("volume")
+    blk = _blk(fx)
+    p = fx.inp(f"plane_{plane}")
+    vol, mask = O.water_volume(blk, p)
+    _eq(vol, fx.out(f"volume_{plane}", 0))
+    _eq(mask, fx.out(f"volume_{plane}", 1))
+    _eq(O.normgrad(blk, p, p), fx.out(f"normgrad_{plane}"))
