@@ -1,0 +1,199 @@
+/*
+ * lsm_hip.h -- C ABI of the MI355X level-set interface-transport library.
+ *
+ * Each entry point replaces one routine of the reference level-set path
+ * (jcsandov/NS-LSM-Single-Phase, src/), keeping its argument meaning.  The
+ * reference routines are internal procedures of `levelsetmethod` and read
+ * their grid, metrics and parameters by host association; here that state is
+ * passed explicitly through `LsmBlock` and `LsmReinitParams`.
+ *
+ * Conventions
+ *  - All arrays are device pointers to float64 (int32 for masks) with the
+ *    reference's Fortran memory order: element (i,j,k) of a block with
+ *    extents il:iu, jl:ju, kl:ku lives at
+ *        (i-il) + (iu-il+1)*((j-jl) + (ju-jl+1)*(k-kl)).
+ *  - Vector fields are structure-of-arrays (three separate arrays) where the
+ *    reference stores them interleaved (csi(1:3,i,j,k)).
+ *  - `stream` is a hipStream_t (NULL = the default stream).  Calls are
+ *    asynchronous on that stream; results are ready after the stream is
+ *    synchronised.  Scalar results (volumes, iteration counts) are written to
+ *    device memory.
+ *  - Every call returns LSM_OK (0) or a negative LSM_ERR_* code; the message
+ *    of the last error on the calling thread is in lsm_last_error().
+ *  - `work` arguments are device scratch of at least lsm_workspace_bytes()
+ *    bytes; NULL lets the library use its own per-device buffer (allocated
+ *    once, on first use).
+ */
+#ifndef LSM_HIP_H
+#define LSM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSM_OK                 0
+#define LSM_ERR_ARG           -1   /* bad argument (NULL pointer, bad extent, bad option) */
+#define LSM_ERR_UNSUPPORTED   -2   /* configuration outside what the library implements */
+#define LSM_ERR_HIP           -3   /* a HIP runtime call failed */
+#define LSM_ERR_WORKSPACE     -4   /* workspace too small */
+
+/* physical-boundary face bits (the reference's my* == mpi_proc_null tests) */
+#define LSM_FACE_I1  0x01   /* myback  : i = il+igp */
+#define LSM_FACE_IM  0x02   /* myfront : i = iu-igp */
+#define LSM_FACE_J1  0x04   /* myleft  : j = jl+jgp */
+#define LSM_FACE_JM  0x08   /* myright : j = ju-jgp */
+#define LSM_FACE_K1  0x10   /* mydown  : k = kl+kgp */
+#define LSM_FACE_KM  0x20   /* myup    : k = ku-kgp */
+#define LSM_FACE_ALL 0x3f
+
+/* advection arithmetic */
+#define LSM_ADVECT_FAST   0   /* WENO3 weights with one division per face value */
+#define LSM_ADVECT_EXACT  1   /* the reference's operation order (four divisions) */
+
+/* workspace queries */
+#define LSM_OP_RK3            1
+#define LSM_OP_REINIT_ENO2    2
+#define LSM_OP_REINIT_GODUNOV 3
+#define LSM_OP_WATER_VOLUME   4
+#define LSM_OP_LEVELSET_STEP  5
+
+/* One curvilinear block (the host-associated state of levelsetmethod.F90:1-44). */
+typedef struct LsmBlock {
+    int32_t il, iu, jl, ju, kl, ku;   /* array extents */
+    int32_t igp, jgp, kgp;            /* ghost widths */
+    int32_t phys_faces;               /* LSM_FACE_* bits of physical boundaries */
+    int32_t btype[6];                 /* btype_lsm(1:6, myzone): 0 copy, 1 extrapolate, 2 advect */
+    double dc, de, dz;                /* 1/d xi, 1/d eta, 1/d zeta (dc = de = dz = 1) */
+    double big_phi;                   /* BigPhi of bcond_lsm */
+    const double *csi[3];             /* xi_x, xi_y, xi_z */
+    const double *eta[3];             /* eta_x, eta_y, eta_z */
+    const double *zet[3];             /* zeta_x, zeta_y, zeta_z */
+    const double *aj;                 /* aj = 1/det(dx/dxi) */
+    const double *x, *y, *z;          /* node coordinates (water volume only) */
+} LsmBlock;
+
+/* Parameters of the pseudo-time reinitialisation (global_lsm.F90:17-58). */
+typedef struct LsmReinitParams {
+    double epslsm;                    /* interface half width */
+    double eps_reinit;                /* epsReinitialisation */
+    double rn_timestep;               /* RNtimestep (d tau) */
+    int32_t rn_itermax;               /* RNitermax */
+    int32_t order_bc;                 /* OrderReinitialisationBoundaries (1, 2, 3) */
+    int32_t eno_bc;                   /* ENOBCReinitialisation (0/1) */
+    int32_t m_sphi0;                  /* Godunov path: 1 piecewise-sine sign, else phi/sqrt(phi^2+(|grad|eps)^2) */
+    int32_t m_ti;                     /* Godunov path: 1 SSP-RK3, else forward Euler */
+} LsmReinitParams;
+
+/* Library version string. */
+const char *lsm_version(void);
+
+/* Message of the last error returned on the calling thread. */
+const char *lsm_last_error(void);
+
+/* Bytes of device scratch an operation needs for this block (LSM_OP_*). */
+size_t lsm_workspace_bytes(const LsmBlock *blk, int op);
+
+/* rhs_contra_j.F90:2-50 (rhs_contra_j): ucn_j = (1/aj) * (grad xi^j . u) at
+ * every owned node.  u, v, w are read with element stride `vel_stride`
+ * (1 for separate arrays, 4 to read q(2:4,...) of the reference directly by
+ * passing q+1, q+2, q+3). */
+int lsm_contra_velocity(const LsmBlock *blk, const double *u, const double *v,
+                        const double *w, int64_t vel_stride,
+                        double *ucn1, double *ucn2, double *ucn3, void *stream);
+
+/* calc_RH_AD_WENO3.F90:1-407 (calc_RH_AD_WENO3): right-hand side of the
+ * WENO3 advection operator on interior nodes with adv_mask != 0 (adv_mask may
+ * be NULL: every node).  Other nodes of `rhs` are left untouched (the caller
+ * zeroes rhs, as ad_RKTVD3.F90:55 does). */
+int lsm_weno3_rhs(const LsmBlock *blk, const double *phi, const double *ucn1,
+                  const double *ucn2, const double *ucn3, const int32_t *adv_mask,
+                  double *rhs, int mode, void *stream);
+
+/* bcond_lsm.F90:2-290 (bcond_lsm): boundary values on the physical faces, in
+ * place, in the reference's face order (i1, im, j1, jm, k1, km). */
+int lsm_bcond(const LsmBlock *blk, double *phi, void *stream);
+
+/* ad_RKTVD3.F90:1-88 (ad_RKTVD3): one SSP-RK3 step of the WENO3 advection,
+ * phi_np1 = RK3(phi_n), with bcond_lsm after each stage.  phi_np1 and phi_n
+ * must not alias.  adv_mask may be NULL (every node advected). */
+int lsm_rk3_step(const LsmBlock *blk, double *phi_np1, const double *phi_n,
+                 const int32_t *adv_mask, const double *ucn1, const double *ucn2,
+                 const double *ucn3, double delti_lsm, int mode,
+                 void *work, size_t work_bytes, void *stream);
+
+/* calc_NormGrad_d.F90:1-2448 (calc_NormGrad_d): |grad d| with ENO2
+ * differences inside and biased differences of order `order_bc` (optionally
+ * ENO2Boundary when eno_bc != 0 and order_bc == 2) on faces, edges and
+ * vertices; phizero selects the ENO2 upwind side. */
+int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizero,
+                      int order_bc, int eno_bc, double *normgrad, void *stream);
+
+/* reinitialisation_test2.F90:1-218 (reinitialisation_test2, Sussman path,
+ * non-hybrid): phi_out = reinitialised phi0.  iters_done (device int32, may be
+ * NULL) receives the number of pseudo-time iterations that ran. */
+int lsm_reinit_sussman_eno2(const LsmBlock *blk, const LsmReinitParams *p,
+                            const double *phi0, double *phi_out, int32_t *iters_done,
+                            void *work, size_t work_bytes, void *stream);
+
+/* calc_sign_ini.F90:1-286 (calc_sign_ini): s(phi0) and the Godunov |grad phi0|. */
+int lsm_sign_ini(const LsmBlock *blk, const double *phi0, double epslsm, int m_sphi0,
+                 double *sign_out, double *normgrad_out, void *stream);
+
+/* calc_RH_RN.F90:1-244 (calc_RH_RN): rhs = s * (|grad d|_Godunov - 1). */
+int lsm_rh_rn(const LsmBlock *blk, const double *d, const double *sign, double *rhs,
+              void *stream);
+
+/* calc_lambda_RN.F90:1-122 (calc_lambda_RN, sussman_correction_method = 0). */
+int lsm_lambda_rn(const LsmBlock *blk, const double *phi0, const double *normgrad0,
+                  const double *d, double timestep, double epslsm, double *lambda,
+                  void *stream);
+
+/* reinitialization_benchmark.F90:1-228 (reinitialization_benchmark, M_Grad = 0,
+ * no obstacles, non-hybrid): Godunov pseudo-time reinitialisation. */
+int lsm_reinit_godunov(const LsmBlock *blk, const LsmReinitParams *p,
+                       const double *phi0, double *phi_out,
+                       void *work, size_t work_bytes, void *stream);
+
+/* TotalWaterVolume.F90:1-297 (TotalWaterVolume): vol_out[0..2] (device) =
+ * (K-tetrahedra, second tetrahedra, bulk cells) water volumes; adv_mask
+ * (optional) receives the AdvectionNodes interface mask. */
+int lsm_water_volume(const LsmBlock *blk, const double *phi, double *vol_out,
+                     int32_t *adv_mask, void *work, size_t work_bytes, void *stream);
+
+/* levelsetmethod.F90:1-372 (levelsetmethod, non-hybrid Sussman path, no
+ * global mass correction): pre-advection water volume (vol_out, device, 3
+ * doubles, may be NULL), contravariant velocity from q, numiter_lsm RK3 steps
+ * from phi_n, then every rn_freq-th step (ntime % rn_freq == 0) the Sussman
+ * reinitialisation and bcond_lsm.  phi is overwritten with phi^(n+1). */
+int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p,
+                      double *phi, const double *phi_n,
+                      const double *u, const double *v, const double *w, int64_t vel_stride,
+                      int numiter_lsm, double delti_lsm, int call_levelset,
+                      int call_reinit, int rn_freq, int ntime, int mode,
+                      double *vol_out, void *work, size_t work_bytes, void *stream);
+
+/* ---- instrumentation (no counterpart in the reference) ------------------
+ * Per-kernel launch timing with HIP events recorded on the launch stream,
+ * for benchmarks that need a kernel's own duration without a profiler.
+ * Kernel ids: */
+#define LSM_KT_ADVECT_STAGE   0   /* fused WENO3 + RK stage (interior nodes) */
+#define LSM_KT_ADVECT_SHELL   1   /* RK stage on the boundary shell + bcond_lsm */
+#define LSM_KT_REINIT_STAGE   2   /* fused ENO2 |grad d| + reinitialisation stage */
+#define LSM_KT_VOLUME         3   /* water volume cell kernel */
+#define LSM_KT_COUNT          4
+
+/* Turns event recording on (1) or off (0); clears accumulated timings. */
+int lsm_timing_enable(int on);
+
+/* Total milliseconds and number of timed launches of kernel `id` since the
+ * last enable; waits for the recorded events. */
+int lsm_timing_get(int id, double *ms_total, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LSM_HIP_H */

@@ -1,0 +1,490 @@
+// WENO3 + SSP-RK3 level-set advection (ad_RKTVD3 / calc_RH_AD_WENO3), the
+// contravariant velocity (rhs_contra_j) and the level-set boundary conditions
+// (bcond_lsm) for one block resident in HBM.
+//
+// Hot kernel: advect_stage.  One workgroup owns a 64 (i) x 4 (j) column tile
+// and marches up a chunk of k planes.  Each thread keeps phi(k-2..k+2) of its
+// own column in registers; the current k plane of the tile, with the two-node
+// i/j halo the WENO3 stencil needs, is staged in LDS (double buffered, one
+// barrier per plane).  Each stage reads phi once, the stage-0 field phi^n, the
+// three U/J components and aj at the centre, and writes the new phi: the
+// whole stage update (RHS + RK combination) is fused.  The boundary shell is
+// written by a second, tiny kernel that applies the RK combination and the
+// face extrapolations of bcond_lsm in the reference's face order.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "lsm_internal.hpp"
+
+namespace lsm {
+
+namespace {
+
+constexpr int TX = 64;    // tile width in i (one wavefront per j row)
+constexpr int TY = 4;     // tile height in j
+constexpr int HALO = 2;
+
+struct Range {
+    int ilo, ihi, jlo, jhi, klo, khi;   // interior nodes (RHS nodes)
+    int ia, ib, ja, jb, ka, kb;         // owned nodes
+    bool fb, ff, fl, fr, fd, fu;        // physical faces
+};
+
+Range make_range(const Blk &b)
+{
+    Range r;
+    const int f = b.e.faces;
+    r.fb = f & LSM_FACE_I1; r.ff = f & LSM_FACE_IM;
+    r.fl = f & LSM_FACE_J1; r.fr = f & LSM_FACE_JM;
+    r.fd = f & LSM_FACE_K1; r.fu = f & LSM_FACE_KM;
+    r.ia = 0; r.ib = b.e.ni - 1;
+    r.ja = 0; r.jb = b.e.nj - 1;
+    r.ka = 0; r.kb = b.e.nk - 1;
+    r.ilo = r.ia + (r.fb ? 1 : 0); r.ihi = r.ib - (r.ff ? 1 : 0);
+    r.jlo = r.ja + (r.fl ? 1 : 0); r.jhi = r.jb - (r.fr ? 1 : 0);
+    r.klo = r.ka + (r.fd ? 1 : 0); r.khi = r.kb - (r.fu ? 1 : 0);
+    return r;
+}
+
+// workgroup -> tile, keeping consecutive tiles on one XCD: the dispatcher
+// deals workgroups round-robin over the 8 XCDs, so workgroup b runs on XCD
+// b % 8; giving each XCD a contiguous run of tiles keeps the j/k halo reuse
+// inside one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    if ((nb & 7) != 0) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+struct StageArgs {
+    const double *ps, *pn;
+    double *out;
+    const double *U1, *U2, *U3, *aj;
+    const int32_t *mask;
+    int ni, nj, nk;
+    long sj, sk;
+    Range r;
+    double c1, c2, c3, dt, dc, de, dz;
+    int tiles_i, tiles_j, kc, ntiles;
+};
+
+template <bool FAST, bool MASK>
+__global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
+{
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= a.ntiles) return;
+    const int ti = tile % a.tiles_i;
+    const int t2 = tile / a.tiles_i;
+    const int tj = t2 % a.tiles_j;
+    const int tk = t2 / a.tiles_j;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = ti * TX + tx, j = tj * TY + ty;
+    const int kb = a.r.klo + tk * a.kc;
+    const int ke = min(kb + a.kc - 1, a.r.khi);
+
+    __shared__ double sm[2][TY + 2 * HALO][TX + 2 * HALO];
+
+    auto ld = [&](int ii, int jj, int kk) -> double {
+        if (ii < 0 || ii >= a.ni || jj < 0 || jj >= a.nj || kk < 0 || kk >= a.nk) return 0.0;
+        return a.ps[(long)ii + a.sj * jj + a.sk * kk];
+    };
+
+    // j-halo row handled by this thread, i-halo cell (16 of them) handled by the first 16 threads
+    const int rr = (ty < 2) ? ty : ty + TY;
+    const int jh = tj * TY - HALO + rr;
+    const int lin = ty * TX + tx;
+    const int hr = lin >> 2, hc4 = lin & 3;
+    const int hcol = (hc4 < 2) ? hc4 : hc4 + TX;
+    const int hi_i = ti * TX - HALO + hcol;
+    const int hi_j = tj * TY + hr;
+
+    const bool act = i >= a.r.ilo && i <= a.r.ihi && j >= a.r.jlo && j <= a.r.jhi;
+    const bool fmb = a.r.fb && i == a.r.ilo, fmf = a.r.ff && i == a.r.ihi;
+    const bool fml = a.r.fl && j == a.r.jlo, fmr = a.r.fr && j == a.r.jhi;
+
+    double q0 = ld(i, j, kb - 2), q1 = ld(i, j, kb - 1), q2 = ld(i, j, kb), q3 = ld(i, j, kb + 1);
+    int buf = 0;
+    for (int k = kb; k <= ke; ++k) {
+        const double q4 = ld(i, j, k + 2);
+        sm[buf][ty + HALO][tx + HALO] = q2;
+        sm[buf][rr][tx + HALO] = ld(i, jh, k);
+        if (lin < 4 * TY) sm[buf][hr + HALO][hcol] = ld(hi_i, hi_j, k);
+        const long l = (long)i + a.sj * j + a.sk * k;
+        double u1 = 0.0, u2 = 0.0, u3 = 0.0, ajv = 0.0, pnv = 0.0;
+        int mk = 1;
+        if (act) {
+            u1 = a.U1[l]; u2 = a.U2[l]; u3 = a.U3[l]; ajv = a.aj[l]; pnv = a.pn[l];
+            if (MASK) mk = a.mask[l];
+        }
+        __syncthreads();
+        if (act) {
+            double v = q2;
+            if (mk > 0) {
+                const double (*p)[TX + 2 * HALO] = sm[buf];
+                const int cx = tx + HALO, cy = ty + HALO;
+                const double Lc = weno3_dir<FAST>(p[cy][cx - 2], p[cy][cx - 1], q2, p[cy][cx + 1],
+                                                  p[cy][cx + 2], u1, fmb, fmf, a.dc);
+                const double Le = weno3_dir<FAST>(p[cy - 2][cx], p[cy - 1][cx], q2, p[cy + 1][cx],
+                                                  p[cy + 2][cx], u2, fml, fmr, a.de);
+                const bool fmd = a.r.fd && k == a.r.klo, fmu = a.r.fu && k == a.r.khi;
+                const double Lz = weno3_dir<FAST>(q0, q1, q2, q3, q4, u3, fmd, fmu, a.dz);
+                const double rh = (Lc + Le) + Lz;
+                v = (a.c1 * pnv + a.c2 * q2) - ((a.c3 * ajv) * a.dt) * rh;
+            }
+            a.out[l] = v;
+        }
+        q0 = q1; q1 = q2; q2 = q3; q3 = q4;
+        buf ^= 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// boundary shell of one RK stage: RK combination with rh = 0 on physical
+// boundary nodes (ad_RKTVD3.F90:62-81) followed by bcond_lsm (ad_RKTVD3.F90:83)
+// evaluated per node.  Node values after the i faces, j faces and k faces
+// (f1, f2, f3) are expanded recursively from f0 = the stage result before
+// boundary conditions, so every thread only reads arrays no thread writes.
+// ---------------------------------------------------------------------------
+struct ShellArgs {
+    const double *out_ro;   // interior results of the stage (read)
+    double *out;            // the same array (boundary nodes written)
+    const double *ps, *pn, *aj;
+    const int32_t *mask;
+    int ni, nj, nk;
+    long sj, sk;
+    Range r;
+    int bt[6];
+    double big, c1, c2, c3, dt;
+    long nI, nJ, nK;        // shell nodes owned by the i, j and k faces
+    int cnt_i, cnt_j;       // owned i / j nodes not on a physical i / j face
+};
+
+template <bool MASK>
+__device__ double sh_f0(const ShellArgs &a, int i, int j, int k)
+{
+    const long l = (long)i + a.sj * j + a.sk * k;
+    if (i >= a.r.ilo && i <= a.r.ihi && j >= a.r.jlo && j <= a.r.jhi && k >= a.r.klo && k <= a.r.khi)
+        return a.out_ro[l];
+    if (MASK && a.mask[l] <= 0) return a.ps[l];
+    return (a.c1 * a.pn[l] + a.c2 * a.ps[l]) - ((a.c3 * a.aj[l]) * a.dt) * 0.0;
+}
+
+template <bool MASK>
+__device__ double sh_f1(const ShellArgs &a, int i, int j, int k)
+{
+    if (a.r.fb && i == a.r.ia) {
+        const int bt = a.bt[0];
+        if (bt == 2) return sh_f0<MASK>(a, i, j, k);
+        return bc_face(bt, sh_f0<MASK>(a, i + 1, j, k), sh_f0<MASK>(a, i + 2, j, k), 0.0, a.big, false);
+    }
+    if (a.r.ff && i == a.r.ib) {
+        return bc_face(a.bt[1], sh_f0<MASK>(a, i - 1, j, k), sh_f0<MASK>(a, i - 2, j, k), 0.0, a.big, true);
+    }
+    return sh_f0<MASK>(a, i, j, k);
+}
+
+template <bool MASK>
+__device__ double sh_f2(const ShellArgs &a, int i, int j, int k)
+{
+    if (a.r.fl && j == a.r.ja) {
+        const int bt = a.bt[2];
+        if (bt == 2) return sh_f1<MASK>(a, i, j, k);
+        return bc_face(bt, sh_f1<MASK>(a, i, j + 1, k), sh_f1<MASK>(a, i, j + 2, k), 0.0, a.big, false);
+    }
+    if (a.r.fr && j == a.r.jb) {
+        const int bt = a.bt[3];
+        if (bt == 2) return sh_f1<MASK>(a, i, j, k);
+        return bc_face(bt, sh_f1<MASK>(a, i, j - 1, k), sh_f1<MASK>(a, i, j - 2, k), 0.0, a.big, false);
+    }
+    return sh_f1<MASK>(a, i, j, k);
+}
+
+template <bool MASK>
+__device__ double sh_f3(const ShellArgs &a, int i, int j, int k)
+{
+    if (a.r.fd && k == a.r.ka) {
+        const int bt = a.bt[4];
+        if (bt == 2) return sh_f2<MASK>(a, i, j, k);
+        return bc_face(bt, sh_f2<MASK>(a, i, j, k + 1), sh_f2<MASK>(a, i, j, k + 2), 0.0, a.big, false);
+    }
+    if (a.r.fu && k == a.r.kb) {
+        const int bt = a.bt[5];
+        if (bt == 2) return sh_f2<MASK>(a, i, j, k);
+        return bc_face(bt, sh_f2<MASK>(a, i, j, k - 1), sh_f2<MASK>(a, i, j, k - 2), 0.0, a.big, false);
+    }
+    return sh_f2<MASK>(a, i, j, k);
+}
+
+// decode shell node t: i faces own all their (j,k); j faces own the i not on
+// a physical i face; k faces own the (i,j) not on a physical i or j face
+__device__ __forceinline__ bool shell_node(const ShellArgs &a, long t, int &i, int &j, int &k)
+{
+    const int ilo2 = a.r.ia + (a.r.fb ? 1 : 0);
+    const int jlo2 = a.r.ja + (a.r.fl ? 1 : 0);
+    if (t < a.nI) {
+        const long per = (long)a.nj * a.nk;
+        const int f = (int)(t / per);
+        const long rem = t % per;
+        j = (int)(rem % a.nj);
+        k = (int)(rem / a.nj);
+        i = (f == 0 && a.r.fb) ? a.r.ia : a.r.ib;
+        return true;
+    }
+    t -= a.nI;
+    if (t < a.nJ) {
+        const long per = (long)a.cnt_i * a.nk;
+        const int f = (int)(t / per);
+        const long rem = t % per;
+        i = ilo2 + (int)(rem % a.cnt_i);
+        k = (int)(rem / a.cnt_i);
+        j = (f == 0 && a.r.fl) ? a.r.ja : a.r.jb;
+        return true;
+    }
+    t -= a.nJ;
+    if (t < a.nK) {
+        const long per = (long)a.cnt_i * a.cnt_j;
+        const int f = (int)(t / per);
+        const long rem = t % per;
+        i = ilo2 + (int)(rem % a.cnt_i);
+        j = jlo2 + (int)(rem / a.cnt_i);
+        k = (f == 0 && a.r.fd) ? a.r.ka : a.r.kb;
+        return true;
+    }
+    return false;
+}
+
+template <bool MASK>
+__global__ __launch_bounds__(256) void stage_shell(ShellArgs a)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!shell_node(a, t, i, j, k)) return;
+    const double v = sh_f3<MASK>(a, i, j, k);
+    a.out[(long)i + a.sj * j + a.sk * k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// standalone bcond_lsm: three in-place phases (i faces, j faces, k faces)
+// ---------------------------------------------------------------------------
+struct BcArgs {
+    double *phi;
+    int ni, nj, nk;
+    long sj, sk;
+    Range r;
+    int bt[6];
+    double big;
+};
+
+__global__ __launch_bounds__(256) void bcond_phase(BcArgs a, int axis)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    int n1, n2;
+    long s, s1, s2;
+    bool f_lo, f_hi;
+    int lo, hi, bt_lo, bt_hi;
+    if (axis == 0) { n1 = a.nj; n2 = a.nk; s = 1; s1 = a.sj; s2 = a.sk; f_lo = a.r.fb; f_hi = a.r.ff;
+                     lo = a.r.ia; hi = a.r.ib; bt_lo = a.bt[0]; bt_hi = a.bt[1]; }
+    else if (axis == 1) { n1 = a.ni; n2 = a.nk; s = a.sj; s1 = 1; s2 = a.sk; f_lo = a.r.fl; f_hi = a.r.fr;
+                          lo = a.r.ja; hi = a.r.jb; bt_lo = a.bt[2]; bt_hi = a.bt[3]; }
+    else { n1 = a.ni; n2 = a.nj; s = a.sk; s1 = 1; s2 = a.sj; f_lo = a.r.fd; f_hi = a.r.fu;
+           lo = a.r.ka; hi = a.r.kb; bt_lo = a.bt[4]; bt_hi = a.bt[5]; }
+    const long per = (long)n1 * n2;
+    if (t >= 2 * per) return;
+    const int f = (int)(t / per);
+    const long rem = t % per;
+    const long base = (rem % n1) * s1 + (rem / n1) * s2;
+    if (f == 0 && f_lo) {
+        const long l = base + lo * s;
+        if (bt_lo != 2) a.phi[l] = bc_face(bt_lo, a.phi[l + s], a.phi[l + 2 * s], a.phi[l], a.big, false);
+    } else if (f == 1 && f_hi) {
+        const long l = base + hi * s;
+        a.phi[l] = bc_face(bt_hi, a.phi[l - s], a.phi[l - 2 * s], a.phi[l], a.big, axis == 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// standalone WENO3 right-hand side (calc_RH_AD_WENO3) on interior nodes
+// ---------------------------------------------------------------------------
+struct RhsArgs {
+    const double *phi, *U1, *U2, *U3;
+    const int32_t *mask;
+    double *rhs;
+    int ni, nj, nk;
+    long sj, sk;
+    Range r;
+    double dc, de, dz;
+};
+
+template <bool FAST>
+__global__ __launch_bounds__(256) void weno3_rhs_kernel(RhsArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + a.r.ilo;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + a.r.jlo;
+    const int k = blockIdx.z + a.r.klo;
+    if (i > a.r.ihi || j > a.r.jhi || k > a.r.khi) return;
+    const long l = (long)i + a.sj * j + a.sk * k;
+    if (a.mask && a.mask[l] == 0) return;
+    const double *p = a.phi;
+    const long sj = a.sj, sk = a.sk;
+    auto at = [&](long q) -> double { return p[q]; };
+    // out-of-range stencil entries are only read when the stencil is not
+    // forced; forced stencils replace them by a dummy, so guard the reads
+    const bool fmb = a.r.fb && i == a.r.ilo, fmf = a.r.ff && i == a.r.ihi;
+    const bool fml = a.r.fl && j == a.r.jlo, fmr = a.r.fr && j == a.r.jhi;
+    const bool fmd = a.r.fd && k == a.r.klo, fmu = a.r.fu && k == a.r.khi;
+    const double xm2 = fmb ? 0.0 : at(l - 2), xp2 = fmf ? 0.0 : at(l + 2);
+    const double ym2 = fml ? 0.0 : at(l - 2 * sj), yp2 = fmr ? 0.0 : at(l + 2 * sj);
+    const double zm2 = fmd ? 0.0 : at(l - 2 * sk), zp2 = fmu ? 0.0 : at(l + 2 * sk);
+    const double c = at(l);
+    const double Lc = weno3_dir<FAST>(xm2, at(l - 1), c, at(l + 1), xp2, a.U1[l], fmb, fmf, a.dc);
+    const double Le = weno3_dir<FAST>(ym2, at(l - sj), c, at(l + sj), yp2, a.U2[l], fml, fmr, a.de);
+    const double Lz = weno3_dir<FAST>(zm2, at(l - sk), c, at(l + sk), zp2, a.U3[l], fmd, fmu, a.dz);
+    a.rhs[l] = (Lc + Le) + Lz;
+}
+
+// ---------------------------------------------------------------------------
+// rhs_contra_j: ucn_j = (1/aj) * (grad xi^j . u)
+// ---------------------------------------------------------------------------
+struct ContraArgs {
+    const double *u, *v, *w;
+    long stride;
+    const double *c0, *c1, *c2, *e0, *e1, *e2, *z0, *z1, *z2, *aj;
+    double *U1, *U2, *U3;
+    long n;
+};
+
+__global__ __launch_bounds__(256) void contra_kernel(ContraArgs a)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= a.n) return;
+    const double u = a.u[l * a.stride], v = a.v[l * a.stride], w = a.w[l * a.stride];
+    const double tmp = 1.0 / a.aj[l];
+    a.U1[l] = tmp * ((a.c0[l] * u + a.c1[l] * v) + a.c2[l] * w);
+    a.U2[l] = tmp * ((a.e0[l] * u + a.e1[l] * v) + a.e2[l] * w);
+    a.U3[l] = tmp * ((a.z0[l] * u + a.z1[l] * v) + a.z2[l] * w);
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(const double *__restrict__ s, double *__restrict__ d, long n)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) d[l] = s[l];
+}
+
+inline unsigned blocks_for(long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+}  // namespace
+
+int launch_copy(const double *src, double *dst, long n, hipStream_t s)
+{
+    if (n <= 0) return LSM_OK;
+    copy_kernel<<<blocks_for(n, 256), 256, 0, s>>>(src, dst, n);
+    return check_hip(hipGetLastError(), "copy_kernel");
+}
+
+int launch_contra(const Blk &b, const double *u, const double *v, const double *w, long stride,
+                  double *U1, double *U2, double *U3, hipStream_t s)
+{
+    ContraArgs a{u, v, w, stride, b.csi0, b.csi1, b.csi2, b.eta0, b.eta1, b.eta2,
+                 b.zet0, b.zet1, b.zet2, b.aj, U1, U2, U3, b.n};
+    contra_kernel<<<blocks_for(b.n, 256), 256, 0, s>>>(a);
+    return check_hip(hipGetLastError(), "contra_kernel");
+}
+
+int launch_weno3_rhs(const Blk &b, const double *phi, const double *U1, const double *U2,
+                     const double *U3, const int32_t *mask, double *rhs, int mode, hipStream_t s)
+{
+    const Range r = make_range(b);
+    RhsArgs a{phi, U1, U2, U3, mask, rhs, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, r, b.dc, b.de, b.dz};
+    const int ni = r.ihi - r.ilo + 1, nj = r.jhi - r.jlo + 1, nk = r.khi - r.klo + 1;
+    if (ni <= 0 || nj <= 0 || nk <= 0) return LSM_OK;
+    dim3 blk(64, 4, 1), grd(blocks_for(ni, 64), blocks_for(nj, 4), nk);
+    if (mode == LSM_ADVECT_EXACT) weno3_rhs_kernel<false><<<grd, blk, 0, s>>>(a);
+    else weno3_rhs_kernel<true><<<grd, blk, 0, s>>>(a);
+    return check_hip(hipGetLastError(), "weno3_rhs_kernel");
+}
+
+int launch_bcond(const Blk &b, double *phi, hipStream_t s)
+{
+    BcArgs a{phi, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, make_range(b), {0}, b.big_phi};
+    for (int q = 0; q < 6; ++q) a.bt[q] = b.btype[q];
+    const long per[3] = {(long)b.e.nj * b.e.nk, (long)b.e.ni * b.e.nk, (long)b.e.ni * b.e.nj};
+    for (int ax = 0; ax < 3; ++ax) {
+        bcond_phase<<<blocks_for(2 * per[ax], 256), 256, 0, s>>>(a, ax);
+        int rc = check_hip(hipGetLastError(), "bcond_phase");
+        if (rc) return rc;
+    }
+    return LSM_OK;
+}
+
+static int stage_kc(const Blk &b, const Range &r, int tiles_ij)
+{
+    // enough k chunks for ~16 workgroups per CU (256 CUs), at least 8 planes each
+    const int nk_int = r.khi - r.klo + 1;
+    int chunks = (4096 + tiles_ij - 1) / tiles_ij;
+    int kc = (nk_int + chunks - 1) / chunks;
+    if (kc < 8) kc = 8;
+    if (const char *e = getenv("LSM_KCHUNK")) { int v = atoi(e); if (v > 0) kc = v; }
+    return kc;
+}
+
+int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t *mask,
+               const double *U1, const double *U2, const double *U3, double dt, int mode,
+               double *scratch, hipStream_t s)
+{
+    const Range r = make_range(b);
+    static const double C[3][3] = {{1.0, 0.0, 1.0},
+                                   {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
+                                   {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    const int tiles_i = (b.e.ni + TX - 1) / TX;
+    const int tiles_j = (b.e.nj + TY - 1) / TY;
+    const int kc = stage_kc(b, r, tiles_i * tiles_j);
+    const int nk_int = r.khi - r.klo + 1;
+    const int tiles_k = nk_int > 0 ? (nk_int + kc - 1) / kc : 0;
+    const int ntiles = tiles_i * tiles_j * tiles_k;
+    const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
+
+    ShellArgs sh{};
+    sh.ni = b.e.ni; sh.nj = b.e.nj; sh.nk = b.e.nk; sh.sj = b.e.sj; sh.sk = b.e.sk; sh.r = r;
+    for (int q = 0; q < 6; ++q) sh.bt[q] = b.btype[q];
+    sh.big = b.big_phi; sh.aj = b.aj; sh.mask = mask; sh.dt = dt;
+    const int nfi = (r.fb ? 1 : 0) + (r.ff ? 1 : 0);
+    const int nfj = (r.fl ? 1 : 0) + (r.fr ? 1 : 0);
+    const int nfk = (r.fd ? 1 : 0) + (r.fu ? 1 : 0);
+    sh.cnt_i = b.e.ni - nfi;
+    sh.cnt_j = b.e.nj - nfj;
+    sh.nI = (long)nfi * b.e.nj * b.e.nk;
+    sh.nJ = (long)nfj * sh.cnt_i * b.e.nk;
+    sh.nK = (long)nfk * sh.cnt_i * sh.cnt_j;
+    const long nshell = sh.nI + sh.nJ + sh.nK;
+
+    const double *src[3] = {phi_n, phi_np1, scratch};
+    double *dst[3] = {phi_np1, scratch, phi_np1};
+    for (int st = 0; st < 3; ++st) {
+        StageArgs a{src[st], phi_n, dst[st], U1, U2, U3, b.aj, mask, b.e.ni, b.e.nj, b.e.nk,
+                    b.e.sj, b.e.sk, r, C[st][0], C[st][1], C[st][2], dt, b.dc, b.de, b.dz,
+                    tiles_i, tiles_j, kc, ntiles};
+        if (ntiles > 0) {
+            KTimer kt(LSM_KT_ADVECT_STAGE, s);
+            dim3 blk(TX, TY, 1);
+            const bool fast = mode != LSM_ADVECT_EXACT;
+            if (fast && mask) advect_stage<true, true><<<grid, blk, 0, s>>>(a);
+            else if (fast) advect_stage<true, false><<<grid, blk, 0, s>>>(a);
+            else if (mask) advect_stage<false, true><<<grid, blk, 0, s>>>(a);
+            else advect_stage<false, false><<<grid, blk, 0, s>>>(a);
+            int rc = check_hip(hipGetLastError(), "advect_stage");
+            if (rc) return rc;
+        }
+        sh.out_ro = dst[st]; sh.out = dst[st]; sh.ps = src[st]; sh.pn = phi_n;
+        sh.c1 = C[st][0]; sh.c2 = C[st][1]; sh.c3 = C[st][2];
+        if (nshell > 0) {
+            KTimer kt(LSM_KT_ADVECT_SHELL, s);
+            if (mask) stage_shell<true><<<blocks_for(nshell, 256), 256, 0, s>>>(sh);
+            else stage_shell<false><<<blocks_for(nshell, 256), 256, 0, s>>>(sh);
+            int rc = check_hip(hipGetLastError(), "stage_shell");
+            if (rc) return rc;
+        }
+    }
+    return LSM_OK;
+}
+
+}  // namespace lsm

@@ -1,0 +1,394 @@
+// extern "C" entry points of liblsm_hip.so (declared in include/lsm_hip.h).
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "lsm_internal.hpp"
+
+namespace lsm {
+
+static thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int check_hip(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return LSM_OK;
+    return fail(LSM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int make_blk(const LsmBlock *b, Blk *o, bool need_metrics, bool need_coords)
+{
+    if (!b) return fail(LSM_ERR_ARG, "LsmBlock is NULL");
+    const int ni = b->iu - b->il + 1, nj = b->ju - b->jl + 1, nk = b->ku - b->kl + 1;
+    if (ni < 4 || nj < 4 || nk < 4)
+        return fail(LSM_ERR_ARG, "block extents %dx%dx%d: every extent must be >= 4", ni, nj, nk);
+    if (b->igp != 0 || b->jgp != 0 || b->kgp != 0 || (b->phys_faces & LSM_FACE_ALL) != LSM_FACE_ALL)
+        return fail(LSM_ERR_UNSUPPORTED,
+                    "only single blocks without ghost layers and with six physical faces are "
+                    "supported (igp=%d jgp=%d kgp=%d faces=0x%x)", b->igp, b->jgp, b->kgp, b->phys_faces);
+    for (int q = 0; q < 6; ++q)
+        if (b->btype[q] < 0 || b->btype[q] > 2)
+            return fail(LSM_ERR_ARG, "btype[%d] = %d (expected 0, 1 or 2)", q, b->btype[q]);
+    if ((long)ni * nj * nk > (1L << 40)) return fail(LSM_ERR_ARG, "block too large");
+    o->e.ni = ni; o->e.nj = nj; o->e.nk = nk;
+    o->e.sj = ni; o->e.sk = (long)ni * nj;
+    o->e.faces = b->phys_faces & LSM_FACE_ALL;
+    for (int q = 0; q < 6; ++q) o->btype[q] = b->btype[q];
+    o->dc = b->dc; o->de = b->de; o->dz = b->dz; o->big_phi = b->big_phi;
+    o->csi0 = b->csi[0]; o->csi1 = b->csi[1]; o->csi2 = b->csi[2];
+    o->eta0 = b->eta[0]; o->eta1 = b->eta[1]; o->eta2 = b->eta[2];
+    o->zet0 = b->zet[0]; o->zet1 = b->zet[1]; o->zet2 = b->zet[2];
+    o->aj = b->aj; o->x = b->x; o->y = b->y; o->z = b->z;
+    o->n = (long)ni * nj * nk;
+    if (need_metrics) {
+        for (int q = 0; q < 3; ++q)
+            if (!b->csi[q] || !b->eta[q] || !b->zet[q])
+                return fail(LSM_ERR_ARG, "metric pointer csi/eta/zet[%d] is NULL", q);
+        if (!b->aj) return fail(LSM_ERR_ARG, "aj is NULL");
+    }
+    if (need_coords && (!b->x || !b->y || !b->z)) return fail(LSM_ERR_ARG, "x/y/z is NULL");
+    return LSM_OK;
+}
+
+// one cached scratch buffer per device, grown on demand and kept until exit
+static std::mutex g_work_mu;
+static void *g_work[64];
+static size_t g_work_size[64];
+
+int default_work(size_t bytes, void **ptr)
+{
+    int dev = 0;
+    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) return fail(LSM_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_work_mu);
+    if (g_work_size[dev] < bytes) {
+        if (g_work[dev]) {
+            // the buffer may still be in use by queued work of earlier calls
+            rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+            if (rc) return rc;
+            (void)hipFree(g_work[dev]);
+            g_work[dev] = nullptr;
+            g_work_size[dev] = 0;
+        }
+        rc = check_hip(hipMalloc(&g_work[dev], bytes), "hipMalloc(workspace)");
+        if (rc) return rc;
+        g_work_size[dev] = bytes;
+    }
+    *ptr = g_work[dev];
+    return LSM_OK;
+}
+
+static int workspace(void *work, size_t work_bytes, size_t need, Carver *c)
+{
+    if (need == 0) need = 256;
+    if (work) {
+        if (work_bytes < need)
+            return fail(LSM_ERR_WORKSPACE, "workspace of %zu bytes, %zu needed", work_bytes, need);
+        *c = Carver{(char *)work, work_bytes, 0};
+        return LSM_OK;
+    }
+    void *p = nullptr;
+    int rc = default_work(need, &p);
+    if (rc) return rc;
+    *c = Carver{(char *)p, need, 0};
+    return LSM_OK;
+}
+
+// ---- launch timing ----
+static std::mutex g_t_mu;
+static bool g_t_on = false;
+struct TPair { int id; hipEvent_t a, b; };
+static std::vector<TPair> g_t_pending;
+static double g_t_ms[LSM_KT_COUNT];
+static long long g_t_n[LSM_KT_COUNT];
+
+KTimer::KTimer(int kid, hipStream_t st) : id(kid), s(st)
+{
+    if (!g_t_on) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
+    (void)hipEventRecord(a, s);
+}
+
+KTimer::~KTimer()
+{
+    if (!a) return;
+    (void)hipEventRecord(b, s);
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    g_t_pending.push_back(TPair{id, a, b});
+}
+
+static void drain_timings()
+{
+    for (auto &p : g_t_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            g_t_ms[p.id] += ms;
+            g_t_n[p.id] += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    g_t_pending.clear();
+}
+
+static size_t field_bytes(const Blk &b) { return ((size_t)b.n * sizeof(double) + 255) & ~(size_t)255; }
+
+static size_t op_bytes(const Blk &b, int op)
+{
+    switch (op) {
+    case LSM_OP_RK3: return field_bytes(b) + 256;
+    case LSM_OP_REINIT_ENO2: return reinit_eno2_work(b);
+    case LSM_OP_REINIT_GODUNOV: return reinit_godunov_work(b);
+    case LSM_OP_WATER_VOLUME: return water_volume_work(b);
+    case LSM_OP_LEVELSET_STEP: {
+        size_t a = 3 * field_bytes(b) + field_bytes(b) + field_bytes(b) + 1024;   // ucn, phi copy, rk3 scratch
+        size_t r = reinit_eno2_work(b);
+        size_t v = water_volume_work(b);
+        return a + (r > v ? r : v);
+    }
+    }
+    return 0;
+}
+
+}  // namespace lsm
+
+using namespace lsm;
+
+extern "C" {
+
+const char *lsm_version(void) { return "lsm-mi355x 0.1 (gfx950, fp64)"; }
+
+const char *lsm_last_error(void) { return g_err; }
+
+int lsm_timing_enable(int on)
+{
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    drain_timings();
+    for (int q = 0; q < LSM_KT_COUNT; ++q) { g_t_ms[q] = 0.0; g_t_n[q] = 0; }
+    g_t_on = on != 0;
+    return LSM_OK;
+}
+
+int lsm_timing_get(int id, double *ms_total, int64_t *launches)
+{
+    if (id < 0 || id >= LSM_KT_COUNT) return fail(LSM_ERR_ARG, "timing id %d out of range", id);
+    std::lock_guard<std::mutex> lk(g_t_mu);
+    drain_timings();
+    if (ms_total) *ms_total = g_t_ms[id];
+    if (launches) *launches = g_t_n[id];
+    return LSM_OK;
+}
+
+size_t lsm_workspace_bytes(const LsmBlock *blk, int op)
+{
+    Blk b;
+    if (make_blk(blk, &b, false, false)) return 0;
+    return op_bytes(b, op);
+}
+
+int lsm_contra_velocity(const LsmBlock *blk, const double *u, const double *v, const double *w,
+                        int64_t vel_stride, double *ucn1, double *ucn2, double *ucn3, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false);
+    if (rc) return rc;
+    if (!u || !v || !w || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL array");
+    if (vel_stride < 1) return fail(LSM_ERR_ARG, "vel_stride must be >= 1");
+    return launch_contra(b, u, v, w, (long)vel_stride, ucn1, ucn2, ucn3, (hipStream_t)stream);
+}
+
+int lsm_weno3_rhs(const LsmBlock *blk, const double *phi, const double *ucn1, const double *ucn2,
+                  const double *ucn3, const int32_t *adv_mask, double *rhs, int mode, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!phi || !ucn1 || !ucn2 || !ucn3 || !rhs) return fail(LSM_ERR_ARG, "NULL array");
+    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
+    return launch_weno3_rhs(b, phi, ucn1, ucn2, ucn3, adv_mask, rhs, mode, (hipStream_t)stream);
+}
+
+int lsm_bcond(const LsmBlock *blk, double *phi, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!phi) return fail(LSM_ERR_ARG, "NULL array");
+    return launch_bcond(b, phi, (hipStream_t)stream);
+}
+
+int lsm_rk3_step(const LsmBlock *blk, double *phi_np1, const double *phi_n, const int32_t *adv_mask,
+                 const double *ucn1, const double *ucn2, const double *ucn3, double delti_lsm,
+                 int mode, void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!b.aj) return fail(LSM_ERR_ARG, "aj is NULL");
+    if (!phi_np1 || !phi_n || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL array");
+    if (phi_np1 == phi_n) return fail(LSM_ERR_ARG, "phi_np1 and phi_n must not alias");
+    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3), &c);
+    if (rc) return rc;
+    double *scratch = (double *)c.take(b.n * sizeof(double));
+    if (!scratch) return fail(LSM_ERR_WORKSPACE, "workspace too small");
+    return launch_rk3(b, phi_np1, phi_n, adv_mask, ucn1, ucn2, ucn3, delti_lsm, mode, scratch,
+                      (hipStream_t)stream);
+}
+
+int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizero, int order_bc,
+                      int eno_bc, double *normgrad, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false);
+    if (rc) return rc;
+    if (!d || !phizero || !normgrad) return fail(LSM_ERR_ARG, "NULL array");
+    if (order_bc < 1 || order_bc > 3) return fail(LSM_ERR_ARG, "order_bc = %d (1, 2 or 3)", order_bc);
+    return launch_normgrad(b, d, phizero, order_bc, eno_bc, normgrad, (hipStream_t)stream);
+}
+
+static int check_reinit(const LsmReinitParams *p)
+{
+    if (!p) return fail(LSM_ERR_ARG, "LsmReinitParams is NULL");
+    if (p->order_bc < 1 || p->order_bc > 3) return fail(LSM_ERR_ARG, "order_bc = %d (1, 2 or 3)", p->order_bc);
+    if (p->rn_itermax < 0) return fail(LSM_ERR_ARG, "rn_itermax < 0");
+    if (!(p->epslsm > 0.0)) return fail(LSM_ERR_ARG, "epslsm must be > 0");
+    return LSM_OK;
+}
+
+int lsm_reinit_sussman_eno2(const LsmBlock *blk, const LsmReinitParams *p, const double *phi0,
+                            double *phi_out, int32_t *iters_done, void *work, size_t work_bytes,
+                            void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false);
+    if (rc) return rc;
+    if ((rc = check_reinit(p))) return rc;
+    if (!phi0 || !phi_out) return fail(LSM_ERR_ARG, "NULL array");
+    if (phi0 == phi_out) return fail(LSM_ERR_ARG, "phi0 and phi_out must not alias");
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_ENO2), &c);
+    if (rc) return rc;
+    return launch_reinit_eno2(b, *p, phi0, phi_out, iters_done, c, (hipStream_t)stream);
+}
+
+int lsm_sign_ini(const LsmBlock *blk, const double *phi0, double epslsm, int m_sphi0,
+                 double *sign_out, double *normgrad_out, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false);
+    if (rc) return rc;
+    if (!phi0 || !sign_out || !normgrad_out) return fail(LSM_ERR_ARG, "NULL array");
+    return launch_sign_ini(b, phi0, epslsm, m_sphi0, sign_out, normgrad_out, (hipStream_t)stream);
+}
+
+int lsm_rh_rn(const LsmBlock *blk, const double *d, const double *sign, double *rhs, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false);
+    if (rc) return rc;
+    if (!d || !sign || !rhs) return fail(LSM_ERR_ARG, "NULL array");
+    return launch_rh_rn(b, d, sign, rhs, (hipStream_t)stream);
+}
+
+int lsm_lambda_rn(const LsmBlock *blk, const double *phi0, const double *normgrad0, const double *d,
+                  double timestep, double epslsm, double *lambda, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!phi0 || !normgrad0 || !d || !lambda) return fail(LSM_ERR_ARG, "NULL array");
+    return launch_lambda_rn(b, phi0, normgrad0, d, timestep, epslsm, lambda, (hipStream_t)stream);
+}
+
+int lsm_reinit_godunov(const LsmBlock *blk, const LsmReinitParams *p, const double *phi0,
+                       double *phi_out, void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false);
+    if (rc) return rc;
+    if ((rc = check_reinit(p))) return rc;
+    if (!phi0 || !phi_out) return fail(LSM_ERR_ARG, "NULL array");
+    if (phi0 == phi_out) return fail(LSM_ERR_ARG, "phi0 and phi_out must not alias");
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_GODUNOV), &c);
+    if (rc) return rc;
+    return launch_reinit_godunov(b, *p, phi0, phi_out, c, (hipStream_t)stream);
+}
+
+int lsm_water_volume(const LsmBlock *blk, const double *phi, double *vol_out, int32_t *adv_mask,
+                     void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, true);
+    if (rc) return rc;
+    if (!phi || !vol_out) return fail(LSM_ERR_ARG, "NULL array");
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_WATER_VOLUME), &c);
+    if (rc) return rc;
+    return launch_water_volume(b, phi, vol_out, adv_mask, c, (hipStream_t)stream);
+}
+
+int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p, double *phi,
+                      const double *phi_n, const double *u, const double *v, const double *w,
+                      int64_t vel_stride, int numiter_lsm, double delti_lsm, int call_levelset,
+                      int call_reinit, int rn_freq, int ntime, int mode, double *vol_out,
+                      void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, vol_out != nullptr);
+    if (rc) return rc;
+    if (!phi || !phi_n) return fail(LSM_ERR_ARG, "NULL array");
+    if (call_levelset && (!u || !v || !w)) return fail(LSM_ERR_ARG, "NULL velocity");
+    if (vel_stride < 1) return fail(LSM_ERR_ARG, "vel_stride must be >= 1");
+    if (numiter_lsm < 0) return fail(LSM_ERR_ARG, "numiter_lsm < 0");
+    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
+    const bool do_reinit = call_reinit && rn_freq > 0 && ntime % rn_freq == 0;
+    if (do_reinit && (rc = check_reinit(p))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_LEVELSET_STEP), &c);
+    if (rc) return rc;
+    const size_t fb = b.n * sizeof(double);
+    double *U1 = (double *)c.take(fb), *U2 = (double *)c.take(fb), *U3 = (double *)c.take(fb);
+    double *cur = (double *)c.take(fb), *tmp = (double *)c.take(fb);
+    if (!U1 || !U2 || !U3 || !cur || !tmp) return fail(LSM_ERR_WORKSPACE, "workspace too small");
+    // levelsetmethod.F90:178-196: pre-advection water volume
+    if (vol_out) {
+        Carver vc = c;
+        if ((rc = launch_water_volume(b, phi, vol_out, nullptr, vc, s))) return rc;
+    }
+    // levelsetmethod.F90:221-243: contravariant velocity, numiter_lsm RK3 steps from phi_n
+    if (call_levelset) {
+        if ((rc = launch_contra(b, u, v, w, (long)vel_stride, U1, U2, U3, s))) return rc;
+        if ((rc = launch_copy(phi_n, cur, b.n, s))) return rc;
+        for (int it = 0; it < numiter_lsm; ++it) {
+            if ((rc = launch_rk3(b, phi, cur, nullptr, U1, U2, U3, delti_lsm, mode, tmp, s))) return rc;
+            if (it + 1 < numiter_lsm && (rc = launch_copy(phi, cur, b.n, s))) return rc;
+        }
+    }
+    // levelsetmethod.F90:290-326: Sussman reinitialisation, bcond_lsm, phi = phicorrected
+    if (do_reinit) {
+        Carver rcv = c;
+        if ((rc = launch_reinit_eno2(b, *p, phi, tmp, nullptr, rcv, s))) return rc;
+        if ((rc = launch_bcond(b, tmp, s))) return rc;
+        if ((rc = launch_copy(tmp, phi, b.n, s))) return rc;
+    }
+    return LSM_OK;
+}
+
+}  // extern "C"

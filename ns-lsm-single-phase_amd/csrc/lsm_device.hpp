@@ -1,0 +1,201 @@
+// Device-side arithmetic of the level-set transport path.
+//
+// Every function keeps the operation order of the reference routine it
+// restates (cited per function); the library is compiled with
+// -ffp-contract=off so no multiply-add is fused behind the source's back.
+// The one deliberate exception is weno3_fast(), which rewrites the WENO3
+// weights with a single division (opted in by LSM_ADVECT_FAST).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lsm {
+
+constexpr double kEpsSims = 2.220446049250313e-16;   // epsilon(1._rdf), precision.F90:62
+constexpr double kPi = 3.141592653589793;            // precision.F90:51
+constexpr double kTiny = 2.2250738585072014e-308;    // tiny(eps_sims), TetrahedronMethods.F90:177
+
+__device__ __forceinline__ double sign1(double x) { return copysign(1.0, x); }
+
+// GetWENO3Reconstruction, AdvectionMethods.F90:195-245 (reference order: four divisions)
+__device__ __forceinline__ double weno3_exact(double f0, double f1, double f2, double f3, double ws)
+{
+    const double d0 = 2.0 / 3.0, d1 = 1.0 / 3.0, e = 0.000001;
+    const bool pos = ws > 0.0;
+    const double fL = pos ? f0 : f3;
+    const double fC = pos ? f1 : f2;
+    const double fR = pos ? f2 : f1;
+    const double b0 = (fC - fR) * (fC - fR);
+    const double b1 = (fL - fC) * (fL - fC);
+    const double a0 = d0 / ((b0 + e) * (b0 + e));
+    const double a1 = d1 / ((b1 + e) * (b1 + e));
+    const double w0 = a0 / (a0 + a1);
+    const double w1 = a1 / (a0 + a1);
+    const double u0 = 0.5 * fC + 0.5 * fR;
+    const double u1 = (-0.5) * fL + 1.5 * fC;
+    return u0 * w0 + u1 * w1;
+}
+
+// The same reconstruction with the weights folded into one quotient:
+//   w0 = d0/(b0+e)^2 / (d0/(b0+e)^2 + d1/(b1+e)^2) = 2 A1 / (2 A1 + A0),
+//   A0 = (b0+e)^2, A1 = (b1+e)^2, so the face value is
+//   ((fC+fR) A1 + (1.5 fC - 0.5 fL) A0) / (2 A1 + A0).
+// Agrees with weno3_exact to a few ulp of the face value.
+__device__ __forceinline__ double weno3_fast(double f0, double f1, double f2, double f3, double ws)
+{
+#pragma clang fp contract(fast)
+    const bool pos = ws > 0.0;
+    const double fL = pos ? f0 : f3;
+    const double fC = pos ? f1 : f2;
+    const double fR = pos ? f2 : f1;
+    const double e = 0.000001;
+    const double s0 = (fC - fR) * (fC - fR) + e;
+    const double s1 = (fL - fC) * (fL - fC) + e;
+    const double A0 = s0 * s0;
+    const double A1 = s1 * s1;
+    const double u1 = 1.5 * fC - 0.5 * fL;
+    const double num = (fC + fR) * A1 + u1 * A0;
+    const double den = 2.0 * A1 + A0;
+    return num / den;
+}
+
+template <bool FAST>
+__device__ __forceinline__ double weno3(double f0, double f1, double f2, double f3, double ws)
+{
+    if constexpr (FAST) return weno3_fast(f0, f1, f2, f3, ws);
+    else return weno3_exact(f0, f1, f2, f3, ws);
+}
+
+// One direction of calc_RH_AD_WENO3.F90:195-254: U/J * (d * (Phi+ - Phi-)),
+// with the outward face forced upwind at the first/last interior node
+// (calc_RH_AD_WENO3.F90:201-237).
+template <bool FAST>
+__device__ __forceinline__ double weno3_dir(double m2, double m1, double c0, double p1, double p2,
+                                            double ws, bool forced_minus, bool forced_plus, double d)
+{
+    const double wsp = forced_plus ? 1.0 : ws;
+    const double f3 = forced_plus ? 0.0 : p2;
+    const double fp = weno3<FAST>(m1, c0, p1, f3, wsp);
+    const double wsm = forced_minus ? -1.0 : ws;
+    const double f0 = forced_minus ? 0.0 : m2;
+    const double fm = weno3<FAST>(f0, m1, c0, p1, wsm);
+    return ws * (d * (fp - fm));
+}
+
+// GetENO2Reconstruction, AdvectionMethods.F90:64-191
+__device__ __forceinline__ double eno2(double f0, double fLL, double fL, double fC, double fR,
+                                       double fRR, bool left_out, bool right_out)
+{
+    double a, b, Mf, dp, dm;
+    a = (fR - 2.0 * fC) + fL;
+    if (right_out) {
+        dp = (fR - fC) - 0.5 * a;
+    } else {
+        b = (fRR - 2.0 * fR) + fC;
+        Mf = (fabs(b) < fabs(a)) ? b : a;
+        dp = (fR - fC) - 0.5 * Mf;
+    }
+    if (left_out) {
+        dm = (fC - fL) + 0.5 * a;
+    } else {
+        b = (fC - 2.0 * fL) + fLL;
+        Mf = (fabs(b) < fabs(a)) ? b : a;
+        dm = (fC - fL) + 0.5 * Mf;
+    }
+    // a left-outbounded stencil takes precedence (the reference tests it first)
+    if (left_out && right_out) {
+        b = (fRR - 2.0 * fR) + fC;
+        Mf = (fabs(b) < fabs(a)) ? b : a;
+        dp = (fR - fC) - 0.5 * Mf;
+        dm = (fC - fL) + 0.5 * a;
+    }
+    const double s = (fabs(f0) > kEpsSims) ? sign1(f0) : 0.0;
+    if (s * (fR - fC) < 0.0 && s * (fC - fL) < -(s * (fR - fC))) return dp;
+    if (s * (fC - fL) > 0.0 && s * (fR - fC) > -(s * (fC - fL))) return dm;
+    return 0.5 * (dp + dm);
+}
+
+// ENO2Boundary, AdvectionMethods.F90:654-707
+__device__ __forceinline__ double eno2_boundary(double f0, double f1, double f2, int dir)
+{
+    const double as = (double)dir;
+    const double D1 = as * (f1 - f0);
+    const double D2 = as * (f2 - f1);
+    const double D3 = as * (f2 - f0) / 2.0;
+    if (fabs(D1 - D2) < kEpsSims) return D1;
+    if (fabs(D1) < fabs(D2)) return (2.0 * D1 + D2) / 3.0;
+    if (fabs(D2) < fabs(D1)) return (D1 + 2.0 * D2) / 3.0;
+    if (fabs(D1) == fabs(D2)) return (4.0 * D1 + D2) / 5.0;
+    if (fabs(D1) < fabs(D3) && fabs(D2) < fabs(D3)) return ((3.0 * D1 + 6.0 * D2) - D3) / 7.0;
+    if (fabs(D1) >= fabs(D3) && fabs(D2) >= fabs(D3)) return D3;
+    return D1;
+}
+
+// BiasedDerivative, AdvectionMethods.F90:248-302
+__device__ __forceinline__ double biased(double f0, double f1, double f2, double f3, int order,
+                                         int dir, bool enobc)
+{
+    const double as = (double)dir;
+    double r = 0.0;
+    if (order == 1) r = as * (-(1.0 * f0) + 1.0 * f1);
+    if (order == 2) {
+        r = as * ((-(3.0 * f0) + 4.0 * f1) - 1.0 * f2) / 2.0;
+        if (enobc) r = eno2_boundary(f0, f1, f2, dir);
+    }
+    if (order == 3) r = as * (((-(11.0 * f0) + 18.0 * f1) - 9.0 * f2) + 2.0 * f3) / 6.0;
+    return r;
+}
+
+// GetSignPhi0, AdvectionMethods.F90:576-587
+__device__ __forceinline__ double sign_phi0(double p, double eps)
+{
+    if (fabs(p) > eps) return sign1(p);
+    return p / eps - (1.0 / kPi) * sin((kPi * p) / eps);
+}
+
+// deltaf, global_lsm.F90:120-133
+__device__ __forceinline__ double deltaf(double v, double eps)
+{
+    if (fabs(v) > eps) return 0.0;
+    return (1.0 / (2.0 * eps)) * (1.0 + cos((kPi * v) / eps));
+}
+
+// GetLambdaMassCorrection, global_lsm.F90:200-219
+__device__ __forceinline__ double lambda_mass(double p, double s, double g, double eps)
+{
+    const double del = deltaf(p, eps);
+    const double num = -((del * s) * (1.0 - g));
+    const double den = (del * del) * g;
+    if (fabs(den) < kEpsSims) return 0.0;
+    return num / den;
+}
+
+// One face of bcond_lsm.F90:53-287.  bt: 0 copy the neighbour, 1 second-order
+// extrapolation switching to first order where |phi(2)| >= BigPhi, 2 nothing
+// except linear extrapolation on the im face.  `self` is the node's current
+// value (returned unchanged for type 2 on the other faces).
+__device__ __forceinline__ double bc_face(int bt, double p1, double p2, double self, double big,
+                                          bool is_im)
+{
+    if (bt == 0) return p1;
+    if (bt == 1) {
+        const double a2 = 4.0 / 3.0, b2 = -1.0 / 3.0, a1 = 1.0, b1 = 0.0;
+        const double ex = (1.0 + sign1(fabs(p2) - big)) / 2.0;
+        const double ta = a2 + ex * (a1 - a2);
+        const double tb = b2 + ex * (b1 - b2);
+        return ta * p1 + tb * p2;
+    }
+    if (bt == 2 && is_im) return 2.0 * p1 - 1.0 * p2;
+    return self;
+}
+
+// block-level extents passed to kernels (0-based local indices)
+struct Ext {
+    int ni, nj, nk;      // array extents
+    long sj, sk;         // strides of j and k
+    int faces;           // LSM_FACE_* bits
+    __host__ __device__ long idx(int i, int j, int k) const { return (long)i + sj * j + sk * k; }
+};
+
+}  // namespace lsm

@@ -1,0 +1,88 @@
+// Host-side plumbing shared by the kernel files: the validated block
+// description, error reporting and launcher declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "lsm_device.hpp"
+#include "lsm_hip.h"
+
+namespace lsm {
+
+// Everything a kernel needs about one block, by value.
+struct Blk {
+    Ext e;
+    int btype[6];
+    double dc, de, dz, big_phi;
+    const double *csi0, *csi1, *csi2, *eta0, *eta1, *eta2, *zet0, *zet1, *zet2;
+    const double *aj, *x, *y, *z;
+    long n;                 // nodes in the block
+};
+
+// thread-local last-error text
+int fail(int code, const char *fmt, ...);
+int check_hip(hipError_t e, const char *what);
+
+// validates the descriptor; `need_metrics` / `need_coords` select which
+// device pointers must be present
+int make_blk(const LsmBlock *b, Blk *out, bool need_metrics, bool need_coords);
+
+// library-owned per-device scratch (used when the caller passes work = NULL)
+int default_work(size_t bytes, void **ptr);
+
+// carve `n` bytes, 256-byte aligned, from a workspace cursor
+struct Carver {
+    char *base;
+    size_t size, used;
+    void *take(size_t n)
+    {
+        size_t off = (used + 255) & ~(size_t)255;
+        if (off + n > size) return nullptr;
+        used = off + n;
+        return base + off;
+    }
+};
+
+// per-kernel launch timing (lsm_timing_*): begin/end bracket one launch
+struct KTimer {
+    int id;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(int kid, hipStream_t st);
+    ~KTimer();
+};
+
+// ---- launchers (lsm_advect.hip) ----
+int launch_contra(const Blk &b, const double *u, const double *v, const double *w, long stride,
+                  double *U1, double *U2, double *U3, hipStream_t s);
+int launch_weno3_rhs(const Blk &b, const double *phi, const double *U1, const double *U2,
+                     const double *U3, const int32_t *mask, double *rhs, int mode, hipStream_t s);
+int launch_bcond(const Blk &b, double *phi, hipStream_t s);
+int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t *mask,
+               const double *U1, const double *U2, const double *U3, double dt, int mode,
+               double *scratch, hipStream_t s);
+int launch_copy(const double *src, double *dst, long n, hipStream_t s);
+
+// ---- launchers (lsm_reinit.hip) ----
+size_t reinit_eno2_work(const Blk &b);
+size_t reinit_godunov_work(const Blk &b);
+int launch_normgrad(const Blk &b, const double *d, const double *phi0, int order_bc, int eno_bc,
+                    double *ng, hipStream_t s);
+int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi0, double *out,
+                       int32_t *iters, Carver &w, hipStream_t s);
+int launch_sign_ini(const Blk &b, const double *phi0, double eps, int m_sphi0, double *sgn,
+                    double *ng, hipStream_t s);
+int launch_rh_rn(const Blk &b, const double *d, const double *sgn, double *rh, hipStream_t s);
+int launch_lambda_rn(const Blk &b, const double *phi0, const double *ng0, const double *d,
+                     double dt, double eps, double *lam, hipStream_t s);
+int launch_reinit_godunov(const Blk &b, const LsmReinitParams &p, const double *phi0,
+                          double *out, Carver &w, hipStream_t s);
+
+// ---- launchers (lsm_volume.hip) ----
+size_t water_volume_work(const Blk &b);
+int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *mask, Carver &w,
+                        hipStream_t s);
+
+}  // namespace lsm

@@ -1,0 +1,468 @@
+// Pseudo-time reinitialisation of the level-set function.
+//
+// Sussman/ENO2 path (reinitialisation_test2 + calc_NormGrad_d): each RK stage
+// is one kernel that evaluates |grad d| at a node and applies the update
+// there (s(phi0), delta(d) and the Sussman-Fatemi lambda are pointwise), so a
+// stage reads d once with its stencil, phi0 and the nine metric terms.  The
+// stage-1 |grad d| is the convergence mask of the iteration (d^(1) = d^n), kept
+// as one byte per node for stages 2 and 3.  The exit test of the next
+// iteration (max/min of |grad d| over the block) is reduced inside stage 3
+// with wavefront shuffles and 64-bit atomics; a one-thread kernel turns it
+// into a device flag, so all RNitermax iterations are queued without a host
+// round trip and iterations after convergence return at once.
+//
+// Godunov path (reinitialization_benchmark + calc_sign_ini / calc_RH_RN /
+// calc_lambda_RN): one kernel per RK stage; the lambda correction is fused
+// into the last stage.
+#include "lsm_internal.hpp"
+
+namespace lsm {
+
+namespace {
+
+struct Geo {
+    int ni, nj, nk;
+    long sj, sk;
+    double dd[3];
+    const double *m[9];   // csi0..2, eta0..2, zet0..2
+};
+
+Geo make_geo(const Blk &b)
+{
+    Geo g;
+    g.ni = b.e.ni; g.nj = b.e.nj; g.nk = b.e.nk;
+    g.sj = b.e.sj; g.sk = b.e.sk;
+    g.dd[0] = b.dc; g.dd[1] = b.de; g.dd[2] = b.dz;
+    g.m[0] = b.csi0; g.m[1] = b.csi1; g.m[2] = b.csi2;
+    g.m[3] = b.eta0; g.m[4] = b.eta1; g.m[5] = b.eta2;
+    g.m[6] = b.zet0; g.m[7] = b.zet1; g.m[8] = b.zet2;
+    return g;
+}
+
+struct Ctl {
+    unsigned long long maxbits, minbits;
+    int active, iters;
+};
+
+constexpr unsigned long long kInfBits = 0x7ff0000000000000ull;
+
+__device__ __forceinline__ void node_of(const Geo &g, long l, int &i, int &j, int &k)
+{
+    i = (int)(l % g.ni);
+    const long r = l / g.ni;
+    j = (int)(r % g.nj);
+    k = (int)(r / g.nj);
+}
+
+// |grad d| at node (i,j,k): calc_NormGrad_d.F90:39-2446 for a block whose six
+// faces are physical.  phi0s is phizero at the last interior node, which the
+// i = 1 / i = im face blocks use for their ENO2 upwind sign (those blocks never
+// assign phi0, calc_NormGrad_d.F90:187-367).
+__device__ double normgrad_node(const Geo &g, const double *__restrict__ d,
+                                const double *__restrict__ phi0, double phi0s, int order_bc,
+                                bool eno_bc, int i, int j, int k, long l)
+{
+    const int n[3] = {g.ni, g.nj, g.nk};
+    const int idx[3] = {i, j, k};
+    const long st[3] = {1, g.sj, g.sk};
+    bool on[3];
+    for (int a = 0; a < 3; ++a) on[a] = (idx[a] == 0 || idx[a] == n[a] - 1);
+    const double p0 = (on[0] && !on[1] && !on[2]) ? phi0s : phi0[l];
+    double gr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const long s = st[a];
+        if (on[a]) {
+            const int dir = (idx[a] == 0) ? 1 : -1;
+            const long ss = s * dir;
+            if (a == 0 && i == 0 && k == 0 && !on[1]) {
+                // edge i = 1, k = 1: fixed second-order one-sided formula
+                // (calc_NormGrad_d.F90:1281-1283)
+                const double dc2 = 0.5 * g.dd[0];
+                gr[a] = dc2 * ((-(1.0 * d[l + 2]) + 4.0 * d[l + 1]) - 3.0 * d[l]);
+            } else {
+                gr[a] = g.dd[a] * biased(d[l], d[l + ss], d[l + 2 * ss], d[l + 3 * ss], order_bc, dir, eno_bc);
+            }
+        } else {
+            const bool lo = (idx[a] == 1), hi = (idx[a] == n[a] - 2);
+            const double fLL = lo ? 0.0 : d[l - 2 * s];
+            const double fRR = hi ? 0.0 : d[l + 2 * s];
+            gr[a] = g.dd[a] * eno2(p0, fLL, d[l - s], d[l], d[l + s], fRR, lo, hi);
+        }
+    }
+    const double gx = (g.m[0][l] * gr[0] + g.m[3][l] * gr[1]) + g.m[6][l] * gr[2];
+    const double gy = (g.m[1][l] * gr[0] + g.m[4][l] * gr[1]) + g.m[7][l] * gr[2];
+    const double gz = (g.m[2][l] * gr[0] + g.m[5][l] * gr[1]) + g.m[8][l] * gr[2];
+    return sqrt((gx * gx + gy * gy) + gz * gz);
+}
+
+__global__ __launch_bounds__(256) void normgrad_kernel(Geo g, const double *__restrict__ d,
+                                                        const double *__restrict__ phi0, int order_bc,
+                                                        int eno_bc, double *__restrict__ ng)
+{
+    const long n = (long)g.ni * g.nj * g.nk;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    int i, j, k;
+    node_of(g, l, i, j, k);
+    const double p0s = phi0[(long)(g.ni - 2) + g.sj * (g.nj - 2) + g.sk * (g.nk - 2)];
+    ng[l] = normgrad_node(g, d, phi0, p0s, order_bc, eno_bc != 0, i, j, k, l);
+}
+
+// block-wide max/min of non-negative doubles into ctl (bit patterns order like the values)
+__device__ __forceinline__ void reduce_maxmin(double vmax, double vmin, Ctl *ctl)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+        vmin = fmin(vmin, __shfl_xor(vmin, off, 64));
+    }
+    __shared__ double smax[16], smin[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { smax[wv] = vmax; smin[wv] = vmin; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        double a = smax[0], b = smin[0];
+        for (int q = 1; q < nw; ++q) { a = fmax(a, smax[q]); b = fmin(b, smin[q]); }
+        atomicMax(&ctl->maxbits, (unsigned long long)__double_as_longlong(a));
+        atomicMin(&ctl->minbits, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// max/min of |grad phi0| before the first iteration (reinitialisation_test2.F90:96-101)
+__global__ __launch_bounds__(256) void reinit_reduce0(Geo g, const double *__restrict__ phi0,
+                                                       int order_bc, int eno_bc, Ctl *ctl)
+{
+    const long n = (long)g.ni * g.nj * g.nk;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double vmax = 0.0, vmin = __longlong_as_double((long long)kInfBits);
+    if (l < n) {
+        int i, j, k;
+        node_of(g, l, i, j, k);
+        const double p0s = phi0[(long)(g.ni - 2) + g.sj * (g.nj - 2) + g.sk * (g.nk - 2)];
+        const double v = normgrad_node(g, phi0, phi0, p0s, order_bc, eno_bc != 0, i, j, k, l);
+        vmax = v; vmin = v;
+    }
+    reduce_maxmin(vmax, vmin, ctl);
+}
+
+__global__ void reinit_init_ctl(Ctl *ctl, int32_t *iters_out)
+{
+    ctl->maxbits = 0ull;
+    ctl->minbits = kInfBits;
+    ctl->active = 1;
+    ctl->iters = 0;
+    if (iters_out) *iters_out = 0;
+}
+
+// exit test at the top of a pseudo-time iteration (reinitialisation_test2.F90:100-107)
+__global__ void reinit_check(Ctl *ctl, double eps, int32_t *iters_out)
+{
+    if (ctl->active) {
+        const double mx = __longlong_as_double((long long)ctl->maxbits);
+        const double mn = __longlong_as_double((long long)ctl->minbits);
+        if (fabs(mx - 1.0) < eps && fabs(mn - 1.0) < eps) ctl->active = 0;
+        else ctl->iters += 1;
+    }
+    ctl->maxbits = 0ull;
+    ctl->minbits = kInfBits;
+    if (iters_out) *iters_out = ctl->iters;
+}
+
+struct StageP {
+    double c1, c2, c3, dt, epslsm, eps_r;
+    int order_bc, eno_bc;
+};
+
+// one RK stage of reinitialisation_test2.F90:137-181
+template <int STAGE>
+__global__ __launch_bounds__(256) void reinit_stage(Geo g, StageP p, const double *__restrict__ din,
+                                                     const double *dn, double *dout,
+                                                     const double *__restrict__ phi0,
+                                                     uint8_t *__restrict__ conv, Ctl *ctl)
+{
+    if (!ctl->active) return;
+    const long n = (long)g.ni * g.nj * g.nk;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double ngv = 0.0;
+    const bool in = l < n;
+    if (in) {
+        int i, j, k;
+        node_of(g, l, i, j, k);
+        const double p0s = phi0[(long)(g.ni - 2) + g.sj * (g.nj - 2) + g.sk * (g.nk - 2)];
+        ngv = normgrad_node(g, din, phi0, p0s, p.order_bc, p.eno_bc != 0, i, j, k, l);
+        bool upd;
+        if (STAGE == 1) {
+            upd = ngv - 1.0 > p.eps_r;
+            conv[l] = upd ? 1 : 0;
+        } else {
+            upd = conv[l] != 0;
+        }
+        const double dv = din[l];
+        double v = dv;
+        if (upd) {
+            const double dnv = (STAGE == 1) ? dv : dn[l];
+            const double s = sign_phi0(phi0[l], p.epslsm);
+            double rhs = s * (ngv - 1.0);
+            const double lam = lambda_mass(dv, s, ngv, p.epslsm);
+            const double aux = deltaf(dv, p.epslsm) * ngv;
+            rhs = rhs - lam * aux;
+            v = (p.c1 * dnv + p.c2 * dv) - (p.c3 * p.dt) * rhs;
+        }
+        dout[l] = v;
+    }
+    if (STAGE == 3) {
+        const double inf = __longlong_as_double((long long)kInfBits);
+        reduce_maxmin(in ? ngv : 0.0, in ? ngv : inf, ctl);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Godunov path
+// ---------------------------------------------------------------------------
+
+// one-sided differences with the boundary copies of calc_sign_ini.F90:41-176,
+// then the Godunov |grad| pair of calc_sign_ini.F90:204-242
+__device__ void godunov_norm(const Geo &g, const double *__restrict__ f, long l, int i, int j, int k,
+                             double &ngp, double &ngm)
+{
+    const int n[3] = {g.ni, g.nj, g.nk};
+    const int idx[3] = {i, j, k};
+    const long st[3] = {1, g.sj, g.sk};
+    double mas[3], men[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const long s = st[a];
+        if (idx[a] == 0) {
+            mas[a] = g.dd[a] * (f[l + s] - f[l]);
+            men[a] = mas[a];
+        } else if (idx[a] == n[a] - 1) {
+            men[a] = g.dd[a] * (f[l] - f[l - s]);
+            mas[a] = men[a];
+        } else {
+            mas[a] = g.dd[a] * (f[l + s] - f[l]);
+            men[a] = g.dd[a] * (f[l] - f[l - s]);
+        }
+    }
+    double bm[3], bn[3], fm[3], fn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double db = (g.m[c][l] * men[0] + g.m[3 + c][l] * men[1]) + g.m[6 + c][l] * men[2];
+        const double df = (g.m[c][l] * mas[0] + g.m[3 + c][l] * mas[1]) + g.m[6 + c][l] * mas[2];
+        double a = db / 2.0;
+        bm[c] = a + fabs(a); bn[c] = a - fabs(a);
+        a = df / 2.0;
+        fm[c] = a + fabs(a); fn[c] = a - fabs(a);
+    }
+    auto mx = [](double p, double q) { return p > q ? p : q; };
+    ngp = sqrt((mx(bm[0] * bm[0], fn[0] * fn[0]) + mx(bm[1] * bm[1], fn[1] * fn[1]))
+               + mx(bm[2] * bm[2], fn[2] * fn[2]));
+    ngm = sqrt((mx(bn[0] * bn[0], fm[0] * fm[0]) + mx(bn[1] * bn[1], fm[1] * fm[1]))
+               + mx(bn[2] * bn[2], fm[2] * fm[2]));
+}
+
+// calc_sign_ini.F90:199-270
+__global__ __launch_bounds__(256) void sign_ini_kernel(Geo g, const double *__restrict__ phi0, double eps,
+                                                        int m_sphi0, double *__restrict__ sgn,
+                                                        double *__restrict__ ng)
+{
+    const long n = (long)g.ni * g.nj * g.nk;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    int i, j, k;
+    node_of(g, l, i, j, k);
+    double gp, gm;
+    godunov_norm(g, phi0, l, i, j, k, gp, gm);
+    const double p = phi0[l];
+    const double gv = (p > 0.0) ? gp : ((p < 0.0) ? gm : 1.0);
+    double s;
+    if (m_sphi0 == 1) {
+        if (p >= eps) s = 1.0;
+        else if (p <= (-1.0) * eps) s = -1.0;
+        else s = p / eps - (1.0 / kPi) * sin((kPi * p) / eps);
+    } else {
+        s = p / sqrt(p * p + (gv * eps) * (gv * eps));
+    }
+    sgn[l] = s;
+    ng[l] = gv;
+}
+
+// calc_RH_RN.F90:199-240
+__device__ __forceinline__ double rh_rn_node(const Geo &g, const double *__restrict__ d, double s,
+                                             long l, int i, int j, int k)
+{
+    double gp, gm;
+    godunov_norm(g, d, l, i, j, k, gp, gm);
+    const double gv = (s > 0.0) ? gp : ((s < 0.0) ? gm : 1.0);
+    return s * (gv - 1.0);
+}
+
+__global__ __launch_bounds__(256) void rh_rn_kernel(Geo g, const double *__restrict__ d,
+                                                     const double *__restrict__ sgn, double *__restrict__ rh)
+{
+    const long n = (long)g.ni * g.nj * g.nk;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    int i, j, k;
+    node_of(g, l, i, j, k);
+    rh[l] = rh_rn_node(g, d, sgn[l], l, i, j, k);
+}
+
+// calc_lambda_RN.F90:42-63 (method 0)
+__device__ __forceinline__ double lambda_node(double p0, double ng0, double d, double dt, double eps,
+                                              double dc, double de, double dz)
+{
+    const double vol = ((1.0 / dc) * (1.0 / de)) * (1.0 / dz);
+    const double Lop = deltaf(p0, eps) * ((1.0 / dt) * (d - p0));
+    const double vn = vol * Lop;
+    const double del = deltaf(p0, eps);
+    const double vd = (vol * (del * del)) * ng0;
+    return (vd == 0.0) ? 0.0 : -(vn / vd);
+}
+
+__global__ __launch_bounds__(256) void lambda_kernel(long n, const double *__restrict__ phi0,
+                                                      const double *__restrict__ ng0,
+                                                      const double *__restrict__ d, double dt, double eps,
+                                                      double dc, double de, double dz,
+                                                      double *__restrict__ lam)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    lam[l] = lambda_node(phi0[l], ng0[l], d[l], dt, eps, dc, de, dz);
+}
+
+// one RK stage (or the Euler step) of reinitialization_benchmark.F90:96-158;
+// LAMBDA fuses the correction of reinitialization_benchmark.F90:166-181
+template <bool LAMBDA>
+__global__ __launch_bounds__(256) void godunov_stage(Geo g, const double *__restrict__ din,
+                                                      const double *sn, double *dout,
+                                                      const double *__restrict__ sgn,
+                                                      const double *__restrict__ phi0,
+                                                      const double *__restrict__ ng0, double c1,
+                                                      double c2, double c3, double dt, double eps)
+{
+    const long n = (long)g.ni * g.nj * g.nk;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    int i, j, k;
+    node_of(g, l, i, j, k);
+    const double rh = rh_rn_node(g, din, sgn[l], l, i, j, k);
+    double v;
+    if (c3 < 0.0) v = sn[l] - dt * rh;                         // forward Euler
+    else v = (c1 * sn[l] + c2 * din[l]) - (c3 * dt) * rh;
+    if (LAMBDA) {
+        const double lam = lambda_node(phi0[l], ng0[l], v, dt, eps, g.dd[0], g.dd[1], g.dd[2]);
+        v = v + ((dt * lam) * deltaf(phi0[l], eps)) * ng0[l];
+    }
+    dout[l] = v;
+}
+
+inline unsigned nb(long n) { return (unsigned)((n + 255) / 256); }
+
+size_t fb(const Blk &b) { return ((size_t)b.n * sizeof(double) + 255) & ~(size_t)255; }
+
+}  // namespace
+
+size_t reinit_eno2_work(const Blk &b) { return 2 * fb(b) + (((size_t)b.n + 255) & ~(size_t)255) + 1024; }
+
+size_t reinit_godunov_work(const Blk &b) { return 4 * fb(b) + 1024; }
+
+int launch_normgrad(const Blk &b, const double *d, const double *phi0, int order_bc, int eno_bc,
+                    double *ng, hipStream_t s)
+{
+    normgrad_kernel<<<nb(b.n), 256, 0, s>>>(make_geo(b), d, phi0, order_bc, eno_bc, ng);
+    return check_hip(hipGetLastError(), "normgrad_kernel");
+}
+
+int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi0, double *out,
+                       int32_t *iters, Carver &w, hipStream_t s)
+{
+    const Geo g = make_geo(b);
+    double *A = (double *)w.take(b.n * sizeof(double));
+    double *B = (double *)w.take(b.n * sizeof(double));
+    uint8_t *conv = (uint8_t *)w.take((size_t)b.n);
+    Ctl *ctl = (Ctl *)w.take(sizeof(Ctl));
+    if (!A || !B || !conv || !ctl) return fail(LSM_ERR_WORKSPACE, "reinit workspace too small");
+    int rc;
+    if ((rc = launch_copy(phi0, out, b.n, s))) return rc;
+    reinit_init_ctl<<<1, 1, 0, s>>>(ctl, iters);
+    if ((rc = check_hip(hipGetLastError(), "reinit_init_ctl"))) return rc;
+    if (p.rn_itermax <= 0) return LSM_OK;
+    reinit_reduce0<<<nb(b.n), 256, 0, s>>>(g, phi0, p.order_bc, p.eno_bc, ctl);
+    if ((rc = check_hip(hipGetLastError(), "reinit_reduce0"))) return rc;
+    static const double C[3][3] = {{1.0, 0.0, 1.0},
+                                   {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
+                                   {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    for (int it = 0; it < p.rn_itermax; ++it) {
+        reinit_check<<<1, 1, 0, s>>>(ctl, p.eps_reinit, iters);
+        StageP sp{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
+        sp.c1 = C[0][0]; sp.c2 = C[0][1]; sp.c3 = C[0][2];
+        {
+            KTimer kt(LSM_KT_REINIT_STAGE, s);
+            reinit_stage<1><<<nb(b.n), 256, 0, s>>>(g, sp, out, out, A, phi0, conv, ctl);
+        }
+        sp.c1 = C[1][0]; sp.c2 = C[1][1]; sp.c3 = C[1][2];
+        {
+            KTimer kt(LSM_KT_REINIT_STAGE, s);
+            reinit_stage<2><<<nb(b.n), 256, 0, s>>>(g, sp, A, out, B, phi0, conv, ctl);
+        }
+        sp.c1 = C[2][0]; sp.c2 = C[2][1]; sp.c3 = C[2][2];
+        {
+            KTimer kt(LSM_KT_REINIT_STAGE, s);
+            reinit_stage<3><<<nb(b.n), 256, 0, s>>>(g, sp, B, out, out, phi0, conv, ctl);
+        }
+        if ((rc = check_hip(hipGetLastError(), "reinit_stage"))) return rc;
+    }
+    return LSM_OK;
+}
+
+int launch_sign_ini(const Blk &b, const double *phi0, double eps, int m_sphi0, double *sgn, double *ng,
+                    hipStream_t s)
+{
+    sign_ini_kernel<<<nb(b.n), 256, 0, s>>>(make_geo(b), phi0, eps, m_sphi0, sgn, ng);
+    return check_hip(hipGetLastError(), "sign_ini_kernel");
+}
+
+int launch_rh_rn(const Blk &b, const double *d, const double *sgn, double *rh, hipStream_t s)
+{
+    rh_rn_kernel<<<nb(b.n), 256, 0, s>>>(make_geo(b), d, sgn, rh);
+    return check_hip(hipGetLastError(), "rh_rn_kernel");
+}
+
+int launch_lambda_rn(const Blk &b, const double *phi0, const double *ng0, const double *d, double dt,
+                     double eps, double *lam, hipStream_t s)
+{
+    lambda_kernel<<<nb(b.n), 256, 0, s>>>(b.n, phi0, ng0, d, dt, eps, b.dc, b.de, b.dz, lam);
+    return check_hip(hipGetLastError(), "lambda_kernel");
+}
+
+int launch_reinit_godunov(const Blk &b, const LsmReinitParams &p, const double *phi0, double *out,
+                          Carver &w, hipStream_t s)
+{
+    const Geo g = make_geo(b);
+    const size_t f = b.n * sizeof(double);
+    double *sgn = (double *)w.take(f), *ng0 = (double *)w.take(f);
+    double *A = (double *)w.take(f), *B = (double *)w.take(f);
+    if (!sgn || !ng0 || !A || !B) return fail(LSM_ERR_WORKSPACE, "reinit workspace too small");
+    int rc;
+    if ((rc = launch_copy(phi0, out, b.n, s))) return rc;
+    if ((rc = launch_sign_ini(b, phi0, p.epslsm, p.m_sphi0, sgn, ng0, s))) return rc;
+    const double dt = p.rn_timestep, eps = p.epslsm;
+    static const double C[3][3] = {{1.0, 0.0, 1.0},
+                                   {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
+                                   {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    for (int it = 0; it < p.rn_itermax; ++it) {
+        if (p.m_ti == 1) {
+            godunov_stage<false><<<nb(b.n), 256, 0, s>>>(g, out, out, A, sgn, phi0, ng0, C[0][0], C[0][1], C[0][2], dt, eps);
+            godunov_stage<false><<<nb(b.n), 256, 0, s>>>(g, A, out, B, sgn, phi0, ng0, C[1][0], C[1][1], C[1][2], dt, eps);
+            godunov_stage<true><<<nb(b.n), 256, 0, s>>>(g, B, out, out, sgn, phi0, ng0, C[2][0], C[2][1], C[2][2], dt, eps);
+        } else {
+            godunov_stage<true><<<nb(b.n), 256, 0, s>>>(g, out, out, A, sgn, phi0, ng0, 0.0, 0.0, -1.0, dt, eps);
+            if ((rc = launch_copy(A, out, b.n, s))) return rc;
+        }
+        if ((rc = check_hip(hipGetLastError(), "godunov_stage"))) return rc;
+    }
+    return LSM_OK;
+}
+
+}  // namespace lsm

@@ -1,0 +1,100 @@
+"""ctypes binding of liblsm_hip.so (the C ABI declared in include/lsm_hip.h).
+
+The library must be built (python -m lsm_amd.build); loading fails loudly
+otherwise -- there is no host fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "liblsm_hip.so")
+
+LSM_OK = 0
+LSM_FACE_ALL = 0x3F
+LSM_ADVECT_FAST = 0
+LSM_ADVECT_EXACT = 1
+LSM_OP_RK3, LSM_OP_REINIT_ENO2, LSM_OP_REINIT_GODUNOV, LSM_OP_WATER_VOLUME, LSM_OP_LEVELSET_STEP = 1, 2, 3, 4, 5
+
+_vp = C.c_void_p
+
+
+class LsmBlock(C.Structure):
+    _fields_ = [("il", C.c_int32), ("iu", C.c_int32), ("jl", C.c_int32), ("ju", C.c_int32),
+                ("kl", C.c_int32), ("ku", C.c_int32),
+                ("igp", C.c_int32), ("jgp", C.c_int32), ("kgp", C.c_int32),
+                ("phys_faces", C.c_int32), ("btype", C.c_int32 * 6),
+                ("dc", C.c_double), ("de", C.c_double), ("dz", C.c_double),
+                ("big_phi", C.c_double),
+                ("csi", _vp * 3), ("eta", _vp * 3), ("zet", _vp * 3),
+                ("aj", _vp), ("x", _vp), ("y", _vp), ("z", _vp)]
+
+
+class LsmReinitParams(C.Structure):
+    _fields_ = [("epslsm", C.c_double), ("eps_reinit", C.c_double), ("rn_timestep", C.c_double),
+                ("rn_itermax", C.c_int32), ("order_bc", C.c_int32), ("eno_bc", C.c_int32),
+                ("m_sphi0", C.c_int32), ("m_ti", C.c_int32)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "lsm_version": (C.c_char_p, []),
+    "lsm_last_error": (C.c_char_p, []),
+    "lsm_workspace_bytes": (C.c_size_t, [C.POINTER(LsmBlock), C.c_int]),
+    "lsm_contra_velocity": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp]),
+    "lsm_weno3_rhs": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "lsm_bcond": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp]),
+    "lsm_rk3_step": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int,
+                               _vp, C.c_size_t, _vp]),
+    "lsm_normgrad_eno2": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, C.c_int, C.c_int, _vp, _vp]),
+    "lsm_reinit_sussman_eno2": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp,
+                                          _vp, _vp, C.c_size_t, _vp]),
+    "lsm_sign_ini": (C.c_int, [C.POINTER(LsmBlock), _vp, C.c_double, C.c_int, _vp, _vp, _vp]),
+    "lsm_rh_rn": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp]),
+    "lsm_lambda_rn": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp]),
+    "lsm_reinit_godunov": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp,
+                                     C.c_size_t, _vp]),
+    "lsm_water_volume": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "lsm_levelset_step": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp, _vp,
+                                    _vp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, _vp, _vp, C.c_size_t, _vp]),
+    "lsm_timing_enable": (C.c_int, [C.c_int]),
+    "lsm_timing_get": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+}
+
+EXPORTS = tuple(_SIGS)
+LSM_KT_ADVECT_STAGE, LSM_KT_ADVECT_SHELL, LSM_KT_REINIT_STAGE, LSM_KT_VOLUME = 0, 1, 2, 3
+
+_lib = None
+
+
+class LsmError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads liblsm_hip.so (importing torch first so the process uses one HIP
+    runtime) and declares every exported signature."""
+    global _lib
+    if _lib is None:
+        try:
+            import torch  # noqa: F401  (brings in the HIP runtime torch uses)
+        except ImportError:
+            pass
+        if not os.path.exists(LIB_PATH):
+            raise LsmError(f"{LIB_PATH} is not built: run `python -m lsm_amd.build` in "
+                           "ns-lsm-single-phase_amd/ (there is no host fallback)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != LSM_OK:
+        msg = load().lsm_last_error().decode(errors="replace")
+        raise LsmError(f"{what} failed ({rc}): {msg}")

@@ -1,0 +1,276 @@
+"""PyTorch-facing level-set transport operators.
+
+Each function mirrors one routine of the reference level-set path (same name,
+same argument meaning) and runs it on the GPU through the C ABI of
+liblsm_hip.so.  Fields are float64 CUDA tensors of shape (nk, nj, ni),
+contiguous (i fastest, the reference's Fortran order).  Calls are queued on
+torch's current stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import LsmBlock, LsmReinitParams, LsmError, check
+
+MODES = {"fast": _lib.LSM_ADVECT_FAST, "exact": _lib.LSM_ADVECT_EXACT}
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _f64(t: torch.Tensor, shape, name: str):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch.Tensor")
+    if not t.is_cuda or t.dtype != torch.float64 or not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous float64 CUDA tensor")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)} != block shape {tuple(shape)}")
+    return C.c_void_p(t.data_ptr())
+
+
+def _i32(t: Optional[torch.Tensor], shape, name: str):
+    if t is None:
+        return None
+    if not t.is_cuda or t.dtype != torch.int32 or not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous int32 CUDA tensor")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)} != block shape {tuple(shape)}")
+    return C.c_void_p(t.data_ptr())
+
+
+class Block:
+    """One curvilinear block resident on the GPU: metrics, coordinates and the
+    face/boundary description (the host-associated state of levelsetmethod)."""
+
+    def __init__(self, grid, btype: Sequence[int] = (1, 1, 1, 1, 1, 1), big_phi: float = 5.0,
+                 device="cuda"):
+        def dev(a):
+            return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+
+        self.shape = tuple(grid.shape)
+        nk, nj, ni = self.shape
+        self.csi = [dev(grid.csi[c]) for c in range(3)]
+        self.eta = [dev(grid.eta[c]) for c in range(3)]
+        self.zet = [dev(grid.zet[c]) for c in range(3)]
+        self.aj = dev(grid.aj)
+        self.x, self.y, self.z = dev(grid.x), dev(grid.y), dev(grid.z)
+        s = LsmBlock()
+        s.il, s.iu, s.jl, s.ju, s.kl, s.ku = 1, ni, 1, nj, 1, nk
+        s.igp = s.jgp = s.kgp = 0
+        s.phys_faces = _lib.LSM_FACE_ALL
+        s.btype[:] = [int(b) for b in btype]
+        s.dc, s.de, s.dz = grid.dc, grid.de, grid.dz
+        s.big_phi = float(big_phi)
+        s.csi[:] = [t.data_ptr() for t in self.csi]
+        s.eta[:] = [t.data_ptr() for t in self.eta]
+        s.zet[:] = [t.data_ptr() for t in self.zet]
+        s.aj = self.aj.data_ptr()
+        s.x, s.y, s.z = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
+        self.desc = s
+        self.device = self.aj.device
+
+    @property
+    def ref(self):
+        return C.byref(self.desc)
+
+    def empty(self, dtype=torch.float64):
+        return torch.empty(self.shape, dtype=dtype, device=self.device)
+
+    def zeros(self, dtype=torch.float64):
+        return torch.zeros(self.shape, dtype=dtype, device=self.device)
+
+    def workspace(self, op: int) -> torch.Tensor:
+        n = _lib.load().lsm_workspace_bytes(self.ref, op)
+        if n == 0:
+            raise LsmError("lsm_workspace_bytes failed: " + _lib.load().lsm_last_error().decode())
+        return torch.empty(n, dtype=torch.uint8, device=self.device)
+
+
+@dataclass
+class ReinitParams:
+    """Pseudo-time reinitialisation parameters (global_lsm.F90 names)."""
+    epslsm: float
+    RNtimestep: float
+    RNitermax: int = 4
+    epsReinitialisation: float = 1e-5
+    OrderReinitialisationBoundaries: int = 1
+    ENOBCReinitialisation: bool = False
+    M_Sphi0: int = 1
+    M_TI: int = 1
+
+    def c(self) -> LsmReinitParams:
+        p = LsmReinitParams()
+        p.epslsm = self.epslsm
+        p.eps_reinit = self.epsReinitialisation
+        p.rn_timestep = self.RNtimestep
+        p.rn_itermax = int(self.RNitermax)
+        p.order_bc = int(self.OrderReinitialisationBoundaries)
+        p.eno_bc = int(bool(self.ENOBCReinitialisation))
+        p.m_sphi0 = int(self.M_Sphi0)
+        p.m_ti = int(self.M_TI)
+        return p
+
+
+def _ws(blk: Block, op: int, work: Optional[torch.Tensor]):
+    if work is None:
+        work = blk.workspace(op)
+    return C.c_void_p(work.data_ptr()), work.numel(), work
+
+
+# ---------------------------------------------------------------------------
+# advection
+# ---------------------------------------------------------------------------
+
+def rhs_contra_j(blk: Block, u, v, w):
+    """rhs_contra_j.F90:2-50: contravariant velocity divided by the Jacobian,
+    returned as three tensors (ucn_j(1:3,...))."""
+    L = _lib.load()
+    out = [blk.empty() for _ in range(3)]
+    check(L.lsm_contra_velocity(blk.ref, _f64(u, blk.shape, "u"), _f64(v, blk.shape, "v"),
+                                _f64(w, blk.shape, "w"), 1, *[C.c_void_p(t.data_ptr()) for t in out],
+                                _stream()), "lsm_contra_velocity")
+    return out
+
+
+def calc_RH_AD_WENO3(blk: Block, phi, ucn, adv_mask=None, rhs=None, mode="fast"):
+    """calc_RH_AD_WENO3.F90:1-407: WENO3 advection right-hand side on interior
+    nodes (rhs is zeroed first, as ad_RKTVD3.F90:55 does)."""
+    L = _lib.load()
+    if rhs is None:
+        rhs = blk.zeros()
+    else:
+        rhs.zero_()
+    check(L.lsm_weno3_rhs(blk.ref, _f64(phi, blk.shape, "phi"), *[_f64(t, blk.shape, "ucn") for t in ucn],
+                          _i32(adv_mask, blk.shape, "adv_mask"), _f64(rhs, blk.shape, "rhs"),
+                          MODES[mode], _stream()), "lsm_weno3_rhs")
+    return rhs
+
+
+def bcond_lsm(blk: Block, phi):
+    """bcond_lsm.F90:2-290, in place."""
+    check(_lib.load().lsm_bcond(blk.ref, _f64(phi, blk.shape, "phi"), _stream()), "lsm_bcond")
+    return phi
+
+
+def ad_RKTVD3(blk: Block, phi_n, ucn, delti_lsm: float, adv_mask=None, out=None, mode="fast",
+              work=None):
+    """ad_RKTVD3.F90:1-88: phi^(n+1) = SSP-RK3 WENO3 step of phi^n."""
+    L = _lib.load()
+    if out is None:
+        out = blk.empty()
+    wp, wn, keep = _ws(blk, _lib.LSM_OP_RK3, work)
+    check(L.lsm_rk3_step(blk.ref, _f64(out, blk.shape, "out"), _f64(phi_n, blk.shape, "phi_n"),
+                         _i32(adv_mask, blk.shape, "adv_mask"),
+                         *[_f64(t, blk.shape, "ucn") for t in ucn], float(delti_lsm), MODES[mode],
+                         wp, wn, _stream()), "lsm_rk3_step")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# reinitialisation
+# ---------------------------------------------------------------------------
+
+def calc_NormGrad_d(blk: Block, d, phizero, order_bc: int = 1, eno_bc: bool = False, out=None):
+    """calc_NormGrad_d.F90:1-2448: ENO2 |grad d| with biased boundary derivatives."""
+    if out is None:
+        out = blk.empty()
+    check(_lib.load().lsm_normgrad_eno2(blk.ref, _f64(d, blk.shape, "d"), _f64(phizero, blk.shape, "phizero"),
+                                        int(order_bc), int(bool(eno_bc)), _f64(out, blk.shape, "out"),
+                                        _stream()), "lsm_normgrad_eno2")
+    return out
+
+
+def reinitialisation_test2(blk: Block, phi0, params: ReinitParams, out=None, work=None):
+    """reinitialisation_test2.F90:1-218 (non-hybrid): returns (phi_reinitialised,
+    iterations) with iterations a one-element int32 CUDA tensor."""
+    L = _lib.load()
+    if out is None:
+        out = blk.empty()
+    iters = torch.zeros(1, dtype=torch.int32, device=blk.device)
+    wp, wn, keep = _ws(blk, _lib.LSM_OP_REINIT_ENO2, work)
+    p = params.c()
+    check(L.lsm_reinit_sussman_eno2(blk.ref, C.byref(p), _f64(phi0, blk.shape, "phi0"),
+                                    _f64(out, blk.shape, "out"), C.c_void_p(iters.data_ptr()),
+                                    wp, wn, _stream()), "lsm_reinit_sussman_eno2")
+    return out, iters
+
+
+def calc_sign_ini(blk: Block, phi0, epslsm: float, M_Sphi0: int = 1):
+    """calc_sign_ini.F90:1-286: (s(phi0), |grad phi0| Godunov)."""
+    s, g = blk.empty(), blk.empty()
+    check(_lib.load().lsm_sign_ini(blk.ref, _f64(phi0, blk.shape, "phi0"), float(epslsm), int(M_Sphi0),
+                                   _f64(s, blk.shape, "sign"), _f64(g, blk.shape, "ng"), _stream()),
+          "lsm_sign_ini")
+    return s, g
+
+
+def calc_RH_RN(blk: Block, d, sign):
+    """calc_RH_RN.F90:1-244: s * (|grad d| - 1)."""
+    rh = blk.empty()
+    check(_lib.load().lsm_rh_rn(blk.ref, _f64(d, blk.shape, "d"), _f64(sign, blk.shape, "sign"),
+                                _f64(rh, blk.shape, "rh"), _stream()), "lsm_rh_rn")
+    return rh
+
+
+def calc_lambda_RN(blk: Block, phi0, normgrad0, d, timestep: float, epslsm: float):
+    """calc_lambda_RN.F90:1-122 (point-constant method)."""
+    lam = blk.empty()
+    check(_lib.load().lsm_lambda_rn(blk.ref, _f64(phi0, blk.shape, "phi0"), _f64(normgrad0, blk.shape, "ng0"),
+                                    _f64(d, blk.shape, "d"), float(timestep), float(epslsm),
+                                    _f64(lam, blk.shape, "lambda"), _stream()), "lsm_lambda_rn")
+    return lam
+
+
+def reinitialization_benchmark(blk: Block, phi0, params: ReinitParams, out=None, work=None):
+    """reinitialization_benchmark.F90:1-228 (M_Grad = 0, no obstacles, non-hybrid)."""
+    L = _lib.load()
+    if out is None:
+        out = blk.empty()
+    wp, wn, keep = _ws(blk, _lib.LSM_OP_REINIT_GODUNOV, work)
+    p = params.c()
+    check(L.lsm_reinit_godunov(blk.ref, C.byref(p), _f64(phi0, blk.shape, "phi0"),
+                               _f64(out, blk.shape, "out"), wp, wn, _stream()), "lsm_reinit_godunov")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# water volume and the full step
+# ---------------------------------------------------------------------------
+
+def TotalWaterVolume(blk: Block, phi, with_mask: bool = True, work=None):
+    """TotalWaterVolume.F90:1-297: returns (volumes, AdvectionNodes) where
+    volumes = [K-tetrahedra, second tetrahedra, bulk cells] (CUDA float64)."""
+    L = _lib.load()
+    vol = torch.zeros(3, dtype=torch.float64, device=blk.device)
+    mask = blk.empty(torch.int32) if with_mask else None
+    wp, wn, keep = _ws(blk, _lib.LSM_OP_WATER_VOLUME, work)
+    check(L.lsm_water_volume(blk.ref, _f64(phi, blk.shape, "phi"), C.c_void_p(vol.data_ptr()),
+                             _i32(mask, blk.shape, "mask"), wp, wn, _stream()), "lsm_water_volume")
+    return vol, mask
+
+
+def levelsetmethod(blk: Block, phi, phi_n, u, v, w, params: ReinitParams, delti_lsm: float,
+                   numiter_lsm: int = 1, call_levelsetmethod: bool = True,
+                   call_reinitialisation: bool = True, RNfreq: int = 1, ntime: int = 1,
+                   mode="fast", want_volume: bool = True, work=None):
+    """levelsetmethod.F90:1-372 (non-hybrid Sussman path): phi is overwritten
+    with phi^(n+1); returns the pre-advection water volumes (or None)."""
+    L = _lib.load()
+    vol = torch.zeros(3, dtype=torch.float64, device=blk.device) if want_volume else None
+    wp, wn, keep = _ws(blk, _lib.LSM_OP_LEVELSET_STEP, work)
+    p = params.c()
+    check(L.lsm_levelset_step(blk.ref, C.byref(p), _f64(phi, blk.shape, "phi"),
+                              _f64(phi_n, blk.shape, "phi_n"), _f64(u, blk.shape, "u"),
+                              _f64(v, blk.shape, "v"), _f64(w, blk.shape, "w"), 1, int(numiter_lsm),
+                              float(delti_lsm), int(bool(call_levelsetmethod)),
+                              int(bool(call_reinitialisation)), int(RNfreq), int(ntime), MODES[mode],
+                              C.c_void_p(vol.data_ptr()) if vol is not None else None, wp, wn,
+                              _stream()), "lsm_levelset_step")
+    return vol
