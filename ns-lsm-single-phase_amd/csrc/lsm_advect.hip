@@ -84,39 +84,58 @@ __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
 
     __shared__ double sm[2][TY + 2 * HALO][TX + 2 * HALO];
 
-    auto ld = [&](int ii, int jj, int kk) -> double {
-        if (ii < 0 || ii >= a.ni || jj < 0 || jj >= a.nj || kk < 0 || kk >= a.nk) return 0.0;
-        return a.ps[(long)ii + a.sj * jj + a.sk * kk];
-    };
-
-    // j-halo row handled by this thread, i-halo cell (16 of them) handled by the first 16 threads
-    const int rr = (ty < 2) ? ty : ty + TY;
-    const int jh = tj * TY - HALO + rr;
+    // Loads are clamped into the array instead of guarded: a clamped value is
+    // only ever read by a thread that does not compute (outside the interior)
+    // or by a stencil slot the boundary forcing replaces with a dummy, and
+    // unguarded loads issue back to back without branches.
+    const int ic = min(i, a.ni - 1), jc = min(j, a.nj - 1);
+    const long col = (long)ic + a.sj * jc;
+    const int rr = (ty < 2) ? ty : ty + TY;                 // j-halo row of this thread
+    const int jh = min(max(tj * TY - HALO + rr, 0), a.nj - 1);
+    const long col_jh = (long)ic + a.sj * jh;
     const int lin = ty * TX + tx;
-    const int hr = lin >> 2, hc4 = lin & 3;
+    const bool has_ih = lin < 4 * TY;                       // 16 i-halo cells
+    const int hr = (lin >> 2) & 3, hc4 = lin & 3;
     const int hcol = (hc4 < 2) ? hc4 : hc4 + TX;
-    const int hi_i = ti * TX - HALO + hcol;
-    const int hi_j = tj * TY + hr;
+    const int hi_i = min(max(ti * TX - HALO + hcol, 0), a.ni - 1);
+    const int hi_j = min(tj * TY + hr, a.nj - 1);
+    const long col_ih = (long)hi_i + a.sj * hi_j;
+    auto kcl = [&](int kk) -> long { return a.sk * (long)min(max(kk, 0), a.nk - 1); };
 
     const bool act = i >= a.r.ilo && i <= a.r.ihi && j >= a.r.jlo && j <= a.r.jhi;
     const bool fmb = a.r.fb && i == a.r.ilo, fmf = a.r.ff && i == a.r.ihi;
     const bool fml = a.r.fl && j == a.r.jlo, fmr = a.r.fr && j == a.r.jhi;
+    const double *__restrict__ ps = a.ps;
 
-    double q0 = ld(i, j, kb - 2), q1 = ld(i, j, kb - 1), q2 = ld(i, j, kb), q3 = ld(i, j, kb + 1);
+    // register queue phi(k-2 .. k+1) of this column; next-plane operands
+    double q0 = ps[col + kcl(kb - 2)], q1 = ps[col + kcl(kb - 1)];
+    double q2 = ps[col + kcl(kb)], q3 = ps[col + kcl(kb + 1)];
+    double nq4 = ps[col + kcl(kb + 2)];
+    double njh = ps[col_jh + kcl(kb)];
+    double nih = has_ih ? ps[col_ih + kcl(kb)] : 0.0;
+    long l = col + a.sk * (long)kb;
+    double nu1 = a.U1[l], nu2 = a.U2[l], nu3 = a.U3[l], naj = a.aj[l], npn = a.pn[l];
+    int nmk = MASK ? a.mask[l] : 1;
+
     int buf = 0;
     for (int k = kb; k <= ke; ++k) {
-        const double q4 = ld(i, j, k + 2);
+        const double q4 = nq4;
+        const double u1 = nu1, u2 = nu2, u3 = nu3, ajv = naj, pnv = npn;
+        const int mk = nmk;
         sm[buf][ty + HALO][tx + HALO] = q2;
-        sm[buf][rr][tx + HALO] = ld(i, jh, k);
-        if (lin < 4 * TY) sm[buf][hr + HALO][hcol] = ld(hi_i, hi_j, k);
-        const long l = (long)i + a.sj * j + a.sk * k;
-        double u1 = 0.0, u2 = 0.0, u3 = 0.0, ajv = 0.0, pnv = 0.0;
-        int mk = 1;
-        if (act) {
-            u1 = a.U1[l]; u2 = a.U2[l]; u3 = a.U3[l]; ajv = a.aj[l]; pnv = a.pn[l];
-            if (MASK) mk = a.mask[l];
-        }
+        sm[buf][rr][tx + HALO] = njh;
+        if (has_ih) sm[buf][hr + HALO][hcol] = nih;
         __syncthreads();
+        // operands of the next plane, in flight while this plane computes
+        if (k < ke) {
+            const long kn = kcl(k + 1);
+            nq4 = ps[col + kcl(k + 3)];
+            njh = ps[col_jh + kn];
+            if (has_ih) nih = ps[col_ih + kn];
+            const long ln = col + kn;
+            nu1 = a.U1[ln]; nu2 = a.U2[ln]; nu3 = a.U3[ln]; naj = a.aj[ln]; npn = a.pn[ln];
+            if (MASK) nmk = a.mask[ln];
+        }
         if (act) {
             double v = q2;
             if (mk > 0) {
@@ -131,7 +150,7 @@ __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
                 const double rh = (Lc + Le) + Lz;
                 v = (a.c1 * pnv + a.c2 * q2) - ((a.c3 * ajv) * a.dt) * rh;
             }
-            a.out[l] = v;
+            a.out[col + a.sk * (long)k] = v;
         }
         q0 = q1; q1 = q2; q2 = q3; q3 = q4;
         buf ^= 1;
@@ -160,7 +179,7 @@ struct ShellArgs {
 };
 
 template <bool MASK>
-__device__ double sh_f0(const ShellArgs &a, int i, int j, int k)
+__device__ __forceinline__ double sh_f0(const ShellArgs &a, int i, int j, int k)
 {
     const long l = (long)i + a.sj * j + a.sk * k;
     if (i >= a.r.ilo && i <= a.r.ihi && j >= a.r.jlo && j <= a.r.jhi && k >= a.r.klo && k <= a.r.khi)
@@ -170,7 +189,7 @@ __device__ double sh_f0(const ShellArgs &a, int i, int j, int k)
 }
 
 template <bool MASK>
-__device__ double sh_f1(const ShellArgs &a, int i, int j, int k)
+__device__ __forceinline__ double sh_f1(const ShellArgs &a, int i, int j, int k)
 {
     if (a.r.fb && i == a.r.ia) {
         const int bt = a.bt[0];
@@ -184,7 +203,7 @@ __device__ double sh_f1(const ShellArgs &a, int i, int j, int k)
 }
 
 template <bool MASK>
-__device__ double sh_f2(const ShellArgs &a, int i, int j, int k)
+__device__ __forceinline__ double sh_f2(const ShellArgs &a, int i, int j, int k)
 {
     if (a.r.fl && j == a.r.ja) {
         const int bt = a.bt[2];
@@ -200,7 +219,7 @@ __device__ double sh_f2(const ShellArgs &a, int i, int j, int k)
 }
 
 template <bool MASK>
-__device__ double sh_f3(const ShellArgs &a, int i, int j, int k)
+__device__ __forceinline__ double sh_f3(const ShellArgs &a, int i, int j, int k)
 {
     if (a.r.fd && k == a.r.ka) {
         const int bt = a.bt[4];

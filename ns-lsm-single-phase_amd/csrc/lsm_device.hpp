@@ -3,8 +3,10 @@
 // Every function keeps the operation order of the reference routine it
 // restates (cited per function); the library is compiled with
 // -ffp-contract=off so no multiply-add is fused behind the source's back.
-// The one deliberate exception is weno3_fast(), which rewrites the WENO3
-// weights with a single division (opted in by LSM_ADVECT_FAST).
+// The deliberate exceptions are weno3_fast() / weno3_dir_fast() / quot(),
+// which rewrite the WENO3 weights with fewer divisions and allow fused
+// multiply-adds (selected by LSM_ADVECT_FAST; LSM_ADVECT_EXACT keeps the
+// reference arithmetic).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -67,20 +69,72 @@ __device__ __forceinline__ double weno3(double f0, double f1, double f2, double 
     else return weno3_exact(f0, f1, f2, f3, ws);
 }
 
+// n / d through the hardware reciprocal: two Newton steps and one residual
+// correction (accurate to about an ulp; d is a sum of squares in (1e-24, 1e4)
+// here, far from the range where the IEEE division's scaling matters)
+__device__ __forceinline__ double quot(double n, double d)
+{
+#pragma clang fp contract(fast)
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    double q = n * r;
+    const double res = fma(-d, q, n);
+    return fma(r, res, q);
+}
+
 // One direction of calc_RH_AD_WENO3.F90:195-254: U/J * (d * (Phi+ - Phi-)),
 // with the outward face forced upwind at the first/last interior node
-// (calc_RH_AD_WENO3.F90:201-237).
+// (calc_RH_AD_WENO3.F90:201-237).  Reference arithmetic.
+__device__ __forceinline__ double weno3_dir_exact(double m2, double m1, double c0, double p1, double p2,
+                                                  double ws, bool forced_minus, bool forced_plus, double d)
+{
+    const double wsp = forced_plus ? 1.0 : ws;
+    const double f3 = forced_plus ? 0.0 : p2;
+    const double fp = weno3_exact(m1, c0, p1, f3, wsp);
+    const double wsm = forced_minus ? -1.0 : ws;
+    const double f0 = forced_minus ? 0.0 : m2;
+    const double fm = weno3_exact(f0, m1, c0, p1, wsm);
+    return ws * (d * (fp - fm));
+}
+
+// The same direction term with the weights of weno3_fast, sharing work
+// between the two faces: with Dk the squared differences of consecutive
+// stencil values, Sk = (Dk + 1e-6)^2 serve as A0/A1 of both faces and both
+// upwind choices (4 squarings instead of 8), and the two face quotients are
+// combined, Phi+ - Phi- = (Np Dm - Nm Dp) / (Dp Dm), into one division.
+__device__ __forceinline__ double weno3_dir_fast(double m2, double m1, double c0, double p1, double p2,
+                                                 double ws, bool forced_minus, bool forced_plus, double d)
+{
+#pragma clang fp contract(fast)
+    const double e = 0.000001;
+    const double am = m2 - m1, a0 = m1 - c0, a1 = c0 - p1, a2 = p1 - p2;
+    double t;
+    t = am * am + e; const double Sm = t * t;
+    t = a0 * a0 + e; const double S0 = t * t;
+    t = a1 * a1 + e; const double S1 = t * t;
+    t = a2 * a2 + e; const double S2 = t * t;
+    const bool pp = forced_plus || ws > 0.0;      // plus face: left-biased stencil
+    const bool pm = !forced_minus && ws > 0.0;    // minus face: left-biased stencil
+    const double Ap = pp ? S0 : S2;
+    const double up = pp ? (1.5 * c0 - 0.5 * m1) : (1.5 * p1 - 0.5 * p2);
+    const double Np = (c0 + p1) * Ap + up * S1;
+    const double Dp = 2.0 * Ap + S1;
+    const double Am = pm ? Sm : S1;
+    const double um = pm ? (1.5 * m1 - 0.5 * m2) : (1.5 * c0 - 0.5 * p1);
+    const double Nm = (m1 + c0) * Am + um * S0;
+    const double Dm = 2.0 * Am + S0;
+    return (ws * d) * quot(Np * Dm - Nm * Dp, Dp * Dm);
+}
+
 template <bool FAST>
 __device__ __forceinline__ double weno3_dir(double m2, double m1, double c0, double p1, double p2,
                                             double ws, bool forced_minus, bool forced_plus, double d)
 {
-    const double wsp = forced_plus ? 1.0 : ws;
-    const double f3 = forced_plus ? 0.0 : p2;
-    const double fp = weno3<FAST>(m1, c0, p1, f3, wsp);
-    const double wsm = forced_minus ? -1.0 : ws;
-    const double f0 = forced_minus ? 0.0 : m2;
-    const double fm = weno3<FAST>(f0, m1, c0, p1, wsm);
-    return ws * (d * (fp - fm));
+    if constexpr (FAST) return weno3_dir_fast(m2, m1, c0, p1, p2, ws, forced_minus, forced_plus, d);
+    else return weno3_dir_exact(m2, m1, c0, p1, p2, ws, forced_minus, forced_plus, d);
 }
 
 // GetENO2Reconstruction, AdvectionMethods.F90:64-191
