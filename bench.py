@@ -103,7 +103,22 @@ def main():
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (scales rocprof "
+                         "FETCH_SIZE/WRITE_SIZE to bytes)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": nb, "copy_calibration_bytes_written": nb,
+                          "launches": 5}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {args.gpus} is not supported "

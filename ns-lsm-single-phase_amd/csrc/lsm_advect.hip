@@ -21,7 +21,6 @@ namespace lsm {
 namespace {
 
 constexpr int TX = 64;    // tile width in i (one wavefront per j row)
-constexpr int TY = 4;     // tile height in j
 constexpr int HALO = 2;
 
 struct Range {
@@ -68,7 +67,24 @@ struct StageArgs {
     int tiles_i, tiles_j, kc, ntiles;
 };
 
-template <bool FAST, bool MASK>
+// streamed operands are used once per stage: non-temporal loads/stores keep
+// them from evicting the phi planes whose halos neighbouring tiles re-read
+// (LSM_NT=0 turns this off)
+template <bool NT>
+__device__ __forceinline__ double ldnt_(const double *p)
+{
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+template <bool NT>
+__device__ __forceinline__ void stnt_(double *p, double v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <bool FAST, bool MASK, int TY, bool NT = true>
 __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
 {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -90,12 +106,14 @@ __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
     // unguarded loads issue back to back without branches.
     const int ic = min(i, a.ni - 1), jc = min(j, a.nj - 1);
     const long col = (long)ic + a.sj * jc;
-    const int rr = (ty < 2) ? ty : ty + TY;                 // j-halo row of this thread
+    // waves 0,1 load the two halo rows below the tile, waves 2,3 the two above
+    const bool has_jh = ty < 4;
+    const int rr = (ty < 2) ? ty : ty - 2 + TY + HALO;
     const int jh = min(max(tj * TY - HALO + rr, 0), a.nj - 1);
     const long col_jh = (long)ic + a.sj * jh;
     const int lin = ty * TX + tx;
     const bool has_ih = lin < 4 * TY;                       // 16 i-halo cells
-    const int hr = (lin >> 2) & 3, hc4 = lin & 3;
+    const int hr = (lin >> 2) % TY, hc4 = lin & 3;
     const int hcol = (hc4 < 2) ? hc4 : hc4 + TX;
     const int hi_i = min(max(ti * TX - HALO + hcol, 0), a.ni - 1);
     const int hi_j = min(tj * TY + hr, a.nj - 1);
@@ -106,15 +124,18 @@ __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
     const bool fmb = a.r.fb && i == a.r.ilo, fmf = a.r.ff && i == a.r.ihi;
     const bool fml = a.r.fl && j == a.r.jlo, fmr = a.r.fr && j == a.r.jhi;
     const double *__restrict__ ps = a.ps;
+    auto ldnt = [](const double *p) { return ldnt_<NT>(p); };
+    auto stnt = [](double *p, double v) { stnt_<NT>(p, v); };
 
     // register queue phi(k-2 .. k+1) of this column; next-plane operands
     double q0 = ps[col + kcl(kb - 2)], q1 = ps[col + kcl(kb - 1)];
     double q2 = ps[col + kcl(kb)], q3 = ps[col + kcl(kb + 1)];
     double nq4 = ps[col + kcl(kb + 2)];
-    double njh = ps[col_jh + kcl(kb)];
+    double njh = has_jh ? ps[col_jh + kcl(kb)] : 0.0;
     double nih = has_ih ? ps[col_ih + kcl(kb)] : 0.0;
     long l = col + a.sk * (long)kb;
-    double nu1 = a.U1[l], nu2 = a.U2[l], nu3 = a.U3[l], naj = a.aj[l], npn = a.pn[l];
+    double nu1 = ldnt(a.U1 + l), nu2 = ldnt(a.U2 + l), nu3 = ldnt(a.U3 + l), naj = ldnt(a.aj + l),
+           npn = ldnt(a.pn + l);
     int nmk = MASK ? a.mask[l] : 1;
 
     int buf = 0;
@@ -123,17 +144,18 @@ __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
         const double u1 = nu1, u2 = nu2, u3 = nu3, ajv = naj, pnv = npn;
         const int mk = nmk;
         sm[buf][ty + HALO][tx + HALO] = q2;
-        sm[buf][rr][tx + HALO] = njh;
+        if (has_jh) sm[buf][rr][tx + HALO] = njh;
         if (has_ih) sm[buf][hr + HALO][hcol] = nih;
         __syncthreads();
         // operands of the next plane, in flight while this plane computes
         if (k < ke) {
             const long kn = kcl(k + 1);
             nq4 = ps[col + kcl(k + 3)];
-            njh = ps[col_jh + kn];
+            if (has_jh) njh = ps[col_jh + kn];
             if (has_ih) nih = ps[col_ih + kn];
             const long ln = col + kn;
-            nu1 = a.U1[ln]; nu2 = a.U2[ln]; nu3 = a.U3[ln]; naj = a.aj[ln]; npn = a.pn[ln];
+            nu1 = ldnt(a.U1 + ln); nu2 = ldnt(a.U2 + ln); nu3 = ldnt(a.U3 + ln);
+            naj = ldnt(a.aj + ln); npn = ldnt(a.pn + ln);
             if (MASK) nmk = a.mask[ln];
         }
         if (act) {
@@ -150,7 +172,7 @@ __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
                 const double rh = (Lc + Le) + Lz;
                 v = (a.c1 * pnv + a.c2 * q2) - ((a.c3 * ajv) * a.dt) * rh;
             }
-            a.out[col + a.sk * (long)k] = v;
+            stnt(a.out + col + a.sk * (long)k, v);
         }
         q0 = q1; q1 = q2; q2 = q3; q3 = q4;
         buf ^= 1;
@@ -437,9 +459,10 @@ int launch_bcond(const Blk &b, double *phi, hipStream_t s)
 
 static int stage_kc(const Blk &b, const Range &r, int tiles_ij)
 {
-    // enough k chunks for ~16 workgroups per CU (256 CUs), at least 8 planes each
+    // k chunks of 8 planes (measured best at 256^3: 8192 workgroups, the
+    // 4-plane halo of a chunk re-read from L2), more if the block is small
     const int nk_int = r.khi - r.klo + 1;
-    int chunks = (4096 + tiles_ij - 1) / tiles_ij;
+    int chunks = (8192 + tiles_ij - 1) / tiles_ij;
     int kc = (nk_int + chunks - 1) / chunks;
     if (kc < 8) kc = 8;
     if (const char *e = getenv("LSM_KCHUNK")) { int v = atoi(e); if (v > 0) kc = v; }
@@ -454,6 +477,10 @@ int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    int TY = 4;
+    if (const char *e = getenv("LSM_TILE_Y")) { int v = atoi(e); if (v == 8) TY = 8; }
+    bool nt_off = false;
+    if (const char *e = getenv("LSM_NT")) nt_off = atoi(e) == 0;
     const int tiles_i = (b.e.ni + TX - 1) / TX;
     const int tiles_j = (b.e.nj + TY - 1) / TY;
     const int kc = stage_kc(b, r, tiles_i * tiles_j);
@@ -486,10 +513,18 @@ int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t
             KTimer kt(LSM_KT_ADVECT_STAGE, s);
             dim3 blk(TX, TY, 1);
             const bool fast = mode != LSM_ADVECT_EXACT;
-            if (fast && mask) advect_stage<true, true><<<grid, blk, 0, s>>>(a);
-            else if (fast) advect_stage<true, false><<<grid, blk, 0, s>>>(a);
-            else if (mask) advect_stage<false, true><<<grid, blk, 0, s>>>(a);
-            else advect_stage<false, false><<<grid, blk, 0, s>>>(a);
+            if (TY == 8) {
+                if (fast && mask) advect_stage<true, true, 8><<<grid, blk, 0, s>>>(a);
+                else if (fast) advect_stage<true, false, 8><<<grid, blk, 0, s>>>(a);
+                else if (mask) advect_stage<false, true, 8><<<grid, blk, 0, s>>>(a);
+                else advect_stage<false, false, 8><<<grid, blk, 0, s>>>(a);
+            } else {
+                if (fast && mask) advect_stage<true, true, 4><<<grid, blk, 0, s>>>(a);
+                else if (fast && nt_off) advect_stage<true, false, 4, false><<<grid, blk, 0, s>>>(a);
+                else if (fast) advect_stage<true, false, 4><<<grid, blk, 0, s>>>(a);
+                else if (mask) advect_stage<false, true, 4><<<grid, blk, 0, s>>>(a);
+                else advect_stage<false, false, 4><<<grid, blk, 0, s>>>(a);
+            }
             int rc = check_hip(hipGetLastError(), "advect_stage");
             if (rc) return rc;
         }
