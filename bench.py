@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
+    ap.add_argument("--data-model", default="folded", choices=["folded", "reference"],
+                    help="folded: frozen-velocity operator aj*dt*d*ucn (3 fields per stage); "
+                         "reference: ucn and aj (4 fields), the reference's data model")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -144,31 +147,40 @@ def main():
     work = blk.workspace(_lib.LSM_OP_RK3)
     bufs = [phi.clone(), blk.empty()]
     interior = (n - 2) ** 3
+    folded = args.data_model == "folded" and args.mode == "fast"
+    V = api.fold_velocity(blk, ucn, dt) if folded else None
 
-    def step(src, dst):
-        api.ad_RKTVD3(blk, src, ucn, dt, out=dst, mode=args.mode, work=work)
+    def make_step(kind):
+        if kind == "folded":
+            return lambda src, dst: api.ad_RKTVD3_folded(blk, src, V, out=dst, work=work)
+        return lambda src, dst: api.ad_RKTVD3(blk, src, ucn, dt, out=dst, mode=kind, work=work)
 
-    for s in range(args.warmup):
-        step(bufs[s % 2], bufs[(s + 1) % 2])
-    torch.cuda.synchronize()
-    base = args.warmup % 2
+    def timed(kind, steps, warmup):
+        step = make_step(kind)
+        for s in range(warmup):
+            step(bufs[s % 2], bufs[(s + 1) % 2])
+        torch.cuda.synchronize()
+        base = warmup % 2
+        L.lsm_timing_enable(1)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        wall0 = time.perf_counter()
+        ev0.record()
+        for s in range(steps):
+            step(bufs[(base + s) % 2], bufs[(base + s + 1) % 2])
+        ev1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall0
+        kms, kn = C.c_double(), C.c_int64()
+        L.lsm_timing_get(_lib.LSM_KT_ADVECT_STAGE, C.byref(kms), C.byref(kn))
+        sms, sn = C.c_double(), C.c_int64()
+        L.lsm_timing_get(_lib.LSM_KT_ADVECT_SHELL, C.byref(sms), C.byref(sn))
+        L.lsm_timing_enable(0)
+        return ev0.elapsed_time(ev1), wall, kms, kn, sms, sn, (base + steps) % 2
 
-    L.lsm_timing_enable(1)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    ev0.record()
-    for s in range(args.steps):
-        step(bufs[(base + s) % 2], bufs[(base + s + 1) % 2])
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms_total = ev0.elapsed_time(ev1)
-    kms, kn = C.c_double(), C.c_int64()
-    L.lsm_timing_get(_lib.LSM_KT_ADVECT_STAGE, C.byref(kms), C.byref(kn))
-    sms, sn = C.c_double(), C.c_int64()
-    L.lsm_timing_get(_lib.LSM_KT_ADVECT_SHELL, C.byref(sms), C.byref(sn))
-    L.lsm_timing_enable(0)
+    head_kind = "folded" if folded else args.mode
+    ms_total, wall, kms, kn, sms, sn, last = timed(head_kind, args.steps, args.warmup)
+    base = last
 
     ms_per_step = ms_total / args.steps
     value = interior * args.steps / (ms_total / 1e3)
@@ -186,7 +198,17 @@ def main():
     log(f"{args.steps} steps: {ms_per_step:.3f} ms/step, {value:.3e} cell-updates/s, "
         f"stage kernel {stage_ms:.4f} ms ({achieved:.0f} GB/s alg), shell {sms.value / max(sn.value, 1):.4f} ms")
 
-    extra = {"wall_s_timed_region": round(wall, 4), "shell_avg_ms": round(sms.value / max(sn.value, 1), 5)}
+    extra = {"wall_s_timed_region": round(wall, 4), "shell_avg_ms": round(sms.value / max(sn.value, 1), 5),
+             "data_model": head_kind}
+    if not args.no_extra:
+        # the other arithmetic / data-model variants of the same step
+        for kind in ("fast", "exact"):
+            if kind == head_kind:
+                continue
+            t_ms, _, kms2, kn2, _, _, _ = timed(kind, 10, 2)
+            extra[f"{kind}_refdata_ms_per_step"] = round(t_ms / 10, 5)
+            extra[f"{kind}_refdata_cell_updates_per_s"] = interior * 10 / (t_ms / 1e3)
+            extra[f"{kind}_refdata_stage_avg_ms"] = round(kms2.value / max(kn2.value, 1), 5)
     if not args.no_extra:
         # secondary units at the same size: Sussman reinitialisation and water volume
         hmin, hmax = __import__("lsm_amd.inputs", fromlist=["x"]).spacing_bounds(g)

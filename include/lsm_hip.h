@@ -124,6 +124,21 @@ int lsm_rk3_step(const LsmBlock *blk, double *phi_np1, const double *phi_n,
                  const double *ucn3, double delti_lsm, int mode,
                  void *work, size_t work_bytes, void *stream);
 
+/* Frozen-velocity form of ad_RKTVD3 (no reference counterpart; an
+ * arithmetic rearrangement of ad_RKTVD3.F90:70-72 for callers that advance
+ * many steps with one velocity, e.g. a frozen velocity field): the operator
+ * V_d = (aj * delti_lsm * dc_d) * ucn_d is built once, then each stage
+ * streams V (3 fields) instead of ucn and aj (4 fields).  Fast arithmetic
+ * only.  lsm_advect_fold waits for the stream and fails with
+ * LSM_ERR_UNSUPPORTED if aj <= 0 anywhere (the upwind sign of V would differ
+ * from that of ucn) or delti_lsm <= 0. */
+int lsm_advect_fold(const LsmBlock *blk, const double *ucn1, const double *ucn2,
+                    const double *ucn3, double delti_lsm, double *v1, double *v2, double *v3,
+                    void *stream);
+int lsm_rk3_step_folded(const LsmBlock *blk, double *phi_np1, const double *phi_n,
+                        const int32_t *adv_mask, const double *v1, const double *v2,
+                        const double *v3, void *work, size_t work_bytes, void *stream);
+
 /* calc_NormGrad_d.F90:1-2448 (calc_NormGrad_d): |grad d| with ENO2
  * differences inside and biased differences of order `order_bc` (optionally
  * ENO2Boundary when eno_bc != 0 and order_bc == 2) on faces, edges and

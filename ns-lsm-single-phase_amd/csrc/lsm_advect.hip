@@ -84,7 +84,7 @@ __device__ __forceinline__ void stnt_(double *p, double v)
     else *p = v;
 }
 
-template <bool FAST, bool MASK, int TY, bool NT = true>
+template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false>
 __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
 {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -134,8 +134,8 @@ __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
     double njh = has_jh ? ps[col_jh + kcl(kb)] : 0.0;
     double nih = has_ih ? ps[col_ih + kcl(kb)] : 0.0;
     long l = col + a.sk * (long)kb;
-    double nu1 = ldnt(a.U1 + l), nu2 = ldnt(a.U2 + l), nu3 = ldnt(a.U3 + l), naj = ldnt(a.aj + l),
-           npn = ldnt(a.pn + l);
+    double nu1 = ldnt(a.U1 + l), nu2 = ldnt(a.U2 + l), nu3 = ldnt(a.U3 + l),
+           naj = FOLD ? 0.0 : ldnt(a.aj + l), npn = ldnt(a.pn + l);
     int nmk = MASK ? a.mask[l] : 1;
 
     int buf = 0;
@@ -155,7 +155,8 @@ __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
             if (has_ih) nih = ps[col_ih + kn];
             const long ln = col + kn;
             nu1 = ldnt(a.U1 + ln); nu2 = ldnt(a.U2 + ln); nu3 = ldnt(a.U3 + ln);
-            naj = ldnt(a.aj + ln); npn = ldnt(a.pn + ln);
+            if (!FOLD) naj = ldnt(a.aj + ln);
+            npn = ldnt(a.pn + ln);
             if (MASK) nmk = a.mask[ln];
         }
         if (act) {
@@ -163,14 +164,17 @@ __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
             if (mk > 0) {
                 const double (*p)[TX + 2 * HALO] = sm[buf];
                 const int cx = tx + HALO, cy = ty + HALO;
+                // folded operator: u1..u3 are aj*dt*d*ucn, so the metric scale is 1
+                const double sc = FOLD ? 1.0 : a.dc, se = FOLD ? 1.0 : a.de, sz = FOLD ? 1.0 : a.dz;
                 const double Lc = weno3_dir<FAST>(p[cy][cx - 2], p[cy][cx - 1], q2, p[cy][cx + 1],
-                                                  p[cy][cx + 2], u1, fmb, fmf, a.dc);
+                                                  p[cy][cx + 2], u1, fmb, fmf, sc);
                 const double Le = weno3_dir<FAST>(p[cy - 2][cx], p[cy - 1][cx], q2, p[cy + 1][cx],
-                                                  p[cy + 2][cx], u2, fml, fmr, a.de);
+                                                  p[cy + 2][cx], u2, fml, fmr, se);
                 const bool fmd = a.r.fd && k == a.r.klo, fmu = a.r.fu && k == a.r.khi;
-                const double Lz = weno3_dir<FAST>(q0, q1, q2, q3, q4, u3, fmd, fmu, a.dz);
+                const double Lz = weno3_dir<FAST>(q0, q1, q2, q3, q4, u3, fmd, fmu, sz);
                 const double rh = (Lc + Le) + Lz;
-                v = (a.c1 * pnv + a.c2 * q2) - ((a.c3 * ajv) * a.dt) * rh;
+                if (FOLD) v = (a.c1 * pnv + a.c2 * q2) - a.c3 * rh;
+                else v = (a.c1 * pnv + a.c2 * q2) - ((a.c3 * ajv) * a.dt) * rh;
             }
             stnt(a.out + col + a.sk * (long)k, v);
         }
@@ -405,6 +409,26 @@ __global__ __launch_bounds__(256) void contra_kernel(ContraArgs a)
     a.U3[l] = tmp * ((a.z0[l] * u + a.z1[l] * v) + a.z2[l] * w);
 }
 
+// frozen-velocity operator of the fast path: V_d = (aj * dt * d) * ucn_d;
+// counts nodes with aj <= 0, where sign(V) would not be the upwind sign
+__global__ __launch_bounds__(256) void fold_kernel(long n, const double *__restrict__ aj,
+                                                   const double *__restrict__ U1,
+                                                   const double *__restrict__ U2,
+                                                   const double *__restrict__ U3, double dt, double dc,
+                                                   double de, double dz, double *__restrict__ V1,
+                                                   double *__restrict__ V2, double *__restrict__ V3,
+                                                   int *bad)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const double a = aj[l];
+    if (!(a > 0.0)) atomicAdd(bad, 1);
+    const double s = a * dt;
+    V1[l] = (s * dc) * U1[l];
+    V2[l] = (s * de) * U2[l];
+    V3[l] = (s * dz) * U3[l];
+}
+
 __global__ __launch_bounds__(256) void copy_kernel(const double *__restrict__ s, double *__restrict__ d, long n)
 {
     const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -469,16 +493,30 @@ static int stage_kc(const Blk &b, const Range &r, int tiles_ij)
     return kc;
 }
 
+int launch_fold(const Blk &b, const double *U1, const double *U2, const double *U3, double dt,
+                double *V1, double *V2, double *V3, int *bad, hipStream_t s)
+{
+    fold_kernel<<<blocks_for(b.n, 256), 256, 0, s>>>(b.n, b.aj, U1, U2, U3, dt, b.dc, b.de, b.dz,
+                                                     V1, V2, V3, bad);
+    return check_hip(hipGetLastError(), "fold_kernel");
+}
+
 int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t *mask,
                const double *U1, const double *U2, const double *U3, double dt, int mode,
-               double *scratch, hipStream_t s)
+               double *scratch, hipStream_t s, bool folded)
 {
     const Range r = make_range(b);
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    const bool fast = mode != LSM_ADVECT_EXACT;
+    const double *src[3] = {phi_n, phi_np1, scratch};
+    double *dst[3] = {phi_np1, scratch, phi_np1};
+
+    // interior stage kernel, then the boundary shell kernel, per stage
     int TY = 4;
-    if (const char *e = getenv("LSM_TILE_Y")) { int v = atoi(e); if (v == 8) TY = 8; }
+    if (!folded)
+        if (const char *e = getenv("LSM_TILE_Y")) { int v = atoi(e); if (v == 8) TY = 8; }
     bool nt_off = false;
     if (const char *e = getenv("LSM_NT")) nt_off = atoi(e) == 0;
     const int tiles_i = (b.e.ni + TX - 1) / TX;
@@ -503,8 +541,6 @@ int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t
     sh.nK = (long)nfk * sh.cnt_i * sh.cnt_j;
     const long nshell = sh.nI + sh.nJ + sh.nK;
 
-    const double *src[3] = {phi_n, phi_np1, scratch};
-    double *dst[3] = {phi_np1, scratch, phi_np1};
     for (int st = 0; st < 3; ++st) {
         StageArgs a{src[st], phi_n, dst[st], U1, U2, U3, b.aj, mask, b.e.ni, b.e.nj, b.e.nk,
                     b.e.sj, b.e.sk, r, C[st][0], C[st][1], C[st][2], dt, b.dc, b.de, b.dz,
@@ -512,8 +548,10 @@ int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t
         if (ntiles > 0) {
             KTimer kt(LSM_KT_ADVECT_STAGE, s);
             dim3 blk(TX, TY, 1);
-            const bool fast = mode != LSM_ADVECT_EXACT;
-            if (TY == 8) {
+            if (folded) {
+                if (mask) advect_stage<true, true, 4, true, true><<<grid, blk, 0, s>>>(a);
+                else advect_stage<true, false, 4, true, true><<<grid, blk, 0, s>>>(a);
+            } else if (TY == 8) {
                 if (fast && mask) advect_stage<true, true, 8><<<grid, blk, 0, s>>>(a);
                 else if (fast) advect_stage<true, false, 8><<<grid, blk, 0, s>>>(a);
                 else if (mask) advect_stage<false, true, 8><<<grid, blk, 0, s>>>(a);

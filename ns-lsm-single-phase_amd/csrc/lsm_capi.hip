@@ -250,6 +250,49 @@ int lsm_rk3_step(const LsmBlock *blk, double *phi_np1, const double *phi_n, cons
                       (hipStream_t)stream);
 }
 
+int lsm_advect_fold(const LsmBlock *blk, const double *ucn1, const double *ucn2, const double *ucn3,
+                    double delti_lsm, double *v1, double *v2, double *v3, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!b.aj) return fail(LSM_ERR_ARG, "aj is NULL");
+    if (!ucn1 || !ucn2 || !ucn3 || !v1 || !v2 || !v3) return fail(LSM_ERR_ARG, "NULL array");
+    if (!(delti_lsm > 0.0) || !(b.dc > 0.0) || !(b.de > 0.0) || !(b.dz > 0.0))
+        return fail(LSM_ERR_UNSUPPORTED, "folded advection needs delti_lsm > 0 and dc, de, dz > 0");
+    hipStream_t s = (hipStream_t)stream;
+    void *p = nullptr;
+    if ((rc = default_work(256, &p))) return rc;
+    int *bad = (int *)p;
+    if ((rc = check_hip(hipMemsetAsync(bad, 0, sizeof(int), s), "hipMemsetAsync"))) return rc;
+    if ((rc = launch_fold(b, ucn1, ucn2, ucn3, delti_lsm, v1, v2, v3, bad, s))) return rc;
+    int nbad = 0;
+    if ((rc = check_hip(hipMemcpyAsync(&nbad, bad, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync")))
+        return rc;
+    if ((rc = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize"))) return rc;
+    if (nbad) return fail(LSM_ERR_UNSUPPORTED, "aj <= 0 at %d nodes: folded advection needs aj > 0", nbad);
+    return LSM_OK;
+}
+
+int lsm_rk3_step_folded(const LsmBlock *blk, double *phi_np1, const double *phi_n,
+                        const int32_t *adv_mask, const double *v1, const double *v2, const double *v3,
+                        void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!phi_np1 || !phi_n || !v1 || !v2 || !v3) return fail(LSM_ERR_ARG, "NULL array");
+    if (phi_np1 == phi_n) return fail(LSM_ERR_ARG, "phi_np1 and phi_n must not alias");
+    if (!b.aj) return fail(LSM_ERR_ARG, "aj is NULL");
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3), &c);
+    if (rc) return rc;
+    double *scratch = (double *)c.take(b.n * sizeof(double));
+    if (!scratch) return fail(LSM_ERR_WORKSPACE, "workspace too small");
+    return launch_rk3(b, phi_np1, phi_n, adv_mask, v1, v2, v3, 1.0, LSM_ADVECT_FAST, scratch,
+                      (hipStream_t)stream, true);
+}
+
 int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizero, int order_bc,
                       int eno_bc, double *normgrad, void *stream)
 {

@@ -62,7 +62,9 @@ int launch_weno3_rhs(const Blk &b, const double *phi, const double *U1, const do
 int launch_bcond(const Blk &b, double *phi, hipStream_t s);
 int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t *mask,
                const double *U1, const double *U2, const double *U3, double dt, int mode,
-               double *scratch, hipStream_t s);
+               double *scratch, hipStream_t s, bool folded = false);
+int launch_fold(const Blk &b, const double *U1, const double *U2, const double *U3, double dt,
+                double *V1, double *V2, double *V3, int *bad, hipStream_t s);
 int launch_copy(const double *src, double *dst, long n, hipStream_t s);
 
 // ---- launchers (lsm_reinit.hip) ----

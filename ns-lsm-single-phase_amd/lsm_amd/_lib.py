@@ -47,6 +47,9 @@ _SIGS = {
     "lsm_bcond": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp]),
     "lsm_rk3_step": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int,
                                _vp, C.c_size_t, _vp]),
+    "lsm_advect_fold": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, C.c_double, _vp, _vp, _vp, _vp]),
+    "lsm_rk3_step_folded": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t,
+                                      _vp]),
     "lsm_normgrad_eno2": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, C.c_int, C.c_int, _vp, _vp]),
     "lsm_reinit_sussman_eno2": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp,
                                           _vp, _vp, C.c_size_t, _vp]),

@@ -173,6 +173,30 @@ def ad_RKTVD3(blk: Block, phi_n, ucn, delti_lsm: float, adv_mask=None, out=None,
     return out
 
 
+def fold_velocity(blk: Block, ucn, delti_lsm: float):
+    """Frozen-velocity operator V_d = aj*delti_lsm*d_d*ucn_d for ad_RKTVD3_folded
+    (requires aj > 0; raises LsmError otherwise)."""
+    L = _lib.load()
+    V = [blk.empty() for _ in range(3)]
+    check(L.lsm_advect_fold(blk.ref, *[_f64(t, blk.shape, "ucn") for t in ucn], float(delti_lsm),
+                            *[C.c_void_p(t.data_ptr()) for t in V], _stream()), "lsm_advect_fold")
+    return V
+
+
+def ad_RKTVD3_folded(blk: Block, phi_n, V, adv_mask=None, out=None, work=None):
+    """ad_RKTVD3 with the fast arithmetic on a folded frozen-velocity operator
+    (fold_velocity); the step size is the one V was folded with."""
+    L = _lib.load()
+    if out is None:
+        out = blk.empty()
+    wp, wn, keep = _ws(blk, _lib.LSM_OP_RK3, work)
+    check(L.lsm_rk3_step_folded(blk.ref, _f64(out, blk.shape, "out"), _f64(phi_n, blk.shape, "phi_n"),
+                                _i32(adv_mask, blk.shape, "adv_mask"),
+                                *[_f64(t, blk.shape, "V") for t in V], wp, wn, _stream()),
+          "lsm_rk3_step_folded")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # reinitialisation
 # ---------------------------------------------------------------------------

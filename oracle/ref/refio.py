@@ -99,7 +99,16 @@ def run(mode, grid, phi, phi0=None, aux=None, mask=None, u=None, v=None, w=None,
     cmd = [os.path.abspath(HARNESS), fin, fout]
     if nprocs > 1:
         cmd = ["/opt/conda/bin/mpiexec", "-n", str(nprocs)] + cmd
-    r = subprocess.run(cmd, cwd=wd, env=env, capture_output=True, text=True)
+
+    def big_stack():
+        # the reference keeps full-size work arrays as automatic (stack)
+        # arrays (ad_RKTVD3.F90:9, reinitialisation_test2.F90:21-26): run it
+        # with the stack limit raised to the hard limit ("ulimit -s unlimited")
+        import resource
+        soft, hard = resource.getrlimit(resource.RLIMIT_STACK)
+        resource.setrlimit(resource.RLIMIT_STACK, (hard, hard))
+
+    r = subprocess.run(cmd, cwd=wd, env=env, capture_output=True, text=True, preexec_fn=big_stack)
     if r.returncode != 0:
         raise RuntimeError(f"reference harness failed ({r.returncode}):\n{r.stdout}\n{r.stderr}")
     secs = None

@@ -302,3 +302,53 @@ def test_fast_vs_exact_256_and_volume_conservation():
     v0 = float(v0.sum())
     v1 = float(v1.sum())
     assert abs(v1 - v0) <= 1e-3 * v0
+
+
+@pytest.mark.parametrize("shape", [(65, 33, 20), (66, 9, 11), (17, 13, 5), (129, 7, 4)])
+def test_rk3_ragged_extents_exact(shape):
+    """Extents that leave partial tiles in i and j and short k chunks give the
+    reference result bit for bit (exact arithmetic)."""
+    ni, nj, nk = shape
+    g = inputs.sinusoidal_cube_grid(ni, amp=0.03, seed=51, nj=nj, nk=nk)
+    phi = inputs.spheres_sdf(g, nspheres=5, seed=52, rmin=0.15, rmax=0.3)
+    vel = inputs.fourier_velocity(g, seed=53)
+    dt = inputs.cfl_dt(g, *vel)
+    for btype in ((1, 1, 1, 1, 1, 1), (0, 2, 2, 1, 2, 0)):
+        blk = api.Block(g, btype=btype, big_phi=0.3)
+        ob = O.Block(g, btype=btype, big_phi=0.3)
+        ucn = api.rhs_contra_j(blk, *(dev(a) for a in vel))
+        p = api.ad_RKTVD3(blk, dev(phi), ucn, dt, mode="exact")
+        p = api.ad_RKTVD3(blk, p, ucn, dt, mode="exact")
+        ref = O.rk3(ob, phi, O.contra(ob, *vel), dt, nsteps=2)
+        assert np.array_equal(host(p), ref), (shape, btype, linf_rel(host(p), ref))
+        pf = api.ad_RKTVD3(blk, dev(phi), ucn, dt, mode="fast")
+        assert linf_rel(host(pf), O.rk3(ob, phi, O.contra(ob, *vel), dt)) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [24, 65])
+def test_rk3_folded_vs_oracle(n):
+    """The frozen-velocity (folded) form stays within the fast-arithmetic
+    bound of the reference, with and without an advection mask."""
+    g, phi, vel, dt = _case(n, 61)
+    blk = api.Block(g)
+    ob = O.Block(g)
+    ucn = api.rhs_contra_j(blk, *(dev(a) for a in vel))
+    V = api.fold_velocity(blk, ucn, dt)
+    U = O.contra(ob, *vel)
+    p = dev(phi)
+    for _ in range(3):
+        p = api.ad_RKTVD3_folded(blk, p, V)
+    assert linf_rel(host(p), O.rk3(ob, phi, U, dt, nsteps=3)) <= 3e-12
+    rng = np.random.default_rng(62)
+    mask = (rng.uniform(size=g.shape) > 0.2).astype(np.int32)
+    pm = api.ad_RKTVD3_folded(blk, dev(phi), V, adv_mask=dev(mask, torch.int32))
+    assert linf_rel(host(pm), O.rk3(ob, phi, U, dt, mask=mask)) <= 1e-12
+
+
+def test_fold_rejects_nonpositive_jacobian():
+    g = inputs.uniform_grid(8, 8, 8, h=0.1)
+    g.aj = -g.aj
+    blk = api.Block(g)
+    ucn = [dev(np.ones(g.shape)) for _ in range(3)]
+    with pytest.raises(api.LsmError):
+        api.fold_velocity(blk, ucn, 1e-3)
