@@ -190,6 +190,49 @@ int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p,
                       int call_reinit, int rn_freq, int ntime, int mode,
                       double *vol_out, void *work, size_t work_bytes, void *stream);
 
+/* ---- host-memory adapter (for Fortran ISO_C_BINDING shims) -------------
+ * Same operations on HOST arrays in the reference's layout: the adapter keeps
+ * device mirrors of one block and copies arguments in and results out; every
+ * call is synchronous.  Metric vectors are the reference's interleaved
+ * csi(1:3,i,j,k) arrays and q is q(1:4,i,j,k) with u, v, w in entries 2..4.
+ * lsm_host_setup uploads the grid (again only when the extents or array
+ * addresses change) and must precede the other calls; ext = il, iu, jl, ju,
+ * kl, ku, igp, jgp, kgp.  The contravariant velocity stays on the device
+ * between lsm_host_contra_velocity and the advection calls (one
+ * levelsetmethod call computes it once, levelsetmethod.F90:227). */
+int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype[6], double dc,
+                   double de, double dz, double big_phi, const double *csi, const double *eta,
+                   const double *zet, const double *aj, const double *x, const double *y,
+                   const double *z);
+int lsm_host_release(void);
+/* rhs_contra_j.F90:2 */
+int lsm_host_contra_velocity(const double *q, double *ucn_aos);
+/* calc_RH_AD_WENO3.F90:1 */
+int lsm_host_weno3_rhs(const double *phi, double *rhs, const int32_t *adv_mask, int mode);
+/* ad_RKTVD3.F90:1 */
+int lsm_host_rk3(double *phi_np1, const double *phi_n, const int32_t *adv_mask, double delti_lsm,
+                 int mode);
+/* bcond_lsm.F90:2 */
+int lsm_host_bcond(double *phi);
+/* calc_NormGrad_d.F90:1 */
+int lsm_host_normgrad(const double *d, const double *phizero, int order_bc, int eno_bc,
+                      double *normgrad);
+/* reinitialisation_test2.F90:1 */
+int lsm_host_reinit_eno2(const double *phi0, double *phi_out, const LsmReinitParams *p,
+                         int32_t *iters_done);
+/* calc_sign_ini.F90:1 */
+int lsm_host_sign_ini(const double *phi0, double epslsm, int m_sphi0, double *sign_out,
+                      double *normgrad_out);
+/* calc_RH_RN.F90:1 */
+int lsm_host_rh_rn(const double *d, const double *sign, double *rhs);
+/* calc_lambda_RN.F90:1 */
+int lsm_host_lambda_rn(const double *phi0, const double *normgrad0, const double *d,
+                       double timestep, double epslsm, double *lambda);
+/* reinitialization_benchmark.F90:1 */
+int lsm_host_reinit_godunov(const double *phi0, double *phi_out, const LsmReinitParams *p);
+/* TotalWaterVolume.F90:1 */
+int lsm_host_water_volume(const double *phi, double vol[3], int32_t *adv_mask);
+
 /* ---- instrumentation (no counterpart in the reference) ------------------
  * Per-kernel launch timing with HIP events recorded on the launch stream,
  * for benchmarks that need a kernel's own duration without a profiler.

@@ -1,0 +1,325 @@
+// Host-memory adapter of the C ABI for callers whose arrays live in host
+// memory -- the Fortran ISO_C_BINDING shims that replace the reference's
+// level-set routines inside levelsetmethod.  The adapter keeps device mirrors
+// of one block (metrics uploaded once per grid, SoA), the contravariant
+// velocity of the last rhs_contra_j call, and staging buffers; every call
+// copies its array arguments in, runs the device entry point on the default
+// stream and copies results out before returning (the reference reads results
+// right after each call).  Output arrays are copied in as well, so entries a
+// routine does not write (boundary nodes of a right-hand side, say) keep the
+// caller's values, as they do in the reference.
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "lsm_internal.hpp"
+
+namespace lsm {
+namespace {
+
+struct HostState {
+    // identity of the uploaded grid
+    int ext[9] = {0};
+    const void *key[7] = {nullptr};
+    bool ready = false;
+    LsmBlock blk{};
+    long n = 0;
+    double *dm[13] = {nullptr};      // csi0..2, eta0..2, zet0..2, aj, x, y, z
+    double *ucn[3] = {nullptr};
+    bool have_ucn = false;
+    double *f[4] = {nullptr};        // staging fields
+    int32_t *mask = nullptr;
+    double *vol = nullptr;
+    int32_t *iters = nullptr;
+    std::vector<double> aos;         // host transposition buffer
+};
+
+std::mutex g_mu;
+HostState g_h;
+
+void release()
+{
+    for (auto &p : g_h.dm) { if (p) (void)hipFree(p); p = nullptr; }
+    for (auto &p : g_h.ucn) { if (p) (void)hipFree(p); p = nullptr; }
+    for (auto &p : g_h.f) { if (p) (void)hipFree(p); p = nullptr; }
+    if (g_h.mask) (void)hipFree(g_h.mask);
+    if (g_h.vol) (void)hipFree(g_h.vol);
+    if (g_h.iters) (void)hipFree(g_h.iters);
+    g_h.mask = nullptr; g_h.vol = nullptr; g_h.iters = nullptr;
+    g_h.ready = false;
+    g_h.have_ucn = false;
+}
+
+int alloc(void **p, size_t bytes) { return check_hip(hipMalloc(p, bytes), "hipMalloc(host adapter)"); }
+
+int up(double *d, const double *h, long n)
+{
+    return check_hip(hipMemcpy(d, h, n * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+int down(double *h, const double *d, long n)
+{
+    return check_hip(hipMemcpy(h, d, n * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+// interleaved (3, n) host array -> three device arrays
+int up_aos(double *d0, double *d1, double *d2, const double *h, long n)
+{
+    g_h.aos.resize((size_t)n);
+    double *dst[3] = {d0, d1, d2};
+    for (int c = 0; c < 3; ++c) {
+        for (long l = 0; l < n; ++l) g_h.aos[l] = h[3 * l + c];
+        int rc = up(dst[c], g_h.aos.data(), n);
+        if (rc) return rc;
+    }
+    return LSM_OK;
+}
+
+int need_ready()
+{
+    if (!g_h.ready) return fail(LSM_ERR_ARG, "lsm_host_setup has not been called");
+    return LSM_OK;
+}
+
+int sync() { return check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+}  // namespace
+}  // namespace lsm
+
+using namespace lsm;
+
+extern "C" {
+
+int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype[6], double dc,
+                   double de, double dz, double big_phi, const double *csi, const double *eta,
+                   const double *zet, const double *aj, const double *x, const double *y,
+                   const double *z)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!ext || !btype || !csi || !eta || !zet || !aj) return fail(LSM_ERR_ARG, "NULL argument");
+    const void *key[7] = {csi, eta, zet, aj, x, y, z};
+    bool same = g_h.ready && memcmp(g_h.ext, ext, sizeof g_h.ext) == 0 &&
+                memcmp(g_h.key, key, sizeof key) == 0;
+    LsmBlock b{};
+    b.il = ext[0]; b.iu = ext[1]; b.jl = ext[2]; b.ju = ext[3]; b.kl = ext[4]; b.ku = ext[5];
+    b.igp = ext[6]; b.jgp = ext[7]; b.kgp = ext[8];
+    b.phys_faces = phys_faces;
+    for (int q = 0; q < 6; ++q) b.btype[q] = btype[q];
+    b.dc = dc; b.de = de; b.dz = dz; b.big_phi = big_phi;
+    Blk chk;
+    int rc = make_blk(&b, &chk, false, false);
+    if (rc) return rc;
+    if (!same) {
+        release();
+        const long n = chk.n;
+        for (auto &p : g_h.dm) if ((rc = alloc((void **)&p, n * sizeof(double)))) return rc;
+        for (auto &p : g_h.ucn) if ((rc = alloc((void **)&p, n * sizeof(double)))) return rc;
+        for (auto &p : g_h.f) if ((rc = alloc((void **)&p, n * sizeof(double)))) return rc;
+        if ((rc = alloc((void **)&g_h.mask, n * sizeof(int32_t)))) return rc;
+        if ((rc = alloc((void **)&g_h.vol, 4 * sizeof(double)))) return rc;
+        if ((rc = alloc((void **)&g_h.iters, sizeof(int32_t)))) return rc;
+        if ((rc = up_aos(g_h.dm[0], g_h.dm[1], g_h.dm[2], csi, n))) return rc;
+        if ((rc = up_aos(g_h.dm[3], g_h.dm[4], g_h.dm[5], eta, n))) return rc;
+        if ((rc = up_aos(g_h.dm[6], g_h.dm[7], g_h.dm[8], zet, n))) return rc;
+        if ((rc = up(g_h.dm[9], aj, n))) return rc;
+        if (x && y && z) {
+            if ((rc = up(g_h.dm[10], x, n)) || (rc = up(g_h.dm[11], y, n)) || (rc = up(g_h.dm[12], z, n)))
+                return rc;
+        }
+        memcpy(g_h.ext, ext, sizeof g_h.ext);
+        memcpy(g_h.key, key, sizeof key);
+        g_h.n = n;
+        g_h.ready = true;
+    }
+    for (int c = 0; c < 3; ++c) {
+        b.csi[c] = g_h.dm[c];
+        b.eta[c] = g_h.dm[3 + c];
+        b.zet[c] = g_h.dm[6 + c];
+    }
+    b.aj = g_h.dm[9];
+    b.x = (x && y && z) ? g_h.dm[10] : nullptr;
+    b.y = (x && y && z) ? g_h.dm[11] : nullptr;
+    b.z = (x && y && z) ? g_h.dm[12] : nullptr;
+    g_h.blk = b;
+    return LSM_OK;
+}
+
+int lsm_host_release(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    release();
+    return LSM_OK;
+}
+
+int lsm_host_contra_velocity(const double *q, double *ucn_aos)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    const long n = g_h.n;
+    // q(1:4, nodes): u, v, w are entries 2..4 of each group of four
+    std::vector<double> tmp((size_t)n);
+    double *uvw[3] = {g_h.f[1], g_h.f[2], g_h.f[3]};
+    for (int c = 0; c < 3; ++c) {
+        for (long l = 0; l < n; ++l) tmp[l] = q[4 * l + 1 + c];
+        if ((rc = up(uvw[c], tmp.data(), n))) return rc;
+    }
+    if ((rc = lsm_contra_velocity(&g_h.blk, uvw[0], uvw[1], uvw[2], 1, g_h.ucn[0], g_h.ucn[1],
+                                  g_h.ucn[2], nullptr)))
+        return rc;
+    if ((rc = sync())) return rc;
+    g_h.have_ucn = true;
+    if (ucn_aos) {
+        for (int c = 0; c < 3; ++c) {
+            if ((rc = down(tmp.data(), g_h.ucn[c], n))) return rc;
+            for (long l = 0; l < n; ++l) ucn_aos[3 * l + c] = tmp[l];
+        }
+    }
+    return LSM_OK;
+}
+
+int lsm_host_weno3_rhs(const double *phi, double *rhs, const int32_t *mask, int mode)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    if (!g_h.have_ucn) return fail(LSM_ERR_ARG, "contravariant velocity not computed (rhs_contra_j)");
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi, n)) || (rc = up(g_h.f[1], rhs, n))) return rc;
+    if (mask && (rc = check_hip(hipMemcpy(g_h.mask, mask, n * sizeof(int32_t), hipMemcpyHostToDevice), "H2D")))
+        return rc;
+    if ((rc = lsm_weno3_rhs(&g_h.blk, g_h.f[0], g_h.ucn[0], g_h.ucn[1], g_h.ucn[2],
+                            mask ? g_h.mask : nullptr, g_h.f[1], mode, nullptr)))
+        return rc;
+    if ((rc = sync())) return rc;
+    return down(rhs, g_h.f[1], n);
+}
+
+int lsm_host_rk3(double *phi_np1, const double *phi_n, const int32_t *mask, double delti_lsm, int mode)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    if (!g_h.have_ucn) return fail(LSM_ERR_ARG, "contravariant velocity not computed (rhs_contra_j)");
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi_n, n))) return rc;
+    if (mask && (rc = check_hip(hipMemcpy(g_h.mask, mask, n * sizeof(int32_t), hipMemcpyHostToDevice), "H2D")))
+        return rc;
+    if ((rc = lsm_rk3_step(&g_h.blk, g_h.f[1], g_h.f[0], mask ? g_h.mask : nullptr, g_h.ucn[0],
+                           g_h.ucn[1], g_h.ucn[2], delti_lsm, mode, nullptr, 0, nullptr)))
+        return rc;
+    if ((rc = sync())) return rc;
+    return down(phi_np1, g_h.f[1], n);
+}
+
+int lsm_host_bcond(double *phi)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi, n))) return rc;
+    if ((rc = lsm_bcond(&g_h.blk, g_h.f[0], nullptr))) return rc;
+    if ((rc = sync())) return rc;
+    return down(phi, g_h.f[0], n);
+}
+
+int lsm_host_normgrad(const double *d, const double *phi0, int order_bc, int eno_bc, double *ng)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], d, n)) || (rc = up(g_h.f[1], phi0, n)) || (rc = up(g_h.f[2], ng, n))) return rc;
+    if ((rc = lsm_normgrad_eno2(&g_h.blk, g_h.f[0], g_h.f[1], order_bc, eno_bc, g_h.f[2], nullptr))) return rc;
+    if ((rc = sync())) return rc;
+    return down(ng, g_h.f[2], n);
+}
+
+int lsm_host_reinit_eno2(const double *phi0, double *phi_out, const LsmReinitParams *p, int32_t *iters)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi0, n))) return rc;
+    if ((rc = lsm_reinit_sussman_eno2(&g_h.blk, p, g_h.f[0], g_h.f[1], g_h.iters, nullptr, 0, nullptr)))
+        return rc;
+    if ((rc = sync())) return rc;
+    if (iters && (rc = check_hip(hipMemcpy(iters, g_h.iters, sizeof(int32_t), hipMemcpyDeviceToHost), "D2H")))
+        return rc;
+    return down(phi_out, g_h.f[1], n);
+}
+
+int lsm_host_sign_ini(const double *phi0, double epslsm, int m_sphi0, double *sgn, double *ng)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi0, n)) || (rc = up(g_h.f[1], sgn, n)) || (rc = up(g_h.f[2], ng, n))) return rc;
+    if ((rc = lsm_sign_ini(&g_h.blk, g_h.f[0], epslsm, m_sphi0, g_h.f[1], g_h.f[2], nullptr))) return rc;
+    if ((rc = sync())) return rc;
+    if ((rc = down(sgn, g_h.f[1], n))) return rc;
+    return down(ng, g_h.f[2], n);
+}
+
+int lsm_host_rh_rn(const double *d, const double *sgn, double *rh)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], d, n)) || (rc = up(g_h.f[1], sgn, n)) || (rc = up(g_h.f[2], rh, n))) return rc;
+    if ((rc = lsm_rh_rn(&g_h.blk, g_h.f[0], g_h.f[1], g_h.f[2], nullptr))) return rc;
+    if ((rc = sync())) return rc;
+    return down(rh, g_h.f[2], n);
+}
+
+int lsm_host_lambda_rn(const double *phi0, const double *ng0, const double *d, double dt, double eps,
+                       double *lam)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi0, n)) || (rc = up(g_h.f[1], ng0, n)) || (rc = up(g_h.f[2], d, n)) ||
+        (rc = up(g_h.f[3], lam, n)))
+        return rc;
+    if ((rc = lsm_lambda_rn(&g_h.blk, g_h.f[0], g_h.f[1], g_h.f[2], dt, eps, g_h.f[3], nullptr))) return rc;
+    if ((rc = sync())) return rc;
+    return down(lam, g_h.f[3], n);
+}
+
+int lsm_host_reinit_godunov(const double *phi0, double *phi_out, const LsmReinitParams *p)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi0, n))) return rc;
+    if ((rc = lsm_reinit_godunov(&g_h.blk, p, g_h.f[0], g_h.f[1], nullptr, 0, nullptr))) return rc;
+    if ((rc = sync())) return rc;
+    return down(phi_out, g_h.f[1], n);
+}
+
+int lsm_host_water_volume(const double *phi, double vol[3], int32_t *adv_mask)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    if (!g_h.blk.x) return fail(LSM_ERR_ARG, "coordinates were not passed to lsm_host_setup");
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi, n))) return rc;
+    if ((rc = lsm_water_volume(&g_h.blk, g_h.f[0], g_h.vol, adv_mask ? g_h.mask : nullptr, nullptr, 0,
+                               nullptr)))
+        return rc;
+    if ((rc = sync())) return rc;
+    if ((rc = check_hip(hipMemcpy(vol, g_h.vol, 3 * sizeof(double), hipMemcpyDeviceToHost), "D2H"))) return rc;
+    if (adv_mask)
+        return check_hip(hipMemcpy(adv_mask, g_h.mask, n * sizeof(int32_t), hipMemcpyDeviceToHost), "D2H");
+    return LSM_OK;
+}
+
+}  // extern "C"

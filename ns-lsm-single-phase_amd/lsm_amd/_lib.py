@@ -64,6 +64,21 @@ _SIGS = {
                                     C.c_int, C.c_int, _vp, _vp, C.c_size_t, _vp]),
     "lsm_timing_enable": (C.c_int, [C.c_int]),
     "lsm_timing_get": (C.c_int, [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    # host-memory adapter (arrays in host memory, synchronous)
+    "lsm_host_setup": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32), C.c_double,
+                                 C.c_double, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lsm_host_release": (C.c_int, []),
+    "lsm_host_contra_velocity": (C.c_int, [_vp, _vp]),
+    "lsm_host_weno3_rhs": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "lsm_host_rk3": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int]),
+    "lsm_host_bcond": (C.c_int, [_vp]),
+    "lsm_host_normgrad": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp]),
+    "lsm_host_reinit_eno2": (C.c_int, [_vp, _vp, C.POINTER(LsmReinitParams), C.POINTER(C.c_int32)]),
+    "lsm_host_sign_ini": (C.c_int, [_vp, C.c_double, C.c_int, _vp, _vp]),
+    "lsm_host_rh_rn": (C.c_int, [_vp, _vp, _vp]),
+    "lsm_host_lambda_rn": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp]),
+    "lsm_host_reinit_godunov": (C.c_int, [_vp, _vp, C.POINTER(LsmReinitParams)]),
+    "lsm_host_water_volume": (C.c_int, [_vp, _vp, _vp]),
 }
 
 EXPORTS = tuple(_SIGS)

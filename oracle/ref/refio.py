@@ -2,7 +2,9 @@
 
 Test infrastructure only: used to write golden vectors and to time the
 reference on the host.  The input/output layout is documented in
-lsm_ref_harness.F90.
+lsm_ref_harness.F90.  The same driver runs oracle/_ref/lsm_shim_harness, the
+harness built with the GPU drop-in routines in place of the reference's
+(`make -C oracle/ref shim`), for the drop-in parity tests.
 """
 from __future__ import annotations
 
@@ -14,6 +16,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HARNESS = os.path.join(HERE, "..", "_ref", "lsm_ref_harness")
+SHIM_HARNESS = os.path.join(HERE, "..", "_ref", "lsm_shim_harness")
 
 MODES = {
     "advect": 1, "weno_rhs": 2, "bcond": 3, "contra": 4, "normgrad": 5,
@@ -37,8 +40,9 @@ def run(mode, grid, phi, phi0=None, aux=None, mask=None, u=None, v=None, w=None,
         hybrid=False, call_reinit=True, rn_freq=1, ntime0=1,
         sussman_method=0, call_levelset=True, delti_lsm=1e-3, epslsm=0.05,
         rn_timestep=1e-3, eps_reinit=1e-5, big_phi=5.0, lambda_dt=1e-3,
-        timing=False, workdir=None, nprocs=1):
-    """Run one harness mode; returns (list of outputs, seconds or None)."""
+        timing=False, workdir=None, nprocs=1, harness=None):
+    """Run one harness mode; returns (list of outputs, seconds or None, extra).
+    harness: path of the executable (default: the reference harness)."""
     nk, nj, ni = grid.shape
     ip = np.zeros(64, dtype="<i4")
     ip[0] = MODES[mode] if isinstance(mode, str) else int(mode)
@@ -96,7 +100,7 @@ def run(mode, grid, phi, phi0=None, aux=None, mask=None, u=None, v=None, w=None,
         f.write(np.ascontiguousarray(mask, dtype="<i4").tobytes())
 
     env = dict(os.environ)
-    cmd = [os.path.abspath(HARNESS), fin, fout]
+    cmd = [os.path.abspath(harness or HARNESS), fin, fout]
     if nprocs > 1:
         cmd = ["/opt/conda/bin/mpiexec", "-n", str(nprocs)] + cmd
 
