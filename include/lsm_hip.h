@@ -112,7 +112,7 @@ int lsm_weno3_rhs(const LsmBlock *blk, const double *phi, const double *ucn1,
                   const double *ucn2, const double *ucn3, const int32_t *adv_mask,
                   double *rhs, int mode, void *stream);
 
-/* bcond_lsm.F90:2-290 (bcond_lsm): boundary values on the physical faces, in
+/* bcond_lsm.F90:2-568 (bcond_lsm): boundary values on the physical faces, in
  * place, in the reference's face order (i1, im, j1, jm, k1, km). */
 int lsm_bcond(const LsmBlock *blk, double *phi, void *stream);
 

@@ -154,7 +154,7 @@ def calc_RH_AD_WENO3(blk: Block, phi, ucn, adv_mask=None, rhs=None, mode="fast")
 
 
 def bcond_lsm(blk: Block, phi):
-    """bcond_lsm.F90:2-290, in place."""
+    """bcond_lsm.F90:2-568, in place."""
     check(_lib.load().lsm_bcond(blk.ref, _f64(phi, blk.shape, "phi"), _stream()), "lsm_bcond")
     return phi
 
