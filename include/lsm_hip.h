@@ -130,7 +130,7 @@ int lsm_rk3_step(const LsmBlock *blk, double *phi_np1, const double *phi_n,
  * V_d = (aj * delti_lsm * dc_d) * ucn_d is built once, then each stage
  * streams V (3 fields) instead of ucn and aj (4 fields).  Fast arithmetic
  * only.  lsm_advect_fold waits for the stream and fails with
- * LSM_ERR_UNSUPPORTED if aj <= 0 anywhere (the upwind sign of V would differ
+ * LSM_ERR_UNSUPPORTED if aj <= 0 or aj = inf anywhere (the upwind sign of V would differ
  * from that of ucn) or delti_lsm <= 0. */
 int lsm_advect_fold(const LsmBlock *blk, const double *ucn1, const double *ucn2,
                     const double *ucn3, double delti_lsm, double *v1, double *v2, double *v3,

@@ -410,7 +410,8 @@ __global__ __launch_bounds__(256) void contra_kernel(ContraArgs a)
 }
 
 // frozen-velocity operator of the fast path: V_d = (aj * dt * d) * ucn_d;
-// counts nodes with aj <= 0, where sign(V) would not be the upwind sign
+// counts nodes where aj is not finite and positive (for aj <= 0 sign(V) would
+// not be the upwind sign)
 __global__ __launch_bounds__(256) void fold_kernel(long n, const double *__restrict__ aj,
                                                    const double *__restrict__ U1,
                                                    const double *__restrict__ U2,
@@ -422,7 +423,7 @@ __global__ __launch_bounds__(256) void fold_kernel(long n, const double *__restr
     const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
     const double a = aj[l];
-    if (!(a > 0.0)) atomicAdd(bad, 1);
+    if (!(a > 0.0) || __builtin_isinf(a)) atomicAdd(bad, 1);
     const double s = a * dt;
     V1[l] = (s * dc) * U1[l];
     V2[l] = (s * de) * U2[l];

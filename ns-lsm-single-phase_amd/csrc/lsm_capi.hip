@@ -270,7 +270,7 @@ int lsm_advect_fold(const LsmBlock *blk, const double *ucn1, const double *ucn2,
     if ((rc = check_hip(hipMemcpyAsync(&nbad, bad, sizeof(int), hipMemcpyDeviceToHost, s), "hipMemcpyAsync")))
         return rc;
     if ((rc = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize"))) return rc;
-    if (nbad) return fail(LSM_ERR_UNSUPPORTED, "aj <= 0 at %d nodes: folded advection needs aj > 0", nbad);
+    if (nbad) return fail(LSM_ERR_UNSUPPORTED, "aj <= 0 or infinite at %d nodes: folded advection needs 0 < aj < inf", nbad);
     return LSM_OK;
 }
 
