@@ -2,10 +2,13 @@
 //
 // Sussman/ENO2 path (reinitialisation_test2 + calc_NormGrad_d): each RK stage
 // is one kernel that evaluates |grad d| at a node and applies the update
-// there (s(phi0), delta(d) and the Sussman-Fatemi lambda are pointwise), so a
-// stage reads d once with its stencil, phi0 and the nine metric terms.  The
-// stage-1 |grad d| is the convergence mask of the iteration (d^(1) = d^n), kept
-// as one byte per node for stages 2 and 3.  The exit test of the next
+// there (s(phi0), delta(d) and the Sussman-Fatemi lambda are pointwise).
+// Interior nodes run in 64 x 4 column tiles marching up k with the stencil of
+// d in LDS and registers, so a stage reads d, phi0 and the nine metric terms
+// once; boundary nodes (one-sided, edge and vertex formulas) run in extra
+// workgroups of the same launch.  The stage-1 |grad d| is the convergence
+// mask of the iteration (d^(1) = d^n), kept as one byte per node for stages 2
+// and 3.  The exit test of the next
 // iteration (max/min of |grad d| over the block) is reduced inside stage 3
 // with wavefront shuffles and 64-bit atomics; a one-thread kernel turns it
 // into a device flag, so all RNitermax iterations are queued without a host
@@ -14,6 +17,8 @@
 // Godunov path (reinitialization_benchmark + calc_sign_ini / calc_RH_RN /
 // calc_lambda_RN): one kernel per RK stage; the lambda correction is fused
 // into the last stage.
+#include <stdlib.h>
+
 #include "lsm_internal.hpp"
 
 namespace lsm {
@@ -109,43 +114,6 @@ __global__ __launch_bounds__(256) void normgrad_kernel(Geo g, const double *__re
     ng[l] = normgrad_node(g, d, phi0, p0s, order_bc, eno_bc != 0, i, j, k, l);
 }
 
-// block-wide max/min of non-negative doubles into ctl (bit patterns order like the values)
-__device__ __forceinline__ void reduce_maxmin(double vmax, double vmin, Ctl *ctl)
-{
-    for (int off = 32; off > 0; off >>= 1) {
-        vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
-        vmin = fmin(vmin, __shfl_xor(vmin, off, 64));
-    }
-    __shared__ double smax[16], smin[16];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { smax[wv] = vmax; smin[wv] = vmin; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = (blockDim.x + 63) >> 6;
-        double a = smax[0], b = smin[0];
-        for (int q = 1; q < nw; ++q) { a = fmax(a, smax[q]); b = fmin(b, smin[q]); }
-        atomicMax(&ctl->maxbits, (unsigned long long)__double_as_longlong(a));
-        atomicMin(&ctl->minbits, (unsigned long long)__double_as_longlong(b));
-    }
-}
-
-// max/min of |grad phi0| before the first iteration (reinitialisation_test2.F90:96-101)
-__global__ __launch_bounds__(256) void reinit_reduce0(Geo g, const double *__restrict__ phi0,
-                                                       int order_bc, int eno_bc, Ctl *ctl)
-{
-    const long n = (long)g.ni * g.nj * g.nk;
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double vmax = 0.0, vmin = __longlong_as_double((long long)kInfBits);
-    if (l < n) {
-        int i, j, k;
-        node_of(g, l, i, j, k);
-        const double p0s = phi0[(long)(g.ni - 2) + g.sj * (g.nj - 2) + g.sk * (g.nk - 2)];
-        const double v = normgrad_node(g, phi0, phi0, p0s, order_bc, eno_bc != 0, i, j, k, l);
-        vmax = v; vmin = v;
-    }
-    reduce_maxmin(vmax, vmin, ctl);
-}
-
 __global__ void reinit_init_ctl(Ctl *ctl, int32_t *iters_out)
 {
     ctl->maxbits = 0ull;
@@ -174,47 +142,201 @@ struct StageP {
     int order_bc, eno_bc;
 };
 
-// one RK stage of reinitialisation_test2.F90:137-181
+// ---------------------------------------------------------------------------
+// One RK stage of reinitialisation_test2.F90:137-181.  Interior nodes (no index on a face) run in
+// 64 (i) x 4 (j) column tiles marching up a chunk of k planes: d of the tile's
+// current plane with its two-node i/j halo sits in LDS, d(k-2..k+2) of the
+// thread's own column in registers, so d is read from HBM once per stage.
+// The boundary shell (every node with an index on a face, whose one-sided
+// and edge/vertex formulas live in normgrad_node) is handled by extra
+// workgroups of the same launch, one node per thread; the two parts write
+// disjoint nodes and read only din, so they need no ordering.
+// ---------------------------------------------------------------------------
+constexpr int RTX = 64, RTY = 4, RH = 2;
+
+struct RTArgs {
+    Geo g;
+    StageP p;
+    const double *din, *dn;
+    double *dout;
+    const double *phi0;
+    uint8_t *conv;
+    Ctl *ctl;
+    int tiles_i, tiles_j, kc, ntiles, grid_tiles;   // interior tiles (grid_tiles: padded to 8)
+    long nI, nJ, nK;                                 // shell nodes on i, j and k faces
+};
+
+// the stage update at one node, given |grad d| (reinitialisation_test2.F90:137-181)
 template <int STAGE>
-__global__ __launch_bounds__(256) void reinit_stage(Geo g, StageP p, const double *__restrict__ din,
-                                                     const double *dn, double *dout,
-                                                     const double *__restrict__ phi0,
-                                                     uint8_t *__restrict__ conv, Ctl *ctl)
+__device__ __forceinline__ double reinit_update(const StageP &p, double ngv, double dv, double dnv,
+                                                double p0, bool upd)
 {
-    if (!ctl->active) return;
-    const long n = (long)g.ni * g.nj * g.nk;
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double ngv = 0.0;
-    const bool in = l < n;
-    if (in) {
-        int i, j, k;
-        node_of(g, l, i, j, k);
-        const double p0s = phi0[(long)(g.ni - 2) + g.sj * (g.nj - 2) + g.sk * (g.nk - 2)];
-        ngv = normgrad_node(g, din, phi0, p0s, p.order_bc, p.eno_bc != 0, i, j, k, l);
-        bool upd;
-        if (STAGE == 1) {
-            upd = ngv - 1.0 > p.eps_r;
-            conv[l] = upd ? 1 : 0;
+    double v = dv;
+    if (upd) {
+        const double s = sign_phi0(p0, p.epslsm);
+        double rhs = s * (ngv - 1.0);
+        const double lam = lambda_mass(dv, s, ngv, p.epslsm);
+        const double aux = deltaf(dv, p.epslsm) * ngv;
+        rhs = rhs - lam * aux;
+        v = (p.c1 * dnv + p.c2 * dv) - (p.c3 * p.dt) * rhs;
+    }
+    return v;
+}
+
+// block-wide max/min (256 threads, linear id tid) into ctl
+__device__ __forceinline__ void reduce_maxmin_t(double vmax, double vmin, int tid, Ctl *ctl)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+        vmin = fmin(vmin, __shfl_xor(vmin, off, 64));
+    }
+    __shared__ double smax[4], smin[4];
+    const int lane = tid & 63, wv = tid >> 6;
+    if (lane == 0) { smax[wv] = vmax; smin[wv] = vmin; }
+    __syncthreads();
+    if (tid == 0) {
+        double a = smax[0], b = smin[0];
+        for (int q = 1; q < 4; ++q) { a = fmax(a, smax[q]); b = fmin(b, smin[q]); }
+        atomicMax(&ctl->maxbits, (unsigned long long)__double_as_longlong(a));
+        atomicMin(&ctl->minbits, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// STAGE 0: only the max/min of |grad din| (reinitialisation_test2.F90:96-101)
+template <int STAGE>
+__global__ __launch_bounds__(RTX * RTY) void reinit_stage_tiled(RTArgs a)
+{
+    if (!a.ctl->active) return;
+    const Geo &g = a.g;
+    const int tid = threadIdx.y * RTX + threadIdx.x;
+    const double inf = __longlong_as_double((long long)kInfBits);
+    double vmax = 0.0, vmin = inf;
+    const int bid = blockIdx.x;
+
+    if (bid >= a.grid_tiles) {
+        // ---- boundary shell: one node per thread, generic formulas ----
+        const long t = (long)(bid - a.grid_tiles) * (RTX * RTY) + tid;
+        int i = 0, j = 0, k = 0;
+        bool ok = true;
+        if (t < a.nI) {
+            const long per = (long)g.nj * g.nk;
+            j = (int)((t % per) % g.nj); k = (int)((t % per) / g.nj);
+            i = (t < per) ? 0 : g.ni - 1;
+        } else if (t < a.nI + a.nJ) {
+            const long u = t - a.nI, per = (long)(g.ni - 2) * g.nk;
+            i = 1 + (int)((u % per) % (g.ni - 2)); k = (int)((u % per) / (g.ni - 2));
+            j = (u < per) ? 0 : g.nj - 1;
+        } else if (t < a.nI + a.nJ + a.nK) {
+            const long u = t - a.nI - a.nJ, per = (long)(g.ni - 2) * (g.nj - 2);
+            i = 1 + (int)((u % per) % (g.ni - 2)); j = 1 + (int)((u % per) / (g.ni - 2));
+            k = (u < per) ? 0 : g.nk - 1;
         } else {
-            upd = conv[l] != 0;
+            ok = false;
         }
-        const double dv = din[l];
-        double v = dv;
-        if (upd) {
-            const double dnv = (STAGE == 1) ? dv : dn[l];
-            const double s = sign_phi0(phi0[l], p.epslsm);
-            double rhs = s * (ngv - 1.0);
-            const double lam = lambda_mass(dv, s, ngv, p.epslsm);
-            const double aux = deltaf(dv, p.epslsm) * ngv;
-            rhs = rhs - lam * aux;
-            v = (p.c1 * dnv + p.c2 * dv) - (p.c3 * p.dt) * rhs;
+        if (ok) {
+            const long l = (long)i + g.sj * j + g.sk * k;
+            const double p0s = a.phi0[(long)(g.ni - 2) + g.sj * (g.nj - 2) + g.sk * (g.nk - 2)];
+            const double ngv = normgrad_node(g, a.din, a.phi0, p0s, a.p.order_bc, a.p.eno_bc != 0, i, j, k, l);
+            if (STAGE > 0) {
+                bool upd;
+                if (STAGE == 1) { upd = ngv - 1.0 > a.p.eps_r; a.conv[l] = upd ? 1 : 0; }
+                else upd = a.conv[l] != 0;
+                const double dv = a.din[l];
+                const double dnv = (STAGE == 1) ? dv : a.dn[l];
+                a.dout[l] = reinit_update<STAGE>(a.p, ngv, dv, dnv, a.phi0[l], upd);
+            }
+            vmax = ngv; vmin = ngv;
         }
-        dout[l] = v;
+        if (STAGE == 3 || STAGE == 0) reduce_maxmin_t(vmax, vmin, tid, a.ctl);
+        return;
     }
-    if (STAGE == 3) {
-        const double inf = __longlong_as_double((long long)kInfBits);
-        reduce_maxmin(in ? ngv : 0.0, in ? ngv : inf, ctl);
+
+    // ---- interior tiles ----
+    int tile = bid;
+    if ((a.grid_tiles & 7) == 0) tile = (bid & 7) * (a.grid_tiles >> 3) + (bid >> 3);
+    const bool live = tile < a.ntiles;
+    const int tile_c = live ? tile : 0;
+    const int ti = tile_c % a.tiles_i;
+    const int t2 = tile_c / a.tiles_i;
+    const int tj = t2 % a.tiles_j;
+    const int tk = t2 / a.tiles_j;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = ti * RTX + tx, j = tj * RTY + ty;
+    const int kb = 1 + tk * a.kc;
+    const int ke = live ? min(kb + a.kc - 1, g.nk - 2) : kb - 1;
+
+    __shared__ double sm[2][RTY + 2 * RH][RTX + 2 * RH];
+
+    const int ic = min(i, g.ni - 1), jc = min(j, g.nj - 1);
+    const long col = (long)ic + g.sj * jc;
+    const bool has_jh = ty < 4;
+    const int rr = (ty < 2) ? ty : ty - 2 + RTY + RH;
+    const int jh = min(max(tj * RTY - RH + rr, 0), g.nj - 1);
+    const long col_jh = (long)ic + g.sj * jh;
+    const int lin = tid;
+    const bool has_ih = lin < 4 * RTY;
+    const int hr = (lin >> 2) % RTY, hc4 = lin & 3;
+    const int hcol = (hc4 < 2) ? hc4 : hc4 + RTX;
+    const int hi_i = min(max(ti * RTX - RH + hcol, 0), g.ni - 1);
+    const int hi_j = min(tj * RTY + hr, g.nj - 1);
+    const long col_ih = (long)hi_i + g.sj * hi_j;
+    auto kcl = [&](int kk) -> long { return g.sk * (long)min(max(kk, 0), g.nk - 1); };
+    const bool act = live && i >= 1 && i <= g.ni - 2 && j >= 1 && j <= g.nj - 2;
+    const bool ilo = i == 1, ihi = i == g.ni - 2, jlo = j == 1, jhi = j == g.nj - 2;
+    const double *__restrict__ d = a.din;
+
+    double q0 = d[col + kcl(kb - 2)], q1 = d[col + kcl(kb - 1)];
+    double q2 = d[col + kcl(kb)], q3 = d[col + kcl(kb + 1)];
+    double nq4 = d[col + kcl(kb + 2)];
+    double njh = has_jh ? d[col_jh + kcl(kb)] : 0.0;
+    double nih = has_ih ? d[col_ih + kcl(kb)] : 0.0;
+    int buf = 0;
+    for (int k = kb; k <= ke; ++k) {
+        const double q4 = nq4;
+        sm[buf][ty + RH][tx + RH] = q2;
+        if (has_jh) sm[buf][rr][tx + RH] = njh;
+        if (has_ih) sm[buf][hr + RH][hcol] = nih;
+        __syncthreads();
+        if (k < ke) {
+            const long kn = kcl(k + 1);
+            nq4 = d[col + kcl(k + 3)];
+            if (has_jh) njh = d[col_jh + kn];
+            if (has_ih) nih = d[col_ih + kn];
+        }
+        if (act) {
+            const long l = col + g.sk * (long)k;
+            const double p0 = a.phi0[l];
+            double m[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) m[c] = __builtin_nontemporal_load(g.m[c] + l);
+            double dnv = q2;
+            uint8_t cv = 0;
+            if (STAGE > 1) { dnv = __builtin_nontemporal_load(a.dn + l); cv = a.conv[l]; }
+            const double (*pl)[RTX + 2 * RH] = sm[buf];
+            const int cx = tx + RH, cy = ty + RH;
+            // calc_NormGrad_d.F90:70-163 (ENO2 in each direction, upwind side by phi0)
+            const double g0 = g.dd[0] * eno2(p0, ilo ? 0.0 : pl[cy][cx - 2], pl[cy][cx - 1], q2,
+                                             pl[cy][cx + 1], ihi ? 0.0 : pl[cy][cx + 2], ilo, ihi);
+            const double g1 = g.dd[1] * eno2(p0, jlo ? 0.0 : pl[cy - 2][cx], pl[cy - 1][cx], q2,
+                                             pl[cy + 1][cx], jhi ? 0.0 : pl[cy + 2][cx], jlo, jhi);
+            const bool klo = k == 1, khi = k == g.nk - 2;
+            const double g2 = g.dd[2] * eno2(p0, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi);
+            const double gx = (m[0] * g0 + m[3] * g1) + m[6] * g2;
+            const double gy = (m[1] * g0 + m[4] * g1) + m[7] * g2;
+            const double gz = (m[2] * g0 + m[5] * g1) + m[8] * g2;
+            const double ngv = sqrt((gx * gx + gy * gy) + gz * gz);
+            if (STAGE > 0) {
+                bool upd;
+                if (STAGE == 1) { upd = ngv - 1.0 > a.p.eps_r; a.conv[l] = upd ? 1 : 0; }
+                else upd = cv != 0;
+                __builtin_nontemporal_store(reinit_update<STAGE>(a.p, ngv, q2, dnv, p0, upd), a.dout + l);
+            }
+            if (STAGE == 3 || STAGE == 0) { vmax = fmax(vmax, ngv); vmin = fmin(vmin, ngv); }
+        }
+        q0 = q1; q1 = q2; q2 = q3; q3 = q4;
+        buf ^= 1;
     }
+    if (STAGE == 3 || STAGE == 0) reduce_maxmin_t(vmax, vmin, tid, a.ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -388,28 +510,48 @@ int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi
     reinit_init_ctl<<<1, 1, 0, s>>>(ctl, iters);
     if ((rc = check_hip(hipGetLastError(), "reinit_init_ctl"))) return rc;
     if (p.rn_itermax <= 0) return LSM_OK;
-    reinit_reduce0<<<nb(b.n), 256, 0, s>>>(g, phi0, p.order_bc, p.eno_bc, ctl);
-    if ((rc = check_hip(hipGetLastError(), "reinit_reduce0"))) return rc;
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    // tiled stages: interior tiles of 64 x 4 nodes x kc planes plus shell blocks
+    RTArgs ra{};
+    ra.g = g; ra.phi0 = phi0; ra.conv = conv; ra.ctl = ctl;
+    ra.tiles_i = (b.e.ni + RTX - 1) / RTX;
+    ra.tiles_j = (b.e.nj + RTY - 1) / RTY;
+    {
+        const int nk_int = b.e.nk - 2;
+        const int tij = ra.tiles_i * ra.tiles_j;
+        int kc = (nk_int + (8192 + tij - 1) / tij - 1) / ((8192 + tij - 1) / tij);
+        if (kc < 8) kc = 8;
+        if (const char *e = getenv("LSM_REINIT_KCHUNK")) { int v = atoi(e); if (v > 0) kc = v; }
+        ra.kc = kc;
+        ra.ntiles = tij * ((nk_int + kc - 1) / kc);
+    }
+    ra.grid_tiles = (ra.ntiles + 7) / 8 * 8;
+    ra.nI = 2L * b.e.nj * b.e.nk;
+    ra.nJ = 2L * (b.e.ni - 2) * b.e.nk;
+    ra.nK = 2L * (b.e.ni - 2) * (b.e.nj - 2);
+    const unsigned shell_blocks = (unsigned)((ra.nI + ra.nJ + ra.nK + 255) / 256);
+    const unsigned rgrid = (unsigned)ra.grid_tiles + shell_blocks;
+    const dim3 rblk(RTX, RTY, 1);
+    // max/min of |grad phi0| before the first iteration (reinitialisation_test2.F90:96-101)
+    ra.p = StageP{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
+    ra.din = phi0; ra.dn = phi0; ra.dout = nullptr;
+    reinit_stage_tiled<0><<<rgrid, rblk, 0, s>>>(ra);
+    if ((rc = check_hip(hipGetLastError(), "reinit_stage_tiled"))) return rc;
+    const double *sin_[3] = {out, A, B};
+    const double *sdn[3] = {out, out, out};
+    double *sout[3] = {A, B, out};
     for (int it = 0; it < p.rn_itermax; ++it) {
         reinit_check<<<1, 1, 0, s>>>(ctl, p.eps_reinit, iters);
-        StageP sp{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
-        sp.c1 = C[0][0]; sp.c2 = C[0][1]; sp.c3 = C[0][2];
-        {
+        for (int st = 0; st < 3; ++st) {
+            StageP sp{C[st][0], C[st][1], C[st][2], p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc,
+                      p.eno_bc};
             KTimer kt(LSM_KT_REINIT_STAGE, s);
-            reinit_stage<1><<<nb(b.n), 256, 0, s>>>(g, sp, out, out, A, phi0, conv, ctl);
-        }
-        sp.c1 = C[1][0]; sp.c2 = C[1][1]; sp.c3 = C[1][2];
-        {
-            KTimer kt(LSM_KT_REINIT_STAGE, s);
-            reinit_stage<2><<<nb(b.n), 256, 0, s>>>(g, sp, A, out, B, phi0, conv, ctl);
-        }
-        sp.c1 = C[2][0]; sp.c2 = C[2][1]; sp.c3 = C[2][2];
-        {
-            KTimer kt(LSM_KT_REINIT_STAGE, s);
-            reinit_stage<3><<<nb(b.n), 256, 0, s>>>(g, sp, B, out, out, phi0, conv, ctl);
+            ra.p = sp; ra.din = sin_[st]; ra.dn = sdn[st]; ra.dout = sout[st];
+            if (st == 0) reinit_stage_tiled<1><<<rgrid, rblk, 0, s>>>(ra);
+            else if (st == 1) reinit_stage_tiled<2><<<rgrid, rblk, 0, s>>>(ra);
+            else reinit_stage_tiled<3><<<rgrid, rblk, 0, s>>>(ra);
         }
         if ((rc = check_hip(hipGetLastError(), "reinit_stage"))) return rc;
     }
