@@ -6,6 +6,7 @@
 // go through marching tetrahedra.  Per-workgroup partial sums (wavefront
 // shuffles, then LDS) are written to scratch and summed by a single
 // workgroup in a fixed order, so the result does not depend on scheduling.
+// The AdvectionNodes mask is built by two stencil passes of its own.
 #include "lsm_internal.hpp"
 
 namespace lsm {
@@ -169,7 +170,7 @@ __device__ double marching_tet_water(const double v[4][3], const double ph[4])
 }
 
 // CellAnalysis volumes, TetrahedronMethods.F90:1823-2121 (TriangulationCase 1)
-__device__ __noinline__ void cell_analysis(const double ph[8], const double V[8][3], double &pc_vol,
+__device__ __forceinline__ void cell_analysis(const double ph[8], const double V[8][3], double &pc_vol,
                                            double &st_vol)
 {
     pc_vol = 0.0; st_vol = 0.0;
@@ -204,87 +205,231 @@ __device__ __noinline__ void cell_analysis(const double ph[8], const double V[8]
     }
 }
 
+// Cells are processed in workgroups of 64 (i) x 4 (j) cells and a chunk of
+// VKC k layers.  Pass 1 (volume_wet) sums the hexahedron volumes of fully
+// wet cells and flags the workgroups that contain cut cells; pass 2
+// (volume_cut) lists the cut cells of a flagged workgroup in LDS (in a fixed
+// order) and runs their marching tetrahedra on consecutive threads, so the
+// heavy path neither costs registers in pass 1 nor occupies wavefronts whose
+// cells are not cut.
+constexpr int VTX = 64, VTY = 4, VKC = 8;
+
 struct VolArgs {
     const double *phi, *x, *y, *z;
-    int32_t *mask;
-    double *partial;   // [gridDim.x][3]
+    double *partial;    // [nblocks][3]: K-tetrahedra, second tetrahedra, bulk cells
+    int32_t *cutflag;   // [nblocks]
     int ni, nj, nk;
     long sj, sk;
-    long ncells;
+    int tiles_i, tiles_j;
 };
 
-// TotalWaterVolume.F90:136-294
-__global__ __launch_bounds__(256) void volume_kernel(VolArgs a)
+struct CellTile {
+    int i, j, k0, k1;   // this thread's cell column (i, j) and the workgroup's layers [k0, k1)
+    bool ok;
+};
+
+__device__ __forceinline__ CellTile tile_of(const VolArgs &a)
 {
-    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double vk = 0.0, vs = 0.0, vb = 0.0;
-    if (c < a.ncells) {
-        const int ci = a.ni - 1, cj = a.nj - 1;
-        const int i = (int)(c % ci);
-        const long r = c / ci;
-        const int j = (int)(r % cj);
-        const int k = (int)(r / cj);
-        const long l = (long)i + a.sj * j + a.sk * k;
-        const long off[8] = {0, 1, 1 + a.sj, a.sj, a.sk, 1 + a.sk, 1 + a.sj + a.sk, a.sj + a.sk};
-        double ph[8];
-        bool allp = true, alln = true;
-        for (int q = 0; q < 8; ++q) {
-            ph[q] = a.phi[l + off[q]];
-            if (!(ph[q] > kEpsSims)) allp = false;
-            if (!(ph[q] < -kEpsSims)) alln = false;
-        }
-        if (a.mask) {
-            // AdvectionNodes: 3x3x3 neighbourhood of a node next to a sign change
-            const int km = max(k - 1, 0), kp = min(k + 1, a.nk - 1);
-            const int jm = max(j - 1, 0), jp = min(j + 1, a.nj - 1);
-            const int im = max(i - 1, 0), ip = min(i + 1, a.ni - 1);
-            const double p = ph[0];
-            const double sgn = sign1(p);
-            bool change = fabs(p) < kEpsSims;
-            for (int kk = km; kk <= kp && !change; ++kk)
-                for (int jj = jm; jj <= jp && !change; ++jj)
-                    for (int ii = im; ii <= ip; ++ii) {
-                        const double q = a.phi[(long)ii + a.sj * jj + a.sk * kk];
-                        if (sgn * q < 0.0 || fabs(q) < kEpsSims) { change = true; break; }
-                    }
-            if (change)
-                for (int kk = km; kk <= kp; ++kk)
-                    for (int jj = jm; jj <= jp; ++jj)
-                        for (int ii = im; ii <= ip; ++ii) a.mask[(long)ii + a.sj * jj + a.sk * kk] = 1;
-        }
-        if (!alln) {
-            double V[8][3];
-            for (int q = 0; q < 8; ++q) {
-                V[q][0] = a.x[l + off[q]]; V[q][1] = a.y[l + off[q]]; V[q][2] = a.z[l + off[q]];
-            }
-            if (allp) vb = hex_volume(V);
-            else cell_analysis(ph, V, vk, vs);
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        vk += __shfl_xor(vk, o, 64);
-        vs += __shfl_xor(vs, o, 64);
-        vb += __shfl_xor(vb, o, 64);
-    }
-    __shared__ double sm[4][3];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (lane == 0) { sm[wv][0] = vk; sm[wv][1] = vs; sm[wv][2] = vb; }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        const int q = threadIdx.x;
-        a.partial[(long)blockIdx.x * 3 + q] = ((sm[0][q] + sm[1][q]) + sm[2][q]) + sm[3][q];
-    }
+    const int b = blockIdx.x;
+    const int ti = b % a.tiles_i, t2 = b / a.tiles_i;
+    CellTile c;
+    c.i = ti * VTX + threadIdx.x;
+    c.j = (t2 % a.tiles_j) * VTY + threadIdx.y;
+    c.k0 = (t2 / a.tiles_j) * VKC;
+    c.k1 = min(c.k0 + VKC, a.nk - 1);
+    c.ok = c.i < a.ni - 1 && c.j < a.nj - 1;
+    return c;
 }
 
-__global__ __launch_bounds__(256) void volume_final(const double *partial, long nblocks, double *vol)
+// the eight corners of cell (i,j,k), in CellAnalysis' vertex order
+struct Corners {
+    long off[8];
+    __device__ Corners(long sj, long sk)
+    {
+        off[0] = 0; off[1] = 1; off[2] = 1 + sj; off[3] = sj;
+        off[4] = sk; off[5] = 1 + sk; off[6] = 1 + sj + sk; off[7] = sj + sk;
+    }
+};
+
+// block sum of per-thread values (wavefront shuffles, then LDS; fixed order)
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double *out)
 {
-    __shared__ double sm[256][3];
+    for (int o = 32; o > 0; o >>= 1)
+        for (int q = 0; q < N; ++q) v[q] += __shfl_xor(v[q], o, 64);
+    __shared__ double sm[VTY][N];
+    const int lane = threadIdx.x, wv = threadIdx.y;
+    if (lane == 0)
+        for (int q = 0; q < N; ++q) sm[wv][q] = v[q];
+    __syncthreads();
+    if (wv == 0 && lane < N) out[lane] = ((sm[0][lane] + sm[1][lane]) + sm[2][lane]) + sm[3][lane];
+}
+
+// TotalWaterVolume.F90:136-294, fully wet cells (HexahedronVolume) and the
+// classification of the rest
+__global__ __launch_bounds__(VTX * VTY) void volume_wet(VolArgs a)
+{
+    const CellTile c = tile_of(a);
+    const Corners cn(a.sj, a.sk);
+    double v[3] = {0.0, 0.0, 0.0};
+    bool cut = false;
+    if (c.ok) {
+        for (int k = c.k0; k < c.k1; ++k) {
+            const long l = (long)c.i + a.sj * c.j + a.sk * k;
+            bool allp = true, alln = true;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double ph = a.phi[l + cn.off[q]];
+                if (!(ph > kEpsSims)) allp = false;
+                if (!(ph < -kEpsSims)) alln = false;
+            }
+            if (allp) {
+                double V[8][3];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    V[q][0] = a.x[l + cn.off[q]]; V[q][1] = a.y[l + cn.off[q]]; V[q][2] = a.z[l + cn.off[q]];
+                }
+                v[2] += hex_volume(V);
+            }
+            cut = cut || (!allp && !alln);
+        }
+    }
+    __shared__ int anycut;
+    if (threadIdx.x == 0 && threadIdx.y == 0) anycut = 0;
+    __syncthreads();
+    if (cut) anycut = 1;
+    block_sum<3>(v, a.partial + (long)blockIdx.x * 3);   // contains a barrier
+    if (threadIdx.x == 0 && threadIdx.y == 0) a.cutflag[blockIdx.x] = anycut;
+}
+
+// CellAnalysis of the cut cells (TetrahedronMethods.F90:1823-2121) in the
+// workgroups pass 1 flagged; their K- and second-tetrahedra sums replace the
+// zeros pass 1 wrote
+__global__ __launch_bounds__(VTX * VTY) void volume_cut(VolArgs a)
+{
+    if (!a.cutflag[blockIdx.x]) return;
+    const CellTile c = tile_of(a);
+    const Corners cn(a.sj, a.sk);
+    const int tid = threadIdx.y * VTX + threadIdx.x;
+    // list the cut cells: layer by layer, wavefront by wavefront, lane order
+    __shared__ int list[VKC * VTX * VTY];
+    __shared__ int wcount[VTY];
+    int n = 0;
+    for (int k = c.k0; k < c.k1; ++k) {
+        bool cut = false;
+        if (c.ok) {
+            const long l = (long)c.i + a.sj * c.j + a.sk * k;
+            bool allp = true, alln = true;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double ph = a.phi[l + cn.off[q]];
+                if (!(ph > kEpsSims)) allp = false;
+                if (!(ph < -kEpsSims)) alln = false;
+            }
+            cut = !allp && !alln;
+        }
+        const unsigned long long bal = __ballot(cut);
+        if (threadIdx.x == 0) wcount[threadIdx.y] = __popcll(bal);
+        __syncthreads();
+        int base = n;
+        for (int w = 0; w < (int)threadIdx.y; ++w) base += wcount[w];
+        int tot = 0;
+        for (int w = 0; w < VTY; ++w) tot += wcount[w];
+        if (cut) {
+            const unsigned long long below = threadIdx.x ? (bal & ((1ull << threadIdx.x) - 1ull)) : 0ull;
+            list[base + __popcll(below)] = (k - c.k0) * (VTX * VTY) + tid;
+        }
+        n += tot;
+        __syncthreads();
+    }
+    double v[2] = {0.0, 0.0};
+    const int ti = blockIdx.x % a.tiles_i, tj = (blockIdx.x / a.tiles_i) % a.tiles_j;
+    for (int e = tid; e < n; e += VTX * VTY) {
+        const int code = list[e];
+        const int kk = c.k0 + code / (VTX * VTY), t = code % (VTX * VTY);
+        const int ii = ti * VTX + t % VTX, jj = tj * VTY + t / VTX;
+        const long l = (long)ii + a.sj * jj + a.sk * kk;
+        double ph[8], V[8][3];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            ph[q] = a.phi[l + cn.off[q]];
+            V[q][0] = a.x[l + cn.off[q]]; V[q][1] = a.y[l + cn.off[q]]; V[q][2] = a.z[l + cn.off[q]];
+        }
+        double vk, vs;
+        cell_analysis(ph, V, vk, vs);
+        v[0] += vk; v[1] += vs;
+    }
+    block_sum<2>(v, a.partial + (long)blockIdx.x * 3);
+}
+
+// AdvectionNodes (TotalWaterVolume.F90:187-233): a cell's lower corner node
+// n is next to the interface when |phi(n)| < eps_sims or a node of its 3x3x3
+// neighbourhood (clipped to the block) has the opposite sign or |phi| <
+// eps_sims; the mask is 1 on the neighbourhood of every such node.  Pass 1
+// writes the per-node flag (min / max / min |.| over the neighbourhood), pass
+// 2 the dilation, so every mask entry is written exactly once.
+struct MaskArgs {
+    const double *phi;
+    uint8_t *chg;
+    int32_t *mask;
+    int ni, nj, nk;
+    long sj, sk;
+    int tiles_i, tiles_j;
+};
+
+__global__ __launch_bounds__(VTX * VTY) void mask_change(MaskArgs a)
+{
+    const int b = blockIdx.x;
+    const int ti = b % a.tiles_i, t2 = b / a.tiles_i;
+    const int i = ti * VTX + threadIdx.x, j = (t2 % a.tiles_j) * VTY + threadIdx.y, k = t2 / a.tiles_j;
+    if (i >= a.ni || j >= a.nj) return;
+    const long l = (long)i + a.sj * j + a.sk * k;
+    uint8_t flag = 0;
+    if (i <= a.ni - 2 && j <= a.nj - 2 && k <= a.nk - 2) {
+        const double p = a.phi[l];
+        double mn = p, mx = p, ma = fabs(p);
+        const int km = max(k - 1, 0), kp = min(k + 1, a.nk - 1);
+        const int jm = max(j - 1, 0), jp = min(j + 1, a.nj - 1);
+        const int im = max(i - 1, 0), ip = min(i + 1, a.ni - 1);
+        for (int kk = km; kk <= kp; ++kk)
+            for (int jj = jm; jj <= jp; ++jj)
+                for (int ii = im; ii <= ip; ++ii) {
+                    const double q = a.phi[(long)ii + a.sj * jj + a.sk * kk];
+                    mn = fmin(mn, q); mx = fmax(mx, q); ma = fmin(ma, fabs(q));
+                }
+        // sign(p) * q < 0 for some q  <=>  min q < 0 (p >= +0) or max q > 0 (p <= -0)
+        const bool opp = (sign1(p) > 0.0) ? (mn < 0.0) : (mx > 0.0);
+        flag = (fabs(p) < kEpsSims || opp || ma < kEpsSims) ? 1 : 0;
+    }
+    a.chg[l] = flag;
+}
+
+__global__ __launch_bounds__(VTX * VTY) void mask_dilate(MaskArgs a)
+{
+    const int b = blockIdx.x;
+    const int ti = b % a.tiles_i, t2 = b / a.tiles_i;
+    const int i = ti * VTX + threadIdx.x, j = (t2 % a.tiles_j) * VTY + threadIdx.y, k = t2 / a.tiles_j;
+    if (i >= a.ni || j >= a.nj) return;
+    const int km = max(k - 1, 0), kp = min(k + 1, a.nk - 1);
+    const int jm = max(j - 1, 0), jp = min(j + 1, a.nj - 1);
+    const int im = max(i - 1, 0), ip = min(i + 1, a.ni - 1);
+    int m = 0;
+    for (int kk = km; kk <= kp; ++kk)
+        for (int jj = jm; jj <= jp; ++jj)
+            for (int ii = im; ii <= ip; ++ii) m |= a.chg[(long)ii + a.sj * jj + a.sk * kk];
+    a.mask[(long)i + a.sj * j + a.sk * k] = m;
+}
+
+// sum of the per-workgroup partials in a fixed order (1024 threads)
+__global__ __launch_bounds__(1024) void volume_final(const double *partial, long nblocks, double *vol)
+{
+    __shared__ double sm[1024][3];
     double s[3] = {0.0, 0.0, 0.0};
-    for (long b = threadIdx.x; b < nblocks; b += 256)
+    for (long b = threadIdx.x; b < nblocks; b += 1024)
         for (int q = 0; q < 3; ++q) s[q] += partial[b * 3 + q];
     for (int q = 0; q < 3; ++q) sm[threadIdx.x][q] = s[q];
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = 512; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
             for (int q = 0; q < 3; ++q) sm[threadIdx.x][q] += sm[threadIdx.x + w][q];
         __syncthreads();
@@ -292,42 +437,50 @@ __global__ __launch_bounds__(256) void volume_final(const double *partial, long 
     if (threadIdx.x < 3) vol[threadIdx.x] = sm[0][threadIdx.x];
 }
 
-__global__ __launch_bounds__(256) void fill_i32(int32_t *p, long n, int32_t v)
-{
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < n) p[l] = v;
-}
-
 long vol_blocks(const Blk &b)
 {
-    const long nc = (long)(b.e.ni - 1) * (b.e.nj - 1) * (b.e.nk - 1);
-    return (nc + 255) / 256;
+    return (long)((b.e.ni - 1 + VTX - 1) / VTX) * ((b.e.nj - 1 + VTY - 1) / VTY) *
+           ((b.e.nk - 1 + VKC - 1) / VKC);
 }
 
 }  // namespace
 
-size_t water_volume_work(const Blk &b) { return (size_t)vol_blocks(b) * 3 * sizeof(double) + 1024; }
+size_t water_volume_work(const Blk &b)
+{
+    const size_t nb = (size_t)vol_blocks(b);
+    return ((nb * 3 * sizeof(double) + 255) & ~(size_t)255) + ((nb * sizeof(int32_t) + 255) & ~(size_t)255) +
+           (((size_t)b.n + 255) & ~(size_t)255) + 1024;
+}
 
 int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *mask, Carver &w,
                         hipStream_t s)
 {
     const long nblk = vol_blocks(b);
     double *partial = (double *)w.take((size_t)nblk * 3 * sizeof(double));
-    if (!partial) return fail(LSM_ERR_WORKSPACE, "volume workspace too small");
+    int32_t *cutflag = (int32_t *)w.take((size_t)nblk * sizeof(int32_t));
+    uint8_t *chg = mask ? (uint8_t *)w.take((size_t)b.n) : nullptr;
+    if (!partial || !cutflag || (mask && !chg)) return fail(LSM_ERR_WORKSPACE, "volume workspace too small");
     int rc;
-    if (mask) {
-        fill_i32<<<(unsigned)((b.n + 255) / 256), 256, 0, s>>>(mask, b.n, 0);
-        if ((rc = check_hip(hipGetLastError(), "fill_i32"))) return rc;
-    }
-    VolArgs a{phi, b.x, b.y, b.z, mask, partial, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk,
-              (long)(b.e.ni - 1) * (b.e.nj - 1) * (b.e.nk - 1)};
+    const dim3 blk(VTX, VTY, 1);
+    VolArgs a{phi, b.x, b.y, b.z, partial, cutflag, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk,
+              (b.e.ni - 1 + VTX - 1) / VTX, (b.e.nj - 1 + VTY - 1) / VTY};
     {
         KTimer kt(LSM_KT_VOLUME, s);
-        volume_kernel<<<(unsigned)nblk, 256, 0, s>>>(a);
+        volume_wet<<<(unsigned)nblk, blk, 0, s>>>(a);
+        volume_cut<<<(unsigned)nblk, blk, 0, s>>>(a);
     }
-    if ((rc = check_hip(hipGetLastError(), "volume_kernel"))) return rc;
-    volume_final<<<1, 256, 0, s>>>(partial, nblk, vol);
-    return check_hip(hipGetLastError(), "volume_final");
+    if ((rc = check_hip(hipGetLastError(), "volume kernels"))) return rc;
+    volume_final<<<1, 1024, 0, s>>>(partial, nblk, vol);
+    if ((rc = check_hip(hipGetLastError(), "volume_final"))) return rc;
+    if (mask) {
+        MaskArgs m{phi, chg, mask, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, (b.e.ni + VTX - 1) / VTX,
+                   (b.e.nj + VTY - 1) / VTY};
+        const unsigned mb = (unsigned)(m.tiles_i * m.tiles_j * b.e.nk);
+        mask_change<<<mb, blk, 0, s>>>(m);
+        mask_dilate<<<mb, blk, 0, s>>>(m);
+        if ((rc = check_hip(hipGetLastError(), "mask kernels"))) return rc;
+    }
+    return LSM_OK;
 }
 
 }  // namespace lsm
