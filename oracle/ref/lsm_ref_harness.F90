@@ -14,6 +14,8 @@
 ! The harness is only used to write golden vectors and to time the reference
 ! on the host; it is never linked into or called by the product library.
 !
+! Mode 13 calls the compiled mg_metrics (metrics of the grid x, y, z).
+!
 ! I/O (stream, little endian):
 !   input : int32 ip(64), real64 rp(32), then ni*nj*nk-sized arrays
 !           x, y, z, csi(3,:), eta(3,:), zet(3,:), aj, q(4,:), phi, phi0,
@@ -143,7 +145,10 @@ program lsm_ref_harness
    OUTPUTdir = './'
 
    t0 = mpi_wtime()
-   if (ip(1) == 12) then
+   if (ip(1) == 13) then
+      ! mg_metrics itself on the grid x, y, z (mg_metrics.F90:53-262)
+      call metrics_driver( ni, nj, nk, x, y, z, rp(1), rp(2), rp(3), trim(fout) )
+   else if (ip(1) == 12) then
       ! levelsetmethod itself, called as mg_driver.F90:563-581 does
       do kk = 1, max(ip(5), 1)
          ntime = ip(22) + kk - 1
@@ -167,6 +172,56 @@ program lsm_ref_harness
    call mpi_finalize(ierr)
 
 end program lsm_ref_harness
+
+
+subroutine metrics_driver( ni, nj, nk, xin, yin, zin, dcin, dein, dzin, fout )
+
+   ! Fills the module arrays mg_metrics reads (global.F90:46-54,150-155; one
+   ! block, no ghost layers, every face physical) and calls it.
+   use precision
+   use global, only : x, y, z, csi, eta, zet, aj, xc, xe, xz, yc, ye, yz, zc, ze, zz, &
+                      dc, de, dz, le_ia, le_ib, le_ja, le_jb, le_ka, le_kb, ns
+   implicit none
+   integer, intent(in) :: ni, nj, nk
+   real (kind = rdf), intent(in) :: xin(ni*nj*nk), yin(ni*nj*nk), zin(ni*nj*nk)
+   real (kind = rdf), intent(in) :: dcin, dein, dzin
+   character(len=*), intent(in) :: fout
+   integer :: n, c
+
+   n = ni * nj * nk
+   allocate( x(n), y(n), z(n), aj(n), csi(3,n), eta(3,n), zet(3,n) )
+   allocate( xc(n), xe(n), xz(n), yc(n), ye(n), yz(n), zc(n), ze(n), zz(n) )
+   allocate( dc(1), de(1), dz(1) )
+   allocate( le_ia(1), le_ib(1), le_ja(1), le_jb(1), le_ka(1), le_kb(1) )
+   x = xin ; y = yin ; z = zin
+   dc(1) = dcin ; de(1) = dein ; dz(1) = dzin
+   le_ia(1) = 1 ; le_ib(1) = ni
+   le_ja(1) = 1 ; le_jb(1) = nj
+   le_ka(1) = 1 ; le_kb(1) = nk
+   ns = 1
+   csi = 0.0_rdf ; eta = 0.0_rdf ; zet = 0.0_rdf ; aj = 0.0_rdf
+
+   call mg_metrics
+
+   open(22, file=fout, access='stream', form='unformatted', status='replace')
+   write(22) int(10,4)
+   do c = 1, 3
+      write(22) int(0,4), int(n,4)
+      write(22) csi(c,:)
+   end do
+   do c = 1, 3
+      write(22) int(0,4), int(n,4)
+      write(22) eta(c,:)
+   end do
+   do c = 1, 3
+      write(22) int(0,4), int(n,4)
+      write(22) zet(c,:)
+   end do
+   write(22) int(0,4), int(n,4)
+   write(22) aj
+   close(22)
+
+end subroutine metrics_driver
 
 
 subroutine ref_driver( il, iu, jl, ju, kl, ku, igp, jgp, kgp, dc, de, dz, &

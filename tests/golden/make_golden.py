@@ -102,6 +102,7 @@ def curvilinear_case():
                          {"m_sphi0": ms, "m_ti": mt, "rn_itermax": 4}, {"phi": phi0}))
     runs.append(("volume", "volume", {}, {}))
     runs.append(("levelset", "levelset", {"nsteps": 3, "rn_itermax": 4}, {}))
+    runs.append(("metrics", "metrics", {}, {}))
     case("curvilinear", g, phi, phi0, u, v, w, params, runs,
          extra_inputs={"sign": sg, "ng0": ng0})
 
@@ -118,7 +119,8 @@ def channel_case():
     runs = [("advect", "advect", {"nsteps": 5}, {}),
             ("volume", "volume", {}, {}),
             ("levelset", "levelset", {"nsteps": 2, "rn_itermax": 4}, {}),
-            ("reinit_eno2_o1", "reinit_eno2", {"rn_itermax": 4, "order_reinit_bc": 1}, {})]
+            ("reinit_eno2_o1", "reinit_eno2", {"rn_itermax": 4, "order_reinit_bc": 1}, {}),
+            ("metrics", "metrics", {}, {})]
     case("channel", g, phi, phi, u, v, w, params, runs)
 
 
