@@ -30,6 +30,8 @@ HBM_PEAK_GBS = 8000.0
 # stage 1 reads phi^n, U/J (3), aj and writes phi (48 B); stages 2-3 also read
 # the stage input phi (56 B): 160 B per advection cell-update
 ADV_BYTES_PER_CELL_UPDATE = 160.0
+ADV_FOLDED_BYTES_PER_CELL_UPDATE = 136.0
+VOLUME_BYTES_PER_CELL = 32.0
 REINIT_BYTES_PER_CELL_ITER = 304.0
 
 
@@ -47,10 +49,13 @@ def make_case(n, seed_grid=2, seed_phi=3, seed_vel=4):
     return g, np.ascontiguousarray(phi), vel, dt
 
 
-def cpu_baseline(n_ij=256, nk=64, steps=3):
-    """The reference Fortran (oracle/_ref, one MPI rank, one core) on a slab of
-    the same i-j extent, or the C restatement of the oracle if the Fortran
-    build is absent."""
+def cpu_baseline(n_ij=256, nk=64, steps=6):
+    """The reference Fortran (oracle/_ref, compiled with amdflang) on the
+    host cores: one MPI rank per core, every rank advancing its own copy of a
+    slab of the same i-j extent (the single-block case each rank of a domain
+    decomposition owns, without the halo exchange), so the aggregate rate is
+    an upper bound for the reference on these cores.  Falls back to the C
+    restatement of the oracle on one core if the Fortran build is absent."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle", "ref"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -60,16 +65,19 @@ def cpu_baseline(n_ij=256, nk=64, steps=3):
     u, v, w = inputs.fourier_velocity(g, seed=4)
     dt = inputs.cfl_dt(g, u, v, w)
     cells = (n_ij - 2) * (n_ij - 2) * (nk - 2) * steps
-    sample = f"{steps} ad_RKTVD3 steps on a {n_ij}x{n_ij}x{nk} block of the same mapping ({cells} cell-updates)"
+    ranks = max(1, min(16, os.cpu_count() or 1))
+    sample = (f"{steps} ad_RKTVD3 steps on a {n_ij}x{n_ij}x{nk} block of the same mapping "
+              f"({cells} cell-updates per rank)")
     try:
         import refio
         if refio.available():
             _, secs, _ = refio.run("advect", g, phi, u=u, v=v, w=w, delti_lsm=dt, nsteps=steps,
-                                   timing=True)
+                                   timing=True, nprocs=ranks)
             if secs:
-                return {"value": cells / secs, "unit": "cell-updates/s", "cores": 1,
+                return {"value": ranks * cells / secs, "unit": "cell-updates/s", "cores": ranks,
                         "kind": "reference", "seconds": secs,
-                        "sample": sample + ", compiled reference Fortran (amdflang -O2, 1 rank)"}
+                        "sample": sample + f", compiled reference Fortran (amdflang -O2), {ranks} MPI "
+                                           "ranks, one per core, each on its own block; slowest rank timed"}
     except Exception as e:  # noqa: BLE001
         log(f"reference harness unavailable ({e}); timing the C oracle instead")
     import oracle as O
@@ -98,12 +106,12 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
-    ap.add_argument("--data-model", default="folded", choices=["folded", "reference"],
-                    help="folded: frozen-velocity operator aj*dt*d*ucn (3 fields per stage); "
-                         "reference: ucn and aj (4 fields), the reference's data model")
+    ap.add_argument("--data-model", default="reference", choices=["folded", "reference"],
+                    help="reference: ucn and aj (4 fields per stage), the reference's data model; "
+                         "folded: frozen-velocity operator aj*dt*d*ucn (3 fields per stage)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -188,13 +196,19 @@ def main():
     # one launch processes every interior cell once; on average 160/3 B each
     alg_bytes_launch = interior * ADV_BYTES_PER_CELL_UPDATE / 3.0
     achieved = alg_bytes_launch / (stage_ms / 1e3) / 1e9
-    traffic = load_traffic(f"advect_stage_{n}")
+    traffic = load_traffic(f"advect_stage_{n}_{'folded' if folded else args.mode}")
     roofline = {"bound": "hbm", "kernel": "advect_stage", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
                 "avg_launch_ms": round(stage_ms, 5), "launches": int(kn.value),
                 "alg_bytes_per_launch": alg_bytes_launch,
                 "step_frac_of_roofline": round(value * ADV_BYTES_PER_CELL_UPDATE / 1e9 / HBM_PEAK_GBS, 4)}
+    if folded:
+        # the folded kernel's own compulsory traffic: stage 1 reads phi^n and
+        # V (3) and writes phi (40 B), stages 2-3 also read the stage input (48 B)
+        kb = interior * ADV_FOLDED_BYTES_PER_CELL_UPDATE / 3.0
+        roofline["kernel_alg_bytes_per_launch"] = kb
+        roofline["kernel_frac"] = round(kb / (stage_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     log(f"{args.steps} steps: {ms_per_step:.3f} ms/step, {value:.3e} cell-updates/s, "
         f"stage kernel {stage_ms:.4f} ms ({achieved:.0f} GB/s alg), shell {sms.value / max(sn.value, 1):.4f} ms")
 
@@ -202,13 +216,16 @@ def main():
              "data_model": head_kind}
     if not args.no_extra:
         # the other arithmetic / data-model variants of the same step
-        for kind in ("fast", "exact"):
-            if kind == head_kind:
+        if V is None and args.mode == "fast":
+            V = api.fold_velocity(blk, ucn, dt)
+        for kind in ("fast", "exact", "folded"):
+            if kind == head_kind or (kind == "folded" and V is None):
                 continue
             t_ms, _, kms2, kn2, _, _, _ = timed(kind, 10, 2)
-            extra[f"{kind}_refdata_ms_per_step"] = round(t_ms / 10, 5)
-            extra[f"{kind}_refdata_cell_updates_per_s"] = interior * 10 / (t_ms / 1e3)
-            extra[f"{kind}_refdata_stage_avg_ms"] = round(kms2.value / max(kn2.value, 1), 5)
+            tag = "folded" if kind == "folded" else f"{kind}_refdata"
+            extra[f"{tag}_ms_per_step"] = round(t_ms / 10, 5)
+            extra[f"{tag}_cell_updates_per_s"] = interior * 10 / (t_ms / 1e3)
+            extra[f"{tag}_stage_avg_ms"] = round(kms2.value / max(kn2.value, 1), 5)
     if not args.no_extra:
         # secondary units at the same size: Sussman reinitialisation and water volume
         hmin, hmax = __import__("lsm_amd.inputs", fromlist=["x"]).spacing_bounds(g)
@@ -243,8 +260,27 @@ def main():
         v_ms = e0.elapsed_time(e1)
         extra["volume_cells_per_s"] = ((n - 1) ** 3) / (v_ms / 1e3)
         extra["volume_ms"] = round(v_ms, 4)
+        extra["volume_alg_GBps"] = round(((n - 1) ** 3) * VOLUME_BYTES_PER_CELL / (v_ms / 1e3) / 1e9, 1)
         L.lsm_timing_enable(0)
         log(f"reinit: {it} iterations in {r_ms:.2f} ms (stage {r_stage:.4f} ms), volume {v_ms:.3f} ms")
+        # one whole levelsetmethod call (volume, velocity, RK3, 4 Sussman
+        # iterations, bcond) from the velocity field
+        uvw = [torch.as_tensor(np.ascontiguousarray(a), device="cuda") for a in vel]
+        pl = bufs[base].clone()
+        pn = pl.clone()
+        lw = blk.workspace(_lib.LSM_OP_LEVELSET_STEP)
+        api.levelsetmethod(blk, pl, pn, *uvw, prm, dt, ntime=1, work=lw)
+        torch.cuda.synchronize()
+        e0.record()
+        reps = 3
+        for r in range(reps):
+            api.levelsetmethod(blk, pl, pn, *uvw, prm, dt, ntime=1 + r, work=lw)
+        e1.record()
+        torch.cuda.synchronize()
+        ls_ms = e0.elapsed_time(e1) / reps
+        extra["levelsetmethod_ms"] = round(ls_ms, 3)
+        extra["levelsetmethod_cells_per_s"] = interior / (ls_ms / 1e3)
+        log(f"levelsetmethod (4 reinit iterations): {ls_ms:.2f} ms per call")
 
     cpu = None
     if not args.no_cpu_baseline:

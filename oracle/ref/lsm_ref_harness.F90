@@ -13,6 +13,8 @@
 !
 ! The harness is only used to write golden vectors and to time the reference
 ! on the host; it is never linked into or called by the product library.
+! Started on several MPI ranks, every rank runs the same single-block case
+! independently (all faces are physical boundaries on every rank).
 !
 ! Mode 13 calls the compiled mg_metrics (metrics of the grid x, y, z).
 !
@@ -60,6 +62,9 @@ program lsm_ref_harness
    end if
    call get_command_argument(1, fin)
    call get_command_argument(2, fout)
+   ! several ranks (the host-core baseline) each run the same block; ranks
+   ! other than 0 write their outputs next to rank 0's
+   if (myid /= 0) write(fout, '(a,a,i0)') trim(fout), '.', myid
 
    open(21, file=trim(fin), access='stream', form='unformatted', status='old')
    read(21) ip

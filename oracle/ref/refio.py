@@ -118,7 +118,8 @@ def run(mode, grid, phi, phi0=None, aux=None, mask=None, u=None, v=None, w=None,
     secs = None
     for line in r.stdout.splitlines():
         if line.startswith("REF_SECONDS"):
-            secs = float(line.split()[1])
+            t = float(line.split()[1])
+            secs = t if secs is None else max(secs, t)   # slowest rank
     outs = []
     raw = open(fout, "rb").read()
     off = 0
