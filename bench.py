@@ -227,6 +227,30 @@ def main():
             extra[f"{tag}_cell_updates_per_s"] = interior * 10 / (t_ms / 1e3)
             extra[f"{tag}_stage_avg_ms"] = round(kms2.value / max(kn2.value, 1), 5)
     if not args.no_extra:
+        # reverse mode of the same step (lsm_rk3_step_adjoint: forward recompute,
+        # bcond/WENO3 transposes, dJ/dphi^n and dJ/dU accumulated)
+        aw = blk.workspace(_lib.LSM_OP_RK3_ADJOINT)
+        lam = bufs[base].clone()
+        adj = blk.empty()
+        aU = [blk.zeros() for _ in range(3)]
+        amode = "fast" if args.mode == "fast" else "exact"
+        for _ in range(2):
+            api.ad_RKTVD3_adjoint(blk, bufs[base], ucn, dt, lam, mode=amode, adj_ucn=aU, out=adj, work=aw)
+        torch.cuda.synchronize()
+        a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        a0.record()
+        for _ in range(reps):
+            api.ad_RKTVD3_adjoint(blk, bufs[base], ucn, dt, lam, mode=amode, adj_ucn=aU, out=adj, work=aw)
+        a1.record()
+        torch.cuda.synchronize()
+        adj_ms = a0.elapsed_time(a1) / reps
+        extra["adjoint_ms_per_step"] = round(adj_ms, 4)
+        extra["adjoint_cell_updates_per_s"] = interior / (adj_ms / 1e3)
+        extra["adjoint_forward_ratio"] = round(adj_ms / ms_per_step, 3)
+        log(f"adjoint: {adj_ms:.3f} ms/step ({adj_ms / ms_per_step:.2f}x the forward step)")
+        del aw, lam, adj, aU
+    if not args.no_extra:
         # secondary units at the same size: Sussman reinitialisation and water volume
         hmin, hmax = __import__("lsm_amd.inputs", fromlist=["x"]).spacing_bounds(g)
         prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4,

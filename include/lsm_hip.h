@@ -59,6 +59,7 @@ extern "C" {
 #define LSM_OP_REINIT_GODUNOV 3
 #define LSM_OP_WATER_VOLUME   4
 #define LSM_OP_LEVELSET_STEP  5
+#define LSM_OP_RK3_ADJOINT    6
 
 /* One curvilinear block (the host-associated state of levelsetmethod.F90:1-44). */
 typedef struct LsmBlock {
@@ -138,6 +139,22 @@ int lsm_advect_fold(const LsmBlock *blk, const double *ucn1, const double *ucn2,
 int lsm_rk3_step_folded(const LsmBlock *blk, double *phi_np1, const double *phi_n,
                         const int32_t *adv_mask, const double *v1, const double *v2,
                         const double *v3, void *work, size_t work_bytes, void *stream);
+
+/* Adjoint of lsm_rk3_step (reverse mode of ad_RKTVD3.F90:1-88 with
+ * calc_RH_AD_WENO3.F90:1-407 and bcond_lsm.F90:2-568; the reference has no
+ * adjoint): given adj_np1 = dJ/dphi^(n+1), writes adj_n = dJ/dphi^n and, when
+ * adj_ucn1..3 are given, ADDS dJ/d(U/J) of this step to them (zero them before
+ * the first step of a trajectory).  The forward stages are recomputed from
+ * phi_n with the same arguments (mode selects their arithmetic).  The WENO3
+ * weights are differentiated; the upwind choice, the forced boundary stencils
+ * and the BigPhi switch of bcond_lsm are taken as in the forward.
+ * Deterministic.  Workspace: lsm_workspace_bytes(blk, LSM_OP_RK3_ADJOINT)
+ * (20 fields). */
+int lsm_rk3_step_adjoint(const LsmBlock *blk, const double *phi_n, const int32_t *adv_mask,
+                         const double *ucn1, const double *ucn2, const double *ucn3, double delti_lsm,
+                         int mode, const double *adj_np1, double *adj_n, double *adj_ucn1,
+                         double *adj_ucn2, double *adj_ucn3, void *work, size_t work_bytes,
+                         void *stream);
 
 /* calc_NormGrad_d.F90:1-2448 (calc_NormGrad_d): |grad d| with ENO2
  * differences inside and biased differences of order `order_bc` (optionally

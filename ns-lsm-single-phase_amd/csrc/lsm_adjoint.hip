@@ -1,0 +1,295 @@
+// Reverse mode (adjoint) of one SSP-RK3 step of the WENO3 level-set advection
+// (ad_RKTVD3.F90:1-88 with calc_RH_AD_WENO3.F90:1-407 and bcond_lsm.F90:2-568).
+//
+// Given lam = dJ/dphi^(n+1), returns dJ/dphi^n and accumulates dJ/d(U/J).
+// The three forward stages are recomputed first; then, for stage s = 3, 2, 1:
+//   1. bcond_lsm transposed: its three face phases in reverse order (k, j, i),
+//      each face value's adjoint handed to the two inner nodes it was
+//      extrapolated from (BigPhi switch and btype as taken in the forward);
+//   2. per node, the WENO3 right-hand side differentiated with respect to the
+//      five stencil values of each direction (nonlinear weights included, the
+//      upwind choice and the forced boundary stencils as in the forward), the
+//      partials scaled by the node's adjoint and written out (15 fields), and
+//      the derivative with respect to the node's U/J accumulated;
+//   3. per node, a gather of the partials of the nodes whose stencil holds it,
+//      plus the linear part c2 (and c1 into dJ/dphi^n).
+// Everything is deterministic (no atomics).
+#include "lsm_internal.hpp"
+
+namespace lsm {
+
+namespace {
+
+// GetWENO3Reconstruction (AdvectionMethods.F90:195-245) and its gradient with
+// respect to the stencil values f0..f3 on the upwind branch the forward takes
+__device__ __forceinline__ double weno3_grad(double f0, double f1, double f2, double f3, double ws,
+                                             double g[4])
+{
+    const bool pos = ws > 0.0;
+    const double fL = pos ? f0 : f3;
+    const double fC = pos ? f1 : f2;
+    const double fR = pos ? f2 : f1;
+    const double d0 = 2.0 / 3.0, d1 = 1.0 / 3.0, e = 0.000001;
+    const double b0 = (fC - fR) * (fC - fR);
+    const double b1 = (fL - fC) * (fL - fC);
+    const double s0 = b0 + e, s1 = b1 + e;
+    const double a0 = d0 / (s0 * s0);
+    const double a1 = d1 / (s1 * s1);
+    const double w0 = a0 / (a0 + a1);
+    const double w1 = a1 / (a0 + a1);
+    const double u0 = 0.5 * fC + 0.5 * fR;
+    const double u1 = (-0.5) * fL + 1.5 * fC;
+    // d w0 = w0 w1 (d ln a0 - d ln a1),  d ln a_k = -2 d b_k / s_k
+    const double du = u0 - u1, ww = w0 * w1;
+    const double q0 = 4.0 * (fC - fR) / s0, q1 = 4.0 * (fL - fC) / s1;
+    const double gL = -0.5 * w1 + du * ww * q1;
+    const double gC = 0.5 * w0 + 1.5 * w1 - du * ww * (q0 + q1);
+    const double gR = 0.5 * w0 + du * ww * q0;
+    if (pos) { g[0] = gL; g[1] = gC; g[2] = gR; g[3] = 0.0; }
+    else { g[3] = gL; g[2] = gC; g[1] = gR; g[0] = 0.0; }
+    return u0 * w0 + u1 * w1;
+}
+
+// one direction of calc_RH_AD_WENO3.F90:195-254: dL/dphi at offsets -2..2
+// (L = ws d (Phi+ - Phi-)) and Phi+ - Phi-
+__device__ __forceinline__ double weno3_dir_grad(double m2, double m1, double c0, double p1, double p2,
+                                                 double ws, bool forced_minus, bool forced_plus,
+                                                 double d, double gl[5])
+{
+    double gp[4], gm[4];
+    const double fp = weno3_grad(m1, c0, p1, forced_plus ? 0.0 : p2, forced_plus ? 1.0 : ws, gp);
+    const double fm = weno3_grad(forced_minus ? 0.0 : m2, m1, c0, p1, forced_minus ? -1.0 : ws, gm);
+    const double k = ws * d;
+    gl[0] = k * (-gm[0]);
+    gl[1] = k * (gp[0] - gm[1]);
+    gl[2] = k * (gp[1] - gm[2]);
+    gl[3] = k * (gp[2] - gm[3]);
+    gl[4] = k * gp[3];
+    return fp - fm;
+}
+
+struct AdjGeo {
+    int ni, nj, nk;
+    long sj, sk, n;
+};
+
+__device__ __forceinline__ void node3(const AdjGeo &g, long l, int &i, int &j, int &k)
+{
+    i = (int)(l % g.ni);
+    const long r = l / g.ni;
+    j = (int)(r % g.nj);
+    k = (int)(r / g.nj);
+}
+
+// (2) partials of the stage right-hand side, scaled by the node's adjoint
+struct PartArgs {
+    AdjGeo g;
+    const double *prev, *lam, *U1, *U2, *U3, *aj;
+    const int32_t *mask;
+    double *P;                    // [15][n]: direction d, offset o -> P[(5 d + o + 2) n + l]
+    double *aU1, *aU2, *aU3;      // accumulated dJ/dU (may be NULL)
+    double c3, dt, dd[3];
+};
+
+__global__ __launch_bounds__(256) void adj_partials(PartArgs a)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= a.g.n) return;
+    int i, j, k;
+    node3(a.g, l, i, j, k);
+    const int idx[3] = {i, j, k};
+    const int nn[3] = {a.g.ni, a.g.nj, a.g.nk};
+    const long st[3] = {1, a.g.sj, a.g.sk};
+    const bool interior = i >= 1 && i <= a.g.ni - 2 && j >= 1 && j <= a.g.nj - 2 && k >= 1 &&
+                          k <= a.g.nk - 2;
+    const bool upd = interior && (a.mask == nullptr || a.mask[l] > 0);
+    const long n = a.g.n;
+    if (!upd) {
+        for (int q = 0; q < 15; ++q) a.P[q * n + l] = 0.0;
+        return;
+    }
+    // phi~ = c1 phi^n + c2 phi - ((c3 aj) dt) rh  (ad_RKTVD3.F90:70-72)
+    const double m = a.lam[l] * (-((a.c3 * a.aj[l]) * a.dt));
+    const double U[3] = {a.U1[l], a.U2[l], a.U3[l]};
+    double *aU[3] = {a.aU1, a.aU2, a.aU3};
+    const double *p = a.prev;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const long s = st[d];
+        const bool fm = idx[d] == 1, fp = idx[d] == nn[d] - 2;
+        const double m2 = fm ? 0.0 : p[l - 2 * s], p2 = fp ? 0.0 : p[l + 2 * s];
+        double gl[5];
+        const double dphi = weno3_dir_grad(m2, p[l - s], p[l], p[l + s], p2, U[d], fm, fp, a.dd[d], gl);
+#pragma unroll
+        for (int o = 0; o < 5; ++o) a.P[(5 * d + o) * n + l] = m * gl[o];
+        if (aU[d]) aU[d][l] += m * (a.dd[d] * dphi);
+    }
+}
+
+// (3) gather: adjoint of the stage input (and the c1 part of dJ/dphi^n)
+struct GatherArgs {
+    AdjGeo g;
+    const double *lam, *P;
+    const int32_t *mask;
+    double *prev_out;   // adjoint of the stage input; NULL: add it to adj_n instead
+    double *adj_n;      // accumulated dJ/dphi^n
+    double c1, c2;
+};
+
+__global__ __launch_bounds__(256) void adj_gather(GatherArgs a)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= a.g.n) return;
+    int i, j, k;
+    node3(a.g, l, i, j, k);
+    const int idx[3] = {i, j, k};
+    const int nn[3] = {a.g.ni, a.g.nj, a.g.nk};
+    const long st[3] = {1, a.g.sj, a.g.sk};
+    const long n = a.g.n;
+    const double lam = a.lam[l];
+    const bool upd = a.mask == nullptr || a.mask[l] > 0;
+    double acc = upd ? a.c2 * lam : lam;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+        for (int o = -2; o <= 2; ++o) {
+            const int src = idx[d] - o;   // node whose offset-o stencil entry is this node
+            if (src >= 1 && src <= nn[d] - 2) acc += a.P[(5 * d + o + 2) * n + l - (long)o * st[d]];
+        }
+    }
+    const double lin = upd ? a.c1 * lam : 0.0;
+    if (a.prev_out) {
+        a.prev_out[l] = acc;
+        a.adj_n[l] += lin;
+    } else {
+        a.adj_n[l] += lin + acc;
+    }
+}
+
+// (1) bcond_lsm transposed, one face phase (axis 2, 1, 0 in that order); the
+// BigPhi switch is evaluated on the stage's final values, which at the two
+// inner nodes of a face equal the values the face saw in the forward
+struct BcAdjArgs {
+    AdjGeo g;
+    const double *phi;   // forward stage result after bcond_lsm
+    double *lam;         // adjoint, updated in place
+    int bt[6];
+    bool flo[3], fhi[3];
+    double big;
+};
+
+__device__ __forceinline__ void bc_coefs(int bt, double p2, double big, bool is_im, double &a, double &c)
+{
+    if (bt == 0) { a = 1.0; c = 0.0; return; }
+    if (bt == 1) {
+        const double a2 = 4.0 / 3.0, b2 = -1.0 / 3.0, a1 = 1.0, b1 = 0.0;
+        const double ex = (1.0 + sign1(fabs(p2) - big)) / 2.0;
+        a = a2 + ex * (a1 - a2);
+        c = b2 + ex * (b1 - b2);
+        return;
+    }
+    if (is_im) { a = 2.0; c = -1.0; return; }
+    a = 1.0; c = 0.0;   // not reached: type 2 leaves other faces alone
+}
+
+__global__ __launch_bounds__(256) void adj_bcond_phase(BcAdjArgs a, int axis)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const AdjGeo &g = a.g;
+    int n1, n2, nax;
+    long s, s1, s2;
+    if (axis == 0) { n1 = g.nj; n2 = g.nk; nax = g.ni; s = 1; s1 = g.sj; s2 = g.sk; }
+    else if (axis == 1) { n1 = g.ni; n2 = g.nk; nax = g.nj; s = g.sj; s1 = 1; s2 = g.sk; }
+    else { n1 = g.ni; n2 = g.nj; nax = g.nk; s = g.sk; s1 = 1; s2 = g.sj; }
+    if (t >= (long)n1 * n2) return;
+    const long base = (t % n1) * s1 + (t / n1) * s2;
+    // faces of one phase do not interact (extents >= 4); the high face first, as
+    // the reverse of the forward order
+    const int bt_lo = a.bt[2 * axis], bt_hi = a.bt[2 * axis + 1];
+    if (a.fhi[axis] && !(bt_hi == 2 && axis != 0)) {
+        const long l = base + (long)(nax - 1) * s;
+        double ca, cc;
+        bc_coefs(bt_hi, a.phi[l - 2 * s], a.big, axis == 0, ca, cc);
+        const double v = a.lam[l];
+        a.lam[l - s] += ca * v;
+        a.lam[l - 2 * s] += cc * v;
+        a.lam[l] = 0.0;
+    }
+    if (a.flo[axis] && bt_lo != 2) {
+        const long l = base;
+        double ca, cc;
+        bc_coefs(bt_lo, a.phi[l + 2 * s], a.big, false, ca, cc);
+        const double v = a.lam[l];
+        a.lam[l + s] += ca * v;
+        a.lam[l + 2 * s] += cc * v;
+        a.lam[l] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void adj_fill(double *p, long n, double v)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) p[l] = v;
+}
+
+inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+size_t fbytes(const Blk &b) { return ((size_t)b.n * sizeof(double) + 255) & ~(size_t)255; }
+
+}  // namespace
+
+size_t rk3_adjoint_work(const Blk &b) { return 20 * fbytes(b) + 4096; }
+
+int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
+                       const double *U2, const double *U3, double dt, int mode, const double *adj_np1,
+                       double *adj_n, double *adjU1, double *adjU2, double *adjU3, Carver &w,
+                       hipStream_t s)
+{
+    const size_t fb = (size_t)b.n * sizeof(double);
+    double *F[3], *L[2];
+    for (auto &p : F) p = (double *)w.take(fb);
+    for (auto &p : L) p = (double *)w.take(fb);
+    double *P = (double *)w.take(15 * fb);
+    if (!F[0] || !F[1] || !F[2] || !L[0] || !L[1] || !P)
+        return fail(LSM_ERR_WORKSPACE, "adjoint workspace too small");
+    int rc;
+    // forward stages phi^(1), phi^(2), phi^(3)
+    if ((rc = launch_rk3_stages(b, phi_n, mask, U1, U2, U3, dt, mode, F[0], F[1], F[2], s))) return rc;
+    if ((rc = launch_copy(adj_np1, L[0], b.n, s))) return rc;
+    adj_fill<<<nblk(b.n), 256, 0, s>>>(adj_n, b.n, 0.0);
+
+    static const double C[3][3] = {{1.0, 0.0, 1.0},
+                                   {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
+                                   {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    const AdjGeo g{b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, b.n};
+    BcAdjArgs ba{};
+    ba.g = g;
+    for (int q = 0; q < 6; ++q) ba.bt[q] = b.btype[q];
+    ba.flo[0] = b.e.faces & LSM_FACE_I1; ba.fhi[0] = b.e.faces & LSM_FACE_IM;
+    ba.flo[1] = b.e.faces & LSM_FACE_J1; ba.fhi[1] = b.e.faces & LSM_FACE_JM;
+    ba.flo[2] = b.e.faces & LSM_FACE_K1; ba.fhi[2] = b.e.faces & LSM_FACE_KM;
+    ba.big = b.big_phi;
+    const long lines[3] = {(long)b.e.nj * b.e.nk, (long)b.e.ni * b.e.nk, (long)b.e.ni * b.e.nj};
+
+    double *lam = L[0], *other = L[1];
+    for (int st = 2; st >= 0; --st) {
+        const double *phi_s = F[st];
+        const double *prev = st == 0 ? phi_n : F[st - 1];
+        // (1) bcond_lsm transposed
+        ba.phi = phi_s;
+        ba.lam = lam;
+        for (int ax = 2; ax >= 0; --ax) adj_bcond_phase<<<nblk(lines[ax]), 256, 0, s>>>(ba, ax);
+        // (2) partials of the stage right-hand side
+        PartArgs pa{g, prev, lam, U1, U2, U3, b.aj, mask, P, adjU1, adjU2, adjU3, C[st][2], dt,
+                    {b.dc, b.de, b.dz}};
+        adj_partials<<<nblk(b.n), 256, 0, s>>>(pa);
+        // (3) gather into the adjoint of the stage input
+        GatherArgs ga{g, lam, P, mask, st == 0 ? nullptr : other, adj_n, C[st][0], C[st][1]};
+        adj_gather<<<nblk(b.n), 256, 0, s>>>(ga);
+        if ((rc = check_hip(hipGetLastError(), "adjoint kernels"))) return rc;
+        double *t = lam; lam = other; other = t;
+    }
+    return LSM_OK;
+}
+
+}  // namespace lsm

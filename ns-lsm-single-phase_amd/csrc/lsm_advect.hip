@@ -502,17 +502,16 @@ int launch_fold(const Blk &b, const double *U1, const double *U2, const double *
     return check_hip(hipGetLastError(), "fold_kernel");
 }
 
-int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t *mask,
-               const double *U1, const double *U2, const double *U3, double dt, int mode,
-               double *scratch, hipStream_t s, bool folded)
+// the three stages of one RK3 step, stage st reading src[st] and writing dst[st]
+static int rk3_run(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
+                   const double *U2, const double *U3, double dt, int mode, const double *const src[3],
+                   double *const dst[3], hipStream_t s, bool folded)
 {
     const Range r = make_range(b);
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
     const bool fast = mode != LSM_ADVECT_EXACT;
-    const double *src[3] = {phi_n, phi_np1, scratch};
-    double *dst[3] = {phi_np1, scratch, phi_np1};
 
     // interior stage kernel, then the boundary shell kernel, per stage
     int TY = 4;
@@ -578,6 +577,24 @@ int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t
         }
     }
     return LSM_OK;
+}
+
+int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t *mask,
+               const double *U1, const double *U2, const double *U3, double dt, int mode,
+               double *scratch, hipStream_t s, bool folded)
+{
+    const double *src[3] = {phi_n, phi_np1, scratch};
+    double *dst[3] = {phi_np1, scratch, phi_np1};
+    return rk3_run(b, phi_n, mask, U1, U2, U3, dt, mode, src, dst, s, folded);
+}
+
+int launch_rk3_stages(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
+                      const double *U2, const double *U3, double dt, int mode, double *out1,
+                      double *out2, double *out3, hipStream_t s)
+{
+    const double *src[3] = {phi_n, out1, out2};
+    double *dst[3] = {out1, out2, out3};
+    return rk3_run(b, phi_n, mask, U1, U2, U3, dt, mode, src, dst, s, false);
 }
 
 }  // namespace lsm

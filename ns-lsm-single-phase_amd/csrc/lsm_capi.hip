@@ -153,6 +153,7 @@ static size_t op_bytes(const Blk &b, int op)
     case LSM_OP_REINIT_ENO2: return reinit_eno2_work(b);
     case LSM_OP_REINIT_GODUNOV: return reinit_godunov_work(b);
     case LSM_OP_WATER_VOLUME: return water_volume_work(b);
+    case LSM_OP_RK3_ADJOINT: return rk3_adjoint_work(b);
     case LSM_OP_LEVELSET_STEP: {
         size_t a = 3 * field_bytes(b) + field_bytes(b) + field_bytes(b) + 1024;   // ucn, phi copy, rk3 scratch
         size_t r = reinit_eno2_work(b);
@@ -291,6 +292,27 @@ int lsm_rk3_step_folded(const LsmBlock *blk, double *phi_np1, const double *phi_
     if (!scratch) return fail(LSM_ERR_WORKSPACE, "workspace too small");
     return launch_rk3(b, phi_np1, phi_n, adv_mask, v1, v2, v3, 1.0, LSM_ADVECT_FAST, scratch,
                       (hipStream_t)stream, true);
+}
+
+int lsm_rk3_step_adjoint(const LsmBlock *blk, const double *phi_n, const int32_t *adv_mask,
+                         const double *ucn1, const double *ucn2, const double *ucn3, double delti_lsm,
+                         int mode, const double *adj_np1, double *adj_n, double *adj_ucn1,
+                         double *adj_ucn2, double *adj_ucn3, void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!b.aj) return fail(LSM_ERR_ARG, "aj is NULL");
+    if (!phi_n || !ucn1 || !ucn2 || !ucn3 || !adj_np1 || !adj_n) return fail(LSM_ERR_ARG, "NULL array");
+    if (adj_n == adj_np1) return fail(LSM_ERR_ARG, "adj_n and adj_np1 must not alias");
+    if ((adj_ucn1 != nullptr) != (adj_ucn2 != nullptr) || (adj_ucn1 != nullptr) != (adj_ucn3 != nullptr))
+        return fail(LSM_ERR_ARG, "adj_ucn1..3 must all be given or all be NULL");
+    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3_ADJOINT), &c);
+    if (rc) return rc;
+    return launch_rk3_adjoint(b, phi_n, adv_mask, ucn1, ucn2, ucn3, delti_lsm, mode, adj_np1, adj_n,
+                              adj_ucn1, adj_ucn2, adj_ucn3, c, (hipStream_t)stream);
 }
 
 int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizero, int order_bc,

@@ -63,9 +63,20 @@ int launch_bcond(const Blk &b, double *phi, hipStream_t s);
 int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t *mask,
                const double *U1, const double *U2, const double *U3, double dt, int mode,
                double *scratch, hipStream_t s, bool folded = false);
+// the three stage results of one RK3 step (phi^(1), phi^(2), phi^(3) = phi^(n+1))
+int launch_rk3_stages(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
+                      const double *U2, const double *U3, double dt, int mode, double *out1,
+                      double *out2, double *out3, hipStream_t s);
 int launch_fold(const Blk &b, const double *U1, const double *U2, const double *U3, double dt,
                 double *V1, double *V2, double *V3, int *bad, hipStream_t s);
 int launch_copy(const double *src, double *dst, long n, hipStream_t s);
+
+// ---- launchers (lsm_adjoint.hip) ----
+size_t rk3_adjoint_work(const Blk &b);
+int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
+                       const double *U2, const double *U3, double dt, int mode, const double *adj_np1,
+                       double *adj_n, double *adjU1, double *adjU2, double *adjU3, Carver &w,
+                       hipStream_t s);
 
 // ---- launchers (lsm_reinit.hip) ----
 size_t reinit_eno2_work(const Blk &b);

@@ -16,6 +16,7 @@ LSM_FACE_ALL = 0x3F
 LSM_ADVECT_FAST = 0
 LSM_ADVECT_EXACT = 1
 LSM_OP_RK3, LSM_OP_REINIT_ENO2, LSM_OP_REINIT_GODUNOV, LSM_OP_WATER_VOLUME, LSM_OP_LEVELSET_STEP = 1, 2, 3, 4, 5
+LSM_OP_RK3_ADJOINT = 6
 
 _vp = C.c_void_p
 
@@ -50,6 +51,8 @@ _SIGS = {
     "lsm_advect_fold": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, C.c_double, _vp, _vp, _vp, _vp]),
     "lsm_rk3_step_folded": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t,
                                       _vp]),
+    "lsm_rk3_step_adjoint": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int,
+                                       _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "lsm_normgrad_eno2": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, C.c_int, C.c_int, _vp, _vp]),
     "lsm_reinit_sussman_eno2": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp,
                                           _vp, _vp, C.c_size_t, _vp]),

@@ -173,6 +173,24 @@ def ad_RKTVD3(blk: Block, phi_n, ucn, delti_lsm: float, adv_mask=None, out=None,
     return out
 
 
+def ad_RKTVD3_adjoint(blk: Block, phi_n, ucn, delti_lsm: float, adj_np1, adv_mask=None,
+                      mode="fast", adj_ucn=None, out=None, work=None):
+    """Reverse mode of ad_RKTVD3: given adj_np1 = dJ/dphi^(n+1), returns
+    dJ/dphi^n; when adj_ucn (three tensors) is given, dJ/d(U/J) of this step
+    is ADDED to it.  The forward stages are recomputed from phi_n."""
+    L = _lib.load()
+    if out is None:
+        out = blk.empty()
+    wp, wn, keep = _ws(blk, _lib.LSM_OP_RK3_ADJOINT, work)
+    au = [_f64(t, blk.shape, "adj_ucn") for t in adj_ucn] if adj_ucn is not None else [None] * 3
+    check(L.lsm_rk3_step_adjoint(blk.ref, _f64(phi_n, blk.shape, "phi_n"),
+                                 _i32(adv_mask, blk.shape, "adv_mask"),
+                                 *[_f64(t, blk.shape, "ucn") for t in ucn], float(delti_lsm), MODES[mode],
+                                 _f64(adj_np1, blk.shape, "adj_np1"), _f64(out, blk.shape, "out"),
+                                 *au, wp, wn, _stream()), "lsm_rk3_step_adjoint")
+    return out
+
+
 def fold_velocity(blk: Block, ucn, delti_lsm: float):
     """Frozen-velocity operator V_d = aj*delti_lsm*d_d*ucn_d for ad_RKTVD3_folded
     (requires aj > 0; raises LsmError otherwise)."""

@@ -16,7 +16,8 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "liblsm_hip.so")
-SOURCES = ["lsm_capi.hip", "lsm_advect.hip", "lsm_reinit.hip", "lsm_volume.hip", "lsm_host.hip"]
+SOURCES = ["lsm_capi.hip", "lsm_advect.hip", "lsm_reinit.hip", "lsm_volume.hip", "lsm_host.hip",
+           "lsm_adjoint.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LSM_OFFLOAD_ARCH", "gfx950")
