@@ -258,7 +258,8 @@ int lsm_host_water_volume(const double *phi, double vol[3], int32_t *adv_mask);
 #define LSM_KT_ADVECT_SHELL   1   /* RK stage on the boundary shell + bcond_lsm */
 #define LSM_KT_REINIT_STAGE   2   /* fused ENO2 |grad d| + reinitialisation stage */
 #define LSM_KT_VOLUME         3   /* water volume cell kernel */
-#define LSM_KT_COUNT          4
+#define LSM_KT_ADJOINT_STAGE  4   /* fused adjoint of one RK stage (partials + gather) */
+#define LSM_KT_COUNT          5
 
 /* Turns event recording on (1) or off (0); clears accumulated timings. */
 int lsm_timing_enable(int on);

@@ -14,6 +14,8 @@
 //   3. per node, a gather of the partials of the nodes whose stencil holds it,
 //      plus the linear part c2 (and c1 into dJ/dphi^n).
 // Everything is deterministic (no atomics).
+#include <stdlib.h>
+
 #include "lsm_internal.hpp"
 
 namespace lsm {
@@ -226,6 +228,184 @@ __global__ __launch_bounds__(256) void adj_bcond_phase(BcAdjArgs a, int axis)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused form of steps (2) and (3): the partials stay on chip.  A workgroup of
+// 64 (i) x R (j) threads owns the 60 x (R - 4) nodes in its middle and marches
+// up a chunk of k planes; the two outer lanes/rows on each side only compute
+// partials for their neighbours.  Per plane: the stage input with a two-node
+// halo is staged in LDS, every thread forms the partials of its node, the i and
+// j partials go to LDS and are gathered by the nodes they point at, and the k
+// partials go to a five-plane register window of the thread's own column, so a
+// node's adjoint is complete two planes later.  The WENO3 gradient uses the
+// one-quotient form of the reconstruction (one reciprocal per face value).
+// ---------------------------------------------------------------------------
+
+// gradient of Phi = ((fC + fR) A1 + (1.5 fC - 0.5 fL) A0) / (2 A1 + A0),
+// A_k = ((diff_k)^2 + 1e-6)^2 (the reconstruction of weno3_fast) with respect to
+// f0..f3 on the forward's upwind branch
+__device__ __forceinline__ double weno3_grad_fast(double f0, double f1, double f2, double f3, double ws,
+                                                  double g[4])
+{
+#pragma clang fp contract(fast)
+    const bool pos = ws > 0.0;
+    const double fL = pos ? f0 : f3;
+    const double fC = pos ? f1 : f2;
+    const double fR = pos ? f2 : f1;
+    const double e = 0.000001;
+    const double t0 = fC - fR, t1 = fL - fC;
+    const double s0 = t0 * t0 + e, s1 = t1 * t1 + e;
+    const double A0 = s0 * s0, A1 = s1 * s1;
+    const double u1 = 1.5 * fC - 0.5 * fL;
+    const double N = (fC + fR) * A1 + u1 * A0;
+    const double D = 2.0 * A1 + A0;
+    double r = __builtin_amdgcn_rcp(D);
+    double er = fma(-D, r, 1.0);
+    r = fma(r, er, r);
+    er = fma(-D, r, 1.0);
+    r = fma(r, er, r);
+    const double phi = N * r;
+    // dA0 = 4 s0 t0 (dfC - dfR), dA1 = 4 s1 t1 (dfL - dfC)
+    const double k0 = 4.0 * s0 * t0, k1 = 4.0 * s1 * t1;
+    const double cA0 = u1 - phi, cA1 = (fC + fR) - 2.0 * phi;   // dPhi = (dN - phi dD) / D
+    const double gL = (cA1 * k1 - 0.5 * A0) * r;
+    const double gC = (A1 + 1.5 * A0 + cA0 * k0 - cA1 * k1) * r;
+    const double gR = (A1 - cA0 * k0) * r;
+    if (pos) { g[0] = gL; g[1] = gC; g[2] = gR; g[3] = 0.0; }
+    else { g[3] = gL; g[2] = gC; g[1] = gR; g[0] = 0.0; }
+    return phi;
+}
+
+__device__ __forceinline__ double weno3_dir_grad_fast(double m2, double m1, double c0, double p1, double p2,
+                                                      double ws, bool forced_minus, bool forced_plus,
+                                                      double d, double gl[5])
+{
+    double gp[4], gm[4];
+    const double fp = weno3_grad_fast(m1, c0, p1, forced_plus ? 0.0 : p2, forced_plus ? 1.0 : ws, gp);
+    const double fm = weno3_grad_fast(forced_minus ? 0.0 : m2, m1, c0, p1, forced_minus ? -1.0 : ws, gm);
+    const double k = ws * d;
+    gl[0] = k * (-gm[0]);
+    gl[1] = k * (gp[0] - gm[1]);
+    gl[2] = k * (gp[1] - gm[2]);
+    gl[3] = k * (gp[2] - gm[3]);
+    gl[4] = k * gp[3];
+    return fp - fm;
+}
+
+constexpr int AX = 64, AOX = 60;   // lanes per row, output nodes per row
+
+struct FusedArgs {
+    AdjGeo g;
+    const double *prev, *lam, *U1, *U2, *U3, *aj;
+    const int32_t *mask;
+    double *prev_out;   // adjoint of the stage input; NULL: add it to adj_n instead
+    double *adj_n;
+    double *aU1, *aU2, *aU3;
+    double c1, c2, c3, dt, dd[3];
+    int tiles_i, tiles_j, kc;
+};
+
+template <int R>
+__global__ __launch_bounds__(AX * R) void adj_stage_fused(FusedArgs a)
+{
+    constexpr int OY = R - 4;
+    const AdjGeo &g = a.g;
+    const int b = blockIdx.x;
+    const int ti = b % a.tiles_i, t2 = b / a.tiles_i;
+    const int tj = t2 % a.tiles_j, tk = t2 / a.tiles_j;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * AX + tx;
+    const int i = ti * AOX - 2 + tx, j = tj * OY - 2 + ty;
+    const int kb = tk * a.kc, ke = min(kb + a.kc - 1, g.nk - 1);
+    const bool inb = i >= 0 && i < g.ni && j >= 0 && j < g.nj;
+    const bool outq = inb && tx >= 2 && tx < 2 + AOX && ty >= 2 && ty < 2 + OY;
+    const bool inter_ij = i >= 1 && i <= g.ni - 2 && j >= 1 && j <= g.nj - 2;
+    const int ic = min(max(i, 0), g.ni - 1), jc = min(max(j, 0), g.nj - 1);
+    const long col = (long)ic + g.sj * jc;
+    auto kcl = [&](int kk) -> long { return g.sk * (long)min(max(kk, 0), g.nk - 1); };
+
+    __shared__ double sp[R + 4][AX + 4];      // stage input of the plane, two-node halo
+    __shared__ double Pi[5][R][AX];           // i partials (offset -2..2) of each node
+    __shared__ double Pj[5][R][AX];           // j partials
+
+    const double *__restrict__ prev = a.prev;
+    double q0 = prev[col + kcl(kb - 4)], q1 = prev[col + kcl(kb - 3)], q2 = prev[col + kcl(kb - 2)];
+    double q3 = prev[col + kcl(kb - 1)], q4 = prev[col + kcl(kb)];
+    double kacc0 = 0.0, kacc1 = 0.0, kacc2 = 0.0, kacc3 = 0.0, kacc4 = 0.0;   // planes p-2 .. p+2
+    double pend0 = 0.0, pend1 = 0.0;                                          // planes p-2, p-1
+    const int ncell = (R + 4) * (AX + 4);
+    for (int p = kb - 2; p <= ke + 2; ++p) {
+        // q0..q4 = prev(p-2 .. p+2) of this column
+        const bool live = p >= 0 && p < g.nk;
+        for (int c = tid; c < ncell; c += AX * R) {
+            const int rr = c / (AX + 4), cc = c % (AX + 4);
+            const int ii = min(max(ti * AOX - 4 + cc, 0), g.ni - 1);
+            const int jj = min(max(tj * OY - 4 + rr, 0), g.nj - 1);
+            sp[rr][cc] = prev[(long)ii + g.sj * jj + kcl(p)];
+        }
+        double lam = 0.0;
+        bool upd = false;
+        double U[3] = {0.0, 0.0, 0.0}, ajv = 0.0;
+        if (live && inb) {
+            const long l = col + g.sk * (long)p;
+            lam = a.lam[l];
+            upd = a.mask == nullptr || a.mask[l] > 0;
+            U[0] = a.U1[l]; U[1] = a.U2[l]; U[2] = a.U3[l];
+            ajv = a.aj[l];
+        }
+        __syncthreads();
+        const bool part = live && inb && inter_ij && p >= 1 && p <= g.nk - 2 && upd;
+        if (part) {
+            // one direction at a time, each written out before the next is formed
+            const double m = lam * (-((a.c3 * ajv) * a.dt));
+            const bool own = outq && p >= kb && p <= ke && a.aU1 != nullptr;
+            const long l = col + g.sk * (long)p;
+            const int cx = tx + 2, cy = ty + 2;
+            double gl[5];
+            double dph = weno3_dir_grad_fast(sp[cy][cx - 2], sp[cy][cx - 1], sp[cy][cx], sp[cy][cx + 1],
+                                             sp[cy][cx + 2], U[0], i == 1, i == g.ni - 2, a.dd[0], gl);
+#pragma unroll
+            for (int o = 0; o < 5; ++o) Pi[o][ty][tx] = m * gl[o];
+            if (own) a.aU1[l] += m * (a.dd[0] * dph);
+            __builtin_amdgcn_sched_barrier(0);
+            dph = weno3_dir_grad_fast(sp[cy - 2][cx], sp[cy - 1][cx], sp[cy][cx], sp[cy + 1][cx],
+                                      sp[cy + 2][cx], U[1], j == 1, j == g.nj - 2, a.dd[1], gl);
+#pragma unroll
+            for (int o = 0; o < 5; ++o) Pj[o][ty][tx] = m * gl[o];
+            if (own) a.aU2[l] += m * (a.dd[1] * dph);
+            __builtin_amdgcn_sched_barrier(0);
+            dph = weno3_dir_grad_fast(q0, q1, q2, q3, q4, U[2], p == 1, p == g.nk - 2, a.dd[2], gl);
+            kacc0 += m * gl[0]; kacc1 += m * gl[1]; kacc2 += m * gl[2]; kacc3 += m * gl[3];
+            kacc4 += m * gl[4];
+            if (own) a.aU3[l] += m * (a.dd[2] * dph);
+        } else {
+#pragma unroll
+            for (int o = 0; o < 5; ++o) { Pi[o][ty][tx] = 0.0; Pj[o][ty][tx] = 0.0; }
+        }
+        __syncthreads();
+        double pend2 = 0.0;
+        if (outq && live && p >= kb && p <= ke) {
+            // partial of node tx - (o - 2) with respect to this node
+            double sacc = upd ? a.c2 * lam : lam;
+#pragma unroll
+            for (int o = 0; o < 5; ++o) sacc += Pi[o][ty][tx - (o - 2)] + Pj[o][ty - (o - 2)][tx];
+            pend2 = sacc;
+            if (upd && a.prev_out) a.adj_n[col + g.sk * (long)p] += a.c1 * lam;
+            else if (upd) pend2 += a.c1 * lam;
+        }
+        // plane p - 2 has all its k partials now
+        const int pf = p - 2;
+        if (outq && pf >= kb && pf <= ke) {
+            const double tot = pend0 + kacc0;
+            const long l = col + g.sk * (long)pf;
+            if (a.prev_out) a.prev_out[l] = tot;
+            else a.adj_n[l] += tot;
+        }
+        kacc0 = kacc1; kacc1 = kacc2; kacc2 = kacc3; kacc3 = kacc4; kacc4 = 0.0;
+        pend0 = pend1; pend1 = pend2;
+        q0 = q1; q1 = q2; q2 = q3; q3 = q4; q4 = prev[col + kcl(p + 3)];
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void adj_fill(double *p, long n, double v)
 {
     const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -249,9 +429,8 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
     double *F[3], *L[2];
     for (auto &p : F) p = (double *)w.take(fb);
     for (auto &p : L) p = (double *)w.take(fb);
-    double *P = (double *)w.take(15 * fb);
-    if (!F[0] || !F[1] || !F[2] || !L[0] || !L[1] || !P)
-        return fail(LSM_ERR_WORKSPACE, "adjoint workspace too small");
+    if (!F[0] || !F[1] || !F[2] || !L[0] || !L[1]) return fail(LSM_ERR_WORKSPACE, "adjoint workspace too small");
+    double *P = (double *)w.take(15 * fb);   // only the two-kernel form uses it
     int rc;
     // forward stages phi^(1), phi^(2), phi^(3)
     if ((rc = launch_rk3_stages(b, phi_n, mask, U1, U2, U3, dt, mode, F[0], F[1], F[2], s))) return rc;
@@ -271,6 +450,13 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
     ba.big = b.big_phi;
     const long lines[3] = {(long)b.e.nj * b.e.nk, (long)b.e.ni * b.e.nk, (long)b.e.ni * b.e.nj};
 
+    // fused partials + gather (default) or the two-kernel form with the 15
+    // partial fields in HBM
+    int fused = 16;
+    if (const char *e = getenv("LSM_ADJ_FUSED")) fused = atoi(e);
+    int kc = 16;
+    if (const char *e = getenv("LSM_ADJ_KCHUNK")) { const int v = atoi(e); if (v > 0) kc = v; }
+
     double *lam = L[0], *other = L[1];
     for (int st = 2; st >= 0; --st) {
         const double *phi_s = F[st];
@@ -279,13 +465,26 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
         ba.phi = phi_s;
         ba.lam = lam;
         for (int ax = 2; ax >= 0; --ax) adj_bcond_phase<<<nblk(lines[ax]), 256, 0, s>>>(ba, ax);
-        // (2) partials of the stage right-hand side
-        PartArgs pa{g, prev, lam, U1, U2, U3, b.aj, mask, P, adjU1, adjU2, adjU3, C[st][2], dt,
-                    {b.dc, b.de, b.dz}};
-        adj_partials<<<nblk(b.n), 256, 0, s>>>(pa);
-        // (3) gather into the adjoint of the stage input
-        GatherArgs ga{g, lam, P, mask, st == 0 ? nullptr : other, adj_n, C[st][0], C[st][1]};
-        adj_gather<<<nblk(b.n), 256, 0, s>>>(ga);
+        if (fused == 8 || fused == 16) {
+            FusedArgs fa{g, prev, lam, U1, U2, U3, b.aj, mask, st == 0 ? nullptr : other, adj_n,
+                         adjU1, adjU2, adjU3, C[st][0], C[st][1], C[st][2], dt, {b.dc, b.de, b.dz},
+                         (b.e.ni + AOX - 1) / AOX, 0, kc};
+            const int oy = fused - 4;
+            fa.tiles_j = (b.e.nj + oy - 1) / oy;
+            const unsigned grid = (unsigned)(fa.tiles_i * fa.tiles_j * ((b.e.nk + kc - 1) / kc));
+            KTimer kt(LSM_KT_ADJOINT_STAGE, s);
+            if (fused == 16) adj_stage_fused<16><<<grid, dim3(AX, 16, 1), 0, s>>>(fa);
+            else adj_stage_fused<8><<<grid, dim3(AX, 8, 1), 0, s>>>(fa);
+        } else {
+            if (!P) return fail(LSM_ERR_WORKSPACE, "adjoint workspace too small");
+            // (2) partials of the stage right-hand side
+            PartArgs pa{g, prev, lam, U1, U2, U3, b.aj, mask, P, adjU1, adjU2, adjU3, C[st][2], dt,
+                        {b.dc, b.de, b.dz}};
+            adj_partials<<<nblk(b.n), 256, 0, s>>>(pa);
+            // (3) gather into the adjoint of the stage input
+            GatherArgs ga{g, lam, P, mask, st == 0 ? nullptr : other, adj_n, C[st][0], C[st][1]};
+            adj_gather<<<nblk(b.n), 256, 0, s>>>(ga);
+        }
         if ((rc = check_hip(hipGetLastError(), "adjoint kernels"))) return rc;
         double *t = lam; lam = other; other = t;
     }

@@ -86,6 +86,7 @@ _SIGS = {
 
 EXPORTS = tuple(_SIGS)
 LSM_KT_ADVECT_STAGE, LSM_KT_ADVECT_SHELL, LSM_KT_REINIT_STAGE, LSM_KT_VOLUME = 0, 1, 2, 3
+LSM_KT_ADJOINT_STAGE = 4
 
 _lib = None
 
