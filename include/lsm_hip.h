@@ -60,6 +60,8 @@ extern "C" {
 #define LSM_OP_WATER_VOLUME   4
 #define LSM_OP_LEVELSET_STEP  5
 #define LSM_OP_RK3_ADJOINT    6
+#define LSM_OP_REINIT_ENO2_ADJOINT 7   /* sized by lsm_adjoint_workspace_bytes */
+#define LSM_OP_STEP_ADJOINT   8        /* sized by lsm_adjoint_workspace_bytes */
 
 /* One curvilinear block (the host-associated state of levelsetmethod.F90:1-44). */
 typedef struct LsmBlock {
@@ -96,6 +98,11 @@ const char *lsm_last_error(void);
 
 /* Bytes of device scratch an operation needs for this block (LSM_OP_*). */
 size_t lsm_workspace_bytes(const LsmBlock *blk, int op);
+
+/* Bytes of device scratch of the adjoints that depend on the number of
+ * reinitialisation iterations (LSM_OP_REINIT_ENO2_ADJOINT, LSM_OP_STEP_ADJOINT;
+ * both keep the stage inputs of every iteration).  0 for rn_itermax > 64. */
+size_t lsm_adjoint_workspace_bytes(const LsmBlock *blk, int op, int rn_itermax);
 
 /* rhs_contra_j.F90:2-50 (rhs_contra_j): ucn_j = (1/aj) * (grad xi^j . u) at
  * every owned node.  u, v, w are read with element stride `vel_stride`
@@ -156,6 +163,13 @@ int lsm_rk3_step_adjoint(const LsmBlock *blk, const double *phi_n, const int32_t
                          double *adj_ucn2, double *adj_ucn3, void *work, size_t work_bytes,
                          void *stream);
 
+/* Adjoint of lsm_bcond (reverse mode of bcond_lsm.F90:2-568; the reference
+ * has no adjoint): adj <- (d bcond / d phi)^T adj in place, the face phases
+ * transposed in reverse order (k, j, i); phi is the forward field AFTER
+ * bcond_lsm (its BigPhi switch and boundary types are taken as in the
+ * forward). */
+int lsm_bcond_adjoint(const LsmBlock *blk, const double *phi, double *adj, void *stream);
+
 /* calc_NormGrad_d.F90:1-2448 (calc_NormGrad_d): |grad d| with ENO2
  * differences inside and biased differences of order `order_bc` (optionally
  * ENO2Boundary when eno_bc != 0 and order_bc == 2) on faces, edges and
@@ -169,6 +183,20 @@ int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizer
 int lsm_reinit_sussman_eno2(const LsmBlock *blk, const LsmReinitParams *p,
                             const double *phi0, double *phi_out, int32_t *iters_done,
                             void *work, size_t work_bytes, void *stream);
+
+/* Adjoint of lsm_reinit_sussman_eno2 (reverse mode of
+ * reinitialisation_test2.F90:42-210 with calc_NormGrad_d.F90:39-2446 and
+ * global_lsm.F90:120-133,200-219; the reference has no adjoint): adj_phi0 =
+ * (d phi_out / d phi0)^T adj_out.  The iterations are recomputed from phi0
+ * (phi_out, optional, receives their result).  The ENO2 / one-sided branches,
+ * the upwind side chosen by phi0, the update mask, the lambda switch and the
+ * iteration count are taken as in the forward; s(phi0) is differentiated.
+ * Where lambda is applied the corrected right-hand side is identically zero,
+ * and so is its derivative.  rn_itermax <= 64.  Deterministic.  Workspace:
+ * lsm_adjoint_workspace_bytes(blk, LSM_OP_REINIT_ENO2_ADJOINT, rn_itermax). */
+int lsm_reinit_sussman_eno2_adjoint(const LsmBlock *blk, const LsmReinitParams *p, const double *phi0,
+                                    const double *adj_out, double *adj_phi0, double *phi_out,
+                                    void *work, size_t work_bytes, void *stream);
 
 /* calc_sign_ini.F90:1-286 (calc_sign_ini): s(phi0) and the Godunov |grad phi0|. */
 int lsm_sign_ini(const LsmBlock *blk, const double *phi0, double epslsm, int m_sphi0,
@@ -206,6 +234,20 @@ int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p,
                       int numiter_lsm, double delti_lsm, int call_levelset,
                       int call_reinit, int rn_freq, int ntime, int mode,
                       double *vol_out, void *work, size_t work_bytes, void *stream);
+
+/* Adjoint of one full level-set step as levelsetmethod.F90:221-326 runs it
+ * with numiter_lsm = 1 and a reinitialisation every step: phi^(n+1) =
+ * bcond_lsm(reinitialisation_test2(ad_RKTVD3(phi^n))) (the reference has no
+ * adjoint).  Writes adj_n = dJ/dphi^n and ADDS dJ/d(U/J) of the step to
+ * adj_ucn1..3 when given; phi_np1 (optional) receives the forward result.
+ * Composes lsm_rk3_step_adjoint, lsm_reinit_sussman_eno2_adjoint and
+ * lsm_bcond_adjoint.  Workspace: lsm_adjoint_workspace_bytes(blk,
+ * LSM_OP_STEP_ADJOINT, rn_itermax). */
+int lsm_levelset_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, const double *phi_n,
+                              const double *ucn1, const double *ucn2, const double *ucn3,
+                              double delti_lsm, int mode, const double *adj_np1, double *adj_n,
+                              double *adj_ucn1, double *adj_ucn2, double *adj_ucn3, double *phi_np1,
+                              void *work, size_t work_bytes, void *stream);
 
 /* ---- host-memory adapter (for Fortran ISO_C_BINDING shims) -------------
  * Same operations on HOST arrays in the reference's layout: the adapter keeps

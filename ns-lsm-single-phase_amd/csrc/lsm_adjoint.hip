@@ -420,6 +420,33 @@ size_t fbytes(const Blk &b) { return ((size_t)b.n * sizeof(double) + 255) & ~(si
 
 size_t rk3_adjoint_work(const Blk &b) { return 20 * fbytes(b) + 4096; }
 
+namespace {
+
+BcAdjArgs bc_adj_args(const Blk &b, const AdjGeo &g)
+{
+    BcAdjArgs ba{};
+    ba.g = g;
+    for (int q = 0; q < 6; ++q) ba.bt[q] = b.btype[q];
+    ba.flo[0] = b.e.faces & LSM_FACE_I1; ba.fhi[0] = b.e.faces & LSM_FACE_IM;
+    ba.flo[1] = b.e.faces & LSM_FACE_J1; ba.fhi[1] = b.e.faces & LSM_FACE_JM;
+    ba.flo[2] = b.e.faces & LSM_FACE_K1; ba.fhi[2] = b.e.faces & LSM_FACE_KM;
+    ba.big = b.big_phi;
+    return ba;
+}
+
+}  // namespace
+
+int launch_bcond_adjoint(const Blk &b, const double *phi, double *adj, hipStream_t s)
+{
+    const AdjGeo g{b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, b.n};
+    BcAdjArgs ba = bc_adj_args(b, g);
+    ba.phi = phi;
+    ba.lam = adj;
+    const long lines[3] = {(long)b.e.nj * b.e.nk, (long)b.e.ni * b.e.nk, (long)b.e.ni * b.e.nj};
+    for (int ax = 2; ax >= 0; --ax) adj_bcond_phase<<<nblk(lines[ax]), 256, 0, s>>>(ba, ax);
+    return check_hip(hipGetLastError(), "adj_bcond_phase");
+}
+
 int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
                        const double *U2, const double *U3, double dt, int mode, const double *adj_np1,
                        double *adj_n, double *adjU1, double *adjU2, double *adjU3, Carver &w,
@@ -441,13 +468,7 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
     const AdjGeo g{b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, b.n};
-    BcAdjArgs ba{};
-    ba.g = g;
-    for (int q = 0; q < 6; ++q) ba.bt[q] = b.btype[q];
-    ba.flo[0] = b.e.faces & LSM_FACE_I1; ba.fhi[0] = b.e.faces & LSM_FACE_IM;
-    ba.flo[1] = b.e.faces & LSM_FACE_J1; ba.fhi[1] = b.e.faces & LSM_FACE_JM;
-    ba.flo[2] = b.e.faces & LSM_FACE_K1; ba.fhi[2] = b.e.faces & LSM_FACE_KM;
-    ba.big = b.big_phi;
+    BcAdjArgs ba = bc_adj_args(b, g);
     const long lines[3] = {(long)b.e.nj * b.e.nk, (long)b.e.ni * b.e.nk, (long)b.e.ni * b.e.nj};
 
     // fused partials + gather (default) or the two-kernel form with the 15

@@ -146,7 +146,7 @@ static void drain_timings()
 
 static size_t field_bytes(const Blk &b) { return ((size_t)b.n * sizeof(double) + 255) & ~(size_t)255; }
 
-static size_t op_bytes(const Blk &b, int op)
+static size_t op_bytes(const Blk &b, int op, int itermax = 0)
 {
     switch (op) {
     case LSM_OP_RK3: return field_bytes(b) + 256;
@@ -154,6 +154,15 @@ static size_t op_bytes(const Blk &b, int op)
     case LSM_OP_REINIT_GODUNOV: return reinit_godunov_work(b);
     case LSM_OP_WATER_VOLUME: return water_volume_work(b);
     case LSM_OP_RK3_ADJOINT: return rk3_adjoint_work(b);
+    case LSM_OP_REINIT_ENO2_ADJOINT: return reinit_eno2_adjoint_work(b, itermax);
+    case LSM_OP_STEP_ADJOINT: {
+        // rk3(phi^n), the reinitialised field after bcond_lsm, the RK3 scratch
+        // and two adjoint fields, then the larger of the two adjoints
+        const size_t a = 5 * field_bytes(b) + 1024;
+        const size_t r = reinit_eno2_adjoint_work(b, itermax);
+        const size_t k = rk3_adjoint_work(b);
+        return a + (r > k ? r : k);
+    }
     case LSM_OP_LEVELSET_STEP: {
         size_t a = 3 * field_bytes(b) + field_bytes(b) + field_bytes(b) + 1024;   // ucn, phi copy, rk3 scratch
         size_t r = reinit_eno2_work(b);
@@ -198,6 +207,14 @@ size_t lsm_workspace_bytes(const LsmBlock *blk, int op)
     Blk b;
     if (make_blk(blk, &b, false, false)) return 0;
     return op_bytes(b, op);
+}
+
+size_t lsm_adjoint_workspace_bytes(const LsmBlock *blk, int op, int rn_itermax)
+{
+    Blk b;
+    if (make_blk(blk, &b, false, false)) return 0;
+    if (rn_itermax < 0 || rn_itermax > kMaxTraceIters) return 0;
+    return op_bytes(b, op, rn_itermax);
 }
 
 int lsm_contra_velocity(const LsmBlock *blk, const double *u, const double *v, const double *w,
@@ -315,6 +332,15 @@ int lsm_rk3_step_adjoint(const LsmBlock *blk, const double *phi_n, const int32_t
                               adj_ucn1, adj_ucn2, adj_ucn3, c, (hipStream_t)stream);
 }
 
+int lsm_bcond_adjoint(const LsmBlock *blk, const double *phi, double *adj, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!phi || !adj) return fail(LSM_ERR_ARG, "NULL array");
+    return launch_bcond_adjoint(b, phi, adj, (hipStream_t)stream);
+}
+
 int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizero, int order_bc,
                       int eno_bc, double *normgrad, void *stream)
 {
@@ -349,6 +375,69 @@ int lsm_reinit_sussman_eno2(const LsmBlock *blk, const LsmReinitParams *p, const
     rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_ENO2), &c);
     if (rc) return rc;
     return launch_reinit_eno2(b, *p, phi0, phi_out, iters_done, c, (hipStream_t)stream);
+}
+
+int lsm_reinit_sussman_eno2_adjoint(const LsmBlock *blk, const LsmReinitParams *p, const double *phi0,
+                                    const double *adj_out, double *adj_phi0, double *phi_out,
+                                    void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false);
+    if (rc) return rc;
+    if ((rc = check_reinit(p))) return rc;
+    if (p->rn_itermax > kMaxTraceIters)
+        return fail(LSM_ERR_UNSUPPORTED, "rn_itermax = %d: the adjoint keeps at most %d iterations",
+                    p->rn_itermax, kMaxTraceIters);
+    if (!phi0 || !adj_out || !adj_phi0) return fail(LSM_ERR_ARG, "NULL array");
+    if (adj_phi0 == adj_out) return fail(LSM_ERR_ARG, "adj_phi0 and adj_out must not alias");
+    if (phi_out && (phi_out == phi0 || phi_out == adj_out || phi_out == adj_phi0))
+        return fail(LSM_ERR_ARG, "phi_out must not alias the other arrays");
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_ENO2_ADJOINT, p->rn_itermax), &c);
+    if (rc) return rc;
+    return launch_reinit_eno2_adjoint(b, *p, phi0, adj_out, adj_phi0, phi_out, c, (hipStream_t)stream);
+}
+
+int lsm_levelset_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, const double *phi_n,
+                              const double *ucn1, const double *ucn2, const double *ucn3,
+                              double delti_lsm, int mode, const double *adj_np1, double *adj_n,
+                              double *adj_ucn1, double *adj_ucn2, double *adj_ucn3, double *phi_np1,
+                              void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false);
+    if (rc) return rc;
+    if ((rc = check_reinit(p))) return rc;
+    if (p->rn_itermax > kMaxTraceIters)
+        return fail(LSM_ERR_UNSUPPORTED, "rn_itermax = %d: the adjoint keeps at most %d iterations",
+                    p->rn_itermax, kMaxTraceIters);
+    if (!phi_n || !ucn1 || !ucn2 || !ucn3 || !adj_np1 || !adj_n) return fail(LSM_ERR_ARG, "NULL array");
+    if (adj_n == adj_np1) return fail(LSM_ERR_ARG, "adj_n and adj_np1 must not alias");
+    if ((adj_ucn1 != nullptr) != (adj_ucn2 != nullptr) || (adj_ucn1 != nullptr) != (adj_ucn3 != nullptr))
+        return fail(LSM_ERR_ARG, "adj_ucn1..3 must all be given or all be NULL");
+    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
+    hipStream_t s = (hipStream_t)stream;
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_STEP_ADJOINT, p->rn_itermax), &c);
+    if (rc) return rc;
+    const size_t fb = (size_t)b.n * sizeof(double);
+    double *X = (double *)c.take(fb), *Z = (double *)c.take(fb), *scr = (double *)c.take(fb);
+    double *L = (double *)c.take(fb), *LX = (double *)c.take(fb);
+    if (!X || !Z || !scr || !L || !LX) return fail(LSM_ERR_WORKSPACE, "step adjoint workspace too small");
+    // forward: X = ad_RKTVD3(phi^n); reinitialisation of X (trace kept); Z = bcond_lsm of it
+    if ((rc = launch_rk3(b, X, phi_n, nullptr, ucn1, ucn2, ucn3, delti_lsm, mode, scr, s))) return rc;
+    Carver rest = c;
+    ReinitTrace t;
+    if ((rc = launch_reinit_eno2_trace(b, *p, X, t, Z, rest, s))) return rc;
+    if ((rc = launch_bcond(b, Z, s))) return rc;
+    if (phi_np1 && (rc = launch_copy(Z, phi_np1, b.n, s))) return rc;
+    // reverse: bcond_lsm, the reinitialisation, ad_RKTVD3
+    if ((rc = launch_copy(adj_np1, L, b.n, s))) return rc;
+    if ((rc = launch_bcond_adjoint(b, Z, L, s))) return rc;
+    if ((rc = launch_reinit_eno2_adjoint(b, *p, X, L, LX, nullptr, rest, s, &t))) return rc;
+    Carver rk = c;   // the trace is no longer needed: its memory serves the advection adjoint
+    return launch_rk3_adjoint(b, phi_n, nullptr, ucn1, ucn2, ucn3, delti_lsm, mode, LX, adj_n, adj_ucn1,
+                              adj_ucn2, adj_ucn3, rk, s);
 }
 
 int lsm_sign_ini(const LsmBlock *blk, const double *phi0, double epslsm, int m_sphi0,

@@ -93,6 +93,34 @@ int launch_lambda_rn(const Blk &b, const double *phi0, const double *ng0, const 
 int launch_reinit_godunov(const Blk &b, const LsmReinitParams &p, const double *phi0,
                           double *out, Carver &w, hipStream_t s);
 
+// The Sussman iterations of launch_reinit_eno2 with every stage input kept,
+// for the adjoint: D[it] is the input of iteration it (D[0] = phi0, D[nit] the
+// result if all iterations run), A[it] / B[it] the inputs of its stages 2 / 3,
+// conv[it] its update mask; *iters (device) the number of iterations that ran.
+constexpr int kMaxTraceIters = 64;
+struct ReinitTrace {
+    int nit;
+    const double *D[kMaxTraceIters + 1];
+    double *A[kMaxTraceIters], *B[kMaxTraceIters];
+    uint8_t *conv[kMaxTraceIters];
+    int32_t *iters;
+};
+size_t reinit_trace_work(const Blk &b, int itermax);
+// `out` (optional) receives the reinitialised field, D[*iters]
+int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t,
+                             double *out, Carver &w, hipStream_t s);
+
+// ---- launchers (lsm_reinit_adjoint.hip) ----
+size_t reinit_eno2_adjoint_work(const Blk &b, int itermax);
+// adj_phi0 = (d out / d phi0)^T adj_out for out = reinit(phi0); `out`
+// (optional) receives the forward result.  `pre`: a trace of phi0 already
+// recorded (then `out` is ignored and the trace's workspace stays in use)
+int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const double *phi0,
+                               const double *adj_out, double *adj_phi0, double *out, Carver &w,
+                               hipStream_t s, const ReinitTrace *pre = nullptr);
+// bcond_lsm transposed in place on adj; phi is the forward field after bcond_lsm
+int launch_bcond_adjoint(const Blk &b, const double *phi, double *adj, hipStream_t s);
+
 // ---- launchers (lsm_volume.hip) ----
 size_t water_volume_work(const Blk &b);
 int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *mask, Carver &w,

@@ -20,29 +20,16 @@
 #include <stdlib.h>
 
 #include "lsm_internal.hpp"
+#include "lsm_reinit_common.hpp"
 
 namespace lsm {
 
+using rn::Geo;
+using rn::make_geo;
+using rn::node_of;
+using rn::normgrad_node;
+
 namespace {
-
-struct Geo {
-    int ni, nj, nk;
-    long sj, sk;
-    double dd[3];
-    const double *m[9];   // csi0..2, eta0..2, zet0..2
-};
-
-Geo make_geo(const Blk &b)
-{
-    Geo g;
-    g.ni = b.e.ni; g.nj = b.e.nj; g.nk = b.e.nk;
-    g.sj = b.e.sj; g.sk = b.e.sk;
-    g.dd[0] = b.dc; g.dd[1] = b.de; g.dd[2] = b.dz;
-    g.m[0] = b.csi0; g.m[1] = b.csi1; g.m[2] = b.csi2;
-    g.m[3] = b.eta0; g.m[4] = b.eta1; g.m[5] = b.eta2;
-    g.m[6] = b.zet0; g.m[7] = b.zet1; g.m[8] = b.zet2;
-    return g;
-}
 
 struct Ctl {
     unsigned long long maxbits, minbits;
@@ -50,56 +37,6 @@ struct Ctl {
 };
 
 constexpr unsigned long long kInfBits = 0x7ff0000000000000ull;
-
-__device__ __forceinline__ void node_of(const Geo &g, long l, int &i, int &j, int &k)
-{
-    i = (int)(l % g.ni);
-    const long r = l / g.ni;
-    j = (int)(r % g.nj);
-    k = (int)(r / g.nj);
-}
-
-// |grad d| at node (i,j,k): calc_NormGrad_d.F90:39-2446 for a block whose six
-// faces are physical.  phi0s is phizero at the last interior node, which the
-// i = 1 / i = im face blocks use for their ENO2 upwind sign (those blocks never
-// assign phi0, calc_NormGrad_d.F90:187-367).
-__device__ double normgrad_node(const Geo &g, const double *__restrict__ d,
-                                const double *__restrict__ phi0, double phi0s, int order_bc,
-                                bool eno_bc, int i, int j, int k, long l)
-{
-    const int n[3] = {g.ni, g.nj, g.nk};
-    const int idx[3] = {i, j, k};
-    const long st[3] = {1, g.sj, g.sk};
-    bool on[3];
-    for (int a = 0; a < 3; ++a) on[a] = (idx[a] == 0 || idx[a] == n[a] - 1);
-    const double p0 = (on[0] && !on[1] && !on[2]) ? phi0s : phi0[l];
-    double gr[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const long s = st[a];
-        if (on[a]) {
-            const int dir = (idx[a] == 0) ? 1 : -1;
-            const long ss = s * dir;
-            if (a == 0 && i == 0 && k == 0 && !on[1]) {
-                // edge i = 1, k = 1: fixed second-order one-sided formula
-                // (calc_NormGrad_d.F90:1281-1283)
-                const double dc2 = 0.5 * g.dd[0];
-                gr[a] = dc2 * ((-(1.0 * d[l + 2]) + 4.0 * d[l + 1]) - 3.0 * d[l]);
-            } else {
-                gr[a] = g.dd[a] * biased(d[l], d[l + ss], d[l + 2 * ss], d[l + 3 * ss], order_bc, dir, eno_bc);
-            }
-        } else {
-            const bool lo = (idx[a] == 1), hi = (idx[a] == n[a] - 2);
-            const double fLL = lo ? 0.0 : d[l - 2 * s];
-            const double fRR = hi ? 0.0 : d[l + 2 * s];
-            gr[a] = g.dd[a] * eno2(p0, fLL, d[l - s], d[l], d[l + s], fRR, lo, hi);
-        }
-    }
-    const double gx = (g.m[0][l] * gr[0] + g.m[3][l] * gr[1]) + g.m[6][l] * gr[2];
-    const double gy = (g.m[1][l] * gr[0] + g.m[4][l] * gr[1]) + g.m[7][l] * gr[2];
-    const double gz = (g.m[2][l] * gr[0] + g.m[5][l] * gr[1]) + g.m[8][l] * gr[2];
-    return sqrt((gx * gx + gy * gy) + gz * gz);
-}
 
 __global__ __launch_bounds__(256) void normgrad_kernel(Geo g, const double *__restrict__ d,
                                                         const double *__restrict__ phi0, int order_bc,
@@ -496,10 +433,85 @@ int launch_normgrad(const Blk &b, const double *d, const double *phi0, int order
     return check_hip(hipGetLastError(), "normgrad_kernel");
 }
 
+namespace {
+
+const double kRkC[3][3] = {{1.0, 0.0, 1.0},
+                           {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
+                           {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+
+// launch shape of the tiled stage kernel: interior tiles of 64 x 4 nodes x kc
+// planes (padded to a multiple of 8 workgroups) plus the boundary-shell blocks
+struct RTShape {
+    RTArgs a;
+    unsigned grid;
+};
+
+RTShape rt_shape(const Blk &b, const LsmReinitParams &p, const double *phi0, uint8_t *conv, Ctl *ctl)
+{
+    RTShape r{};
+    RTArgs &ra = r.a;
+    ra.g = make_geo(b); ra.phi0 = phi0; ra.conv = conv; ra.ctl = ctl;
+    ra.tiles_i = (b.e.ni + RTX - 1) / RTX;
+    ra.tiles_j = (b.e.nj + RTY - 1) / RTY;
+    const int nk_int = b.e.nk - 2;
+    const int tij = ra.tiles_i * ra.tiles_j;
+    int kc = (nk_int + (8192 + tij - 1) / tij - 1) / ((8192 + tij - 1) / tij);
+    if (kc < 8) kc = 8;
+    if (const char *e = getenv("LSM_REINIT_KCHUNK")) { int v = atoi(e); if (v > 0) kc = v; }
+    ra.kc = kc;
+    ra.ntiles = tij * ((nk_int + kc - 1) / kc);
+    ra.grid_tiles = (ra.ntiles + 7) / 8 * 8;
+    ra.nI = 2L * b.e.nj * b.e.nk;
+    ra.nJ = 2L * (b.e.ni - 2) * b.e.nk;
+    ra.nK = 2L * (b.e.ni - 2) * (b.e.nj - 2);
+    const unsigned shell_blocks = (unsigned)((ra.nI + ra.nJ + ra.nK + 255) / 256);
+    r.grid = (unsigned)ra.grid_tiles + shell_blocks;
+    ra.p = StageP{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
+    return r;
+}
+
+void rt_launch(RTShape &r, int stage, hipStream_t s)
+{
+    const dim3 rblk(RTX, RTY, 1);
+    switch (stage) {
+    case 0: reinit_stage_tiled<0><<<r.grid, rblk, 0, s>>>(r.a); break;
+    case 1: reinit_stage_tiled<1><<<r.grid, rblk, 0, s>>>(r.a); break;
+    case 2: reinit_stage_tiled<2><<<r.grid, rblk, 0, s>>>(r.a); break;
+    default: reinit_stage_tiled<3><<<r.grid, rblk, 0, s>>>(r.a); break;
+    }
+}
+
+void rt_stage_params(RTShape &r, const LsmReinitParams &p, int st)
+{
+    r.a.p = StageP{kRkC[st][0], kRkC[st][1], kRkC[st][2], p.rn_timestep, p.epslsm, p.eps_reinit,
+                   p.order_bc, p.eno_bc};
+}
+
+// out = D[*iters] (the field the last iteration that ran produced)
+struct SelectArgs {
+    const double *D[kMaxTraceIters + 1];
+    int nit;
+    const int32_t *iters;
+    double *out;
+    long n;
+};
+
+__global__ __launch_bounds__(256) void trace_select(SelectArgs a)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= a.n) return;
+    const int it = min(max(*a.iters, 0), a.nit);
+    const double *src = a.D[0];
+    for (int q = 1; q <= a.nit; ++q)
+        if (q == it) src = a.D[q];
+    a.out[l] = src[l];
+}
+
+}  // namespace
+
 int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi0, double *out,
                        int32_t *iters, Carver &w, hipStream_t s)
 {
-    const Geo g = make_geo(b);
     double *A = (double *)w.take(b.n * sizeof(double));
     double *B = (double *)w.take(b.n * sizeof(double));
     uint8_t *conv = (uint8_t *)w.take((size_t)b.n);
@@ -510,34 +522,10 @@ int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi
     reinit_init_ctl<<<1, 1, 0, s>>>(ctl, iters);
     if ((rc = check_hip(hipGetLastError(), "reinit_init_ctl"))) return rc;
     if (p.rn_itermax <= 0) return LSM_OK;
-    static const double C[3][3] = {{1.0, 0.0, 1.0},
-                                   {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
-                                   {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
-    // tiled stages: interior tiles of 64 x 4 nodes x kc planes plus shell blocks
-    RTArgs ra{};
-    ra.g = g; ra.phi0 = phi0; ra.conv = conv; ra.ctl = ctl;
-    ra.tiles_i = (b.e.ni + RTX - 1) / RTX;
-    ra.tiles_j = (b.e.nj + RTY - 1) / RTY;
-    {
-        const int nk_int = b.e.nk - 2;
-        const int tij = ra.tiles_i * ra.tiles_j;
-        int kc = (nk_int + (8192 + tij - 1) / tij - 1) / ((8192 + tij - 1) / tij);
-        if (kc < 8) kc = 8;
-        if (const char *e = getenv("LSM_REINIT_KCHUNK")) { int v = atoi(e); if (v > 0) kc = v; }
-        ra.kc = kc;
-        ra.ntiles = tij * ((nk_int + kc - 1) / kc);
-    }
-    ra.grid_tiles = (ra.ntiles + 7) / 8 * 8;
-    ra.nI = 2L * b.e.nj * b.e.nk;
-    ra.nJ = 2L * (b.e.ni - 2) * b.e.nk;
-    ra.nK = 2L * (b.e.ni - 2) * (b.e.nj - 2);
-    const unsigned shell_blocks = (unsigned)((ra.nI + ra.nJ + ra.nK + 255) / 256);
-    const unsigned rgrid = (unsigned)ra.grid_tiles + shell_blocks;
-    const dim3 rblk(RTX, RTY, 1);
+    RTShape r = rt_shape(b, p, phi0, conv, ctl);
     // max/min of |grad phi0| before the first iteration (reinitialisation_test2.F90:96-101)
-    ra.p = StageP{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
-    ra.din = phi0; ra.dn = phi0; ra.dout = nullptr;
-    reinit_stage_tiled<0><<<rgrid, rblk, 0, s>>>(ra);
+    r.a.din = phi0; r.a.dn = phi0; r.a.dout = nullptr;
+    rt_launch(r, 0, s);
     if ((rc = check_hip(hipGetLastError(), "reinit_stage_tiled"))) return rc;
     const double *sin_[3] = {out, A, B};
     const double *sdn[3] = {out, out, out};
@@ -545,15 +533,67 @@ int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi
     for (int it = 0; it < p.rn_itermax; ++it) {
         reinit_check<<<1, 1, 0, s>>>(ctl, p.eps_reinit, iters);
         for (int st = 0; st < 3; ++st) {
-            StageP sp{C[st][0], C[st][1], C[st][2], p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc,
-                      p.eno_bc};
             KTimer kt(LSM_KT_REINIT_STAGE, s);
-            ra.p = sp; ra.din = sin_[st]; ra.dn = sdn[st]; ra.dout = sout[st];
-            if (st == 0) reinit_stage_tiled<1><<<rgrid, rblk, 0, s>>>(ra);
-            else if (st == 1) reinit_stage_tiled<2><<<rgrid, rblk, 0, s>>>(ra);
-            else reinit_stage_tiled<3><<<rgrid, rblk, 0, s>>>(ra);
+            rt_stage_params(r, p, st);
+            r.a.din = sin_[st]; r.a.dn = sdn[st]; r.a.dout = sout[st];
+            rt_launch(r, st + 1, s);
         }
         if ((rc = check_hip(hipGetLastError(), "reinit_stage"))) return rc;
+    }
+    return LSM_OK;
+}
+
+size_t reinit_trace_work(const Blk &b, int itermax)
+{
+    const size_t bytes_u8 = ((size_t)b.n + 255) & ~(size_t)255;
+    return (size_t)itermax * (3 * fb(b) + bytes_u8) + 2048;
+}
+
+int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t,
+                             double *out, Carver &w, hipStream_t s)
+{
+    const int nit = p.rn_itermax;
+    if (nit < 0 || nit > kMaxTraceIters)
+        return fail(LSM_ERR_UNSUPPORTED, "rn_itermax = %d outside 0..%d for the adjoint", nit, kMaxTraceIters);
+    t.nit = nit;
+    t.D[0] = phi0;
+    for (int it = 0; it < nit; ++it) {
+        t.A[it] = (double *)w.take(b.n * sizeof(double));
+        t.B[it] = (double *)w.take(b.n * sizeof(double));
+        double *Dn = (double *)w.take(b.n * sizeof(double));
+        t.conv[it] = (uint8_t *)w.take((size_t)b.n);
+        if (!t.A[it] || !t.B[it] || !Dn || !t.conv[it]) return fail(LSM_ERR_WORKSPACE, "reinit trace workspace too small");
+        t.D[it + 1] = Dn;
+    }
+    Ctl *ctl = (Ctl *)w.take(sizeof(Ctl));
+    t.iters = (int32_t *)w.take(sizeof(int32_t));
+    if (!ctl || !t.iters) return fail(LSM_ERR_WORKSPACE, "reinit trace workspace too small");
+    int rc;
+    reinit_init_ctl<<<1, 1, 0, s>>>(ctl, t.iters);
+    if ((rc = check_hip(hipGetLastError(), "reinit_init_ctl"))) return rc;
+    if (nit > 0) {
+        RTShape r = rt_shape(b, p, phi0, nullptr, ctl);
+        r.a.din = phi0; r.a.dn = phi0; r.a.dout = nullptr;
+        rt_launch(r, 0, s);
+        for (int it = 0; it < nit; ++it) {
+            reinit_check<<<1, 1, 0, s>>>(ctl, p.eps_reinit, t.iters);
+            const double *sin_[3] = {t.D[it], t.A[it], t.B[it]};
+            double *sout[3] = {t.A[it], t.B[it], const_cast<double *>(t.D[it + 1])};
+            r.a.conv = t.conv[it];
+            for (int st = 0; st < 3; ++st) {
+                rt_stage_params(r, p, st);
+                r.a.din = sin_[st]; r.a.dn = t.D[it]; r.a.dout = sout[st];
+                rt_launch(r, st + 1, s);
+            }
+        }
+        if ((rc = check_hip(hipGetLastError(), "reinit trace"))) return rc;
+    }
+    if (out) {
+        SelectArgs sa{};
+        for (int q = 0; q <= nit; ++q) sa.D[q] = t.D[q];
+        sa.nit = nit; sa.iters = t.iters; sa.out = out; sa.n = b.n;
+        trace_select<<<nb(b.n), 256, 0, s>>>(sa);
+        if ((rc = check_hip(hipGetLastError(), "trace_select"))) return rc;
     }
     return LSM_OK;
 }

@@ -1,0 +1,80 @@
+// Geometry and the |grad d| node formula shared by the reinitialisation
+// kernels and their adjoint.
+#pragma once
+
+#include "lsm_internal.hpp"
+
+namespace lsm {
+namespace rn {
+
+struct Geo {
+    int ni, nj, nk;
+    long sj, sk;
+    double dd[3];
+    const double *m[9];   // csi0..2, eta0..2, zet0..2
+};
+
+inline Geo make_geo(const Blk &b)
+{
+    Geo g;
+    g.ni = b.e.ni; g.nj = b.e.nj; g.nk = b.e.nk;
+    g.sj = b.e.sj; g.sk = b.e.sk;
+    g.dd[0] = b.dc; g.dd[1] = b.de; g.dd[2] = b.dz;
+    g.m[0] = b.csi0; g.m[1] = b.csi1; g.m[2] = b.csi2;
+    g.m[3] = b.eta0; g.m[4] = b.eta1; g.m[5] = b.eta2;
+    g.m[6] = b.zet0; g.m[7] = b.zet1; g.m[8] = b.zet2;
+    return g;
+}
+
+__device__ __forceinline__ void node_of(const Geo &g, long l, int &i, int &j, int &k)
+{
+    i = (int)(l % g.ni);
+    const long r = l / g.ni;
+    j = (int)(r % g.nj);
+    k = (int)(r / g.nj);
+}
+
+// |grad d| at node (i,j,k): calc_NormGrad_d.F90:39-2446 for a block whose six
+// faces are physical.  phi0s is phizero at the last interior node, which the
+// i = 1 / i = im face blocks use for their ENO2 upwind sign (those blocks never
+// assign phi0, calc_NormGrad_d.F90:187-367).
+__device__ inline double normgrad_node(const Geo &g, const double *__restrict__ d,
+                                const double *__restrict__ phi0, double phi0s, int order_bc,
+                                bool eno_bc, int i, int j, int k, long l)
+{
+    const int n[3] = {g.ni, g.nj, g.nk};
+    const int idx[3] = {i, j, k};
+    const long st[3] = {1, g.sj, g.sk};
+    bool on[3];
+    for (int a = 0; a < 3; ++a) on[a] = (idx[a] == 0 || idx[a] == n[a] - 1);
+    const double p0 = (on[0] && !on[1] && !on[2]) ? phi0s : phi0[l];
+    double gr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const long s = st[a];
+        if (on[a]) {
+            const int dir = (idx[a] == 0) ? 1 : -1;
+            const long ss = s * dir;
+            if (a == 0 && i == 0 && k == 0 && !on[1]) {
+                // edge i = 1, k = 1: fixed second-order one-sided formula
+                // (calc_NormGrad_d.F90:1281-1283)
+                const double dc2 = 0.5 * g.dd[0];
+                gr[a] = dc2 * ((-(1.0 * d[l + 2]) + 4.0 * d[l + 1]) - 3.0 * d[l]);
+            } else {
+                gr[a] = g.dd[a] * biased(d[l], d[l + ss], d[l + 2 * ss], d[l + 3 * ss], order_bc, dir, eno_bc);
+            }
+        } else {
+            const bool lo = (idx[a] == 1), hi = (idx[a] == n[a] - 2);
+            const double fLL = lo ? 0.0 : d[l - 2 * s];
+            const double fRR = hi ? 0.0 : d[l + 2 * s];
+            gr[a] = g.dd[a] * eno2(p0, fLL, d[l - s], d[l], d[l + s], fRR, lo, hi);
+        }
+    }
+    const double gx = (g.m[0][l] * gr[0] + g.m[3][l] * gr[1]) + g.m[6][l] * gr[2];
+    const double gy = (g.m[1][l] * gr[0] + g.m[4][l] * gr[1]) + g.m[7][l] * gr[2];
+    const double gz = (g.m[2][l] * gr[0] + g.m[5][l] * gr[1]) + g.m[8][l] * gr[2];
+    return sqrt((gx * gx + gy * gy) + gz * gz);
+}
+
+}  // namespace rn
+}  // namespace lsm

@@ -17,6 +17,7 @@ LSM_ADVECT_FAST = 0
 LSM_ADVECT_EXACT = 1
 LSM_OP_RK3, LSM_OP_REINIT_ENO2, LSM_OP_REINIT_GODUNOV, LSM_OP_WATER_VOLUME, LSM_OP_LEVELSET_STEP = 1, 2, 3, 4, 5
 LSM_OP_RK3_ADJOINT = 6
+LSM_OP_REINIT_ENO2_ADJOINT, LSM_OP_STEP_ADJOINT = 7, 8
 
 _vp = C.c_void_p
 
@@ -53,6 +54,13 @@ _SIGS = {
                                       _vp]),
     "lsm_rk3_step_adjoint": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int,
                                        _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "lsm_adjoint_workspace_bytes": (C.c_size_t, [C.POINTER(LsmBlock), C.c_int, C.c_int]),
+    "lsm_bcond_adjoint": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp]),
+    "lsm_reinit_sussman_eno2_adjoint": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp,
+                                                  _vp, _vp, _vp, C.c_size_t, _vp]),
+    "lsm_levelset_step_adjoint": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp,
+                                            _vp, C.c_double, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                            C.c_size_t, _vp]),
     "lsm_normgrad_eno2": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, C.c_int, C.c_int, _vp, _vp]),
     "lsm_reinit_sussman_eno2": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp,
                                           _vp, _vp, C.c_size_t, _vp]),

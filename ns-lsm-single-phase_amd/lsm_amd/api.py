@@ -92,6 +92,14 @@ class Block:
             raise LsmError("lsm_workspace_bytes failed: " + _lib.load().lsm_last_error().decode())
         return torch.empty(n, dtype=torch.uint8, device=self.device)
 
+    def adjoint_workspace(self, op: int, rn_itermax: int) -> torch.Tensor:
+        """Scratch of the adjoints that keep every reinitialisation iteration."""
+        n = _lib.load().lsm_adjoint_workspace_bytes(self.ref, op, int(rn_itermax))
+        if n == 0:
+            raise LsmError(f"lsm_adjoint_workspace_bytes failed (rn_itermax = {rn_itermax}): "
+                           + _lib.load().lsm_last_error().decode())
+        return torch.empty(n, dtype=torch.uint8, device=self.device)
+
 
 @dataclass
 class ReinitParams:
@@ -244,6 +252,32 @@ def reinitialisation_test2(blk: Block, phi0, params: ReinitParams, out=None, wor
     return out, iters
 
 
+def reinitialisation_test2_adjoint(blk: Block, phi0, params: ReinitParams, adj_out, out=None,
+                                  phi_out=None, work=None):
+    """Reverse mode of reinitialisation_test2: given adj_out = dJ/dphi_out,
+    returns dJ/dphi0.  The iterations are recomputed from phi0 (phi_out, if
+    given, receives their result)."""
+    L = _lib.load()
+    if out is None:
+        out = blk.empty()
+    if work is None:
+        work = blk.adjoint_workspace(_lib.LSM_OP_REINIT_ENO2_ADJOINT, params.RNitermax)
+    p = params.c()
+    check(L.lsm_reinit_sussman_eno2_adjoint(blk.ref, C.byref(p), _f64(phi0, blk.shape, "phi0"),
+                                            _f64(adj_out, blk.shape, "adj_out"), _f64(out, blk.shape, "out"),
+                                            _f64(phi_out, blk.shape, "phi_out") if phi_out is not None else None,
+                                            C.c_void_p(work.data_ptr()), work.numel(), _stream()),
+          "lsm_reinit_sussman_eno2_adjoint")
+    return out
+
+
+def bcond_lsm_adjoint(blk: Block, phi_bc, adj):
+    """Reverse mode of bcond_lsm, in place on adj (phi_bc: the field after bcond_lsm)."""
+    check(_lib.load().lsm_bcond_adjoint(blk.ref, _f64(phi_bc, blk.shape, "phi"), _f64(adj, blk.shape, "adj"),
+                                        _stream()), "lsm_bcond_adjoint")
+    return adj
+
+
 def calc_sign_ini(blk: Block, phi0, epslsm: float, M_Sphi0: int = 1):
     """calc_sign_ini.F90:1-286: (s(phi0), |grad phi0| Godunov)."""
     s, g = blk.empty(), blk.empty()
@@ -316,3 +350,26 @@ def levelsetmethod(blk: Block, phi, phi_n, u, v, w, params: ReinitParams, delti_
                               C.c_void_p(vol.data_ptr()) if vol is not None else None, wp, wn,
                               _stream()), "lsm_levelset_step")
     return vol
+
+
+def levelset_step_adjoint(blk: Block, phi_n, ucn, delti_lsm: float, params: ReinitParams, adj_np1,
+                          mode="fast", adj_ucn=None, out=None, phi_np1=None, work=None):
+    """Reverse mode of one full step phi^(n+1) = bcond_lsm(reinitialisation_test2(
+    ad_RKTVD3(phi^n))) (levelsetmethod with numiter_lsm = 1, reinitialisation
+    every step): returns dJ/dphi^n and ADDS dJ/d(U/J) to adj_ucn when given;
+    phi_np1 (optional) receives the forward result."""
+    L = _lib.load()
+    if out is None:
+        out = blk.empty()
+    if work is None:
+        work = blk.adjoint_workspace(_lib.LSM_OP_STEP_ADJOINT, params.RNitermax)
+    au = [_f64(t, blk.shape, "adj_ucn") for t in adj_ucn] if adj_ucn is not None else [None] * 3
+    p = params.c()
+    check(L.lsm_levelset_step_adjoint(blk.ref, C.byref(p), _f64(phi_n, blk.shape, "phi_n"),
+                                      *[_f64(t, blk.shape, "ucn") for t in ucn], float(delti_lsm),
+                                      MODES[mode], _f64(adj_np1, blk.shape, "adj_np1"),
+                                      _f64(out, blk.shape, "out"), *au,
+                                      _f64(phi_np1, blk.shape, "phi_np1") if phi_np1 is not None else None,
+                                      C.c_void_p(work.data_ptr()), work.numel(), _stream()),
+          "lsm_levelset_step_adjoint")
+    return out

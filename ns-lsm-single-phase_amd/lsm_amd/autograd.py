@@ -10,13 +10,22 @@
 * contravariant(blk, u, v, w): U/J = (1/aj) grad(xi_d) . u (rhs_contra_j.F90:
   28-48) in torch operations, so gradients reach the Cartesian velocity (and
   whatever parametrises it).
+* reinit(blk, phi0, params): reinitialisation_test2 (Sussman, ENO2) with the
+  hand-written adjoint (lsm_reinit_sussman_eno2_adjoint).
+* levelset_step(blk, phi_n, ucn, dt, params): one full step
+  bcond_lsm(reinitialisation_test2(ad_RKTVD3(phi^n))) with its adjoint
+  (lsm_levelset_step_adjoint); levelset_trajectory(...) runs nsteps of them
+  with checkpointing every k steps.
 
-The adjoint differentiates the WENO3 nonlinear weights; the upwind choice, the
-forced boundary stencils and the BigPhi switch of bcond_lsm are taken as in
-the forward (a.e. derivatives of the piecewise-smooth step).
+The adjoint differentiates the WENO3 nonlinear weights and s(phi0); the upwind
+choice, the forced boundary stencils, the BigPhi switch of bcond_lsm, the ENO2
+and one-sided branches of |grad d|, the reinitialisation update mask and its
+iteration count are taken as in the forward (a.e. derivatives of the
+piecewise-smooth step).
 """
 from __future__ import annotations
 
+import math
 from typing import Optional, Sequence
 
 import torch
@@ -108,3 +117,136 @@ def advect(blk, phi0, ucn: Sequence[torch.Tensor], dt: float, nsteps: int, mode:
     every = max(1, int(checkpoint_every))
     return _Trajectory.apply(phi0, ucn[0], ucn[1], ucn[2], blk, float(dt), int(nsteps), mode, adv_mask,
                              every)
+
+
+class _Reinit(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, phi0, blk, params):
+        phi0 = phi0.contiguous()
+        out, _ = api.reinitialisation_test2(blk, phi0, params)
+        ctx.save_for_backward(phi0)
+        ctx.blk, ctx.params = blk, params
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        (phi0,) = ctx.saved_tensors
+        return api.reinitialisation_test2_adjoint(ctx.blk, phi0, ctx.params, g.contiguous()), None, None
+
+
+def reinit(blk, phi0, params: "api.ReinitParams"):
+    """Differentiable reinitialisation_test2 (Sussman ENO2, non-hybrid)."""
+    return _Reinit.apply(phi0, blk, params)
+
+
+def _step_forward(blk, phi_n, U, dt, params, mode, work_rk=None, work_rn=None, tmp=None):
+    """bcond_lsm(reinitialisation_test2(ad_RKTVD3(phi_n)))."""
+    X = api.ad_RKTVD3(blk, phi_n, U, dt, mode=mode, out=tmp, work=work_rk)
+    out, _ = api.reinitialisation_test2(blk, X, params, work=work_rn)
+    return api.bcond_lsm(blk, out)
+
+
+class _Step(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, phi_n, U1, U2, U3, blk, dt, params, mode):
+        phi_n = phi_n.contiguous()
+        U = tuple(t.contiguous() for t in (U1, U2, U3))
+        out = _step_forward(blk, phi_n, U, dt, params, mode)
+        ctx.save_for_backward(phi_n, *U)
+        ctx.blk, ctx.dt, ctx.params, ctx.mode = blk, dt, params, mode
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        phi_n, U1, U2, U3 = ctx.saved_tensors
+        want_u = any(ctx.needs_input_grad[1:4])
+        aU = [torch.zeros_like(U1) for _ in range(3)] if want_u else None
+        adj = api.levelset_step_adjoint(ctx.blk, phi_n, (U1, U2, U3), ctx.dt, ctx.params, g.contiguous(),
+                                        mode=ctx.mode, adj_ucn=aU)
+        gu = aU if want_u else [None, None, None]
+        return (adj, *gu, None, None, None, None)
+
+
+def levelset_step(blk, phi_n, ucn: Sequence[torch.Tensor], dt: float, params: "api.ReinitParams",
+                  mode: str = "exact"):
+    """One differentiable level-set step: ad_RKTVD3, reinitialisation_test2,
+    bcond_lsm (levelsetmethod with numiter_lsm = 1, reinitialisation every step)."""
+    return _Step.apply(phi_n, ucn[0], ucn[1], ucn[2], blk, float(dt), params, mode)
+
+
+class _StepTrajectory(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, phi0, U1, U2, U3, blk, dt, params, nsteps, mode, every):
+        U = tuple(t.contiguous() for t in (U1, U2, U3))
+        ckpts = []
+        cur = phi0.contiguous().clone()
+        wrk = blk.workspace(api._lib.LSM_OP_RK3)
+        wrn = blk.workspace(api._lib.LSM_OP_REINIT_ENO2)
+        tmp = blk.empty()
+        for s in range(nsteps):
+            if s % every == 0:
+                ckpts.append(cur)
+            cur = _step_forward(blk, cur, U, dt, params, mode, wrk, wrn, tmp).clone()
+        ctx.save_for_backward(*U, *ckpts)
+        ctx.blk, ctx.dt, ctx.params, ctx.nsteps, ctx.mode, ctx.every = blk, dt, params, nsteps, mode, every
+        return cur
+
+    @staticmethod
+    def backward(ctx, g):
+        saved = ctx.saved_tensors
+        U, ckpts = saved[:3], saved[3:]
+        blk, dt, params, nsteps, mode, every = ctx.blk, ctx.dt, ctx.params, ctx.nsteps, ctx.mode, ctx.every
+        want_u = any(ctx.needs_input_grad[1:4])
+        aU = [torch.zeros_like(U[0]) for _ in range(3)] if want_u else None
+        wrk = blk.workspace(api._lib.LSM_OP_RK3)
+        wrn = blk.workspace(api._lib.LSM_OP_REINIT_ENO2)
+        wad = blk.adjoint_workspace(api._lib.LSM_OP_STEP_ADJOINT, params.RNitermax)
+        tmp = blk.empty()
+        lam = g.contiguous().clone()
+        nxt = blk.empty()
+        for sidx in range(len(ckpts) - 1, -1, -1):
+            s0 = sidx * every
+            s1 = min(s0 + every, nsteps)
+            states = [ckpts[sidx]]
+            for _ in range(s0, s1 - 1):
+                states.append(_step_forward(blk, states[-1], U, dt, params, mode, wrk, wrn, tmp).clone())
+            for st in reversed(states):
+                api.levelset_step_adjoint(blk, st, U, dt, params, lam, mode=mode, adj_ucn=aU, out=nxt,
+                                          work=wad)
+                lam, nxt = nxt, lam
+        gu = aU if want_u else [None, None, None]
+        return (lam, *gu, None, None, None, None, None, None)
+
+
+def levelset_trajectory(blk, phi0, ucn: Sequence[torch.Tensor], dt: float, params: "api.ReinitParams",
+                        nsteps: int, mode: str = "exact", checkpoint_every: int = 5):
+    """nsteps differentiable level-set steps (levelset_step) with checkpointing
+    every k steps: memory holds nsteps/k + k fields plus one step's adjoint
+    workspace."""
+    if nsteps < 1:
+        raise ValueError("nsteps must be >= 1")
+    every = max(1, int(checkpoint_every))
+    return _StepTrajectory.apply(phi0, ucn[0], ucn[1], ucn[2], blk, float(dt), params, int(nsteps), mode,
+                                 every)
+
+
+def heaviside(phi, eps: float):
+    """Smoothed Heaviside H_eps (global_lsm.F90:105-118)."""
+    mid = 0.5 * (1.0 + phi / eps + (1.0 / math.pi) * torch.sin(math.pi * phi / eps))
+    return torch.where(phi < -eps, torch.zeros_like(phi), torch.where(phi > eps, torch.ones_like(phi), mid))
+
+
+def fourier_velocity(blk, ks, es, amps, psis):
+    """u = sum_m amps_m e_m cos(2 pi k_m . x + psi_m) at the block's nodes, in
+    torch operations (amps may require grad)."""
+    x, y, z = blk.x, blk.y, blk.z
+    u = torch.zeros_like(x)
+    v = torch.zeros_like(x)
+    w = torch.zeros_like(x)
+    for m in range(len(ks)):
+        k, e = ks[m], es[m]
+        ph = torch.cos(2.0 * math.pi * (float(k[0]) * x + float(k[1]) * y + float(k[2]) * z) + float(psis[m]))
+        u = u + amps[m] * float(e[0]) * ph
+        v = v + amps[m] * float(e[1]) * ph
+        w = w + amps[m] * float(e[2]) * ph
+    return u, v, w

@@ -17,7 +17,7 @@ CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "liblsm_hip.so")
 SOURCES = ["lsm_capi.hip", "lsm_advect.hip", "lsm_reinit.hip", "lsm_volume.hip", "lsm_host.hip",
-           "lsm_adjoint.hip"]
+           "lsm_adjoint.hip", "lsm_reinit_adjoint.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LSM_OFFLOAD_ARCH", "gfx950")
@@ -25,7 +25,8 @@ FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17",
          # reference operation order: no implicit multiply-add contraction
          "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+         "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Werror=infinite-recursion",
+         "-Werror=uninitialized",
          f"-I{INCLUDE}", f"-I{CSRC}"]
 
 

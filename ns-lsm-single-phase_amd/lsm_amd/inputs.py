@@ -199,10 +199,10 @@ def solitary_velocity(g: Grid, H=0.3, d=1.0, x0=5.0, gacc=9.81):
     return u, v, w
 
 
-def fourier_velocity(g: Grid, nmodes=16, kmax=4, seed=4):
-    """Divergence-free random Fourier field u = sum_m a_m e_m cos(2 pi k_m.x + psi_m),
-    k_m in Z^3 with 1 <= |k| <= kmax, e_m a unit vector normal to k_m,
-    a ~ N(0,1), psi ~ U[0, 2 pi); scaled to unit RMS over the nodes."""
+def fourier_modes(nmodes=16, kmax=4, seed=4):
+    """The random modes of fourier_velocity: wavevectors k_m (nmodes x 3,
+    integer-valued), unit directions e_m normal to k_m, amplitudes a_m ~ N(0,1)
+    and phases psi_m ~ U[0, 2 pi)."""
     rng = np.random.default_rng(seed)
     cand = np.array([(a, b, c) for a in range(-kmax, kmax + 1)
                      for b in range(-kmax, kmax + 1)
@@ -210,22 +210,46 @@ def fourier_velocity(g: Grid, nmodes=16, kmax=4, seed=4):
                      if 1 <= a * a + b * b + c * c <= kmax * kmax], dtype=np.float64)
     pick = rng.choice(len(cand), nmodes, replace=False)
     ks = cand[pick]
-    u = np.zeros(g.x.shape)
-    v = np.zeros(g.x.shape)
-    w = np.zeros(g.x.shape)
+    es = np.zeros((nmodes, 3))
+    amps = np.zeros(nmodes)
+    psis = np.zeros(nmodes)
     for m in range(nmodes):
         k = ks[m]
         r = rng.normal(size=3)
         e = r - (r @ k) / (k @ k) * k
-        e /= np.linalg.norm(e)
-        a = rng.normal()
-        psi = rng.uniform(0.0, 2.0 * math.pi)
-        ph = np.cos(2.0 * math.pi * (k[0] * g.x + k[1] * g.y + k[2] * g.z) + psi)
-        u += a * e[0] * ph
-        v += a * e[1] * ph
-        w += a * e[2] * ph
+        es[m] = e / np.linalg.norm(e)
+        amps[m] = rng.normal()
+        psis[m] = rng.uniform(0.0, 2.0 * math.pi)
+    return ks, es, amps, psis
+
+
+def fourier_field(g: Grid, ks, es, amps, psis):
+    """u = sum_m amps_m e_m cos(2 pi k_m . x + psi_m) at the nodes of g."""
+    u = np.zeros(g.x.shape)
+    v = np.zeros(g.x.shape)
+    w = np.zeros(g.x.shape)
+    for m in range(len(amps)):
+        k, e = ks[m], es[m]
+        ph = np.cos(2.0 * math.pi * (k[0] * g.x + k[1] * g.y + k[2] * g.z) + psis[m])
+        u += amps[m] * e[0] * ph
+        v += amps[m] * e[1] * ph
+        w += amps[m] * e[2] * ph
+    return u, v, w
+
+
+def fourier_velocity(g: Grid, nmodes=16, kmax=4, seed=4, return_modes=False):
+    """Divergence-free random Fourier field u = sum_m a_m e_m cos(2 pi k_m.x + psi_m),
+    k_m in Z^3 with 1 <= |k| <= kmax, e_m a unit vector normal to k_m,
+    a ~ N(0,1), psi ~ U[0, 2 pi); scaled to unit RMS over the nodes.  With
+    return_modes the modes are returned too, amplitudes already divided by the
+    RMS, so fourier_field(g, *modes) gives the same field."""
+    ks, es, amps, psis = fourier_modes(nmodes, kmax, seed)
+    u, v, w = fourier_field(g, ks, es, amps, psis)
     rms = math.sqrt(float(np.mean(u * u + v * v + w * w)))
-    return u / rms, v / rms, w / rms
+    out = (u / rms, v / rms, w / rms)
+    if return_modes:
+        return out, (ks, es, amps / rms, psis)
+    return out
 
 
 def contravariant(g: Grid, u, v, w):

@@ -1,0 +1,305 @@
+"""The hand-written adjoints of the Sussman reinitialisation
+(lsm_reinit_sussman_eno2_adjoint), of bcond_lsm (lsm_bcond_adjoint) and of the
+full level-set step bcond_lsm(reinitialisation_test2(ad_RKTVD3(phi)))
+(lsm_levelset_step_adjoint).
+
+Checked three ways:
+  * an FP64 PyTorch re-expression of the reinitialisation (tests/torch_ref.py,
+    CPU) is pinned to the oracle (forward to 1e-14) and its autograd gradient
+    to central differences of the oracle forward (the oracle is bit-identical
+    to the compiled reference Fortran);
+  * the HIP adjoints against that autograd gradient (1e-10 relative: the same
+    derivatives summed in another order);
+  * the HIP adjoint of the full step against central differences of the
+    oracle forward along seeded random directions (1e-6 relative).
+A finite difference is only a derivative where no switch of the forward
+(ENO2 stencil choice, update mask, band edge) lies within the step; the
+difference is taken at step 1e-6 and, where it and the difference at 1e-7
+disagree (a switch lies within the step), at the smallest step of 1e-6,
+1e-7, ... at which consecutive steps agree.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import oracle as O  # noqa: E402
+import torch_ref as T  # noqa: E402
+from lsm_amd import inputs  # noqa: E402
+
+# (shape, OrderReinitialisationBoundaries, ENOBCReinitialisation, iterations)
+CASES = [((12, 10, 9), 1, False, 4), ((11, 9, 8), 2, True, 3), ((10, 12, 11), 3, False, 4),
+         ((9, 8, 10), 2, False, 2)]
+
+
+def _field(shape, seed=81):
+    ni, nj, nk = shape
+    g = inputs.sinusoidal_cube_grid(ni, amp=0.04, seed=seed, nj=nj, nk=nk)
+    phi = inputs.spheres_sdf(g, nspheres=3, seed=seed + 1, rmin=0.2, rmax=0.35)
+    # a distorted distance function (|grad phi| != 1) for the reinitialisation to act on
+    phi = phi * (0.5 + (g.x - 0.3) ** 2)
+    hmin, hmax = inputs.spacing_bounds(g)
+    return g, phi, hmin, hmax
+
+
+def _t(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64)
+
+
+def _met(g):
+    return ([_t(c) for c in g.csi], [_t(c) for c in g.eta], [_t(c) for c in g.zet])
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _fd(fun, x, e, h):
+    return (fun(x + h * e) - fun(x - h * e)) / (2.0 * h)
+
+
+def _fd_smooth(fun, x, e, h=1e-6, hmin=1e-9):
+    """Central difference at step h, unless a switch of the forward lies
+    within the step: then the difference at h and at h/10 disagree (the
+    jump divided by the step), and the step is divided by 10 until two
+    consecutive steps agree to 1e-7 relative."""
+    f1 = _fd(fun, x, e, h)
+    while h > hmin:
+        f2 = _fd(fun, x, e, h / 10.0)
+        if abs(f1 - f2) <= 1e-7 * max(abs(f1), abs(f2)):
+            return f1
+        h, f1 = h / 10.0, f2
+    return f1
+
+
+@pytest.mark.parametrize("shape,order,eno,nit", CASES)
+def test_torch_reinit_matches_oracle(shape, order, eno, nit):
+    g, phi, hmin, hmax = _field(shape)
+    ref, it = O.reinit_eno2(O.Block(g), phi, 1.5 * hmax, 1e-9, 0.5 * hmin, nit, order, eno)
+    out, it2 = T.reinit(_t(phi), _met(g), (g.dc, g.de, g.dz), 1.5 * hmax, 1e-9, 0.5 * hmin, nit, order, eno)
+    assert it == it2 == nit
+    assert _rel(out.numpy(), ref) <= 1e-14
+
+
+@pytest.mark.parametrize("shape,order,eno,nit", CASES[:3])
+def test_torch_reinit_gradient_matches_fd_of_oracle(shape, order, eno, nit):
+    """Pins the re-expression's autograd gradient (1e-6 relative)."""
+    g, phi, hmin, hmax = _field(shape)
+    ob = O.Block(g)
+    w = np.random.default_rng(5).standard_normal(g.shape)
+    p = _t(phi).requires_grad_(True)
+    out, _ = T.reinit(p, _met(g), (g.dc, g.de, g.dz), 1.5 * hmax, 1e-9, 0.5 * hmin, nit, order, eno)
+    (out * _t(w)).sum().backward()
+    gphi = p.grad.numpy()
+    rng = np.random.default_rng(6)
+    for _ in range(3):
+        e = rng.standard_normal(g.shape)
+        fd = _fd_smooth(lambda x: np.sum(w * O.reinit_eno2(ob, x, 1.5 * hmax, 1e-9, 0.5 * hmin, nit, order,
+                                                           eno)[0]), phi, e)
+        an = float(np.sum(gphi * e))
+        assert abs(fd - an) <= 1e-6 * abs(an) + 1e-9, (fd, an)
+
+
+def _gpu_setup(g, btype=(1, 1, 1, 1, 1, 1), big_phi=5.0):
+    from lsm_amd import api
+    blk = api.Block(g, btype=btype, big_phi=big_phi)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
+    return api, blk, dev
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,order,eno,nit", CASES)
+def test_reinit_adjoint_matches_autograd(shape, order, eno, nit):
+    g, phi, hmin, hmax = _field(shape)
+    api, blk, dev = _gpu_setup(g)
+    w = np.random.default_rng(7).standard_normal(g.shape)
+    p = _t(phi).requires_grad_(True)
+    out, _ = T.reinit(p, _met(g), (g.dc, g.de, g.dz), 1.5 * hmax, 1e-9, 0.5 * hmin, nit, order, eno)
+    (out * _t(w)).sum().backward()
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=nit, epsReinitialisation=1e-9,
+                           OrderReinitialisationBoundaries=order, ENOBCReinitialisation=eno)
+    fwd = blk.empty()
+    adj = api.reinitialisation_test2_adjoint(blk, dev(phi), prm, dev(w), phi_out=fwd)
+    assert _rel(fwd.cpu().numpy(), out.detach().numpy()) <= 1e-13
+    assert _rel(adj.cpu().numpy(), p.grad.numpy()) <= 1e-10, _rel(adj.cpu().numpy(), p.grad.numpy())
+
+
+@pytest.mark.gpu
+def test_reinit_adjoint_without_iterations_is_identity():
+    """A field that passes the exit test at once runs no iteration
+    (reinitialisation_test2.F90:100-107): output = input, adjoint = identity,
+    decided on the device."""
+    g = inputs.sinusoidal_cube_grid(12, amp=0.02, seed=81, nj=10, nk=9)
+    hmin, hmax = inputs.spacing_bounds(g)
+    phi = (g.z - 0.45 - 0.1 * g.x) * 1.05
+    api, blk, dev = _gpu_setup(g)
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4, epsReinitialisation=0.1,
+                           OrderReinitialisationBoundaries=2)
+    _, it = O.reinit_eno2(O.Block(g), phi, 1.5 * hmax, 0.1, 0.5 * hmin, 4, 2, False)
+    assert it == 0
+    lam = np.random.default_rng(3).standard_normal(g.shape)
+    fwd = blk.empty()
+    adj = api.reinitialisation_test2_adjoint(blk, dev(phi), prm, dev(lam), phi_out=fwd)
+    assert np.array_equal(fwd.cpu().numpy(), phi)
+    assert np.array_equal(adj.cpu().numpy(), lam)
+
+
+@pytest.mark.gpu
+def test_reinit_adjoint_matches_fd_of_oracle():
+    g, phi, hmin, hmax = _field((14, 12, 11), seed=91)
+    api, blk, dev = _gpu_setup(g)
+    ob = O.Block(g)
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4, epsReinitialisation=1e-9)
+    w = np.random.default_rng(8).standard_normal(g.shape)
+    adj = api.reinitialisation_test2_adjoint(blk, dev(phi), prm, dev(w)).cpu().numpy()
+    rng = np.random.default_rng(9)
+    for _ in range(6):
+        e = rng.standard_normal(g.shape)
+        fd = _fd_smooth(lambda x: np.sum(w * O.reinit_eno2(ob, x, 1.5 * hmax, 1e-9, 0.5 * hmin, 4)[0]), phi, e)
+        an = float(np.sum(adj * e))
+        assert abs(fd - an) <= 1e-6 * abs(an) + 1e-9, (fd, an)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("btype,big", [((1, 1, 1, 1, 1, 1), 5.0), ((0, 2, 1, 2, 1, 0), 0.3),
+                                       ((2, 0, 2, 1, 0, 1), 0.3)])
+def test_bcond_adjoint_matches_autograd(btype, big):
+    g, phi, _, _ = _field((11, 10, 9))
+    api, blk, dev = _gpu_setup(g, btype, big)
+    w = np.random.default_rng(11).standard_normal(g.shape)
+    p = _t(phi).requires_grad_(True)
+    out = T.bcond(p, btype, big)
+    (out * _t(w)).sum().backward()
+    after = api.bcond_lsm(blk, dev(phi))
+    assert np.array_equal(after.cpu().numpy(), out.detach().numpy())
+    adj = api.bcond_lsm_adjoint(blk, after, dev(w))
+    assert _rel(adj.cpu().numpy(), p.grad.numpy()) <= 1e-15
+
+
+def _step_case(shape, seed):
+    g, phi, hmin, hmax = _field(shape, seed)
+    phi = inputs.spheres_sdf(g, nspheres=3, seed=seed + 1, rmin=0.2, rmax=0.35)
+    vel = inputs.fourier_velocity(g, seed=seed + 2)
+    dt = inputs.cfl_dt(g, *vel)
+    return g, phi, vel, dt, hmin, hmax
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_step_adjoint_matches_autograd(mode):
+    """Two full steps: the HIP adjoint of each against autograd of the
+    re-expression (rk3_step -> reinit -> bcond), for phi and U/J."""
+    from lsm_amd import autograd
+    g, phi, vel, dt, hmin, hmax = _step_case((12, 11, 10), 131)
+    btype = (1, 0, 1, 2, 1, 1)
+    api, blk, dev = _gpu_setup(g, btype, 5.0)
+    ob = O.Block(g, btype=btype, big_phi=5.0)
+    U = O.contra(ob, *vel)
+    eps_l, dtau = 1.5 * hmax, 0.5 * hmin
+    w = np.random.default_rng(12).standard_normal(g.shape)
+    p = _t(phi).requires_grad_(True)
+    Ut = [_t(u).requires_grad_(True) for u in U]
+    out = p
+    for _ in range(2):
+        x = T.rk3_step(out, Ut, _t(g.aj), dt, (g.dc, g.de, g.dz), btype, 5.0)
+        y, _ = T.reinit(x, _met(g), (g.dc, g.de, g.dz), eps_l, 1e-9, dtau, 4)
+        out = T.bcond(y, btype, 5.0)
+    (out * _t(w)).sum().backward()
+    prm = api.ReinitParams(epslsm=eps_l, RNtimestep=dtau, RNitermax=4, epsReinitialisation=1e-9)
+    pd = dev(phi).requires_grad_(True)
+    Ud = [dev(u).requires_grad_(True) for u in U]
+    o = pd
+    for _ in range(2):
+        o = autograd.levelset_step(blk, o, Ud, dt, prm, mode=mode)
+    assert _rel(o.detach().cpu().numpy(), out.detach().numpy()) <= (1e-13 if mode == "exact" else 1e-11)
+    (o * dev(w)).sum().backward()
+    tol = 1e-10 if mode == "exact" else 1e-8
+    assert _rel(pd.grad.cpu().numpy(), p.grad.numpy()) <= tol, _rel(pd.grad.cpu().numpy(), p.grad.numpy())
+    for c in range(3):
+        assert _rel(Ud[c].grad.cpu().numpy(), Ut[c].grad.numpy()) <= tol, c
+
+
+@pytest.mark.gpu
+def test_step_trajectory_checkpoints_match_step_chain():
+    """levelset_trajectory with checkpoints every 1, 2 and 3 steps gives the
+    gradients of the step-by-step chain."""
+    from lsm_amd import autograd
+    g, phi, vel, dt, hmin, hmax = _step_case((14, 12, 12), 141)
+    api, blk, dev = _gpu_setup(g)
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=3, epsReinitialisation=1e-9)
+    w = dev(np.random.default_rng(13).standard_normal(g.shape))
+    grads = []
+    for every in (None, 1, 2, 3):
+        p = dev(phi).requires_grad_(True)
+        uvw = [dev(a).requires_grad_(True) for a in vel]
+        Ud = autograd.contravariant(blk, *uvw)
+        if every is None:
+            out = p
+            for _ in range(4):
+                out = autograd.levelset_step(blk, out, Ud, dt, prm)
+        else:
+            out = autograd.levelset_trajectory(blk, p, Ud, dt, prm, 4, checkpoint_every=every)
+        (out * w).sum().backward()
+        grads.append([p.grad.cpu().numpy()] + [a.grad.cpu().numpy() for a in uvw])
+    for gset in grads[1:]:
+        assert np.array_equal(gset[0], grads[0][0])
+        for a, b in zip(gset[1:], grads[0][1:]):
+            assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(b))
+
+
+def _oracle_steps(ob, phi, U, dt, eps_l, dtau, nit, nsteps):
+    cur = phi
+    for _ in range(nsteps):
+        x = O.rk3(ob, cur, U, dt)
+        y, _ = O.reinit_eno2(ob, x, eps_l, 1e-9, dtau, nit)
+        cur = O.bcond(ob, y)
+    return cur
+
+
+@pytest.mark.gpu
+def test_heaviside_loss_gradients_match_fd_of_oracle():
+    """The differentiation workload in small: 4 full steps (RK3 + 4 Sussman
+    iterations) with a checkpoint every 2 steps, loss J = sum_cells V (H(phi_T)
+    - H(phi_target))^2 with V the cell volume 1/aj, gradients with respect to
+    phi0 and to the 16 Fourier amplitudes of the velocity, against central
+    differences (step 1e-6) of the oracle forward along seeded directions."""
+    from lsm_amd import autograd
+    g, phi, _, _, hmin, hmax = _step_case((16, 14, 13), 151)
+    api, blk, dev = _gpu_setup(g)
+    (u0, v0, w0), (ks, es, amps, psis) = inputs.fourier_velocity(g, seed=152, return_modes=True)
+    dt = inputs.cfl_dt(g, u0, v0, w0)
+    eps_l, dtau, nit, nsteps = 1.5 * hmax, 0.5 * hmin, 4, 4
+    heps = 1.5 * hmax
+    ob = O.Block(g)
+    target = _oracle_steps(ob, inputs.spheres_sdf(g, nspheres=3, seed=152, rmin=0.22, rmax=0.37),
+                           O.contra(ob, u0, v0, w0), dt, eps_l, dtau, nit, nsteps)
+    vol = 1.0 / g.aj
+
+    def H(x):
+        mid = 0.5 * (1.0 + x / heps + (1.0 / np.pi) * np.sin(np.pi * x / heps))
+        return np.where(x < -heps, 0.0, np.where(x > heps, 1.0, mid))
+
+    def J_oracle(phi0, a):
+        U = O.contra(ob, *inputs.fourier_field(g, ks, es, a, psis))
+        return float(np.sum(vol * (H(_oracle_steps(ob, phi0, U, dt, eps_l, dtau, nit, nsteps)) - H(target)) ** 2))
+
+    prm = api.ReinitParams(epslsm=eps_l, RNtimestep=dtau, RNitermax=nit, epsReinitialisation=1e-9)
+    p = dev(phi).requires_grad_(True)
+    a = dev(amps).requires_grad_(True)
+    Ud = autograd.contravariant(blk, *autograd.fourier_velocity(blk, ks, es, a, psis))
+    out = autograd.levelset_trajectory(blk, p, Ud, dt, prm, nsteps, checkpoint_every=2)
+    J = (dev(vol) * (autograd.heaviside(out, heps) - dev(H(target))) ** 2).sum()
+    assert abs(float(J) - J_oracle(phi, amps)) <= 1e-10 * abs(J_oracle(phi, amps))
+    J.backward()
+    gphi, gamp = p.grad.cpu().numpy(), a.grad.cpu().numpy()
+    rng = np.random.default_rng(153)
+    for _ in range(6):
+        e = rng.standard_normal(g.shape)
+        fd = _fd_smooth(lambda x: J_oracle(x, amps), phi, e)
+        an = float(np.sum(gphi * e))
+        assert abs(fd - an) <= 1e-6 * abs(an) + 1e-12, (fd, an)
+    for _ in range(4):
+        e = rng.standard_normal(amps.shape)
+        fd = _fd_smooth(lambda s: J_oracle(phi, amps + s * e), 0.0, 1.0)
+        an = float(np.sum(gamp * e))
+        assert abs(fd - an) <= 1e-6 * abs(an) + 1e-12, (fd, an)

@@ -98,3 +98,128 @@ def rk3_step(phi_n, U, aj, dt, dd, btype, big, mask=None):
         new = torch.where(upd, new, prev)
         prev = bcond(new, btype, big)
     return prev
+
+
+# ---------------------------------------------------------------------------
+# Sussman reinitialisation (reinitialisation_test2.F90:42-210 with
+# calc_NormGrad_d.F90:39-2446, AdvectionMethods.F90:64-302,576-587 and
+# global_lsm.F90:120-133,200-219), FP64, every node at once.
+# ---------------------------------------------------------------------------
+EPS_SIMS = 2.220446049250313e-16
+PI_REF = 3.141592653589793
+
+
+def _sign1(x):
+    return torch.copysign(torch.ones_like(x), x)
+
+
+def eno2(f0, fLL, fL, fC, fR, fRR, lo, hi):
+    a = (fR - 2.0 * fC) + fL
+    bp = (fRR - 2.0 * fR) + fC
+    dp = torch.where(hi & ~lo, (fR - fC) - 0.5 * a,
+                     (fR - fC) - 0.5 * torch.where(torch.abs(bp) < torch.abs(a), bp, a))
+    bm = (fC - 2.0 * fL) + fLL
+    dm = torch.where(lo, (fC - fL) + 0.5 * a,
+                     (fC - fL) + 0.5 * torch.where(torch.abs(bm) < torch.abs(a), bm, a))
+    s = torch.where(torch.abs(f0) > EPS_SIMS, _sign1(f0), torch.zeros_like(f0))
+    c1 = (s * (fR - fC) < 0.0) & (s * (fC - fL) < -(s * (fR - fC)))
+    c2 = (s * (fC - fL) > 0.0) & (s * (fR - fC) > -(s * (fC - fL)))
+    return torch.where(c1, dp, torch.where(c2, dm, 0.5 * (dp + dm)))
+
+
+def _eno2_boundary(f0, f1, f2, as_):
+    D1 = as_ * (f1 - f0)
+    D2 = as_ * (f2 - f1)
+    D3 = as_ * (f2 - f0) / 2.0
+    a1, a2, a3 = torch.abs(D1), torch.abs(D2), torch.abs(D3)
+    r = D1
+    r = torch.where((a1 >= a3) & (a2 >= a3), D3, r)
+    r = torch.where((a1 < a3) & (a2 < a3), ((3.0 * D1 + 6.0 * D2) - D3) / 7.0, r)
+    r = torch.where(a1 == a2, (4.0 * D1 + D2) / 5.0, r)
+    r = torch.where(a2 < a1, (D1 + 2.0 * D2) / 3.0, r)
+    r = torch.where(a1 < a2, (2.0 * D1 + D2) / 3.0, r)
+    return torch.where(torch.abs(D1 - D2) < EPS_SIMS, D1, r)
+
+
+def biased(f0, f1, f2, f3, order, as_, enobc):
+    if order == 1:
+        return as_ * (-(1.0 * f0) + 1.0 * f1)
+    if order == 2:
+        if enobc:
+            return _eno2_boundary(f0, f1, f2, as_)
+        return as_ * ((-(3.0 * f0) + 4.0 * f1) - 1.0 * f2) / 2.0
+    return as_ * (((-(11.0 * f0) + 18.0 * f1) - 9.0 * f2) + 2.0 * f3) / 6.0
+
+
+def normgrad(d, phi0, metrics, dd, order_bc=1, eno_bc=False):
+    """calc_NormGrad_d on every node; metrics = (csi, eta, zet), each three arrays."""
+    nk, nj, ni = d.shape
+    dev = d.device
+    ii = torch.arange(ni, device=dev).view(1, 1, -1).expand(d.shape)
+    jj = torch.arange(nj, device=dev).view(1, -1, 1).expand(d.shape)
+    kk = torch.arange(nk, device=dev).view(-1, 1, 1).expand(d.shape)
+    idx = (ii, jj, kk)
+    n = (ni, nj, nk)
+    on = [(idx[a] == 0) | (idx[a] == n[a] - 1) for a in range(3)]
+    stale = phi0[nk - 2, nj - 2, ni - 2]
+    p0 = torch.where(on[0] & ~on[1] & ~on[2], stale.expand(d.shape), phi0)
+    zero = torch.zeros_like(d)
+    gr = []
+    for a, axis in ((0, 2), (1, 1), (2, 0)):
+        sh = lambda o: _shift(d, axis, o)  # noqa: E731
+        lo, hi = idx[a] == 1, idx[a] == n[a] - 2
+        inner = dd[a] * eno2(p0, torch.where(lo, zero, sh(-2)), sh(-1), d, sh(1), torch.where(hi, zero, sh(2)),
+                             lo, hi)
+        f_lo = dd[a] * biased(d, sh(1), sh(2), sh(3), order_bc, 1.0, eno_bc)
+        f_hi = dd[a] * biased(d, sh(-1), sh(-2), sh(-3), order_bc, -1.0, eno_bc)
+        g = torch.where(idx[a] == 0, f_lo, torch.where(idx[a] == n[a] - 1, f_hi, inner))
+        if a == 0:
+            dc2 = 0.5 * dd[0]
+            edge = dc2 * ((-(1.0 * sh(2)) + 4.0 * sh(1)) - 3.0 * d)
+            g = torch.where((ii == 0) & (kk == 0) & ~on[1], edge, g)
+        gr.append(g)
+    csi, eta, zet = metrics
+    gx = (csi[0] * gr[0] + eta[0] * gr[1]) + zet[0] * gr[2]
+    gy = (csi[1] * gr[0] + eta[1] * gr[1]) + zet[1] * gr[2]
+    gz = (csi[2] * gr[0] + eta[2] * gr[1]) + zet[2] * gr[2]
+    return torch.sqrt((gx * gx + gy * gy) + gz * gz)
+
+
+def sign_phi0(p, eps):
+    return torch.where(torch.abs(p) > eps, _sign1(p), p / eps - (1.0 / PI_REF) * torch.sin((PI_REF * p) / eps))
+
+
+def deltaf(v, eps):
+    return torch.where(torch.abs(v) > eps, torch.zeros_like(v),
+                       (1.0 / (2.0 * eps)) * (1.0 + torch.cos((PI_REF * v) / eps)))
+
+
+def lambda_mass(p, s, g, eps):
+    dl = deltaf(p, eps)
+    num = -((dl * s) * (1.0 - g))
+    den = (dl * dl) * g
+    small = torch.abs(den) < EPS_SIMS
+    return torch.where(small, torch.zeros_like(den), num / torch.where(small, torch.ones_like(den), den))
+
+
+def reinit(phi0, metrics, dd, epslsm, eps_reinit, dtau, itermax, order_bc=1, eno_bc=False):
+    """reinitialisation_test2 (non-hybrid): returns (phi_out, iterations)."""
+    s = sign_phi0(phi0, epslsm)
+    out = phi0
+    ng = normgrad(out, phi0, metrics, dd, order_bc, eno_bc)
+    it = 0
+    while it < itermax:
+        mx, mn = float(ng.detach().max()), float(ng.detach().min())
+        if abs(mx - 1.0) < eps_reinit and abs(mn - 1.0) < eps_reinit:
+            break
+        dn = out
+        upd = (normgrad(dn, phi0, metrics, dd, order_bc, eno_bc) - 1.0 > eps_reinit).detach()
+        for c1, c2, c3 in C:
+            ng = normgrad(out, phi0, metrics, dd, order_bc, eno_bc)
+            rhs = s * (ng - 1.0)
+            lam = lambda_mass(out, s, ng, epslsm)
+            aux = deltaf(out, epslsm) * ng
+            rhs = rhs - lam * aux
+            out = torch.where(upd, (c1 * dn + c2 * out) - (c3 * dtau) * rhs, out)
+        it += 1
+    return out, it
