@@ -203,72 +203,87 @@ __device__ __forceinline__ double biased(double f0, double f1, double f2, double
 
 // ---- linear forms of the ENO2 / biased derivatives on the branch taken ----
 // For the adjoint: each derivative above is, once its comparisons are fixed,
-// a linear combination of the stencil values.  These return the coefficients
-// of that combination, evaluating every comparison with the same expressions
-// as the forward functions so both take the same branch.
+// a linear combination of the stencil values.  The *_branch functions return
+// a small code for the branch taken, evaluating every comparison with the
+// same expressions as the forward functions; the *_coef functions turn a code
+// into the coefficients of the combination.
 
-// coefficients of eno2() for (fLL, fL, fC, fR, fRR)
-__device__ __forceinline__ void eno2_coef(double f0, double fLL, double fL, double fC, double fR,
-                                          double fRR, bool left_out, bool right_out, double e[5])
+// branch of eno2(): bits 0-1 the result (0 dp, 1 dm, 2 their mean), bit 2 the
+// plus-side second difference (fC, fR, fRR) instead of (fL, fC, fR), bit 3 the
+// minus-side one (fLL, fL, fC) instead of (fL, fC, fR)
+__device__ __forceinline__ int eno2_branch(double f0, double fLL, double fL, double fC, double fR,
+                                           double fRR, bool left_out, bool right_out)
 {
     const double a = (fR - 2.0 * fC) + fL;
-    // second difference chosen for the plus side: 0 -> a (fL, fC, fR), 1 -> (fC, fR, fRR)
-    bool pb = false, mb = false;
-    if (!right_out || (left_out && right_out)) {
+    int br = 0;
+    if (!right_out || left_out) {
         const double b = (fRR - 2.0 * fR) + fC;
-        pb = fabs(b) < fabs(a);
+        if (fabs(b) < fabs(a)) br |= 4;
     }
     if (!left_out) {
         const double b = (fC - 2.0 * fL) + fLL;
-        mb = fabs(b) < fabs(a);
+        if (fabs(b) < fabs(a)) br |= 8;
     }
-    if (left_out && right_out) mb = false;
+    const double s = (fabs(f0) > kEpsSims) ? sign1(f0) : 0.0;
+    if (s * (fR - fC) < 0.0 && s * (fC - fL) < -(s * (fR - fC))) br |= 0;
+    else if (s * (fC - fL) > 0.0 && s * (fR - fC) > -(s * (fC - fL))) br |= 1;
+    else br |= 2;
+    return br;
+}
+
+// coefficients of eno2() on branch br for (fLL, fL, fC, fR, fRR)
+__device__ __forceinline__ void eno2_coef(int br, double e[5])
+{
     // dp = (fR - fC) - 0.5 Mf+,  dm = (fC - fL) + 0.5 Mf-
     double dp[5] = {0.0, 0.0, -1.0, 1.0, 0.0};
     double dm[5] = {0.0, -1.0, 1.0, 0.0, 0.0};
-    if (pb) { dp[2] -= 0.5; dp[3] += 1.0; dp[4] -= 0.5; }
+    if (br & 4) { dp[2] -= 0.5; dp[3] += 1.0; dp[4] -= 0.5; }
     else { dp[1] -= 0.5; dp[2] += 1.0; dp[3] -= 0.5; }
-    if (mb) { dm[0] += 0.5; dm[1] -= 1.0; dm[2] += 0.5; }
+    if (br & 8) { dm[0] += 0.5; dm[1] -= 1.0; dm[2] += 0.5; }
     else { dm[1] += 0.5; dm[2] -= 1.0; dm[3] += 0.5; }
-    const double s = (fabs(f0) > kEpsSims) ? sign1(f0) : 0.0;
-    int pick = 2;
-    if (s * (fR - fC) < 0.0 && s * (fC - fL) < -(s * (fR - fC))) pick = 0;
-    else if (s * (fC - fL) > 0.0 && s * (fR - fC) > -(s * (fC - fL))) pick = 1;
+    const int pick = br & 3;
 #pragma unroll
     for (int q = 0; q < 5; ++q) e[q] = pick == 0 ? dp[q] : (pick == 1 ? dm[q] : 0.5 * (dp[q] + dm[q]));
 }
 
-// coefficients of eno2_boundary() for (f0, f1, f2)
-__device__ __forceinline__ void eno2_boundary_coef(double f0, double f1, double f2, int dir, double e[3])
+// branch of eno2_boundary(), 0..6 in the order of its tests
+__device__ __forceinline__ int eno2_boundary_branch(double f0, double f1, double f2, int dir)
 {
     const double as = (double)dir;
     const double D1 = as * (f1 - f0);
     const double D2 = as * (f2 - f1);
     const double D3 = as * (f2 - f0) / 2.0;
-    // D1 = as (-1, 1, 0), D2 = as (0, -1, 1), D3 = as (-1/2, 0, 1/2)
-    double w1, w2, w3;
-    if (fabs(D1 - D2) < kEpsSims) { w1 = 1.0; w2 = 0.0; w3 = 0.0; }
-    else if (fabs(D1) < fabs(D2)) { w1 = 2.0 / 3.0; w2 = 1.0 / 3.0; w3 = 0.0; }
-    else if (fabs(D2) < fabs(D1)) { w1 = 1.0 / 3.0; w2 = 2.0 / 3.0; w3 = 0.0; }
-    else if (fabs(D1) == fabs(D2)) { w1 = 4.0 / 5.0; w2 = 1.0 / 5.0; w3 = 0.0; }
-    else if (fabs(D1) < fabs(D3) && fabs(D2) < fabs(D3)) { w1 = 3.0 / 7.0; w2 = 6.0 / 7.0; w3 = -1.0 / 7.0; }
-    else if (fabs(D1) >= fabs(D3) && fabs(D2) >= fabs(D3)) { w1 = 0.0; w2 = 0.0; w3 = 1.0; }
-    else { w1 = 1.0; w2 = 0.0; w3 = 0.0; }
-    e[0] = as * (-w1 - 0.5 * w3);
-    e[1] = as * (w1 - w2);
-    e[2] = as * (w2 + 0.5 * w3);
+    if (fabs(D1 - D2) < kEpsSims) return 0;
+    if (fabs(D1) < fabs(D2)) return 1;
+    if (fabs(D2) < fabs(D1)) return 2;
+    if (fabs(D1) == fabs(D2)) return 3;
+    if (fabs(D1) < fabs(D3) && fabs(D2) < fabs(D3)) return 4;
+    if (fabs(D1) >= fabs(D3) && fabs(D2) >= fabs(D3)) return 5;
+    return 6;
 }
 
-// coefficients of biased() for (f0, f1, f2, f3)
-__device__ __forceinline__ void biased_coef(double f0, double f1, double f2, int order, int dir,
-                                            bool enobc, double e[4])
+// coefficients of biased() for (f0, f1, f2, f3); br: eno2_boundary_branch
+// (used for order 2 with enobc only)
+__device__ __forceinline__ void biased_coef(int br, int order, int dir, bool enobc, double e[4])
 {
     const double as = (double)dir;
     e[0] = e[1] = e[2] = e[3] = 0.0;
     if (order == 1) { e[0] = -as; e[1] = as; }
     if (order == 2) {
-        if (enobc) eno2_boundary_coef(f0, f1, f2, dir, e);
-        else { e[0] = -1.5 * as; e[1] = 2.0 * as; e[2] = -0.5 * as; }
+        if (enobc) {
+            // D1 = as (-1, 1, 0), D2 = as (0, -1, 1), D3 = as (-1/2, 0, 1/2)
+            double w1 = 1.0, w2 = 0.0, w3 = 0.0;
+            if (br == 1) { w1 = 2.0 / 3.0; w2 = 1.0 / 3.0; }
+            else if (br == 2) { w1 = 1.0 / 3.0; w2 = 2.0 / 3.0; }
+            else if (br == 3) { w1 = 4.0 / 5.0; w2 = 1.0 / 5.0; }
+            else if (br == 4) { w1 = 3.0 / 7.0; w2 = 6.0 / 7.0; w3 = -1.0 / 7.0; }
+            else if (br == 5) { w1 = 0.0; w3 = 1.0; }
+            e[0] = as * (-w1 - 0.5 * w3);
+            e[1] = as * (w1 - w2);
+            e[2] = as * (w2 + 0.5 * w3);
+        } else {
+            e[0] = -1.5 * as; e[1] = 2.0 * as; e[2] = -0.5 * as;
+        }
     }
     if (order == 3) { e[0] = -11.0 / 6.0 * as; e[1] = 3.0 * as; e[2] = -1.5 * as; e[3] = 1.0 / 3.0 * as; }
 }
