@@ -249,6 +249,87 @@ int lsm_levelset_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, con
                               double *adj_ucn1, double *adj_ucn2, double *adj_ucn3, double *phi_np1,
                               void *work, size_t work_bytes, void *stream);
 
+/* ---- batched multi-block form -------------------------------------------
+ * Many blocks advanced together.  A connected batch is the block
+ * decomposition of one domain (init.F90:489-1068 held on one device): a
+ * lattice of blocks whose face sizes match along each lattice slab, `ghost`
+ * layers on every face of every block, a face physical exactly where the
+ * lattice ends.  An independent batch holds unrelated instances (no ghost
+ * layers, six physical faces each).  Every batch array (phi, ucn, metrics,
+ * aj) is one device allocation of `total` doubles holding each block's array
+ * (extents ni+2*ghost x nj+2*ghost x nk+2*ghost, i fastest, owned nodes at
+ * local indices ghost .. ghost+n-1) from its `offset`. */
+#define LSM_BATCH_QUEUE     0   /* stage tiles pulled from per-XCD work queues by a persistent grid */
+#define LSM_BATCH_GRID      1   /* stage tiles of all blocks, one workgroup each, one launch */
+#define LSM_BATCH_PERBLOCK  2   /* the single-block launches, block by block */
+
+typedef struct LsmBatchBlock {
+    int32_t ni, nj, nk;               /* owned nodes */
+    int32_t lattice[3];               /* position in the block lattice (connected batches) */
+    int32_t phys_faces;               /* LSM_FACE_* bits */
+    int32_t btype[6];                 /* btype_lsm(1:6) of the block's zone */
+    double big_phi;
+    double dc, de, dz;
+    int64_t offset;                   /* element offset of the block's array in the batch arrays */
+} LsmBatchBlock;
+
+typedef struct LsmBatchDesc {
+    int32_t nblocks;
+    int32_t ghost;                    /* ghost layers (0..3; >= 2 when connected) */
+    int32_t connected;                /* 1: one domain (halo fills), 0: independent instances */
+    int32_t lattice[3];               /* lattice extents (connected) */
+    const LsmBatchBlock *blocks;      /* host array of nblocks entries */
+    const double *aj;                 /* device batch array of aj */
+    int64_t total;                    /* doubles in every batch array */
+} LsmBatchDesc;
+
+typedef struct LsmBatch LsmBatch;
+
+/* Validates the layout and builds the device tables (tile lists, halo and
+ * shell enumerations); the batch keeps the aj pointer. */
+int lsm_batch_create(const LsmBatchDesc *desc, LsmBatch **out);
+int lsm_batch_destroy(LsmBatch *batch);
+/* Launch form (LSM_BATCH_*), tile rows (4 or 8), planes per tile and
+ * persistent workgroups per CU (0: as many as fit) of the stage kernel. */
+int lsm_batch_set_config(LsmBatch *batch, int form, int tile_rows, int kchunk, int wg_per_cu);
+int lsm_batch_get_config(const LsmBatch *batch, int32_t cfg[4]);
+size_t lsm_batch_workspace_bytes(const LsmBatch *batch, int op);
+/* rhs_exchng3_3d.F90:6-359 for every block of a connected batch: each ghost
+ * node inside the domain takes the value of the block owning that node. */
+int lsm_batch_halo_fill(LsmBatch *batch, double *field, void *stream);
+/* rhs_contra_j.F90:2-50 over every node of the batch arrays. */
+int lsm_batch_contra_velocity(LsmBatch *batch, const double *csi[3], const double *eta[3],
+                              const double *zet[3], const double *u, const double *v, const double *w,
+                              double *ucn1, double *ucn2, double *ucn3, void *stream);
+/* ad_RKTVD3.F90:1-88 for every block: three stages, each followed by
+ * bcond_lsm on the physical faces and the halo fill (the reference's
+ * rhs_exchng3_3d, ad_RKTVD3.F90:83-84).  phi_n's ghost layers must be
+ * filled.  Workspace: lsm_batch_workspace_bytes(batch, LSM_OP_RK3). */
+int lsm_batch_rk3_step(LsmBatch *batch, double *phi_np1, const double *phi_n, const double *ucn1,
+                       const double *ucn2, const double *ucn3, double delti_lsm, int mode,
+                       void *work, size_t work_bytes, void *stream);
+
+/* Autotuner (no counterpart in the reference): times the stage-kernel
+ * launch configurations (queue form: tile rows 4 / 8, 4 / 8 / 16 planes per
+ * tile, persistent workgroups per CU as many as fit or 4; grid form with the
+ * same tiles) over `reps` lsm_batch_rk3_step calls on the caller's fields,
+ * keeps the fastest and remembers it per shape signature (block extents,
+ * ghost layers, connectivity, arithmetic): a later call for a batch of the
+ * same shapes applies it without timing unless `force`.  best_ms (optional)
+ * receives the winner's milliseconds per step (-1 when taken from the
+ * cache).  Synchronises the stream.  phi_np1 is overwritten. */
+int lsm_batch_autotune(LsmBatch *batch, double *phi_np1, const double *phi_n, const double *ucn1,
+                       const double *ucn2, const double *ucn3, double delti_lsm, int mode, void *work,
+                       size_t work_bytes, int reps, int force, double *best_ms, void *stream);
+/* Records one lsm_batch_rk3_step (with the current configuration and these
+ * arguments, which the graph keeps) into a HIP graph owned by the batch;
+ * lsm_batch_graph_launch replays it on any stream.  Needs a workspace and
+ * launch timing switched off. */
+int lsm_batch_graph_capture(LsmBatch *batch, double *phi_np1, const double *phi_n, const double *ucn1,
+                            const double *ucn2, const double *ucn3, double delti_lsm, int mode, void *work,
+                            size_t work_bytes);
+int lsm_batch_graph_launch(LsmBatch *batch, void *stream);
+
 /* ---- host-memory adapter (for Fortran ISO_C_BINDING shims) -------------
  * Same operations on HOST arrays in the reference's layout: the adapter keeps
  * device mirrors of one block and copies arguments in and results out; every
@@ -301,7 +382,8 @@ int lsm_host_water_volume(const double *phi, double vol[3], int32_t *adv_mask);
 #define LSM_KT_REINIT_STAGE   2   /* fused ENO2 |grad d| + reinitialisation stage */
 #define LSM_KT_VOLUME         3   /* water volume cell kernel */
 #define LSM_KT_ADJOINT_STAGE  4   /* fused adjoint of one RK stage (partials + gather) */
-#define LSM_KT_COUNT          5
+#define LSM_KT_BATCH_STAGE    5   /* batched stage tiles of every block (one launch) */
+#define LSM_KT_COUNT          6
 
 /* Turns event recording on (1) or off (0); clears accumulated timings. */
 int lsm_timing_enable(int on);

@@ -15,287 +15,18 @@
 #include <stdio.h>
 
 #include "lsm_internal.hpp"
+#include "lsm_advect_tile.hpp"
 
 namespace lsm {
 
 namespace {
-
-constexpr int TX = 64;    // tile width in i (one wavefront per j row)
-constexpr int HALO = 2;
-
-struct Range {
-    int ilo, ihi, jlo, jhi, klo, khi;   // interior nodes (RHS nodes)
-    int ia, ib, ja, jb, ka, kb;         // owned nodes
-    bool fb, ff, fl, fr, fd, fu;        // physical faces
-};
-
-Range make_range(const Blk &b)
-{
-    Range r;
-    const int f = b.e.faces;
-    r.fb = f & LSM_FACE_I1; r.ff = f & LSM_FACE_IM;
-    r.fl = f & LSM_FACE_J1; r.fr = f & LSM_FACE_JM;
-    r.fd = f & LSM_FACE_K1; r.fu = f & LSM_FACE_KM;
-    r.ia = 0; r.ib = b.e.ni - 1;
-    r.ja = 0; r.jb = b.e.nj - 1;
-    r.ka = 0; r.kb = b.e.nk - 1;
-    r.ilo = r.ia + (r.fb ? 1 : 0); r.ihi = r.ib - (r.ff ? 1 : 0);
-    r.jlo = r.ja + (r.fl ? 1 : 0); r.jhi = r.jb - (r.fr ? 1 : 0);
-    r.klo = r.ka + (r.fd ? 1 : 0); r.khi = r.kb - (r.fu ? 1 : 0);
-    return r;
-}
-
-// workgroup -> tile, keeping consecutive tiles on one XCD: the dispatcher
-// deals workgroups round-robin over the 8 XCDs, so workgroup b runs on XCD
-// b % 8; giving each XCD a contiguous run of tiles keeps the j/k halo reuse
-// inside one L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-    if ((nb & 7) != 0) return b;
-    return (b & 7) * (nb >> 3) + (b >> 3);
-}
-
-struct StageArgs {
-    const double *ps, *pn;
-    double *out;
-    const double *U1, *U2, *U3, *aj;
-    const int32_t *mask;
-    int ni, nj, nk;
-    long sj, sk;
-    Range r;
-    double c1, c2, c3, dt, dc, de, dz;
-    int tiles_i, tiles_j, kc, ntiles;
-};
-
-// streamed operands are used once per stage: non-temporal loads/stores keep
-// them from evicting the phi planes whose halos neighbouring tiles re-read
-// (LSM_NT=0 turns this off)
-template <bool NT>
-__device__ __forceinline__ double ldnt_(const double *p)
-{
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-
-template <bool NT>
-__device__ __forceinline__ void stnt_(double *p, double v)
-{
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
-}
 
 template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false>
 __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
 {
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= a.ntiles) return;
-    const int ti = tile % a.tiles_i;
-    const int t2 = tile / a.tiles_i;
-    const int tj = t2 % a.tiles_j;
-    const int tk = t2 / a.tiles_j;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = ti * TX + tx, j = tj * TY + ty;
-    const int kb = a.r.klo + tk * a.kc;
-    const int ke = min(kb + a.kc - 1, a.r.khi);
-
-    __shared__ double sm[2][TY + 2 * HALO][TX + 2 * HALO];
-
-    // Loads are clamped into the array instead of guarded: a clamped value is
-    // only ever read by a thread that does not compute (outside the interior)
-    // or by a stencil slot the boundary forcing replaces with a dummy, and
-    // unguarded loads issue back to back without branches.
-    const int ic = min(i, a.ni - 1), jc = min(j, a.nj - 1);
-    const long col = (long)ic + a.sj * jc;
-    // waves 0,1 load the two halo rows below the tile, waves 2,3 the two above
-    const bool has_jh = ty < 4;
-    const int rr = (ty < 2) ? ty : ty - 2 + TY + HALO;
-    const int jh = min(max(tj * TY - HALO + rr, 0), a.nj - 1);
-    const long col_jh = (long)ic + a.sj * jh;
-    const int lin = ty * TX + tx;
-    const bool has_ih = lin < 4 * TY;                       // 16 i-halo cells
-    const int hr = (lin >> 2) % TY, hc4 = lin & 3;
-    const int hcol = (hc4 < 2) ? hc4 : hc4 + TX;
-    const int hi_i = min(max(ti * TX - HALO + hcol, 0), a.ni - 1);
-    const int hi_j = min(tj * TY + hr, a.nj - 1);
-    const long col_ih = (long)hi_i + a.sj * hi_j;
-    auto kcl = [&](int kk) -> long { return a.sk * (long)min(max(kk, 0), a.nk - 1); };
-
-    const bool act = i >= a.r.ilo && i <= a.r.ihi && j >= a.r.jlo && j <= a.r.jhi;
-    const bool fmb = a.r.fb && i == a.r.ilo, fmf = a.r.ff && i == a.r.ihi;
-    const bool fml = a.r.fl && j == a.r.jlo, fmr = a.r.fr && j == a.r.jhi;
-    const double *__restrict__ ps = a.ps;
-    auto ldnt = [](const double *p) { return ldnt_<NT>(p); };
-    auto stnt = [](double *p, double v) { stnt_<NT>(p, v); };
-
-    // register queue phi(k-2 .. k+1) of this column; next-plane operands
-    double q0 = ps[col + kcl(kb - 2)], q1 = ps[col + kcl(kb - 1)];
-    double q2 = ps[col + kcl(kb)], q3 = ps[col + kcl(kb + 1)];
-    double nq4 = ps[col + kcl(kb + 2)];
-    double njh = has_jh ? ps[col_jh + kcl(kb)] : 0.0;
-    double nih = has_ih ? ps[col_ih + kcl(kb)] : 0.0;
-    long l = col + a.sk * (long)kb;
-    double nu1 = ldnt(a.U1 + l), nu2 = ldnt(a.U2 + l), nu3 = ldnt(a.U3 + l),
-           naj = FOLD ? 0.0 : ldnt(a.aj + l), npn = ldnt(a.pn + l);
-    int nmk = MASK ? a.mask[l] : 1;
-
-    int buf = 0;
-    for (int k = kb; k <= ke; ++k) {
-        const double q4 = nq4;
-        const double u1 = nu1, u2 = nu2, u3 = nu3, ajv = naj, pnv = npn;
-        const int mk = nmk;
-        sm[buf][ty + HALO][tx + HALO] = q2;
-        if (has_jh) sm[buf][rr][tx + HALO] = njh;
-        if (has_ih) sm[buf][hr + HALO][hcol] = nih;
-        __syncthreads();
-        // operands of the next plane, in flight while this plane computes
-        if (k < ke) {
-            const long kn = kcl(k + 1);
-            nq4 = ps[col + kcl(k + 3)];
-            if (has_jh) njh = ps[col_jh + kn];
-            if (has_ih) nih = ps[col_ih + kn];
-            const long ln = col + kn;
-            nu1 = ldnt(a.U1 + ln); nu2 = ldnt(a.U2 + ln); nu3 = ldnt(a.U3 + ln);
-            if (!FOLD) naj = ldnt(a.aj + ln);
-            npn = ldnt(a.pn + ln);
-            if (MASK) nmk = a.mask[ln];
-        }
-        if (act) {
-            double v = q2;
-            if (mk > 0) {
-                const double (*p)[TX + 2 * HALO] = sm[buf];
-                const int cx = tx + HALO, cy = ty + HALO;
-                // folded operator: u1..u3 are aj*dt*d*ucn, so the metric scale is 1
-                const double sc = FOLD ? 1.0 : a.dc, se = FOLD ? 1.0 : a.de, sz = FOLD ? 1.0 : a.dz;
-                const double Lc = weno3_dir<FAST>(p[cy][cx - 2], p[cy][cx - 1], q2, p[cy][cx + 1],
-                                                  p[cy][cx + 2], u1, fmb, fmf, sc);
-                const double Le = weno3_dir<FAST>(p[cy - 2][cx], p[cy - 1][cx], q2, p[cy + 1][cx],
-                                                  p[cy + 2][cx], u2, fml, fmr, se);
-                const bool fmd = a.r.fd && k == a.r.klo, fmu = a.r.fu && k == a.r.khi;
-                const double Lz = weno3_dir<FAST>(q0, q1, q2, q3, q4, u3, fmd, fmu, sz);
-                const double rh = (Lc + Le) + Lz;
-                if (FOLD) v = (a.c1 * pnv + a.c2 * q2) - a.c3 * rh;
-                else v = (a.c1 * pnv + a.c2 * q2) - ((a.c3 * ajv) * a.dt) * rh;
-            }
-            stnt(a.out + col + a.sk * (long)k, v);
-        }
-        q0 = q1; q1 = q2; q2 = q3; q3 = q4;
-        buf ^= 1;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// boundary shell of one RK stage: RK combination with rh = 0 on physical
-// boundary nodes (ad_RKTVD3.F90:62-81) followed by bcond_lsm (ad_RKTVD3.F90:83)
-// evaluated per node.  Node values after the i faces, j faces and k faces
-// (f1, f2, f3) are expanded recursively from f0 = the stage result before
-// boundary conditions, so every thread only reads arrays no thread writes.
-// ---------------------------------------------------------------------------
-struct ShellArgs {
-    const double *out_ro;   // interior results of the stage (read)
-    double *out;            // the same array (boundary nodes written)
-    const double *ps, *pn, *aj;
-    const int32_t *mask;
-    int ni, nj, nk;
-    long sj, sk;
-    Range r;
-    int bt[6];
-    double big, c1, c2, c3, dt;
-    long nI, nJ, nK;        // shell nodes owned by the i, j and k faces
-    int cnt_i, cnt_j;       // owned i / j nodes not on a physical i / j face
-};
-
-template <bool MASK>
-__device__ __forceinline__ double sh_f0(const ShellArgs &a, int i, int j, int k)
-{
-    const long l = (long)i + a.sj * j + a.sk * k;
-    if (i >= a.r.ilo && i <= a.r.ihi && j >= a.r.jlo && j <= a.r.jhi && k >= a.r.klo && k <= a.r.khi)
-        return a.out_ro[l];
-    if (MASK && a.mask[l] <= 0) return a.ps[l];
-    return (a.c1 * a.pn[l] + a.c2 * a.ps[l]) - ((a.c3 * a.aj[l]) * a.dt) * 0.0;
-}
-
-template <bool MASK>
-__device__ __forceinline__ double sh_f1(const ShellArgs &a, int i, int j, int k)
-{
-    if (a.r.fb && i == a.r.ia) {
-        const int bt = a.bt[0];
-        if (bt == 2) return sh_f0<MASK>(a, i, j, k);
-        return bc_face(bt, sh_f0<MASK>(a, i + 1, j, k), sh_f0<MASK>(a, i + 2, j, k), 0.0, a.big, false);
-    }
-    if (a.r.ff && i == a.r.ib) {
-        return bc_face(a.bt[1], sh_f0<MASK>(a, i - 1, j, k), sh_f0<MASK>(a, i - 2, j, k), 0.0, a.big, true);
-    }
-    return sh_f0<MASK>(a, i, j, k);
-}
-
-template <bool MASK>
-__device__ __forceinline__ double sh_f2(const ShellArgs &a, int i, int j, int k)
-{
-    if (a.r.fl && j == a.r.ja) {
-        const int bt = a.bt[2];
-        if (bt == 2) return sh_f1<MASK>(a, i, j, k);
-        return bc_face(bt, sh_f1<MASK>(a, i, j + 1, k), sh_f1<MASK>(a, i, j + 2, k), 0.0, a.big, false);
-    }
-    if (a.r.fr && j == a.r.jb) {
-        const int bt = a.bt[3];
-        if (bt == 2) return sh_f1<MASK>(a, i, j, k);
-        return bc_face(bt, sh_f1<MASK>(a, i, j - 1, k), sh_f1<MASK>(a, i, j - 2, k), 0.0, a.big, false);
-    }
-    return sh_f1<MASK>(a, i, j, k);
-}
-
-template <bool MASK>
-__device__ __forceinline__ double sh_f3(const ShellArgs &a, int i, int j, int k)
-{
-    if (a.r.fd && k == a.r.ka) {
-        const int bt = a.bt[4];
-        if (bt == 2) return sh_f2<MASK>(a, i, j, k);
-        return bc_face(bt, sh_f2<MASK>(a, i, j, k + 1), sh_f2<MASK>(a, i, j, k + 2), 0.0, a.big, false);
-    }
-    if (a.r.fu && k == a.r.kb) {
-        const int bt = a.bt[5];
-        if (bt == 2) return sh_f2<MASK>(a, i, j, k);
-        return bc_face(bt, sh_f2<MASK>(a, i, j, k - 1), sh_f2<MASK>(a, i, j, k - 2), 0.0, a.big, false);
-    }
-    return sh_f2<MASK>(a, i, j, k);
-}
-
-// decode shell node t: i faces own all their (j,k); j faces own the i not on
-// a physical i face; k faces own the (i,j) not on a physical i or j face
-__device__ __forceinline__ bool shell_node(const ShellArgs &a, long t, int &i, int &j, int &k)
-{
-    const int ilo2 = a.r.ia + (a.r.fb ? 1 : 0);
-    const int jlo2 = a.r.ja + (a.r.fl ? 1 : 0);
-    if (t < a.nI) {
-        const long per = (long)a.nj * a.nk;
-        const int f = (int)(t / per);
-        const long rem = t % per;
-        j = (int)(rem % a.nj);
-        k = (int)(rem / a.nj);
-        i = (f == 0 && a.r.fb) ? a.r.ia : a.r.ib;
-        return true;
-    }
-    t -= a.nI;
-    if (t < a.nJ) {
-        const long per = (long)a.cnt_i * a.nk;
-        const int f = (int)(t / per);
-        const long rem = t % per;
-        i = ilo2 + (int)(rem % a.cnt_i);
-        k = (int)(rem / a.cnt_i);
-        j = (f == 0 && a.r.fl) ? a.r.ja : a.r.jb;
-        return true;
-    }
-    t -= a.nJ;
-    if (t < a.nK) {
-        const long per = (long)a.cnt_i * a.cnt_j;
-        const int f = (int)(t / per);
-        const long rem = t % per;
-        i = ilo2 + (int)(rem % a.cnt_i);
-        j = jlo2 + (int)(rem / a.cnt_i);
-        k = (f == 0 && a.r.fd) ? a.r.ka : a.r.kb;
-        return true;
-    }
-    return false;
+    advect_tile<FAST, MASK, TY, NT, FOLD>(a, tile);
 }
 
 template <bool MASK>
@@ -440,6 +171,30 @@ inline unsigned blocks_for(long n, int per) { return (unsigned)((n + per - 1) / 
 
 }  // namespace
 
+// one block's stage tiles and shell with the single-block kernels (the
+// per-block form of the batched step)
+int launch_stage_block(const StageArgs &a, int ty, bool fast, hipStream_t s)
+{
+    const unsigned grid = (unsigned)((a.ntiles + 7) / 8 * 8);
+    const dim3 blk(TX, ty, 1);
+    if (ty == 8) {
+        if (fast) advect_stage<true, false, 8><<<grid, blk, 0, s>>>(a);
+        else advect_stage<false, false, 8><<<grid, blk, 0, s>>>(a);
+    } else {
+        if (fast) advect_stage<true, false, 4><<<grid, blk, 0, s>>>(a);
+        else advect_stage<false, false, 4><<<grid, blk, 0, s>>>(a);
+    }
+    return check_hip(hipGetLastError(), "advect_stage");
+}
+
+int launch_shell_block(const ShellArgs &sh, hipStream_t s)
+{
+    const long n = sh.nI + sh.nJ + sh.nK;
+    if (n <= 0) return LSM_OK;
+    stage_shell<false><<<blocks_for(n, 256), 256, 0, s>>>(sh);
+    return check_hip(hipGetLastError(), "stage_shell");
+}
+
 int launch_copy(const double *src, double *dst, long n, hipStream_t s)
 {
     if (n <= 0) return LSM_OK;
@@ -531,20 +286,13 @@ static int rk3_run(const Blk &b, const double *phi_n, const int32_t *mask, const
     sh.ni = b.e.ni; sh.nj = b.e.nj; sh.nk = b.e.nk; sh.sj = b.e.sj; sh.sk = b.e.sk; sh.r = r;
     for (int q = 0; q < 6; ++q) sh.bt[q] = b.btype[q];
     sh.big = b.big_phi; sh.aj = b.aj; sh.mask = mask; sh.dt = dt;
-    const int nfi = (r.fb ? 1 : 0) + (r.ff ? 1 : 0);
-    const int nfj = (r.fl ? 1 : 0) + (r.fr ? 1 : 0);
-    const int nfk = (r.fd ? 1 : 0) + (r.fu ? 1 : 0);
-    sh.cnt_i = b.e.ni - nfi;
-    sh.cnt_j = b.e.nj - nfj;
-    sh.nI = (long)nfi * b.e.nj * b.e.nk;
-    sh.nJ = (long)nfj * sh.cnt_i * b.e.nk;
-    sh.nK = (long)nfk * sh.cnt_i * sh.cnt_j;
+    shell_counts(r, sh);
     const long nshell = sh.nI + sh.nJ + sh.nK;
 
     for (int st = 0; st < 3; ++st) {
         StageArgs a{src[st], phi_n, dst[st], U1, U2, U3, b.aj, mask, b.e.ni, b.e.nj, b.e.nk,
                     b.e.sj, b.e.sk, r, C[st][0], C[st][1], C[st][2], dt, b.dc, b.de, b.dz,
-                    tiles_i, tiles_j, kc, ntiles};
+                    tiles_i, tiles_j, kc, ntiles, 0};
         if (ntiles > 0) {
             KTimer kt(LSM_KT_ADVECT_STAGE, s);
             dim3 blk(TX, TY, 1);

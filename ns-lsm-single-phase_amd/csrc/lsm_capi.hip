@@ -115,6 +115,8 @@ static std::vector<TPair> g_t_pending;
 static double g_t_ms[LSM_KT_COUNT];
 static long long g_t_n[LSM_KT_COUNT];
 
+bool timing_on() { return g_t_on; }
+
 KTimer::KTimer(int kid, hipStream_t st) : id(kid), s(st)
 {
     if (!g_t_on) return;

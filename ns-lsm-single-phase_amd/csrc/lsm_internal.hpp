@@ -54,6 +54,9 @@ struct KTimer {
     ~KTimer();
 };
 
+// whether lsm_timing_enable(1) is in force (no event recording may enter a graph capture)
+bool timing_on();
+
 // ---- launchers (lsm_advect.hip) ----
 int launch_contra(const Blk &b, const double *u, const double *v, const double *w, long stride,
                   double *U1, double *U2, double *U3, hipStream_t s);

@@ -39,8 +39,35 @@ class LsmReinitParams(C.Structure):
                 ("m_sphi0", C.c_int32), ("m_ti", C.c_int32)]
 
 
+class LsmBatchBlock(C.Structure):
+    _fields_ = [("ni", C.c_int32), ("nj", C.c_int32), ("nk", C.c_int32), ("lattice", C.c_int32 * 3),
+                ("phys_faces", C.c_int32), ("btype", C.c_int32 * 6), ("big_phi", C.c_double),
+                ("dc", C.c_double), ("de", C.c_double), ("dz", C.c_double), ("offset", C.c_int64)]
+
+
+class LsmBatchDesc(C.Structure):
+    _fields_ = [("nblocks", C.c_int32), ("ghost", C.c_int32), ("connected", C.c_int32),
+                ("lattice", C.c_int32 * 3), ("blocks", C.POINTER(LsmBatchBlock)), ("aj", _vp),
+                ("total", C.c_int64)]
+
+
+LSM_BATCH_QUEUE, LSM_BATCH_GRID, LSM_BATCH_PERBLOCK = 0, 1, 2
+_pp = C.POINTER(_vp)
+
 # name -> (restype, argtypes)
 _SIGS = {
+    "lsm_batch_create": (C.c_int, [C.POINTER(LsmBatchDesc), C.POINTER(_vp)]),
+    "lsm_batch_destroy": (C.c_int, [_vp]),
+    "lsm_batch_set_config": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "lsm_batch_get_config": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "lsm_batch_workspace_bytes": (C.c_size_t, [_vp, C.c_int]),
+    "lsm_batch_halo_fill": (C.c_int, [_vp, _vp, _vp]),
+    "lsm_batch_contra_velocity": (C.c_int, [_vp, _pp, _pp, _pp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lsm_batch_rk3_step": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, C.c_size_t, _vp]),
+    "lsm_batch_autotune": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, C.c_size_t, C.c_int,
+                                     C.c_int, C.POINTER(C.c_double), _vp]),
+    "lsm_batch_graph_capture": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, C.c_size_t]),
+    "lsm_batch_graph_launch": (C.c_int, [_vp, _vp]),
     "lsm_version": (C.c_char_p, []),
     "lsm_last_error": (C.c_char_p, []),
     "lsm_workspace_bytes": (C.c_size_t, [C.POINTER(LsmBlock), C.c_int]),
@@ -95,6 +122,7 @@ _SIGS = {
 EXPORTS = tuple(_SIGS)
 LSM_KT_ADVECT_STAGE, LSM_KT_ADVECT_SHELL, LSM_KT_REINIT_STAGE, LSM_KT_VOLUME = 0, 1, 2, 3
 LSM_KT_ADJOINT_STAGE = 4
+LSM_KT_BATCH_STAGE = 5
 
 _lib = None
 

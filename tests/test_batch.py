@@ -1,0 +1,223 @@
+"""Batched multi-block advection (lsm_batch_*).
+
+Parity rests on a property of the reference's block decomposition: with the
+halo exchanged after every RK stage (ad_RKTVD3.F90:83-84) and the forced
+WENO3 stencils and bcond_lsm applied only at physical faces, every owned node
+of a block sees exactly the values the single-block step sees at the same
+global node, in the same operation order.  So a connected batch must
+reproduce, bit for bit, the single-block library step (itself bit-identical to
+the oracle) on the assembled grid, in every launch form and tile shape; an
+independent batch must reproduce each instance's own single-block step.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from lsm_amd import _lib, batch as B, inputs  # noqa: E402
+
+
+def _case(lattice=(3, 2, 2), nmin=6, nmax=14, ghost=3, seed=11, btype=(1, 1, 1, 1, 1, 1)):
+    lay, grids, phi, vel, amps = B.multiblock_case(lattice=lattice, nmin=nmin, nmax=nmax, ghost=ghost, seed=seed,
+                                                   nspheres=6, btype=btype)
+    return lay, grids, phi, vel, amps
+
+
+def test_multiblock_grid_is_one_grid():
+    """Per-block coordinates and metrics at owned nodes equal those of the
+    assembled single-block grid (same mapping, differences inside the domain)."""
+    lay, grids, phi, vel, amps = _case()
+    N = [sum(s) for s in lay.sizes]
+    x, y, z = B.block_coordinates(lay, amps, [0, 0, 0], N)
+    G = inputs.compute_metrics(inputs.Grid(x, y, z))
+    assert np.array_equal(B.assemble(lay, [g.x for g in grids]), x)
+    assert np.array_equal(B.assemble(lay, [g.aj for g in grids]), G.aj)
+    for c in range(3):
+        for name in ("csi", "eta", "zet"):
+            assert np.array_equal(B.assemble(lay, [getattr(g, name)[c] for g in grids]), getattr(G, name)[c])
+    # per-block arrays carry the ghost layers, offsets are disjoint and aligned
+    for b in range(len(lay.blocks)):
+        assert lay.offsets[b] % 32 == 0
+        assert grids[b].x.shape == lay.shape(b)
+    assert lay.offsets[-1] + int(np.prod(lay.shape(len(lay.blocks) - 1))) <= lay.total
+
+
+def _desc(lay, blocks=None, ghost=None, connected=None, lattice=None):
+    arr = (_lib.LsmBatchBlock * len(lay.blocks))()
+    for b, s in enumerate(lay.blocks):
+        e = arr[b]
+        e.ni, e.nj, e.nk = s.ni, s.nj, s.nk
+        e.lattice[:] = list(s.lattice)
+        e.phys_faces = s.phys_faces
+        e.btype[:] = list(s.btype)
+        e.big_phi = 5.0
+        e.dc = e.de = e.dz = 1.0
+        e.offset = lay.offsets[b]
+    if blocks:
+        blocks(arr)
+    d = _lib.LsmBatchDesc()
+    d.nblocks = len(lay.blocks)
+    d.ghost = lay.ghost if ghost is None else ghost
+    d.connected = (1 if lay.connected else 0) if connected is None else connected
+    d.lattice[:] = list(lay.lattice if lattice is None else lattice)
+    d.blocks = C.cast(arr, C.POINTER(_lib.LsmBatchBlock))
+    d.aj = 0x1000   # never dereferenced by the validation
+    d.total = lay.total
+    return d, arr
+
+
+@pytest.mark.parametrize("what", ["faces", "sizes", "lattice", "ghost", "overlap", "independent_ghost"])
+def test_batch_create_rejects_bad_layouts(what):
+    """Descriptor validation (host logic, before any device call)."""
+    L = _lib.load()
+    lay = B.lattice_layout(((6, 7), (5, 6), (8, 5)), ghost=3)
+    kw = {}
+    if what == "faces":
+        kw["blocks"] = lambda a: setattr(a[0], "phys_faces", 0x3F)
+    elif what == "sizes":
+        kw["blocks"] = lambda a: setattr(a[1], "nj", 9)
+    elif what == "lattice":
+        kw["lattice"] = (2, 2, 3)
+    elif what == "ghost":
+        kw["ghost"] = 1
+    elif what == "overlap":
+        kw["blocks"] = lambda a: setattr(a[3], "offset", lay.total - 10)
+    elif what == "independent_ghost":
+        kw["connected"] = 0
+    d, keep = _desc(lay, **kw)
+    h = C.c_void_p()
+    rc = L.lsm_batch_create(C.byref(d), C.byref(h))
+    assert rc == -1, (rc, L.lsm_last_error())
+    assert not h.value
+
+
+def _dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ghost", [2, 3])
+def test_halo_fill_gives_owner_values(ghost):
+    """Every ghost node inside the domain receives the value its owner block
+    holds for that global node (edges and corners included); ghost nodes
+    outside the domain are left alone."""
+    lay, grids, _, _, _ = _case(lattice=(3, 3, 2), ghost=ghost, seed=12)
+    bt = B.Batch(lay, grids)
+    N = [sum(s) for s in lay.sizes]
+    fields = []
+    for b, s in enumerate(lay.blocks):
+        shp = lay.shape(b)
+        f = np.full(shp, -7.0)
+        Gk, Gj, Gi = np.meshgrid(*[np.arange(shp[a]) - ghost + s.gstart[2 - a] for a in range(3)], indexing="ij")
+        code = Gi + 1000.0 * Gj + 1e6 * Gk
+        f[ghost:ghost + s.nk, ghost:ghost + s.nj, ghost:ghost + s.ni] = code[ghost:ghost + s.nk,
+                                                                          ghost:ghost + s.nj, ghost:ghost + s.ni]
+        fields.append((f, code, (Gi >= 0) & (Gi < N[0]) & (Gj >= 0) & (Gj < N[1]) & (Gk >= 0) & (Gk < N[2])))
+    t = bt.pack([f for f, _, _ in fields])
+    bt.halo_fill(t)
+    got = bt.unpack(t)
+    for (f, code, inside), out in zip(fields, got):
+        assert np.array_equal(out[inside], code[inside])
+        assert np.all(out[~inside] == -7.0)
+
+
+def _single_block_steps(lay, grids, phi, vel, dt, nsteps, mode, btype):
+    """The single-block library step on the assembled grid."""
+    from lsm_amd import api
+    N = [sum(s) for s in lay.sizes]
+    amps = None  # noqa: F841
+    g = inputs.Grid(*[B.assemble(lay, [getattr(gd, c) for gd in grids]) for c in "xyz"])
+    g.aj = B.assemble(lay, [gd.aj for gd in grids])
+    for name in ("csi", "eta", "zet"):
+        setattr(g, name, np.stack([B.assemble(lay, [getattr(gd, name)[c] for gd in grids]) for c in range(3)]))
+    blk = api.Block(g, btype=btype)
+    ucn = api.rhs_contra_j(blk, *[_dev(B.assemble(lay, [v[c] for v in vel])) for c in range(3)])
+    cur = _dev(B.assemble(lay, phi))
+    for _ in range(nsteps):
+        cur = api.ad_RKTVD3(blk, cur, ucn, dt, mode=mode)
+    assert tuple(cur.shape) == (N[2], N[1], N[0])
+    return cur.cpu().numpy()
+
+
+CONFIGS = [("queue", 4, 8, 0), ("queue", 8, 4, 2), ("grid", 4, 16, 0), ("grid", 8, 8, 0), ("perblock", 4, 8, 0)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("btype", [(1, 1, 1, 1, 1, 1), (0, 2, 1, 2, 1, 0)])
+def test_connected_batch_matches_single_block(mode, btype):
+    lay, grids, phi, vel, _ = _case(btype=btype)
+    dt = B.cfl_dt(lay, grids, vel)
+    ref = _single_block_steps(lay, grids, phi, vel, dt, 3, mode, btype)
+    bt = B.Batch(lay, grids)
+    ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vel]) for c in range(3)])
+    for form, ty, kc, wg in CONFIGS:
+        bt.set_config(form, ty, kc, wg)
+        assert bt.get_config() == {"form": form, "tile_rows": ty, "kchunk": kc, "wg_per_cu": wg}
+        cur = bt.halo_fill(bt.pack(phi))
+        nxt = bt.empty()
+        for _ in range(3):
+            bt.rk3_step(cur, ucn, dt, out=nxt, mode=mode)
+            cur, nxt = nxt, cur
+        got = B.assemble(lay, bt.unpack(cur))
+        assert np.array_equal(got, ref), (form, ty, kc, float(np.max(np.abs(got - ref))))
+
+
+@pytest.mark.gpu
+def test_independent_batch_matches_each_instance():
+    from lsm_amd import api
+    shapes = [(12, 9, 10), (17, 6, 8), (9, 14, 7), (33, 5, 6), (8, 8, 21)]
+    lay = B.independent_layout(shapes)
+    grids, phis, vels = [], [], []
+    for q, (ni, nj, nk) in enumerate(shapes):
+        gd = inputs.sinusoidal_cube_grid(ni, amp=0.03, seed=40 + q, nj=nj, nk=nk)
+        grids.append(gd)
+        phis.append(inputs.spheres_sdf(gd, nspheres=3, seed=50 + q, rmin=0.2, rmax=0.3))
+        vels.append(inputs.fourier_velocity(gd, seed=60 + q))
+    dt = min(inputs.cfl_dt(gd, *v) for gd, v in zip(grids, vels))
+    bt = B.Batch(lay, grids)
+    ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vels]) for c in range(3)])
+    refs = []
+    for gd, p, v in zip(grids, phis, vels):
+        blk = api.Block(gd)
+        u = api.rhs_contra_j(blk, *[_dev(a) for a in v])
+        refs.append(api.ad_RKTVD3(blk, api.ad_RKTVD3(blk, _dev(p), u, dt), u, dt).cpu().numpy())
+    for form, ty, kc, wg in CONFIGS:
+        bt.set_config(form, ty, kc, wg)
+        a = bt.rk3_step(bt.pack(phis), ucn, dt)
+        out = bt.unpack(bt.rk3_step(a, ucn, dt))
+        for r, o in zip(refs, out):
+            assert np.array_equal(o, r), form
+
+
+@pytest.mark.gpu
+def test_graph_replays_and_autotune():
+    """A captured step replayed several times, with eager work and
+    synchronisations between the replays, gives the eager step every time;
+    the autotuner leaves a configuration that still reproduces it, and a
+    second batch of the same shapes takes that configuration from the cache."""
+    lay, grids, phi, vel, _ = _case(lattice=(2, 3, 2), seed=13)
+    dt = B.cfl_dt(lay, grids, vel)
+    bt = B.Batch(lay, grids)
+    ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vel]) for c in range(3)])
+    p0 = bt.halo_fill(bt.pack(phi))
+    own = lambda t: B.assemble(lay, bt.unpack(t))  # noqa: E731  (ghosts beyond physical faces are not written)
+    ref = own(bt.rk3_step(p0, ucn, dt))
+    cfg, ms = bt.autotune(p0, ucn, dt, reps=2, force=True)
+    assert ms is not None and ms > 0
+    assert np.array_equal(own(bt.rk3_step(p0, ucn, dt)), ref), cfg
+    out = bt.empty()
+    work = bt.workspace()
+    bt.graph_capture(out, p0, ucn, dt, work=work)
+    for r in range(4):
+        out.fill_(-1.0)
+        bt.graph_launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(own(out), ref), r
+        bt.rk3_step(p0, ucn, dt)   # eager work between replays
+        torch.cuda.synchronize()
+    bt2 = B.Batch(lay, grids)
+    cfg2, ms2 = bt2.autotune(p0, ucn, dt)
+    assert cfg2 == cfg and ms2 is None
