@@ -289,10 +289,12 @@ typedef struct LsmBatch LsmBatch;
  * shell enumerations); the batch keeps the aj pointer. */
 int lsm_batch_create(const LsmBatchDesc *desc, LsmBatch **out);
 int lsm_batch_destroy(LsmBatch *batch);
-/* Launch form (LSM_BATCH_*), tile rows (4 or 8), planes per tile and
- * persistent workgroups per CU (0: as many as fit) of the stage kernel. */
-int lsm_batch_set_config(LsmBatch *batch, int form, int tile_rows, int kchunk, int wg_per_cu);
-int lsm_batch_get_config(const LsmBatch *batch, int32_t cfg[4]);
+/* Launch form (LSM_BATCH_*), tile columns (64: a wavefront per row, or 32:
+ * two half rows per wavefront; the per-block form takes 64), tile rows (4 or
+ * 8), planes per tile and persistent workgroups per CU (0: as many as fit) of
+ * the stage kernel.  cfg = {form, cols, rows, kchunk, wg_per_cu}. */
+int lsm_batch_set_config(LsmBatch *batch, int form, int tile_cols, int tile_rows, int kchunk, int wg_per_cu);
+int lsm_batch_get_config(const LsmBatch *batch, int32_t cfg[5]);
 size_t lsm_batch_workspace_bytes(const LsmBatch *batch, int op);
 /* rhs_exchng3_3d.F90:6-359 for every block of a connected batch: each ghost
  * node inside the domain takes the value of the block owning that node. */
@@ -310,9 +312,9 @@ int lsm_batch_rk3_step(LsmBatch *batch, double *phi_np1, const double *phi_n, co
                        void *work, size_t work_bytes, void *stream);
 
 /* Autotuner (no counterpart in the reference): times the stage-kernel
- * launch configurations (queue form: tile rows 4 / 8, 4 / 8 / 16 planes per
- * tile, persistent workgroups per CU as many as fit or 4; grid form with the
- * same tiles) over `reps` lsm_batch_rk3_step calls on the caller's fields,
+ * launch configurations (queue form: tile columns 64 / 32, rows 4 / 8, 4 / 8 /
+ * 16 planes per tile, persistent workgroups per CU as many as fit or 4; grid
+ * form with the same tiles) over `reps` lsm_batch_rk3_step calls on the caller's fields,
  * keeps the fastest and remembers it per shape signature (block extents,
  * ghost layers, connectivity, arithmetic): a later call for a batch of the
  * same shapes applies it without timing unless `force`.  best_ms (optional)

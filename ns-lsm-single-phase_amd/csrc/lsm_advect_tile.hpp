@@ -76,21 +76,25 @@ __device__ __forceinline__ void stnt_(double *p, double v)
     else *p = v;
 }
 
-// one tile of the stage: 64 (i) x TY (j) columns of interior nodes marching
-// up a chunk of kc planes
-template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false>
+// one tile of the stage: W (i) x TY (j) columns of interior nodes marching
+// up a chunk of kc planes.  W = 64: one wavefront per j row (blockDim 64 x
+// TY); W = 32: each wavefront covers two half rows (blockDim 64 x TY/2), for
+// blocks whose rows are not much longer than a multiple of 32
+template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false, int W = TX>
 __device__ __forceinline__ void advect_tile(const StageArgs &a, int tile)
 {
     const int ti = tile % a.tiles_i;
     const int t2 = tile / a.tiles_i;
     const int tj = t2 % a.tiles_j;
     const int tk = t2 / a.tiles_j;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = ti * TX + tx, j = a.jb0 + tj * TY + ty;
+    int tx, ty;
+    if constexpr (W == TX) { tx = threadIdx.x; ty = threadIdx.y; }
+    else { tx = threadIdx.x & (W - 1); ty = threadIdx.y * (TX / W) + (threadIdx.x / W); }
+    const int i = ti * W + tx, j = a.jb0 + tj * TY + ty;
     const int kb = a.r.klo + tk * a.kc;
     const int ke = min(kb + a.kc - 1, a.r.khi);
 
-    __shared__ double sm[2][TY + 2 * HALO][TX + 2 * HALO];
+    __shared__ double sm[2][TY + 2 * HALO][W + 2 * HALO];
 
     // Loads are clamped into the array instead of guarded: a clamped value is
     // only ever read by a thread that does not compute (outside the interior)
@@ -103,11 +107,11 @@ __device__ __forceinline__ void advect_tile(const StageArgs &a, int tile)
     const int rr = (ty < 2) ? ty : ty - 2 + TY + HALO;
     const int jh = min(max(a.jb0 + tj * TY - HALO + rr, 0), a.nj - 1);
     const long col_jh = (long)ic + a.sj * jh;
-    const int lin = ty * TX + tx;
+    const int lin = ty * W + tx;
     const bool has_ih = lin < 4 * TY;                       // 16 i-halo cells
     const int hr = (lin >> 2) % TY, hc4 = lin & 3;
-    const int hcol = (hc4 < 2) ? hc4 : hc4 + TX;
-    const int hi_i = min(max(ti * TX - HALO + hcol, 0), a.ni - 1);
+    const int hcol = (hc4 < 2) ? hc4 : hc4 + W;
+    const int hi_i = min(max(ti * W - HALO + hcol, 0), a.ni - 1);
     const int hi_j = min(a.jb0 + tj * TY + hr, a.nj - 1);
     const long col_ih = (long)hi_i + a.sj * hi_j;
     auto kcl = [&](int kk) -> long { return a.sk * (long)min(max(kk, 0), a.nk - 1); };
@@ -154,7 +158,7 @@ __device__ __forceinline__ void advect_tile(const StageArgs &a, int tile)
         if (act) {
             double v = q2;
             if (mk > 0) {
-                const double (*p)[TX + 2 * HALO] = sm[buf];
+                const double (*p)[W + 2 * HALO] = sm[buf];
                 const int cx = tx + HALO, cy = ty + HALO;
                 // folded operator: u1..u3 are aj*dt*d*ucn, so the metric scale is 1
                 const double sc = FOLD ? 1.0 : a.dc, se = FOLD ? 1.0 : a.de, sz = FOLD ? 1.0 : a.dz;

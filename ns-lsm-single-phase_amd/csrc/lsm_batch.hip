@@ -49,7 +49,6 @@ struct BBlock {
     long shell0;              // first shell node of the block in the batch enumeration
     long nI, nJ, nK;
     int cnt_i, cnt_j;
-    long ghost0, nghost;      // ghost nodes of internal faces (halo fill)
 };
 
 struct BFields {
@@ -80,8 +79,8 @@ struct QueueArgs {
 };
 
 // persistent: each workgroup pops tiles until every queue is empty
-template <bool FAST, int TY>
-__global__ __launch_bounds__(TX * TY) void batch_stage_queue(QueueArgs a)
+template <bool FAST, int TY, int W>
+__global__ __launch_bounds__(TY * W) void batch_stage_queue(QueueArgs a)
 {
     __shared__ int s_tile;
     const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
@@ -105,19 +104,19 @@ __global__ __launch_bounds__(TX * TY) void batch_stage_queue(QueueArgs a)
         __syncthreads();
         if (t < 0) break;
         const BBlock &B = a.blocks[a.tile_block[t]];
-        advect_tile<FAST, false, TY>(stage_args(B, a.f), t - B.tile0);
+        advect_tile<FAST, false, TY, true, false, W>(stage_args(B, a.f), t - B.tile0);
         __syncthreads();   // the next tile reuses the LDS planes
     }
 }
 
 // one workgroup per tile
-template <bool FAST, int TY>
-__global__ __launch_bounds__(TX * TY) void batch_stage_grid(QueueArgs a, int ntiles)
+template <bool FAST, int TY, int W>
+__global__ __launch_bounds__(TY * W) void batch_stage_grid(QueueArgs a, int ntiles)
 {
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= ntiles) return;
     const BBlock &B = a.blocks[a.tile_block[t]];
-    advect_tile<FAST, false, TY>(stage_args(B, a.f), t - B.tile0);
+    advect_tile<FAST, false, TY, true, false, W>(stage_args(B, a.f), t - B.tile0);
 }
 
 __global__ void batch_queue_reset(int *qcount)
@@ -125,13 +124,13 @@ __global__ void batch_queue_reset(int *qcount)
     if (threadIdx.x < 8) qcount[threadIdx.x] = 0;
 }
 
-// block whose [first, first + count) range holds t (prefix sums `first`)
-__device__ __forceinline__ int find_block(const BBlock *blocks, int nb, long t, bool ghost)
+// block whose shell-node range [shell0, next shell0) holds t
+__device__ __forceinline__ int find_block(const BBlock *blocks, int nb, long t)
 {
     int lo = 0, hi = nb - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const long s = ghost ? blocks[mid].ghost0 : blocks[mid].shell0;
+        const long s = blocks[mid].shell0;
         if (s <= t) lo = mid;
         else hi = mid - 1;
     }
@@ -151,7 +150,7 @@ __global__ __launch_bounds__(256) void batch_shell(ShellBArgs a)
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.total) return;
     const long tg = t + a.base;
-    const int b = a.b0 + find_block(a.blocks + a.b0, a.nb, tg, false);
+    const int b = a.b0 + find_block(a.blocks + a.b0, a.nb, tg);
     const BBlock &B = a.blocks[b];
     ShellArgs sh;
     sh.out_ro = a.f.out + B.off; sh.out = a.f.out + B.off;
@@ -167,11 +166,21 @@ __global__ __launch_bounds__(256) void batch_shell(ShellBArgs a)
     sh.out[(long)i + sh.sj * j + sh.sk * k] = v;
 }
 
+// One internal-face ghost slab of one block: i faces span all (j,k) of the
+// array, j faces the owned i and all k, k faces the owned (i,j).
+struct HSeg {
+    int b, slab;        // block, slab 0..5 = i low, i high, j low, j high, k low, k high
+    int n0, n1;         // slab extents along its first two index directions (fastest first)
+    int i0, j0, k0;     // local index of the slab's first node
+    int count, wg0;     // nodes, first workgroup
+};
+
 struct HaloArgs {
     const BBlock *blocks;
+    const HSeg *segs;
+    const int32_t *wg_seg;                 // segment of every workgroup
+    int wg_base;                           // workgroup of blockIdx.x = 0
     double *field;
-    int b0, nb;
-    long total, base;
     int g;                                 // ghost layers
     int lat[3];                            // lattice extents
     int lstart[3][kMaxLattice + 1];        // global start of each lattice slab per axis
@@ -187,48 +196,15 @@ __device__ __forceinline__ int lattice_of(const HaloArgs &a, int ax, int G)
 
 __global__ __launch_bounds__(256) void batch_halo(HaloArgs a)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.total) return;
-    const long tg = t + a.base;
-    const int b = a.b0 + find_block(a.blocks + a.b0, a.nb, tg, true);
-    const BBlock &B = a.blocks[b];
-    long u = tg - B.ghost0;
+    const int w = (int)blockIdx.x + a.wg_base;
+    const HSeg S = a.segs[a.wg_seg[w]];
+    const int e = (w - S.wg0) * 256 + (int)threadIdx.x;
+    if (e >= S.count) return;
+    const BBlock &B = a.blocks[S.b];
+    // node e of the slab, i fastest (n0 along i, n1 along j)
+    const int q0 = e % S.n0, r = e / S.n0;
+    const int i = S.i0 + q0, j = S.j0 + r % S.n1, k = S.k0 + r / S.n1;
     const int g = a.g;
-    const int noi = B.r.ib - B.r.ia + 1, noj = B.r.jb - B.r.ja + 1;
-    // slabs of the internal faces: i faces span all (j,k), j faces the owned
-    // i and all k, k faces the owned (i,j)
-    int i = -1, j = -1, k = -1;
-    const bool sl[6] = {!B.r.fb, !B.r.ff, !B.r.fl, !B.r.fr, !B.r.fd, !B.r.fu};
-    const long cnt[6] = {(long)g * B.nj * B.nk, (long)g * B.nj * B.nk, (long)noi * g * B.nk,
-                         (long)noi * g * B.nk, (long)noi * noj * g, (long)noi * noj * g};
-    int s = 0;
-    for (; s < 6; ++s) {
-        if (!sl[s]) continue;
-        if (u < cnt[s]) break;
-        u -= cnt[s];
-    }
-    if (s >= 6) return;
-    if (s < 2) {
-        const int w = (int)(u % g);
-        const long r = u / g;
-        j = (int)(r % B.nj);
-        k = (int)(r / B.nj);
-        i = (s == 0) ? w : g + noi + w;
-    } else if (s < 4) {
-        const int w = (int)(u % noi);
-        const long r = u / noi;
-        j = (int)(r % g);
-        k = (int)(r / g);
-        i = g + w;
-        j = (s == 2) ? j : g + noj + j;
-    } else {
-        const int w = (int)(u % noi);
-        const long r = u / noi;
-        j = g + (int)(r % noj);
-        k = (int)(r / noj);
-        i = g + w;
-        k = (s == 4) ? k : g + (B.r.kb - B.r.ka + 1) + k;
-    }
     const int loc[3] = {i, j, k};
     int G[3], L[3];
 #pragma unroll
@@ -254,6 +230,7 @@ inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
 struct BatchTune {
     int form;        // LSM_BATCH_QUEUE, LSM_BATCH_GRID, LSM_BATCH_PERBLOCK
+    int w;           // tile columns (64 or 32)
     int ty;          // tile rows (4 or 8)
     int kc;          // planes per tile
     int wg_per_cu;   // persistent workgroups per CU (queue form)
@@ -272,12 +249,16 @@ struct LsmBatch {
     lsm::BBlock *db = nullptr;
     int32_t *d_tile_block = nullptr;
     int32_t *d_block_of = nullptr;
+    lsm::HSeg *d_segs = nullptr;       // halo segments of every block
+    int32_t *d_wg_seg = nullptr;       // segment of every halo workgroup
+    int halo_wgs = 0;
+    std::vector<int> blk_wg;           // first halo workgroup of each block (nb + 1 entries)
     int *d_qcount = nullptr;
     int qstart[9];
     int ntiles = 0;
-    long nshell = 0, nghost = 0;
+    long nshell = 0;
     const double *aj = nullptr;
-    lsm::BatchTune tune{LSM_BATCH_QUEUE, 4, 8, 0};
+    lsm::BatchTune tune{LSM_BATCH_QUEUE, 64, 4, 8, 0};
     int cus = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -303,14 +284,14 @@ Range block_range(const LsmBatchBlock &d, int g)
     return r;
 }
 
-// (re)builds the tile tables for tile rows `ty` and `kc` planes per tile
-int build_tiles(LsmBatch *B, int ty, int kc)
+// (re)builds the tile tables for tiles of w columns, ty rows and kc planes
+int build_tiles(LsmBatch *B, int w, int ty, int kc)
 {
     std::vector<int32_t> tb;
     for (int b = 0; b < B->nb; ++b) {
         BBlock &X = B->hb[b];
         const Range &r = X.r;
-        X.tiles_i = (r.ihi + 1 + TX - 1) / TX;
+        X.tiles_i = (r.ihi + 1 + w - 1) / w;
         X.jb0 = r.jlo;
         const int nj_int = r.jhi - r.jlo + 1, nk_int = r.khi - r.klo + 1;
         X.tiles_j = nj_int > 0 ? (nj_int + ty - 1) / ty : 0;
@@ -332,9 +313,32 @@ int build_tiles(LsmBatch *B, int ty, int kc)
     }
     if ((rc = check_hip(hipMemcpy(B->db, B->hb.data(), B->hb.size() * sizeof(BBlock), hipMemcpyHostToDevice),
                         "hipMemcpy"))) return rc;
+    B->tune.w = w;
     B->tune.ty = ty;
     B->tune.kc = kc;
     return LSM_OK;
+}
+
+template <bool FAST, int TY, int W>
+int stage_tiles_t(LsmBatch *B, const QueueArgs &qa, hipStream_t s)
+{
+    const dim3 blk(TX, TY * W / TX, 1);
+    if (B->tune.form == LSM_BATCH_QUEUE) {
+        batch_queue_reset<<<1, 64, 0, s>>>(B->d_qcount);
+        int per_cu = B->tune.wg_per_cu;
+        if (per_cu <= 0) {
+            int occ = 0;
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_stage_queue<FAST, TY, W>, TY * W, 0);
+            per_cu = occ > 0 ? occ : 4;
+        }
+        unsigned grid = (unsigned)(B->cus * per_cu);
+        if (grid > (unsigned)B->ntiles) grid = (unsigned)((B->ntiles + 7) / 8 * 8);
+        batch_stage_queue<FAST, TY, W><<<grid, blk, 0, s>>>(qa);
+    } else {
+        const unsigned grid = (unsigned)((B->ntiles + 7) / 8 * 8);
+        batch_stage_grid<FAST, TY, W><<<grid, blk, 0, s>>>(qa, B->ntiles);
+    }
+    return check_hip(hipGetLastError(), "batch stage");
 }
 
 template <bool FAST>
@@ -344,28 +348,10 @@ int stage_tiles(LsmBatch *B, const BFields &f, hipStream_t s)
     QueueArgs qa{};
     qa.blocks = B->db; qa.tile_block = B->d_tile_block; qa.qcount = B->d_qcount; qa.f = f;
     for (int q = 0; q <= 8; ++q) qa.qstart[q] = B->qstart[q];
-    const int ty = B->tune.ty;
-    const dim3 blk(TX, ty, 1);
     KTimer kt(LSM_KT_BATCH_STAGE, s);
-    if (B->tune.form == LSM_BATCH_QUEUE) {
-        batch_queue_reset<<<1, 64, 0, s>>>(B->d_qcount);
-        int per_cu = B->tune.wg_per_cu;
-        if (per_cu <= 0) {
-            int occ = 0;
-            if (ty == 8) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_stage_queue<FAST, 8>, TX * 8, 0);
-            else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_stage_queue<FAST, 4>, TX * 4, 0);
-            per_cu = occ > 0 ? occ : 4;
-        }
-        unsigned grid = (unsigned)(B->cus * per_cu);
-        if (grid > (unsigned)B->ntiles) grid = (unsigned)((B->ntiles + 7) / 8 * 8);
-        if (ty == 8) batch_stage_queue<FAST, 8><<<grid, blk, 0, s>>>(qa);
-        else batch_stage_queue<FAST, 4><<<grid, blk, 0, s>>>(qa);
-    } else {
-        const unsigned grid = (unsigned)((B->ntiles + 7) / 8 * 8);
-        if (ty == 8) batch_stage_grid<FAST, 8><<<grid, blk, 0, s>>>(qa, B->ntiles);
-        else batch_stage_grid<FAST, 4><<<grid, blk, 0, s>>>(qa, B->ntiles);
-    }
-    return check_hip(hipGetLastError(), "batch stage");
+    const int ty = B->tune.ty, w = B->tune.w;
+    if (w == 32) return ty == 8 ? stage_tiles_t<FAST, 8, 32>(B, qa, s) : stage_tiles_t<FAST, 4, 32>(B, qa, s);
+    return ty == 8 ? stage_tiles_t<FAST, 8, 64>(B, qa, s) : stage_tiles_t<FAST, 4, 64>(B, qa, s);
 }
 
 int shell_launch(LsmBatch *B, const BFields &f, int b0, int nb, hipStream_t s)
@@ -381,19 +367,63 @@ int shell_launch(LsmBatch *B, const BFields &f, int b0, int nb, hipStream_t s)
 int halo_launch(LsmBatch *B, double *field, int b0, int nb, hipStream_t s)
 {
     if (!B->connected || B->g == 0) return LSM_OK;
-    const long base = B->hb[b0].ghost0;
-    const long end = (b0 + nb < B->nb) ? B->hb[b0 + nb].ghost0 : B->nghost;
-    if (end <= base) return LSM_OK;
+    const int w0 = B->blk_wg[b0], w1 = B->blk_wg[b0 + nb];
+    if (w1 <= w0) return LSM_OK;
     HaloArgs ha{};
-    ha.blocks = B->db; ha.field = field; ha.b0 = b0; ha.nb = nb; ha.total = end - base; ha.base = base;
-    ha.g = B->g;
+    ha.blocks = B->db; ha.segs = B->d_segs; ha.wg_seg = B->d_wg_seg; ha.wg_base = w0;
+    ha.field = field; ha.g = B->g;
     for (int ax = 0; ax < 3; ++ax) {
         ha.lat[ax] = B->lat[ax];
         for (int q = 0; q <= B->lat[ax]; ++q) ha.lstart[ax][q] = B->lstart[ax][q];
     }
     ha.block_of = B->d_block_of;
-    batch_halo<<<nblk(end - base), 256, 0, s>>>(ha);
+    batch_halo<<<(unsigned)(w1 - w0), 256, 0, s>>>(ha);
     return check_hip(hipGetLastError(), "batch_halo");
+}
+
+// halo segments (internal-face slabs) of every block and the workgroup table
+int build_halo(LsmBatch *B)
+{
+    std::vector<HSeg> segs;
+    std::vector<int32_t> wseg;
+    B->blk_wg.assign(B->nb + 1, 0);
+    for (int b = 0; b < B->nb; ++b) {
+        B->blk_wg[b] = (int)wseg.size();
+        if (!B->connected || B->g == 0) continue;
+        const BBlock &X = B->hb[b];
+        const int g = B->g;
+        const int noi = X.r.ib - X.r.ia + 1, noj = X.r.jb - X.r.ja + 1, nok = X.r.kb - X.r.ka + 1;
+        const bool internal[6] = {!X.r.fb, !X.r.ff, !X.r.fl, !X.r.fr, !X.r.fd, !X.r.fu};
+        for (int sl = 0; sl < 6; ++sl) {
+            if (!internal[sl]) continue;
+            HSeg h{};
+            h.b = b; h.slab = sl;
+            if (sl < 2) {        // i slab: g x nj x nk
+                h.n0 = g; h.n1 = X.nj; h.i0 = sl == 0 ? 0 : g + noi; h.j0 = 0; h.k0 = 0;
+                h.count = g * X.nj * X.nk;
+            } else if (sl < 4) { // j slab: noi x g x nk
+                h.n0 = noi; h.n1 = g; h.i0 = g; h.j0 = sl == 2 ? 0 : g + noj; h.k0 = 0;
+                h.count = noi * g * X.nk;
+            } else {             // k slab: noi x noj x g
+                h.n0 = noi; h.n1 = noj; h.i0 = g; h.j0 = g; h.k0 = sl == 4 ? 0 : g + nok;
+                h.count = noi * noj * g;
+            }
+            h.wg0 = (int)wseg.size();
+            const int nw = (h.count + 255) / 256;
+            for (int q = 0; q < nw; ++q) wseg.push_back((int32_t)segs.size());
+            segs.push_back(h);
+        }
+    }
+    B->blk_wg[B->nb] = (int)wseg.size();
+    B->halo_wgs = (int)wseg.size();
+    if (segs.empty()) return LSM_OK;
+    int rc;
+    if ((rc = check_hip(hipMalloc(&B->d_segs, segs.size() * sizeof(HSeg)), "hipMalloc"))) return rc;
+    if ((rc = check_hip(hipMalloc(&B->d_wg_seg, wseg.size() * sizeof(int32_t)), "hipMalloc"))) return rc;
+    if ((rc = check_hip(hipMemcpy(B->d_segs, segs.data(), segs.size() * sizeof(HSeg), hipMemcpyHostToDevice),
+                        "hipMemcpy"))) return rc;
+    return check_hip(hipMemcpy(B->d_wg_seg, wseg.data(), wseg.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+                     "hipMemcpy");
 }
 
 const double kC[3][3] = {{1.0, 0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
@@ -515,7 +545,7 @@ int lsm_batch_create(const LsmBatchDesc *d, LsmBatch **out)
         B->lat[0] = B->lat[1] = B->lat[2] = 1;
     }
     // block descriptors
-    long shell = 0, ghost = 0;
+    long shell = 0;
     B->hb.resize(d->nblocks);
     for (int b = 0; b < d->nblocks; ++b) {
         const LsmBatchBlock &X = d->blocks[b];
@@ -538,23 +568,8 @@ int lsm_batch_create(const LsmBatchDesc *d, LsmBatch **out)
         Y.nI = sh.nI; Y.nJ = sh.nJ; Y.nK = sh.nK; Y.cnt_i = sh.cnt_i; Y.cnt_j = sh.cnt_j;
         Y.shell0 = shell;
         shell += sh.nI + sh.nJ + sh.nK;
-        Y.ghost0 = ghost;
-        if (conn) {
-            const long noi = X.ni, noj = X.nj, nok = X.nk;
-            long ng = 0;
-            if (!Y.r.fb) ng += (long)g * Y.nj * Y.nk;
-            if (!Y.r.ff) ng += (long)g * Y.nj * Y.nk;
-            if (!Y.r.fl) ng += noi * g * Y.nk;
-            if (!Y.r.fr) ng += noi * g * Y.nk;
-            if (!Y.r.fd) ng += noi * noj * g;
-            if (!Y.r.fu) ng += noi * noj * g;
-            (void)nok;
-            Y.nghost = ng;
-            ghost += ng;
-        }
     }
     B->nshell = shell;
-    B->nghost = ghost;
     int rc;
     if ((rc = check_hip(hipGetDevice(&B->dev), "hipGetDevice"))) { delete B; return rc; }
     if ((rc = check_hip(hipDeviceGetAttribute(&B->cus, hipDeviceAttributeMultiprocessorCount, B->dev),
@@ -572,7 +587,8 @@ int lsm_batch_create(const LsmBatchDesc *d, LsmBatch **out)
             return rc;
         }
     }
-    if ((rc = build_tiles(B, 4, 8))) { lsm_batch_destroy(B); return rc; }
+    if ((rc = build_tiles(B, 64, 4, 8))) { lsm_batch_destroy(B); return rc; }
+    if ((rc = build_halo(B))) { lsm_batch_destroy(B); return rc; }
     *out = B;
     return LSM_OK;
 }
@@ -585,34 +601,40 @@ int lsm_batch_destroy(LsmBatch *B)
     if (B->db) (void)hipFree(B->db);
     if (B->d_tile_block) (void)hipFree(B->d_tile_block);
     if (B->d_block_of) (void)hipFree(B->d_block_of);
+    if (B->d_segs) (void)hipFree(B->d_segs);
+    if (B->d_wg_seg) (void)hipFree(B->d_wg_seg);
     if (B->d_qcount) (void)hipFree(B->d_qcount);
     delete B;
     return LSM_OK;
 }
 
-int lsm_batch_set_config(LsmBatch *B, int form, int tile_rows, int kchunk, int wg_per_cu)
+int lsm_batch_set_config(LsmBatch *B, int form, int tile_cols, int tile_rows, int kchunk, int wg_per_cu)
 {
     if (!B) return fail(LSM_ERR_ARG, "NULL batch");
     if (form != LSM_BATCH_QUEUE && form != LSM_BATCH_GRID && form != LSM_BATCH_PERBLOCK)
         return fail(LSM_ERR_ARG, "form %d", form);
+    if (tile_cols != 64 && tile_cols != 32) return fail(LSM_ERR_ARG, "tile_cols %d (64 or 32)", tile_cols);
+    if (form == LSM_BATCH_PERBLOCK && tile_cols != 64)
+        return fail(LSM_ERR_ARG, "the per-block form runs the single-block kernels (tile_cols 64)");
     if (tile_rows != 4 && tile_rows != 8) return fail(LSM_ERR_ARG, "tile_rows %d (4 or 8)", tile_rows);
     if (kchunk < 1 || kchunk > 64) return fail(LSM_ERR_ARG, "kchunk %d (1..64)", kchunk);
     if (wg_per_cu < 0 || wg_per_cu > 32) return fail(LSM_ERR_ARG, "wg_per_cu %d (0..32)", wg_per_cu);
     int rc;
-    if (tile_rows != B->tune.ty || kchunk != B->tune.kc) {
+    if (tile_cols != B->tune.w || tile_rows != B->tune.ty || kchunk != B->tune.kc) {
         // the device tables may still be read by queued work
         if ((rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))) return rc;
-        if ((rc = build_tiles(B, tile_rows, kchunk))) return rc;
+        if ((rc = build_tiles(B, tile_cols, tile_rows, kchunk))) return rc;
     }
     B->tune.form = form;
     B->tune.wg_per_cu = wg_per_cu;
     return LSM_OK;
 }
 
-int lsm_batch_get_config(const LsmBatch *B, int32_t cfg[4])
+int lsm_batch_get_config(const LsmBatch *B, int32_t cfg[5])
 {
     if (!B || !cfg) return fail(LSM_ERR_ARG, "NULL argument");
-    cfg[0] = B->tune.form; cfg[1] = B->tune.ty; cfg[2] = B->tune.kc; cfg[3] = B->tune.wg_per_cu;
+    cfg[0] = B->tune.form; cfg[1] = B->tune.w; cfg[2] = B->tune.ty; cfg[3] = B->tune.kc;
+    cfg[4] = B->tune.wg_per_cu;
     return LSM_OK;
 }
 
@@ -691,17 +713,18 @@ int lsm_batch_autotune(LsmBatch *B, double *phi_np1, const double *phi_n, const 
         if (it != g_tune_cache.end()) {
             const BatchTune t = it->second;
             if (best_ms) *best_ms = -1.0;
-            return lsm_batch_set_config(B, t.form, t.ty, t.kc, t.wg_per_cu);
+            return lsm_batch_set_config(B, t.form, t.w, t.ty, t.kc, t.wg_per_cu);
         }
     }
     hipStream_t s = (hipStream_t)stream;
     std::vector<BatchTune> cand;
-    for (int ty : {4, 8})
-        for (int kc : {4, 8, 16}) {
-            cand.push_back(BatchTune{LSM_BATCH_QUEUE, ty, kc, 0});
-            cand.push_back(BatchTune{LSM_BATCH_QUEUE, ty, kc, 4});
-            cand.push_back(BatchTune{LSM_BATCH_GRID, ty, kc, 0});
-        }
+    for (int w : {64, 32})
+        for (int ty : {4, 8})
+            for (int kc : {4, 8, 16}) {
+                cand.push_back(BatchTune{LSM_BATCH_QUEUE, w, ty, kc, 0});
+                cand.push_back(BatchTune{LSM_BATCH_QUEUE, w, ty, kc, 4});
+                cand.push_back(BatchTune{LSM_BATCH_GRID, w, ty, kc, 0});
+            }
     hipEvent_t e0, e1;
     int rc;
     if ((rc = check_hip(hipEventCreate(&e0), "hipEventCreate"))) return rc;
@@ -709,7 +732,7 @@ int lsm_batch_autotune(LsmBatch *B, double *phi_np1, const double *phi_n, const 
     double best = 1e300;
     BatchTune win = B->tune;
     for (const BatchTune &c : cand) {
-        if ((rc = lsm_batch_set_config(B, c.form, c.ty, c.kc, c.wg_per_cu))) break;
+        if ((rc = lsm_batch_set_config(B, c.form, c.w, c.ty, c.kc, c.wg_per_cu))) break;
         if ((rc = lsm_batch_rk3_step(B, phi_np1, phi_n, ucn1, ucn2, ucn3, delti_lsm, mode, work, work_bytes, stream)))
             break;
         (void)hipEventRecord(e0, s);
@@ -731,7 +754,7 @@ int lsm_batch_autotune(LsmBatch *B, double *phi_np1, const double *phi_n, const 
         g_tune_cache[key] = win;
     }
     if (best_ms) *best_ms = best;
-    return lsm_batch_set_config(B, win.form, win.ty, win.kc, win.wg_per_cu);
+    return lsm_batch_set_config(B, win.form, win.w, win.ty, win.kc, win.wg_per_cu);
 }
 
 // ---- graph of a full step -------------------------------------------------

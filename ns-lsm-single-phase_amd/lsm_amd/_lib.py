@@ -58,7 +58,7 @@ _pp = C.POINTER(_vp)
 _SIGS = {
     "lsm_batch_create": (C.c_int, [C.POINTER(LsmBatchDesc), C.POINTER(_vp)]),
     "lsm_batch_destroy": (C.c_int, [_vp]),
-    "lsm_batch_set_config": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "lsm_batch_set_config": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "lsm_batch_get_config": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "lsm_batch_workspace_bytes": (C.c_size_t, [_vp, C.c_int]),
     "lsm_batch_halo_fill": (C.c_int, [_vp, _vp, _vp]),

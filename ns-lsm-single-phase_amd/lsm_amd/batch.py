@@ -309,15 +309,16 @@ class Batch:
         return C.c_void_p(t.data_ptr())
 
     # operations -----------------------------------------------------------
-    def set_config(self, form="queue", tile_rows=4, kchunk=8, wg_per_cu=0):
+    def set_config(self, form="queue", tile_rows=4, kchunk=8, wg_per_cu=0, tile_cols=64):
         check(_lib.load().lsm_batch_set_config(self.handle, FORMS[form] if isinstance(form, str) else int(form),
-                                               int(tile_rows), int(kchunk), int(wg_per_cu)),
+                                               int(tile_cols), int(tile_rows), int(kchunk), int(wg_per_cu)),
               "lsm_batch_set_config")
 
     def get_config(self):
-        cfg = (C.c_int32 * 4)()
+        cfg = (C.c_int32 * 5)()
         check(_lib.load().lsm_batch_get_config(self.handle, cfg), "lsm_batch_get_config")
-        return {"form": FORM_NAMES[cfg[0]], "tile_rows": cfg[1], "kchunk": cfg[2], "wg_per_cu": cfg[3]}
+        return {"form": FORM_NAMES[cfg[0]], "tile_cols": cfg[1], "tile_rows": cfg[2], "kchunk": cfg[3],
+                "wg_per_cu": cfg[4]}
 
     def halo_fill(self, field):
         from .api import _stream

@@ -141,7 +141,9 @@ def _single_block_steps(lay, grids, phi, vel, dt, nsteps, mode, btype):
     return cur.cpu().numpy()
 
 
-CONFIGS = [("queue", 4, 8, 0), ("queue", 8, 4, 2), ("grid", 4, 16, 0), ("grid", 8, 8, 0), ("perblock", 4, 8, 0)]
+# (form, tile rows, planes per tile, persistent workgroups per CU, tile columns)
+CONFIGS = [("queue", 4, 8, 0, 64), ("queue", 8, 4, 2, 64), ("grid", 4, 16, 0, 64), ("grid", 8, 8, 0, 64),
+           ("perblock", 4, 8, 0, 64), ("queue", 8, 8, 0, 32), ("grid", 4, 4, 0, 32)]
 
 
 @pytest.mark.gpu
@@ -153,9 +155,9 @@ def test_connected_batch_matches_single_block(mode, btype):
     ref = _single_block_steps(lay, grids, phi, vel, dt, 3, mode, btype)
     bt = B.Batch(lay, grids)
     ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vel]) for c in range(3)])
-    for form, ty, kc, wg in CONFIGS:
-        bt.set_config(form, ty, kc, wg)
-        assert bt.get_config() == {"form": form, "tile_rows": ty, "kchunk": kc, "wg_per_cu": wg}
+    for form, ty, kc, wg, w in CONFIGS:
+        bt.set_config(form, ty, kc, wg, tile_cols=w)
+        assert bt.get_config() == {"form": form, "tile_cols": w, "tile_rows": ty, "kchunk": kc, "wg_per_cu": wg}
         cur = bt.halo_fill(bt.pack(phi))
         nxt = bt.empty()
         for _ in range(3):
@@ -184,8 +186,8 @@ def test_independent_batch_matches_each_instance():
         blk = api.Block(gd)
         u = api.rhs_contra_j(blk, *[_dev(a) for a in v])
         refs.append(api.ad_RKTVD3(blk, api.ad_RKTVD3(blk, _dev(p), u, dt), u, dt).cpu().numpy())
-    for form, ty, kc, wg in CONFIGS:
-        bt.set_config(form, ty, kc, wg)
+    for form, ty, kc, wg, w in CONFIGS:
+        bt.set_config(form, ty, kc, wg, tile_cols=w)
         a = bt.rk3_step(bt.pack(phis), ucn, dt)
         out = bt.unpack(bt.rk3_step(a, ucn, dt))
         for r, o in zip(refs, out):
