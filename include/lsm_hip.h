@@ -280,6 +280,8 @@ typedef struct LsmBatchDesc {
     int32_t lattice[3];               /* lattice extents (connected) */
     const LsmBatchBlock *blocks;      /* host array of nblocks entries */
     const double *aj;                 /* device batch array of aj */
+    const double *csi[3], *eta[3], *zet[3];  /* device batch metric arrays (NULL: advection from
+                                                given U/J only, no reinitialisation) */
     int64_t total;                    /* doubles in every batch array */
 } LsmBatchDesc;
 
@@ -299,9 +301,8 @@ size_t lsm_batch_workspace_bytes(const LsmBatch *batch, int op);
 /* rhs_exchng3_3d.F90:6-359 for every block of a connected batch: each ghost
  * node inside the domain takes the value of the block owning that node. */
 int lsm_batch_halo_fill(LsmBatch *batch, double *field, void *stream);
-/* rhs_contra_j.F90:2-50 over every node of the batch arrays. */
-int lsm_batch_contra_velocity(LsmBatch *batch, const double *csi[3], const double *eta[3],
-                              const double *zet[3], const double *u, const double *v, const double *w,
+/* rhs_contra_j.F90:2-50 over every node of the batch arrays (the batch's metrics). */
+int lsm_batch_contra_velocity(LsmBatch *batch, const double *u, const double *v, const double *w,
                               double *ucn1, double *ucn2, double *ucn3, void *stream);
 /* ad_RKTVD3.F90:1-88 for every block: three stages, each followed by
  * bcond_lsm on the physical faces and the halo fill (the reference's
@@ -310,6 +311,27 @@ int lsm_batch_contra_velocity(LsmBatch *batch, const double *csi[3], const doubl
 int lsm_batch_rk3_step(LsmBatch *batch, double *phi_np1, const double *phi_n, const double *ucn1,
                        const double *ucn2, const double *ucn3, double delti_lsm, int mode,
                        void *work, size_t work_bytes, void *stream);
+
+/* reinitialisation_test2.F90:1-218 on every block as one rank of the
+ * reference runs it: owned nodes updated, ghost layers kept at phi0's values
+ * (no exchange inside the loop), |grad d| of ghost nodes 0 in the exit test
+ * (a block with ghost layers runs all rn_itermax iterations), one exit test
+ * per block.  phi_out's ghost layers receive phi0's.  iters (device, one
+ * int32 per block, may be NULL) receives the iterations each block ran.
+ * Workspace: lsm_batch_workspace_bytes(batch, LSM_OP_REINIT_ENO2). */
+int lsm_batch_reinit_sussman_eno2(LsmBatch *batch, const LsmReinitParams *p, const double *phi0,
+                                  double *phi_out, int32_t *iters, void *work, size_t work_bytes,
+                                  void *stream);
+/* bcond_lsm.F90:2-568 on the physical faces of every block, in place. */
+int lsm_batch_bcond(LsmBatch *batch, double *field, void *stream);
+/* levelsetmethod.F90:221-326 for every block (numiter_lsm = 1): ad_RKTVD3
+ * with bcond_lsm and the halo fill after each stage, then (p != NULL)
+ * reinitialisation_test2, bcond_lsm and the halo fill.  Workspace:
+ * lsm_batch_workspace_bytes(batch, p ? LSM_OP_LEVELSET_STEP : LSM_OP_RK3). */
+int lsm_batch_levelset_step(LsmBatch *batch, const LsmReinitParams *p, double *phi_np1,
+                            const double *phi_n, const double *ucn1, const double *ucn2,
+                            const double *ucn3, double delti_lsm, int mode, void *work,
+                            size_t work_bytes, void *stream);
 
 /* Autotuner (no counterpart in the reference): times the stage-kernel
  * launch configurations (queue form: tile columns 64 / 32, rows 4 / 8, 4 / 8 /
@@ -323,12 +345,13 @@ int lsm_batch_rk3_step(LsmBatch *batch, double *phi_np1, const double *phi_n, co
 int lsm_batch_autotune(LsmBatch *batch, double *phi_np1, const double *phi_n, const double *ucn1,
                        const double *ucn2, const double *ucn3, double delti_lsm, int mode, void *work,
                        size_t work_bytes, int reps, int force, double *best_ms, void *stream);
-/* Records one lsm_batch_rk3_step (with the current configuration and these
- * arguments, which the graph keeps) into a HIP graph owned by the batch;
- * lsm_batch_graph_launch replays it on any stream.  Needs a workspace and
- * launch timing switched off. */
-int lsm_batch_graph_capture(LsmBatch *batch, double *phi_np1, const double *phi_n, const double *ucn1,
-                            const double *ucn2, const double *ucn3, double delti_lsm, int mode, void *work,
+/* Records one lsm_batch_levelset_step (with the current configuration and
+ * these arguments, which the graph keeps; p = NULL: advection only) into a
+ * HIP graph owned by the batch; lsm_batch_graph_launch replays it on any
+ * stream.  Needs a workspace and launch timing switched off. */
+int lsm_batch_graph_capture(LsmBatch *batch, const LsmReinitParams *p, double *phi_np1,
+                            const double *phi_n, const double *ucn1, const double *ucn2,
+                            const double *ucn3, double delti_lsm, int mode, void *work,
                             size_t work_bytes);
 int lsm_batch_graph_launch(LsmBatch *batch, void *stream);
 
@@ -385,7 +408,8 @@ int lsm_host_water_volume(const double *phi, double vol[3], int32_t *adv_mask);
 #define LSM_KT_VOLUME         3   /* water volume cell kernel */
 #define LSM_KT_ADJOINT_STAGE  4   /* fused adjoint of one RK stage (partials + gather) */
 #define LSM_KT_BATCH_STAGE    5   /* batched stage tiles of every block (one launch) */
-#define LSM_KT_COUNT          6
+#define LSM_KT_BATCH_REINIT   6   /* batched reinitialisation stage of every block */
+#define LSM_KT_COUNT          7
 
 /* Turns event recording on (1) or off (0); clears accumulated timings. */
 int lsm_timing_enable(int on);

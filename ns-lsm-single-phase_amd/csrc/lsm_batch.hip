@@ -49,6 +49,8 @@ struct BBlock {
     long shell0;              // first shell node of the block in the batch enumeration
     long nI, nJ, nK;
     int cnt_i, cnt_j;
+    int rt_tiles_i, rt_tiles_j, rt_ntiles, rt_tile0, rt_kc;   // reinitialisation tiles (64 x 4 x rt_kc)
+    int sh_wg0;               // first shell workgroup of the block (reinitialisation)
 };
 
 struct BFields {
@@ -220,6 +222,327 @@ __global__ __launch_bounds__(256) void batch_halo(HaloArgs a)
     a.field[B.off + (long)i + B.sj * j + B.sk * k] = a.field[src];
 }
 
+// ---------------------------------------------------------------------------
+// batched reinitialisation (reinitialisation_test2.F90:42-210 on every block
+// as one rank of the reference runs it): owned nodes are updated, ghost
+// layers keep their values of the start (no exchange inside the loop), |grad
+// d| of the ghost nodes counts as 0 in the exit test (so a block with ghost
+// layers runs every iteration), each block has its own exit test.
+// ---------------------------------------------------------------------------
+struct RCtl {
+    unsigned long long maxbits, minbits;
+    int active, iters;
+};
+
+constexpr unsigned long long kInf64 = 0x7ff0000000000000ull;
+
+struct RStage {
+    double c1, c2, c3, dt, eps, eps_r;
+    int order_bc, eno_bc;
+};
+
+struct BRArgs {
+    const BBlock *blocks;
+    const int32_t *rtile_block;   // block of every reinitialisation tile
+    const int32_t *sh_block;      // block of every shell workgroup
+    int tile_base, sh_base;       // first tile / shell workgroup of this launch
+    int ntiles, grid_tiles;       // tiles; tile workgroups (padded to 8), shell workgroups follow
+    const double *m[9];           // batch metric arrays (csi, eta, zet)
+    const double *din, *dn, *phi0;
+    double *dout;
+    uint8_t *conv;
+    RCtl *ctl;                    // one per block
+    RStage p;
+};
+
+// |grad d| at an owned node (calc_NormGrad_d.F90:39-2446 on one rank): one-sided
+// formulas on the block's physical faces, ENO2 elsewhere (ghost values in the
+// stencil at internal faces); phi0s is phizero at the block's last interior
+// node, which the i-face formulas use for their upwind sign
+__device__ double normgrad_node_r(const double *__restrict__ d, const double *__restrict__ phi0,
+                                  const double *const *m, long sj, long sk, const Range &r, const double dd[3],
+                                  double phi0s, int order_bc, bool eno_bc, int i, int j, int k, long l)
+{
+    const int idx[3] = {i, j, k};
+    const long st[3] = {1, sj, sk};
+    const int lo[3] = {r.ia, r.ja, r.ka}, hi[3] = {r.ib, r.jb, r.kb};
+    const bool flo[3] = {r.fb, r.fl, r.fd}, fhi[3] = {r.ff, r.fr, r.fu};
+    bool onl[3], onh[3], on[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        onl[a] = flo[a] && idx[a] == lo[a];
+        onh[a] = fhi[a] && idx[a] == hi[a];
+        on[a] = onl[a] || onh[a];
+    }
+    const double p0 = (on[0] && !on[1] && !on[2]) ? phi0s : phi0[l];
+    double gr[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const long s = st[a];
+        if (on[a]) {
+            const int dir = onl[a] ? 1 : -1;
+            const long ss = s * dir;
+            if (a == 0 && onl[0] && onl[2] && !on[1]) {
+                // edge i = 1, k = 1 (calc_NormGrad_d.F90:1281-1283)
+                const double dc2 = 0.5 * dd[0];
+                gr[a] = dc2 * ((-(1.0 * d[l + 2]) + 4.0 * d[l + 1]) - 3.0 * d[l]);
+            } else {
+                gr[a] = dd[a] * biased(d[l], d[l + ss], d[l + 2 * ss], d[l + 3 * ss], order_bc, dir, eno_bc);
+            }
+        } else {
+            const bool lo_o = flo[a] && idx[a] == lo[a] + 1, hi_o = fhi[a] && idx[a] == hi[a] - 1;
+            const double fLL = lo_o ? 0.0 : d[l - 2 * s];
+            const double fRR = hi_o ? 0.0 : d[l + 2 * s];
+            gr[a] = dd[a] * eno2(p0, fLL, d[l - s], d[l], d[l + s], fRR, lo_o, hi_o);
+        }
+    }
+    const double gx = (m[0][l] * gr[0] + m[3][l] * gr[1]) + m[6][l] * gr[2];
+    const double gy = (m[1][l] * gr[0] + m[4][l] * gr[1]) + m[7][l] * gr[2];
+    const double gz = (m[2][l] * gr[0] + m[5][l] * gr[1]) + m[8][l] * gr[2];
+    return sqrt((gx * gx + gy * gy) + gz * gz);
+}
+
+// the stage update at one node (reinitialisation_test2.F90:137-181)
+__device__ __forceinline__ double rn_update(const RStage &p, double ngv, double dv, double dnv, double p0, bool upd)
+{
+    double v = dv;
+    if (upd) {
+        const double s = sign_phi0(p0, p.eps);
+        double rhs = s * (ngv - 1.0);
+        const double lam = lambda_mass(dv, s, ngv, p.eps);
+        const double aux = deltaf(dv, p.eps) * ngv;
+        rhs = rhs - lam * aux;
+        v = (p.c1 * dnv + p.c2 * dv) - (p.c3 * p.dt) * rhs;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void rn_reduce(double vmax, double vmin, int tid, int nthreads, RCtl *ctl)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+        vmin = fmin(vmin, __shfl_xor(vmin, off, 64));
+    }
+    __shared__ double smax[16], smin[16];
+    const int lane = tid & 63, wv = tid >> 6;
+    if (lane == 0) { smax[wv] = vmax; smin[wv] = vmin; }
+    __syncthreads();
+    if (tid == 0) {
+        double a = smax[0], b = smin[0];
+        for (int q = 1; q < nthreads / 64; ++q) { a = fmax(a, smax[q]); b = fmin(b, smin[q]); }
+        atomicMax(&ctl->maxbits, (unsigned long long)__double_as_longlong(a));
+        atomicMin(&ctl->minbits, (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// STAGE 0: max / min of |grad din| only; 1..3: the RK stage
+template <int STAGE>
+__global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
+{
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const double inf = __longlong_as_double((long long)kInf64);
+    double vmax = 0.0, vmin = inf;
+    const int bid = blockIdx.x;
+    if (bid >= a.grid_tiles) {
+        // ---- owned nodes on physical faces: one node per thread ----
+        const int w = bid - a.grid_tiles + a.sh_base;
+        const int b = a.sh_block[w];
+        const BBlock &B = a.blocks[b];
+        RCtl *ctl = a.ctl + b;
+        if (!ctl->active) return;
+        const long e = (long)(w - B.sh_wg0) * 256 + tid;
+        ShellArgs sh;
+        sh.r = B.r; sh.nI = B.nI; sh.nJ = B.nJ; sh.nK = B.nK; sh.cnt_i = B.cnt_i; sh.cnt_j = B.cnt_j;
+        int i, j, k;
+        if (shell_node(sh, e, i, j, k)) {
+            const long l = (long)i + B.sj * j + B.sk * k;
+            const double *din = a.din + B.off, *phi0 = a.phi0 + B.off;
+            const double *mm[9];
+            for (int c = 0; c < 9; ++c) mm[c] = a.m[c] + B.off;
+            const double dd[3] = {B.dc, B.de, B.dz};
+            const double p0s = phi0[(long)B.r.ihi + B.sj * B.r.jhi + B.sk * B.r.khi];
+            const double ngv = normgrad_node_r(din, phi0, mm, B.sj, B.sk, B.r, dd, p0s, a.p.order_bc, a.p.eno_bc != 0,
+                                               i, j, k, l);
+            if (STAGE > 0) {
+                bool upd;
+                if (STAGE == 1) { upd = ngv - 1.0 > a.p.eps_r; a.conv[B.off + l] = upd ? 1 : 0; }
+                else upd = a.conv[B.off + l] != 0;
+                const double dv = din[l];
+                const double dnv = (STAGE == 1) ? dv : a.dn[B.off + l];
+                a.dout[B.off + l] = rn_update(a.p, ngv, dv, dnv, phi0[l], upd);
+            }
+            vmax = ngv; vmin = ngv;
+        }
+        if (STAGE == 3 || STAGE == 0) rn_reduce(vmax, vmin, tid, 256, ctl);
+        return;
+    }
+    // ---- interior tiles: 64 (i) x 4 (j) columns marching up kc planes ----
+    int tile = xcd_tile(bid, a.grid_tiles);
+    const bool live = tile < a.ntiles;
+    tile += a.tile_base;
+    const int b = a.rtile_block[live ? tile : a.tile_base];
+    const BBlock &B = a.blocks[b];
+    RCtl *ctl = a.ctl + b;
+    if (!ctl->active) return;
+    const int t = live ? tile - B.rt_tile0 : 0;
+    const int ti = t % B.rt_tiles_i, t2 = t / B.rt_tiles_i;
+    const int tj = t2 % B.rt_tiles_j, tk = t2 / B.rt_tiles_j;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const Range &r = B.r;
+    const int i = ti * 64 + tx, j = r.jlo + tj * 4 + ty;
+    const int kb = r.klo + tk * B.rt_kc;
+    const int ke = live ? min(kb + B.rt_kc - 1, r.khi) : kb - 1;
+    __shared__ double sm[2][4 + 4][64 + 4];
+    const int ni = B.ni, nj = B.nj, nk = B.nk;
+    const long sj = B.sj, sk = B.sk;
+    const int ic = min(i, ni - 1), jc = min(j, nj - 1);
+    const long col = (long)ic + sj * jc;
+    const bool has_jh = ty < 4;
+    const int rr = (ty < 2) ? ty : ty - 2 + 4 + 2;
+    const int jh = min(max(r.jlo + tj * 4 - 2 + rr, 0), nj - 1);
+    const long col_jh = (long)ic + sj * jh;
+    const int lin = tid;
+    const bool has_ih = lin < 16;
+    const int hr = (lin >> 2) % 4, hc4 = lin & 3;
+    const int hcol = (hc4 < 2) ? hc4 : hc4 + 64;
+    const int hi_i = min(max(ti * 64 - 2 + hcol, 0), ni - 1);
+    const int hi_j = min(r.jlo + tj * 4 + hr, nj - 1);
+    const long col_ih = (long)hi_i + sj * hi_j;
+    auto kcl = [&](int kk) -> long { return sk * (long)min(max(kk, 0), nk - 1); };
+    const bool act = live && i >= r.ilo && i <= r.ihi && j >= r.jlo && j <= r.jhi;
+    const bool ilo = r.fb && i == r.ilo, ihi = r.ff && i == r.ihi, jlo = r.fl && j == r.jlo, jhi = r.fr && j == r.jhi;
+    const double *__restrict__ d = a.din + B.off;
+    const double *phi0 = a.phi0 + B.off;
+    const double dd0 = B.dc, dd1 = B.de, dd2 = B.dz;
+    double q0 = d[col + kcl(kb - 2)], q1 = d[col + kcl(kb - 1)];
+    double q2 = d[col + kcl(kb)], q3 = d[col + kcl(kb + 1)];
+    double nq4 = d[col + kcl(kb + 2)];
+    double njh = has_jh ? d[col_jh + kcl(kb)] : 0.0;
+    double nih = has_ih ? d[col_ih + kcl(kb)] : 0.0;
+    int buf = 0;
+    for (int k = kb; k <= ke; ++k) {
+        const double q4 = nq4;
+        sm[buf][ty + 2][tx + 2] = q2;
+        if (has_jh) sm[buf][rr][tx + 2] = njh;
+        if (has_ih) sm[buf][hr + 2][hcol] = nih;
+        __syncthreads();
+        if (k < ke) {
+            const long kn = kcl(k + 1);
+            nq4 = d[col + kcl(k + 3)];
+            if (has_jh) njh = d[col_jh + kn];
+            if (has_ih) nih = d[col_ih + kn];
+        }
+        if (act) {
+            const long l = col + sk * (long)k;
+            const double p0 = phi0[l];
+            double m[9];
+#pragma unroll
+            for (int c = 0; c < 9; ++c) m[c] = __builtin_nontemporal_load(a.m[c] + B.off + l);
+            double dnv = q2;
+            uint8_t cv = 0;
+            if (STAGE > 1) { dnv = __builtin_nontemporal_load(a.dn + B.off + l); cv = a.conv[B.off + l]; }
+            const double (*pl)[64 + 4] = sm[buf];
+            const int cx = tx + 2, cy = ty + 2;
+            const double g0 = dd0 * eno2(p0, ilo ? 0.0 : pl[cy][cx - 2], pl[cy][cx - 1], q2, pl[cy][cx + 1],
+                                         ihi ? 0.0 : pl[cy][cx + 2], ilo, ihi);
+            const double g1 = dd1 * eno2(p0, jlo ? 0.0 : pl[cy - 2][cx], pl[cy - 1][cx], q2, pl[cy + 1][cx],
+                                         jhi ? 0.0 : pl[cy + 2][cx], jlo, jhi);
+            const bool klo = r.fd && k == r.klo, khi = r.fu && k == r.khi;
+            const double g2 = dd2 * eno2(p0, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi);
+            const double gx = (m[0] * g0 + m[3] * g1) + m[6] * g2;
+            const double gy = (m[1] * g0 + m[4] * g1) + m[7] * g2;
+            const double gz = (m[2] * g0 + m[5] * g1) + m[8] * g2;
+            const double ngv = sqrt((gx * gx + gy * gy) + gz * gz);
+            if (STAGE > 0) {
+                bool upd;
+                if (STAGE == 1) { upd = ngv - 1.0 > a.p.eps_r; a.conv[B.off + l] = upd ? 1 : 0; }
+                else upd = cv != 0;
+                __builtin_nontemporal_store(rn_update(a.p, ngv, q2, dnv, p0, upd), a.dout + B.off + l);
+            }
+            if (STAGE == 3 || STAGE == 0) { vmax = fmax(vmax, ngv); vmin = fmin(vmin, ngv); }
+        }
+        q0 = q1; q1 = q2; q2 = q3; q3 = q4;
+        buf ^= 1;
+    }
+    if (STAGE == 3 || STAGE == 0) rn_reduce(vmax, vmin, tid, 256, ctl);
+}
+
+// per-block exit-test state: a block with ghost layers counts |grad d| = 0
+// there, so its minimum starts at 0
+__global__ void batch_reinit_init(RCtl *ctl, int nb, int ghosted, int32_t *iters)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    ctl[b].maxbits = 0ull;
+    ctl[b].minbits = ghosted ? 0ull : kInf64;
+    ctl[b].active = 1;
+    ctl[b].iters = 0;
+    if (iters) iters[b] = 0;
+}
+
+// exit test at the top of a pseudo-time iteration (reinitialisation_test2.F90:100-107), per block
+__global__ void batch_reinit_check(RCtl *ctl, int nb, int ghosted, double eps, int32_t *iters)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    RCtl &c = ctl[b];
+    if (c.active) {
+        const double mx = __longlong_as_double((long long)c.maxbits);
+        const double mn = __longlong_as_double((long long)c.minbits);
+        if (fabs(mx - 1.0) < eps && fabs(mn - 1.0) < eps) c.active = 0;
+        else c.iters += 1;
+    }
+    c.maxbits = 0ull;
+    c.minbits = ghosted ? 0ull : kInf64;
+    if (iters) iters[b] = c.iters;
+}
+
+// ghost layers of src copied into dst (same block, same positions)
+__global__ __launch_bounds__(256) void batch_ghost_copy(HaloArgs a, const double *src)
+{
+    const int w = (int)blockIdx.x + a.wg_base;
+    const HSeg S = a.segs[a.wg_seg[w]];
+    const int e = (w - S.wg0) * 256 + (int)threadIdx.x;
+    if (e >= S.count) return;
+    const BBlock &B = a.blocks[S.b];
+    const int q0 = e % S.n0, r = e / S.n0;
+    const long l = B.off + (long)(S.i0 + q0) + B.sj * (S.j0 + r % S.n1) + B.sk * (S.k0 + r / S.n1);
+    a.field[l] = src[l];
+}
+
+// bcond_lsm.F90:26-287 on the physical faces of one axis for every block, in
+// place: faces of one axis read only nodes no face of that axis writes
+struct BcSegArgs {
+    const BBlock *blocks;
+    const HSeg *segs;            // slab 0/1 (axis low/high face), n0 x n1 owned nodes of the other two axes
+    const int32_t *wg_seg;
+    int wg_base, axis;
+    double *field;
+};
+
+__global__ __launch_bounds__(256) void batch_bcond_phase(BcSegArgs a)
+{
+    const int w = (int)blockIdx.x + a.wg_base;
+    const HSeg S = a.segs[a.wg_seg[w]];
+    const int e = (w - S.wg0) * 256 + (int)threadIdx.x;
+    if (e >= S.count) return;
+    const BBlock &B = a.blocks[S.b];
+    const int q0 = e % S.n0, q1 = e / S.n0;
+    const int ax = a.axis;
+    int i, j, k;
+    long s;
+    const bool hi = S.slab & 1;
+    if (ax == 0) { i = hi ? B.r.ib : B.r.ia; j = B.r.ja + q0; k = B.r.ka + q1; s = 1; }
+    else if (ax == 1) { j = hi ? B.r.jb : B.r.ja; i = B.r.ia + q0; k = B.r.ka + q1; s = B.sj; }
+    else { k = hi ? B.r.kb : B.r.ka; i = B.r.ia + q0; j = B.r.ja + q1; s = B.sk; }
+    const int bt = B.bt[2 * ax + (hi ? 1 : 0)];
+    if (bt == 2 && !(ax == 0 && hi)) return;
+    double *f = a.field + B.off;
+    const long l = (long)i + B.sj * j + B.sk * k;
+    const long d = hi ? -s : s;
+    f[l] = bc_face(bt, f[l + d], f[l + 2 * d], f[l], B.big, ax == 0 && hi);
+}
+
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -249,6 +572,14 @@ struct LsmBatch {
     lsm::BBlock *db = nullptr;
     int32_t *d_tile_block = nullptr;
     int32_t *d_block_of = nullptr;
+    const double *met[9] = {nullptr};  // batch metric arrays (csi, eta, zet), may be NULL
+    int32_t *d_rtile_block = nullptr;  // reinitialisation tiles
+    int rt_ntiles = 0;
+    int32_t *d_sh_block = nullptr;     // shell workgroups (reinitialisation)
+    int sh_wgs = 0;
+    lsm::HSeg *d_bseg[3] = {nullptr};  // bcond_lsm face segments per axis
+    int32_t *d_bwg[3] = {nullptr};
+    std::vector<int> bc_wg[3];         // first bcond workgroup of each block per axis (nb + 1)
     lsm::HSeg *d_segs = nullptr;       // halo segments of every block
     int32_t *d_wg_seg = nullptr;       // segment of every halo workgroup
     int halo_wgs = 0;
@@ -426,6 +757,114 @@ int build_halo(LsmBatch *B)
                      "hipMemcpy");
 }
 
+template <class T>
+int upload(const std::vector<T> &v, T **dst)
+{
+    if (*dst) { (void)hipFree(*dst); *dst = nullptr; }
+    if (v.empty()) return LSM_OK;
+    int rc = check_hip(hipMalloc(dst, v.size() * sizeof(T)), "hipMalloc");
+    if (rc) return rc;
+    return check_hip(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+// reinitialisation tiles (64 x 4 x 8 over each block's interior range), shell
+// workgroups (256 physical-face nodes of one block each) and the bcond_lsm
+// face segments of each axis
+int build_reinit_tables(LsmBatch *B)
+{
+    std::vector<int32_t> rt, sh;
+    std::vector<HSeg> bs[3];
+    std::vector<int32_t> bw[3];
+    for (int ax = 0; ax < 3; ++ax) B->bc_wg[ax].assign(B->nb + 1, 0);
+    for (int b = 0; b < B->nb; ++b) {
+        BBlock &X = B->hb[b];
+        const Range &r = X.r;
+        const int kc = 8;
+        X.rt_kc = kc;
+        X.rt_tiles_i = (r.ihi + 1 + 63) / 64;
+        const int nj_int = r.jhi - r.jlo + 1, nk_int = r.khi - r.klo + 1;
+        X.rt_tiles_j = nj_int > 0 ? (nj_int + 3) / 4 : 0;
+        const int tk = nk_int > 0 ? (nk_int + kc - 1) / kc : 0;
+        X.rt_ntiles = (r.ihi >= r.ilo) ? X.rt_tiles_i * X.rt_tiles_j * tk : 0;
+        X.rt_tile0 = (int)rt.size();
+        for (int t = 0; t < X.rt_ntiles; ++t) rt.push_back(b);
+        X.sh_wg0 = (int)sh.size();
+        const long nsh = X.nI + X.nJ + X.nK;
+        for (long q = 0; q < (nsh + 255) / 256; ++q) sh.push_back(b);
+        // bcond faces: axis ax, low (slab 0) / high (slab 1), owned extents of the other axes
+        const int no[3] = {r.ib - r.ia + 1, r.jb - r.ja + 1, r.kb - r.ka + 1};
+        const bool fl[3] = {r.fb, r.fl, r.fd}, fh[3] = {r.ff, r.fr, r.fu};
+        for (int ax = 0; ax < 3; ++ax) {
+            B->bc_wg[ax][b] = (int)bw[ax].size();
+            const int o1 = ax == 0 ? 1 : 0, o2 = ax == 2 ? 1 : 2;
+            for (int side = 0; side < 2; ++side) {
+                if (!(side == 0 ? fl[ax] : fh[ax])) continue;
+                HSeg h{};
+                h.b = b; h.slab = side; h.n0 = no[o1]; h.n1 = no[o2];
+                h.count = no[o1] * no[o2];
+                h.wg0 = (int)bw[ax].size();
+                for (int q = 0; q < (h.count + 255) / 256; ++q) bw[ax].push_back((int32_t)bs[ax].size());
+                bs[ax].push_back(h);
+            }
+        }
+    }
+    for (int ax = 0; ax < 3; ++ax) B->bc_wg[ax][B->nb] = (int)bw[ax].size();
+    B->rt_ntiles = (int)rt.size();
+    B->sh_wgs = (int)sh.size();
+    int rc;
+    if ((rc = upload(rt, &B->d_rtile_block)) || (rc = upload(sh, &B->d_sh_block))) return rc;
+    for (int ax = 0; ax < 3; ++ax)
+        if ((rc = upload(bs[ax], &B->d_bseg[ax])) || (rc = upload(bw[ax], &B->d_bwg[ax]))) return rc;
+    return check_hip(hipMemcpy(B->db, B->hb.data(), B->hb.size() * sizeof(BBlock), hipMemcpyHostToDevice),
+                     "hipMemcpy");
+}
+
+int bcond_launch(LsmBatch *B, double *field, int b0, int nb, hipStream_t s)
+{
+    for (int ax = 0; ax < 3; ++ax) {
+        const int w0 = B->bc_wg[ax][b0], w1 = B->bc_wg[ax][b0 + nb];
+        if (w1 <= w0) continue;
+        BcSegArgs ba{B->db, B->d_bseg[ax], B->d_bwg[ax], w0, ax, field};
+        batch_bcond_phase<<<(unsigned)(w1 - w0), 256, 0, s>>>(ba);
+        int rc = check_hip(hipGetLastError(), "batch_bcond_phase");
+        if (rc) return rc;
+    }
+    return LSM_OK;
+}
+
+int ghost_copy_launch(LsmBatch *B, const double *src, double *dst, int b0, int nb, hipStream_t s)
+{
+    if (!B->connected || B->g == 0) return LSM_OK;
+    const int w0 = B->blk_wg[b0], w1 = B->blk_wg[b0 + nb];
+    if (w1 <= w0) return LSM_OK;
+    HaloArgs ha{};
+    ha.blocks = B->db; ha.segs = B->d_segs; ha.wg_seg = B->d_wg_seg; ha.wg_base = w0; ha.field = dst; ha.g = B->g;
+    batch_ghost_copy<<<(unsigned)(w1 - w0), 256, 0, s>>>(ha, src);
+    return check_hip(hipGetLastError(), "batch_ghost_copy");
+}
+
+template <int STAGE>
+int reinit_stage_launch(LsmBatch *B, BRArgs a, int b0, int nb, hipStream_t s)
+{
+    const int t0 = B->hb[b0].rt_tile0;
+    const int t1 = (b0 + nb < B->nb) ? B->hb[b0 + nb].rt_tile0 : B->rt_ntiles;
+    const int w0 = B->hb[b0].sh_wg0;
+    const int w1 = (b0 + nb < B->nb) ? B->hb[b0 + nb].sh_wg0 : B->sh_wgs;
+    a.tile_base = t0; a.ntiles = t1 - t0; a.sh_base = w0;
+    a.grid_tiles = (a.ntiles + 7) / 8 * 8;
+    const unsigned grid = (unsigned)(a.grid_tiles + (w1 - w0));
+    if (grid == 0) return LSM_OK;
+    batch_reinit_stage<STAGE><<<grid, dim3(64, 4, 1), 0, s>>>(a);
+    return check_hip(hipGetLastError(), "batch_reinit_stage");
+}
+
+size_t reinit_work_bytes(const LsmBatch *B)
+{
+    const size_t fb = ((size_t)B->total * sizeof(double) + 255) & ~(size_t)255;
+    const size_t cb = ((size_t)B->total + 255) & ~(size_t)255;
+    return 2 * fb + cb + (((size_t)B->nb * sizeof(RCtl) + 255) & ~(size_t)255) + 256;
+}
+
 const double kC[3][3] = {{1.0, 0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
 
 }  // namespace
@@ -475,6 +914,84 @@ int batch_rk3(LsmBatch *B, double *phi_np1, const double *phi_n, const double *U
     return LSM_OK;
 }
 
+// reinitialisation_test2 on every block (ghost layers of phi0 kept), phi_out
+// = result; blocks b0 .. b0+nb-1 per launch (all at once, or one by one in
+// the per-block form)
+int batch_reinit(LsmBatch *B, const LsmReinitParams &p, const double *phi0, double *out, int32_t *iters,
+                 void *work, hipStream_t s)
+{
+    const size_t fb = ((size_t)B->total * sizeof(double) + 255) & ~(size_t)255;
+    const size_t cb = ((size_t)B->total + 255) & ~(size_t)255;
+    char *w = (char *)work;
+    double *A = (double *)w, *Bf = (double *)(w + fb);
+    uint8_t *conv = (uint8_t *)(w + 2 * fb);
+    RCtl *ctl = (RCtl *)(w + 2 * fb + cb);
+    const int ghosted = (B->connected && B->g > 0) ? 1 : 0;
+    int rc;
+    if ((rc = launch_copy(phi0, out, B->total, s))) return rc;
+    if ((rc = ghost_copy_launch(B, phi0, A, 0, B->nb, s))) return rc;
+    if ((rc = ghost_copy_launch(B, phi0, Bf, 0, B->nb, s))) return rc;
+    batch_reinit_init<<<(B->nb + 63) / 64, 64, 0, s>>>(ctl, B->nb, ghosted, iters);
+    if ((rc = check_hip(hipGetLastError(), "batch_reinit_init"))) return rc;
+    if (p.rn_itermax <= 0) return LSM_OK;
+    BRArgs a{};
+    a.blocks = B->db; a.rtile_block = B->d_rtile_block; a.sh_block = B->d_sh_block;
+    for (int c = 0; c < 9; ++c) a.m[c] = B->met[c];
+    a.phi0 = phi0; a.conv = conv; a.ctl = ctl;
+    a.p = RStage{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
+    const bool per = B->tune.form == LSM_BATCH_PERBLOCK;
+    const int nl = per ? B->nb : 1, span = per ? 1 : B->nb;
+    a.din = phi0; a.dn = phi0; a.dout = nullptr;
+    for (int q = 0; q < nl; ++q)
+        if ((rc = reinit_stage_launch<0>(B, a, q * span, span, s))) return rc;
+    const double *sin_[3] = {out, A, Bf};
+    double *sout[3] = {A, Bf, out};
+    for (int it = 0; it < p.rn_itermax; ++it) {
+        batch_reinit_check<<<(B->nb + 63) / 64, 64, 0, s>>>(ctl, B->nb, ghosted, p.eps_reinit, iters);
+        for (int st = 0; st < 3; ++st) {
+            a.p = RStage{kC[st][0], kC[st][1], kC[st][2], p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
+            a.din = sin_[st]; a.dn = out; a.dout = sout[st];
+            KTimer kt(LSM_KT_BATCH_REINIT, s);
+            for (int q = 0; q < nl; ++q) {
+                if (st == 0) rc = reinit_stage_launch<1>(B, a, q * span, span, s);
+                else if (st == 1) rc = reinit_stage_launch<2>(B, a, q * span, span, s);
+                else rc = reinit_stage_launch<3>(B, a, q * span, span, s);
+                if (rc) return rc;
+            }
+        }
+    }
+    return LSM_OK;
+}
+
+int batch_bcond_halo(LsmBatch *B, double *field, hipStream_t s)
+{
+    int rc;
+    if (B->tune.form == LSM_BATCH_PERBLOCK) {
+        for (int b = 0; b < B->nb; ++b)
+            if ((rc = bcond_launch(B, field, b, 1, s))) return rc;
+        for (int b = 0; b < B->nb; ++b)
+            if ((rc = halo_launch(B, field, b, 1, s))) return rc;
+        return LSM_OK;
+    }
+    if ((rc = bcond_launch(B, field, 0, B->nb, s))) return rc;
+    return halo_launch(B, field, 0, B->nb, s);
+}
+
+// levelsetmethod.F90:221-326 for every block: ad_RKTVD3 (with bcond and halo
+// fill after each stage), reinitialisation_test2, bcond_lsm, halo fill
+int batch_step(LsmBatch *B, const LsmReinitParams *p, double *phi_np1, const double *phi_n, const double *U1,
+               const double *U2, const double *U3, double dt, int mode, void *work, hipStream_t s)
+{
+    const size_t fb = ((size_t)B->total * sizeof(double) + 255) & ~(size_t)255;
+    char *w = (char *)work;
+    double *scr = (double *)w, *X = (double *)(w + fb);
+    int rc;
+    if (!p) return batch_rk3(B, phi_np1, phi_n, U1, U2, U3, dt, mode, scr, s);
+    if ((rc = batch_rk3(B, X, phi_n, U1, U2, U3, dt, mode, scr, s))) return rc;
+    if ((rc = batch_reinit(B, *p, X, phi_np1, nullptr, w + 2 * fb, s))) return rc;
+    return batch_bcond_halo(B, phi_np1, s);
+}
+
 }  // namespace lsm
 
 using namespace lsm;
@@ -494,6 +1011,7 @@ int lsm_batch_create(const LsmBatchDesc *d, LsmBatch **out)
     LsmBatch *B = new LsmBatch();
     B->nb = d->nblocks; B->g = g; B->connected = conn ? 1 : 0; B->total = d->total;
     B->aj = d->aj;
+    for (int c = 0; c < 3; ++c) { B->met[c] = d->csi[c]; B->met[3 + c] = d->eta[c]; B->met[6 + c] = d->zet[c]; }
     B->desc.assign(d->blocks, d->blocks + d->nblocks);
     auto bad = [&](int code, const char *msg, int b) {
         delete B;
@@ -589,6 +1107,7 @@ int lsm_batch_create(const LsmBatchDesc *d, LsmBatch **out)
     }
     if ((rc = build_tiles(B, 64, 4, 8))) { lsm_batch_destroy(B); return rc; }
     if ((rc = build_halo(B))) { lsm_batch_destroy(B); return rc; }
+    if ((rc = build_reinit_tables(B))) { lsm_batch_destroy(B); return rc; }
     *out = B;
     return LSM_OK;
 }
@@ -602,6 +1121,12 @@ int lsm_batch_destroy(LsmBatch *B)
     if (B->d_tile_block) (void)hipFree(B->d_tile_block);
     if (B->d_block_of) (void)hipFree(B->d_block_of);
     if (B->d_segs) (void)hipFree(B->d_segs);
+    if (B->d_rtile_block) (void)hipFree(B->d_rtile_block);
+    if (B->d_sh_block) (void)hipFree(B->d_sh_block);
+    for (int ax = 0; ax < 3; ++ax) {
+        if (B->d_bseg[ax]) (void)hipFree(B->d_bseg[ax]);
+        if (B->d_bwg[ax]) (void)hipFree(B->d_bwg[ax]);
+    }
     if (B->d_wg_seg) (void)hipFree(B->d_wg_seg);
     if (B->d_qcount) (void)hipFree(B->d_qcount);
     delete B;
@@ -641,7 +1166,10 @@ int lsm_batch_get_config(const LsmBatch *B, int32_t cfg[5])
 size_t lsm_batch_workspace_bytes(const LsmBatch *B, int op)
 {
     if (!B) return 0;
-    if (op == LSM_OP_RK3) return ((size_t)B->total * sizeof(double) + 255) & ~(size_t)255;
+    const size_t fb = ((size_t)B->total * sizeof(double) + 255) & ~(size_t)255;
+    if (op == LSM_OP_RK3) return fb;
+    if (op == LSM_OP_REINIT_ENO2) return reinit_work_bytes(B);
+    if (op == LSM_OP_LEVELSET_STEP) return 2 * fb + reinit_work_bytes(B);
     return 0;
 }
 
@@ -651,18 +1179,17 @@ int lsm_batch_halo_fill(LsmBatch *B, double *field, void *stream)
     return halo_launch(B, field, 0, B->nb, (hipStream_t)stream);
 }
 
-int lsm_batch_contra_velocity(LsmBatch *B, const double *csi[3], const double *eta[3], const double *zet[3],
-                              const double *u, const double *v, const double *w, double *ucn1, double *ucn2,
-                              double *ucn3, void *stream)
+int lsm_batch_contra_velocity(LsmBatch *B, const double *u, const double *v, const double *w, double *ucn1,
+                              double *ucn2, double *ucn3, void *stream)
 {
-    if (!B || !csi || !eta || !zet || !u || !v || !w || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL argument");
+    if (!B || !u || !v || !w || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL argument");
     Blk b{};
-    b.csi0 = csi[0]; b.csi1 = csi[1]; b.csi2 = csi[2];
-    b.eta0 = eta[0]; b.eta1 = eta[1]; b.eta2 = eta[2];
-    b.zet0 = zet[0]; b.zet1 = zet[1]; b.zet2 = zet[2];
+    b.csi0 = B->met[0]; b.csi1 = B->met[1]; b.csi2 = B->met[2];
+    b.eta0 = B->met[3]; b.eta1 = B->met[4]; b.eta2 = B->met[5];
+    b.zet0 = B->met[6]; b.zet1 = B->met[7]; b.zet2 = B->met[8];
     b.aj = B->aj; b.n = B->total;
     for (int q = 0; q < 9; ++q)
-        if (!(&b.csi0)[q]) return fail(LSM_ERR_ARG, "NULL metric");
+        if (!(&b.csi0)[q]) return fail(LSM_ERR_ARG, "the batch was created without metrics");
     return launch_contra(b, u, v, w, 1, ucn1, ucn2, ucn3, (hipStream_t)stream);
 }
 
@@ -758,14 +1285,61 @@ int lsm_batch_autotune(LsmBatch *B, double *phi_np1, const double *phi_n, const 
 }
 
 // ---- graph of a full step -------------------------------------------------
-int lsm_batch_graph_capture(LsmBatch *B, double *phi_np1, const double *phi_n, const double *ucn1,
-                            const double *ucn2, const double *ucn3, double delti_lsm, int mode, void *work,
-                            size_t work_bytes)
+static int check_step_args(LsmBatch *B, const LsmReinitParams *p, double *phi_np1, const double *phi_n,
+                           const double *ucn1, const double *ucn2, const double *ucn3, int mode, void *work,
+                           size_t work_bytes)
 {
-    if (!B || !phi_np1 || !phi_n || !ucn1 || !ucn2 || !ucn3 || !work) return fail(LSM_ERR_ARG, "NULL argument");
+    if (!B || !phi_np1 || !phi_n || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL argument");
     if (phi_np1 == phi_n) return fail(LSM_ERR_ARG, "phi_np1 and phi_n must not alias");
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
-    if (work_bytes < lsm_batch_workspace_bytes(B, LSM_OP_RK3)) return fail(LSM_ERR_WORKSPACE, "workspace too small");
+    if (p) {
+        if (p->order_bc < 1 || p->order_bc > 3) return fail(LSM_ERR_ARG, "order_bc = %d (1, 2 or 3)", p->order_bc);
+        if (p->rn_itermax < 0) return fail(LSM_ERR_ARG, "rn_itermax < 0");
+        if (!(p->epslsm > 0.0)) return fail(LSM_ERR_ARG, "epslsm must be > 0");
+        for (int c = 0; c < 9; ++c)
+            if (!B->met[c]) return fail(LSM_ERR_ARG, "the batch was created without metrics");
+    }
+    if (!work) return fail(LSM_ERR_ARG, "NULL workspace");
+    const size_t need = lsm_batch_workspace_bytes(B, p ? LSM_OP_LEVELSET_STEP : LSM_OP_RK3);
+    if (work_bytes < need) return fail(LSM_ERR_WORKSPACE, "workspace of %zu bytes, %zu needed", work_bytes, need);
+    return LSM_OK;
+}
+
+int lsm_batch_reinit_sussman_eno2(LsmBatch *B, const LsmReinitParams *p, const double *phi0, double *phi_out,
+                                  int32_t *iters, void *work, size_t work_bytes, void *stream)
+{
+    if (!B || !p || !phi0 || !phi_out || !work) return fail(LSM_ERR_ARG, "NULL argument");
+    if (phi0 == phi_out) return fail(LSM_ERR_ARG, "phi0 and phi_out must not alias");
+    if (p->order_bc < 1 || p->order_bc > 3) return fail(LSM_ERR_ARG, "order_bc = %d (1, 2 or 3)", p->order_bc);
+    if (p->rn_itermax < 0) return fail(LSM_ERR_ARG, "rn_itermax < 0");
+    if (!(p->epslsm > 0.0)) return fail(LSM_ERR_ARG, "epslsm must be > 0");
+    for (int c = 0; c < 9; ++c)
+        if (!B->met[c]) return fail(LSM_ERR_ARG, "the batch was created without metrics");
+    if (work_bytes < reinit_work_bytes(B)) return fail(LSM_ERR_WORKSPACE, "workspace too small");
+    return batch_reinit(B, *p, phi0, phi_out, iters, work, (hipStream_t)stream);
+}
+
+int lsm_batch_bcond(LsmBatch *B, double *field, void *stream)
+{
+    if (!B || !field) return fail(LSM_ERR_ARG, "NULL argument");
+    return bcond_launch(B, field, 0, B->nb, (hipStream_t)stream);
+}
+
+int lsm_batch_levelset_step(LsmBatch *B, const LsmReinitParams *p, double *phi_np1, const double *phi_n,
+                            const double *ucn1, const double *ucn2, const double *ucn3, double delti_lsm, int mode,
+                            void *work, size_t work_bytes, void *stream)
+{
+    int rc = check_step_args(B, p, phi_np1, phi_n, ucn1, ucn2, ucn3, mode, work, work_bytes);
+    if (rc) return rc;
+    return batch_step(B, p, phi_np1, phi_n, ucn1, ucn2, ucn3, delti_lsm, mode, work, (hipStream_t)stream);
+}
+
+int lsm_batch_graph_capture(LsmBatch *B, const LsmReinitParams *p, double *phi_np1, const double *phi_n,
+                            const double *ucn1, const double *ucn2, const double *ucn3, double delti_lsm, int mode,
+                            void *work, size_t work_bytes)
+{
+    int rc0 = check_step_args(B, p, phi_np1, phi_n, ucn1, ucn2, ucn3, mode, work, work_bytes);
+    if (rc0) return rc0;
     if (timing_on()) return fail(LSM_ERR_UNSUPPORTED, "switch launch timing off before a graph capture");
     int rc;
     hipStream_t cs;
@@ -775,7 +1349,7 @@ int lsm_batch_graph_capture(LsmBatch *B, double *phi_np1, const double *phi_n, c
     rc = check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     int rc2 = LSM_OK;
     if (!rc) {
-        rc2 = batch_rk3(B, phi_np1, phi_n, ucn1, ucn2, ucn3, delti_lsm, mode, (double *)work, cs);
+        rc2 = batch_step(B, p, phi_np1, phi_n, ucn1, ucn2, ucn3, delti_lsm, mode, work, cs);
         hipGraph_t g = nullptr;
         rc = check_hip(hipStreamEndCapture(cs, &g), "hipStreamEndCapture");
         if (!rc && !rc2) {

@@ -48,7 +48,7 @@ class LsmBatchBlock(C.Structure):
 class LsmBatchDesc(C.Structure):
     _fields_ = [("nblocks", C.c_int32), ("ghost", C.c_int32), ("connected", C.c_int32),
                 ("lattice", C.c_int32 * 3), ("blocks", C.POINTER(LsmBatchBlock)), ("aj", _vp),
-                ("total", C.c_int64)]
+                ("csi", _vp * 3), ("eta", _vp * 3), ("zet", _vp * 3), ("total", C.c_int64)]
 
 
 LSM_BATCH_QUEUE, LSM_BATCH_GRID, LSM_BATCH_PERBLOCK = 0, 1, 2
@@ -62,11 +62,17 @@ _SIGS = {
     "lsm_batch_get_config": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
     "lsm_batch_workspace_bytes": (C.c_size_t, [_vp, C.c_int]),
     "lsm_batch_halo_fill": (C.c_int, [_vp, _vp, _vp]),
-    "lsm_batch_contra_velocity": (C.c_int, [_vp, _pp, _pp, _pp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lsm_batch_contra_velocity": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "lsm_batch_reinit_sussman_eno2": (C.c_int, [_vp, C.POINTER(LsmReinitParams), _vp, _vp, _vp, _vp, C.c_size_t,
+                                                _vp]),
+    "lsm_batch_bcond": (C.c_int, [_vp, _vp, _vp]),
+    "lsm_batch_levelset_step": (C.c_int, [_vp, C.POINTER(LsmReinitParams), _vp, _vp, _vp, _vp, _vp, C.c_double,
+                                          C.c_int, _vp, C.c_size_t, _vp]),
     "lsm_batch_rk3_step": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, C.c_size_t, _vp]),
     "lsm_batch_autotune": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, C.c_size_t, C.c_int,
                                      C.c_int, C.POINTER(C.c_double), _vp]),
-    "lsm_batch_graph_capture": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int, _vp, C.c_size_t]),
+    "lsm_batch_graph_capture": (C.c_int, [_vp, C.POINTER(LsmReinitParams), _vp, _vp, _vp, _vp, _vp, C.c_double,
+                                          C.c_int, _vp, C.c_size_t]),
     "lsm_batch_graph_launch": (C.c_int, [_vp, _vp]),
     "lsm_version": (C.c_char_p, []),
     "lsm_last_error": (C.c_char_p, []),
@@ -123,6 +129,7 @@ EXPORTS = tuple(_SIGS)
 LSM_KT_ADVECT_STAGE, LSM_KT_ADVECT_SHELL, LSM_KT_REINIT_STAGE, LSM_KT_VOLUME = 0, 1, 2, 3
 LSM_KT_ADJOINT_STAGE = 4
 LSM_KT_BATCH_STAGE = 5
+LSM_KT_BATCH_REINIT = 6
 
 _lib = None
 

@@ -259,6 +259,9 @@ class Batch:
         d.lattice[:] = list(lay.lattice)
         d.blocks = C.cast(arr, C.POINTER(LsmBatchBlock))
         d.aj = self.aj.data_ptr()
+        d.csi[:] = [t.data_ptr() for t in self.csi]
+        d.eta[:] = [t.data_ptr() for t in self.eta]
+        d.zet[:] = [t.data_ptr() for t in self.zet]
         d.total = lay.total
         self._arr = arr
         h = C.c_void_p()
@@ -329,11 +332,48 @@ class Batch:
     def contra_velocity(self, u, v, w):
         from .api import _stream
         out = [self.empty() for _ in range(3)]
-        arr = lambda ts: (C.c_void_p * 3)(*[t.data_ptr() for t in ts])  # noqa: E731
-        check(_lib.load().lsm_batch_contra_velocity(self.handle, arr(self.csi), arr(self.eta), arr(self.zet),
-                                                    self._check(u, "u"), self._check(v, "v"), self._check(w, "w"),
-                                                    *[C.c_void_p(t.data_ptr()) for t in out], _stream()),
-              "lsm_batch_contra_velocity")
+        check(_lib.load().lsm_batch_contra_velocity(self.handle, self._check(u, "u"), self._check(v, "v"),
+                                                    self._check(w, "w"), *[C.c_void_p(t.data_ptr()) for t in out],
+                                                    _stream()), "lsm_batch_contra_velocity")
+        return out
+
+    def reinit(self, phi0, params, out=None, work=None):
+        """reinitialisation_test2 on every block (ghost layers frozen);
+        returns (phi_out, iterations per block as an int32 CUDA tensor)."""
+        import torch
+        from .api import _stream
+        if out is None:
+            out = self.empty()
+        if work is None:
+            work = self.workspace(_lib.LSM_OP_REINIT_ENO2)
+        iters = torch.zeros(len(self.lay.blocks), dtype=torch.int32, device=self.device)
+        p = params.c()
+        check(_lib.load().lsm_batch_reinit_sussman_eno2(self.handle, C.byref(p), self._check(phi0, "phi0"),
+                                                        self._check(out, "out"), C.c_void_p(iters.data_ptr()),
+                                                        C.c_void_p(work.data_ptr()), work.numel(), _stream()),
+              "lsm_batch_reinit_sussman_eno2")
+        return out, iters
+
+    def bcond(self, field):
+        from .api import _stream
+        check(_lib.load().lsm_batch_bcond(self.handle, self._check(field, "field"), _stream()), "lsm_batch_bcond")
+        return field
+
+    def levelset_step(self, phi_n, ucn, delti_lsm, params=None, out=None, mode="fast", work=None):
+        """ad_RKTVD3 (+ reinitialisation_test2, bcond_lsm, halo fill when
+        params is given) for every block."""
+        from .api import MODES, _stream
+        if out is None:
+            out = self.empty()
+        op = _lib.LSM_OP_LEVELSET_STEP if params is not None else _lib.LSM_OP_RK3
+        if work is None:
+            work = self.workspace(op)
+        p = params.c() if params is not None else None
+        check(_lib.load().lsm_batch_levelset_step(self.handle, C.byref(p) if p is not None else None,
+                                                  self._check(out, "out"), self._check(phi_n, "phi_n"),
+                                                  *[self._check(t, "ucn") for t in ucn], float(delti_lsm), MODES[mode],
+                                                  C.c_void_p(work.data_ptr()), work.numel(), _stream()),
+              "lsm_batch_levelset_step")
         return out
 
     def workspace(self, op=_lib.LSM_OP_RK3):
@@ -371,17 +411,20 @@ class Batch:
                                              C.byref(ms), _stream()), "lsm_batch_autotune")
         return self.get_config(), (ms.value if ms.value >= 0 else None)
 
-    def graph_capture(self, phi_np1, phi_n, ucn, delti_lsm, mode="fast", work=None):
-        """Records one rk3_step with these tensors into a HIP graph kept by
-        the batch (the tensors must stay alive while the graph is used)."""
+    def graph_capture(self, phi_np1, phi_n, ucn, delti_lsm, params=None, mode="fast", work=None):
+        """Records one levelset_step with these tensors into a HIP graph kept
+        by the batch (the tensors must stay alive while the graph is used)."""
         from .api import MODES
+        op = _lib.LSM_OP_LEVELSET_STEP if params is not None else _lib.LSM_OP_RK3
         if work is None:
-            work = self.workspace()
-        self._graph_keep = (phi_np1, phi_n, tuple(ucn), work)
-        check(_lib.load().lsm_batch_graph_capture(self.handle, self._check(phi_np1, "phi_np1"),
-                                                  self._check(phi_n, "phi_n"), *[self._check(t, "ucn") for t in ucn],
-                                                  float(delti_lsm), MODES[mode], C.c_void_p(work.data_ptr()),
-                                                  work.numel()), "lsm_batch_graph_capture")
+            work = self.workspace(op)
+        p = params.c() if params is not None else None
+        self._graph_keep = (phi_np1, phi_n, tuple(ucn), work, p)
+        check(_lib.load().lsm_batch_graph_capture(self.handle, C.byref(p) if p is not None else None,
+                                                  self._check(phi_np1, "phi_np1"), self._check(phi_n, "phi_n"),
+                                                  *[self._check(t, "ucn") for t in ucn], float(delti_lsm), MODES[mode],
+                                                  C.c_void_p(work.data_ptr()), work.numel()),
+              "lsm_batch_graph_capture")
 
     def graph_launch(self):
         from .api import _stream

@@ -223,3 +223,112 @@ def test_graph_replays_and_autotune():
     bt2 = B.Batch(lay, grids)
     cfg2, ms2 = bt2.autotune(p0, ucn, dt)
     assert cfg2 == cfg and ms2 is None
+
+
+def _params(epslsm, dtau, nit=4, eps=1e-9, order=1, eno=False):
+    from lsm_amd import api
+    return api.ReinitParams(epslsm=epslsm, RNtimestep=dtau, RNitermax=nit, epsReinitialisation=eps,
+                            OrderReinitialisationBoundaries=order, ENOBCReinitialisation=eno)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", ["grid", "perblock"])
+def test_independent_batch_reinit_matches_each_instance(form):
+    """Each instance's reinitialisation (own exit test: the planar instance
+    passes it at once and runs no iteration) equals the single-block call."""
+    from lsm_amd import api
+    shapes = [(12, 9, 10), (17, 6, 8), (9, 14, 7), (12, 10, 9)]
+    lay = B.independent_layout(shapes)
+    grids, phis = [], []
+    for q, (ni, nj, nk) in enumerate(shapes):
+        gd = inputs.sinusoidal_cube_grid(ni, amp=0.02 if q == 3 else 0.04, seed=70 + q, nj=nj, nk=nk)
+        grids.append(gd)
+        if q == 3:
+            phis.append((gd.z - 0.45 - 0.1 * gd.x) * 1.05)
+        else:
+            phis.append(inputs.spheres_sdf(gd, nspheres=3, seed=80 + q, rmin=0.2, rmax=0.35) * (0.5 + (gd.x - 0.3) ** 2))
+    hmin = min(inputs.spacing_bounds(gd)[0] for gd in grids)
+    hmax = max(inputs.spacing_bounds(gd)[1] for gd in grids)
+    prm = _params(1.5 * hmax, 0.5 * hmin, nit=4, eps=0.1, order=2)
+    bt = B.Batch(lay, grids)
+    bt.set_config(form)
+    out, iters = bt.reinit(bt.pack(phis), prm)
+    got = bt.unpack(out)
+    its = iters.cpu().numpy()
+    assert its[3] == 0 and all(its[:3] == 4), its
+    for q, (gd, p) in enumerate(zip(grids, phis)):
+        blk = api.Block(gd)
+        ref, it = api.reinitialisation_test2(blk, _dev(p), prm)
+        assert int(it.cpu()[0]) == its[q]
+        assert np.array_equal(got[q], ref.cpu().numpy()), q
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ghost,order,eno", [(3, 1, False), (2, 2, True), (3, 3, False)])
+def test_connected_batch_reinit_matches_blockwise_reference(ghost, order, eno):
+    """The reinitialisation of a connected batch is the reference's per-rank
+    loop (ghost layers frozen, ENO2 through the ghosts at internal faces,
+    one-sided formulas at physical faces), checked against the FP64 PyTorch
+    re-expression of it block by block (1e-13: device sin/cos)."""
+    import torch_ref as T
+    lay, grids, phi, vel, _ = _case(ghost=ghost, seed=14)
+    phi = [p * (0.5 + (gd.x - 0.3) ** 2) for p, gd in zip(phi, grids)]
+    hmin = min(inputs.spacing_bounds(gd)[0] for gd in grids)
+    hmax = max(inputs.spacing_bounds(gd)[1] for gd in grids)
+    prm = _params(1.5 * hmax, 0.5 * hmin, nit=3, order=order, eno=eno)
+    bt = B.Batch(lay, grids)
+    out, iters = bt.reinit(bt.halo_fill(bt.pack(phi)), prm)
+    assert np.all(iters.cpu().numpy() == 3)
+    got = bt.unpack(out)
+    phi_h = bt.unpack(bt.halo_fill(bt.pack(phi)))
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64)  # noqa: E731
+    for b, (gd, s) in enumerate(zip(grids, lay.blocks)):
+        phys = tuple(bool(s.phys_faces >> q & 1) for q in range(6))
+        met = ([t(c) for c in gd.csi], [t(c) for c in gd.eta], [t(c) for c in gd.zet])
+        ref, it = T.reinit(t(phi_h[b]), met, (1.0, 1.0, 1.0), prm.epslsm, 1e-9, prm.RNtimestep, 3, order, eno,
+                           ghost=ghost, phys=phys)
+        assert it == 3
+        r = ref.numpy()
+        assert np.max(np.abs(got[b] - r)) <= 1e-13 * np.max(np.abs(r)), b
+        g = ghost
+        assert np.array_equal(got[b][:g], phi_h[b][:g])   # ghost layers untouched
+
+
+@pytest.mark.gpu
+def test_connected_batch_bcond_and_step_composition():
+    """bcond_lsm of a connected batch equals the single-block bcond_lsm on the
+    assembled grid; the full step equals rk3 -> reinit -> bcond -> halo fill;
+    a graph of the full step replays it."""
+    from lsm_amd import api
+    btype = (1, 0, 1, 2, 1, 1)
+    lay, grids, phi, vel, _ = _case(btype=btype, seed=15)
+    bt = B.Batch(lay, grids)
+    f = bt.halo_fill(bt.pack(phi))
+    got = B.assemble(lay, bt.unpack(bt.bcond(f.clone())))
+    g = inputs.Grid(*[B.assemble(lay, [getattr(gd, c) for gd in grids]) for c in "xyz"])
+    g.aj = B.assemble(lay, [gd.aj for gd in grids])
+    for name in ("csi", "eta", "zet"):
+        setattr(g, name, np.stack([B.assemble(lay, [getattr(gd, name)[c] for gd in grids]) for c in range(3)]))
+    blk = api.Block(g, btype=btype)
+    ref = api.bcond_lsm(blk, _dev(B.assemble(lay, phi))).cpu().numpy()
+    assert np.array_equal(got, ref)
+    dt = B.cfl_dt(lay, grids, vel)
+    hmin = min(inputs.spacing_bounds(gd)[0] for gd in grids)
+    hmax = max(inputs.spacing_bounds(gd)[1] for gd in grids)
+    prm = _params(1.5 * hmax, 0.5 * hmin, nit=4)
+    ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vel]) for c in range(3)])
+    step = bt.levelset_step(f, ucn, dt, prm)
+    x = bt.rk3_step(f, ucn, dt)
+    y, _ = bt.reinit(x, prm)
+    comp = bt.halo_fill(bt.bcond(y))
+    own = lambda t: B.assemble(lay, bt.unpack(t))  # noqa: E731
+    assert np.array_equal(own(step), own(comp))
+    out = bt.empty()
+    bt.graph_capture(out, f, ucn, dt, params=prm)
+    for r in range(3):
+        out.fill_(0.0)
+        bt.graph_launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(own(out), own(comp)), r
+        bt.levelset_step(f, ucn, dt, prm)
+        torch.cuda.synchronize()

@@ -151,38 +151,50 @@ def biased(f0, f1, f2, f3, order, as_, enobc):
     return as_ * (((-(11.0 * f0) + 18.0 * f1) - 9.0 * f2) + 2.0 * f3) / 6.0
 
 
-def normgrad(d, phi0, metrics, dd, order_bc=1, eno_bc=False):
-    """calc_NormGrad_d on every node; metrics = (csi, eta, zet), each three arrays."""
+def normgrad(d, phi0, metrics, dd, order_bc=1, eno_bc=False, ghost=0, phys=(True,) * 6):
+    """calc_NormGrad_d on the owned nodes of one block (0 on ghost nodes):
+    `ghost` layers on every face, phys = physical flags of the faces (i low,
+    i high, j low, j high, k low, k high); one-sided formulas on physical
+    faces, ENO2 elsewhere (ghost values in the stencil at internal faces)."""
     nk, nj, ni = d.shape
     dev = d.device
+    g = ghost
     ii = torch.arange(ni, device=dev).view(1, 1, -1).expand(d.shape)
     jj = torch.arange(nj, device=dev).view(1, -1, 1).expand(d.shape)
     kk = torch.arange(nk, device=dev).view(-1, 1, 1).expand(d.shape)
     idx = (ii, jj, kk)
     n = (ni, nj, nk)
-    on = [(idx[a] == 0) | (idx[a] == n[a] - 1) for a in range(3)]
-    stale = phi0[nk - 2, nj - 2, ni - 2]
+    onl = [(idx[a] == g) & phys[2 * a] for a in range(3)]
+    onh = [(idx[a] == n[a] - 1 - g) & phys[2 * a + 1] for a in range(3)]
+    on = [onl[a] | onh[a] for a in range(3)]
+    last = [n[a] - 1 - g - (1 if phys[2 * a + 1] else 0) for a in range(3)]
+    stale = phi0[last[2], last[1], last[0]]
     p0 = torch.where(on[0] & ~on[1] & ~on[2], stale.expand(d.shape), phi0)
     zero = torch.zeros_like(d)
     gr = []
     for a, axis in ((0, 2), (1, 1), (2, 0)):
         sh = lambda o: _shift(d, axis, o)  # noqa: E731
-        lo, hi = idx[a] == 1, idx[a] == n[a] - 2
+        lo = (idx[a] == g + 1) & phys[2 * a]
+        hi = (idx[a] == n[a] - 2 - g) & phys[2 * a + 1]
         inner = dd[a] * eno2(p0, torch.where(lo, zero, sh(-2)), sh(-1), d, sh(1), torch.where(hi, zero, sh(2)),
                              lo, hi)
         f_lo = dd[a] * biased(d, sh(1), sh(2), sh(3), order_bc, 1.0, eno_bc)
         f_hi = dd[a] * biased(d, sh(-1), sh(-2), sh(-3), order_bc, -1.0, eno_bc)
-        g = torch.where(idx[a] == 0, f_lo, torch.where(idx[a] == n[a] - 1, f_hi, inner))
+        gv = torch.where(onl[a], f_lo, torch.where(onh[a], f_hi, inner))
         if a == 0:
             dc2 = 0.5 * dd[0]
             edge = dc2 * ((-(1.0 * sh(2)) + 4.0 * sh(1)) - 3.0 * d)
-            g = torch.where((ii == 0) & (kk == 0) & ~on[1], edge, g)
-        gr.append(g)
+            gv = torch.where(onl[0] & onl[2] & ~on[1], edge, gv)
+        gr.append(gv)
     csi, eta, zet = metrics
     gx = (csi[0] * gr[0] + eta[0] * gr[1]) + zet[0] * gr[2]
     gy = (csi[1] * gr[0] + eta[1] * gr[1]) + zet[1] * gr[2]
     gz = (csi[2] * gr[0] + eta[2] * gr[1]) + zet[2] * gr[2]
-    return torch.sqrt((gx * gx + gy * gy) + gz * gz)
+    ng = torch.sqrt((gx * gx + gy * gy) + gz * gz)
+    if g == 0:
+        return ng
+    own = (ii >= g) & (ii <= ni - 1 - g) & (jj >= g) & (jj <= nj - 1 - g) & (kk >= g) & (kk <= nk - 1 - g)
+    return torch.where(own, ng, torch.zeros_like(ng))
 
 
 def sign_phi0(p, eps):
@@ -202,20 +214,30 @@ def lambda_mass(p, s, g, eps):
     return torch.where(small, torch.zeros_like(den), num / torch.where(small, torch.ones_like(den), den))
 
 
-def reinit(phi0, metrics, dd, epslsm, eps_reinit, dtau, itermax, order_bc=1, eno_bc=False):
-    """reinitialisation_test2 (non-hybrid): returns (phi_out, iterations)."""
+def reinit(phi0, metrics, dd, epslsm, eps_reinit, dtau, itermax, order_bc=1, eno_bc=False, ghost=0,
+           phys=(True,) * 6):
+    """reinitialisation_test2 (non-hybrid) on one block: returns (phi_out,
+    iterations).  With ghost layers (one rank of a decomposition) only owned
+    nodes are updated and the ghost values stay those of phi0."""
     s = sign_phi0(phi0, epslsm)
     out = phi0
-    ng = normgrad(out, phi0, metrics, dd, order_bc, eno_bc)
+    kw = dict(ghost=ghost, phys=phys)
+    if ghost:
+        nk, nj, ni = phi0.shape
+        own = torch.zeros(phi0.shape, dtype=torch.bool, device=phi0.device)
+        own[ghost:nk - ghost, ghost:nj - ghost, ghost:ni - ghost] = True
+    ng = normgrad(out, phi0, metrics, dd, order_bc, eno_bc, **kw)
     it = 0
     while it < itermax:
         mx, mn = float(ng.detach().max()), float(ng.detach().min())
         if abs(mx - 1.0) < eps_reinit and abs(mn - 1.0) < eps_reinit:
             break
         dn = out
-        upd = (normgrad(dn, phi0, metrics, dd, order_bc, eno_bc) - 1.0 > eps_reinit).detach()
+        upd = (normgrad(dn, phi0, metrics, dd, order_bc, eno_bc, **kw) - 1.0 > eps_reinit).detach()
+        if ghost:
+            upd = upd & own
         for c1, c2, c3 in C:
-            ng = normgrad(out, phi0, metrics, dd, order_bc, eno_bc)
+            ng = normgrad(out, phi0, metrics, dd, order_bc, eno_bc, **kw)
             rhs = s * (ng - 1.0)
             lam = lambda_mass(out, s, ng, epslsm)
             aux = deltaf(out, epslsm) * ng
