@@ -90,6 +90,156 @@ def cpu_baseline(n_ij=256, nk=64, steps=6):
             "seconds": secs, "sample": sample + ", C restatement of the oracle (gcc -O2, 1 thread)"}
 
 
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def _time(fn, reps, warm=2):
+    import torch
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = _events()
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_step_adjoint(g, phi, ucn, dt, blk):
+    """One full level-set step (ad_RKTVD3, 4 Sussman iterations, bcond_lsm)
+    and its hand-written adjoint (lsm_levelset_step_adjoint: forward
+    recompute with the reinitialisation trace, then the reverse sweep) at the
+    bench size."""
+    from lsm_amd import api, inputs, _lib
+    hmin, hmax = inputs.spacing_bounds(g)
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4, epsReinitialisation=1e-12)
+    wrk, wrn = blk.workspace(_lib.LSM_OP_RK3), blk.workspace(_lib.LSM_OP_REINIT_ENO2)
+    wad = blk.adjoint_workspace(_lib.LSM_OP_STEP_ADJOINT, 4)
+    tmp, out, adj = blk.empty(), blk.empty(), blk.empty()
+    lam = phi.clone()
+    aU = [blk.zeros() for _ in range(3)]
+
+    def fwd():
+        x = api.ad_RKTVD3(blk, phi, ucn, dt, mode="fast", out=tmp, work=wrk)
+        api.reinitialisation_test2(blk, x, prm, out=out, work=wrn)
+        api.bcond_lsm(blk, out)
+
+    def bwd():
+        api.levelset_step_adjoint(blk, phi, ucn, dt, prm, lam, mode="fast", adj_ucn=aU, out=adj, work=wad)
+    f_ms, b_ms = _time(fwd, 5), _time(bwd, 5)
+    n_int = (g.ni - 2) * (g.nj - 2) * (g.nk - 2)
+    return {"step_fwd_ms": round(f_ms, 4), "step_adjoint_ms": round(b_ms, 4),
+            "step_adjoint_forward_ratio": round(b_ms / f_ms, 3),
+            "step_adjoint_cell_updates_per_s": n_int / (b_ms / 1e3),
+            "step_definition": "ad_RKTVD3 + 4 reinitialisation_test2 iterations + bcond_lsm (levelsetmethod, "
+                               "numiter_lsm = 1); adjoint = forward recompute incl. the reinitialisation trace + "
+                               "reverse sweep, gradients w.r.t. phi^n and U/J"}
+
+
+def bench_batch(steps=10):
+    """The 64-block workload: a 4x4x4 lattice of curvilinear blocks, sizes
+    U{32..160} per lattice slab (seed 6), per-block sinusoidal perturbation
+    amplitudes U[0, 0.04], 3 ghost layers, 32 spheres, divergence-free Fourier
+    velocity; per-block plain launches against the batched form (autotuned
+    stage tiles, hipGraph per step), advection-only and full steps (3 RK
+    stages + 4 Sussman iterations + bcond + halo fill)."""
+    import torch
+    from lsm_amd import api, batch as B, inputs
+    t0 = time.time()
+    lay, grids, phi, vel, _ = B.multiblock_case()
+    dt = B.cfl_dt(lay, grids, vel)
+    hmin = min(inputs.spacing_bounds(gd)[0] for gd in grids)
+    hmax = max(inputs.spacing_bounds(gd)[1] for gd in grids)
+    bt = B.Batch(lay, grids)
+    ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vel]) for c in range(3)])
+    p0 = bt.halo_fill(bt.pack(phi))
+    del grids, vel, phi
+    log(f"batch inputs: {len(lay.blocks)} blocks, {lay.cells()} cells, {time.time() - t0:.1f}s")
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4, epsReinitialisation=1e-12)
+    out = bt.empty()
+    w_adv, w_full = bt.workspace(), bt.workspace(api._lib.LSM_OP_LEVELSET_STEP)
+    res = {"workload": "64 blocks (4x4x4 lattice, sizes U{32..160}, 3 ghost layers), 32 spheres, "
+                       "16-mode Fourier velocity, FP64", "cells": lay.cells(), "sizes": lay.sizes}
+    bt.set_config("perblock")
+    res["perblock_adv_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, out=out, work=w_adv), steps), 4)
+    res["perblock_full_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
+    cfg, ms = bt.autotune(p0, ucn, dt, reps=3, force=True, out=out, work=w_adv)
+    res["autotuned"] = cfg
+    res["batched_adv_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, out=out, work=w_adv), steps), 4)
+    res["batched_full_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
+    bt.graph_capture(out, p0, ucn, dt, work=w_adv)
+    res["graph_adv_ms"] = round(_time(bt.graph_launch, steps), 4)
+    bt.graph_capture(out, p0, ucn, dt, params=prm, work=w_full)
+    res["graph_full_ms"] = round(_time(bt.graph_launch, steps), 4)
+    for kind in ("adv", "full"):
+        best = min(res[f"batched_{kind}_ms"], res[f"graph_{kind}_ms"])
+        res[f"{kind}_cell_updates_per_s_batched"] = lay.cells() / (best / 1e3)
+        res[f"{kind}_cell_updates_per_s_perblock"] = lay.cells() / (res[f"perblock_{kind}_ms"] / 1e3)
+        res[f"{kind}_speedup_vs_perblock"] = round(res[f"perblock_{kind}_ms"] / best, 3)
+    bt.set_config("queue", 8, 8, 0)
+    res["queue_adv_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, out=out, work=w_adv), steps), 4)
+    log(f"batch: adv {res['perblock_adv_ms']} -> {res['graph_adv_ms']} ms ({res['adv_speedup_vs_perblock']}x), "
+        f"full {res['perblock_full_ms']} -> {res['graph_full_ms']} ms ({res['full_speedup_vs_perblock']}x)")
+    del bt, ucn, p0, out, w_adv, w_full
+    return res
+
+
+def bench_differentiation(n=192, nsteps=20, every=5):
+    """The differentiation workload at n^3: nsteps full steps (RK3 + 4 Sussman
+    iterations + bcond), checkpoint every `every` steps, loss J = sum_cells
+    V (H_eps(phi_T) - H_eps(phi_target))^2 (V = 1/aj, eps = 1.5 max spacing,
+    phi_target: the same run from perturbed sphere radii), gradients with
+    respect to phi0 and the 16 velocity Fourier amplitudes."""
+    import numpy as np
+    import torch
+    from lsm_amd import api, autograd, inputs
+    t0 = time.time()
+    g = inputs.sinusoidal_cube_grid(n, amp=0.03, seed=2)
+    phi = inputs.spheres_sdf(g, nspheres=8, seed=3)
+    phi_t = inputs.spheres_sdf(g, nspheres=8, seed=3, rmin=0.055, rmax=0.155)
+    (u, v, w), modes = inputs.fourier_velocity(g, seed=4, return_modes=True)
+    dt = inputs.cfl_dt(g, u, v, w)
+    hmin, hmax = inputs.spacing_bounds(g)
+    blk = api.Block(g)
+    log(f"differentiation inputs {n}^3: {time.time() - t0:.1f}s")
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4, epsReinitialisation=1e-12)
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
+    ks, es, amps, psis = modes
+    heps = 1.5 * hmax
+    vol = 1.0 / blk.aj
+    with torch.no_grad():
+        Ut = autograd.contravariant(blk, *autograd.fourier_velocity(blk, ks, es, dev(amps), psis))
+        target = autograd.levelset_trajectory(blk, dev(phi_t), Ut, dt, prm, nsteps, mode="fast",
+                                              checkpoint_every=every)
+        Ht = autograd.heaviside(target, heps)
+
+    def forward():
+        with torch.no_grad():
+            U = autograd.contravariant(blk, *autograd.fourier_velocity(blk, ks, es, dev(amps), psis))
+            out = autograd.levelset_trajectory(blk, dev(phi), U, dt, prm, nsteps, mode="fast", checkpoint_every=every)
+            return (vol * (autograd.heaviside(out, heps) - Ht) ** 2).sum()
+
+    def gradient():
+        p = dev(phi).requires_grad_(True)
+        a = dev(amps).requires_grad_(True)
+        U = autograd.contravariant(blk, *autograd.fourier_velocity(blk, ks, es, a, psis))
+        out = autograd.levelset_trajectory(blk, p, U, dt, prm, nsteps, mode="fast", checkpoint_every=every)
+        J = (vol * (autograd.heaviside(out, heps) - Ht) ** 2).sum()
+        J.backward()
+        return p.grad, a.grad
+    f_ms = _time(forward, 2, warm=1)
+    g_ms = _time(gradient, 2, warm=1)
+    return {"workload": f"{n}^3 block, {nsteps} steps x (RK3 + 4 reinitialisation iterations + bcond), checkpoint "
+                        f"every {every}, smoothed-Heaviside misfit, gradients w.r.t. phi0 and 16 Fourier amplitudes",
+            "forward_ms": round(f_ms, 2), "forward_plus_gradient_ms": round(g_ms, 2),
+            "gradient_over_forward": round(g_ms / f_ms, 3),
+            "backward_over_forward": round((g_ms - f_ms) / f_ms, 3)}
+
+
 def load_traffic(kernel):
     """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.json)."""
     p = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
@@ -114,6 +264,9 @@ def main():
                          "folded: frozen-velocity operator aj*dt*d*ucn (3 fields per stage)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-block batched workload")
+    ap.add_argument("--no-diff", action="store_true", help="skip the 192^3 differentiation workload")
+    ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scales rocprof "
                          "FETCH_SIZE/WRITE_SIZE to bytes)")
@@ -305,6 +458,19 @@ def main():
         extra["levelsetmethod_ms"] = round(ls_ms, 3)
         extra["levelsetmethod_cells_per_s"] = interior / (ls_ms / 1e3)
         log(f"levelsetmethod (4 reinit iterations): {ls_ms:.2f} ms per call")
+
+    if not args.no_extra:
+        extra.update(bench_step_adjoint(g, bufs[base], ucn, dt, blk))
+        log(f"full step: fwd {extra['step_fwd_ms']:.3f} ms, adjoint {extra['step_adjoint_ms']:.3f} ms "
+            f"({extra['step_adjoint_forward_ratio']:.2f}x)")
+    del work, V
+    torch.cuda.empty_cache()
+    if not args.no_extra and not args.no_batch:
+        extra["batch"] = bench_batch(args.batch_steps)
+        torch.cuda.empty_cache()
+    if not args.no_extra and not args.no_diff:
+        extra["differentiation"] = bench_differentiation()
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
