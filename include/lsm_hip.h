@@ -62,6 +62,8 @@ extern "C" {
 #define LSM_OP_RK3_ADJOINT    6
 #define LSM_OP_REINIT_ENO2_ADJOINT 7   /* sized by lsm_adjoint_workspace_bytes */
 #define LSM_OP_STEP_ADJOINT   8        /* sized by lsm_adjoint_workspace_bytes */
+#define LSM_OP_STEP_TRACE     9        /* sized by lsm_adjoint_workspace_bytes */
+#define LSM_OP_STEP_ADJOINT_RECORDED 10 /* sized by lsm_adjoint_workspace_bytes */
 
 /* One curvilinear block (the host-associated state of levelsetmethod.F90:1-44). */
 typedef struct LsmBlock {
@@ -354,6 +356,25 @@ int lsm_batch_graph_capture(LsmBatch *batch, const LsmReinitParams *p, double *p
                             const double *ucn3, double delti_lsm, int mode, void *work,
                             size_t work_bytes);
 int lsm_batch_graph_launch(LsmBatch *batch, void *stream);
+
+/* The same step split in two for checkpointed trajectories (no counterpart
+ * in the reference): lsm_levelset_step_record runs the forward step from
+ * phi_n (phi_np1, optional, receives the result) and keeps in `trace` what
+ * the reverse sweep needs (the three RK stage results, the reinitialisation
+ * trace, the field after bcond_lsm; lsm_adjoint_workspace_bytes(blk,
+ * LSM_OP_STEP_TRACE, rn_itermax) bytes); lsm_levelset_step_adjoint_recorded
+ * runs the reverse sweep from such a trace (same blk, p, phi_n, ucn, dt and
+ * mode), without recomputing the forward.  Workspace:
+ * lsm_adjoint_workspace_bytes(blk, LSM_OP_STEP_ADJOINT_RECORDED, rn_itermax). */
+int lsm_levelset_step_record(const LsmBlock *blk, const LsmReinitParams *p, const double *phi_n,
+                             const double *ucn1, const double *ucn2, const double *ucn3, double delti_lsm,
+                             int mode, double *phi_np1, void *trace, size_t trace_bytes, void *stream);
+int lsm_levelset_step_adjoint_recorded(const LsmBlock *blk, const LsmReinitParams *p, const double *phi_n,
+                                       const double *ucn1, const double *ucn2, const double *ucn3,
+                                       double delti_lsm, int mode, const void *trace, size_t trace_bytes,
+                                       const double *adj_np1, double *adj_n, double *adj_ucn1,
+                                       double *adj_ucn2, double *adj_ucn3, void *work, size_t work_bytes,
+                                       void *stream);
 
 /* ---- host-memory adapter (for Fortran ISO_C_BINDING shims) -------------
  * Same operations on HOST arrays in the reference's layout: the adapter keeps

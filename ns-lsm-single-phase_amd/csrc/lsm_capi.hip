@@ -148,6 +148,71 @@ static void drain_timings()
 
 static size_t field_bytes(const Blk &b) { return ((size_t)b.n * sizeof(double) + 255) & ~(size_t)255; }
 
+// ---- one full step recorded for its adjoint ----
+// trace: the three RK stage results (the last is the reinitialisation's
+// input), the reinitialisation trace and the field after bcond_lsm
+struct StepTrace {
+    double *F[3], *Z;
+    ReinitTrace rt;
+};
+
+static size_t step_trace_bytes(const Blk &b, int itermax)
+{
+    return 4 * field_bytes(b) + reinit_trace_work(b, itermax) + 1024;
+}
+
+// the reverse sweep: two adjoint fields, then the reinitialisation adjoint's
+// own fields or the advection adjoint's (the two run one after the other)
+static size_t step_reverse_bytes(const Blk &b, int itermax)
+{
+    const size_t code = ((size_t)b.n * sizeof(uint16_t) + 255) & ~(size_t)255;
+    const size_t r = 7 * field_bytes(b) + code + 4096;
+    const size_t k = 2 * field_bytes(b) + 1024;
+    return 2 * field_bytes(b) + (r > k ? r : k) + 1024;
+    (void)itermax;
+}
+
+static int carve_step(const Blk &b, const LsmReinitParams &p, Carver &c, StepTrace &T, void **ctl)
+{
+    const size_t fb = (size_t)b.n * sizeof(double);
+    for (auto &f : T.F) f = (double *)c.take(fb);
+    T.Z = (double *)c.take(fb);
+    if (!T.F[0] || !T.F[1] || !T.F[2] || !T.Z) return fail(LSM_ERR_WORKSPACE, "step trace too small");
+    return carve_reinit_trace(b, p, T.F[2], T.rt, c, ctl);
+}
+
+static int step_record(const Blk &b, const LsmReinitParams &p, const double *phi_n, const double *U1,
+                       const double *U2, const double *U3, double dt, int mode, Carver c, StepTrace &T,
+                       hipStream_t s)
+{
+    const size_t fb = (size_t)b.n * sizeof(double);
+    for (auto &f : T.F) f = (double *)c.take(fb);
+    T.Z = (double *)c.take(fb);
+    if (!T.F[0] || !T.F[1] || !T.F[2] || !T.Z) return fail(LSM_ERR_WORKSPACE, "step trace too small");
+    int rc;
+    if ((rc = launch_rk3_stages(b, phi_n, nullptr, U1, U2, U3, dt, mode, T.F[0], T.F[1], T.F[2], s))) return rc;
+    if ((rc = launch_reinit_eno2_trace(b, p, T.F[2], T.rt, T.Z, c, s))) return rc;
+    return launch_bcond(b, T.Z, s);
+}
+
+static int step_reverse(const Blk &b, const LsmReinitParams &p, const double *phi_n, const double *U1,
+                        const double *U2, const double *U3, double dt, int mode, const StepTrace &T,
+                        const double *adj_np1, double *adj_n, double *aU1, double *aU2, double *aU3, Carver c,
+                        hipStream_t s)
+{
+    const size_t fb = (size_t)b.n * sizeof(double);
+    double *L = (double *)c.take(fb), *LX = (double *)c.take(fb);
+    if (!L || !LX) return fail(LSM_ERR_WORKSPACE, "step adjoint workspace too small");
+    int rc;
+    if ((rc = launch_copy(adj_np1, L, b.n, s))) return rc;
+    if ((rc = launch_bcond_adjoint(b, T.Z, L, s))) return rc;
+    Carver r = c;
+    if ((rc = launch_reinit_eno2_adjoint(b, p, T.F[2], L, LX, nullptr, r, s, &T.rt))) return rc;
+    const double *st[3] = {T.F[0], T.F[1], T.F[2]};
+    Carver k = c;   // the reinitialisation adjoint's fields are free again
+    return launch_rk3_adjoint(b, phi_n, nullptr, U1, U2, U3, dt, mode, LX, adj_n, aU1, aU2, aU3, k, s, st);
+}
+
 static size_t op_bytes(const Blk &b, int op, int itermax = 0)
 {
     switch (op) {
@@ -157,14 +222,9 @@ static size_t op_bytes(const Blk &b, int op, int itermax = 0)
     case LSM_OP_WATER_VOLUME: return water_volume_work(b);
     case LSM_OP_RK3_ADJOINT: return rk3_adjoint_work(b);
     case LSM_OP_REINIT_ENO2_ADJOINT: return reinit_eno2_adjoint_work(b, itermax);
-    case LSM_OP_STEP_ADJOINT: {
-        // rk3(phi^n), the reinitialised field after bcond_lsm, the RK3 scratch
-        // and two adjoint fields, then the larger of the two adjoints
-        const size_t a = 5 * field_bytes(b) + 1024;
-        const size_t r = reinit_eno2_adjoint_work(b, itermax);
-        const size_t k = rk3_adjoint_work(b);
-        return a + (r > k ? r : k);
-    }
+    case LSM_OP_STEP_TRACE: return step_trace_bytes(b, itermax);
+    case LSM_OP_STEP_ADJOINT_RECORDED: return step_reverse_bytes(b, itermax);
+    case LSM_OP_STEP_ADJOINT: return step_trace_bytes(b, itermax) + step_reverse_bytes(b, itermax);
     case LSM_OP_LEVELSET_STEP: {
         size_t a = 3 * field_bytes(b) + field_bytes(b) + field_bytes(b) + 1024;   // ucn, phi copy, rk3 scratch
         size_t r = reinit_eno2_work(b);
@@ -400,6 +460,29 @@ int lsm_reinit_sussman_eno2_adjoint(const LsmBlock *blk, const LsmReinitParams *
     return launch_reinit_eno2_adjoint(b, *p, phi0, adj_out, adj_phi0, phi_out, c, (hipStream_t)stream);
 }
 
+static int check_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, const double *phi_n,
+                              const double *ucn1, const double *ucn2, const double *ucn3, int mode, Blk &b)
+{
+    int rc = make_blk(blk, &b, true, false);
+    if (rc) return rc;
+    if ((rc = check_reinit(p))) return rc;
+    if (p->rn_itermax > kMaxTraceIters)
+        return fail(LSM_ERR_UNSUPPORTED, "rn_itermax = %d: the adjoint keeps at most %d iterations",
+                    p->rn_itermax, kMaxTraceIters);
+    if (!phi_n || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL array");
+    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
+    return LSM_OK;
+}
+
+static int check_adj_args(const double *adj_np1, double *adj_n, double *a1, double *a2, double *a3)
+{
+    if (!adj_np1 || !adj_n) return fail(LSM_ERR_ARG, "NULL array");
+    if (adj_n == adj_np1) return fail(LSM_ERR_ARG, "adj_n and adj_np1 must not alias");
+    if ((a1 != nullptr) != (a2 != nullptr) || (a1 != nullptr) != (a3 != nullptr))
+        return fail(LSM_ERR_ARG, "adj_ucn1..3 must all be given or all be NULL");
+    return LSM_OK;
+}
+
 int lsm_levelset_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, const double *phi_n,
                               const double *ucn1, const double *ucn2, const double *ucn3,
                               double delti_lsm, int mode, const double *adj_np1, double *adj_n,
@@ -407,39 +490,62 @@ int lsm_levelset_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, con
                               void *work, size_t work_bytes, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, true, false);
+    int rc = check_step_adjoint(blk, p, phi_n, ucn1, ucn2, ucn3, mode, b);
     if (rc) return rc;
-    if ((rc = check_reinit(p))) return rc;
-    if (p->rn_itermax > kMaxTraceIters)
-        return fail(LSM_ERR_UNSUPPORTED, "rn_itermax = %d: the adjoint keeps at most %d iterations",
-                    p->rn_itermax, kMaxTraceIters);
-    if (!phi_n || !ucn1 || !ucn2 || !ucn3 || !adj_np1 || !adj_n) return fail(LSM_ERR_ARG, "NULL array");
-    if (adj_n == adj_np1) return fail(LSM_ERR_ARG, "adj_n and adj_np1 must not alias");
-    if ((adj_ucn1 != nullptr) != (adj_ucn2 != nullptr) || (adj_ucn1 != nullptr) != (adj_ucn3 != nullptr))
-        return fail(LSM_ERR_ARG, "adj_ucn1..3 must all be given or all be NULL");
-    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
+    if ((rc = check_adj_args(adj_np1, adj_n, adj_ucn1, adj_ucn2, adj_ucn3))) return rc;
     hipStream_t s = (hipStream_t)stream;
     Carver c;
     rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_STEP_ADJOINT, p->rn_itermax), &c);
     if (rc) return rc;
-    const size_t fb = (size_t)b.n * sizeof(double);
-    double *X = (double *)c.take(fb), *Z = (double *)c.take(fb), *scr = (double *)c.take(fb);
-    double *L = (double *)c.take(fb), *LX = (double *)c.take(fb);
-    if (!X || !Z || !scr || !L || !LX) return fail(LSM_ERR_WORKSPACE, "step adjoint workspace too small");
-    // forward: X = ad_RKTVD3(phi^n); reinitialisation of X (trace kept); Z = bcond_lsm of it
-    if ((rc = launch_rk3(b, X, phi_n, nullptr, ucn1, ucn2, ucn3, delti_lsm, mode, scr, s))) return rc;
-    Carver rest = c;
-    ReinitTrace t;
-    if ((rc = launch_reinit_eno2_trace(b, *p, X, t, Z, rest, s))) return rc;
-    if ((rc = launch_bcond(b, Z, s))) return rc;
-    if (phi_np1 && (rc = launch_copy(Z, phi_np1, b.n, s))) return rc;
-    // reverse: bcond_lsm, the reinitialisation, ad_RKTVD3
-    if ((rc = launch_copy(adj_np1, L, b.n, s))) return rc;
-    if ((rc = launch_bcond_adjoint(b, Z, L, s))) return rc;
-    if ((rc = launch_reinit_eno2_adjoint(b, *p, X, L, LX, nullptr, rest, s, &t))) return rc;
-    Carver rk = c;   // the trace is no longer needed: its memory serves the advection adjoint
-    return launch_rk3_adjoint(b, phi_n, nullptr, ucn1, ucn2, ucn3, delti_lsm, mode, LX, adj_n, adj_ucn1,
-                              adj_ucn2, adj_ucn3, rk, s);
+    // forward, recorded into the first part of the workspace, then the reverse sweep
+    StepTrace T;
+    Carver tr{c.base, step_trace_bytes(b, p->rn_itermax), 0};
+    if ((rc = step_record(b, *p, phi_n, ucn1, ucn2, ucn3, delti_lsm, mode, tr, T, s))) return rc;
+    if (phi_np1 && (rc = launch_copy(T.Z, phi_np1, b.n, s))) return rc;
+    const size_t off = (step_trace_bytes(b, p->rn_itermax) + 255) & ~(size_t)255;
+    Carver rv{c.base + off, c.size - off, 0};
+    return step_reverse(b, *p, phi_n, ucn1, ucn2, ucn3, delti_lsm, mode, T, adj_np1, adj_n, adj_ucn1, adj_ucn2,
+                        adj_ucn3, rv, s);
+}
+
+int lsm_levelset_step_record(const LsmBlock *blk, const LsmReinitParams *p, const double *phi_n,
+                             const double *ucn1, const double *ucn2, const double *ucn3, double delti_lsm,
+                             int mode, double *phi_np1, void *trace, size_t trace_bytes, void *stream)
+{
+    Blk b;
+    int rc = check_step_adjoint(blk, p, phi_n, ucn1, ucn2, ucn3, mode, b);
+    if (rc) return rc;
+    if (!trace) return fail(LSM_ERR_ARG, "NULL trace");
+    if (trace_bytes < step_trace_bytes(b, p->rn_itermax))
+        return fail(LSM_ERR_WORKSPACE, "trace of %zu bytes, %zu needed", trace_bytes, step_trace_bytes(b, p->rn_itermax));
+    hipStream_t s = (hipStream_t)stream;
+    StepTrace T;
+    if ((rc = step_record(b, *p, phi_n, ucn1, ucn2, ucn3, delti_lsm, mode, Carver{(char *)trace, trace_bytes, 0}, T, s)))
+        return rc;
+    if (phi_np1 && (rc = launch_copy(T.Z, phi_np1, b.n, s))) return rc;
+    return LSM_OK;
+}
+
+int lsm_levelset_step_adjoint_recorded(const LsmBlock *blk, const LsmReinitParams *p, const double *phi_n,
+                                       const double *ucn1, const double *ucn2, const double *ucn3,
+                                       double delti_lsm, int mode, const void *trace, size_t trace_bytes,
+                                       const double *adj_np1, double *adj_n, double *adj_ucn1, double *adj_ucn2,
+                                       double *adj_ucn3, void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = check_step_adjoint(blk, p, phi_n, ucn1, ucn2, ucn3, mode, b);
+    if (rc) return rc;
+    if ((rc = check_adj_args(adj_np1, adj_n, adj_ucn1, adj_ucn2, adj_ucn3))) return rc;
+    if (!trace) return fail(LSM_ERR_ARG, "NULL trace");
+    if (trace_bytes < step_trace_bytes(b, p->rn_itermax)) return fail(LSM_ERR_WORKSPACE, "trace too small");
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_STEP_ADJOINT_RECORDED, p->rn_itermax), &c);
+    if (rc) return rc;
+    StepTrace T;
+    Carver tr{(char *)const_cast<void *>(trace), trace_bytes, 0};
+    if ((rc = carve_step(b, *p, tr, T, nullptr))) return rc;
+    return step_reverse(b, *p, phi_n, ucn1, ucn2, ucn3, delti_lsm, mode, T, adj_np1, adj_n, adj_ucn1, adj_ucn2,
+                        adj_ucn3, c, (hipStream_t)stream);
 }
 
 int lsm_sign_ini(const LsmBlock *blk, const double *phi0, double epslsm, int m_sphi0,

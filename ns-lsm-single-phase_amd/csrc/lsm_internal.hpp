@@ -76,10 +76,12 @@ int launch_copy(const double *src, double *dst, long n, hipStream_t s);
 
 // ---- launchers (lsm_adjoint.hip) ----
 size_t rk3_adjoint_work(const Blk &b);
+// `stages` (optional): the three stage results of the forward step, recorded
+// earlier; otherwise they are recomputed into the workspace
 int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
                        const double *U2, const double *U3, double dt, int mode, const double *adj_np1,
                        double *adj_n, double *adjU1, double *adjU2, double *adjU3, Carver &w,
-                       hipStream_t s);
+                       hipStream_t s, const double *const *stages = nullptr);
 
 // ---- launchers (lsm_reinit.hip) ----
 size_t reinit_eno2_work(const Blk &b);
@@ -112,6 +114,10 @@ size_t reinit_trace_work(const Blk &b, int itermax);
 // `out` (optional) receives the reinitialised field, D[*iters]
 int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t,
                              double *out, Carver &w, hipStream_t s);
+// the trace's buffers carved from w as launch_reinit_eno2_trace carves them
+// (no launch): re-opens a trace recorded earlier in the same memory
+int carve_reinit_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t, Carver &w,
+                       void **ctl);
 
 // ---- launchers (lsm_reinit_adjoint.hip) ----
 size_t reinit_eno2_adjoint_work(const Blk &b, int itermax);

@@ -549,8 +549,8 @@ size_t reinit_trace_work(const Blk &b, int itermax)
     return (size_t)itermax * (3 * fb(b) + bytes_u8) + 2048;
 }
 
-int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t,
-                             double *out, Carver &w, hipStream_t s)
+int carve_reinit_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t, Carver &w,
+                       void **ctl_out)
 {
     const int nit = p.rn_itermax;
     if (nit < 0 || nit > kMaxTraceIters)
@@ -565,10 +565,21 @@ int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const doubl
         if (!t.A[it] || !t.B[it] || !Dn || !t.conv[it]) return fail(LSM_ERR_WORKSPACE, "reinit trace workspace too small");
         t.D[it + 1] = Dn;
     }
-    Ctl *ctl = (Ctl *)w.take(sizeof(Ctl));
+    void *ctl = w.take(sizeof(Ctl));
     t.iters = (int32_t *)w.take(sizeof(int32_t));
     if (!ctl || !t.iters) return fail(LSM_ERR_WORKSPACE, "reinit trace workspace too small");
-    int rc;
+    if (ctl_out) *ctl_out = ctl;
+    return LSM_OK;
+}
+
+int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t,
+                             double *out, Carver &w, hipStream_t s)
+{
+    void *cv = nullptr;
+    int rc = carve_reinit_trace(b, p, phi0, t, w, &cv);
+    if (rc) return rc;
+    const int nit = t.nit;
+    Ctl *ctl = (Ctl *)cv;
     reinit_init_ctl<<<1, 1, 0, s>>>(ctl, t.iters);
     if ((rc = check_hip(hipGetLastError(), "reinit_init_ctl"))) return rc;
     if (nit > 0) {

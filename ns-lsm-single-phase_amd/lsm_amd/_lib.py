@@ -17,7 +17,7 @@ LSM_ADVECT_FAST = 0
 LSM_ADVECT_EXACT = 1
 LSM_OP_RK3, LSM_OP_REINIT_ENO2, LSM_OP_REINIT_GODUNOV, LSM_OP_WATER_VOLUME, LSM_OP_LEVELSET_STEP = 1, 2, 3, 4, 5
 LSM_OP_RK3_ADJOINT = 6
-LSM_OP_REINIT_ENO2_ADJOINT, LSM_OP_STEP_ADJOINT = 7, 8
+LSM_OP_REINIT_ENO2_ADJOINT, LSM_OP_STEP_ADJOINT, LSM_OP_STEP_TRACE, LSM_OP_STEP_ADJOINT_RECORDED = 7, 8, 9, 10
 
 _vp = C.c_void_p
 
@@ -94,6 +94,11 @@ _SIGS = {
     "lsm_levelset_step_adjoint": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp,
                                             _vp, C.c_double, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                             C.c_size_t, _vp]),
+    "lsm_levelset_step_record": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp, _vp,
+                                           C.c_double, C.c_int, _vp, _vp, C.c_size_t, _vp]),
+    "lsm_levelset_step_adjoint_recorded": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp,
+                                                     _vp, C.c_double, C.c_int, _vp, C.c_size_t, _vp, _vp, _vp, _vp,
+                                                     _vp, _vp, C.c_size_t, _vp]),
     "lsm_normgrad_eno2": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, C.c_int, C.c_int, _vp, _vp]),
     "lsm_reinit_sussman_eno2": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp,
                                           _vp, _vp, C.c_size_t, _vp]),

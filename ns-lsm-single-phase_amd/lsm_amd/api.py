@@ -373,3 +373,38 @@ def levelset_step_adjoint(blk: Block, phi_n, ucn, delti_lsm: float, params: Rein
                                       C.c_void_p(work.data_ptr()), work.numel(), _stream()),
           "lsm_levelset_step_adjoint")
     return out
+
+
+def levelset_step_record(blk: Block, phi_n, ucn, delti_lsm: float, params: ReinitParams, mode="fast",
+                         phi_np1=None, trace=None):
+    """Forward step recorded for its adjoint: returns (phi^(n+1), trace)."""
+    L = _lib.load()
+    if phi_np1 is None:
+        phi_np1 = blk.empty()
+    if trace is None:
+        trace = blk.adjoint_workspace(_lib.LSM_OP_STEP_TRACE, params.RNitermax)
+    p = params.c()
+    check(L.lsm_levelset_step_record(blk.ref, C.byref(p), _f64(phi_n, blk.shape, "phi_n"),
+                                     *[_f64(t, blk.shape, "ucn") for t in ucn], float(delti_lsm), MODES[mode],
+                                     _f64(phi_np1, blk.shape, "phi_np1"), C.c_void_p(trace.data_ptr()),
+                                     trace.numel(), _stream()), "lsm_levelset_step_record")
+    return phi_np1, trace
+
+
+def levelset_step_adjoint_recorded(blk: Block, phi_n, ucn, delti_lsm: float, params: ReinitParams, trace,
+                                   adj_np1, mode="fast", adj_ucn=None, out=None, work=None):
+    """Reverse sweep of a recorded step: dJ/dphi^n (dJ/d(U/J) added to adj_ucn)."""
+    L = _lib.load()
+    if out is None:
+        out = blk.empty()
+    if work is None:
+        work = blk.adjoint_workspace(_lib.LSM_OP_STEP_ADJOINT_RECORDED, params.RNitermax)
+    au = [_f64(t, blk.shape, "adj_ucn") for t in adj_ucn] if adj_ucn is not None else [None] * 3
+    p = params.c()
+    check(L.lsm_levelset_step_adjoint_recorded(blk.ref, C.byref(p), _f64(phi_n, blk.shape, "phi_n"),
+                                               *[_f64(t, blk.shape, "ucn") for t in ucn], float(delti_lsm),
+                                               MODES[mode], C.c_void_p(trace.data_ptr()), trace.numel(),
+                                               _f64(adj_np1, blk.shape, "adj_np1"), _f64(out, blk.shape, "out"),
+                                               *au, C.c_void_p(work.data_ptr()), work.numel(), _stream()),
+          "lsm_levelset_step_adjoint_recorded")
+    return out

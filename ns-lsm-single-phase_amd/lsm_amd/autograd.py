@@ -198,21 +198,26 @@ class _StepTrajectory(torch.autograd.Function):
         blk, dt, params, nsteps, mode, every = ctx.blk, ctx.dt, ctx.params, ctx.nsteps, ctx.mode, ctx.every
         want_u = any(ctx.needs_input_grad[1:4])
         aU = [torch.zeros_like(U[0]) for _ in range(3)] if want_u else None
-        wrk = blk.workspace(api._lib.LSM_OP_RK3)
-        wrn = blk.workspace(api._lib.LSM_OP_REINIT_ENO2)
-        wad = blk.adjoint_workspace(api._lib.LSM_OP_STEP_ADJOINT, params.RNitermax)
-        tmp = blk.empty()
+        wad = blk.adjoint_workspace(api._lib.LSM_OP_STEP_ADJOINT_RECORDED, params.RNitermax)
         lam = g.contiguous().clone()
         nxt = blk.empty()
+        traces = []
         for sidx in range(len(ckpts) - 1, -1, -1):
             s0 = sidx * every
             s1 = min(s0 + every, nsteps)
+            # replay the segment once, recording every step for its reverse sweep
             states = [ckpts[sidx]]
-            for _ in range(s0, s1 - 1):
-                states.append(_step_forward(blk, states[-1], U, dt, params, mode, wrk, wrn, tmp).clone())
-            for st in reversed(states):
-                api.levelset_step_adjoint(blk, st, U, dt, params, lam, mode=mode, adj_ucn=aU, out=nxt,
-                                          work=wad)
+            for q in range(s1 - s0):
+                if len(traces) <= q:
+                    traces.append(blk.adjoint_workspace(api._lib.LSM_OP_STEP_TRACE, params.RNitermax))
+                nxt_state = blk.empty() if q < s1 - s0 - 1 else None
+                api.levelset_step_record(blk, states[-1], U, dt, params, mode=mode, phi_np1=nxt_state,
+                                         trace=traces[q])
+                if nxt_state is not None:
+                    states.append(nxt_state)
+            for q in range(s1 - s0 - 1, -1, -1):
+                api.levelset_step_adjoint_recorded(blk, states[q], U, dt, params, traces[q], lam, mode=mode,
+                                                   adj_ucn=aU, out=nxt, work=wad)
                 lam, nxt = nxt, lam
         gu = aU if want_u else [None, None, None]
         return (lam, *gu, None, None, None, None, None, None)
@@ -221,8 +226,10 @@ class _StepTrajectory(torch.autograd.Function):
 def levelset_trajectory(blk, phi0, ucn: Sequence[torch.Tensor], dt: float, params: "api.ReinitParams",
                         nsteps: int, mode: str = "exact", checkpoint_every: int = 5):
     """nsteps differentiable level-set steps (levelset_step) with checkpointing
-    every k steps: memory holds nsteps/k + k fields plus one step's adjoint
-    workspace."""
+    every k steps: the forward keeps nsteps/k checkpoints; the backward replays
+    one segment at a time, recording each of its k steps (lsm_levelset_step_
+    record) and then running their reverse sweeps from the records, so each
+    step's forward is computed twice in all (forward pass, replay)."""
     if nsteps < 1:
         raise ValueError("nsteps must be >= 1")
     every = max(1, int(checkpoint_every))

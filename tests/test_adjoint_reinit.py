@@ -303,3 +303,26 @@ def test_heaviside_loss_gradients_match_fd_of_oracle():
         fd = _fd_smooth(lambda s: J_oracle(phi, amps + s * e), 0.0, 1.0)
         an = float(np.sum(gamp * e))
         assert abs(fd - an) <= 1e-6 * abs(an) + 1e-12, (fd, an)
+
+
+@pytest.mark.gpu
+def test_recorded_step_adjoint_matches_one_call():
+    """lsm_levelset_step_record + lsm_levelset_step_adjoint_recorded give the
+    forward result and the gradients of lsm_levelset_step_adjoint bit for bit
+    (the same operations; the recorded stages are the recomputed ones)."""
+    from lsm_amd import api
+    g, phi, vel, dt, hmin, hmax = _step_case((15, 13, 12), 161)
+    api_, blk, dev = _gpu_setup(g, (1, 2, 0, 1, 1, 0), 5.0)
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4, epsReinitialisation=1e-9)
+    ucn = api.rhs_contra_j(blk, *(dev(a) for a in vel))
+    lam = dev(np.random.default_rng(17).standard_normal(g.shape))
+    for mode in ("exact", "fast"):
+        aU1 = [blk.zeros() for _ in range(3)]
+        ph1 = blk.empty()
+        a1 = api.levelset_step_adjoint(blk, dev(phi), ucn, dt, prm, lam, mode=mode, adj_ucn=aU1, phi_np1=ph1)
+        ph2, tr = api.levelset_step_record(blk, dev(phi), ucn, dt, prm, mode=mode)
+        aU2 = [blk.zeros() for _ in range(3)]
+        a2 = api.levelset_step_adjoint_recorded(blk, dev(phi), ucn, dt, prm, tr, lam, mode=mode, adj_ucn=aU2)
+        assert torch.equal(ph1, ph2) and torch.equal(a1, a2), mode
+        for x, y in zip(aU1, aU2):
+            assert torch.equal(x, y), mode
