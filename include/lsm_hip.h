@@ -430,7 +430,8 @@ int lsm_host_water_volume(const double *phi, double vol[3], int32_t *adv_mask);
 #define LSM_KT_ADJOINT_STAGE  4   /* fused adjoint of one RK stage (partials + gather) */
 #define LSM_KT_BATCH_STAGE    5   /* batched stage tiles of every block (one launch) */
 #define LSM_KT_BATCH_REINIT   6   /* batched reinitialisation stage of every block */
-#define LSM_KT_COUNT          7
+#define LSM_KT_REINIT_ADJOINT 7   /* reverse sweep of one reinitialisation stage (tape gather) */
+#define LSM_KT_COUNT          8
 
 /* Turns event recording on (1) or off (0); clears accumulated timings. */
 int lsm_timing_enable(int on);

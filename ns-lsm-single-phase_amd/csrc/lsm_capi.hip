@@ -153,7 +153,7 @@ static size_t field_bytes(const Blk &b) { return ((size_t)b.n * sizeof(double) +
 // input), the reinitialisation trace and the field after bcond_lsm
 struct StepTrace {
     double *F[3], *Z;
-    ReinitTrace rt;
+    ReinitTape rt;
 };
 
 static size_t step_trace_bytes(const Blk &b, int itermax)
@@ -165,8 +165,7 @@ static size_t step_trace_bytes(const Blk &b, int itermax)
 // own fields or the advection adjoint's (the two run one after the other)
 static size_t step_reverse_bytes(const Blk &b, int itermax)
 {
-    const size_t code = ((size_t)b.n * sizeof(uint16_t) + 255) & ~(size_t)255;
-    const size_t r = 7 * field_bytes(b) + code + 4096;
+    const size_t r = 4 * field_bytes(b) + 4096;
     const size_t k = 2 * field_bytes(b) + 1024;
     return 2 * field_bytes(b) + (r > k ? r : k) + 1024;
     (void)itermax;
@@ -178,7 +177,8 @@ static int carve_step(const Blk &b, const LsmReinitParams &p, Carver &c, StepTra
     for (auto &f : T.F) f = (double *)c.take(fb);
     T.Z = (double *)c.take(fb);
     if (!T.F[0] || !T.F[1] || !T.F[2] || !T.Z) return fail(LSM_ERR_WORKSPACE, "step trace too small");
-    return carve_reinit_trace(b, p, T.F[2], T.rt, c, ctl);
+    double *A, *B;
+    return carve_reinit_trace(b, p, T.rt, c, ctl, &A, &B);
 }
 
 static int step_record(const Blk &b, const LsmReinitParams &p, const double *phi_n, const double *U1,

@@ -98,26 +98,28 @@ int launch_lambda_rn(const Blk &b, const double *phi0, const double *ng0, const 
 int launch_reinit_godunov(const Blk &b, const LsmReinitParams &p, const double *phi0,
                           double *out, Carver &w, hipStream_t s);
 
-// The Sussman iterations of launch_reinit_eno2 with every stage input kept,
-// for the adjoint: D[it] is the input of iteration it (D[0] = phi0, D[nit] the
-// result if all iterations run), A[it] / B[it] the inputs of its stages 2 / 3,
-// conv[it] its update mask; *iters (device) the number of iterations that ran.
+// The Sussman iterations of launch_reinit_eno2 with a tape for the adjoint:
+// per iteration the update mask, per stage and node the linearisation of the
+// update (direction weights q, branch codes, the s(phi0) term T inside the
+// interface band); *iters (device) the number of iterations that ran.
 constexpr int kMaxTraceIters = 64;
-struct ReinitTrace {
+struct ReinitTape {
     int nit;
-    const double *D[kMaxTraceIters + 1];
-    double *A[kMaxTraceIters], *B[kMaxTraceIters];
     uint8_t *conv[kMaxTraceIters];
+    double *q[kMaxTraceIters][3][3];     // [iteration][stage][direction]
+    uint16_t *code[kMaxTraceIters][3];
+    double *T[kMaxTraceIters][3];
     int32_t *iters;
 };
 size_t reinit_trace_work(const Blk &b, int itermax);
-// `out` (optional) receives the reinitialised field, D[*iters]
-int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t,
+// runs the iterations from phi0 into `out` (required, not aliasing phi0),
+// recording the tape in w
+int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTape &t,
                              double *out, Carver &w, hipStream_t s);
-// the trace's buffers carved from w as launch_reinit_eno2_trace carves them
-// (no launch): re-opens a trace recorded earlier in the same memory
-int carve_reinit_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t, Carver &w,
-                       void **ctl);
+// the tape's buffers carved from w as launch_reinit_eno2_trace carves them
+// (no launch): re-opens a tape recorded earlier in the same memory
+int carve_reinit_trace(const Blk &b, const LsmReinitParams &p, ReinitTape &t, Carver &w, void **ctl,
+                       double **A, double **B);
 
 // ---- launchers (lsm_reinit_adjoint.hip) ----
 size_t reinit_eno2_adjoint_work(const Blk &b, int itermax);
@@ -126,7 +128,7 @@ size_t reinit_eno2_adjoint_work(const Blk &b, int itermax);
 // recorded (then `out` is ignored and the trace's workspace stays in use)
 int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const double *phi0,
                                const double *adj_out, double *adj_phi0, double *out, Carver &w,
-                               hipStream_t s, const ReinitTrace *pre = nullptr);
+                               hipStream_t s, const ReinitTape *pre = nullptr);
 // bcond_lsm transposed in place on adj; phi is the forward field after bcond_lsm
 int launch_bcond_adjoint(const Blk &b, const double *phi, double *adj, hipStream_t s);
 

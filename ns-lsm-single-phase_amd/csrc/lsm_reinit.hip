@@ -99,6 +99,12 @@ struct RTArgs {
     const double *phi0;
     uint8_t *conv;
     Ctl *ctl;
+    // tape of the stage for the adjoint (TAPE kernels): per node the
+    // linearisation of its update (-c3 dtau s(phi0) x d|grad d|/d(D_a d) x
+    // spacing per direction, branch codes, -c3 dtau (|grad d| - 1) in the band)
+    double *tq[3];
+    uint16_t *tcode;
+    double *tT;
     int tiles_i, tiles_j, kc, ntiles, grid_tiles;   // interior tiles (grid_tiles: padded to 8)
     long nI, nJ, nK;                                 // shell nodes on i, j and k faces
 };
@@ -139,8 +145,29 @@ __device__ __forceinline__ void reduce_maxmin_t(double vmax, double vmin, int ti
     }
 }
 
+// the tape entry of one node: w are the direction weights already divided by
+// |grad d| and scaled by the spacing, br the branch codes
+__device__ __forceinline__ void tape_write(const RTArgs &a, long l, bool upd, double ngv, double dv, double p0,
+                                           const double w[3], int br)
+{
+    uint16_t code = 0;
+    if (upd) {
+        const double del = deltaf(dv, a.p.epslsm);
+        if (fabs((del * del) * ngv) < kEpsSims) {   // lambda = 0: rhs = s (|grad d| - 1)
+            const double c3dt = a.p.c3 * a.p.dt;
+            if (fabs(p0) <= a.p.epslsm) { a.tT[l] = -c3dt * (ngv - 1.0); code |= rn::kHasT; }
+            const double f = -c3dt * sign_phi0(p0, a.p.epslsm);
+            if (f != 0.0 && ngv > 0.0) {
+                a.tq[0][l] = f * w[0]; a.tq[1][l] = f * w[1]; a.tq[2][l] = f * w[2];
+                code |= (uint16_t)(rn::kLive | br);
+            }
+        }
+    }
+    a.tcode[l] = code;
+}
+
 // STAGE 0: only the max/min of |grad din| (reinitialisation_test2.F90:96-101)
-template <int STAGE>
+template <int STAGE, bool TAPE = false>
 __global__ __launch_bounds__(RTX * RTY) void reinit_stage_tiled(RTArgs a)
 {
     if (!a.ctl->active) return;
@@ -173,7 +200,11 @@ __global__ __launch_bounds__(RTX * RTY) void reinit_stage_tiled(RTArgs a)
         if (ok) {
             const long l = (long)i + g.sj * j + g.sk * k;
             const double p0s = a.phi0[(long)(g.ni - 2) + g.sj * (g.nj - 2) + g.sk * (g.nk - 2)];
-            const double ngv = normgrad_node(g, a.din, a.phi0, p0s, a.p.order_bc, a.p.eno_bc != 0, i, j, k, l);
+            double tw[3];
+            int tbr = 0;
+            const double ngv = TAPE ? rn::normgrad_branch(g, a.din, a.phi0, p0s, a.p.order_bc, a.p.eno_bc != 0, i, j,
+                                                          k, l, tw, tbr)
+                                    : normgrad_node(g, a.din, a.phi0, p0s, a.p.order_bc, a.p.eno_bc != 0, i, j, k, l);
             if (STAGE > 0) {
                 bool upd;
                 if (STAGE == 1) { upd = ngv - 1.0 > a.p.eps_r; a.conv[l] = upd ? 1 : 0; }
@@ -181,6 +212,7 @@ __global__ __launch_bounds__(RTX * RTY) void reinit_stage_tiled(RTArgs a)
                 const double dv = a.din[l];
                 const double dnv = (STAGE == 1) ? dv : a.dn[l];
                 a.dout[l] = reinit_update<STAGE>(a.p, ngv, dv, dnv, a.phi0[l], upd);
+                if constexpr (TAPE) tape_write(a, l, upd, ngv, dv, a.phi0[l], tw, tbr);
             }
             vmax = ngv; vmin = ngv;
         }
@@ -267,6 +299,19 @@ __global__ __launch_bounds__(RTX * RTY) void reinit_stage_tiled(RTArgs a)
                 if (STAGE == 1) { upd = ngv - 1.0 > a.p.eps_r; a.conv[l] = upd ? 1 : 0; }
                 else upd = cv != 0;
                 __builtin_nontemporal_store(reinit_update<STAGE>(a.p, ngv, q2, dnv, p0, upd), a.dout + l);
+                if constexpr (TAPE) {
+                    const int br = eno2_branch(p0, ilo ? 0.0 : pl[cy][cx - 2], pl[cy][cx - 1], q2, pl[cy][cx + 1],
+                                               ihi ? 0.0 : pl[cy][cx + 2], ilo, ihi) |
+                                   (eno2_branch(p0, jlo ? 0.0 : pl[cy - 2][cx], pl[cy - 1][cx], q2, pl[cy + 1][cx],
+                                                jhi ? 0.0 : pl[cy + 2][cx], jlo, jhi) << 4) |
+                                   (eno2_branch(p0, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi) << 8);
+                    const double inv = ngv > 0.0 ? 1.0 / ngv : 0.0;
+                    double tw[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        tw[d] = ((gx * m[3 * d] + gy * m[3 * d + 1]) + gz * m[3 * d + 2]) * inv * g.dd[d];
+                    tape_write(a, l, upd, ngv, q2, p0, tw, br);
+                }
             }
             if (STAGE == 3 || STAGE == 0) { vmax = fmax(vmax, ngv); vmin = fmin(vmin, ngv); }
         }
@@ -470,14 +515,23 @@ RTShape rt_shape(const Blk &b, const LsmReinitParams &p, const double *phi0, uin
     return r;
 }
 
-void rt_launch(RTShape &r, int stage, hipStream_t s)
+void rt_launch(RTShape &r, int stage, hipStream_t s, bool tape = false)
 {
     const dim3 rblk(RTX, RTY, 1);
     switch (stage) {
     case 0: reinit_stage_tiled<0><<<r.grid, rblk, 0, s>>>(r.a); break;
-    case 1: reinit_stage_tiled<1><<<r.grid, rblk, 0, s>>>(r.a); break;
-    case 2: reinit_stage_tiled<2><<<r.grid, rblk, 0, s>>>(r.a); break;
-    default: reinit_stage_tiled<3><<<r.grid, rblk, 0, s>>>(r.a); break;
+    case 1:
+        if (tape) reinit_stage_tiled<1, true><<<r.grid, rblk, 0, s>>>(r.a);
+        else reinit_stage_tiled<1><<<r.grid, rblk, 0, s>>>(r.a);
+        break;
+    case 2:
+        if (tape) reinit_stage_tiled<2, true><<<r.grid, rblk, 0, s>>>(r.a);
+        else reinit_stage_tiled<2><<<r.grid, rblk, 0, s>>>(r.a);
+        break;
+    default:
+        if (tape) reinit_stage_tiled<3, true><<<r.grid, rblk, 0, s>>>(r.a);
+        else reinit_stage_tiled<3><<<r.grid, rblk, 0, s>>>(r.a);
+        break;
     }
 }
 
@@ -485,26 +539,6 @@ void rt_stage_params(RTShape &r, const LsmReinitParams &p, int st)
 {
     r.a.p = StageP{kRkC[st][0], kRkC[st][1], kRkC[st][2], p.rn_timestep, p.epslsm, p.eps_reinit,
                    p.order_bc, p.eno_bc};
-}
-
-// out = D[*iters] (the field the last iteration that ran produced)
-struct SelectArgs {
-    const double *D[kMaxTraceIters + 1];
-    int nit;
-    const int32_t *iters;
-    double *out;
-    long n;
-};
-
-__global__ __launch_bounds__(256) void trace_select(SelectArgs a)
-{
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= a.n) return;
-    const int it = min(max(*a.iters, 0), a.nit);
-    const double *src = a.D[0];
-    for (int q = 1; q <= a.nit; ++q)
-        if (q == it) src = a.D[q];
-    a.out[l] = src[l];
 }
 
 }  // namespace
@@ -545,68 +579,70 @@ int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi
 
 size_t reinit_trace_work(const Blk &b, int itermax)
 {
-    const size_t bytes_u8 = ((size_t)b.n + 255) & ~(size_t)255;
-    return (size_t)itermax * (3 * fb(b) + bytes_u8) + 2048;
+    const size_t u8 = ((size_t)b.n + 255) & ~(size_t)255;
+    const size_t u16 = ((size_t)b.n * sizeof(uint16_t) + 255) & ~(size_t)255;
+    return (size_t)itermax * (u8 + 3 * (4 * fb(b) + u16)) + 2 * fb(b) + 2048;
 }
 
-int carve_reinit_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t, Carver &w,
-                       void **ctl_out)
+int carve_reinit_trace(const Blk &b, const LsmReinitParams &p, ReinitTape &t, Carver &w, void **ctl_out,
+                       double **A, double **B)
 {
     const int nit = p.rn_itermax;
     if (nit < 0 || nit > kMaxTraceIters)
         return fail(LSM_ERR_UNSUPPORTED, "rn_itermax = %d outside 0..%d for the adjoint", nit, kMaxTraceIters);
+    const size_t f = (size_t)b.n * sizeof(double);
     t.nit = nit;
-    t.D[0] = phi0;
     for (int it = 0; it < nit; ++it) {
-        t.A[it] = (double *)w.take(b.n * sizeof(double));
-        t.B[it] = (double *)w.take(b.n * sizeof(double));
-        double *Dn = (double *)w.take(b.n * sizeof(double));
         t.conv[it] = (uint8_t *)w.take((size_t)b.n);
-        if (!t.A[it] || !t.B[it] || !Dn || !t.conv[it]) return fail(LSM_ERR_WORKSPACE, "reinit trace workspace too small");
-        t.D[it + 1] = Dn;
+        if (!t.conv[it]) return fail(LSM_ERR_WORKSPACE, "reinit trace workspace too small");
+        for (int st = 0; st < 3; ++st) {
+            for (int d = 0; d < 3; ++d) t.q[it][st][d] = (double *)w.take(f);
+            t.T[it][st] = (double *)w.take(f);
+            t.code[it][st] = (uint16_t *)w.take((size_t)b.n * sizeof(uint16_t));
+            if (!t.q[it][st][0] || !t.q[it][st][1] || !t.q[it][st][2] || !t.T[it][st] || !t.code[it][st])
+                return fail(LSM_ERR_WORKSPACE, "reinit trace workspace too small");
+        }
     }
+    *A = (double *)w.take(f);
+    *B = (double *)w.take(f);
     void *ctl = w.take(sizeof(Ctl));
     t.iters = (int32_t *)w.take(sizeof(int32_t));
-    if (!ctl || !t.iters) return fail(LSM_ERR_WORKSPACE, "reinit trace workspace too small");
+    if (!*A || !*B || !ctl || !t.iters) return fail(LSM_ERR_WORKSPACE, "reinit trace workspace too small");
     if (ctl_out) *ctl_out = ctl;
     return LSM_OK;
 }
 
-int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTrace &t,
+int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const double *phi0, ReinitTape &t,
                              double *out, Carver &w, hipStream_t s)
 {
     void *cv = nullptr;
-    int rc = carve_reinit_trace(b, p, phi0, t, w, &cv);
+    double *A = nullptr, *B = nullptr;
+    int rc = carve_reinit_trace(b, p, t, w, &cv, &A, &B);
     if (rc) return rc;
-    const int nit = t.nit;
+    if (!out || out == phi0) return fail(LSM_ERR_ARG, "reinit trace output missing or aliasing phi0");
     Ctl *ctl = (Ctl *)cv;
+    if ((rc = launch_copy(phi0, out, b.n, s))) return rc;
     reinit_init_ctl<<<1, 1, 0, s>>>(ctl, t.iters);
     if ((rc = check_hip(hipGetLastError(), "reinit_init_ctl"))) return rc;
-    if (nit > 0) {
-        RTShape r = rt_shape(b, p, phi0, nullptr, ctl);
-        r.a.din = phi0; r.a.dn = phi0; r.a.dout = nullptr;
-        rt_launch(r, 0, s);
-        for (int it = 0; it < nit; ++it) {
-            reinit_check<<<1, 1, 0, s>>>(ctl, p.eps_reinit, t.iters);
-            const double *sin_[3] = {t.D[it], t.A[it], t.B[it]};
-            double *sout[3] = {t.A[it], t.B[it], const_cast<double *>(t.D[it + 1])};
-            r.a.conv = t.conv[it];
-            for (int st = 0; st < 3; ++st) {
-                rt_stage_params(r, p, st);
-                r.a.din = sin_[st]; r.a.dn = t.D[it]; r.a.dout = sout[st];
-                rt_launch(r, st + 1, s);
-            }
+    if (t.nit == 0) return LSM_OK;
+    RTShape r = rt_shape(b, p, phi0, nullptr, ctl);
+    r.a.din = phi0; r.a.dn = phi0; r.a.dout = nullptr;
+    rt_launch(r, 0, s);
+    const double *sin_[3] = {out, A, B};
+    double *sout[3] = {A, B, out};
+    for (int it = 0; it < t.nit; ++it) {
+        reinit_check<<<1, 1, 0, s>>>(ctl, p.eps_reinit, t.iters);
+        r.a.conv = t.conv[it];
+        for (int st = 0; st < 3; ++st) {
+            rt_stage_params(r, p, st);
+            r.a.din = sin_[st]; r.a.dn = out; r.a.dout = sout[st];
+            for (int d = 0; d < 3; ++d) r.a.tq[d] = t.q[it][st][d];
+            r.a.tcode = t.code[it][st];
+            r.a.tT = t.T[it][st];
+            rt_launch(r, st + 1, s, true);
         }
-        if ((rc = check_hip(hipGetLastError(), "reinit trace"))) return rc;
     }
-    if (out) {
-        SelectArgs sa{};
-        for (int q = 0; q <= nit; ++q) sa.D[q] = t.D[q];
-        sa.nit = nit; sa.iters = t.iters; sa.out = out; sa.n = b.n;
-        trace_select<<<nb(b.n), 256, 0, s>>>(sa);
-        if ((rc = check_hip(hipGetLastError(), "trace_select"))) return rc;
-    }
-    return LSM_OK;
+    return check_hip(hipGetLastError(), "reinit trace");
 }
 
 int launch_sign_ini(const Blk &b, const double *phi0, double eps, int m_sphi0, double *sgn, double *ng,

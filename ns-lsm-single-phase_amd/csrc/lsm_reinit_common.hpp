@@ -76,5 +76,64 @@ __device__ inline double normgrad_node(const Geo &g, const double *__restrict__ 
     return sqrt((gx * gx + gy * gy) + gz * gz);
 }
 
+// branch codes of the adjoint: bits 0-3, 4-7, 8-11 the difference formula of
+// each direction, kLive: the node's update depends on its stencil, kHasT: it
+// depends on s(phi0) (inside the interface band)
+constexpr uint16_t kLive = 0x8000;
+constexpr uint16_t kHasT = 0x4000;
+
+// |grad d| at node (i,j,k) (the formula of normgrad_node), the direction
+// weights w_d = (grad d . grad xi_d / |grad d|) x spacing and the branch codes
+// of the three difference formulas (4 bits each)
+__device__ inline double normgrad_branch(const Geo &g, const double *__restrict__ d, const double *__restrict__ phi0,
+                                  double phi0s, int order_bc, bool eno_bc, int i, int j, int k, long l,
+                                  double w[3], int &code)
+{
+    const int n[3] = {g.ni, g.nj, g.nk};
+    const int idx[3] = {i, j, k};
+    const long st[3] = {1, g.sj, g.sk};
+    bool on[3];
+    for (int a = 0; a < 3; ++a) on[a] = (idx[a] == 0 || idx[a] == n[a] - 1);
+    const double p0 = (on[0] && !on[1] && !on[2]) ? phi0s : phi0[l];
+    double gr[3], scale[3];
+    code = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const long s = st[a];
+        int br = 0;
+        scale[a] = g.dd[a];
+        if (on[a]) {
+            const int dir = (idx[a] == 0) ? 1 : -1;
+            const long ss = s * dir;
+            if (a == 0 && i == 0 && k == 0 && !on[1]) {
+                // edge i = 1, k = 1 (calc_NormGrad_d.F90:1281-1283)
+                scale[a] = 0.5 * g.dd[0];
+                gr[a] = scale[a] * ((-(1.0 * d[l + 2]) + 4.0 * d[l + 1]) - 3.0 * d[l]);
+            } else {
+                const double f0 = d[l], f1 = d[l + ss], f2 = d[l + 2 * ss], f3 = d[l + 3 * ss];
+                gr[a] = g.dd[a] * biased(f0, f1, f2, f3, order_bc, dir, eno_bc);
+                if (order_bc == 2 && eno_bc) br = eno2_boundary_branch(f0, f1, f2, dir);
+            }
+        } else {
+            const bool lo = (idx[a] == 1), hi = (idx[a] == n[a] - 2);
+            const double fLL = lo ? 0.0 : d[l - 2 * s];
+            const double fRR = hi ? 0.0 : d[l + 2 * s];
+            const double fL = d[l - s], fC = d[l], fR = d[l + s];
+            gr[a] = g.dd[a] * eno2(p0, fLL, fL, fC, fR, fRR, lo, hi);
+            br = eno2_branch(p0, fLL, fL, fC, fR, fRR, lo, hi);
+        }
+        code |= br << (4 * a);
+    }
+    const double gx = (g.m[0][l] * gr[0] + g.m[3][l] * gr[1]) + g.m[6][l] * gr[2];
+    const double gy = (g.m[1][l] * gr[0] + g.m[4][l] * gr[1]) + g.m[7][l] * gr[2];
+    const double gz = (g.m[2][l] * gr[0] + g.m[5][l] * gr[1]) + g.m[8][l] * gr[2];
+    const double ng = sqrt((gx * gx + gy * gy) + gz * gz);
+    const double inv = ng > 0.0 ? 1.0 / ng : 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        w[a] = ((gx * g.m[3 * a][l] + gy * g.m[3 * a + 1][l]) + gz * g.m[3 * a + 2][l]) * inv * scale[a];
+    return ng;
+}
+
 }  // namespace rn
 }  // namespace lsm

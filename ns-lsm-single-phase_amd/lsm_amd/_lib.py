@@ -135,6 +135,7 @@ LSM_KT_ADVECT_STAGE, LSM_KT_ADVECT_SHELL, LSM_KT_REINIT_STAGE, LSM_KT_VOLUME = 0
 LSM_KT_ADJOINT_STAGE = 4
 LSM_KT_BATCH_STAGE = 5
 LSM_KT_BATCH_REINIT = 6
+LSM_KT_REINIT_ADJOINT = 7
 
 _lib = None
 
