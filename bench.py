@@ -140,6 +140,30 @@ def bench_step_adjoint(g, phi, ucn, dt, blk):
                                "reverse sweep, gradients w.r.t. phi^n and U/J"}
 
 
+def bench_consumers(g, phi, blk):
+    """The caller-side consumers of phi at the bench size: calc_phi_gradient_ENO2
+    (phi + 9 metrics in, 3 components out: 104 B per node), calc_h (phi and z
+    in, h out: 24 B per node) and calc_h_gradient_ENO2 (h + 6 metrics in, 2
+    components out: 72 B per node)."""
+    import numpy as np
+    import torch
+    from lsm_amd import api
+    n = g.ni * g.nj * g.nk
+    z = torch.as_tensor(np.ascontiguousarray(g.z), device="cuda")
+    gp = [blk.zeros() for _ in range(3)]
+    gh = [blk.zeros() for _ in range(2)]
+    h = blk.empty()
+    res = {}
+    for name, fn, nbytes in (
+            ("phi_gradient", lambda: api.calc_phi_gradient_ENO2(blk, phi, 2, True, out=gp), 104),
+            ("calc_h", lambda: api.calc_h(blk, z, phi, out=h), 24),
+            ("h_gradient", lambda: api.calc_h_gradient_ENO2(blk, h, 2, True, out=gh), 72)):
+        ms = _time(fn, 10)
+        res[f"{name}_ms"] = round(ms, 4)
+        res[f"{name}_alg_GBps"] = round(n * nbytes / (ms / 1e3) / 1e9, 1)
+    return res
+
+
 def bench_batch(steps=10):
     """The 64-block workload: a 4x4x4 lattice of curvilinear blocks, sizes
     U{32..160} per lattice slab (seed 6), per-block sinusoidal perturbation
@@ -186,6 +210,56 @@ def bench_batch(steps=10):
         f"full {res['perblock_full_ms']} -> {res['graph_full_ms']} ms ({res['full_speedup_vs_perblock']}x)")
     del bt, ucn, p0, out, w_adv, w_full
     return res
+
+
+def bench_inverse(nsteps=60, every=10, ninst=32):
+    """The batched inverse problem: 32 solitary-wave channels (nx ~ U{128..512},
+    ny ~ U{16..64}, nz ~ U{48..128}, H ~ U[0.1, 0.5], seed 8) with one time
+    step (the CFL 0.4 step of the fastest instance) and 4 reinitialisation
+    iterations per step; loss = sum over instances of the mean squared misfit
+    of the surface elevation eta(x, y, t_end) against the instance's run from a
+    perturbed amplitude (seed 9); gradients with respect to each instance's
+    initial phi and amplitude H, checkpoint every 10 steps.  The forward runs
+    as one independent batch (one launch per stage over all instances); the
+    gradient runs instance by instance."""
+    import torch
+    from lsm_amd import _lib, batch as B, channels, inputs
+    t0 = time.time()
+    specs = channels.channel_specs(ninst, seed=8)
+    cb = channels.channel_batch(specs)
+    cells = sum(s.nx * s.ny * s.nz for s in specs)
+    lay = B.independent_layout([(s.nx, s.ny, s.nz) for s in specs])
+    bt = B.Batch(lay, cb.grids)
+    vel = [inputs.solitary_velocity(g, H=s.H) for g, s in zip(cb.grids, specs)]
+    ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vel]) for c in range(3)])
+    p0 = bt.pack([inputs.solitary_phi(g, H=s.H) for g, s in zip(cb.grids, specs)])
+    del vel
+    log(f"inverse inputs: {ninst} channels, {cells} cells, {time.time() - t0:.1f}s")
+    bufs = [p0.clone(), bt.empty()]
+    work = bt.workspace(_lib.LSM_OP_LEVELSET_STEP)
+
+    def forward():
+        for n in range(nsteps):
+            bt.levelset_step(bufs[n % 2], ucn, cb.dt, cb.params, out=bufs[(n + 1) % 2], work=work)
+    fwd_ms = _time(forward, 1, warm=1)
+    del bufs, work, bt, ucn, p0
+    torch.cuda.empty_cache()
+    Ht = channels.perturbed_amplitudes(specs, seed=9)
+    eta_t = channels.targets(cb, Ht, nsteps, every)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    J, g_phi, g_H = channels.loss_and_gradients(cb, eta_t, nsteps, every)
+    torch.cuda.synchronize()
+    grad_ms = (time.perf_counter() - t1) * 1e3
+    return {"workload": f"{ninst} solitary-wave channels (nx U{{128..512}}, ny U{{16..64}}, nz U{{48..128}}, "
+                        f"H U[0.1,0.5]), {nsteps} steps x (RK3 + 4 reinitialisation iterations + bcond), "
+                        f"surface-elevation misfit, gradients w.r.t. phi0 and H, checkpoint every {every}",
+            "cells": cells, "dt": cb.dt,
+            "batched_forward_ms": round(fwd_ms, 2),
+            "batched_forward_cell_updates_per_s": cells * nsteps / (fwd_ms / 1e3),
+            "loss": J, "dJ_dH_first": g_H[:4],
+            "forward_plus_gradient_ms": round(grad_ms, 1),
+            "gradient_over_batched_forward": round(grad_ms / fwd_ms, 3)}
 
 
 def bench_differentiation(n=192, nsteps=20, every=5):
@@ -266,7 +340,12 @@ def main():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-block batched workload")
     ap.add_argument("--no-diff", action="store_true", help="skip the 192^3 differentiation workload")
+    ap.add_argument("--no-inverse", action="store_true", help="skip the 32-channel inverse problem")
     ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "step-adjoint", "batch", "differentiation", "consumers", "inverse"],
+                    help="run only one secondary workload and print its figures (for profiling); "
+                         "'all' is the benchmark line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scales rocprof "
                          "FETCH_SIZE/WRITE_SIZE to bytes)")
@@ -299,6 +378,23 @@ def main():
         sys.exit("bench.py: no GPU visible")
     L = _lib.load()
     n = args.size
+    if args.workload != "all":
+        if args.workload == "batch":
+            res = bench_batch(args.batch_steps)
+        elif args.workload == "differentiation":
+            res = bench_differentiation()
+        elif args.workload == "inverse":
+            res = bench_inverse()
+        elif args.workload == "consumers":
+            g, phi_np, _, _ = make_case(n)
+            res = bench_consumers(g, torch.as_tensor(phi_np, device="cuda"), api.Block(g))
+        else:
+            g, phi_np, vel, dt = make_case(n)
+            blk = api.Block(g)
+            ucn = api.rhs_contra_j(blk, *(torch.as_tensor(np.ascontiguousarray(a), device="cuda") for a in vel))
+            res = bench_step_adjoint(g, torch.as_tensor(phi_np, device="cuda"), ucn, dt, blk)
+        print(json.dumps({"workload": args.workload, "result": res}), flush=True)
+        return
     t0 = time.time()
     g, phi_np, vel, dt = make_case(n)
     log(f"inputs {n}^3 built in {time.time() - t0:.1f}s (dt = {dt:.3e})")
@@ -460,6 +556,9 @@ def main():
         log(f"levelsetmethod (4 reinit iterations): {ls_ms:.2f} ms per call")
 
     if not args.no_extra:
+        extra["consumers"] = bench_consumers(g, bufs[base], blk)
+        log(f"consumers: {extra['consumers']}")
+    if not args.no_extra:
         extra.update(bench_step_adjoint(g, bufs[base], ucn, dt, blk))
         log(f"full step: fwd {extra['step_fwd_ms']:.3f} ms, adjoint {extra['step_adjoint_ms']:.3f} ms "
             f"({extra['step_adjoint_forward_ratio']:.2f}x)")
@@ -470,6 +569,11 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_extra and not args.no_diff:
         extra["differentiation"] = bench_differentiation()
+        torch.cuda.empty_cache()
+    if not args.no_extra and not args.no_inverse:
+        extra["inverse"] = bench_inverse()
+        log(f"inverse: forward {extra['inverse']['batched_forward_ms']} ms, forward + gradient "
+            f"{extra['inverse']['forward_plus_gradient_ms']} ms")
         torch.cuda.empty_cache()
 
     cpu = None

@@ -179,6 +179,47 @@ int lsm_bcond_adjoint(const LsmBlock *blk, const double *phi, double *adj, void 
 int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizero,
                       int order_bc, int eno_bc, double *normgrad, void *stream);
 
+/* calc_phi_gradient_ENO2.F90:1-2025 (calc_phi_gradient_ENO2, called every
+ * step by mg_driver.F90:106-116): grad phi = (g1, g2, g3) with the
+ * calc_NormGrad_d stencils, the node's own phi as the ENO2 upwind sign.
+ * Written on interior, face and edge nodes; the eight vertices keep the
+ * caller's values (the reference's output is intent(inout) and has no
+ * vertex blocks).  Needs the csi/eta/zet metrics of blk only. */
+int lsm_phi_gradient_eno2(const LsmBlock *blk, const double *phi, int order_bc, int eno_bc,
+                          double *g1, double *g2, double *g3, void *stream);
+
+/* calc_h_gradient_ENO2.F90:1-2018 (calc_h_gradient_ENO2,
+ * mg_driver.F90:140-150): the x and y components of grad h, same stencils
+ * and vertex rule as lsm_phi_gradient_eno2. */
+int lsm_h_gradient_eno2(const LsmBlock *blk, const double *h, int order_bc, int eno_bc,
+                        double *g1, double *g2, void *stream);
+
+/* calc_h.F90:1-394 with find_z_free_surface.F90:1-62 (calc_h,
+ * mg_driver.F90:127-135): in every (i,j) column, z_fs from the first k with
+ * phi(k) phi(k+1) <= 0 (at a node where |phi| < eps, midway between two such
+ * nodes, else linearly interpolated) and h = z_fs - z; h = 0 in a column
+ * without a free surface.  `missing` (device int32, may be NULL) receives the
+ * number of such columns (the reference prints 'free surface not found'
+ * for each). */
+int lsm_calc_h(const LsmBlock *blk, const double *z, const double *phi, double *h,
+               int32_t *missing, void *stream);
+
+/* find_z_free_surface.F90:1-62 over every (i,j) column, as calc_h.F90:101-135
+ * calls it: zfs (device, ni x nj, i fastest) receives z_free_surface of each
+ * column (0 in a column without a free surface, the value the reference
+ * leaves there); missing as in lsm_calc_h.  The surface elevation of a
+ * water column of depth d is zfs - d. */
+int lsm_free_surface(const LsmBlock *blk, const double *z, const double *phi, double *zfs,
+                     int32_t *missing, void *stream);
+
+/* Reverse mode of lsm_free_surface (the reference has no adjoint): adj_phi
+ * (written at every node) = (d zfs / d phi)^T adj_zfs.  Only the
+ * interpolating case of find_z_free_surface depends on phi (a surface at a
+ * node and a column without one contribute 0); the interval is the one the
+ * forward finds. */
+int lsm_free_surface_adjoint(const LsmBlock *blk, const double *z, const double *phi,
+                             const double *adj_zfs, double *adj_phi, void *stream);
+
 /* reinitialisation_test2.F90:1-218 (reinitialisation_test2, Sussman path,
  * non-hybrid): phi_out = reinitialised phi0.  iters_done (device int32, may be
  * NULL) receives the number of pseudo-time iterations that ran. */
@@ -418,6 +459,19 @@ int lsm_host_lambda_rn(const double *phi0, const double *normgrad0, const double
 int lsm_host_reinit_godunov(const double *phi0, double *phi_out, const LsmReinitParams *p);
 /* TotalWaterVolume.F90:1 */
 int lsm_host_water_volume(const double *phi, double vol[3], int32_t *adv_mask);
+/* The consumers of phi are external subroutines that receive their metrics as
+ * arguments (calc_phi_gradient_ENO2.F90:1-11, calc_h.F90:1-8,
+ * calc_h_gradient_ENO2.F90:1-11): ext = (il, iu, jl, ju, kl, ku, igp, jgp,
+ * kgp), metrics interleaved csi(1:3, ...), outputs interleaved grad(1:nc, ...)
+ * and updated in place (vertices keep their values). */
+int lsm_host_phi_gradient_eno2(const int32_t ext[9], int32_t phys_faces, double dc, double de, double dz,
+                               const double *csi, const double *eta, const double *zet, const double *phi,
+                               int order_bc, int eno_bc, double *grad);
+int lsm_host_h_gradient_eno2(const int32_t ext[9], int32_t phys_faces, double dc, double de, double dz,
+                             const double *csi, const double *eta, const double *zet, const double *h,
+                             int order_bc, int eno_bc, double *grad);
+int lsm_host_calc_h(const int32_t ext[9], int32_t phys_faces, const double *z, const double *phi,
+                    double *h, int32_t *missing);
 
 /* ---- instrumentation (no counterpart in the reference) ------------------
  * Per-kernel launch timing with HIP events recorded on the launch stream,

@@ -414,6 +414,62 @@ int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizer
     return launch_normgrad(b, d, phizero, order_bc, eno_bc, normgrad, (hipStream_t)stream);
 }
 
+static int gradient_entry(const LsmBlock *blk, const double *f, int nc, int order_bc, int eno_bc,
+                          double *g1, double *g2, double *g3, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    for (int q = 0; q < nc; ++q)
+        if (!blk->csi[q] || !blk->eta[q] || !blk->zet[q])
+            return fail(LSM_ERR_ARG, "metric pointer csi/eta/zet[%d] is NULL", q);
+    if (!f || !g1 || !g2 || (nc == 3 && !g3)) return fail(LSM_ERR_ARG, "NULL array");
+    if (order_bc < 1 || order_bc > 3) return fail(LSM_ERR_ARG, "order_bc = %d (1, 2 or 3)", order_bc);
+    return launch_gradient_eno2(b, f, nc, order_bc, eno_bc, g1, g2, g3, (hipStream_t)stream);
+}
+
+int lsm_phi_gradient_eno2(const LsmBlock *blk, const double *phi, int order_bc, int eno_bc, double *g1,
+                          double *g2, double *g3, void *stream)
+{
+    return gradient_entry(blk, phi, 3, order_bc, eno_bc, g1, g2, g3, stream);
+}
+
+int lsm_h_gradient_eno2(const LsmBlock *blk, const double *h, int order_bc, int eno_bc, double *g1,
+                        double *g2, void *stream)
+{
+    return gradient_entry(blk, h, 2, order_bc, eno_bc, g1, g2, nullptr, stream);
+}
+
+int lsm_calc_h(const LsmBlock *blk, const double *z, const double *phi, double *h, int32_t *missing,
+               void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!z || !phi || !h) return fail(LSM_ERR_ARG, "NULL array");
+    return launch_free_surface(b, z, phi, h, nullptr, missing, (hipStream_t)stream);
+}
+
+int lsm_free_surface(const LsmBlock *blk, const double *z, const double *phi, double *zfs, int32_t *missing,
+                     void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!z || !phi || !zfs) return fail(LSM_ERR_ARG, "NULL array");
+    return launch_free_surface(b, z, phi, nullptr, zfs, missing, (hipStream_t)stream);
+}
+
+int lsm_free_surface_adjoint(const LsmBlock *blk, const double *z, const double *phi, const double *adj_zfs,
+                             double *adj_phi, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!z || !phi || !adj_zfs || !adj_phi) return fail(LSM_ERR_ARG, "NULL array");
+    return launch_free_surface_adjoint(b, z, phi, adj_zfs, adj_phi, (hipStream_t)stream);
+}
+
 static int check_reinit(const LsmReinitParams *p)
 {
     if (!p) return fail(LSM_ERR_ARG, "LsmReinitParams is NULL");

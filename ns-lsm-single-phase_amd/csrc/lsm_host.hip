@@ -76,13 +76,126 @@ int up_aos(double *d0, double *d1, double *d2, const double *h, long n)
     return LSM_OK;
 }
 
+int sync() { return check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+// interleaved (nc, n) host array <-> nc device arrays
+int up_aosn(double *const *d, int nc, const double *h, long n, std::vector<double> &buf)
+{
+    buf.resize((size_t)n);
+    for (int c = 0; c < nc; ++c) {
+        for (long l = 0; l < n; ++l) buf[l] = h[nc * l + c];
+        int rc = up(d[c], buf.data(), n);
+        if (rc) return rc;
+    }
+    return LSM_OK;
+}
+
+int down_aosn(double *h, double *const *d, int nc, long n, std::vector<double> &buf)
+{
+    buf.resize((size_t)n);
+    for (int c = 0; c < nc; ++c) {
+        int rc = down(buf.data(), d[c], n);
+        if (rc) return rc;
+        for (long l = 0; l < n; ++l) h[nc * l + c] = buf[l];
+    }
+    return LSM_OK;
+}
+
+// The consumers of phi (calc_phi_gradient_ENO2, calc_h, calc_h_gradient_ENO2)
+// are external subroutines with their own argument lists, called outside
+// levelsetmethod: they get a mirror of their own, keyed by the extents and
+// the metric arrays they receive.
+struct ConsumerState {
+    int ext[9] = {0};
+    int faces = -1;
+    const void *key[3] = {nullptr};
+    bool have_metrics = false;
+    long n = 0;
+    double *dm[9] = {nullptr};
+    double *f[4] = {nullptr};
+    int32_t *cnt = nullptr;
+    std::vector<double> buf;
+};
+
+ConsumerState g_c;
+
+void consumer_release()
+{
+    for (auto &p : g_c.dm) { if (p) (void)hipFree(p); p = nullptr; }
+    for (auto &p : g_c.f) { if (p) (void)hipFree(p); p = nullptr; }
+    if (g_c.cnt) (void)hipFree(g_c.cnt);
+    g_c.cnt = nullptr;
+    g_c.n = 0;
+    g_c.have_metrics = false;
+}
+
+// validates the extents, (re)allocates the mirror and, when csi is given,
+// uploads the metrics unless the same arrays were uploaded last time
+int consumer_block(const int32_t ext[9], int32_t faces, double dc, double de, double dz, const double *csi,
+                   const double *eta, const double *zet, LsmBlock *b)
+{
+    if (!ext) return fail(LSM_ERR_ARG, "NULL argument");
+    *b = LsmBlock{};
+    b->il = ext[0]; b->iu = ext[1]; b->jl = ext[2]; b->ju = ext[3]; b->kl = ext[4]; b->ku = ext[5];
+    b->igp = ext[6]; b->jgp = ext[7]; b->kgp = ext[8];
+    b->phys_faces = faces;
+    for (int q = 0; q < 6; ++q) b->btype[q] = 1;
+    b->dc = dc; b->de = de; b->dz = dz;
+    Blk chk;
+    int rc = make_blk(b, &chk, false, false);
+    if (rc) return rc;
+    if (memcmp(g_c.ext, ext, sizeof g_c.ext) != 0 || g_c.faces != faces || !g_c.f[0]) {
+        consumer_release();
+        for (auto &p : g_c.dm) if ((rc = alloc((void **)&p, chk.n * sizeof(double)))) return rc;
+        for (auto &p : g_c.f) if ((rc = alloc((void **)&p, chk.n * sizeof(double)))) return rc;
+        if ((rc = alloc((void **)&g_c.cnt, sizeof(int32_t)))) return rc;
+        memcpy(g_c.ext, ext, sizeof g_c.ext);
+        g_c.faces = faces;
+        g_c.n = chk.n;
+    }
+    if (csi) {
+        const void *key[3] = {csi, eta, zet};
+        if (!g_c.have_metrics || memcmp(g_c.key, key, sizeof key) != 0) {
+            if ((rc = up_aosn(g_c.dm, 3, csi, g_c.n, g_c.buf)) || (rc = up_aosn(g_c.dm + 3, 3, eta, g_c.n, g_c.buf)) ||
+                (rc = up_aosn(g_c.dm + 6, 3, zet, g_c.n, g_c.buf)))
+                return rc;
+            memcpy(g_c.key, key, sizeof key);
+            g_c.have_metrics = true;
+        }
+        for (int c = 0; c < 3; ++c) {
+            b->csi[c] = g_c.dm[c];
+            b->eta[c] = g_c.dm[3 + c];
+            b->zet[c] = g_c.dm[6 + c];
+        }
+    }
+    return LSM_OK;
+}
+
+int consumer_gradient(const int32_t ext[9], int32_t faces, double dc, double de, double dz, const double *csi,
+                      const double *eta, const double *zet, const double *f, int nc, int order_bc, int eno_bc,
+                      double *grad)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!csi || !eta || !zet || !f || !grad) return fail(LSM_ERR_ARG, "NULL argument");
+    LsmBlock b;
+    int rc = consumer_block(ext, faces, dc, de, dz, csi, eta, zet, &b);
+    if (rc) return rc;
+    const long n = g_c.n;
+    if ((rc = up(g_c.f[0], f, n))) return rc;
+    if ((rc = up_aosn(g_c.f + 1, nc, grad, n, g_c.buf))) return rc;
+    rc = nc == 3 ? lsm_phi_gradient_eno2(&b, g_c.f[0], order_bc, eno_bc, g_c.f[1], g_c.f[2], g_c.f[3], nullptr)
+                 : lsm_h_gradient_eno2(&b, g_c.f[0], order_bc, eno_bc, g_c.f[1], g_c.f[2], nullptr);
+    if (rc) return rc;
+    if ((rc = sync())) return rc;
+    return down_aosn(grad, g_c.f + 1, nc, n, g_c.buf);
+}
+
 int need_ready()
 {
     if (!g_h.ready) return fail(LSM_ERR_ARG, "lsm_host_setup has not been called");
     return LSM_OK;
 }
 
-int sync() { return check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
 
 }  // namespace
 }  // namespace lsm
@@ -149,6 +262,7 @@ int lsm_host_release(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);
     release();
+    consumer_release();
     return LSM_OK;
 }
 
@@ -319,6 +433,38 @@ int lsm_host_water_volume(const double *phi, double vol[3], int32_t *adv_mask)
     if ((rc = check_hip(hipMemcpy(vol, g_h.vol, 3 * sizeof(double), hipMemcpyDeviceToHost), "D2H"))) return rc;
     if (adv_mask)
         return check_hip(hipMemcpy(adv_mask, g_h.mask, n * sizeof(int32_t), hipMemcpyDeviceToHost), "D2H");
+    return LSM_OK;
+}
+
+int lsm_host_phi_gradient_eno2(const int32_t ext[9], int32_t phys_faces, double dc, double de, double dz,
+                               const double *csi, const double *eta, const double *zet, const double *phi,
+                               int order_bc, int eno_bc, double *grad)
+{
+    return consumer_gradient(ext, phys_faces, dc, de, dz, csi, eta, zet, phi, 3, order_bc, eno_bc, grad);
+}
+
+int lsm_host_h_gradient_eno2(const int32_t ext[9], int32_t phys_faces, double dc, double de, double dz,
+                             const double *csi, const double *eta, const double *zet, const double *h,
+                             int order_bc, int eno_bc, double *grad)
+{
+    return consumer_gradient(ext, phys_faces, dc, de, dz, csi, eta, zet, h, 2, order_bc, eno_bc, grad);
+}
+
+int lsm_host_calc_h(const int32_t ext[9], int32_t phys_faces, const double *z, const double *phi, double *h,
+                    int32_t *missing)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!z || !phi || !h) return fail(LSM_ERR_ARG, "NULL argument");
+    LsmBlock b;
+    int rc = consumer_block(ext, phys_faces, 1.0, 1.0, 1.0, nullptr, nullptr, nullptr, &b);
+    if (rc) return rc;
+    const long n = g_c.n;
+    if ((rc = up(g_c.f[0], z, n)) || (rc = up(g_c.f[1], phi, n))) return rc;
+    if ((rc = lsm_calc_h(&b, g_c.f[0], g_c.f[1], g_c.f[2], g_c.cnt, nullptr))) return rc;
+    if ((rc = sync())) return rc;
+    if ((rc = down(h, g_c.f[2], n))) return rc;
+    if (missing)
+        return check_hip(hipMemcpy(missing, g_c.cnt, sizeof(int32_t), hipMemcpyDeviceToHost), "D2H");
     return LSM_OK;
 }
 

@@ -74,6 +74,16 @@ int launch_fold(const Blk &b, const double *U1, const double *U2, const double *
                 double *V1, double *V2, double *V3, int *bad, hipStream_t s);
 int launch_copy(const double *src, double *dst, long n, hipStream_t s);
 
+// ---- launchers (lsm_consumers.hip) ----
+// nc = 3: calc_phi_gradient_ENO2, nc = 2: calc_h_gradient_ENO2 (o2 unused)
+int launch_gradient_eno2(const Blk &b, const double *f, int nc, int order_bc, int eno_bc, double *o0,
+                         double *o1, double *o2, hipStream_t s);
+// calc_h (h may be NULL) and the per-column free-surface height (zfs may be NULL)
+int launch_free_surface(const Blk &b, const double *z, const double *phi, double *h, double *zfs,
+                        int32_t *missing, hipStream_t s);
+int launch_free_surface_adjoint(const Blk &b, const double *z, const double *phi, const double *adj_zfs,
+                                double *adj_phi, hipStream_t s);
+
 // ---- launchers (lsm_adjoint.hip) ----
 size_t rk3_adjoint_work(const Blk &b);
 // `stages` (optional): the three stage results of the forward step, recorded

@@ -34,21 +34,19 @@ __device__ __forceinline__ void node_of(const Geo &g, long l, int &i, int &j, in
     k = (int)(r / g.nj);
 }
 
-// |grad d| at node (i,j,k): calc_NormGrad_d.F90:39-2446 for a block whose six
-// faces are physical.  phi0s is phizero at the last interior node, which the
-// i = 1 / i = im face blocks use for their ENO2 upwind sign (those blocks never
-// assign phi0, calc_NormGrad_d.F90:187-367).
-__device__ inline double normgrad_node(const Geo &g, const double *__restrict__ d,
-                                const double *__restrict__ phi0, double phi0s, int order_bc,
-                                bool eno_bc, int i, int j, int k, long l)
+// Derivatives of d along xi, eta, zeta at node (i,j,k) (not a vertex of a
+// six-face block) with the stencils of calc_NormGrad_d.F90:39-2446: ENO2 with
+// upwind sign p0 where the direction is not normal to a physical face, the
+// one-sided formula of order order_bc on the face, and the fixed second-order
+// xi formula on the edge i = 1, k = 1 (calc_NormGrad_d.F90:1281-1283; the
+// same in calc_phi_gradient_ENO2.F90:1413-1415).
+__device__ __forceinline__ void curv_derivs(const Geo &g, const double *__restrict__ d, double p0,
+                                            int order_bc, bool eno_bc, int i, int j, int k, long l,
+                                            const bool on[3], double gr[3])
 {
     const int n[3] = {g.ni, g.nj, g.nk};
     const int idx[3] = {i, j, k};
     const long st[3] = {1, g.sj, g.sk};
-    bool on[3];
-    for (int a = 0; a < 3; ++a) on[a] = (idx[a] == 0 || idx[a] == n[a] - 1);
-    const double p0 = (on[0] && !on[1] && !on[2]) ? phi0s : phi0[l];
-    double gr[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const long s = st[a];
@@ -56,8 +54,6 @@ __device__ inline double normgrad_node(const Geo &g, const double *__restrict__ 
             const int dir = (idx[a] == 0) ? 1 : -1;
             const long ss = s * dir;
             if (a == 0 && i == 0 && k == 0 && !on[1]) {
-                // edge i = 1, k = 1: fixed second-order one-sided formula
-                // (calc_NormGrad_d.F90:1281-1283)
                 const double dc2 = 0.5 * g.dd[0];
                 gr[a] = dc2 * ((-(1.0 * d[l + 2]) + 4.0 * d[l + 1]) - 3.0 * d[l]);
             } else {
@@ -70,6 +66,23 @@ __device__ inline double normgrad_node(const Geo &g, const double *__restrict__ 
             gr[a] = g.dd[a] * eno2(p0, fLL, d[l - s], d[l], d[l + s], fRR, lo, hi);
         }
     }
+}
+
+// |grad d| at node (i,j,k): calc_NormGrad_d.F90:39-2446 for a block whose six
+// faces are physical.  phi0s is phizero at the last interior node, which the
+// i = 1 / i = im face blocks use for their ENO2 upwind sign (those blocks never
+// assign phi0, calc_NormGrad_d.F90:187-367).
+__device__ inline double normgrad_node(const Geo &g, const double *__restrict__ d,
+                                const double *__restrict__ phi0, double phi0s, int order_bc,
+                                bool eno_bc, int i, int j, int k, long l)
+{
+    bool on[3];
+    on[0] = (i == 0 || i == g.ni - 1);
+    on[1] = (j == 0 || j == g.nj - 1);
+    on[2] = (k == 0 || k == g.nk - 1);
+    const double p0 = (on[0] && !on[1] && !on[2]) ? phi0s : phi0[l];
+    double gr[3];
+    curv_derivs(g, d, p0, order_bc, eno_bc, i, j, k, l, on, gr);
     const double gx = (g.m[0][l] * gr[0] + g.m[3][l] * gr[1]) + g.m[6][l] * gr[2];
     const double gy = (g.m[1][l] * gr[0] + g.m[4][l] * gr[1]) + g.m[7][l] * gr[2];
     const double gz = (g.m[2][l] * gr[0] + g.m[5][l] * gr[1]) + g.m[8][l] * gr[2];

@@ -113,6 +113,40 @@ module lsm_hip_f
          integer(c_int32_t), intent(inout), optional :: mask(*)
       end function lsm_host_water_volume
 
+      ! consumers of phi outside levelsetmethod (external subroutines)
+      integer(c_int) function lsm_host_phi_gradient_eno2(ext, faces, dc, de, dz, csi, eta, zet, phi, &
+                                                         order_bc, eno_bc, grad)                   &
+                                                         bind(C, name="lsm_host_phi_gradient_eno2")
+         import :: c_int, c_int32_t, c_double
+         integer(c_int32_t), intent(in) :: ext(9)
+         integer(c_int32_t), value      :: faces
+         real(c_double), value          :: dc, de, dz
+         real(c_double), intent(in)     :: csi(*), eta(*), zet(*), phi(*)
+         integer(c_int), value          :: order_bc, eno_bc
+         real(c_double), intent(inout)  :: grad(*)
+      end function lsm_host_phi_gradient_eno2
+
+      integer(c_int) function lsm_host_h_gradient_eno2(ext, faces, dc, de, dz, csi, eta, zet, h, &
+                                                       order_bc, eno_bc, grad)                 &
+                                                       bind(C, name="lsm_host_h_gradient_eno2")
+         import :: c_int, c_int32_t, c_double
+         integer(c_int32_t), intent(in) :: ext(9)
+         integer(c_int32_t), value      :: faces
+         real(c_double), value          :: dc, de, dz
+         real(c_double), intent(in)     :: csi(*), eta(*), zet(*), h(*)
+         integer(c_int), value          :: order_bc, eno_bc
+         real(c_double), intent(inout)  :: grad(*)
+      end function lsm_host_h_gradient_eno2
+
+      integer(c_int) function lsm_host_calc_h(ext, faces, z, phi, h, missing) bind(C, name="lsm_host_calc_h")
+         import :: c_int, c_int32_t, c_double
+         integer(c_int32_t), intent(in)    :: ext(9)
+         integer(c_int32_t), value         :: faces
+         real(c_double), intent(in)        :: z(*), phi(*)
+         real(c_double), intent(inout)     :: h(*)
+         integer(c_int32_t), intent(inout) :: missing
+      end function lsm_host_calc_h
+
       type(c_ptr) function lsm_last_error() bind(C, name="lsm_last_error")
          import :: c_ptr
       end function lsm_last_error
@@ -159,6 +193,17 @@ contains
       call lsm_f_check(lsm_host_setup(e, faces, bt, dc, de, dz, big_phi, &
                                       csi, eta, zet, aj, x, y, z), 'lsm_host_setup')
    end subroutine lsm_f_bind
+
+   ! physical-face mask from the neighbour ranks (back, front, left, right,
+   ! down, up; a face is physical when its neighbour is proc_null)
+   integer(c_int32_t) function lsm_f_faces(nbr, proc_null) result(faces)
+      integer, intent(in) :: nbr(6), proc_null
+      integer :: f
+      faces = 0
+      do f = 1, 6
+         if (nbr(f) == proc_null) faces = ior(faces, int(ishft(1, f - 1), c_int32_t))
+      end do
+   end function lsm_f_faces
 
    ! text of the library's last error message
    function lsm_error_text() result(msg)

@@ -100,6 +100,11 @@ _SIGS = {
                                                      _vp, C.c_double, C.c_int, _vp, C.c_size_t, _vp, _vp, _vp, _vp,
                                                      _vp, _vp, C.c_size_t, _vp]),
     "lsm_normgrad_eno2": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, C.c_int, C.c_int, _vp, _vp]),
+    "lsm_phi_gradient_eno2": (C.c_int, [C.POINTER(LsmBlock), _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "lsm_h_gradient_eno2": (C.c_int, [C.POINTER(LsmBlock), _vp, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "lsm_calc_h": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp]),
+    "lsm_free_surface": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp]),
+    "lsm_free_surface_adjoint": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp]),
     "lsm_reinit_sussman_eno2": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp,
                                           _vp, _vp, C.c_size_t, _vp]),
     "lsm_sign_ini": (C.c_int, [C.POINTER(LsmBlock), _vp, C.c_double, C.c_int, _vp, _vp, _vp]),
@@ -128,6 +133,11 @@ _SIGS = {
     "lsm_host_lambda_rn": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp]),
     "lsm_host_reinit_godunov": (C.c_int, [_vp, _vp, C.POINTER(LsmReinitParams)]),
     "lsm_host_water_volume": (C.c_int, [_vp, _vp, _vp]),
+    "lsm_host_phi_gradient_eno2": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, _vp, _vp, _vp,
+                                             _vp, C.c_int, C.c_int, _vp]),
+    "lsm_host_h_gradient_eno2": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, _vp, _vp, _vp,
+                                           _vp, C.c_int, C.c_int, _vp]),
+    "lsm_host_calc_h": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp]),
 }
 
 EXPORTS = tuple(_SIGS)

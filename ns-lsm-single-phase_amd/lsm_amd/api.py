@@ -237,6 +237,69 @@ def calc_NormGrad_d(blk: Block, d, phizero, order_bc: int = 1, eno_bc: bool = Fa
     return out
 
 
+def calc_phi_gradient_ENO2(blk: Block, phi, order_bc: int = 1, eno_bc: bool = False, out=None):
+    """calc_phi_gradient_ENO2.F90:1-2025: (dphi/dx, dphi/dy, dphi/dz) with the
+    calc_NormGrad_d stencils and the node's own phi as the ENO2 upwind sign.
+    The eight vertices are not written: they keep what `out` holds (zeros
+    when `out` is None)."""
+    if out is None:
+        out = [blk.zeros() for _ in range(3)]
+    check(_lib.load().lsm_phi_gradient_eno2(blk.ref, _f64(phi, blk.shape, "phi"), int(order_bc),
+                                            int(bool(eno_bc)), *[_f64(o, blk.shape, "out") for o in out],
+                                            _stream()), "lsm_phi_gradient_eno2")
+    return out
+
+
+def calc_h_gradient_ENO2(blk: Block, h, order_bc: int = 1, eno_bc: bool = False, out=None):
+    """calc_h_gradient_ENO2.F90:1-2018: (dh/dx, dh/dy), vertices as in
+    calc_phi_gradient_ENO2."""
+    if out is None:
+        out = [blk.zeros() for _ in range(2)]
+    check(_lib.load().lsm_h_gradient_eno2(blk.ref, _f64(h, blk.shape, "h"), int(order_bc), int(bool(eno_bc)),
+                                          *[_f64(o, blk.shape, "out") for o in out], _stream()),
+          "lsm_h_gradient_eno2")
+    return out
+
+
+def calc_h(blk: Block, z, phi, out=None):
+    """calc_h.F90:1-394 with find_z_free_surface.F90: h = z_fs - z in every
+    column (0 where the column has no free surface).  Returns (h, missing)
+    with missing a one-element int32 CUDA tensor: the number of columns
+    without a free surface."""
+    if out is None:
+        out = blk.empty()
+    missing = torch.zeros(1, dtype=torch.int32, device=blk.device)
+    check(_lib.load().lsm_calc_h(blk.ref, _f64(z, blk.shape, "z"), _f64(phi, blk.shape, "phi"),
+                                 _f64(out, blk.shape, "out"), C.c_void_p(missing.data_ptr()), _stream()),
+          "lsm_calc_h")
+    return out, missing
+
+
+def free_surface(blk: Block, z, phi, out=None):
+    """find_z_free_surface over every column (as calc_h obtains z_free_surface):
+    returns (zfs, missing), zfs of shape (nj, ni) (0 in a column without a
+    free surface), missing a one-element int32 CUDA tensor."""
+    nk, nj, ni = blk.shape
+    if out is None:
+        out = torch.empty((nj, ni), dtype=torch.float64, device=blk.device)
+    missing = torch.zeros(1, dtype=torch.int32, device=blk.device)
+    check(_lib.load().lsm_free_surface(blk.ref, _f64(z, blk.shape, "z"), _f64(phi, blk.shape, "phi"),
+                                       _f64(out, (nj, ni), "out"), C.c_void_p(missing.data_ptr()), _stream()),
+          "lsm_free_surface")
+    return out, missing
+
+
+def free_surface_adjoint(blk: Block, z, phi, adj_zfs, out=None):
+    """Reverse mode of free_surface: (d zfs / d phi)^T adj_zfs at every node."""
+    nk, nj, ni = blk.shape
+    if out is None:
+        out = blk.empty()
+    check(_lib.load().lsm_free_surface_adjoint(blk.ref, _f64(z, blk.shape, "z"), _f64(phi, blk.shape, "phi"),
+                                               _f64(adj_zfs, (nj, ni), "adj_zfs"), _f64(out, blk.shape, "out"),
+                                               _stream()), "lsm_free_surface_adjoint")
+    return out
+
+
 def reinitialisation_test2(blk: Block, phi0, params: ReinitParams, out=None, work=None):
     """reinitialisation_test2.F90:1-218 (non-hybrid): returns (phi_reinitialised,
     iterations) with iterations a one-element int32 CUDA tensor."""

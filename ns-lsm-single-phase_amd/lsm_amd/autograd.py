@@ -17,6 +17,13 @@
   (lsm_levelset_step_adjoint); levelset_trajectory(...) runs nsteps of them
   with checkpointing every k steps.
 
+* free_surface(blk, phi): the free-surface height z_fs of every column
+  (find_z_free_surface, as calc_h obtains it) with its adjoint
+  (lsm_free_surface_adjoint); the surface elevation is z_fs - d.
+* solitary_phi / solitary_velocity(blk, H): the solitary-wave channel's
+  initial level set and frozen velocity in torch operations, differentiable
+  in the amplitude H.
+
 The adjoint differentiates the WENO3 nonlinear weights and s(phi0); the upwind
 choice, the forced boundary stencils, the BigPhi switch of bcond_lsm, the ENO2
 and one-sided branches of |grad d|, the reinitialisation update mask and its
@@ -257,3 +264,51 @@ def fourier_velocity(blk, ks, es, amps, psis):
         v = v + amps[m] * float(e[1]) * ph
         w = w + amps[m] * float(e[2]) * ph
     return u, v, w
+
+
+class _FreeSurface(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, phi, blk, z):
+        zfs, _ = api.free_surface(blk, z, phi.detach().contiguous())
+        ctx.blk, ctx.z = blk, z
+        ctx.save_for_backward(phi.detach().contiguous())
+        return zfs
+
+    @staticmethod
+    def backward(ctx, g):
+        (phi,) = ctx.saved_tensors
+        return api.free_surface_adjoint(ctx.blk, ctx.z, phi, g.contiguous()), None, None
+
+
+def free_surface(blk, phi, z: Optional[torch.Tensor] = None):
+    """z_fs of every (i, j) column, shape (nj, ni): the lowest sign change of
+    phi in z, linearly interpolated (find_z_free_surface.F90:16-58; 0 in a
+    column without one).  Differentiable in phi: only the interpolating case
+    depends on phi, along the interval the forward found."""
+    return _FreeSurface.apply(phi, blk, blk.z if z is None else z)
+
+
+def _solitary(blk, H, d, x0):
+    """sech^2 profile of the first-order solitary wave (the formulas of
+    inputs.solitary_wave) in torch operations."""
+    k = torch.sqrt(3.0 * H / (4.0 * d ** 3))
+    arg = k * (blk.x - x0)
+    s = 1.0 / torch.cosh(arg)
+    eta = H * s * s
+    deta = -2.0 * H * k * s * s * torch.tanh(arg)
+    return eta, deta
+
+
+def solitary_phi(blk, H, d: float = 1.0, x0: float = 5.0):
+    """phi = z - (d + H sech^2(k (x - x0))), k = sqrt(3 H / (4 d^3))."""
+    eta, _ = _solitary(blk, H, d, x0)
+    return blk.z - (d + eta)
+
+
+def solitary_velocity(blk, H, d: float = 1.0, x0: float = 5.0, gacc: float = 9.81):
+    """u = c eta / d, v = 0, w = -(c / d) eta'(x) z, c = sqrt(g (d + H))."""
+    c = torch.sqrt(gacc * (d + H))
+    eta, deta = _solitary(blk, H, d, x0)
+    u = c * eta / d
+    w = -(c / d) * deta * blk.z
+    return u, torch.zeros_like(u), w

@@ -756,6 +756,117 @@ void oracle_water_volume(const OBlock *b, const double *phi, double vol[3], int3
 }
 
 /* ------------------------------------------------------------------------- */
+/* Caller-side consumers of phi (mg_driver.F90:106-150).                      */
+/* calc_phi_gradient_ENO2.F90:1-2025 (nc = 3: gradient of phi) and            */
+/* calc_h_gradient_ENO2.F90:1-2018 (nc = 2: x and y components of grad h):   */
+/* the calc_NormGrad_d stencils with the node's own value as the ENO2         */
+/* upwind sign (GetENO2Reconstruction(phiC, ...), calc_phi_gradient_ENO2.F90: */
+/* 178,195,212) and no stale-phi0 face quirk; interior, face and edge nodes   */
+/* are written, the eight vertices are not (the routines have no vertex       */
+/* blocks: their outputs are intent(inout) and keep the caller's values).    */
+/* The edge i = 1, k = 1 keeps the fixed second-order xi formula              */
+/* (calc_phi_gradient_ENO2.F90:1413-1415, calc_h_gradient_ENO2.F90:1406).    */
+/* ------------------------------------------------------------------------- */
+void oracle_gradient_eno2(const OBlock *b, const double *f, double *g0, double *g1, double *g2,
+                          int nc, int order_bc, int eno_bc)
+{
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    const int n[3] = {ni, nj, nk};
+    const size_t st[3] = {1, (size_t)ni, (size_t)ni * nj};
+    const double dd[3] = {b->dc, b->de, b->dz};
+    double *out[3] = {g0, g1, g2};
+    for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+    for (int i = 0; i < ni; ++i) {
+        const int idx[3] = {i, j, k};
+        const size_t l = IX(b, i, j, k);
+        int on[3];
+        for (int a = 0; a < 3; ++a) on[a] = (idx[a] == 0 || idx[a] == n[a] - 1);
+        if (on[0] && on[1] && on[2]) continue;
+        double g[3];
+        for (int a = 0; a < 3; ++a) {
+            const size_t s = st[a];
+            if (on[a]) {
+                const int dir = (idx[a] == 0) ? 1 : -1;
+                if (a == 0 && i == 0 && k == 0 && !on[1]) {
+                    const double dc2 = 0.5 * b->dc;
+                    g[a] = dc2 * ((-(1.0 * f[l + 2]) + 4.0 * f[l + 1]) - 3.0 * f[l]);
+                } else {
+                    const long ss = (long)s * dir;
+                    g[a] = dd[a] * oracle_biased(f[l], f[l + ss], f[l + 2 * ss], f[l + 3 * ss],
+                                                 order_bc, dir, eno_bc);
+                }
+            } else {
+                const int lo = (idx[a] == 1), hi = (idx[a] == n[a] - 2);
+                const double fLL = lo ? 0.0 : f[l - 2 * s];
+                const double fRR = hi ? 0.0 : f[l + 2 * s];
+                g[a] = dd[a] * oracle_eno2(f[l], fLL, f[l - s], f[l], f[l + s], fRR, lo, hi);
+            }
+        }
+        for (int c = 0; c < nc; ++c)
+            out[c][l] = (b->csi[c][l] * g[0] + b->eta[c][l] * g[1]) + b->zet[c][l] * g[2];
+    }
+}
+
+/* find_z_free_surface.F90:1-62: the first k (from the bottom) with          */
+/* phi(k) * phi(k+1) <= 0 and the free-surface height in that interval.      */
+/* Returns 0 when no such interval exists.                                    */
+static int find_z_free_surface(const double *z, const double *phi, size_t sk, int nk, double *zfs)
+{
+    for (int k = 0; k < nk - 1; ++k) {
+        const double p0 = phi[k * sk], p1 = phi[(k + 1) * sk];
+        if (p0 * p1 <= 0.0) {
+            const double z0 = z[k * sk], z1 = z[(k + 1) * sk];
+            if (fabs(p0) < EPS_SIMS && fabs(p1) > EPS_SIMS) *zfs = z0;
+            else if (fabs(p1) < EPS_SIMS && fabs(p0) > EPS_SIMS) *zfs = z1;
+            else if (fabs(p1) < EPS_SIMS && fabs(p0) < EPS_SIMS) *zfs = 0.5 * (z1 + z0);
+            else *zfs = z0 + ((z1 - z0) * (-p0)) / (p1 - p0);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* calc_h.F90:101-185: h(i,j,k) = z_fs(i,j) - z(i,j,k) over every column,     */
+/* h = 0 in a column without a free surface (the reference prints 'free       */
+/* surface not found' there).  Returns the number of such columns.            */
+int oracle_calc_h(const OBlock *b, const double *z, const double *phi, double *h)
+{
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    const size_t sk = (size_t)ni * nj;
+    int missing = 0;
+    for (int j = 0; j < nj; ++j)
+    for (int i = 0; i < ni; ++i) {
+        const size_t l0 = IX(b, i, j, 0);
+        double zfs = 0.0;
+        if (find_z_free_surface(z + l0, phi + l0, sk, nk, &zfs)) {
+            for (int k = 0; k < nk; ++k) h[l0 + k * sk] = zfs - z[l0 + k * sk];
+        } else {
+            ++missing;
+            for (int k = 0; k < nk; ++k) h[l0 + k * sk] = 0.0;
+        }
+    }
+    return missing;
+}
+
+/* z_free_surface of every column as calc_h.F90:119-135 obtains it (0 where   */
+/* find_z_free_surface finds no interval).  zfs: ni x nj, i fastest.          */
+int oracle_free_surface(const OBlock *b, const double *z, const double *phi, double *zfs)
+{
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    const size_t sk = (size_t)ni * nj;
+    int missing = 0;
+    for (int j = 0; j < nj; ++j)
+    for (int i = 0; i < ni; ++i) {
+        const size_t l0 = IX(b, i, j, 0);
+        double v = 0.0;
+        if (!find_z_free_surface(z + l0, phi + l0, sk, nk, &v)) { ++missing; v = 0.0; }
+        zfs[(size_t)i + (size_t)ni * j] = v;
+    }
+    return missing;
+}
+
+/* ------------------------------------------------------------------------- */
 /* R1: levelsetmethod, levelsetmethod.F90:135-326 (non-hybrid Sussman path,   */
 /* no global mass correction).  phi: in = phi^n, out = phi^(n+1).  Returns    */
 /* the pre-advection water volume (the value WriteTotalWaterVolume appends).  */

@@ -59,6 +59,11 @@ def lib():
         L.oracle_levelset_step.argtypes = [bp, rp, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                                            C.c_int, C.c_int, C.c_int, C.c_int]
         L.oracle_levelset_step.restype = C.c_double
+        L.oracle_gradient_eno2.argtypes = [bp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
+        L.oracle_calc_h.argtypes = [bp, _dp, _dp, _dp]
+        L.oracle_calc_h.restype = C.c_int
+        L.oracle_free_surface.argtypes = [bp, _dp, _dp, _dp]
+        L.oracle_free_surface.restype = C.c_int
         L.oracle_weno3.argtypes = [C.c_double] * 5
         L.oracle_weno3.restype = C.c_double
         L.oracle_eno2.argtypes = [C.c_double] * 6 + [C.c_int, C.c_int]
@@ -152,6 +157,34 @@ def normgrad(blk, d, phi0, order_bc=1, eno_bc=False):
     ng = np.zeros(blk.shape)
     lib().oracle_normgrad(blk.ref, _p(d), _p(phi0), _p(ng), int(order_bc), int(bool(eno_bc)))
     return ng
+
+
+def gradient_eno2(blk, f, init, nc=3, order_bc=1, eno_bc=False):
+    """calc_phi_gradient_ENO2 (nc = 3) / calc_h_gradient_ENO2 (nc = 2) of f;
+    `init` (nc arrays) is what the output holds where the routine does not
+    write (the eight vertices)."""
+    f = _c(f)
+    out = [_c(a).copy() for a in init]
+    ptr = [_p(a) for a in out] + [None] * (3 - nc)
+    lib().oracle_gradient_eno2(blk.ref, _p(f), ptr[0], ptr[1], ptr[2], int(nc), int(order_bc),
+                               int(bool(eno_bc)))
+    return out
+
+
+def calc_h(blk, z, phi):
+    """calc_h: (h, number of columns without a free surface)."""
+    z, phi = _c(z), _c(phi)
+    h = np.empty(blk.shape)
+    missing = lib().oracle_calc_h(blk.ref, _p(z), _p(phi), _p(h))
+    return h, missing
+
+
+def free_surface(blk, z, phi):
+    """z_free_surface of every column, shape (nj, ni); (zfs, missing)."""
+    z, phi = _c(z), _c(phi)
+    zfs = np.empty(blk.shape[1:])
+    missing = lib().oracle_free_surface(blk.ref, _p(z), _p(phi), _p(zfs))
+    return zfs, missing
 
 
 def reinit_eno2(blk, phi0, epslsm, eps_reinit, rn_timestep, rn_itermax, order_bc=1, eno_bc=False):

@@ -17,6 +17,9 @@
 ! independently (all faces are physical boundaries on every rank).
 !
 ! Mode 13 calls the compiled mg_metrics (metrics of the grid x, y, z).
+! Modes 14-16 call the compiled caller-side consumers of phi, external
+! subroutines of the reference (mg_driver.F90:106-150): calc_phi_gradient_ENO2,
+! calc_h and calc_h_gradient_ENO2.
 !
 ! I/O (stream, little endian):
 !   input : int32 ip(64), real64 rp(32), then ni*nj*nk-sized arrays
@@ -97,6 +100,11 @@ program lsm_ref_harness
    nblk  = 0
    nblke = 0
    is_obstacle = 0
+   ! the 1-D line communicators of one rank (calc_h reads comm1d_csi(1)%myrank)
+   allocate(comm1d_csi(1), comm1d_eta(1), comm1d_zet(1))
+   comm1d_csi(1) = cart_comm_1d(0, 1, mpi_proc_null, mpi_proc_null, mpi_comm_self)
+   comm1d_eta(1) = comm1d_csi(1)
+   comm1d_zet(1) = comm1d_csi(1)
 
    ! LSM parameters (init_LSM.F90:136-181 semantics)
    allocate(btype_lsm(6,1))
@@ -153,6 +161,9 @@ program lsm_ref_harness
    if (ip(1) == 13) then
       ! mg_metrics itself on the grid x, y, z (mg_metrics.F90:53-262)
       call metrics_driver( ni, nj, nk, x, y, z, rp(1), rp(2), rp(3), trim(fout) )
+   else if (ip(1) >= 14 .and. ip(1) <= 16) then
+      call consumer_driver( ni, nj, nk, rp(1), rp(2), rp(3), csi, eta, zet, z, &
+                            fphi, faux, ip(1), trim(fout) )
    else if (ip(1) == 12) then
       ! levelsetmethod itself, called as mg_driver.F90:563-581 does
       do kk = 1, max(ip(5), 1)
@@ -227,6 +238,57 @@ subroutine metrics_driver( ni, nj, nk, xin, yin, zin, dcin, dein, dzin, fout )
    close(22)
 
 end subroutine metrics_driver
+
+
+subroutine consumer_driver( ni, nj, nk, dc, de, dz, csi, eta, zet, z, phi, aux, mode, fout )
+
+   ! One block, no ghost layers, every face physical (the module state set by
+   ! the main program).  The gradient outputs are intent(inout) in the
+   ! reference and not written at the eight vertices, so they start as
+   ! aux + c (component c); calc_h's output starts as aux.
+   use precision
+   implicit none
+   integer, intent(in) :: ni, nj, nk, mode
+   real (kind = rdf), intent(in) :: dc, de, dz
+   real (kind = rdf), intent(in) :: csi(3,ni,nj,nk), eta(3,ni,nj,nk), zet(3,ni,nj,nk)
+   real (kind = rdf), intent(in) :: z(ni,nj,nk), phi(ni,nj,nk), aux(ni,nj,nk)
+   character(len=*), intent(in) :: fout
+   real (kind = rdf), allocatable :: g(:,:,:,:), h(:,:,:)
+   integer :: c, nc, n
+
+   n = ni * nj * nk
+   open(22, file=fout, access='stream', form='unformatted', status='replace')
+   select case (mode)
+   case (14, 16)   ! calc_phi_gradient_ENO2( phi ) / calc_h_gradient_ENO2( h = phi )
+      nc = 3
+      if (mode == 16) nc = 2
+      allocate(g(nc,ni,nj,nk))
+      do c = 1, nc
+         g(c,:,:,:) = aux + real(c, rdf)
+      end do
+      if (mode == 14) then
+         call calc_phi_gradient_ENO2( 1, ni, 1, nj, 1, nk, 0, 0, 0, dc, de, dz, &
+                                      csi, eta, zet, phi, g )
+      else
+         call calc_h_gradient_ENO2( 1, ni, 1, nj, 1, nk, 0, 0, 0, dc, de, dz, &
+                                    csi, eta, zet, phi, g )
+      end if
+      write(22) int(nc,4)
+      do c = 1, nc
+         write(22) int(0,4), int(n,4)
+         write(22) g(c,:,:,:)
+      end do
+   case (15)       ! calc_h( z, phi ) -> h
+      allocate(h(ni,nj,nk))
+      h = aux
+      call calc_h( 1, ni, 1, nj, 1, nk, 0, 0, 0, z, phi, h )
+      write(22) int(1,4)
+      write(22) int(0,4), int(n,4)
+      write(22) h
+   end select
+   close(22)
+
+end subroutine consumer_driver
 
 
 subroutine ref_driver( il, iu, jl, ju, kl, ku, igp, jgp, kgp, dc, de, dz, &

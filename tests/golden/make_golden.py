@@ -170,10 +170,63 @@ def volume_case():
          extra_inputs=extra)
 
 
+def special_columns(g, phi):
+    """phi with columns (i, j = 0) that take every branch of
+    find_z_free_surface.F90:16-58: no sign change (all positive, all
+    negative), a zero node above a wet one (case 2), |phi| < eps at the
+    bottom above a dry node (case 1), two tiny values (case 3), a product
+    that underflows to zero between two positive values (case 4 with equal
+    signs), and a sign change exactly between two grid nodes."""
+    p = phi.copy()
+    nk = p.shape[0]
+    p[:, 0, 0] = 1.0 + np.arange(nk)                   # no free surface (dry)
+    p[:, 0, 1] = -1.0 - np.arange(nk)                  # no free surface (wet)
+    p[:, 0, 2] = np.arange(nk) - 3.0                   # phi = 0 exactly at k = 3
+    p[:, 0, 3] = np.arange(nk) - 3.0
+    p[0, 0, 3] = 1e-17                                 # case 1 at the bottom pair
+    p[:, 0, 4] = np.arange(nk) + 1.0
+    p[0, 0, 4], p[1, 0, 4] = 1e-17, -1e-17             # case 3
+    p[:, 0, 5] = np.arange(nk) + 1.0
+    p[4, 0, 5], p[5, 0, 5] = 1e-200, 2e-200            # product underflows to 0
+    p[:, 0, 6] = np.arange(nk) - 4.5                   # plain interpolation
+    return p
+
+
+def consumers_case():
+    """Caller-side consumers of phi: calc_phi_gradient_ENO2, calc_h and
+    calc_h_gradient_ENO2 (all boundary orders, ENO boundary stencils on and
+    off), on the curvilinear cube and on the solitary-wave channel with
+    columns that exercise every branch of the free-surface search."""
+    g = inputs.sinusoidal_cube_grid(11, amp=0.04, seed=2, nj=12, nk=13)
+    phi = inputs.spheres_sdf(g, nspheres=3, seed=3, rmin=0.2, rmax=0.3)
+    aux = np.random.default_rng(21).uniform(-1.0, 1.0, g.shape)
+    zeros = np.zeros(g.shape)
+    params = dict(btype=(1, 1, 1, 1, 1, 1))
+    runs = []
+    for ob in (1, 2, 3):
+        for eb in (0, 1):
+            runs.append((f"phigrad_o{ob}_e{eb}", 14, {"order_reinit_bc": ob, "eno_bc": eb}, {"aux": aux}))
+            runs.append((f"hgrad_o{ob}_e{eb}", 16, {"order_reinit_bc": ob, "eno_bc": eb}, {"aux": aux}))
+    case("consumers_cube", g, phi, phi, zeros, zeros, zeros, params, runs, extra_inputs={"aux": aux})
+
+    c = inputs.channel_grid(17, 6, 19)
+    cphi = special_columns(c, inputs.solitary_phi(c, H=0.3))
+    caux = np.random.default_rng(22).uniform(-1.0, 1.0, c.shape)
+    czeros = np.zeros(c.shape)
+    (h,), _, _ = refio.run(15, c, cphi, aux=caux)
+    runs = [("calc_h", 15, {}, {"aux": caux}),
+            ("phigrad_o2_e1", 14, {"order_reinit_bc": 2, "eno_bc": 1}, {"aux": caux}),
+            ("hgrad_o2_e1", 16, {"order_reinit_bc": 2, "eno_bc": 1}, {"phi": h, "aux": caux}),
+            ("hgrad_o1_e0", 16, {"order_reinit_bc": 1, "eno_bc": 0}, {"phi": h, "aux": caux})]
+    case("consumers_channel", c, cphi, cphi, czeros, czeros, czeros, params, runs,
+         extra_inputs={"aux": caux, "h": h})
+
+
+CASES = {"curvilinear": curvilinear_case, "channel": channel_case, "boundary": boundary_case,
+         "volume": volume_case, "consumers": consumers_case}
+
 if __name__ == "__main__":
     if not refio.available():
         sys.exit("reference harness not built: make -C oracle/ref")
-    curvilinear_case()
-    channel_case()
-    boundary_case()
-    volume_case()
+    for name in (sys.argv[1:] or list(CASES)):
+        CASES[name]()

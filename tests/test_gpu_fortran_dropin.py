@@ -134,6 +134,52 @@ def test_dropin_levelsetmethod(name):
     assert np.max(np.abs(extra["global_volume"] - gv)) <= 1e-12 * np.max(gv)
 
 
+# consumers of phi (external subroutines): (fixture, key, harness mode, phi input)
+CONSUMERS = [("consumers_cube", f"phigrad_o{o}_e{e}", "phi_gradient", "phi") for o in (1, 2, 3) for e in (0, 1)] + \
+            [("consumers_cube", f"hgrad_o{o}_e{e}", "h_gradient", "phi") for o in (1, 2, 3) for e in (0, 1)] + \
+            [("consumers_channel", "calc_h", "calc_h", "phi"),
+             ("consumers_channel", "phigrad_o2_e1", "phi_gradient", "phi"),
+             ("consumers_channel", "hgrad_o2_e1", "h_gradient", "h"),
+             ("consumers_channel", "hgrad_o1_e0", "h_gradient", "h")]
+
+
+def replay_consumer(fx, key, mode, phi_key, harness):
+    print(f"[dropin] {fx.name}/{key} mode={mode} {os.path.basename(harness)}", flush=True)
+    outs, _, _ = refio.run(mode, fx.grid, fx.inp(phi_key), aux=fx.inp("aux"),
+                           order_reinit_bc=fx.kw(key, "order_reinit_bc", 1),
+                           eno_bc=fx.kw(key, "eno_bc", 0), harness=harness)
+    return outs
+
+
+@pytest.mark.parametrize("name,key,mode,phi_key", CONSUMERS, ids=[f"{c[0]}-{c[1]}" for c in CONSUMERS])
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(refio.SHIM_HARNESS),
+                    reason="drop-in harness not built (make -C oracle/ref shim)")
+def test_dropin_consumers_bitwise(name, key, mode, phi_key):
+    """calc_phi_gradient_ENO2, calc_h and calc_h_gradient_ENO2 replaced by
+    their drop-ins: the outputs (vertices included, which keep the caller's
+    values) equal the reference's bit for bit."""
+    fx = load(name)
+    outs = replay_consumer(fx, key, mode, phi_key, refio.SHIM_HARNESS)
+    n = 0
+    while f"out_{key}_{n}" in fx.d:
+        assert np.array_equal(outs[n], fx.out(key, n)), (key, n, linf_rel(outs[n], fx.out(key, n)))
+        n += 1
+    assert n == len(outs)
+
+
+@pytest.mark.skipif(not refio.available(), reason="reference harness not built (make -C oracle/ref)")
+@pytest.mark.parametrize("name,key,mode,phi_key", CONSUMERS, ids=[f"{c[0]}-{c[1]}" for c in CONSUMERS])
+def test_consumer_replay_reproduces_golden_with_reference(name, key, mode, phi_key):
+    fx = load(name)
+    outs = replay_consumer(fx, key, mode, phi_key, refio.HARNESS)
+    n = 0
+    while f"out_{key}_{n}" in fx.d:
+        assert np.array_equal(outs[n], fx.out(key, n)), (key, n)
+        n += 1
+    assert n == len(outs)
+
+
 @pytest.mark.skipif(not refio.available(), reason="reference harness not built (make -C oracle/ref)")
 @pytest.mark.parametrize("name,key,mode,over", EXACT + CLOSE, ids=[f"{c[0]}-{c[1]}" for c in EXACT + CLOSE])
 def test_replay_reproduces_golden_with_reference(name, key, mode, over):

@@ -1,0 +1,168 @@
+"""The surface-elevation inverse problem over many solitary-wave channels
+(uneven grids and amplitudes, a common time step), in small: checkpointed
+full-step trajectories, the free-surface height of every column and the
+misfit of the surface elevation, with gradients with respect to each
+instance's initial phi and amplitude H.
+
+CPU: the free-surface adjoint formula against central differences of the
+oracle's find_z_free_surface restatement.  GPU: the free-surface height and
+its adjoint against the oracle; the loss against the oracle forward; dJ/dH
+and directional derivatives dJ/dphi0 . v against central differences of the
+oracle forward (1e-6 relative); the independent batch forward equal to the
+per-instance steps bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from lsm_amd import channels, inputs
+
+SPECS = [channels.ChannelSpec(24, 5, 14, 0.32), channels.ChannelSpec(28, 6, 12, 0.18),
+         channels.ChannelSpec(20, 7, 16, 0.45)]
+NSTEPS, EVERY = 4, 2
+
+
+def _fd(fun, h):
+    return (fun(h) - fun(-h)) / (2.0 * h)
+
+
+def _fd_smooth(fun, h=1e-6, hmin=1e-10):
+    """Central difference at step h, divided by 10 while the differences at h
+    and h/10 disagree (a switch of the forward within the step)."""
+    f1 = _fd(fun, h)
+    while h > hmin:
+        f2 = _fd(fun, h / 10.0)
+        if abs(f1 - f2) <= 1e-7 * max(abs(f1), abs(f2)):
+            return f1
+        h, f1 = h / 10.0, f2
+    return f1
+
+
+def _oracle_eta(g, H, dphi, dt, prm, nsteps, depth=1.0):
+    blk = O.Block(g)
+    phi = inputs.solitary_phi(g, H=H) + dphi
+    u, v, w = inputs.solitary_velocity(g, H=H)
+    for n in range(nsteps):
+        phi, _ = O.levelset_step(blk, phi, u, v, w, dt, prm.epslsm, prm.epsReinitialisation,
+                                 prm.RNtimestep, prm.RNitermax, ntime=n + 1)
+    zfs, _ = O.free_surface(blk, g.z, phi)
+    return zfs - depth
+
+
+def test_oracle_free_surface_derivative_matches_fd():
+    """d z_fs / d phi in the interpolating case, the formula the adjoint
+    kernel uses, against central differences of the oracle."""
+    g = inputs.channel_grid(9, 4, 13)
+    phi = inputs.solitary_phi(g, H=0.3)
+    blk = O.Block(g)
+    zfs, _ = O.free_surface(blk, g.z, phi)
+    rng = np.random.default_rng(3)
+    for _ in range(6):
+        i, j = int(rng.integers(0, 9)), int(rng.integers(0, 4))
+        col = phi[:, j, i]
+        kf = int(np.nonzero(col[:-1] * col[1:] <= 0.0)[0][0])
+        p0, p1 = col[kf], col[kf + 1]
+        dz = g.z[kf + 1, j, i] - g.z[kf, j, i]
+        for kk, an in ((kf, -(dz * p1) / (p1 - p0) ** 2), (kf + 1, (dz * p0) / (p1 - p0) ** 2),
+                       (kf + 2 if kf + 2 < 13 else kf - 1, 0.0)):
+            def f(h, kk=kk, i=i, j=j):
+                q = phi.copy()
+                q[kk, j, i] += h
+                return O.free_surface(blk, g.z, q)[0][j, i]
+            fd = _fd(f, 1e-7)
+            assert abs(fd - an) <= 1e-6 * max(abs(an), 1e-3), (kk, fd, an)
+
+
+def _dev(a):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+
+def _host(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+@pytest.mark.gpu
+def test_gpu_free_surface_and_adjoint_match_oracle():
+    import torch
+    from lsm_amd import api
+    print("[inverse] free surface vs oracle", flush=True)
+    g = inputs.channel_grid(70, 9, 33)
+    phi = inputs.solitary_phi(g, H=0.4)
+    phi[:, 0, 0] = 1.0                                  # a column without a free surface
+    blk, oblk = api.Block(g), O.Block(g)
+    zfs, m = api.free_surface(blk, _dev(g.z), _dev(phi))
+    ref, missing = O.free_surface(oblk, g.z, phi)
+    assert np.array_equal(_host(zfs), ref)
+    assert int(_host(m)[0]) == missing == 1
+    lam = np.random.default_rng(4).standard_normal(ref.shape)
+    adj = _host(api.free_surface_adjoint(blk, _dev(g.z), _dev(phi), _dev(lam)))
+    # the adjoint is the transpose of the derivative: <adj, v> = lam . (d zfs / d phi) v
+    v = np.random.default_rng(5).standard_normal(phi.shape)
+    fd = _fd(lambda h: float(np.sum(lam * O.free_surface(oblk, g.z, phi + h * v)[0])), 1e-7)
+    an = float(np.sum(adj * v))
+    assert abs(fd - an) <= 1e-6 * abs(an)
+    assert np.all(adj[:, 0, 0] == 0.0)
+    torch.cuda.synchronize()
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    return channels.channel_batch(SPECS)
+
+
+@pytest.mark.gpu
+def test_gpu_inverse_gradients_match_fd_of_oracle(small_batch):
+    import torch
+    cb = small_batch
+    Ht = channels.perturbed_amplitudes(cb.specs, seed=9, rel=0.1)
+    print(f"[inverse] {len(cb.specs)} channels, {NSTEPS} steps, dt {cb.dt:.3e}", flush=True)
+    eta_t = channels.targets(cb, Ht, NSTEPS, EVERY, mode="exact")
+    J, g_phi, g_H = channels.loss_and_gradients(cb, eta_t, NSTEPS, EVERY, mode="exact")
+    eta_t_np = [_host(e) for e in eta_t]
+
+    def oracle_Jb(b, H, dphi):
+        eta = _oracle_eta(cb.grids[b], H, dphi, cb.dt, cb.params, NSTEPS)
+        return float(np.mean((eta - eta_t_np[b]) ** 2))
+
+    zero = [np.zeros(g.shape) for g in cb.grids]
+    Jo = sum(oracle_Jb(b, s.H, zero[b]) for b, s in enumerate(cb.specs))
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    rng = np.random.default_rng(11)
+    for b, s in enumerate(cb.specs):
+        fdH = _fd_smooth(lambda h: oracle_Jb(b, s.H + h, zero[b]))
+        assert abs(g_H[b] - fdH) <= 1e-6 * abs(fdH), (b, g_H[b], fdH)
+        v = rng.standard_normal(cb.grids[b].shape)
+        fdv = _fd_smooth(lambda h: oracle_Jb(b, s.H, h * v))
+        an = float(np.sum(_host(g_phi[b]) * v))
+        assert abs(an - fdv) <= 1e-6 * abs(fdv), (b, an, fdv)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+def test_gpu_batched_forward_equals_instances(small_batch):
+    """The instances advanced together as an independent batch (one launch per
+    stage over all blocks) give each instance's single-block steps bit for
+    bit."""
+    import torch
+    from lsm_amd import api, autograd, batch as B
+    cb = small_batch
+    print("[inverse] batched forward vs per-instance steps", flush=True)
+    lay = B.independent_layout([(s.nx, s.ny, s.nz) for s in cb.specs])
+    bt = B.Batch(lay, cb.grids)
+    phis = [inputs.solitary_phi(g, H=s.H) for g, s in zip(cb.grids, cb.specs)]
+    vels = [inputs.solitary_velocity(g, H=s.H) for g, s in zip(cb.grids, cb.specs)]
+    ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vels]) for c in range(3)])
+    cur, nxt = bt.pack(phis), bt.empty()
+    for _ in range(NSTEPS):
+        bt.levelset_step(cur, ucn, cb.dt, cb.params, out=nxt, mode="exact")
+        cur, nxt = nxt, cur
+    got = bt.unpack(cur)
+    for b, blk in enumerate(cb.blocks):
+        U = api.rhs_contra_j(blk, *[_dev(u) for u in vels[b]])
+        p = _dev(phis[b])
+        with torch.no_grad():
+            for _ in range(NSTEPS):
+                p = autograd.levelset_step(blk, p, U, cb.dt, cb.params, mode="exact")
+        assert np.array_equal(B.owned(lay, b, got[b]), _host(p)), b
