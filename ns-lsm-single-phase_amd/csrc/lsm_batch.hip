@@ -49,7 +49,7 @@ struct BBlock {
     long shell0;              // first shell node of the block in the batch enumeration
     long nI, nJ, nK;
     int cnt_i, cnt_j;
-    int rt_tiles_i, rt_tiles_j, rt_ntiles, rt_tile0, rt_kc;   // reinitialisation tiles (64 x 4 x rt_kc)
+    int rt_tiles_i, rt_tiles_j, rt_ntiles, rt_tile0, rt_kc;   // reinitialisation tiles (W x 256/W x rt_kc)
     int sh_wg0;               // first shell workgroup of the block (reinitialisation)
 };
 
@@ -253,6 +253,7 @@ struct BRArgs {
     uint8_t *conv;
     RCtl *ctl;                    // one per block
     RStage p;
+    int reduce;                   // max / min of |grad d| for the exit test (0: the test cannot pass)
 };
 
 // |grad d| at an owned node (calc_NormGrad_d.F90:39-2446 on one rank): one-sided
@@ -335,11 +336,15 @@ __device__ __forceinline__ void rn_reduce(double vmax, double vmin, int tid, int
     }
 }
 
-// STAGE 0: max / min of |grad din| only; 1..3: the RK stage
-template <int STAGE>
+// STAGE 0: max / min of |grad din| only; 1..3: the RK stage.  Interior tiles
+// of W (i) x 256/W (j) columns; W = 64 puts one row in a wavefront, W = 32
+// and 16 put 2 and 4 rows in one, which wastes fewer lanes on blocks whose i
+// extent is not a multiple of 64 (and loads fewer halo values per node).
+template <int STAGE, int W>
 __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
 {
-    const int tid = threadIdx.y * 64 + threadIdx.x;
+    constexpr int TY = 256 / W;
+    const int tid = threadIdx.y * W + threadIdx.x;
     const double inf = __longlong_as_double((long long)kInf64);
     double vmax = 0.0, vmin = inf;
     const int bid = blockIdx.x;
@@ -373,10 +378,10 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
             }
             vmax = ngv; vmin = ngv;
         }
-        if (STAGE == 3 || STAGE == 0) rn_reduce(vmax, vmin, tid, 256, ctl);
+        if ((STAGE == 3 || STAGE == 0) && a.reduce) rn_reduce(vmax, vmin, tid, 256, ctl);
         return;
     }
-    // ---- interior tiles: 64 (i) x 4 (j) columns marching up kc planes ----
+    // ---- interior tiles: W (i) x TY (j) columns marching up kc planes ----
     int tile = xcd_tile(bid, a.grid_tiles);
     const bool live = tile < a.ntiles;
     tile += a.tile_base;
@@ -389,24 +394,27 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
     const int tj = t2 % B.rt_tiles_j, tk = t2 / B.rt_tiles_j;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const Range &r = B.r;
-    const int i = ti * 64 + tx, j = r.jlo + tj * 4 + ty;
+    const int i0 = r.ilo + ti * W;
+    const int i = i0 + tx, j = r.jlo + tj * TY + ty;
     const int kb = r.klo + tk * B.rt_kc;
     const int ke = live ? min(kb + B.rt_kc - 1, r.khi) : kb - 1;
-    __shared__ double sm[2][4 + 4][64 + 4];
+    __shared__ double sm[2][TY + 4][W + 4];
     const int ni = B.ni, nj = B.nj, nk = B.nk;
     const long sj = B.sj, sk = B.sk;
     const int ic = min(i, ni - 1), jc = min(j, nj - 1);
     const long col = (long)ic + sj * jc;
+    // j halo: rows -2, -1, TY, TY+1 of the tile, loaded by the first 4 rows of threads
     const bool has_jh = ty < 4;
-    const int rr = (ty < 2) ? ty : ty - 2 + 4 + 2;
-    const int jh = min(max(r.jlo + tj * 4 - 2 + rr, 0), nj - 1);
+    const int rr = (ty < 2) ? ty : ty - 2 + TY + 2;
+    const int jh = min(max(r.jlo + tj * TY - 2 + rr, 0), nj - 1);
     const long col_jh = (long)ic + sj * jh;
+    // i halo: columns -2, -1, W, W+1 of every tile row, loaded by threads 0 .. 4 TY - 1
     const int lin = tid;
-    const bool has_ih = lin < 16;
-    const int hr = (lin >> 2) % 4, hc4 = lin & 3;
-    const int hcol = (hc4 < 2) ? hc4 : hc4 + 64;
-    const int hi_i = min(max(ti * 64 - 2 + hcol, 0), ni - 1);
-    const int hi_j = min(r.jlo + tj * 4 + hr, nj - 1);
+    const bool has_ih = lin < 4 * TY;
+    const int hr = min(lin >> 2, TY - 1), hc4 = lin & 3;
+    const int hcol = (hc4 < 2) ? hc4 : hc4 + W;
+    const int hi_i = min(max(i0 - 2 + hcol, 0), ni - 1);
+    const int hi_j = min(r.jlo + tj * TY + hr, nj - 1);
     const long col_ih = (long)hi_i + sj * hi_j;
     auto kcl = [&](int kk) -> long { return sk * (long)min(max(kk, 0), nk - 1); };
     const bool act = live && i >= r.ilo && i <= r.ihi && j >= r.jlo && j <= r.jhi;
@@ -441,7 +449,7 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
             double dnv = q2;
             uint8_t cv = 0;
             if (STAGE > 1) { dnv = __builtin_nontemporal_load(a.dn + B.off + l); cv = a.conv[B.off + l]; }
-            const double (*pl)[64 + 4] = sm[buf];
+            const double (*pl)[W + 4] = sm[buf];
             const int cx = tx + 2, cy = ty + 2;
             const double g0 = dd0 * eno2(p0, ilo ? 0.0 : pl[cy][cx - 2], pl[cy][cx - 1], q2, pl[cy][cx + 1],
                                          ihi ? 0.0 : pl[cy][cx + 2], ilo, ihi);
@@ -464,7 +472,7 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
         q0 = q1; q1 = q2; q2 = q3; q3 = q4;
         buf ^= 1;
     }
-    if (STAGE == 3 || STAGE == 0) rn_reduce(vmax, vmin, tid, 256, ctl);
+    if ((STAGE == 3 || STAGE == 0) && a.reduce) rn_reduce(vmax, vmin, tid, 256, ctl);
 }
 
 // per-block exit-test state: a block with ghost layers counts |grad d| = 0
@@ -575,6 +583,8 @@ struct LsmBatch {
     const double *met[9] = {nullptr};  // batch metric arrays (csi, eta, zet), may be NULL
     int32_t *d_rtile_block = nullptr;  // reinitialisation tiles
     int rt_ntiles = 0;
+    int rt_w = 64;                     // reinitialisation tile columns (64, 32 or 16; rows 256 / rt_w)
+    int rt_w_auto = 1;                 // rt_w chosen from the block extents
     int32_t *d_sh_block = nullptr;     // shell workgroups (reinitialisation)
     int sh_wgs = 0;
     lsm::HSeg *d_bseg[3] = {nullptr};  // bcond_lsm face segments per axis
@@ -767,11 +777,33 @@ int upload(const std::vector<T> &v, T **dst)
     return check_hip(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
 }
 
-// reinitialisation tiles (64 x 4 x 8 over each block's interior range), shell
-// workgroups (256 physical-face nodes of one block each) and the bcond_lsm
-// face segments of each axis
+// tile columns that leave the fewest idle lanes over the blocks' interior
+// ranges (a narrower tile only when it saves more than 3 % of the lanes)
+int pick_reinit_width(const LsmBatch *B)
+{
+    int best = 64;
+    double best_lanes = 0.0;
+    for (int w : {64, 32, 16}) {
+        const int ty = 256 / w;
+        double lanes = 0.0;
+        for (const BBlock &X : B->hb) {
+            const Range &r = X.r;
+            const int ni = r.ihi - r.ilo + 1, nj = r.jhi - r.jlo + 1, nk = r.khi - r.klo + 1;
+            if (ni <= 0 || nj <= 0 || nk <= 0) continue;
+            lanes += (double)((ni + w - 1) / w * w) * (double)((nj + ty - 1) / ty * ty) * nk;
+        }
+        if (w == 64 || lanes < 0.97 * best_lanes) { best = w; best_lanes = lanes; }
+    }
+    return best;
+}
+
+// reinitialisation tiles (rt_w x 256/rt_w x 8 over each block's interior
+// range), shell workgroups (256 physical-face nodes of one block each) and
+// the bcond_lsm face segments of each axis
 int build_reinit_tables(LsmBatch *B)
 {
+    if (B->rt_w_auto) B->rt_w = pick_reinit_width(B);
+    const int W = B->rt_w, TY = 256 / W;
     std::vector<int32_t> rt, sh;
     std::vector<HSeg> bs[3];
     std::vector<int32_t> bw[3];
@@ -781,9 +813,9 @@ int build_reinit_tables(LsmBatch *B)
         const Range &r = X.r;
         const int kc = 8;
         X.rt_kc = kc;
-        X.rt_tiles_i = (r.ihi + 1 + 63) / 64;
-        const int nj_int = r.jhi - r.jlo + 1, nk_int = r.khi - r.klo + 1;
-        X.rt_tiles_j = nj_int > 0 ? (nj_int + 3) / 4 : 0;
+        const int ni_int = r.ihi - r.ilo + 1, nj_int = r.jhi - r.jlo + 1, nk_int = r.khi - r.klo + 1;
+        X.rt_tiles_i = ni_int > 0 ? (ni_int + W - 1) / W : 0;
+        X.rt_tiles_j = nj_int > 0 ? (nj_int + TY - 1) / TY : 0;
         const int tk = nk_int > 0 ? (nk_int + kc - 1) / kc : 0;
         X.rt_ntiles = (r.ihi >= r.ilo) ? X.rt_tiles_i * X.rt_tiles_j * tk : 0;
         X.rt_tile0 = (int)rt.size();
@@ -854,7 +886,9 @@ int reinit_stage_launch(LsmBatch *B, BRArgs a, int b0, int nb, hipStream_t s)
     a.grid_tiles = (a.ntiles + 7) / 8 * 8;
     const unsigned grid = (unsigned)(a.grid_tiles + (w1 - w0));
     if (grid == 0) return LSM_OK;
-    batch_reinit_stage<STAGE><<<grid, dim3(64, 4, 1), 0, s>>>(a);
+    if (B->rt_w == 16) batch_reinit_stage<STAGE, 16><<<grid, dim3(16, 16, 1), 0, s>>>(a);
+    else if (B->rt_w == 32) batch_reinit_stage<STAGE, 32><<<grid, dim3(32, 8, 1), 0, s>>>(a);
+    else batch_reinit_stage<STAGE, 64><<<grid, dim3(64, 4, 1), 0, s>>>(a);
     return check_hip(hipGetLastError(), "batch_reinit_stage");
 }
 
@@ -941,9 +975,14 @@ int batch_reinit(LsmBatch *B, const LsmReinitParams &p, const double *phi0, doub
     a.p = RStage{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
     const bool per = B->tune.form == LSM_BATCH_PERBLOCK;
     const int nl = per ? B->nb : 1, span = per ? 1 : B->nb;
+    // with ghost layers the minimum starts at 0, so the exit test can only
+    // pass when eps_reinit > 1: otherwise every block runs rn_itermax
+    // iterations and the max / min reductions are skipped
+    a.reduce = (ghosted && !(p.eps_reinit > 1.0)) ? 0 : 1;
     a.din = phi0; a.dn = phi0; a.dout = nullptr;
-    for (int q = 0; q < nl; ++q)
-        if ((rc = reinit_stage_launch<0>(B, a, q * span, span, s))) return rc;
+    if (a.reduce)
+        for (int q = 0; q < nl; ++q)
+            if ((rc = reinit_stage_launch<0>(B, a, q * span, span, s))) return rc;
     const double *sin_[3] = {out, A, Bf};
     double *sout[3] = {A, Bf, out};
     for (int it = 0; it < p.rn_itermax; ++it) {
@@ -1153,6 +1192,24 @@ int lsm_batch_set_config(LsmBatch *B, int form, int tile_cols, int tile_rows, in
     B->tune.form = form;
     B->tune.wg_per_cu = wg_per_cu;
     return LSM_OK;
+}
+
+int lsm_batch_set_reinit_tile(LsmBatch *B, int tile_cols)
+{
+    if (!B) return fail(LSM_ERR_ARG, "NULL batch");
+    if (tile_cols != 0 && tile_cols != 64 && tile_cols != 32 && tile_cols != 16)
+        return fail(LSM_ERR_ARG, "tile_cols %d (0, 64, 32 or 16)", tile_cols);
+    int rc;
+    // the device tables may still be read by queued work
+    if ((rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))) return rc;
+    B->rt_w_auto = tile_cols == 0;
+    if (tile_cols) B->rt_w = tile_cols;
+    return build_reinit_tables(B);
+}
+
+int lsm_batch_get_reinit_tile(const LsmBatch *B)
+{
+    return B ? B->rt_w : 0;
 }
 
 int lsm_batch_get_config(const LsmBatch *B, int32_t cfg[5])

@@ -60,6 +60,8 @@ _SIGS = {
     "lsm_batch_destroy": (C.c_int, [_vp]),
     "lsm_batch_set_config": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     "lsm_batch_get_config": (C.c_int, [_vp, C.POINTER(C.c_int32)]),
+    "lsm_batch_set_reinit_tile": (C.c_int, [_vp, C.c_int]),
+    "lsm_batch_get_reinit_tile": (C.c_int, [_vp]),
     "lsm_batch_workspace_bytes": (C.c_size_t, [_vp, C.c_int]),
     "lsm_batch_halo_fill": (C.c_int, [_vp, _vp, _vp]),
     "lsm_batch_contra_velocity": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -317,6 +317,14 @@ class Batch:
                                                int(tile_cols), int(tile_rows), int(kchunk), int(wg_per_cu)),
               "lsm_batch_set_config")
 
+    def set_reinit_tile(self, cols=0):
+        """Tile columns of the reinitialisation stage kernel (64, 32, 16; 0 =
+        chosen from the block extents)."""
+        check(_lib.load().lsm_batch_set_reinit_tile(self.handle, int(cols)), "lsm_batch_set_reinit_tile")
+
+    def get_reinit_tile(self):
+        return int(_lib.load().lsm_batch_get_reinit_tile(self.handle))
+
     def get_config(self):
         cfg = (C.c_int32 * 5)()
         check(_lib.load().lsm_batch_get_config(self.handle, cfg), "lsm_batch_get_config")

@@ -123,7 +123,7 @@ def loss_and_gradients(cb: ChannelBatch, eta_t, nsteps: int = 60, every: int = 1
         eta = elevation(cb, b, instance_forward(cb, b, H, dphi, nsteps, every, mode))
         Jb = ((eta - eta_t[b]) ** 2).mean()
         Jb.backward()
-        J += float(Jb)
+        J += float(Jb.detach())
         g_phi.append(dphi.grad)
         g_H.append(float(H.grad))
     return J, g_phi, g_H
