@@ -193,6 +193,12 @@ def bench_batch(steps=10):
     res["perblock_full_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
     cfg, ms = bt.autotune(p0, ucn, dt, reps=3, force=True, out=out, work=w_adv)
     res["autotuned"] = cfg
+    res["full_ms_by_reinit_tile"] = {}
+    for w in (64, 32, 16):
+        bt.set_reinit_tile(w)
+        res["full_ms_by_reinit_tile"][w] = round(
+            _time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
+    bt.set_reinit_tile(0)
     res["batched_adv_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, out=out, work=w_adv), steps), 4)
     res["batched_full_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
     bt.graph_capture(out, p0, ucn, dt, work=w_adv)

@@ -341,9 +341,8 @@ int lsm_batch_destroy(LsmBatch *batch);
 int lsm_batch_set_config(LsmBatch *batch, int form, int tile_cols, int tile_rows, int kchunk, int wg_per_cu);
 int lsm_batch_get_config(const LsmBatch *batch, int32_t cfg[5]);
 /* Tile columns of the batched reinitialisation stage kernel: 64, 32 or 16
- * (tiles of 64 x 4, 32 x 8 or 16 x 16 nodes), 0 = chosen from the block
- * extents (fewest idle lanes; the default).  lsm_batch_get_reinit_tile
- * returns the width in use. */
+ * (tiles of 64 x 4, 32 x 8 or 16 x 16 nodes), 0 = the default (64).
+ * lsm_batch_get_reinit_tile returns the width in use. */
 int lsm_batch_set_reinit_tile(LsmBatch *batch, int tile_cols);
 int lsm_batch_get_reinit_tile(const LsmBatch *batch);
 size_t lsm_batch_workspace_bytes(const LsmBatch *batch, int op);

@@ -338,8 +338,10 @@ __device__ __forceinline__ void rn_reduce(double vmax, double vmin, int tid, int
 
 // STAGE 0: max / min of |grad din| only; 1..3: the RK stage.  Interior tiles
 // of W (i) x 256/W (j) columns; W = 64 puts one row in a wavefront, W = 32
-// and 16 put 2 and 4 rows in one, which wastes fewer lanes on blocks whose i
-// extent is not a multiple of 64 (and loads fewer halo values per node).
+// and 16 put 2 and 4 rows in one (fewer idle lanes on blocks whose i extent
+// is not a multiple of 64, but more partly used cache lines per load: on the
+// 64-block workload 64 columns are fastest, 30.3 / 30.7 / 34.1 ms per full
+// step for 64 / 32 / 16).
 template <int STAGE, int W>
 __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
 {
@@ -584,7 +586,6 @@ struct LsmBatch {
     int32_t *d_rtile_block = nullptr;  // reinitialisation tiles
     int rt_ntiles = 0;
     int rt_w = 64;                     // reinitialisation tile columns (64, 32 or 16; rows 256 / rt_w)
-    int rt_w_auto = 1;                 // rt_w chosen from the block extents
     int32_t *d_sh_block = nullptr;     // shell workgroups (reinitialisation)
     int sh_wgs = 0;
     lsm::HSeg *d_bseg[3] = {nullptr};  // bcond_lsm face segments per axis
@@ -777,32 +778,11 @@ int upload(const std::vector<T> &v, T **dst)
     return check_hip(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
 }
 
-// tile columns that leave the fewest idle lanes over the blocks' interior
-// ranges (a narrower tile only when it saves more than 3 % of the lanes)
-int pick_reinit_width(const LsmBatch *B)
-{
-    int best = 64;
-    double best_lanes = 0.0;
-    for (int w : {64, 32, 16}) {
-        const int ty = 256 / w;
-        double lanes = 0.0;
-        for (const BBlock &X : B->hb) {
-            const Range &r = X.r;
-            const int ni = r.ihi - r.ilo + 1, nj = r.jhi - r.jlo + 1, nk = r.khi - r.klo + 1;
-            if (ni <= 0 || nj <= 0 || nk <= 0) continue;
-            lanes += (double)((ni + w - 1) / w * w) * (double)((nj + ty - 1) / ty * ty) * nk;
-        }
-        if (w == 64 || lanes < 0.97 * best_lanes) { best = w; best_lanes = lanes; }
-    }
-    return best;
-}
-
 // reinitialisation tiles (rt_w x 256/rt_w x 8 over each block's interior
 // range), shell workgroups (256 physical-face nodes of one block each) and
 // the bcond_lsm face segments of each axis
 int build_reinit_tables(LsmBatch *B)
 {
-    if (B->rt_w_auto) B->rt_w = pick_reinit_width(B);
     const int W = B->rt_w, TY = 256 / W;
     std::vector<int32_t> rt, sh;
     std::vector<HSeg> bs[3];
@@ -1202,8 +1182,7 @@ int lsm_batch_set_reinit_tile(LsmBatch *B, int tile_cols)
     int rc;
     // the device tables may still be read by queued work
     if ((rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))) return rc;
-    B->rt_w_auto = tile_cols == 0;
-    if (tile_cols) B->rt_w = tile_cols;
+    B->rt_w = tile_cols ? tile_cols : 64;
     return build_reinit_tables(B);
 }
 

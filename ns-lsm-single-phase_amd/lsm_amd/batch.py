@@ -319,7 +319,7 @@ class Batch:
 
     def set_reinit_tile(self, cols=0):
         """Tile columns of the reinitialisation stage kernel (64, 32, 16; 0 =
-        chosen from the block extents)."""
+        the default, 64)."""
         check(_lib.load().lsm_batch_set_reinit_tile(self.handle, int(cols)), "lsm_batch_set_reinit_tile")
 
     def get_reinit_tile(self):

@@ -232,10 +232,11 @@ def _params(epslsm, dtau, nit=4, eps=1e-9, order=1, eno=False):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("form", ["grid", "perblock"])
-def test_independent_batch_reinit_matches_each_instance(form):
+@pytest.mark.parametrize("form,rtw", [("grid", 64), ("grid", 32), ("grid", 16), ("perblock", 0)])
+def test_independent_batch_reinit_matches_each_instance(form, rtw):
     """Each instance's reinitialisation (own exit test: the planar instance
-    passes it at once and runs no iteration) equals the single-block call."""
+    passes it at once and runs no iteration) equals the single-block call,
+    for every tile width of the batched stage kernel."""
     from lsm_amd import api
     shapes = [(12, 9, 10), (17, 6, 8), (9, 14, 7), (12, 10, 9)]
     lay = B.independent_layout(shapes)
@@ -252,6 +253,8 @@ def test_independent_batch_reinit_matches_each_instance(form):
     prm = _params(1.5 * hmax, 0.5 * hmin, nit=4, eps=0.1, order=2)
     bt = B.Batch(lay, grids)
     bt.set_config(form)
+    bt.set_reinit_tile(rtw)
+    print(f"[batch] independent reinit form={form} tile={bt.get_reinit_tile()}", flush=True)
     out, iters = bt.reinit(bt.pack(phis), prm)
     got = bt.unpack(out)
     its = iters.cpu().numpy()
@@ -264,8 +267,9 @@ def test_independent_batch_reinit_matches_each_instance(form):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("ghost,order,eno", [(3, 1, False), (2, 2, True), (3, 3, False)])
-def test_connected_batch_reinit_matches_blockwise_reference(ghost, order, eno):
+@pytest.mark.parametrize("ghost,order,eno,rtw", [(3, 1, False, 0), (2, 2, True, 32), (3, 3, False, 16),
+                                                 (3, 2, False, 64)])
+def test_connected_batch_reinit_matches_blockwise_reference(ghost, order, eno, rtw):
     """The reinitialisation of a connected batch is the reference's per-rank
     loop (ghost layers frozen, ENO2 through the ghosts at internal faces,
     one-sided formulas at physical faces), checked against the FP64 PyTorch
@@ -277,6 +281,8 @@ def test_connected_batch_reinit_matches_blockwise_reference(ghost, order, eno):
     hmax = max(inputs.spacing_bounds(gd)[1] for gd in grids)
     prm = _params(1.5 * hmax, 0.5 * hmin, nit=3, order=order, eno=eno)
     bt = B.Batch(lay, grids)
+    bt.set_reinit_tile(rtw)
+    print(f"[batch] connected reinit ghost={ghost} tile={bt.get_reinit_tile()}", flush=True)
     out, iters = bt.reinit(bt.halo_fill(bt.pack(phi)), prm)
     assert np.all(iters.cpu().numpy() == 3)
     got = bt.unpack(out)
