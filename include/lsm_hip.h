@@ -52,6 +52,8 @@ extern "C" {
 /* advection arithmetic */
 #define LSM_ADVECT_FAST   0   /* WENO3 weights with one division per face value */
 #define LSM_ADVECT_EXACT  1   /* the reference's operation order (four divisions) */
+#define LSM_ADVECT_UPWIND1 2  /* orderAD = 0: first-order upwind right-hand side (calc_RH_AD.F90);
+                                 lsm_rk3_step and lsm_levelset_step only */
 
 /* workspace queries */
 #define LSM_OP_RK3            1
@@ -125,6 +127,13 @@ int lsm_weno3_rhs(const LsmBlock *blk, const double *phi, const double *ucn1,
 /* bcond_lsm.F90:2-568 (bcond_lsm): boundary values on the physical faces, in
  * place, in the reference's face order (i1, im, j1, jm, k1, km). */
 int lsm_bcond(const LsmBlock *blk, double *phi, void *stream);
+
+/* calc_RH_AD.F90:1-273 (calc_RH_AD, orderAD = 0): the first-order upwind
+ * right-hand side (u/2 +- |u/2| split, one-sided differences times U on the
+ * physical faces) at every node with AdvectionNodes > 0 (adv_mask may be
+ * NULL: every node); other entries of rhs keep their values. */
+int lsm_upwind_rhs(const LsmBlock *blk, const double *phi, const double *ucn1, const double *ucn2,
+                   const double *ucn3, const int32_t *adv_mask, double *rhs, void *stream);
 
 /* ad_RKTVD3.F90:1-88 (ad_RKTVD3): one SSP-RK3 step of the WENO3 advection,
  * phi_np1 = RK3(phi_n), with bcond_lsm after each stage.  phi_np1 and phi_n
@@ -441,6 +450,8 @@ int lsm_host_release(void);
 int lsm_host_contra_velocity(const double *q, double *ucn_aos);
 /* calc_RH_AD_WENO3.F90:1 */
 int lsm_host_weno3_rhs(const double *phi, double *rhs, const int32_t *adv_mask, int mode);
+/* calc_RH_AD (orderAD = 0) on host arrays, as lsm_upwind_rhs */
+int lsm_host_upwind_rhs(const double *phi, double *rhs, const int32_t *adv_mask);
 /* ad_RKTVD3.F90:1 */
 int lsm_host_rk3(double *phi_np1, const double *phi_n, const int32_t *adv_mask, double delti_lsm,
                  int mode);

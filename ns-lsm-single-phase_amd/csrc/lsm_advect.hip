@@ -262,6 +262,7 @@ static int rk3_run(const Blk &b, const double *phi_n, const int32_t *mask, const
                    const double *U2, const double *U3, double dt, int mode, const double *const src[3],
                    double *const dst[3], hipStream_t s, bool folded)
 {
+    if (mode == LSM_ADVECT_UPWIND1) return launch_upwind_rk3(b, phi_n, mask, U1, U2, U3, dt, src, dst, s);
     const Range r = make_range(b);
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},

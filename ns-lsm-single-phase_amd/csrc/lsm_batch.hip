@@ -1234,6 +1234,8 @@ int lsm_batch_rk3_step(LsmBatch *B, double *phi_np1, const double *phi_n, const 
 {
     if (!B || !phi_np1 || !phi_n || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL argument");
     if (phi_np1 == phi_n) return fail(LSM_ERR_ARG, "phi_np1 and phi_n must not alias");
+    if (mode == LSM_ADVECT_UPWIND1)
+        return fail(LSM_ERR_UNSUPPORTED, "orderAD = 0 (LSM_ADVECT_UPWIND1): the batched form covers the WENO3 right-hand side only");
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
     const size_t need = lsm_batch_workspace_bytes(B, LSM_OP_RK3);
     void *scr = work;
@@ -1327,6 +1329,8 @@ static int check_step_args(LsmBatch *B, const LsmReinitParams *p, double *phi_np
 {
     if (!B || !phi_np1 || !phi_n || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL argument");
     if (phi_np1 == phi_n) return fail(LSM_ERR_ARG, "phi_np1 and phi_n must not alias");
+    if (mode == LSM_ADVECT_UPWIND1)
+        return fail(LSM_ERR_UNSUPPORTED, "orderAD = 0 (LSM_ADVECT_UPWIND1): the batched form covers the WENO3 right-hand side only");
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
     if (p) {
         if (p->order_bc < 1 || p->order_bc > 3) return fail(LSM_ERR_ARG, "order_bc = %d (1, 2 or 3)", p->order_bc);

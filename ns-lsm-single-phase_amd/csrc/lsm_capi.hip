@@ -297,8 +297,21 @@ int lsm_weno3_rhs(const LsmBlock *blk, const double *phi, const double *ucn1, co
     int rc = make_blk(blk, &b, false, false);
     if (rc) return rc;
     if (!phi || !ucn1 || !ucn2 || !ucn3 || !rhs) return fail(LSM_ERR_ARG, "NULL array");
+    if (mode == LSM_ADVECT_UPWIND1)
+        return fail(LSM_ERR_UNSUPPORTED, "orderAD = 0 (LSM_ADVECT_UPWIND1): this entry point covers the WENO3 right-hand side only");
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
     return launch_weno3_rhs(b, phi, ucn1, ucn2, ucn3, adv_mask, rhs, mode, (hipStream_t)stream);
+}
+
+int lsm_upwind_rhs(const LsmBlock *blk, const double *phi, const double *ucn1, const double *ucn2,
+                   const double *ucn3, const int32_t *adv_mask, double *rhs, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, false);
+    if (rc) return rc;
+    if (!phi || !ucn1 || !ucn2 || !ucn3 || !rhs) return fail(LSM_ERR_ARG, "NULL array");
+    if (phi == rhs) return fail(LSM_ERR_ARG, "phi and rhs must not alias");
+    return launch_upwind_rhs(b, phi, ucn1, ucn2, ucn3, adv_mask, rhs, (hipStream_t)stream);
 }
 
 int lsm_bcond(const LsmBlock *blk, double *phi, void *stream)
@@ -320,7 +333,8 @@ int lsm_rk3_step(const LsmBlock *blk, double *phi_np1, const double *phi_n, cons
     if (!b.aj) return fail(LSM_ERR_ARG, "aj is NULL");
     if (!phi_np1 || !phi_n || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL array");
     if (phi_np1 == phi_n) return fail(LSM_ERR_ARG, "phi_np1 and phi_n must not alias");
-    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
+    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT && mode != LSM_ADVECT_UPWIND1)
+        return fail(LSM_ERR_ARG, "bad mode %d", mode);
     Carver c;
     rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3), &c);
     if (rc) return rc;
@@ -386,6 +400,8 @@ int lsm_rk3_step_adjoint(const LsmBlock *blk, const double *phi_n, const int32_t
     if (adj_n == adj_np1) return fail(LSM_ERR_ARG, "adj_n and adj_np1 must not alias");
     if ((adj_ucn1 != nullptr) != (adj_ucn2 != nullptr) || (adj_ucn1 != nullptr) != (adj_ucn3 != nullptr))
         return fail(LSM_ERR_ARG, "adj_ucn1..3 must all be given or all be NULL");
+    if (mode == LSM_ADVECT_UPWIND1)
+        return fail(LSM_ERR_UNSUPPORTED, "orderAD = 0 (LSM_ADVECT_UPWIND1): this entry point covers the WENO3 right-hand side only");
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
     Carver c;
     rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3_ADJOINT), &c);
@@ -526,6 +542,8 @@ static int check_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, con
         return fail(LSM_ERR_UNSUPPORTED, "rn_itermax = %d: the adjoint keeps at most %d iterations",
                     p->rn_itermax, kMaxTraceIters);
     if (!phi_n || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL array");
+    if (mode == LSM_ADVECT_UPWIND1)
+        return fail(LSM_ERR_UNSUPPORTED, "orderAD = 0 (LSM_ADVECT_UPWIND1): this entry point covers the WENO3 right-hand side only");
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
     return LSM_OK;
 }
@@ -674,7 +692,8 @@ int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p, double *phi
     if (call_levelset && (!u || !v || !w)) return fail(LSM_ERR_ARG, "NULL velocity");
     if (vel_stride < 1) return fail(LSM_ERR_ARG, "vel_stride must be >= 1");
     if (numiter_lsm < 0) return fail(LSM_ERR_ARG, "numiter_lsm < 0");
-    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
+    if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT && mode != LSM_ADVECT_UPWIND1)
+        return fail(LSM_ERR_ARG, "bad mode %d", mode);
     const bool do_reinit = call_reinit && rn_freq > 0 && ntime % rn_freq == 0;
     if (do_reinit && (rc = check_reinit(p))) return rc;
     hipStream_t s = (hipStream_t)stream;

@@ -310,6 +310,23 @@ int lsm_host_weno3_rhs(const double *phi, double *rhs, const int32_t *mask, int 
     return down(rhs, g_h.f[1], n);
 }
 
+int lsm_host_upwind_rhs(const double *phi, double *rhs, const int32_t *mask)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    if (!g_h.have_ucn) return fail(LSM_ERR_ARG, "contravariant velocity not computed (rhs_contra_j)");
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi, n)) || (rc = up(g_h.f[1], rhs, n))) return rc;
+    if (mask && (rc = check_hip(hipMemcpy(g_h.mask, mask, n * sizeof(int32_t), hipMemcpyHostToDevice), "H2D")))
+        return rc;
+    if ((rc = lsm_upwind_rhs(&g_h.blk, g_h.f[0], g_h.ucn[0], g_h.ucn[1], g_h.ucn[2], mask ? g_h.mask : nullptr,
+                             g_h.f[1], nullptr)))
+        return rc;
+    if ((rc = sync())) return rc;
+    return down(rhs, g_h.f[1], n);
+}
+
 int lsm_host_rk3(double *phi_np1, const double *phi_n, const int32_t *mask, double delti_lsm, int mode)
 {
     std::lock_guard<std::mutex> lk(g_mu);

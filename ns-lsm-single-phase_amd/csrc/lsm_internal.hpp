@@ -74,6 +74,13 @@ int launch_fold(const Blk &b, const double *U1, const double *U2, const double *
                 double *V1, double *V2, double *V3, int *bad, hipStream_t s);
 int launch_copy(const double *src, double *dst, long n, hipStream_t s);
 
+// ---- launchers (lsm_upwind.hip): orderAD = 0 ----
+int launch_upwind_rhs(const Blk &b, const double *phi, const double *U1, const double *U2, const double *U3,
+                      const int32_t *mask, double *rhs, hipStream_t s);
+int launch_upwind_rk3(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
+                      const double *U2, const double *U3, double dt, const double *const src[3],
+                      double *const dst[3], hipStream_t s);
+
 // ---- launchers (lsm_consumers.hip) ----
 // nc = 3: calc_phi_gradient_ENO2, nc = 2: calc_h_gradient_ENO2 (o2 unused)
 int launch_gradient_eno2(const Blk &b, const double *f, int nc, int order_bc, int eno_bc, double *o0,

@@ -12,6 +12,7 @@ module lsm_hip_f
 
    integer(c_int), parameter :: LSM_ADVECT_FAST  = 0
    integer(c_int), parameter :: LSM_ADVECT_EXACT = 1
+   integer(c_int), parameter :: LSM_ADVECT_UPWIND1 = 2
 
    ! arithmetic of the advection calls: the reference's (bit-identical results)
    ! unless the caller opts into the one-division WENO3 weights
@@ -47,6 +48,13 @@ module lsm_hip_f
          integer(c_int32_t), intent(in) :: mask(*)
          integer(c_int), value          :: mode
       end function lsm_host_weno3_rhs
+
+      integer(c_int) function lsm_host_upwind_rhs(phi, rhs, mask) bind(C, name="lsm_host_upwind_rhs")
+         import :: c_int, c_int32_t, c_double
+         real(c_double), intent(in)     :: phi(*)
+         real(c_double), intent(inout)  :: rhs(*)
+         integer(c_int32_t), intent(in) :: mask(*)
+      end function lsm_host_upwind_rhs
 
       integer(c_int) function lsm_host_rk3(phi_np1, phi_n, mask, dt, mode) bind(C, name="lsm_host_rk3")
          import :: c_int, c_int32_t, c_double

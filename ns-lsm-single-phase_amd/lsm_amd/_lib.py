@@ -15,6 +15,7 @@ LSM_OK = 0
 LSM_FACE_ALL = 0x3F
 LSM_ADVECT_FAST = 0
 LSM_ADVECT_EXACT = 1
+LSM_ADVECT_UPWIND1 = 2
 LSM_OP_RK3, LSM_OP_REINIT_ENO2, LSM_OP_REINIT_GODUNOV, LSM_OP_WATER_VOLUME, LSM_OP_LEVELSET_STEP = 1, 2, 3, 4, 5
 LSM_OP_RK3_ADJOINT = 6
 LSM_OP_REINIT_ENO2_ADJOINT, LSM_OP_STEP_ADJOINT, LSM_OP_STEP_TRACE, LSM_OP_STEP_ADJOINT_RECORDED = 7, 8, 9, 10
@@ -81,6 +82,7 @@ _SIGS = {
     "lsm_workspace_bytes": (C.c_size_t, [C.POINTER(LsmBlock), C.c_int]),
     "lsm_contra_velocity": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, C.c_int64, _vp, _vp, _vp, _vp]),
     "lsm_weno3_rhs": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "lsm_upwind_rhs": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lsm_bcond": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp]),
     "lsm_rk3_step": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_int,
                                _vp, C.c_size_t, _vp]),
@@ -126,6 +128,7 @@ _SIGS = {
     "lsm_host_release": (C.c_int, []),
     "lsm_host_contra_velocity": (C.c_int, [_vp, _vp]),
     "lsm_host_weno3_rhs": (C.c_int, [_vp, _vp, _vp, C.c_int]),
+    "lsm_host_upwind_rhs": (C.c_int, [_vp, _vp, _vp]),
     "lsm_host_rk3": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_int]),
     "lsm_host_bcond": (C.c_int, [_vp]),
     "lsm_host_normgrad": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp]),

@@ -18,7 +18,8 @@ import torch
 from . import _lib
 from ._lib import LsmBlock, LsmReinitParams, LsmError, check
 
-MODES = {"fast": _lib.LSM_ADVECT_FAST, "exact": _lib.LSM_ADVECT_EXACT}
+# "upwind": orderAD = 0 (first-order upwind right-hand side, calc_RH_AD.F90); forward steps only
+MODES = {"fast": _lib.LSM_ADVECT_FAST, "exact": _lib.LSM_ADVECT_EXACT, "upwind": _lib.LSM_ADVECT_UPWIND1}
 
 
 def _stream():
@@ -161,6 +162,19 @@ def calc_RH_AD_WENO3(blk: Block, phi, ucn, adv_mask=None, rhs=None, mode="fast")
     return rhs
 
 
+def calc_RH_AD(blk: Block, phi, ucn, adv_mask=None, out=None):
+    """calc_RH_AD.F90:1-273 (orderAD = 0): first-order upwind right-hand side
+    at every node with AdvectionNodes > 0 (rhs is zeroed first, as
+    ad_RKTVD3.F90:55 does, unless `out` is given: its other entries are kept)."""
+    if out is None:
+        out = blk.zeros()
+    check(_lib.load().lsm_upwind_rhs(blk.ref, _f64(phi, blk.shape, "phi"),
+                                     *[_f64(t, blk.shape, "ucn") for t in ucn],
+                                     _i32(adv_mask, blk.shape, "adv_mask"), _f64(out, blk.shape, "out"),
+                                     _stream()), "lsm_upwind_rhs")
+    return out
+
+
 def bcond_lsm(blk: Block, phi):
     """bcond_lsm.F90:2-568, in place."""
     check(_lib.load().lsm_bcond(blk.ref, _f64(phi, blk.shape, "phi"), _stream()), "lsm_bcond")
@@ -169,7 +183,9 @@ def bcond_lsm(blk: Block, phi):
 
 def ad_RKTVD3(blk: Block, phi_n, ucn, delti_lsm: float, adv_mask=None, out=None, mode="fast",
               work=None):
-    """ad_RKTVD3.F90:1-88: phi^(n+1) = SSP-RK3 WENO3 step of phi^n."""
+    """ad_RKTVD3.F90:1-88: phi^(n+1) = SSP-RK3 step of phi^n with the WENO3
+    right-hand side (mode "fast" / "exact") or the first-order upwind one
+    (mode "upwind", orderAD = 0)."""
     L = _lib.load()
     if out is None:
         out = blk.empty()

@@ -203,8 +203,8 @@ class LsmParams:
         """Raises NotImplementedError for options outside the GPU path (the
         C ABI rejects them the same way)."""
         bad = []
-        if self.orderAD != 1:
-            bad.append(f"orderAD = {self.orderAD} (only WENO3, orderAD = 1)")
+        if self.orderAD not in (0, 1):
+            bad.append(f"orderAD = {self.orderAD} (0: first-order upwind, 1: WENO3)")
         if self.sussman_correction_method != 0:
             bad.append("sussman_correction_method = 1 (Simpson filter)")
         if self.hybrid_reinitialisation and self.call_reinitialisation:

@@ -251,10 +251,51 @@ void oracle_bcond(const OBlock *b, double *phi)
 }
 
 /* ------------------------------------------------------------------------- */
-/* R3: ad_RKTVD3, ad_RKTVD3.F90:35-86                                         */
+/* calc_RH_AD (orderAD = 0), calc_RH_AD.F90:27-270: first-order upwind        */
+/* differences with the velocity split u+ = u/2 + |u/2|, u- = u/2 - |u/2|;    */
+/* one-sided (forward at the low face, backward at the high face) times U on  */
+/* the faces of every direction.  Per direction the term is assigned (xi) or  */
+/* added in front (eta, zeta); nodes with AdvectionNodes <= 0 are left as     */
+/* they are.                                                                 */
 /* ------------------------------------------------------------------------- */
-void oracle_rk3(const OBlock *b, double *phi_np1, const double *phi_n, const int32_t *mask,
-                const double *U1, const double *U2, const double *U3, double dt)
+void oracle_upwind_rhs(const OBlock *b, const double *phi, const double *U1, const double *U2,
+                       const double *U3, const int32_t *mask, double *rh)
+{
+    const int n[3] = {b->ni, b->nj, b->nk};
+    const size_t st[3] = {1, (size_t)b->ni, (size_t)b->ni * b->nj};
+    const double *Uc[3] = {U1, U2, U3};
+    const double dd[3] = {b->dc, b->de, b->dz};
+    for (int k = 0; k < b->nk; ++k)
+    for (int j = 0; j < b->nj; ++j)
+    for (int i = 0; i < b->ni; ++i) {
+        const size_t l = IX(b, i, j, k);
+        if (!(mask[l] > 0)) continue;
+        const int idx[3] = {i, j, k};
+        double r = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const size_t s = st[d];
+            double t;
+            if (idx[d] == 0) {
+                t = (dd[d] * Uc[d][l]) * (phi[l + s] - phi[l]);
+            } else if (idx[d] == n[d] - 1) {
+                t = (dd[d] * Uc[d][l]) * (phi[l] - phi[l - s]);
+            } else {
+                const double u = 0.5 * Uc[d][l];
+                const double up = u + fabs(u), um = u - fabs(u);
+                t = dd[d] * (up * (phi[l] - phi[l - s]) + um * (phi[l + s] - phi[l]));
+            }
+            r = (d == 0) ? t : t + r;
+        }
+        rh[l] = r;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* R3: ad_RKTVD3, ad_RKTVD3.F90:35-86; order_ad selects calc_RH_AD_WENO3 (1)  */
+/* or calc_RH_AD (0), ad_RKTVD3.F90:57-58                                     */
+/* ------------------------------------------------------------------------- */
+void oracle_rk3_order(const OBlock *b, double *phi_np1, const double *phi_n, const int32_t *mask,
+                      const double *U1, const double *U2, const double *U3, double dt, int order_ad)
 {
     const size_t n = (size_t)b->ni * b->nj * b->nk;
     const double c[3][3] = {{1.0, 0.0, 1.0},
@@ -264,7 +305,8 @@ void oracle_rk3(const OBlock *b, double *phi_np1, const double *phi_n, const int
     memcpy(phi_np1, phi_n, n * sizeof(double));
     for (int s = 0; s < 3; ++s) {
         memset(rh, 0, n * sizeof(double));
-        oracle_weno3_rhs(b, phi_np1, U1, U2, U3, mask, rh);
+        if (order_ad == 0) oracle_upwind_rhs(b, phi_np1, U1, U2, U3, mask, rh);
+        else oracle_weno3_rhs(b, phi_np1, U1, U2, U3, mask, rh);
         for (size_t l = 0; l < n; ++l)
             if (mask[l] > 0)
                 phi_np1[l] = (c[s][0] * phi_n[l] + c[s][1] * phi_np1[l])
@@ -272,6 +314,12 @@ void oracle_rk3(const OBlock *b, double *phi_np1, const double *phi_n, const int
         oracle_bcond(b, phi_np1);
     }
     free(rh);
+}
+
+void oracle_rk3(const OBlock *b, double *phi_np1, const double *phi_n, const int32_t *mask,
+                const double *U1, const double *U2, const double *U3, double dt)
+{
+    oracle_rk3_order(b, phi_np1, phi_n, mask, U1, U2, U3, dt, 1);
 }
 
 /* ------------------------------------------------------------------------- */

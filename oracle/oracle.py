@@ -48,6 +48,8 @@ def lib():
         L.oracle_weno3_rhs.argtypes = [bp, _dp, _dp, _dp, _dp, _ip, _dp]
         L.oracle_bcond.argtypes = [bp, _dp]
         L.oracle_rk3.argtypes = [bp, _dp, _dp, _ip, _dp, _dp, _dp, C.c_double]
+        L.oracle_rk3_order.argtypes = [bp, _dp, _dp, _ip, _dp, _dp, _dp, C.c_double, C.c_int]
+        L.oracle_upwind_rhs.argtypes = [bp, _dp, _dp, _dp, _dp, _ip, _dp]
         L.oracle_normgrad.argtypes = [bp, _dp, _dp, _dp, C.c_int, C.c_int]
         L.oracle_reinit_eno2.argtypes = [bp, rp, _dp, _dp]
         L.oracle_reinit_eno2.restype = C.c_int
@@ -139,17 +141,28 @@ def bcond(blk, phi):
     return out
 
 
-def rk3(blk, phi_n, U, dt, mask=None, nsteps=1, numiter=1):
-    """nsteps physical steps of numiter ad_RKTVD3 sub-steps each."""
+def rk3(blk, phi_n, U, dt, mask=None, nsteps=1, numiter=1, order_ad=1):
+    """nsteps physical steps of numiter ad_RKTVD3 sub-steps each (order_ad 1:
+    WENO3, 0: first-order upwind)."""
     mask = np.ones(blk.shape, np.int32) if mask is None else _c(mask, np.int32)
     Uc = [_c(a) for a in U]
     cur = _c(phi_n).copy()
     out = np.empty(blk.shape)
     for _ in range(nsteps * numiter):
-        lib().oracle_rk3(blk.ref, _p(out), _p(cur), mask.ctypes.data_as(_ip),
-                         _p(Uc[0]), _p(Uc[1]), _p(Uc[2]), float(dt))
+        lib().oracle_rk3_order(blk.ref, _p(out), _p(cur), mask.ctypes.data_as(_ip),
+                               _p(Uc[0]), _p(Uc[1]), _p(Uc[2]), float(dt), int(order_ad))
         cur, out = out, cur
     return cur
+
+
+def upwind_rhs(blk, phi, U, mask=None, init=None):
+    """calc_RH_AD (orderAD = 0); entries with mask <= 0 keep `init` (zeros)."""
+    mask = np.ones(blk.shape, np.int32) if mask is None else _c(mask, np.int32)
+    phi = _c(phi)
+    Uc = [_c(a) for a in U]
+    rh = np.zeros(blk.shape) if init is None else _c(init).copy()
+    lib().oracle_upwind_rhs(blk.ref, _p(phi), _p(Uc[0]), _p(Uc[1]), _p(Uc[2]), mask.ctypes.data_as(_ip), _p(rh))
+    return rh
 
 
 def normgrad(blk, d, phi0, order_bc=1, eno_bc=False):

@@ -425,6 +425,13 @@ subroutine ref_driver( il, iu, jl, ju, kl, ku, igp, jgp, kgp, dc, de, dz, &
       call put_reals_header(1)
       call put_real(wb)
 
+   case (17)  ! calc_RH_AD (orderAD = 0 upwind right-hand side); rh zeroed first
+      call rhs_contra_j()
+      wa = 0.0_rdf
+      call calc_RH_AD( phi, wa, imask )
+      call put_reals_header(1)
+      call put_real(wa)
+
    case (11)  ! TotalWaterVolume with the AdvectionNodes mask
       call TotalWaterVolume( phi, v1, v2, v3, imask )
       write(22) int(2,4)

@@ -22,7 +22,7 @@ MODES = {
     "advect": 1, "weno_rhs": 2, "bcond": 3, "contra": 4, "normgrad": 5,
     "reinit_eno2": 6, "sign_ini": 7, "rh_rn": 8, "lambda_rn": 9,
     "reinit_godunov": 10, "volume": 11, "levelset": 12, "metrics": 13,
-    "phi_gradient": 14, "calc_h": 15, "h_gradient": 16,
+    "phi_gradient": 14, "calc_h": 15, "h_gradient": 16, "upwind_rhs": 17,
 }
 
 

@@ -222,8 +222,29 @@ def consumers_case():
          extra_inputs={"aux": caux, "h": h})
 
 
+def upwind_case():
+    """orderAD = 0: calc_RH_AD and ad_RKTVD3 with the first-order upwind
+    right-hand side, on the curvilinear cube and with mixed boundary types,
+    a partial advection mask and zero contravariant speeds."""
+    g = inputs.sinusoidal_cube_grid(11, amp=0.04, seed=2, nj=12, nk=13)
+    phi = inputs.spheres_sdf(g, nspheres=3, seed=3, rmin=0.2, rmax=0.3)
+    u, v, w = inputs.fourier_velocity(g, seed=4)
+    u[:, :, 5] = 0.0
+    rng = np.random.default_rng(15)
+    mask = (rng.uniform(size=g.shape) > 0.25).astype(np.int32)
+    dt = inputs.cfl_dt(g, u, v, w)
+    params = dict(delti_lsm=dt, epslsm=0.1, rn_timestep=1e-3, eps_reinit=1e-5, big_phi=0.3,
+                  btype=(0, 2, 1, 1, 2, 1), order_ad=0)
+    runs = [("rhs", "upwind_rhs", {}, {}),
+            ("rhs_mask", "upwind_rhs", {}, {"mask": mask}),
+            ("advect", "advect", {"nsteps": 3}, {}),
+            ("advect_mask", "advect", {"nsteps": 2}, {"mask": mask}),
+            ("advect_iter2", "advect", {"nsteps": 2, "numiter_lsm": 2, "btype": (1, 1, 1, 1, 1, 1)}, {})]
+    case("upwind", g, phi, phi, u, v, w, params, runs, extra_inputs={"mask": mask})
+
+
 CASES = {"curvilinear": curvilinear_case, "channel": channel_case, "boundary": boundary_case,
-         "volume": volume_case, "consumers": consumers_case}
+         "volume": volume_case, "consumers": consumers_case, "upwind": upwind_case}
 
 if __name__ == "__main__":
     if not refio.available():

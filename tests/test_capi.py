@@ -96,7 +96,7 @@ def test_shim_files_cover_the_hot_path():
     routines = ["rhs_contra_j", "ad_RKTVD3", "calc_RH_AD_WENO3", "bcond_lsm", "calc_NormGrad_d",
                 "reinitialisation_test2", "calc_sign_ini", "calc_RH_RN", "calc_lambda_RN",
                 "reinitialization_benchmark", "TotalWaterVolume", "calc_phi_gradient_ENO2", "calc_h",
-                "calc_h_gradient_ENO2"]
+                "calc_h_gradient_ENO2", "calc_RH_AD"]
     for r in routines:
         text = open(os.path.join(FORTRAN, r + ".F90")).read()
         assert re.search(rf"^\s*subroutine\s+{r}\b", text, flags=re.M | re.I), r

@@ -143,7 +143,9 @@ def test_unsupported_options_are_rejected(ctl):
     with pytest.raises(NotImplementedError, match="hybrid"):
         prm.check_supported()
     prm.hybrid_reinitialisation = False
-    prm.orderAD = 0
+    prm.orderAD = 0                    # the first-order upwind right-hand side is supported
+    prm.check_supported()
+    prm.orderAD = 2
     with pytest.raises(NotImplementedError, match="orderAD"):
         prm.check_supported()
 

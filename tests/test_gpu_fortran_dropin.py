@@ -35,6 +35,8 @@ def linf_rel(a, b):
 
 def replay(fx, key, mode, harness=refio.SHIM_HARNESS, **over):
     args = {k: fx.par(k) for k in PARAM_KEYS}
+    if "p_order_ad" in fx.d:
+        args["order_ad"] = fx.par("order_ad")
     for k in ("nsteps", "numiter_lsm", "rn_itermax", "order_reinit_bc", "eno_bc", "m_sphi0",
               "m_ti", "lambda_dt", "btype", "big_phi"):
         v = fx.kw(key, k)
@@ -63,7 +65,13 @@ EXACT = [
     ("boundary", "weno_rhs_mask", "weno_rhs", {"mask": "mask"}),
     ("boundary", "advect_mask", "advect", {"mask": "mask"}),
     ("boundary", "advect", "advect", {}),
-] + [("curvilinear", f"normgrad_o{o}_e{e}", "normgrad", {}) for o in (1, 2, 3) for e in (0, 1)]
+] + [("curvilinear", f"normgrad_o{o}_e{e}", "normgrad", {}) for o in (1, 2, 3) for e in (0, 1)] + [
+    ("upwind", "rhs", "upwind_rhs", {}),
+    ("upwind", "rhs_mask", "upwind_rhs", {"mask": "mask"}),
+    ("upwind", "advect", "advect", {}),
+    ("upwind", "advect_mask", "advect", {"mask": "mask"}),
+    ("upwind", "advect_iter2", "advect", {}),
+]
 
 
 @pytest.mark.parametrize("name,key,mode,over", EXACT, ids=[f"{c[0]}-{c[1]}" for c in EXACT])
