@@ -915,14 +915,391 @@ int oracle_free_surface(const OBlock *b, const double *z, const double *phi, dou
 }
 
 /* ------------------------------------------------------------------------- */
+/* Geometric reinitialisation (hybrid_reinitialisation = .true., no global    */
+/* mass correction): geometric_reinitialisation.F90:1-93 with                  */
+/* GetTetrahedraList.F90, GetIsosurfaceAndDistances.F90,                       */
+/* NeighboursPhiCorrection.F90 and TetrahedronMethods.F90.                     */
+/* ------------------------------------------------------------------------- */
+
+/* NORM2 as the Fortran runtime evaluates it for REAL(8): the largest |x| so
+   far times sqrt(1 + sum of the others' squared ratios), rescaled when a
+   larger element arrives (checked against amdflang on random vectors). */
+static double fnorm2(const double v[3])
+{
+    double mx = 0.0, sum = 0.0;
+    for (int q = 0; q < 3; ++q) {
+        const double a = fabs(v[q]);
+        if (mx == 0.0) {
+            mx = a;
+        } else if (a > mx) {
+            const double t = mx / a, tsq = t * t;
+            sum = sum * tsq;
+            sum = sum + tsq;
+            mx = a;
+        } else {
+            const double t = a / mx;
+            sum = sum + t * t;
+        }
+    }
+    return mx * sqrt(1.0 + sum);
+}
+
+static double fdot(const double a[3], const double b[3])
+{
+    double r = 0.0;
+    for (int q = 0; q < 3; ++q) r = r + a[q] * b[q];
+    return r;
+}
+
+static void fsub(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2];
+}
+
+/* DistancePointLine, TetrahedronMethods.F90:1356-1399 */
+static double dist_point_line(const double A[3], const double B[3], const double P[3])
+{
+    double AB[3], AP[3], BA[3], BP[3], c[3];
+    fsub(B, A, AB); fsub(P, A, AP); fsub(A, B, BA); fsub(P, B, BP);
+    int valid = 1;
+    if (fnorm2(AB) < EPS_SIMS) valid = 0;
+    if (fdot(AB, AP) < 0.0) valid = 0;
+    if (fdot(BA, BP) < 0.0) valid = 0;
+    if (valid) {
+        cross3(AP, BP, c);
+        return fnorm2(c) / fnorm2(AB);
+    }
+    const double da = fnorm2(AP), db = fnorm2(BP);
+    return db < da ? db : da;
+}
+
+/* CheckProjectionPointOnATriangle3, TetrahedronMethods.F90:1593-1684:
+   returns 1 and the distance to the projection when the projection falls
+   strictly inside the triangle (the barycentric weights as the reference
+   forms them) */
+static int projection_inside(const double T[3][3], const double P[3], double *dist)
+{
+    double d12[3], d13[3], d23[3];
+    fsub(T[0], T[1], d12); fsub(T[0], T[2], d13); fsub(T[1], T[2], d23);
+    if (fnorm2(d12) <= EPS_SIMS || fnorm2(d13) <= EPS_SIMS || fnorm2(d23) <= EPS_SIMS) return 0;
+    double u[3], v[3], w[3], N[3], c[3], pp[3], d[3];
+    fsub(T[1], T[0], u); fsub(T[2], T[0], v); fsub(P, T[0], w);
+    cross3(u, v, N);
+    cross3(u, w, c);
+    const double alpha = fdot(c, N) / fdot(N, N);
+    cross3(w, v, c);
+    const double beta = fdot(c, N) / fdot(N, N);
+    const double gamma = (1.0 - beta) - alpha;
+    for (int q = 0; q < 3; ++q) pp[q] = (alpha * T[0][q] + beta * T[1][q]) + gamma * T[2][q];
+    fsub(P, pp, d);
+    const double dpp = fnorm2(d);
+    if (alpha > 0.0 && alpha < 1.0 && beta > 0.0 && beta < 1.0 && gamma > 0.0 && gamma < 1.0) {
+        *dist = dpp;
+        return 1;
+    }
+    return 0;
+}
+
+/* DistancePointTriangle, TetrahedronMethods.F90:1008-1070 */
+static double dist_point_triangle(const double T[3][3], const double P[3])
+{
+    double d[3];
+    double m[7];
+    fsub(P, T[0], d); m[0] = fnorm2(d);
+    fsub(P, T[1], d); m[1] = fnorm2(d);
+    fsub(P, T[2], d); m[2] = fnorm2(d);
+    m[3] = dist_point_line(T[0], T[1], P);
+    m[4] = dist_point_line(T[1], T[2], P);
+    m[5] = dist_point_line(T[2], T[0], P);
+    double pd;
+    const int n = projection_inside(T, P, &pd) ? 7 : 6;
+    if (n == 7) m[6] = pd;
+    double r = m[0];
+    for (int q = 1; q < n; ++q) if (m[q] < r) r = m[q];
+    return r;
+}
+
+/* GetTriangleArea, TetrahedronMethods.F90:691-707 */
+static double triangle_area(const double T[3][3])
+{
+    double a[3], b[3], c[3];
+    fsub(T[1], T[0], a); fsub(T[2], T[0], b);
+    cross3(a, b, c);
+    return 0.5 * fnorm2(c);
+}
+
+/* triangles of each case, TetrahedronMethods.F90:81-98 (edge indices) */
+static const int TET_TRIANGLES[16][6] = {
+    {-1, -1, -1, -1, -1, -1}, {0, 3, 2, -1, -1, -1}, {0, 1, 4, -1, -1, -1}, {1, 4, 2, 2, 4, 3},
+    {1, 2, 5, -1, -1, -1}, {0, 3, 5, 0, 5, 1}, {0, 2, 5, 0, 5, 4}, {5, 4, 3, -1, -1, -1},
+    {3, 4, 5, -1, -1, -1}, {4, 5, 0, 5, 2, 0}, {1, 5, 0, 5, 3, 0}, {5, 2, 1, -1, -1, -1},
+    {3, 4, 2, 2, 4, 1}, {4, 1, 0, -1, -1, -1}, {2, 3, 0, -1, -1, -1}, {-1, -1, -1, -1, -1, -1}};
+
+/* isosurface of MarchingTetrahedron, TetrahedronMethods.F90:26-664: the
+   number of triangles (negative when the zero set is a vertex or an edge),
+   their vertices and their total area */
+static void marching_tet_isosurface(const double v[4][3], const double ph[4], double tri[2][3][3], int *ntri,
+                                    double *area)
+{
+    const double tiny = 2.2250738585072014e-308;   /* tiny(eps_sims) */
+    int sg[4], npos = 0, nneg = 0, nzero = 0;
+    *ntri = 0; *area = 0.0;
+    memset(tri, 0, sizeof(double) * 18);
+    for (int q = 0; q < 4; ++q) {
+        sg[q] = 0;
+        if (ph[q] > tiny) sg[q] = 1;
+        if (ph[q] < -tiny) sg[q] = -1;
+        npos += sg[q] == 1; nneg += sg[q] == -1; nzero += sg[q] == 0;
+    }
+    if (npos == 4 || nneg == 4) return;
+    int flag = 0;
+    if (nzero > 0 && (nneg == 0 || npos == 0)) {
+        if (nzero == 1) { *ntri = -1; return; }
+        if (nzero == 2) { *ntri = -2; return; }
+        if (nzero == 3) {
+            /* the face of the three zero vertices, in vertex order */
+            int r = 0;
+            for (int q = 0; q < 4; ++q)
+                if (sg[q] == 0) { memcpy(tri[0][r], v[q], sizeof(double) * 3); ++r; }
+            *ntri = 1;
+            *area = triangle_area(tri[0]);
+            return;
+        }
+        return;                                         /* four zero vertices */
+    }
+    for (int q = 0; q < 4; ++q) if (sg[q] == -1) flag |= 1 << q;
+    const int ef = TET_EDGE_FLAGS[flag];
+    if (ef == 0) return;
+    double E[6][3];
+    for (int e = 0; e < 6; ++e) { E[e][0] = E[e][1] = E[e][2] = -9.9; }
+    for (int e = 0; e < 6; ++e) {
+        if (!(ef & (1 << e))) continue;
+        const int s0 = TET_EDGE[e][0], s1 = TET_EDGE[e][1];
+        const double den = ph[s1] - ph[s0];
+        double sv = 0.0;
+        if (!(fabs(den) < EPS_SIMS * EPS_SIMS * EPS_SIMS)) sv = ph[s1] / den;
+        for (int c = 0; c < 3; ++c) E[e][c] = v[s1][c] + sv * (v[s0][c] - v[s1][c]);
+    }
+    for (int t = 0; t < 2; ++t) {
+        if (TET_TRIANGLES[flag][3 * t] < 0) break;
+        *ntri += 1;
+        for (int q = 0; q < 3; ++q) memcpy(tri[t][q], E[TET_TRIANGLES[flag][3 * t + q]], sizeof(double) * 3);
+        *area = *area + triangle_area(tri[t]);
+    }
+}
+
+/* CellPnodeTetrahedronIdentification, TetrahedronMethods.F90:2124-2361
+   (TriangulationCase 1): bits 1..6 = tetrahedra with a phase change, bits
+   0..7 of *pnodes = their cell vertices */
+static int cell_ktets(const double ph[8], int *pnodes)
+{
+    *pnodes = 0;
+    int all_pos = 1, all_neg = 1;
+    for (int q = 0; q < 8; ++q) { if (!(ph[q] > EPS_SIMS)) all_pos = 0; if (!(ph[q] < -EPS_SIMS)) all_neg = 0; }
+    if (all_pos || all_neg) return 0;
+    int pcell = 0;
+    for (int q = 0; q < 8; ++q) if (fabs(ph[q]) < EPS_SIMS) pcell = 1;
+    if (!pcell) {
+        const double sgn = fsign1(ph[0]);
+        for (int q = 1; q < 8; ++q) if (sgn * ph[q] < 0.0) { pcell = 1; break; }
+    }
+    if (!pcell) return 0;
+    int tets = 0;
+    for (int t = 0; t < 6; ++t) {
+        int pct = 0, second = 1;
+        double pa = ph[CELL_TETS[t][0]];
+        if (pa <= EPS_SIMS) second = 0;
+        if (fabs(pa) < EPS_SIMS) pct = 1;
+        const double sgn = fsign1(pa);
+        for (int q = 1; q < 4; ++q) {
+            pa = ph[CELL_TETS[t][q]];
+            if (sgn * pa < 0.0 || fabs(pa) < EPS_SIMS) {
+                if (pa <= EPS_SIMS) second = 0;
+                pct = 1;
+            }
+        }
+        if (pct && !second) {
+            tets |= 1 << (t + 1);
+            for (int q = 0; q < 4; ++q) *pnodes |= 1 << CELL_TETS[t][q];
+        }
+    }
+    return tets;
+}
+
+/* cell vertex q -> node offsets, GetTetrahedraList.F90:240-247 */
+static const int CELL_VTX[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                   {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+typedef struct {
+    int ntri;
+    double area;
+    double tri[2][3][3];
+} OKTet;
+
+/* every K-tetrahedron with a vertex at node m, in the order
+   GetTetrahedraList.F90:346-418 appends them to the node's list (cells in
+   k, j, i order, tetrahedra 1..6) */
+static int node_ktets(const OBlock *b, const int *cell_tets, const int *cell_slot, int im, int jm, int km,
+                      int out[24])
+{
+    int n = 0;
+    for (int dk = -1; dk <= 0; ++dk)
+    for (int dj = -1; dj <= 0; ++dj)
+    for (int di = -1; di <= 0; ++di) {
+        const int ci = im + di, cj = jm + dj, ck = km + dk;
+        if (ci < 0 || cj < 0 || ck < 0 || ci > b->ni - 2 || cj > b->nj - 2 || ck > b->nk - 2) continue;
+        const size_t c = (size_t)ci + (size_t)(b->ni - 1) * ((size_t)cj + (size_t)(b->nj - 1) * ck);
+        const int tm = cell_tets[c];
+        if (!tm) continue;
+        int vq = -1;
+        for (int q = 0; q < 8; ++q)
+            if (CELL_VTX[q][0] == -di && CELL_VTX[q][1] == -dj && CELL_VTX[q][2] == -dk) vq = q;
+        for (int t = 0; t < 6; ++t) {
+            if (!(tm & (1 << (t + 1)))) continue;
+            int has = 0;
+            for (int q = 0; q < 4; ++q) if (CELL_TETS[t][q] == vq) has = 1;
+            if (has) out[n++] = cell_slot[c] * 6 + t;
+        }
+    }
+    return n;
+}
+
+/* the running minimum of GetIsosurfaceAndDistances.F90:201-233 over the
+   isosurface of one K-tetrahedron */
+static double ktet_min(const OKTet *K, const double P[3], double D)
+{
+    if (K->ntri > 0 && K->area > 0.0) {
+        double Da = dist_point_triangle(K->tri[0], P);
+        double ex = (fsign1(D - Da) + 1.0) / 2.0;
+        D = D + ex * (Da - D);
+        if (K->ntri > 1) {
+            Da = dist_point_triangle(K->tri[1], P);
+            ex = (fsign1(D - Da) + 1.0) / 2.0;
+            D = D + ex * (Da - D);
+        }
+    }
+    return D;
+}
+
+/* geometric_reinitialisation (GlobalMassCorrection = .false.):
+   pc (phicorrected, in/out) receives the signed distance at the interface
+   nodes and min(|pc|, distance) with the sign of pc at the nodes within 4
+   nodes of them; ids (may be NULL) = 1 interface node, -1 such a neighbour,
+   0 otherwise.  Returns the number of interface nodes. */
+int oracle_geometric_reinit(const OBlock *b, const double *phi0, double *pc, int32_t *ids)
+{
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    const size_t n = (size_t)ni * nj * nk, nc = (size_t)(ni - 1) * (nj - 1) * (nk - 1);
+    int *cell_tets = (int *)calloc(nc, sizeof(int));
+    int *cell_slot = (int *)malloc(nc * sizeof(int));
+    int *id = (int *)calloc(n, sizeof(int));
+    int nslots = 0;
+    for (int k = 0; k < nk - 1; ++k)
+    for (int j = 0; j < nj - 1; ++j)
+    for (int i = 0; i < ni - 1; ++i) {
+        double ph[8];
+        for (int q = 0; q < 8; ++q) ph[q] = phi0[IX(b, i + CELL_VTX[q][0], j + CELL_VTX[q][1], k + CELL_VTX[q][2])];
+        int pn;
+        const size_t c = (size_t)i + (size_t)(ni - 1) * ((size_t)j + (size_t)(nj - 1) * k);
+        cell_tets[c] = cell_ktets(ph, &pn);
+        cell_slot[c] = cell_tets[c] ? nslots++ : -1;
+        for (int q = 0; q < 8; ++q)
+            if (pn & (1 << q)) id[IX(b, i + CELL_VTX[q][0], j + CELL_VTX[q][1], k + CELL_VTX[q][2])] = 1;
+    }
+    /* isosurfaces of the K-tetrahedra, GetIsosurfaceAndDistances.F90:78-130 */
+    OKTet *kt = (OKTet *)calloc((size_t)nslots * 6 + 1, sizeof(OKTet));
+    const double *X[3] = {b->x, b->y, b->z};
+    for (int k = 0; k < nk - 1; ++k)
+    for (int j = 0; j < nj - 1; ++j)
+    for (int i = 0; i < ni - 1; ++i) {
+        const size_t c = (size_t)i + (size_t)(ni - 1) * ((size_t)j + (size_t)(nj - 1) * k);
+        if (!cell_tets[c]) continue;
+        for (int t = 0; t < 6; ++t) {
+            if (!(cell_tets[c] & (1 << (t + 1)))) continue;
+            double v[4][3], ph[4];
+            for (int q = 0; q < 4; ++q) {
+                const int vq = CELL_TETS[t][q];
+                const size_t l = IX(b, i + CELL_VTX[vq][0], j + CELL_VTX[vq][1], k + CELL_VTX[vq][2]);
+                for (int a = 0; a < 3; ++a) v[q][a] = X[a][l];
+                ph[q] = phi0[l];
+            }
+            OKTet *K = &kt[cell_slot[c] * 6 + t];
+            marching_tet_isosurface(v, ph, K->tri, &K->ntri, &K->area);
+        }
+    }
+    /* signed distances of the interface nodes, GetIsosurfaceAndDistances.F90:134-243,
+       and their neighbours (-1) */
+    double *sd = (double *)malloc(n * sizeof(double));
+    int nint = 0;
+    for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+    for (int i = 0; i < ni; ++i) {
+        const size_t l = IX(b, i, j, k);
+        if (id[l] <= 0) continue;
+        ++nint;
+        const double P[3] = {b->x[l], b->y[l], b->z[l]};
+        double D = 10.0, SD = 99.9;
+        for (int ks = (k - 4 < 0 ? 0 : k - 4); ks <= (k + 4 > nk - 1 ? nk - 1 : k + 4); ++ks)
+        for (int js = (j - 4 < 0 ? 0 : j - 4); js <= (j + 4 > nj - 1 ? nj - 1 : j + 4); ++js)
+        for (int is = (i - 4 < 0 ? 0 : i - 4); is <= (i + 4 > ni - 1 ? ni - 1 : i + 4); ++is) {
+            const size_t ls = IX(b, is, js, ks);
+            if (id[ls] > 0) {
+                int lst[24];
+                const int nkt = node_ktets(b, cell_tets, cell_slot, is, js, ks, lst);
+                for (int q = 0; q < nkt; ++q) {
+                    D = ktet_min(&kt[lst[q]], P, D);
+                    const double ex2 = (fsign1(fabs(SD) - D) + 1.0) / 2.0;
+                    SD = fabs(SD) + ex2 * (D - fabs(SD));
+                    SD = fsign1(phi0[l]) * SD;
+                }
+            } else {
+                id[ls] = -1;
+            }
+        }
+        sd[l] = SD;
+    }
+    /* NeighboursPhiCorrection.F90:89-130 (C = 0): the interface nodes */
+    for (size_t l = 0; l < n; ++l)
+        if (id[l] > 0) pc[l] = sd[l] + 0.0;
+    /* NeighboursPhiCorrection.F90:133-249: the neighbours */
+    for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+    for (int i = 0; i < ni; ++i) {
+        const size_t l = IX(b, i, j, k);
+        if (id[l] >= 0) continue;
+        const int k0 = k - 4 < 0 ? 0 : k - 4, k1 = k + 4 > nk - 1 ? nk - 1 : k + 4;
+        const int j0 = j - 4 < 0 ? 0 : j - 4, j1 = j + 4 > nj - 1 ? nj - 1 : j + 4;
+        const int i0 = i - 4 < 0 ? 0 : i - 4, i1 = i + 4 > ni - 1 ? ni - 1 : i + 4;
+        const double P[3] = {b->x[l], b->y[l], b->z[l]};
+        for (int ks = k0; ks <= k1; ++ks)
+        for (int js = j0; js <= j1; ++js)
+        for (int is = i0; is <= i1; ++is) {
+            if (id[IX(b, is, js, ks)] <= 0) continue;
+            double D = 10.0;
+            int lst[24];
+            const int nkt = node_ktets(b, cell_tets, cell_slot, is, js, ks, lst);
+            for (int q = 0; q < nkt; ++q) {
+                D = ktet_min(&kt[lst[q]], P, D);
+                const double ex2 = (fsign1(fabs(pc[l]) - D) + 1.0) / 2.0;
+                const double aux = fabs(pc[l]) + ex2 * (D - fabs(pc[l]));
+                pc[l] = fsign1(pc[l]) * aux;
+            }
+        }
+    }
+    if (ids) for (size_t l = 0; l < n; ++l) ids[l] = id[l] > 0 ? 1 : id[l];
+    free(cell_tets); free(cell_slot); free(id); free(kt); free(sd);
+    return nint;
+}
+
+/* ------------------------------------------------------------------------- */
 /* R1: levelsetmethod, levelsetmethod.F90:135-326 (non-hybrid Sussman path,   */
 /* no global mass correction).  phi: in = phi^n, out = phi^(n+1).  Returns    */
 /* the pre-advection water volume (the value WriteTotalWaterVolume appends).  */
 /* ------------------------------------------------------------------------- */
-double oracle_levelset_step(const OBlock *b, const OReinit *p, double *phi,
-                            const double *u, const double *v, const double *w,
-                            int numiter_lsm, double delti_lsm, int call_levelset,
-                            int call_reinit, int rn_freq, int ntime)
+double oracle_levelset_step_h(const OBlock *b, const OReinit *p, double *phi,
+                              const double *u, const double *v, const double *w,
+                              int numiter_lsm, double delti_lsm, int call_levelset,
+                              int call_reinit, int rn_freq, int ntime, int hybrid)
 {
     const size_t n = (size_t)b->ni * b->nj * b->nk;
     double vol[3];
@@ -930,7 +1307,10 @@ double oracle_levelset_step(const OBlock *b, const OReinit *p, double *phi,
     double *pc = (double *)calloc(n, sizeof(double));
     oracle_water_volume(b, phi, vol, mask);
     const double total = (vol[0] + vol[1]) + vol[2];
-    for (size_t l = 0; l < n; ++l) mask[l] = 1;
+    /* the AdvectionNodes band restricts the advection only on the hybrid
+       path (levelsetmethod.F90:198-199) */
+    if (!call_reinit || !hybrid)
+        for (size_t l = 0; l < n; ++l) mask[l] = 1;
     if (call_levelset) {
         double *U1 = (double *)malloc(n * sizeof(double));
         double *U2 = (double *)malloc(n * sizeof(double));
@@ -944,10 +1324,28 @@ double oracle_levelset_step(const OBlock *b, const OReinit *p, double *phi,
         free(U1); free(U2); free(U3);
     }
     if (call_reinit && rn_freq > 0 && ntime % rn_freq == 0) {
-        oracle_reinit_eno2(b, p, phi, pc);
+        if (hybrid) {
+            /* levelsetmethod.F90:300-308 */
+            for (size_t l = 0; l < n; ++l) {
+                if (phi[l] < -EPS_SIMS) pc[l] = -9.9;
+                if (phi[l] > EPS_SIMS) pc[l] = 9.9;
+            }
+            oracle_geometric_reinit(b, phi, pc, NULL);
+        } else {
+            oracle_reinit_eno2(b, p, phi, pc);
+        }
         oracle_bcond(b, pc);
         memcpy(phi, pc, n * sizeof(double));
     }
     free(mask); free(pc);
     return total;
+}
+
+double oracle_levelset_step(const OBlock *b, const OReinit *p, double *phi,
+                            const double *u, const double *v, const double *w,
+                            int numiter_lsm, double delti_lsm, int call_levelset,
+                            int call_reinit, int rn_freq, int ntime)
+{
+    return oracle_levelset_step_h(b, p, phi, u, v, w, numiter_lsm, delti_lsm, call_levelset, call_reinit,
+                                  rn_freq, ntime, 0);
 }

@@ -61,6 +61,11 @@ def lib():
         L.oracle_levelset_step.argtypes = [bp, rp, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                                            C.c_int, C.c_int, C.c_int, C.c_int]
         L.oracle_levelset_step.restype = C.c_double
+        L.oracle_levelset_step_h.argtypes = [bp, rp, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
+                                             C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.oracle_levelset_step_h.restype = C.c_double
+        L.oracle_geometric_reinit.argtypes = [bp, _dp, _dp, _ip]
+        L.oracle_geometric_reinit.restype = C.c_int
         L.oracle_gradient_eno2.argtypes = [bp, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int]
         L.oracle_calc_h.argtypes = [bp, _dp, _dp, _dp]
         L.oracle_calc_h.restype = C.c_int
@@ -246,14 +251,25 @@ def water_volume(blk, phi):
     return vol, mask
 
 
+def geometric_reinit(blk, phi0, pc):
+    """geometric_reinitialisation (no global mass correction): returns
+    (phicorrected, ids) from phicorrected's initial values pc; ids = 1 at the
+    interface nodes, -1 at their neighbours, 0 elsewhere."""
+    phi0 = _c(phi0)
+    out = _c(pc).copy()
+    ids = np.zeros(blk.shape, np.int32)
+    lib().oracle_geometric_reinit(blk.ref, _p(phi0), _p(out), ids.ctypes.data_as(_ip))
+    return out, ids
+
+
 def levelset_step(blk, phi, u, v, w, delti_lsm, epslsm, eps_reinit, rn_timestep,
                   rn_itermax, order_bc=1, eno_bc=False, numiter_lsm=1,
-                  call_levelset=True, call_reinit=True, rn_freq=1, ntime=1):
+                  call_levelset=True, call_reinit=True, rn_freq=1, ntime=1, hybrid=False):
     """One call of levelsetmethod (non-hybrid); returns (phi_new, volume)."""
     out = _c(phi).copy()
     u, v, w = _c(u), _c(v), _c(w)
     r = _reinit(epslsm, eps_reinit, rn_timestep, rn_itermax, order_bc, eno_bc)
-    vol = lib().oracle_levelset_step(blk.ref, C.byref(r), _p(out), _p(u), _p(v), _p(w),
-                                     int(numiter_lsm), float(delti_lsm), int(bool(call_levelset)),
-                                     int(bool(call_reinit)), int(rn_freq), int(ntime))
+    vol = lib().oracle_levelset_step_h(blk.ref, C.byref(r), _p(out), _p(u), _p(v), _p(w),
+                                       int(numiter_lsm), float(delti_lsm), int(bool(call_levelset)),
+                                       int(bool(call_reinit)), int(rn_freq), int(ntime), int(bool(hybrid)))
     return out, vol

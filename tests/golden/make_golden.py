@@ -243,8 +243,53 @@ def upwind_case():
     case("upwind", g, phi, phi, u, v, w, params, runs, extra_inputs={"mask": mask})
 
 
+def hybrid_case():
+    """hybrid_reinitialisation = .true.: the geometric reinitialisation
+    (GetTetrahedraList, GetIsosurfaceAndDistances, NeighboursPhiCorrection)
+    inside levelsetmethod, alone (call_levelsetmethod = .false.) and after
+    advection restricted to the AdvectionNodes band, on the curvilinear cube,
+    the channel and level sets that vanish exactly on grid nodes."""
+    g = inputs.sinusoidal_cube_grid(14, amp=0.04, seed=2, nj=15, nk=16)
+    phi = inputs.spheres_sdf(g, nspheres=3, seed=3, rmin=0.2, rmax=0.3)
+    phi = phi * (0.6 + (g.x - 0.4) ** 2)            # not a distance: the reinitialisation has work
+    u, v, w = inputs.fourier_velocity(g, seed=4)
+    dt = inputs.cfl_dt(g, u, v, w)
+    params = dict(delti_lsm=dt, epslsm=0.1, rn_timestep=1e-3, eps_reinit=1e-5, big_phi=5.0,
+                  btype=(1, 1, 1, 1, 1, 1), hybrid=True, rn_freq=1)
+    runs = [("geometric", "levelset", {"call_levelset": False}, {}),
+            ("levelset", "levelset", {"nsteps": 2}, {})]
+    case("hybrid_cube", g, phi, phi, u, v, w, params, runs)
+
+    c = inputs.channel_grid(32, 6, 16)
+    cphi = inputs.solitary_phi(c, H=0.3)
+    cu, cv, cw = inputs.solitary_velocity(c, H=0.3)
+    cdt = inputs.cfl_dt(c, cu, cv, cw)
+    params = dict(delti_lsm=cdt, epslsm=0.1, rn_timestep=1e-3, eps_reinit=1e-5, big_phi=5.0,
+                  btype=(1, 1, 1, 1, 1, 1), hybrid=True, rn_freq=1)
+    runs = [("geometric", "levelset", {"call_levelset": False}, {}),
+            ("levelset", "levelset", {"nsteps": 2}, {})]
+    case("hybrid_channel", c, cphi, cphi, cu, cv, cw, params, runs)
+
+    gu = inputs.uniform_grid(10, 9, 8, h=0.125)
+    planes = {
+        "flat": gu.z - 0.375,                       # zero on a node layer
+        "tilt": 0.3 * gu.x + 0.2 * gu.y - gu.z + 0.21,
+        "diag": gu.x - gu.y,                        # zero on a diagonal node set
+        "sphere": 0.31 - np.sqrt((gu.x - 0.5) ** 2 + (gu.y - 0.45) ** 2 + (gu.z - 0.4) ** 2),
+    }
+    zeros = np.zeros(gu.shape)
+    params = dict(delti_lsm=1e-3, epslsm=0.2, rn_timestep=0.05, eps_reinit=1e-5, big_phi=5.0,
+                  btype=(1, 1, 1, 1, 1, 1), hybrid=True, rn_freq=1, call_levelset=False)
+    runs, extra = [], {}
+    for name, p in planes.items():
+        extra[f"plane_{name}"] = p
+        runs.append((f"geometric_{name}", "levelset", {}, {"phi": p}))
+    case("hybrid_planes", gu, planes["tilt"], planes["tilt"], zeros, zeros, zeros, params, runs,
+         extra_inputs=extra)
+
+
 CASES = {"curvilinear": curvilinear_case, "channel": channel_case, "boundary": boundary_case,
-         "volume": volume_case, "consumers": consumers_case, "upwind": upwind_case}
+         "volume": volume_case, "consumers": consumers_case, "upwind": upwind_case, "hybrid": hybrid_case}
 
 if __name__ == "__main__":
     if not refio.available():
