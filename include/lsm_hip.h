@@ -66,6 +66,7 @@ extern "C" {
 #define LSM_OP_STEP_ADJOINT   8        /* sized by lsm_adjoint_workspace_bytes */
 #define LSM_OP_STEP_TRACE     9        /* sized by lsm_adjoint_workspace_bytes */
 #define LSM_OP_STEP_ADJOINT_RECORDED 10 /* sized by lsm_adjoint_workspace_bytes */
+#define LSM_OP_REINIT_GEOMETRIC 11
 
 /* One curvilinear block (the host-associated state of levelsetmethod.F90:1-44). */
 typedef struct LsmBlock {
@@ -92,6 +93,8 @@ typedef struct LsmReinitParams {
     int32_t eno_bc;                   /* ENOBCReinitialisation (0/1) */
     int32_t m_sphi0;                  /* Godunov path: 1 piecewise-sine sign, else phi/sqrt(phi^2+(|grad|eps)^2) */
     int32_t m_ti;                     /* Godunov path: 1 SSP-RK3, else forward Euler */
+    int32_t hybrid;                   /* hybrid_reinitialisation: geometric instead of Sussman
+                                         in lsm_levelset_step (GlobalMassCorrection = .false.) */
 } LsmReinitParams;
 
 /* Library version string. */
@@ -187,6 +190,21 @@ int lsm_bcond_adjoint(const LsmBlock *blk, const double *phi, double *adj, void 
  * vertices; phizero selects the ENO2 upwind side. */
 int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizero,
                       int order_bc, int eno_bc, double *normgrad, void *stream);
+
+/* geometric_reinitialisation.F90:1-93 (hybrid_reinitialisation = .true.,
+ * GlobalMassCorrection = .false.) with GetTetrahedraList.F90,
+ * GetIsosurfaceAndDistances.F90 and NeighboursPhiCorrection.F90:
+ * phicorrected (in/out) receives, at every vertex of a tetrahedron whose phi0
+ * changes sign (an interface node), its signed distance to the marching-
+ * tetrahedra isosurface of phi0 within 4 nodes, and at every other node within
+ * 4 nodes of an interface node min(|phicorrected|, that distance) with the
+ * sign of phicorrected; other nodes keep their values.  ids (device int32,
+ * may be NULL) = 1 interface node, -1 such a neighbour, 0 other.  Needs the
+ * coordinates x, y, z of blk.  Synchronises the stream once (the number of
+ * cut cells sizes the isosurface store, which the library keeps per device).
+ * Workspace: lsm_workspace_bytes(blk, LSM_OP_REINIT_GEOMETRIC). */
+int lsm_reinit_geometric(const LsmBlock *blk, const double *phi0, double *phicorrected, int32_t *ids,
+                         void *work, size_t work_bytes, void *stream);
 
 /* calc_phi_gradient_ENO2.F90:1-2025 (calc_phi_gradient_ENO2, called every
  * step by mg_driver.F90:106-116): grad phi = (g1, g2, g3) with the
@@ -463,6 +481,9 @@ int lsm_host_normgrad(const double *d, const double *phizero, int order_bc, int 
 /* reinitialisation_test2.F90:1 */
 int lsm_host_reinit_eno2(const double *phi0, double *phi_out, const LsmReinitParams *p,
                          int32_t *iters_done);
+/* geometric_reinitialisation.F90:1 (GlobalMassCorrection = .false.,
+ * TriangulationCase 1), as lsm_reinit_geometric; InterfaceNodesID may be NULL */
+int lsm_host_reinit_geometric(const double *phizero, double *phicorrected, int32_t *InterfaceNodesID);
 /* calc_sign_ini.F90:1 */
 int lsm_host_sign_ini(const double *phi0, double epslsm, int m_sphi0, double *sign_out,
                       double *normgrad_out);
@@ -501,7 +522,8 @@ int lsm_host_calc_h(const int32_t ext[9], int32_t phys_faces, const double *z, c
 #define LSM_KT_BATCH_STAGE    5   /* batched stage tiles of every block (one launch) */
 #define LSM_KT_BATCH_REINIT   6   /* batched reinitialisation stage of every block */
 #define LSM_KT_REINIT_ADJOINT 7   /* reverse sweep of one reinitialisation stage (tape gather) */
-#define LSM_KT_COUNT          8
+#define LSM_KT_GEOMETRIC      8   /* distances of the geometric reinitialisation */
+#define LSM_KT_COUNT          9
 
 /* Turns event recording on (1) or off (0); clears accumulated timings. */
 int lsm_timing_enable(int on);

@@ -1333,6 +1333,8 @@ static int check_step_args(LsmBatch *B, const LsmReinitParams *p, double *phi_np
         return fail(LSM_ERR_UNSUPPORTED, "orderAD = 0 (LSM_ADVECT_UPWIND1): the batched form covers the WENO3 right-hand side only");
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
     if (p) {
+        if (p->hybrid)
+            return fail(LSM_ERR_UNSUPPORTED, "hybrid (geometric) reinitialisation: the batched form covers the Sussman path only");
         if (p->order_bc < 1 || p->order_bc > 3) return fail(LSM_ERR_ARG, "order_bc = %d (1, 2 or 3)", p->order_bc);
         if (p->rn_itermax < 0) return fail(LSM_ERR_ARG, "rn_itermax < 0");
         if (!(p->epslsm > 0.0)) return fail(LSM_ERR_ARG, "epslsm must be > 0");

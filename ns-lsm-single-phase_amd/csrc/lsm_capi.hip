@@ -225,11 +225,15 @@ static size_t op_bytes(const Blk &b, int op, int itermax = 0)
     case LSM_OP_STEP_TRACE: return step_trace_bytes(b, itermax);
     case LSM_OP_STEP_ADJOINT_RECORDED: return step_reverse_bytes(b, itermax);
     case LSM_OP_STEP_ADJOINT: return step_trace_bytes(b, itermax) + step_reverse_bytes(b, itermax);
+    case LSM_OP_REINIT_GEOMETRIC: return geometric_work(b);
     case LSM_OP_LEVELSET_STEP: {
-        size_t a = 3 * field_bytes(b) + field_bytes(b) + field_bytes(b) + 1024;   // ucn, phi copy, rk3 scratch
+        // ucn, phi copy, rk3 scratch, AdvectionNodes mask
+        size_t a = 3 * field_bytes(b) + field_bytes(b) + field_bytes(b) + (b.n * 4 + 256) + 1024;
         size_t r = reinit_eno2_work(b);
         size_t v = water_volume_work(b);
-        return a + (r > v ? r : v);
+        size_t gw = geometric_work(b);
+        size_t m = r > v ? r : v;
+        return a + (m > gw ? m : gw);
     }
     }
     return 0;
@@ -444,6 +448,20 @@ static int gradient_entry(const LsmBlock *blk, const double *f, int nc, int orde
     return launch_gradient_eno2(b, f, nc, order_bc, eno_bc, g1, g2, g3, (hipStream_t)stream);
 }
 
+int lsm_reinit_geometric(const LsmBlock *blk, const double *phi0, double *phicorrected, int32_t *ids,
+                         void *work, size_t work_bytes, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, false, true);
+    if (rc) return rc;
+    if (!phi0 || !phicorrected) return fail(LSM_ERR_ARG, "NULL array");
+    if (phi0 == phicorrected) return fail(LSM_ERR_ARG, "phi0 and phicorrected must not alias");
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_GEOMETRIC), &c);
+    if (rc) return rc;
+    return launch_geometric(b, phi0, phicorrected, ids, c, (hipStream_t)stream);
+}
+
 int lsm_phi_gradient_eno2(const LsmBlock *blk, const double *phi, int order_bc, int eno_bc, double *g1,
                           double *g2, double *g3, void *stream)
 {
@@ -538,6 +556,8 @@ static int check_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, con
     int rc = make_blk(blk, &b, true, false);
     if (rc) return rc;
     if ((rc = check_reinit(p))) return rc;
+    if (p->hybrid)
+        return fail(LSM_ERR_UNSUPPORTED, "hybrid (geometric) reinitialisation: the step adjoint covers the Sussman path only");
     if (p->rn_itermax > kMaxTraceIters)
         return fail(LSM_ERR_UNSUPPORTED, "rn_itermax = %d: the adjoint keeps at most %d iterations",
                     p->rn_itermax, kMaxTraceIters);
@@ -685,8 +705,9 @@ int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p, double *phi
                       int call_reinit, int rn_freq, int ntime, int mode, double *vol_out,
                       void *work, size_t work_bytes, void *stream)
 {
+    const bool hybrid = p && p->hybrid && call_reinit;
     Blk b;
-    int rc = make_blk(blk, &b, true, vol_out != nullptr);
+    int rc = make_blk(blk, &b, true, vol_out != nullptr || hybrid);
     if (rc) return rc;
     if (!phi || !phi_n) return fail(LSM_ERR_ARG, "NULL array");
     if (call_levelset && (!u || !v || !w)) return fail(LSM_ERR_ARG, "NULL velocity");
@@ -695,7 +716,7 @@ int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p, double *phi
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT && mode != LSM_ADVECT_UPWIND1)
         return fail(LSM_ERR_ARG, "bad mode %d", mode);
     const bool do_reinit = call_reinit && rn_freq > 0 && ntime % rn_freq == 0;
-    if (do_reinit && (rc = check_reinit(p))) return rc;
+    if (do_reinit && !hybrid && (rc = check_reinit(p))) return rc;
     hipStream_t s = (hipStream_t)stream;
     Carver c;
     rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_LEVELSET_STEP), &c);
@@ -703,25 +724,37 @@ int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p, double *phi
     const size_t fb = b.n * sizeof(double);
     double *U1 = (double *)c.take(fb), *U2 = (double *)c.take(fb), *U3 = (double *)c.take(fb);
     double *cur = (double *)c.take(fb), *tmp = (double *)c.take(fb);
-    if (!U1 || !U2 || !U3 || !cur || !tmp) return fail(LSM_ERR_WORKSPACE, "workspace too small");
-    // levelsetmethod.F90:178-196: pre-advection water volume
-    if (vol_out) {
+    int32_t *mask = hybrid ? (int32_t *)c.take(b.n * 4) : nullptr;
+    if (!U1 || !U2 || !U3 || !cur || !tmp || (hybrid && !mask))
+        return fail(LSM_ERR_WORKSPACE, "workspace too small");
+    // levelsetmethod.F90:178-196: pre-advection water volume and the
+    // AdvectionNodes band, which restricts the advection on the hybrid path
+    // only (levelsetmethod.F90:198-199)
+    if (vol_out || hybrid) {
         Carver vc = c;
-        if ((rc = launch_water_volume(b, phi, vol_out, nullptr, vc, s))) return rc;
+        if ((rc = launch_water_volume(b, phi, vol_out, mask, vc, s))) return rc;
     }
     // levelsetmethod.F90:221-243: contravariant velocity, numiter_lsm RK3 steps from phi_n
     if (call_levelset) {
         if ((rc = launch_contra(b, u, v, w, (long)vel_stride, U1, U2, U3, s))) return rc;
         if ((rc = launch_copy(phi_n, cur, b.n, s))) return rc;
         for (int it = 0; it < numiter_lsm; ++it) {
-            if ((rc = launch_rk3(b, phi, cur, nullptr, U1, U2, U3, delti_lsm, mode, tmp, s))) return rc;
+            if ((rc = launch_rk3(b, phi, cur, mask, U1, U2, U3, delti_lsm, mode, tmp, s))) return rc;
             if (it + 1 < numiter_lsm && (rc = launch_copy(phi, cur, b.n, s))) return rc;
         }
     }
-    // levelsetmethod.F90:290-326: Sussman reinitialisation, bcond_lsm, phi = phicorrected
+    // levelsetmethod.F90:290-326: reinitialisation (geometric on the hybrid
+    // path, Sussman otherwise), bcond_lsm, phi = phicorrected
     if (do_reinit) {
         Carver rcv = c;
-        if ((rc = launch_reinit_eno2(b, *p, phi, tmp, nullptr, rcv, s))) return rc;
+        if (hybrid) {
+            // phicorrected: +-9.9 away from the zero set, the advected phi
+            // (0 without advection) where |phi| <= eps (levelsetmethod.F90:300-301)
+            if ((rc = launch_hybrid_init(phi, call_levelset ? phi : nullptr, tmp, b.n, s))) return rc;
+            if ((rc = launch_geometric(b, phi, tmp, nullptr, rcv, s))) return rc;
+        } else {
+            if ((rc = launch_reinit_eno2(b, *p, phi, tmp, nullptr, rcv, s))) return rc;
+        }
         if ((rc = launch_bcond(b, tmp, s))) return rc;
         if ((rc = launch_copy(tmp, phi, b.n, s))) return rc;
     }

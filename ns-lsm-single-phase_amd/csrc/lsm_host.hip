@@ -383,6 +383,23 @@ int lsm_host_reinit_eno2(const double *phi0, double *phi_out, const LsmReinitPar
     return down(phi_out, g_h.f[1], n);
 }
 
+int lsm_host_reinit_geometric(const double *phizero, double *phicorrected, int32_t *ids)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    if (!g_h.blk.x) return fail(LSM_ERR_ARG, "coordinates were not passed to lsm_host_setup");
+    if (!phizero || !phicorrected) return fail(LSM_ERR_ARG, "NULL argument");
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phizero, n)) || (rc = up(g_h.f[1], phicorrected, n))) return rc;
+    if ((rc = lsm_reinit_geometric(&g_h.blk, g_h.f[0], g_h.f[1], ids ? g_h.mask : nullptr, nullptr, 0, nullptr)))
+        return rc;
+    if ((rc = sync())) return rc;
+    if ((rc = down(phicorrected, g_h.f[1], n))) return rc;
+    if (ids) return check_hip(hipMemcpy(ids, g_h.mask, n * sizeof(int32_t), hipMemcpyDeviceToHost), "D2H");
+    return LSM_OK;
+}
+
 int lsm_host_sign_ini(const double *phi0, double epslsm, int m_sphi0, double *sgn, double *ng)
 {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -74,6 +74,12 @@ int launch_fold(const Blk &b, const double *U1, const double *U2, const double *
                 double *V1, double *V2, double *V3, int *bad, hipStream_t s);
 int launch_copy(const double *src, double *dst, long n, hipStream_t s);
 
+// ---- launchers (lsm_geometric.hip): hybrid reinitialisation ----
+size_t geometric_work(const Blk &b);
+// pc = -9.9 where phi < -eps, 9.9 where phi > eps, base (0 when NULL) elsewhere
+int launch_hybrid_init(const double *phi, const double *base, double *pc, long n, hipStream_t s);
+int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids, Carver &c, hipStream_t s);
+
 // ---- launchers (lsm_upwind.hip): orderAD = 0 ----
 int launch_upwind_rhs(const Blk &b, const double *phi, const double *U1, const double *U2, const double *U3,
                       const int32_t *mask, double *rhs, hipStream_t s);

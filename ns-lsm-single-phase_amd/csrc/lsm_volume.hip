@@ -462,16 +462,19 @@ int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *m
     if (!partial || !cutflag || (mask && !chg)) return fail(LSM_ERR_WORKSPACE, "volume workspace too small");
     int rc;
     const dim3 blk(VTX, VTY, 1);
-    VolArgs a{phi, b.x, b.y, b.z, partial, cutflag, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk,
-              (b.e.ni - 1 + VTX - 1) / VTX, (b.e.nj - 1 + VTY - 1) / VTY};
-    {
-        KTimer kt(LSM_KT_VOLUME, s);
-        volume_wet<<<(unsigned)nblk, blk, 0, s>>>(a);
-        volume_cut<<<(unsigned)nblk, blk, 0, s>>>(a);
+    // vol == NULL: the AdvectionNodes mask alone
+    if (vol) {
+        VolArgs a{phi, b.x, b.y, b.z, partial, cutflag, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk,
+                  (b.e.ni - 1 + VTX - 1) / VTX, (b.e.nj - 1 + VTY - 1) / VTY};
+        {
+            KTimer kt(LSM_KT_VOLUME, s);
+            volume_wet<<<(unsigned)nblk, blk, 0, s>>>(a);
+            volume_cut<<<(unsigned)nblk, blk, 0, s>>>(a);
+        }
+        if ((rc = check_hip(hipGetLastError(), "volume kernels"))) return rc;
+        volume_final<<<1, 1024, 0, s>>>(partial, nblk, vol);
+        if ((rc = check_hip(hipGetLastError(), "volume_final"))) return rc;
     }
-    if ((rc = check_hip(hipGetLastError(), "volume kernels"))) return rc;
-    volume_final<<<1, 1024, 0, s>>>(partial, nblk, vol);
-    if ((rc = check_hip(hipGetLastError(), "volume_final"))) return rc;
     if (mask) {
         MaskArgs m{phi, chg, mask, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, (b.e.ni + VTX - 1) / VTX,
                    (b.e.nj + VTY - 1) / VTY};

@@ -21,6 +21,7 @@ module lsm_hip_f
    type, bind(C) :: lsm_reinit_params
       real(c_double)     :: epslsm, eps_reinit, rn_timestep
       integer(c_int32_t) :: rn_itermax, order_bc, eno_bc, m_sphi0, m_ti
+      integer(c_int32_t) :: hybrid = 0
    end type lsm_reinit_params
 
    interface
@@ -112,6 +113,14 @@ module lsm_hip_f
          real(c_double), intent(inout)       :: phi_out(*)
          type(lsm_reinit_params), intent(in) :: p
       end function lsm_host_reinit_godunov
+
+      integer(c_int) function lsm_host_reinit_geometric(phizero, phicorrected, ids) &
+            bind(C, name="lsm_host_reinit_geometric")
+         import :: c_int, c_int32_t, c_double
+         real(c_double), intent(in)        :: phizero(*)
+         real(c_double), intent(inout)     :: phicorrected(*)
+         integer(c_int32_t), intent(inout) :: ids(*)
+      end function lsm_host_reinit_geometric
 
       ! an absent mask is passed as NULL (no AdvectionNodes output)
       integer(c_int) function lsm_host_water_volume(phi, vol, mask) bind(C, name="lsm_host_water_volume")

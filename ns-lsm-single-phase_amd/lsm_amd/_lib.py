@@ -19,6 +19,7 @@ LSM_ADVECT_UPWIND1 = 2
 LSM_OP_RK3, LSM_OP_REINIT_ENO2, LSM_OP_REINIT_GODUNOV, LSM_OP_WATER_VOLUME, LSM_OP_LEVELSET_STEP = 1, 2, 3, 4, 5
 LSM_OP_RK3_ADJOINT = 6
 LSM_OP_REINIT_ENO2_ADJOINT, LSM_OP_STEP_ADJOINT, LSM_OP_STEP_TRACE, LSM_OP_STEP_ADJOINT_RECORDED = 7, 8, 9, 10
+LSM_OP_REINIT_GEOMETRIC = 11
 
 _vp = C.c_void_p
 
@@ -37,7 +38,7 @@ class LsmBlock(C.Structure):
 class LsmReinitParams(C.Structure):
     _fields_ = [("epslsm", C.c_double), ("eps_reinit", C.c_double), ("rn_timestep", C.c_double),
                 ("rn_itermax", C.c_int32), ("order_bc", C.c_int32), ("eno_bc", C.c_int32),
-                ("m_sphi0", C.c_int32), ("m_ti", C.c_int32)]
+                ("m_sphi0", C.c_int32), ("m_ti", C.c_int32), ("hybrid", C.c_int32)]
 
 
 class LsmBatchBlock(C.Structure):
@@ -114,6 +115,7 @@ _SIGS = {
     "lsm_sign_ini": (C.c_int, [C.POINTER(LsmBlock), _vp, C.c_double, C.c_int, _vp, _vp, _vp]),
     "lsm_rh_rn": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp]),
     "lsm_lambda_rn": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp]),
+    "lsm_reinit_geometric": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "lsm_reinit_godunov": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp,
                                      C.c_size_t, _vp]),
     "lsm_water_volume": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
@@ -143,6 +145,7 @@ _SIGS = {
     "lsm_host_h_gradient_eno2": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, _vp, _vp, _vp,
                                            _vp, C.c_int, C.c_int, _vp]),
     "lsm_host_calc_h": (C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, _vp]),
+    "lsm_host_reinit_geometric": (C.c_int, [_vp, _vp, _vp]),
 }
 
 EXPORTS = tuple(_SIGS)
@@ -151,6 +154,7 @@ LSM_KT_ADJOINT_STAGE = 4
 LSM_KT_BATCH_STAGE = 5
 LSM_KT_BATCH_REINIT = 6
 LSM_KT_REINIT_ADJOINT = 7
+LSM_KT_GEOMETRIC = 8
 
 _lib = None
 

@@ -113,6 +113,7 @@ class ReinitParams:
     ENOBCReinitialisation: bool = False
     M_Sphi0: int = 1
     M_TI: int = 1
+    hybrid_reinitialisation: bool = False   # levelsetmethod only: geometric reinitialisation
 
     def c(self) -> LsmReinitParams:
         p = LsmReinitParams()
@@ -124,6 +125,7 @@ class ReinitParams:
         p.eno_bc = int(bool(self.ENOBCReinitialisation))
         p.m_sphi0 = int(self.M_Sphi0)
         p.m_ti = int(self.M_TI)
+        p.hybrid = int(bool(self.hybrid_reinitialisation))
         return p
 
 
@@ -331,6 +333,22 @@ def reinitialisation_test2(blk: Block, phi0, params: ReinitParams, out=None, wor
     return out, iters
 
 
+def geometric_reinitialisation(blk: Block, phizero, phicorrected, InterfaceNodesID=None, work=None):
+    """geometric_reinitialisation.F90:1-94 (GlobalMassCorrection = .false.,
+    TriangulationCase 1): phicorrected (updated in place and returned) gets the
+    signed distance to the marching-tetrahedra isosurface of phizero at the
+    interface nodes and min(|phicorrected|, distance) with its own sign at the
+    nodes within 4 nodes of them.  InterfaceNodesID (int32, optional) receives
+    1 / -1 / 0.  Needs the block's coordinates."""
+    L = _lib.load()
+    wp, wn, keep = _ws(blk, _lib.LSM_OP_REINIT_GEOMETRIC, work)
+    check(L.lsm_reinit_geometric(blk.ref, _f64(phizero, blk.shape, "phizero"),
+                                 _f64(phicorrected, blk.shape, "phicorrected"),
+                                 _i32(InterfaceNodesID, blk.shape, "InterfaceNodesID"), wp, wn, _stream()),
+          "lsm_reinit_geometric")
+    return phicorrected
+
+
 def reinitialisation_test2_adjoint(blk: Block, phi0, params: ReinitParams, adj_out, out=None,
                                   phi_out=None, work=None):
     """Reverse mode of reinitialisation_test2: given adj_out = dJ/dphi_out,
@@ -415,8 +433,10 @@ def levelsetmethod(blk: Block, phi, phi_n, u, v, w, params: ReinitParams, delti_
                    numiter_lsm: int = 1, call_levelsetmethod: bool = True,
                    call_reinitialisation: bool = True, RNfreq: int = 1, ntime: int = 1,
                    mode="fast", want_volume: bool = True, work=None):
-    """levelsetmethod.F90:1-372 (non-hybrid Sussman path): phi is overwritten
-    with phi^(n+1); returns the pre-advection water volumes (or None)."""
+    """levelsetmethod.F90:1-372: phi is overwritten with phi^(n+1); returns
+    the pre-advection water volumes (or None).  Sussman reinitialisation, or
+    with params.hybrid_reinitialisation the geometric one, the advection then
+    restricted to the AdvectionNodes band."""
     L = _lib.load()
     vol = torch.zeros(3, dtype=torch.float64, device=blk.device) if want_volume else None
     wp, wn, keep = _ws(blk, _lib.LSM_OP_LEVELSET_STEP, work)

@@ -207,8 +207,8 @@ class LsmParams:
             bad.append(f"orderAD = {self.orderAD} (0: first-order upwind, 1: WENO3)")
         if self.sussman_correction_method != 0:
             bad.append("sussman_correction_method = 1 (Simpson filter)")
-        if self.hybrid_reinitialisation and self.call_reinitialisation:
-            bad.append("hybrid (geometric) reinitialisation")
+        if self.hybrid_reinitialisation and self.call_reinitialisation and self.GlobalMassCorrection:
+            bad.append("hybrid reinitialisation with GlobalMassCorrection (global mass correction)")
         if self.nobstacles != 0:
             bad.append(f"{self.nobstacles} obstacles")
         if self.OrderReinitialisationBoundaries not in (1, 2, 3):
@@ -226,7 +226,8 @@ class LsmParams:
                                 epsReinitialisation=self.epsReinitialisation,
                                 OrderReinitialisationBoundaries=self.OrderReinitialisationBoundaries,
                                 ENOBCReinitialisation=self.ENOBCReinitialisation,
-                                M_Sphi0=M_Sphi0, M_TI=M_TI)
+                                M_Sphi0=M_Sphi0, M_TI=M_TI,
+                                hybrid_reinitialisation=self.hybrid_reinitialisation)
 
 
 def read_ind3dmg_lsm(src: str, nzone: int = 1, delti: Optional[float] = None) -> LsmParams:

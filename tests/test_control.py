@@ -139,10 +139,15 @@ def test_list_directed_forms():
 
 def test_unsupported_options_are_rejected(ctl):
     prm = C.from_control(ctl)
-    prm.hybrid_reinitialisation = True
-    with pytest.raises(NotImplementedError, match="hybrid"):
+    prm.hybrid_reinitialisation = True          # geometric reinitialisation: supported
+    prm.GlobalMassCorrection = False
+    prm.check_supported()
+    assert prm.reinit_params().hybrid_reinitialisation
+    prm.GlobalMassCorrection = True             # ... without the global mass correction
+    with pytest.raises(NotImplementedError, match="GlobalMassCorrection"):
         prm.check_supported()
     prm.hybrid_reinitialisation = False
+    prm.GlobalMassCorrection = False
     prm.orderAD = 0                    # the first-order upwind right-hand side is supported
     prm.check_supported()
     prm.orderAD = 2
@@ -155,11 +160,12 @@ REF_CONTROL = "/root/reference/control.dat"
 
 @pytest.mark.skipif(not os.path.exists(REF_CONTROL), reason="reference tree not present")
 def test_reference_control_file_maps():
-    """The reference's own control.dat goes through the same mapping (its
-    hybrid geometric reinitialisation is reported as unsupported)."""
+    """The reference's own control.dat goes through the same mapping; its
+    configuration (hybrid geometric reinitialisation without the global mass
+    correction) is supported."""
     prm = C.from_control(REF_CONTROL)
     assert prm.numiter_lsm >= 1 and prm.delti_lsm > 0
     assert len(prm.btype_lsm[0]) == 6
-    if prm.hybrid_reinitialisation and prm.call_reinitialisation:
-        with pytest.raises(NotImplementedError):
-            prm.check_supported()
+    assert prm.hybrid_reinitialisation and not prm.GlobalMassCorrection
+    prm.check_supported()
+    assert prm.reinit_params().hybrid_reinitialisation

@@ -201,3 +201,28 @@ def test_replay_reproduces_golden_with_reference(name, key, mode, over):
         assert np.array_equal(outs[n], fx.out(key, n)), (key, n)
         n += 1
     assert n == len(outs)
+
+
+HYBRID = [("hybrid_cube", "geometric"), ("hybrid_cube", "levelset"), ("hybrid_channel", "geometric"),
+          ("hybrid_channel", "levelset")] + [("hybrid_planes", f"geometric_{p}")
+                                             for p in ("flat", "tilt", "diag", "sphere")]
+
+
+@pytest.mark.parametrize("name,key", HYBRID, ids=[f"{a}-{b}" for a, b in HYBRID])
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(refio.SHIM_HARNESS),
+                    reason="drop-in harness not built (make -C oracle/ref shim)")
+def test_dropin_hybrid_levelsetmethod(name, key):
+    """The unmodified levelsetmethod.F90 with hybrid_reinitialisation =
+    .true.: the geometric_reinitialisation drop-in (and the band-restricted
+    advection) on the GPU, bit for bit."""
+    fx = load(name)
+    cl = fx.kw(key, "call_levelset")
+    if cl is None:
+        cl = bool(fx.d["p_call_levelset"]) if "p_call_levelset" in fx.d else True
+    over = {"hybrid": True, "rn_freq": 1, "call_levelset": cl}
+    if name == "hybrid_planes":
+        p = fx.inp("plane_" + key.split("_", 1)[1])
+        over.update(phi=p, phi0=p)
+    outs, _ = replay(fx, key, "levelset", **over)
+    assert np.array_equal(outs[0], fx.out(key)), linf_rel(outs[0], fx.out(key))
