@@ -109,6 +109,54 @@ def _time(fn, reps, warm=2):
     return e0.elapsed_time(e1) / reps
 
 
+def bench_geometric(g, phi, blk, vel, dt):
+    """The geometric (hybrid) reinitialisation at the bench size:
+    geometric_reinitialisation alone (cut-cell search, isosurfaces, interface
+    and neighbour flags, distances; one stream synchronisation) and a whole
+    hybrid levelsetmethod call (AdvectionNodes band, band-restricted RK3,
+    geometric reinitialisation, bcond).  The distance kernel is bound by FP64
+    arithmetic and L2 traffic, not HBM: its figure is band nodes per second."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from lsm_amd import api, _lib
+    L = _lib.load()
+    pc0 = torch.where(phi < -2.220446049250313e-16, torch.full_like(phi, -9.9),
+                      torch.where(phi > 2.220446049250313e-16, torch.full_like(phi, 9.9), torch.zeros_like(phi)))
+    pc = pc0.clone()
+    ids = blk.empty(torch.int32)
+    work = blk.workspace(_lib.LSM_OP_REINIT_GEOMETRIC)
+    api.geometric_reinitialisation(blk, phi, pc, ids, work=work)
+    torch.cuda.synchronize()
+    nint = int((ids == 1).sum())
+    nnb = int((ids == -1).sum())
+    L.lsm_timing_enable(1)
+
+    def one():
+        pc.copy_(pc0)
+        api.geometric_reinitialisation(blk, phi, pc, None, work=work)
+    ms = _time(one, 5)
+    kms, kn = C.c_double(0), C.c_int64(0)
+    L.lsm_timing_get(_lib.LSM_KT_GEOMETRIC, C.byref(kms), C.byref(kn))
+    L.lsm_timing_enable(0)
+    prm = api.ReinitParams(epslsm=0.05, RNtimestep=1e-3, RNitermax=4, epsReinitialisation=1e-5,
+                           hybrid_reinitialisation=True)
+    uvw = [torch.as_tensor(np.ascontiguousarray(a), device="cuda") for a in vel]
+    pl = phi.clone()
+    pn = phi.clone()
+    lw = blk.workspace(_lib.LSM_OP_LEVELSET_STEP)
+
+    def step():
+        pn.copy_(phi)
+        api.levelsetmethod(blk, pl, pn, *uvw, prm, dt, ntime=1, want_volume=False, work=lw)
+    ls_ms = _time(step, 3)
+    dist_ms = kms.value / max(kn.value, 1)
+    return {"interface_nodes": nint, "neighbour_nodes": nnb, "reinit_ms": round(ms, 3),
+            "distance_kernel_ms": round(dist_ms, 3),
+            "band_nodes_per_s": (nint + nnb) / (ms / 1e3),
+            "hybrid_levelsetmethod_ms": round(ls_ms, 3)}
+
+
 def bench_step_adjoint(g, phi, ucn, dt, blk):
     """One full level-set step (ad_RKTVD3, 4 Sussman iterations, bcond_lsm)
     and its hand-written adjoint (lsm_levelset_step_adjoint: forward
@@ -349,7 +397,8 @@ def main():
     ap.add_argument("--no-inverse", action="store_true", help="skip the 32-channel inverse problem")
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "step-adjoint", "batch", "differentiation", "consumers", "inverse"],
+                    choices=["all", "step-adjoint", "batch", "differentiation", "consumers", "inverse",
+                             "geometric"],
                     help="run only one secondary workload and print its figures (for profiling); "
                          "'all' is the benchmark line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -394,6 +443,9 @@ def main():
         elif args.workload == "consumers":
             g, phi_np, _, _ = make_case(n)
             res = bench_consumers(g, torch.as_tensor(phi_np, device="cuda"), api.Block(g))
+        elif args.workload == "geometric":
+            g, phi_np, vel, dt = make_case(n)
+            res = bench_geometric(g, torch.as_tensor(phi_np, device="cuda"), api.Block(g), vel, dt)
         else:
             g, phi_np, vel, dt = make_case(n)
             blk = api.Block(g)
@@ -561,6 +613,9 @@ def main():
         extra["levelsetmethod_cells_per_s"] = interior / (ls_ms / 1e3)
         log(f"levelsetmethod (4 reinit iterations): {ls_ms:.2f} ms per call")
 
+    if not args.no_extra:
+        extra["geometric"] = bench_geometric(g, bufs[base], blk, vel, dt)
+        log(f"geometric: {extra['geometric']}")
     if not args.no_extra:
         extra["consumers"] = bench_consumers(g, bufs[base], blk)
         log(f"consumers: {extra['consumers']}")

@@ -258,9 +258,30 @@ __device__ int tet_isosurface(const double v[4][3], const double ph[4], double *
     return area > 0.0 ? ntri : 0;
 }
 
+// bounding sphere (centroid, radius rounded up) and shortest edge (rounded
+// down) of a triangle: the distance bound that lets the search skip it
+__device__ void tri_bounds(const double *T, double *bd)
+{
+    const double cx = (T[0] + T[3] + T[6]) / 3.0, cy = (T[1] + T[4] + T[7]) / 3.0, cz = (T[2] + T[5] + T[8]) / 3.0;
+    double r = 0.0;
+    for (int q = 0; q < 3; ++q) {
+        const double dx = T[3 * q] - cx, dy = T[3 * q + 1] - cy, dz = T[3 * q + 2] - cz;
+        r = fmax(r, sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    double emin = 1e300;
+    for (int q = 0; q < 3; ++q) {
+        const int p2 = (q + 1) % 3;
+        const double dx = T[3 * q] - T[3 * p2], dy = T[3 * q + 1] - T[3 * p2 + 1], dz = T[3 * q + 2] - T[3 * p2 + 2];
+        emin = fmin(emin, sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    bd[0] = cx; bd[1] = cy; bd[2] = cz;
+    bd[3] = r * (1.0 + 1e-12);
+    bd[4] = emin * (1.0 - 1e-12);
+}
+
 __global__ __launch_bounds__(256) void geo_iso(Geom g, long ncell, const uint8_t *__restrict__ ctets,
                                                const int32_t *__restrict__ cslot, int8_t *__restrict__ nuse,
-                                               double *__restrict__ tris)
+                                               double *__restrict__ tris, double *__restrict__ bnd)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
@@ -280,8 +301,10 @@ __global__ __launch_bounds__(256) void geo_iso(Geom g, long ncell, const uint8_t
         }
         double tri[18];
         const int nu = tet_isosurface(v, ph, tri);
-        nuse[slot + t] = (int8_t)nu;
-        for (int q = 0; q < 9 * nu; ++q) tris[(slot + t) * 18 + q] = tri[q];
+        const long rec = slot + t;
+        nuse[rec] = (int8_t)nu;
+        for (int q = 0; q < 9 * nu; ++q) tris[rec * 18 + q] = tri[q];
+        for (int u = 0; u < nu; ++u) tri_bounds(tri + 9 * u, bnd + rec * 10 + 5 * u);
     }
 }
 
@@ -293,28 +316,50 @@ __device__ __forceinline__ void node_of(const Geom &g, long l, int &i, int &j, i
     k = (int)(r / g.nj);
 }
 
+// appends the flagged lanes' values to list (one atomic per wavefront); the
+// whole wavefront must call it
+__device__ __forceinline__ int wave_append(bool flag, int value, int32_t *list, int32_t *count)
+{
+    const unsigned long long bal = __ballot(flag);
+    if (!bal) return -1;
+    const int lane = threadIdx.x & 63;
+    const int first = __ffsll((long long)bal) - 1;
+    int base = 0;
+    if (lane == first) base = atomicAdd(count, (int32_t)__popcll(bal));
+    base = __shfl(base, first);
+    if (!flag) return -1;
+    const int pos = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
+    list[pos] = value;
+    return pos;
+}
+
 // a node is an interface node when it is a vertex of a K-tetrahedron of one
-// of its cells (GetTetrahedraList.F90:262-269)
+// of its cells (GetTetrahedraList.F90:262-269); interface nodes are listed
+// and mid maps each to its list position
 __global__ __launch_bounds__(256) void geo_nodes(Geom g, long n, const uint8_t *__restrict__ cpn,
-                                                 int8_t *__restrict__ nid)
+                                                 int8_t *__restrict__ nid, int32_t *__restrict__ mid,
+                                                 int32_t *__restrict__ ilist, int32_t *__restrict__ count)
 {
     const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
-    int i, j, k;
-    node_of(g, l, i, j, k);
-    int8_t f = 0;
-    for (int dk = -1; dk <= 0; ++dk)
-        for (int dj = -1; dj <= 0; ++dj)
-            for (int di = -1; di <= 0; ++di) {
-                const int ci = i + di, cj = j + dj, ck = k + dk;
-                if (ci < 0 || cj < 0 || ck < 0 || ci > g.ni - 2 || cj > g.nj - 2 || ck > g.nk - 2) continue;
-                const int pn = cpn[(long)ci + g.csj * cj + g.csk * ck];
-                // vertex index of this node in that cell
-                const int ox = -di, oy = -dj, oz = -dk;
-                const int vq = oz * 4 + (oy ? (ox ? 2 : 3) : ox);
-                if (pn & (1 << vq)) f = 1;
-            }
-    nid[l] = f;
+    bool f = false;
+    if (l < n) {
+        int i, j, k;
+        node_of(g, l, i, j, k);
+        for (int dk = -1; dk <= 0; ++dk)
+            for (int dj = -1; dj <= 0; ++dj)
+                for (int di = -1; di <= 0; ++di) {
+                    const int ci = i + di, cj = j + dj, ck = k + dk;
+                    if (ci < 0 || cj < 0 || ck < 0 || ci > g.ni - 2 || cj > g.nj - 2 || ck > g.nk - 2) continue;
+                    const int pn = cpn[(long)ci + g.csj * cj + g.csk * ck];
+                    // vertex index of this node in that cell
+                    const int ox = -di, oy = -dj, oz = -dk;
+                    const int vq = oz * 4 + (oy ? (ox ? 2 : 3) : ox);
+                    if (pn & (1 << vq)) f = true;
+                }
+        nid[l] = f ? 1 : 0;
+    }
+    const int pos = wave_append(f, (int)l, ilist, count);
+    if (f) mid[l] = pos;
 }
 
 __device__ __forceinline__ void box(int c, int n, int &lo, int &hi)
@@ -324,99 +369,224 @@ __device__ __forceinline__ void box(int c, int n, int &lo, int &hi)
 }
 
 // neighbours: the nodes the search boxes of GetIsosurfaceAndDistances.F90:153-240
-// visit without being interface nodes (InterfaceNodesID = -1)
+// visit without being interface nodes (InterfaceNodesID = -1), listed
 __global__ __launch_bounds__(256) void geo_mark(Geom g, long n, const int8_t *__restrict__ nid,
-                                                uint8_t *__restrict__ nmark)
+                                                uint8_t *__restrict__ nmark, int32_t *__restrict__ nlist,
+                                                int32_t *__restrict__ count)
 {
     const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
+    bool m = false;
+    if (l < n) {
+        if (nid[l] == 0) {
+            int i, j, k;
+            node_of(g, l, i, j, k);
+            int i0, i1, j0, j1, k0, k1;
+            box(i, g.ni, i0, i1); box(j, g.nj, j0, j1); box(k, g.nk, k0, k1);
+            for (int ks = k0; ks <= k1 && !m; ++ks)
+                for (int js = j0; js <= j1 && !m; ++js)
+                    for (int is = i0; is <= i1; ++is)
+                        if (nid[(long)is + g.sj * js + g.sk * ks] == 1) { m = true; break; }
+        }
+        nmark[l] = m ? 1 : 0;
+    }
+    (void)wave_append(m, (int)l, nlist, count);
+}
+
+// per interface node: its K-tetrahedra in the order GetTetrahedraList.F90:346-418
+// appends them (cells in k, j, i order, tetrahedra 1..6), their number, and
+// a sphere about the node holding all their triangles (radius up, shortest
+// edge down)
+__global__ __launch_bounds__(256) void geo_iface(Geom g, long nint, const int32_t *__restrict__ ilist,
+                                                 const uint8_t *__restrict__ ctets, const int32_t *__restrict__ cslot,
+                                                 const int8_t *__restrict__ nuse, const double *__restrict__ bnd,
+                                                 int32_t *__restrict__ mrec, int8_t *__restrict__ nkt,
+                                                 double *__restrict__ mrad, double *__restrict__ memin)
+{
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nint) return;
+    const long l = ilist[m];
     int i, j, k;
     node_of(g, l, i, j, k);
-    uint8_t m = 0;
-    if (nid[l] == 0) {
-        int i0, i1, j0, j1, k0, k1;
-        box(i, g.ni, i0, i1); box(j, g.nj, j0, j1); box(k, g.nk, k0, k1);
-        for (int ks = k0; ks <= k1 && !m; ++ks)
-            for (int js = j0; js <= j1 && !m; ++js)
-                for (int is = i0; is <= i1; ++is)
-                    if (nid[(long)is + g.sj * js + g.sk * ks] == 1) { m = 1; break; }
-    }
-    nmark[l] = m;
+    const double X = g.x[l], Y = g.y[l], Z = g.z[l];
+    int q = 0;
+    double R = 0.0, em = 1e300;
+    bool any = false;
+    for (int dk = -1; dk <= 0; ++dk)
+        for (int dj = -1; dj <= 0; ++dj)
+            for (int di = -1; di <= 0; ++di) {
+                const int ci = i + di, cj = j + dj, ck = k + dk;
+                if (ci < 0 || cj < 0 || ck < 0 || ci > g.ni - 2 || cj > g.nj - 2 || ck > g.nk - 2) continue;
+                const long c = (long)ci + g.csj * cj + g.csk * ck;
+                const int tm = ctets[c];
+                if (!tm) continue;
+                const int ox = -di, oy = -dj, oz = -dk;
+                const int vq = oz * 4 + (oy ? (ox ? 2 : 3) : ox);
+                const long slot = (long)cslot[c] * 6;
+                for (int t = 0; t < 6; ++t) {
+                    if (!((tm >> t) & 1) || !((kTetHas[t] >> vq) & 1)) continue;
+                    const long rec = slot + t;
+                    mrec[m * 24 + q] = (int32_t)rec;
+                    ++q;
+                    const int nu = nuse[rec];
+                    for (int u = 0; u < nu; ++u) {
+                        const double *bd = bnd + rec * 10 + 5 * u;
+                        const double dx = bd[0] - X, dy = bd[1] - Y, dz = bd[2] - Z;
+                        R = fmax(R, sqrt(dx * dx + dy * dy + dz * dz) + bd[3]);
+                        em = fmin(em, bd[4]);
+                        any = true;
+                    }
+                }
+            }
+    nkt[m] = (int8_t)q;
+    mrad[m] = any ? R * (1.0 + 1e-12) : -1.0;   // -1: no usable triangle
+    memin[m] = em;
 }
 
 struct DistArgs {
     Geom g;
-    long n;
+    long nint, nnb;
+    const int32_t *ilist, *nlist, *mid;
     const int8_t *nid;
-    const uint8_t *nmark, *ctets;
-    const int32_t *cslot;
+    const int32_t *mrec;
+    const int8_t *nkt;
+    const double *mrad, *memin;
     const int8_t *nuse;
-    const double *tris;
+    const double *tris, *bnd;
     double *pc;
 };
 
-// the running minimum of GetIsosurfaceAndDistances.F90:201-233 over the
-// isosurface of one K-tetrahedron
-__device__ __forceinline__ double ktet_min(const DistArgs &a, long rec, const V3 &P, double D)
+// True when every distance the reference's DistancePointTriangle can return
+// for points of a sphere (centre c, radius r, shortest triangle edge emin)
+// exceeds cmp: |P - c| - r bounds the exact distance from below, and the
+// computed distance is within err of the exact one -- rounding of the
+// coordinates (cmax), of the norms and of the cross-product line distance,
+// whose error grows like |AP| |BP| / |AB| (L^2 / emin) -- with a margin of
+// more than three orders of magnitude.  A triangle with a vanishing edge
+// (emin = 0) is never skipped.
+__device__ __forceinline__ bool beyond(double cx, double cy, double cz, double r, double emin, const V3 &P,
+                                       double cmax, double cmp)
 {
-    const int nu = a.nuse[rec];
-    for (int t = 0; t < nu; ++t) {
-        const double Da = dist_point_triangle(a.tris + rec * 18 + 9 * t, P);
-        const double ex = (copysign(1.0, D - Da) + 1.0) / 2.0;
-        D = D + ex * (Da - D);
-    }
-    return D;
+    const double dx = P.x - cx, dy = P.y - cy, dz = P.z - cz;
+    const double dP = sqrt(dx * dx + dy * dy + dz * dz);
+    const double L = dP + r;
+    const double err = 1e-12 * (L + L * (L / emin) + cmax + L);
+    return dP - r - err > cmp;
 }
 
+__device__ __forceinline__ double sd_update(double SD, double D, double sgn0)
+{
+    const double ex2 = (copysign(1.0, fabs(SD) - D) + 1.0) / 2.0;
+    return sgn0 * (fabs(SD) + ex2 * (D - fabs(SD)));
+}
+
+__device__ __forceinline__ double pc_update(double pc, double D)
+{
+    const double ex2 = (copysign(1.0, fabs(pc) - D) + 1.0) / 2.0;
+    const double aux = fabs(pc) + ex2 * (D - fabs(pc));
+    return copysign(1.0, pc) * aux;
+}
+
+// One thread per listed node (interface nodes, then neighbours).  The node
+// walks the sequence of GetIsosurfaceAndDistances.F90:153-240 (interface
+// nodes: D and SD carried over the whole search box) or
+// NeighboursPhiCorrection.F90:133-249 (neighbours: D restarted per interface
+// node, phicorrected updated after every K-tetrahedron): the interface nodes
+// of its box in k, j, i order, their K-tetrahedra, their triangles.  A
+// triangle whose distance provably exceeds the running minimum D leaves D
+// unchanged and is not evaluated; an interface node none of whose triangles
+// can reach D (neighbours: |phicorrected|) leaves everything but the
+// repeated SD updates unchanged.  Every update that does happen sees the
+// values and the order of the reference, so the results are bit-identical.
+// The walk is a cursor: lanes advance through the cheap steps until each has
+// a triangle to evaluate, then evaluate together.
 __global__ __launch_bounds__(256) void geo_distance(DistArgs a)
 {
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= a.n) return;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nint + a.nnb) return;
     const Geom &g = a.g;
-    const bool iface = a.nid[l] == 1;
-    if (!iface && !a.nmark[l]) return;
+    const bool iface = t < a.nint;
+    const long l = iface ? a.ilist[t] : a.nlist[t - a.nint];
     int i, j, k;
     node_of(g, l, i, j, k);
     const V3 P{g.x[l], g.y[l], g.z[l]};
+    const double cmax = fmax(fabs(P.x), fmax(fabs(P.y), fabs(P.z)));
     int i0, i1, j0, j1, k0, k1;
     box(i, g.ni, i0, i1); box(j, g.nj, j0, j1); box(k, g.nk, k0, k1);
     const double sgn0 = copysign(1.0, g.phi0[l]);
     double D = 10.0, SD = 99.9;
     double pc = iface ? 0.0 : a.pc[l];
-    for (int ks = k0; ks <= k1; ++ks)
-        for (int js = j0; js <= j1; ++js)
-            for (int is = i0; is <= i1; ++is) {
-                if (a.nid[(long)is + g.sj * js + g.sk * ks] != 1) continue;
-                if (!iface) D = 10.0;
-                // the K-tetrahedra of interface node (is, js, ks) in the order
-                // GetTetrahedraList.F90:346-418 appends them
-                for (int dk = -1; dk <= 0; ++dk)
-                    for (int dj = -1; dj <= 0; ++dj)
-                        for (int di = -1; di <= 0; ++di) {
-                            const int ci = is + di, cj = js + dj, ck = ks + dk;
-                            if (ci < 0 || cj < 0 || ck < 0 || ci > g.ni - 2 || cj > g.nj - 2 || ck > g.nk - 2)
-                                continue;
-                            const long c = (long)ci + g.csj * cj + g.csk * ck;
-                            const int tm = a.ctets[c];
-                            if (!tm) continue;
-                            const int ox = -di, oy = -dj, oz = -dk;
-                            const int vq = oz * 4 + (oy ? (ox ? 2 : 3) : ox);
-                            const long slot = (long)a.cslot[c] * 6;
-                            for (int t = 0; t < 6; ++t) {
-                                if (!((tm >> t) & 1) || !((kTetHas[t] >> vq) & 1)) continue;
-                                D = ktet_min(a, slot + t, P, D);
-                                if (iface) {
-                                    const double ex2 = (copysign(1.0, fabs(SD) - D) + 1.0) / 2.0;
-                                    SD = fabs(SD) + ex2 * (D - fabs(SD));
-                                    SD = sgn0 * SD;
-                                } else {
-                                    const double ex2 = (copysign(1.0, fabs(pc) - D) + 1.0) / 2.0;
-                                    const double aux = fabs(pc) + ex2 * (D - fabs(pc));
-                                    pc = copysign(1.0, pc) * aux;
-                                }
-                            }
-                        }
+    int is = i0, js = j0, ks = k0;
+    bool box_left = true;
+    long m = -1;
+    int q = 0, nk = 0, tri = 0, nu = 0;
+    long rec = -1;
+    bool done = false;
+    while (!done) {
+        bool need = false;
+        for (;;) {
+            if (rec >= 0) {
+                if (tri < nu) {
+                    const double *bd = a.bnd + rec * 10 + 5 * tri;
+                    if (beyond(bd[0], bd[1], bd[2], bd[3], bd[4], P, cmax, D)) { ++tri; continue; }
+                    need = true;
+                    break;
+                }
+                // end of a K-tetrahedron (GetIsosurfaceAndDistances.F90:221-229,
+                // NeighboursPhiCorrection.F90:203-215)
+                if (iface) SD = sd_update(SD, D, sgn0);
+                else pc = pc_update(pc, D);
+                rec = -1;
             }
+            if (q < nk) {
+                rec = a.mrec[m * 24 + q];
+                ++q;
+                tri = 0;
+                nu = a.nuse[rec];
+                continue;
+            }
+            if (!box_left) { done = true; break; }
+            const long ls = (long)is + g.sj * js + g.sk * ks;
+            if (++is > i1) {
+                is = i0;
+                if (++js > j1) {
+                    js = j0;
+                    if (++ks > k1) box_left = false;
+                }
+            }
+            if (a.nid[ls] != 1) continue;
+            const long mm = a.mid[ls];
+            const int cnt = a.nkt[mm];
+            if (!iface) D = 10.0;
+            // a neighbour's updates with D >= 10 leave |phicorrected| <= 10
+            // unchanged; D dips below a distance by at most a rounding of 10
+            const double R = a.mrad[mm];
+            bool skip;
+            if (iface)
+                skip = R < 0.0 || beyond(g.x[ls], g.y[ls], g.z[ls], R, a.memin[mm], P, cmax, D);
+            else
+                skip = fabs(pc) <= 10.0 &&
+                       (R < 0.0 || beyond(g.x[ls], g.y[ls], g.z[ls], R, a.memin[mm], P, cmax, fabs(pc) + 1e-14));
+            if (skip) {
+                // D stays; SD takes its per-tetrahedron update cnt times
+                if (iface)
+                    for (int c = 0; c < cnt; ++c) {
+                        const double s2 = sd_update(SD, D, sgn0);
+                        if (s2 == SD) break;
+                        SD = s2;
+                    }
+                continue;
+            }
+            m = mm;
+            q = 0;
+            nk = cnt;
+        }
+        if (need) {
+            const double Da = dist_point_triangle(a.tris + rec * 18 + 9 * tri, P);
+            const double ex = (copysign(1.0, D - Da) + 1.0) / 2.0;
+            D = D + ex * (Da - D);
+            ++tri;
+        }
+    }
     // NeighboursPhiCorrection.F90:89-130 with C = 0: phi = SD + 0
     a.pc[l] = iface ? SD + 0.0 : pc;
 }
@@ -429,7 +599,10 @@ __global__ void geo_ids(long n, const int8_t *__restrict__ nid, const uint8_t *_
     ids[l] = nid[l] == 1 ? 1 : (nmark[l] ? -1 : 0);
 }
 
-__global__ void geo_zero(int32_t *c) { *c = 0; }
+__global__ void geo_zero(int32_t *c)
+{
+    if (threadIdx.x < 4) c[threadIdx.x] = 0;
+}
 
 __global__ void hybrid_init(const double *__restrict__ phi, const double *__restrict__ base,
                             double *__restrict__ pc, long n)
@@ -477,8 +650,9 @@ unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 size_t geometric_work(const Blk &b)
 {
     const size_t nc = (size_t)(b.e.ni - 1) * (b.e.nj - 1) * (b.e.nk - 1);
+    const size_t n = (size_t)b.n;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    return al(b.n) + al(b.n) + al(nc) + al(nc) + al(nc * 4) + 256 + 1024;
+    return al(n) + al(n) + al(nc) + al(nc) + al(nc * 4) + al(n * 4) * 3 + 256 + 2048;
 }
 
 int launch_hybrid_init(const double *phi, const double *base, double *pc, long n, hipStream_t s)
@@ -496,38 +670,58 @@ int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids,
     uint8_t *nmark = (uint8_t *)c.take(n);
     uint8_t *ctets = (uint8_t *)c.take(ncell), *cpn = (uint8_t *)c.take(ncell);
     int32_t *cslot = (int32_t *)c.take(ncell * 4);
+    int32_t *mid = (int32_t *)c.take(n * 4), *ilist = (int32_t *)c.take(n * 4), *nlist = (int32_t *)c.take(n * 4);
     int32_t *count = (int32_t *)c.take(256);
-    if (!nid || !nmark || !ctets || !cpn || !cslot || !count) return fail(LSM_ERR_WORKSPACE, "workspace too small");
+    if (!nid || !nmark || !ctets || !cpn || !cslot || !mid || !ilist || !nlist || !count)
+        return fail(LSM_ERR_WORKSPACE, "workspace too small");
+    if (n > 0x7fffffffL) return fail(LSM_ERR_ARG, "block of %ld nodes: at most 2^31 - 1", n);
     Geom g{b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, (long)(b.e.ni - 1), (long)(b.e.ni - 1) * (b.e.nj - 1),
            b.x, b.y, b.z, phi0};
     int rc;
-    geo_zero<<<1, 1, 0, s>>>(count);
+    // counts: cells with K-tetrahedra, interface nodes, neighbours
+    geo_zero<<<1, 64, 0, s>>>(count);
     if ((rc = check_hip(hipGetLastError(), "geo_zero"))) return rc;
     geo_cells<<<nblk(ncell), 256, 0, s>>>(g, ncell, ctets, cpn, cslot, count);
     if ((rc = check_hip(hipGetLastError(), "geo_cells"))) return rc;
-    // the number of cells with K-tetrahedra sizes the isosurface store
-    int32_t nslots = 0;
-    if ((rc = check_hip(hipMemcpyAsync(&nslots, count, sizeof(int32_t), hipMemcpyDeviceToHost, s), "D2H")) ||
+    geo_nodes<<<nblk(n), 256, 0, s>>>(g, n, cpn, nid, mid, ilist, count + 1);
+    if ((rc = check_hip(hipGetLastError(), "geo_nodes"))) return rc;
+    geo_mark<<<nblk(n), 256, 0, s>>>(g, n, nid, nmark, nlist, count + 2);
+    if ((rc = check_hip(hipGetLastError(), "geo_mark"))) return rc;
+    // the counts size the isosurface store and the distance launch
+    int32_t cnt[3] = {0, 0, 0};
+    if ((rc = check_hip(hipMemcpyAsync(cnt, count, sizeof(cnt), hipMemcpyDeviceToHost, s), "D2H")) ||
         (rc = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize")))
         return rc;
-    const size_t nrec = (size_t)nslots * 6;
-    const size_t tri_off = (nrec + 255) & ~(size_t)255;
-    void *store = nullptr;
-    if ((rc = store_take(tri_off + nrec * 18 * sizeof(double) + 256, &store))) return rc;
-    int8_t *nuse = (int8_t *)store;
-    double *tris = (double *)((char *)store + tri_off);
-    geo_iso<<<nblk(ncell), 256, 0, s>>>(g, ncell, ctets, cslot, nuse, tris);
-    if ((rc = check_hip(hipGetLastError(), "geo_iso"))) return rc;
-    geo_nodes<<<nblk(n), 256, 0, s>>>(g, n, cpn, nid);
-    if ((rc = check_hip(hipGetLastError(), "geo_nodes"))) return rc;
-    geo_mark<<<nblk(n), 256, 0, s>>>(g, n, nid, nmark);
-    if ((rc = check_hip(hipGetLastError(), "geo_mark"))) return rc;
-    DistArgs a{g, n, nid, nmark, ctets, cslot, nuse, tris, pc};
-    {
-        KTimer kt(LSM_KT_GEOMETRIC, s);
-        geo_distance<<<nblk(n), 256, 0, s>>>(a);
+    const long nslots = cnt[0], nint = cnt[1], nnb = cnt[2];
+    if (nint > 0) {
+        const size_t nrec = (size_t)nslots * 6;
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        const size_t o_tris = al(nrec);
+        const size_t o_bnd = o_tris + al(nrec * 18 * sizeof(double));
+        const size_t o_mrec = o_bnd + al(nrec * 10 * sizeof(double));
+        const size_t o_nkt = o_mrec + al((size_t)nint * 24 * sizeof(int32_t));
+        const size_t o_rad = o_nkt + al((size_t)nint);
+        const size_t o_emin = o_rad + al((size_t)nint * sizeof(double));
+        const size_t total = o_emin + al((size_t)nint * sizeof(double));
+        void *store = nullptr;
+        if ((rc = store_take(total, &store))) return rc;
+        char *sb = (char *)store;
+        int8_t *nuse = (int8_t *)sb;
+        double *tris = (double *)(sb + o_tris), *bnd = (double *)(sb + o_bnd);
+        int32_t *mrec = (int32_t *)(sb + o_mrec);
+        int8_t *nkt = (int8_t *)(sb + o_nkt);
+        double *mrad = (double *)(sb + o_rad), *memin = (double *)(sb + o_emin);
+        geo_iso<<<nblk(ncell), 256, 0, s>>>(g, ncell, ctets, cslot, nuse, tris, bnd);
+        if ((rc = check_hip(hipGetLastError(), "geo_iso"))) return rc;
+        geo_iface<<<nblk(nint), 256, 0, s>>>(g, nint, ilist, ctets, cslot, nuse, bnd, mrec, nkt, mrad, memin);
+        if ((rc = check_hip(hipGetLastError(), "geo_iface"))) return rc;
+        DistArgs a{g, nint, nnb, ilist, nlist, mid, nid, mrec, nkt, mrad, memin, nuse, tris, bnd, pc};
+        {
+            KTimer kt(LSM_KT_GEOMETRIC, s);
+            geo_distance<<<nblk(nint + nnb), 256, 0, s>>>(a);
+        }
+        if ((rc = check_hip(hipGetLastError(), "geo_distance"))) return rc;
     }
-    if ((rc = check_hip(hipGetLastError(), "geo_distance"))) return rc;
     if (ids) {
         geo_ids<<<nblk(n), 256, 0, s>>>(n, nid, nmark, ids);
         if ((rc = check_hip(hipGetLastError(), "geo_ids"))) return rc;
