@@ -93,36 +93,52 @@ __device__ __forceinline__ double nrm(const V3 &a) { return fnorm2(a.x, a.y, a.z
 // MINVAL order: a later candidate replaces the minimum only when smaller
 __device__ __forceinline__ double fmin_keep(double r, double x) { return x < r ? x : r; }
 
-// DistancePointLine, TetrahedronMethods.F90:1356-1399
-__device__ double dist_point_line(const V3 &A, const V3 &B, const V3 &P)
+// Per-triangle record of the distance search (kTS doubles): the vertices,
+// what DistancePointTriangle recomputes for every point although it depends on
+// the triangle only (the edge norms, the normal and its square), and the
+// bounds that let the search skip the triangle.
+constexpr int kTS = 25;
+enum {
+    kA = 0, kB = 3, kC = 6,
+    kN = 9, kNN = 12,
+    kNAB = 13, kNBC = 14, kNCA = 15,   // NORM2(B-A), NORM2(C-B), NORM2(A-C)
+    kInvN = 16,                        // 1/|N| of a well-shaped triangle, else 0
+    kInvAB = 17, kInvBC = 18, kInvCA = 19,   // 1/|edge|^2
+    kCx = 20, kCy = 21, kCz = 22, kR = 23, kEmin = 24   // sphere, shortest edge
+};
+
+// one edge of DistancePointLine (TetrahedronMethods.F90:1356-1399) folded
+// into the running minimum: the perpendicular distance when the foot lies on
+// the segment; otherwise the distance to the nearer end point, which is one
+// of the vertex distances already in the minimum
+__device__ __forceinline__ double edge_min(const V3 &X, const V3 &Y, const V3 &XP, const V3 &YP, double nXY,
+                                           double r)
 {
-    const V3 AB = sub(B, A), AP = sub(P, A), BA = sub(A, B), BP = sub(P, B);
-    bool valid = true;
-    if (nrm(AB) < kEpsSims) valid = false;
-    if (dot(AB, AP) < 0.0) valid = false;
-    if (dot(BA, BP) < 0.0) valid = false;
-    if (valid) return nrm(cross(AP, BP)) / nrm(AB);
-    const double da = nrm(AP), db = nrm(BP);
-    return db < da ? db : da;
+    if (nXY < kEpsSims) return r;
+    if (dot(sub(Y, X), XP) < 0.0 || dot(sub(X, Y), YP) < 0.0) return r;
+    return fmin_keep(r, nrm(cross(XP, YP)) / nXY);
 }
 
 // DistancePointTriangle with CheckProjectionPointOnATriangle3,
 // TetrahedronMethods.F90:1008-1070,1593-1684 (the 'projection' weights P1,
-// P2, P3 with alpha, beta, gamma as the reference forms them)
-__device__ double dist_point_triangle(const double *T, const V3 &P)
+// P2, P3 with alpha, beta, gamma as the reference forms them): the minimum
+// of the same values, so the same result bit for bit
+__device__ double tri_distance(const double *T, const V3 &P)
 {
-    const V3 A{T[0], T[1], T[2]}, B{T[3], T[4], T[5]}, C{T[6], T[7], T[8]};
-    double r = nrm(sub(P, A));
-    r = fmin_keep(r, nrm(sub(P, B)));
-    r = fmin_keep(r, nrm(sub(P, C)));
-    r = fmin_keep(r, dist_point_line(A, B, P));
-    r = fmin_keep(r, dist_point_line(B, C, P));
-    r = fmin_keep(r, dist_point_line(C, A, P));
-    if (nrm(sub(A, B)) <= kEpsSims || nrm(sub(A, C)) <= kEpsSims || nrm(sub(B, C)) <= kEpsSims) return r;
-    const V3 u = sub(B, A), v = sub(C, A), w = sub(P, A);
-    const V3 N = cross(u, v);
-    const double alpha = dot(cross(u, w), N) / dot(N, N);
-    const double beta = dot(cross(w, v), N) / dot(N, N);
+    const V3 A{T[kA], T[kA + 1], T[kA + 2]}, B{T[kB], T[kB + 1], T[kB + 2]}, C{T[kC], T[kC + 1], T[kC + 2]};
+    const V3 AP = sub(P, A), BP = sub(P, B), CP = sub(P, C);
+    double r = nrm(AP);
+    r = fmin_keep(r, nrm(BP));
+    r = fmin_keep(r, nrm(CP));
+    r = edge_min(A, B, AP, BP, T[kNAB], r);
+    r = edge_min(B, C, BP, CP, T[kNBC], r);
+    r = edge_min(C, A, CP, AP, T[kNCA], r);
+    if (T[kNAB] <= kEpsSims || T[kNCA] <= kEpsSims || T[kNBC] <= kEpsSims) return r;
+    const V3 N{T[kN], T[kN + 1], T[kN + 2]};
+    const double NN = T[kNN];
+    const V3 u = sub(B, A), v = sub(C, A);
+    const double alpha = dot(cross(u, AP), N) / NN;
+    const double beta = dot(cross(AP, v), N) / NN;
     const double gamma = (1.0 - beta) - alpha;
     const V3 pp{(alpha * A.x + beta * B.x) + gamma * C.x, (alpha * A.y + beta * B.y) + gamma * C.y,
                 (alpha * A.z + beta * B.z) + gamma * C.z};
@@ -207,7 +223,7 @@ __global__ __launch_bounds__(256) void geo_cells(Geom g, long ncell, uint8_t *__
     const int tm = cell_ktets(ph, pn);
     ctets[c] = (uint8_t)tm;
     cpn[c] = (uint8_t)pn;
-    cslot[c] = tm ? atomicAdd(count, 1) : -1;
+    cslot[c] = tm ? atomicAdd(count, __popc(tm)) : -1;   // first K-tetrahedron record
 }
 
 // marching-tetrahedra isosurface (TetrahedronMethods.F90:26-664): the number
@@ -258,30 +274,45 @@ __device__ int tet_isosurface(const double v[4][3], const double ph[4], double *
     return area > 0.0 ? ntri : 0;
 }
 
-// bounding sphere (centroid, radius rounded up) and shortest edge (rounded
-// down) of a triangle: the distance bound that lets the search skip it
-__device__ void tri_bounds(const double *T, double *bd)
+// the search record of a triangle (layout kTS above)
+__device__ void tri_record(const double *tri, double *T)
 {
-    const double cx = (T[0] + T[3] + T[6]) / 3.0, cy = (T[1] + T[4] + T[7]) / 3.0, cz = (T[2] + T[5] + T[8]) / 3.0;
+    for (int q = 0; q < 9; ++q) T[q] = tri[q];
+    const V3 A{tri[0], tri[1], tri[2]}, B{tri[3], tri[4], tri[5]}, C{tri[6], tri[7], tri[8]};
+    const V3 u = sub(B, A), v = sub(C, A);
+    const V3 N = cross(u, v);
+    T[kN] = N.x; T[kN + 1] = N.y; T[kN + 2] = N.z;
+    T[kNN] = dot(N, N);
+    T[kNAB] = nrm(sub(B, A));
+    T[kNBC] = nrm(sub(C, B));
+    T[kNCA] = nrm(sub(A, C));
+    // bounds: plain Euclidean arithmetic, rounded outward
+    auto len2 = [](const V3 &a) { return a.x * a.x + a.y * a.y + a.z * a.z; };
+    const V3 w = sub(C, B), e3 = sub(A, C);
+    const double lu = sqrt(len2(u)), lv = sqrt(len2(v)), lw = sqrt(len2(w));
+    const double nN = sqrt(len2(N));
+    T[kInvN] = (nN > 0.0 && nN >= 1e-2 * lu * lv) ? 1.0 / nN : 0.0;
+    T[kInvAB] = 1.0 / len2(u);
+    T[kInvBC] = 1.0 / len2(w);
+    T[kInvCA] = 1.0 / len2(e3);
+    const double cx = (A.x + B.x + C.x) / 3.0, cy = (A.y + B.y + C.y) / 3.0, cz = (A.z + B.z + C.z) / 3.0;
     double r = 0.0;
-    for (int q = 0; q < 3; ++q) {
-        const double dx = T[3 * q] - cx, dy = T[3 * q + 1] - cy, dz = T[3 * q + 2] - cz;
-        r = fmax(r, sqrt(dx * dx + dy * dy + dz * dz));
-    }
-    double emin = 1e300;
-    for (int q = 0; q < 3; ++q) {
-        const int p2 = (q + 1) % 3;
-        const double dx = T[3 * q] - T[3 * p2], dy = T[3 * q + 1] - T[3 * p2 + 1], dz = T[3 * q + 2] - T[3 * p2 + 2];
-        emin = fmin(emin, sqrt(dx * dx + dy * dy + dz * dz));
-    }
-    bd[0] = cx; bd[1] = cy; bd[2] = cz;
-    bd[3] = r * (1.0 + 1e-12);
-    bd[4] = emin * (1.0 - 1e-12);
+    r = fmax(r, sqrt(len2(V3{A.x - cx, A.y - cy, A.z - cz})));
+    r = fmax(r, sqrt(len2(V3{B.x - cx, B.y - cy, B.z - cz})));
+    r = fmax(r, sqrt(len2(V3{C.x - cx, C.y - cy, C.z - cz})));
+    T[kCx] = cx; T[kCy] = cy; T[kCz] = cz;
+    T[kR] = r * (1.0 + 1e-12);
+    T[kEmin] = fmin(lu, fmin(lv, lw)) * (1.0 - 1e-12);
+}
+
+__device__ __forceinline__ long ktet_rec(const int32_t *cslot, long c, int tm, int t)
+{
+    return (long)cslot[c] + __popc(tm & ((1 << t) - 1));
 }
 
 __global__ __launch_bounds__(256) void geo_iso(Geom g, long ncell, const uint8_t *__restrict__ ctets,
                                                const int32_t *__restrict__ cslot, int8_t *__restrict__ nuse,
-                                               double *__restrict__ tris, double *__restrict__ bnd)
+                                               double *__restrict__ trec)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
@@ -289,7 +320,6 @@ __global__ __launch_bounds__(256) void geo_iso(Geom g, long ncell, const uint8_t
     if (!tm) return;
     int i, j, k;
     cell_of(g, c, i, j, k);
-    const long slot = (long)cslot[c] * 6;
     for (int t = 0; t < 6; ++t) {
         if (!(tm & (1 << t))) continue;
         double v[4][3], ph[4];
@@ -301,10 +331,9 @@ __global__ __launch_bounds__(256) void geo_iso(Geom g, long ncell, const uint8_t
         }
         double tri[18];
         const int nu = tet_isosurface(v, ph, tri);
-        const long rec = slot + t;
+        const long rec = ktet_rec(cslot, c, tm, t);
         nuse[rec] = (int8_t)nu;
-        for (int q = 0; q < 9 * nu; ++q) tris[rec * 18 + q] = tri[q];
-        for (int u = 0; u < nu; ++u) tri_bounds(tri + 9 * u, bnd + rec * 10 + 5 * u);
+        for (int u = 0; u < nu; ++u) tri_record(tri + 9 * u, trec + (rec * 2 + u) * kTS);
     }
 }
 
@@ -398,7 +427,7 @@ __global__ __launch_bounds__(256) void geo_mark(Geom g, long n, const int8_t *__
 // edge down)
 __global__ __launch_bounds__(256) void geo_iface(Geom g, long nint, const int32_t *__restrict__ ilist,
                                                  const uint8_t *__restrict__ ctets, const int32_t *__restrict__ cslot,
-                                                 const int8_t *__restrict__ nuse, const double *__restrict__ bnd,
+                                                 const int8_t *__restrict__ nuse, const double *__restrict__ trec,
                                                  int32_t *__restrict__ mrec, int8_t *__restrict__ nkt,
                                                  double *__restrict__ mrad, double *__restrict__ memin)
 {
@@ -421,18 +450,17 @@ __global__ __launch_bounds__(256) void geo_iface(Geom g, long nint, const int32_
                 if (!tm) continue;
                 const int ox = -di, oy = -dj, oz = -dk;
                 const int vq = oz * 4 + (oy ? (ox ? 2 : 3) : ox);
-                const long slot = (long)cslot[c] * 6;
                 for (int t = 0; t < 6; ++t) {
                     if (!((tm >> t) & 1) || !((kTetHas[t] >> vq) & 1)) continue;
-                    const long rec = slot + t;
+                    const long rec = ktet_rec(cslot, c, tm, t);
                     mrec[m * 24 + q] = (int32_t)rec;
                     ++q;
                     const int nu = nuse[rec];
                     for (int u = 0; u < nu; ++u) {
-                        const double *bd = bnd + rec * 10 + 5 * u;
-                        const double dx = bd[0] - X, dy = bd[1] - Y, dz = bd[2] - Z;
-                        R = fmax(R, sqrt(dx * dx + dy * dy + dz * dz) + bd[3]);
-                        em = fmin(em, bd[4]);
+                        const double *T = trec + (rec * 2 + u) * kTS;
+                        const double dx = T[kCx] - X, dy = T[kCy] - Y, dz = T[kCz] - Z;
+                        R = fmax(R, sqrt(dx * dx + dy * dy + dz * dz) + T[kR]);
+                        em = fmin(em, T[kEmin]);
                         any = true;
                     }
                 }
@@ -451,26 +479,79 @@ struct DistArgs {
     const int8_t *nkt;
     const double *mrad, *memin;
     const int8_t *nuse;
-    const double *tris, *bnd;
+    const double *trec;
     double *pc;
 };
 
-// True when every distance the reference's DistancePointTriangle can return
-// for points of a sphere (centre c, radius r, shortest triangle edge emin)
-// exceeds cmp: |P - c| - r bounds the exact distance from below, and the
-// computed distance is within err of the exact one -- rounding of the
-// coordinates (cmax), of the norms and of the cross-product line distance,
-// whose error grows like |AP| |BP| / |AB| (L^2 / emin) -- with a margin of
-// more than three orders of magnitude.  A triangle with a vanishing edge
-// (emin = 0) is never skipped.
-__device__ __forceinline__ bool beyond(double cx, double cy, double cz, double r, double emin, const V3 &P,
-                                       double cmax, double cmp)
+// How far the reference's DistancePointTriangle may fall below the exact
+// distance for points within L of P (shortest edge emin, coordinates up to
+// cmax): rounding of the coordinates and of the norms, and the cross-product
+// line distance, whose error grows like |AP| |BP| / |AB| <= L^2 / emin --
+// with a margin of more than three orders of magnitude.  A vanishing edge
+// makes it infinite.
+__device__ __forceinline__ double dist_err(double L, double emin, double cmax)
 {
-    const double dx = P.x - cx, dy = P.y - cy, dz = P.z - cz;
+    return 1e-12 * (L + L * (L / emin) + cmax + L);
+}
+
+__device__ __forceinline__ double seg_dist(const V3 &X, const V3 &e, double inv_e2, const V3 &P)
+{
+    const V3 w = sub(P, X);
+    double t = (w.x * e.x + w.y * e.y + w.z * e.z) * inv_e2;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+    const double dx = w.x - t * e.x, dy = w.y - t * e.y, dz = w.z - t * e.z;
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+// A lower bound of the reference's distance from P to triangle T, error
+// allowance subtracted.  The sphere bound; for a well-shaped triangle also the
+// exact Euclidean distance: the distance to the triangle's plane when the foot
+// of the perpendicular may lie in the triangle (the plane distance never
+// exceeds the distance to the triangle, so an uncertain classification errs
+// low), else the least distance to its three edges.  -inf when the
+// allowance is not finite.
+__device__ __forceinline__ double tri_lower(const double *T, const V3 &P, double cmax)
+{
+    const double dx = P.x - T[kCx], dy = P.y - T[kCy], dz = P.z - T[kCz];
     const double dP = sqrt(dx * dx + dy * dy + dz * dz);
+    const double r = T[kR], emin = T[kEmin];
     const double L = dP + r;
-    const double err = 1e-12 * (L + L * (L / emin) + cmax + L);
-    return dP - r - err > cmp;
+    const double err = dist_err(L, emin, cmax);
+    double lb = dP - r;
+    if (!(err < 1e300)) return -1e300;
+    const double invN = T[kInvN];
+    if (invN != 0.0) {
+        const V3 A{T[kA], T[kA + 1], T[kA + 2]}, B{T[kB], T[kB + 1], T[kB + 2]}, C{T[kC], T[kC + 1], T[kC + 2]};
+        const V3 N{T[kN], T[kN + 1], T[kN + 2]};
+        const double NN = T[kNN];
+        const V3 u = sub(B, A), v = sub(C, A), w = sub(P, A);
+        const V3 cu = cross(u, w), cv = cross(w, v);
+        const double an = cu.x * N.x + cu.y * N.y + cu.z * N.z;
+        const double bn = cv.x * N.x + cv.y * N.y + cv.z * N.z;
+        const double tol = 1e-9 * NN * (1.0 + L / emin);
+        double t;
+        if (an >= -tol && bn >= -tol && an + bn <= NN + tol) {
+            t = fabs(N.x * w.x + N.y * w.y + N.z * w.z) * invN;
+        } else {
+            t = seg_dist(A, u, T[kInvAB], P);
+            t = fmin(t, seg_dist(B, sub(C, B), T[kInvBC], P));
+            t = fmin(t, seg_dist(C, sub(A, C), T[kInvCA], P));
+        }
+        lb = fmax(lb, t);
+    }
+    return lb - err;
+}
+
+// An upper bound of the reference's distance from P to triangle T: its
+// minimum includes the three vertex distances.
+__device__ __forceinline__ double tri_upper(const double *T, const V3 &P)
+{
+    double m = 1e300;
+    for (int q = 0; q < 3; ++q) {
+        const double dx = P.x - T[3 * q], dy = P.y - T[3 * q + 1], dz = P.z - T[3 * q + 2];
+        m = fmin(m, dx * dx + dy * dy + dz * dz);
+    }
+    return sqrt(m) * (1.0 + 1e-12) + 1e-300;
 }
 
 __device__ __forceinline__ double sd_update(double SD, double D, double sgn0)
@@ -486,23 +567,32 @@ __device__ __forceinline__ double pc_update(double pc, double D)
     return copysign(1.0, pc) * aux;
 }
 
-// One thread per listed node (interface nodes, then neighbours).  The node
-// walks the sequence of GetIsosurfaceAndDistances.F90:153-240 (interface
-// nodes: D and SD carried over the whole search box) or
-// NeighboursPhiCorrection.F90:133-249 (neighbours: D restarted per interface
-// node, phicorrected updated after every K-tetrahedron): the interface nodes
-// of its box in k, j, i order, their K-tetrahedra, their triangles.  A
-// triangle whose distance provably exceeds the running minimum D leaves D
-// unchanged and is not evaluated; an interface node none of whose triangles
-// can reach D (neighbours: |phicorrected|) leaves everything but the
-// repeated SD updates unchanged.  Every update that does happen sees the
-// values and the order of the reference, so the results are bit-identical.
-// The walk is a cursor: lanes advance through the cheap steps until each has
-// a triangle to evaluate, then evaluate together.
-__global__ __launch_bounds__(256) void geo_distance(DistArgs a)
+constexpr int kBoxMax = 729;   // 9 x 9 x 9 search box
+constexpr double kSlack = 2e-14;   // D and |phicorrected| exceed a bound by at most roundings of 10
+
+// One wavefront (one workgroup) per listed node: interface nodes first, then
+// neighbours.  The node's result follows the sequence of
+// GetIsosurfaceAndDistances.F90:153-240 (interface nodes: D and SD carried over
+// the whole search box) or NeighboursPhiCorrection.F90:133-249 (neighbours: D
+// restarted per interface node, phicorrected updated after every
+// K-tetrahedron): the interface nodes of the 9^3 box in k, j, i order, their
+// K-tetrahedra, their triangles.
+//  1. The lanes list the box's interface nodes in that order.
+//  2. Per listed node, in parallel: a lower bound of the distances to all its
+//     triangles and an upper bound of the least one; the prefix minimum of
+//     the upper bounds bounds the comparand (D, or |phicorrected|) every
+//     later node meets, so a node whose lower bound exceeds it cannot change
+//     D (or phicorrected) and needs no distance.
+//  3. In order, groups of remaining nodes: the lanes evaluate the groups'
+//     triangle distances together, then every lane runs the reference's
+//     updates over them in the reference's order (skipped nodes take their
+//     unchanged-D updates), so the results are bit-identical.
+__global__ __launch_bounds__(64) void geo_distance(DistArgs a)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nint + a.nnb) return;
+    __shared__ int32_t s_m[kBoxMax];
+    __shared__ uint8_t s_go[kBoxMax];
+    const int lane = threadIdx.x;
+    const long t = blockIdx.x;
     const Geom &g = a.g;
     const bool iface = t < a.nint;
     const long l = iface ? a.ilist[t] : a.nlist[t - a.nint];
@@ -512,62 +602,133 @@ __global__ __launch_bounds__(256) void geo_distance(DistArgs a)
     const double cmax = fmax(fabs(P.x), fmax(fabs(P.y), fabs(P.z)));
     int i0, i1, j0, j1, k0, k1;
     box(i, g.ni, i0, i1); box(j, g.nj, j0, j1); box(k, g.nk, k0, k1);
-    const double sgn0 = copysign(1.0, g.phi0[l]);
-    double D = 10.0, SD = 99.9;
-    double pc = iface ? 0.0 : a.pc[l];
-    int is = i0, js = j0, ks = k0;
-    bool box_left = true;
-    long m = -1;
-    int q = 0, nk = 0, tri = 0, nu = 0;
-    long rec = -1;
-    bool done = false;
-    while (!done) {
-        bool need = false;
-        for (;;) {
-            if (rec >= 0) {
-                if (tri < nu) {
-                    const double *bd = a.bnd + rec * 10 + 5 * tri;
-                    if (beyond(bd[0], bd[1], bd[2], bd[3], bd[4], P, cmax, D)) { ++tri; continue; }
-                    need = true;
-                    break;
-                }
-                // end of a K-tetrahedron (GetIsosurfaceAndDistances.F90:221-229,
-                // NeighboursPhiCorrection.F90:203-215)
-                if (iface) SD = sd_update(SD, D, sgn0);
-                else pc = pc_update(pc, D);
-                rec = -1;
+    const int bx = i1 - i0 + 1, by = j1 - j0 + 1, nbox = bx * by * (k1 - k0 + 1);
+    // 1. interface nodes of the box, in box order
+    int nm = 0;
+    for (int e0 = 0; e0 < nbox; e0 += 64) {
+        const int e = e0 + lane;
+        int mm = -1;
+        if (e < nbox) {
+            const int r = e / bx;
+            const long ls = (long)(i0 + e - r * bx) + g.sj * (j0 + r % by) + g.sk * (k0 + r / by);
+            if (a.nid[ls] == 1) mm = a.mid[ls];
+        }
+        const unsigned long long bal = __ballot(mm >= 0);
+        if (mm >= 0) s_m[nm + (int)__popcll(bal & ((1ull << lane) - 1ull))] = mm;
+        nm += (int)__popcll(bal);
+    }
+    __syncthreads();
+    const double pc0 = iface ? 0.0 : a.pc[l];
+    // a neighbour's updates with D >= 10 leave |phicorrected| <= 10 unchanged
+    const bool can = iface || fabs(pc0) <= 10.0;
+    // 2. which listed nodes need their distances
+    double carry = iface ? 10.0 : fabs(pc0);
+    for (int c0 = 0; c0 < nm; c0 += 64) {
+        const int e = c0 + lane;
+        long mm = -1;
+        double lb = 1e300, ub = 1e300;   // no usable triangle: out of reach, no bound
+        double Xm = 0.0, Ym = 0.0, Zm = 0.0, R = -1.0, em = 0.0;
+        if (e < nm) {
+            mm = s_m[e];
+            R = a.mrad[mm];
+            const long lm = a.ilist[mm];
+            Xm = g.x[lm]; Ym = g.y[lm]; Zm = g.z[lm];
+            em = a.memin[mm];
+            if (R >= 0.0) {
+                const double dx = P.x - Xm, dy = P.y - Ym, dz = P.z - Zm;
+                const double dP = sqrt(dx * dx + dy * dy + dz * dz);
+                const double err = dist_err(dP + R, em, cmax);
+                lb = err < 1e300 ? dP - R - err : -1e300;
+                ub = (dP + R) * (1.0 + 1e-12) + 1e-300;
             }
-            if (q < nk) {
-                rec = a.mrec[m * 24 + q];
-                ++q;
-                tri = 0;
-                nu = a.nuse[rec];
-                continue;
-            }
-            if (!box_left) { done = true; break; }
-            const long ls = (long)is + g.sj * js + g.sk * ks;
-            if (++is > i1) {
-                is = i0;
-                if (++js > j1) {
-                    js = j0;
-                    if (++ks > k1) box_left = false;
-                }
-            }
-            if (a.nid[ls] != 1) continue;
-            const long mm = a.mid[ls];
+        }
+        // exclusive prefix minimum of the upper bounds over the list
+        double incl = ub;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double o = __shfl_up(incl, d);
+            if (lane >= d) incl = fmin(incl, o);
+        }
+        double excl = __shfl_up(incl, 1);
+        if (lane == 0) excl = 1e300;
+        bool go = !(can && lb > fmin(carry, excl) + kSlack);
+        if (go && e < nm && R >= 0.0) {
+            // the sphere cannot tell: bound every triangle
             const int cnt = a.nkt[mm];
-            if (!iface) D = 10.0;
-            // a neighbour's updates with D >= 10 leave |phicorrected| <= 10
-            // unchanged; D dips below a distance by at most a rounding of 10
-            const double R = a.mrad[mm];
-            bool skip;
-            if (iface)
-                skip = R < 0.0 || beyond(g.x[ls], g.y[ls], g.z[ls], R, a.memin[mm], P, cmax, D);
-            else
-                skip = fabs(pc) <= 10.0 &&
-                       (R < 0.0 || beyond(g.x[ls], g.y[ls], g.z[ls], R, a.memin[mm], P, cmax, fabs(pc) + 1e-14));
-            if (skip) {
-                // D stays; SD takes its per-tetrahedron update cnt times
+            double lo = 1e300, hi = 1e300;
+            for (int q = 0; q < cnt; ++q) {
+                const long rr = a.mrec[mm * 24 + q];
+                const int nn = a.nuse[rr];
+                for (int u = 0; u < nn; ++u) {
+                    const double *T = a.trec + (rr * 2 + u) * kTS;
+                    lo = fmin(lo, tri_lower(T, P, cmax));
+                    hi = fmin(hi, tri_upper(T, P));
+                }
+            }
+            lb = lo;
+            ub = fmin(ub, hi);
+        }
+        incl = ub;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double o = __shfl_up(incl, d);
+            if (lane >= d) incl = fmin(incl, o);
+        }
+        excl = __shfl_up(incl, 1);
+        if (lane == 0) excl = 1e300;
+        go = !(can && lb > fmin(carry, excl) + kSlack);
+        if (e < nm) s_go[e] = go ? 1 : 0;
+        carry = fmin(carry, __shfl(incl, 63));
+    }
+    __syncthreads();
+    // 3. the reference's updates, in order
+    const double sgn0 = copysign(1.0, g.phi0[l]);
+    double D = 10.0, SD = 99.9, pc = pc0;
+    int e = 0;
+    while (e < nm) {
+        if (!s_go[e]) {
+            // D stays; SD takes its per-tetrahedron update cnt times
+            if (iface) {
+                const int cnt = a.nkt[s_m[e]];
+                for (int c = 0; c < cnt; ++c) {
+                    const double s2 = sd_update(SD, D, sgn0);
+                    if (s2 == SD) break;
+                    SD = s2;
+                }
+            }
+            ++e;
+            continue;
+        }
+        // a group of listed nodes whose triangles fit the 64 lanes (two per K-tetrahedron)
+        int gend = e, slots = 0;
+        while (gend < nm) {
+            if (s_go[gend]) {
+                const int need = 2 * a.nkt[s_m[gend]];
+                if (slots + need > 64) break;
+                slots += need;
+            }
+            ++gend;
+        }
+        double myDa = 0.0;
+        {
+            int off = 0;
+            long myrec = -1;
+            int myu = 0;
+            for (int f = e; f < gend; ++f) {
+                if (!s_go[f]) continue;
+                const long mm = s_m[f];
+                const int cnt = a.nkt[mm];
+                if (lane >= off && lane < off + 2 * cnt) {
+                    myrec = a.mrec[mm * 24 + ((lane - off) >> 1)];
+                    myu = (lane - off) & 1;
+                }
+                off += 2 * cnt;
+            }
+            if (myrec >= 0 && myu < a.nuse[myrec]) myDa = tri_distance(a.trec + (myrec * 2 + myu) * kTS, P);
+        }
+        int off = 0;
+        for (int f = e; f < gend; ++f) {
+            const long mm = s_m[f];
+            const int cnt = a.nkt[mm];
+            if (!s_go[f]) {
                 if (iface)
                     for (int c = 0; c < cnt; ++c) {
                         const double s2 = sd_update(SD, D, sgn0);
@@ -576,19 +737,25 @@ __global__ __launch_bounds__(256) void geo_distance(DistArgs a)
                     }
                 continue;
             }
-            m = mm;
-            q = 0;
-            nk = cnt;
+            if (!iface) D = 10.0;
+            for (int q = 0; q < cnt; ++q) {
+                const int nn = a.nuse[a.mrec[mm * 24 + q]];
+                for (int u = 0; u < nn; ++u) {
+                    const double Da = __shfl(myDa, off + 2 * q + u);
+                    const double ex = (copysign(1.0, D - Da) + 1.0) / 2.0;
+                    D = D + ex * (Da - D);
+                }
+                // end of a K-tetrahedron (GetIsosurfaceAndDistances.F90:221-229,
+                // NeighboursPhiCorrection.F90:203-215)
+                if (iface) SD = sd_update(SD, D, sgn0);
+                else pc = pc_update(pc, D);
+            }
+            off += 2 * cnt;
         }
-        if (need) {
-            const double Da = dist_point_triangle(a.tris + rec * 18 + 9 * tri, P);
-            const double ex = (copysign(1.0, D - Da) + 1.0) / 2.0;
-            D = D + ex * (Da - D);
-            ++tri;
-        }
+        e = gend;
     }
     // NeighboursPhiCorrection.F90:89-130 with C = 0: phi = SD + 0
-    a.pc[l] = iface ? SD + 0.0 : pc;
+    if (lane == 0) a.pc[l] = iface ? SD + 0.0 : pc;
 }
 
 __global__ void geo_ids(long n, const int8_t *__restrict__ nid, const uint8_t *__restrict__ nmark,
@@ -694,11 +861,10 @@ int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids,
         return rc;
     const long nslots = cnt[0], nint = cnt[1], nnb = cnt[2];
     if (nint > 0) {
-        const size_t nrec = (size_t)nslots * 6;
+        const size_t nrec = (size_t)nslots;   // K-tetrahedra
         auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-        const size_t o_tris = al(nrec);
-        const size_t o_bnd = o_tris + al(nrec * 18 * sizeof(double));
-        const size_t o_mrec = o_bnd + al(nrec * 10 * sizeof(double));
+        const size_t o_trec = al(nrec);
+        const size_t o_mrec = o_trec + al(nrec * 2 * kTS * sizeof(double));
         const size_t o_nkt = o_mrec + al((size_t)nint * 24 * sizeof(int32_t));
         const size_t o_rad = o_nkt + al((size_t)nint);
         const size_t o_emin = o_rad + al((size_t)nint * sizeof(double));
@@ -707,18 +873,18 @@ int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids,
         if ((rc = store_take(total, &store))) return rc;
         char *sb = (char *)store;
         int8_t *nuse = (int8_t *)sb;
-        double *tris = (double *)(sb + o_tris), *bnd = (double *)(sb + o_bnd);
+        double *trec = (double *)(sb + o_trec);
         int32_t *mrec = (int32_t *)(sb + o_mrec);
         int8_t *nkt = (int8_t *)(sb + o_nkt);
         double *mrad = (double *)(sb + o_rad), *memin = (double *)(sb + o_emin);
-        geo_iso<<<nblk(ncell), 256, 0, s>>>(g, ncell, ctets, cslot, nuse, tris, bnd);
+        geo_iso<<<nblk(ncell), 256, 0, s>>>(g, ncell, ctets, cslot, nuse, trec);
         if ((rc = check_hip(hipGetLastError(), "geo_iso"))) return rc;
-        geo_iface<<<nblk(nint), 256, 0, s>>>(g, nint, ilist, ctets, cslot, nuse, bnd, mrec, nkt, mrad, memin);
+        geo_iface<<<nblk(nint), 256, 0, s>>>(g, nint, ilist, ctets, cslot, nuse, trec, mrec, nkt, mrad, memin);
         if ((rc = check_hip(hipGetLastError(), "geo_iface"))) return rc;
-        DistArgs a{g, nint, nnb, ilist, nlist, mid, nid, mrec, nkt, mrad, memin, nuse, tris, bnd, pc};
+        DistArgs a{g, nint, nnb, ilist, nlist, mid, nid, mrec, nkt, mrad, memin, nuse, trec, pc};
         {
             KTimer kt(LSM_KT_GEOMETRIC, s);
-            geo_distance<<<nblk(nint + nnb), 256, 0, s>>>(a);
+            geo_distance<<<(unsigned)(nint + nnb), 64, 0, s>>>(a);
         }
         if ((rc = check_hip(hipGetLastError(), "geo_distance"))) return rc;
     }
