@@ -589,8 +589,9 @@ constexpr double kSlack = 2e-14;   // D and |phicorrected| exceed a bound by at 
 //     unchanged-D updates), so the results are bit-identical.
 __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
 {
-    __shared__ int32_t s_m[kBoxMax];
-    __shared__ uint8_t s_go[kBoxMax];
+    __shared__ int32_t s_m[kBoxMax];     // listed interface nodes
+    __shared__ uint8_t s_cnt[kBoxMax];   // their K-tetrahedra (bits 0-4), distances needed (bit 7)
+    __shared__ uint64_t s_pat[kBoxMax];  // usable triangles of each K-tetrahedron, 2 bits each
     const int lane = threadIdx.x;
     const long t = blockIdx.x;
     const Geom &g = a.g;
@@ -603,18 +604,29 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
     int i0, i1, j0, j1, k0, k1;
     box(i, g.ni, i0, i1); box(j, g.nj, j0, j1); box(k, g.nk, k0, k1);
     const int bx = i1 - i0 + 1, by = j1 - j0 + 1, nbox = bx * by * (k1 - k0 + 1);
-    // 1. interface nodes of the box, in box order
-    int nm = 0;
-    for (int e0 = 0; e0 < nbox; e0 += 64) {
-        const int e = e0 + lane;
-        int mm = -1;
+    // 1. interface nodes of the box, in box order (all flags loaded first)
+    constexpr int kChunks = (kBoxMax + 63) / 64;
+    int8_t f[kChunks];
+    long ls[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        const int e = c * 64 + lane;
+        f[c] = 0;
+        ls[c] = 0;
         if (e < nbox) {
             const int r = e / bx;
-            const long ls = (long)(i0 + e - r * bx) + g.sj * (j0 + r % by) + g.sk * (k0 + r / by);
-            if (a.nid[ls] == 1) mm = a.mid[ls];
+            ls[c] = (long)(i0 + e - r * bx) + g.sj * (j0 + r % by) + g.sk * (k0 + r / by);
+            f[c] = a.nid[ls[c]];
         }
-        const unsigned long long bal = __ballot(mm >= 0);
-        if (mm >= 0) s_m[nm + (int)__popcll(bal & ((1ull << lane) - 1ull))] = mm;
+    }
+    int mmv[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) mmv[c] = f[c] == 1 ? a.mid[ls[c]] : -1;
+    int nm = 0;
+#pragma unroll
+    for (int c = 0; c < kChunks; ++c) {
+        const unsigned long long bal = __ballot(mmv[c] >= 0);
+        if (mmv[c] >= 0) s_m[nm + (int)__popcll(bal & ((1ull << lane) - 1ull))] = mmv[c];
         nm += (int)__popcll(bal);
     }
     __syncthreads();
@@ -627,15 +639,16 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
         const int e = c0 + lane;
         long mm = -1;
         double lb = 1e300, ub = 1e300;   // no usable triangle: out of reach, no bound
-        double Xm = 0.0, Ym = 0.0, Zm = 0.0, R = -1.0, em = 0.0;
+        double R = -1.0;
+        int cnt = 0;
         if (e < nm) {
             mm = s_m[e];
-            R = a.mrad[mm];
             const long lm = a.ilist[mm];
-            Xm = g.x[lm]; Ym = g.y[lm]; Zm = g.z[lm];
-            em = a.memin[mm];
+            R = a.mrad[mm];
+            const double em = a.memin[mm];
+            cnt = a.nkt[mm];
+            const double dx = P.x - g.x[lm], dy = P.y - g.y[lm], dz = P.z - g.z[lm];
             if (R >= 0.0) {
-                const double dx = P.x - Xm, dy = P.y - Ym, dz = P.z - Zm;
                 const double dP = sqrt(dx * dx + dy * dy + dz * dz);
                 const double err = dist_err(dP + R, em, cmax);
                 lb = err < 1e300 ? dP - R - err : -1e300;
@@ -651,21 +664,24 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
         double excl = __shfl_up(incl, 1);
         if (lane == 0) excl = 1e300;
         bool go = !(can && lb > fmin(carry, excl) + kSlack);
-        if (go && e < nm && R >= 0.0) {
+        uint64_t pat = 0;
+        if (go && e < nm) {
             // the sphere cannot tell: bound every triangle
-            const int cnt = a.nkt[mm];
             double lo = 1e300, hi = 1e300;
             for (int q = 0; q < cnt; ++q) {
                 const long rr = a.mrec[mm * 24 + q];
                 const int nn = a.nuse[rr];
+                pat |= (uint64_t)nn << (2 * q);
                 for (int u = 0; u < nn; ++u) {
                     const double *T = a.trec + (rr * 2 + u) * kTS;
                     lo = fmin(lo, tri_lower(T, P, cmax));
                     hi = fmin(hi, tri_upper(T, P));
                 }
             }
-            lb = lo;
-            ub = fmin(ub, hi);
+            if (R >= 0.0) {
+                lb = lo;
+                ub = fmin(ub, hi);
+            }
         }
         incl = ub;
         for (int d = 1; d < 64; d <<= 1) {
@@ -675,7 +691,10 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
         excl = __shfl_up(incl, 1);
         if (lane == 0) excl = 1e300;
         go = !(can && lb > fmin(carry, excl) + kSlack);
-        if (e < nm) s_go[e] = go ? 1 : 0;
+        if (e < nm) {
+            s_cnt[e] = (uint8_t)(cnt | (go ? 0x80 : 0));
+            s_pat[e] = pat;
+        }
         carry = fmin(carry, __shfl(incl, 63));
     }
     __syncthreads();
@@ -684,51 +703,44 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
     double D = 10.0, SD = 99.9, pc = pc0;
     int e = 0;
     while (e < nm) {
-        if (!s_go[e]) {
+        const int ce = s_cnt[e];
+        if (!(ce & 0x80)) {
             // D stays; SD takes its per-tetrahedron update cnt times
-            if (iface) {
-                const int cnt = a.nkt[s_m[e]];
-                for (int c = 0; c < cnt; ++c) {
+            if (iface)
+                for (int c = 0; c < (ce & 0x7f); ++c) {
                     const double s2 = sd_update(SD, D, sgn0);
                     if (s2 == SD) break;
                     SD = s2;
                 }
-            }
             ++e;
             continue;
         }
-        // a group of listed nodes whose triangles fit the 64 lanes (two per K-tetrahedron)
+        // a group of listed nodes whose triangles fit the 64 lanes (two per
+        // K-tetrahedron); this lane's triangle
         int gend = e, slots = 0;
+        long myrec = -1;
+        int myu = 0;
         while (gend < nm) {
-            if (s_go[gend]) {
-                const int need = 2 * a.nkt[s_m[gend]];
+            const int cg = s_cnt[gend];
+            if (cg & 0x80) {
+                const int need = 2 * (cg & 0x7f);
                 if (slots + need > 64) break;
+                if (lane >= slots && lane < slots + need) {
+                    const int q = (lane - slots) >> 1;
+                    myu = (lane - slots) & 1;
+                    if (myu < (int)((s_pat[gend] >> (2 * q)) & 3)) myrec = a.mrec[(long)s_m[gend] * 24 + q];
+                }
                 slots += need;
             }
             ++gend;
         }
         double myDa = 0.0;
-        {
-            int off = 0;
-            long myrec = -1;
-            int myu = 0;
-            for (int f = e; f < gend; ++f) {
-                if (!s_go[f]) continue;
-                const long mm = s_m[f];
-                const int cnt = a.nkt[mm];
-                if (lane >= off && lane < off + 2 * cnt) {
-                    myrec = a.mrec[mm * 24 + ((lane - off) >> 1)];
-                    myu = (lane - off) & 1;
-                }
-                off += 2 * cnt;
-            }
-            if (myrec >= 0 && myu < a.nuse[myrec]) myDa = tri_distance(a.trec + (myrec * 2 + myu) * kTS, P);
-        }
+        if (myrec >= 0) myDa = tri_distance(a.trec + (myrec * 2 + myu) * kTS, P);
         int off = 0;
-        for (int f = e; f < gend; ++f) {
-            const long mm = s_m[f];
-            const int cnt = a.nkt[mm];
-            if (!s_go[f]) {
+        for (int f2 = e; f2 < gend; ++f2) {
+            const int cg = s_cnt[f2];
+            const int cnt = cg & 0x7f;
+            if (!(cg & 0x80)) {
                 if (iface)
                     for (int c = 0; c < cnt; ++c) {
                         const double s2 = sd_update(SD, D, sgn0);
@@ -737,9 +749,10 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
                     }
                 continue;
             }
+            const uint64_t pat = s_pat[f2];
             if (!iface) D = 10.0;
             for (int q = 0; q < cnt; ++q) {
-                const int nn = a.nuse[a.mrec[mm * 24 + q]];
+                const int nn = (int)((pat >> (2 * q)) & 3);
                 for (int u = 0; u < nn; ++u) {
                     const double Da = __shfl(myDa, off + 2 * q + u);
                     const double ex = (copysign(1.0, D - Da) + 1.0) / 2.0;
