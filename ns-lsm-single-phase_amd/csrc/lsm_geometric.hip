@@ -425,11 +425,15 @@ __global__ __launch_bounds__(256) void geo_mark(Geom g, long n, const int8_t *__
 // appends them (cells in k, j, i order, tetrahedra 1..6), their number, and
 // a sphere about the node holding all their triangles (radius up, shortest
 // edge down)
+// per interface node: frame (3 unit axes), extents along them, 4 vertices
+constexpr int kMF = 0, kME = 9, kMV = 15, kMB = 27;
+
 __global__ __launch_bounds__(256) void geo_iface(Geom g, long nint, const int32_t *__restrict__ ilist,
                                                  const uint8_t *__restrict__ ctets, const int32_t *__restrict__ cslot,
                                                  const int8_t *__restrict__ nuse, const double *__restrict__ trec,
                                                  int32_t *__restrict__ mrec, int8_t *__restrict__ nkt,
-                                                 double *__restrict__ mrad, double *__restrict__ memin)
+                                                 double *__restrict__ mrad, double *__restrict__ memin,
+                                                 double *__restrict__ mbox)
 {
     const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nint) return;
@@ -468,6 +472,62 @@ __global__ __launch_bounds__(256) void geo_iface(Geom g, long nint, const int32_
     nkt[m] = (int8_t)q;
     mrad[m] = any ? R * (1.0 + 1e-12) : -1.0;   // -1: no usable triangle
     memin[m] = em;
+    if (!any) return;
+    // a box about the node's triangles in a frame along their mean normal
+    // (extents rounded outward), and four of their vertices
+    double *bx = mbox + m * kMB;
+    double nx = 0.0, ny = 0.0, nz = 0.0, rx = 0.0, ry = 0.0, rz = 0.0;
+    int nv = 0;
+    for (int qq = 0; qq < q; ++qq) {
+        const long rec = mrec[m * 24 + qq];
+        const int nu = nuse[rec];
+        for (int u = 0; u < nu; ++u) {
+            const double *T = trec + (rec * 2 + u) * kTS;
+            const double a0 = T[kN], a1 = T[kN + 1], a2 = T[kN + 2];
+            if (nv == 0) { rx = a0; ry = a1; rz = a2; }
+            const double sg = (a0 * rx + a1 * ry + a2 * rz) < 0.0 ? -1.0 : 1.0;
+            nx += sg * a0; ny += sg * a1; nz += sg * a2;
+            if (nv < 4) { bx[kMV + 3 * nv] = T[kA]; bx[kMV + 3 * nv + 1] = T[kA + 1]; bx[kMV + 3 * nv + 2] = T[kA + 2]; }
+            ++nv;
+        }
+    }
+    for (int v = nv; v < 4; ++v)
+        for (int c = 0; c < 3; ++c) bx[kMV + 3 * v + c] = bx[kMV + c];
+    double nn = sqrt(nx * nx + ny * ny + nz * nz);
+    if (!(nn > 0.0)) { nx = 0.0; ny = 0.0; nz = 1.0; nn = 1.0; }
+    nx /= nn; ny /= nn; nz /= nn;
+    // a second axis across the normal's smallest component
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (fabs(nx) <= fabs(ny) && fabs(nx) <= fabs(nz)) ax = 1.0;
+    else if (fabs(ny) <= fabs(nz)) ay = 1.0;
+    else az = 1.0;
+    double t1x = ny * az - nz * ay, t1y = nz * ax - nx * az, t1z = nx * ay - ny * ax;
+    const double t1n = sqrt(t1x * t1x + t1y * t1y + t1z * t1z);
+    t1x /= t1n; t1y /= t1n; t1z /= t1n;
+    const double t2x = ny * t1z - nz * t1y, t2y = nz * t1x - nx * t1z, t2z = nx * t1y - ny * t1x;
+    const double F[9] = {nx, ny, nz, t1x, t1y, t1z, t2x, t2y, t2z};
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int qq = 0; qq < q; ++qq) {
+        const long rec = mrec[m * 24 + qq];
+        const int nu = nuse[rec];
+        for (int u = 0; u < nu; ++u) {
+            const double *T = trec + (rec * 2 + u) * kTS;
+            for (int vq = 0; vq < 3; ++vq) {
+                const double d0 = T[3 * vq] - X, d1 = T[3 * vq + 1] - Y, d2 = T[3 * vq + 2] - Z;
+                for (int c = 0; c < 3; ++c) {
+                    const double pr = d0 * F[3 * c] + d1 * F[3 * c + 1] + d2 * F[3 * c + 2];
+                    lo[c] = fmin(lo[c], pr);
+                    hi[c] = fmax(hi[c], pr);
+                }
+            }
+        }
+    }
+    for (int c = 0; c < 9; ++c) bx[kMF + c] = F[c];
+    for (int c = 0; c < 3; ++c) {
+        const double w = 1e-12 * (fabs(lo[c]) + fabs(hi[c]) + R);
+        bx[kME + 2 * c] = lo[c] - w;
+        bx[kME + 2 * c + 1] = hi[c] + w;
+    }
 }
 
 struct DistArgs {
@@ -477,7 +537,7 @@ struct DistArgs {
     const int8_t *nid;
     const int32_t *mrec;
     const int8_t *nkt;
-    const double *mrad, *memin;
+    const double *mrad, *memin, *mbox;
     const int8_t *nuse;
     const double *trec;
     double *pc;
@@ -653,6 +713,24 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
                 const double err = dist_err(dP + R, em, cmax);
                 lb = err < 1e300 ? dP - R - err : -1e300;
                 ub = (dP + R) * (1.0 + 1e-12) + 1e-300;
+                const double *bx = a.mbox + mm * kMB;
+                if (err < 1e300) {
+                    // distance to the box, from the offsets along its axes
+                    double q2 = 0.0;
+                    for (int c = 0; c < 3; ++c) {
+                        const double pr = dx * bx[kMF + 3 * c] + dy * bx[kMF + 3 * c + 1] + dz * bx[kMF + 3 * c + 2];
+                        const double o = fmax(0.0, fmax(bx[kME + 2 * c] - pr, pr - bx[kME + 2 * c + 1]));
+                        q2 += o * o;
+                    }
+                    lb = fmax(lb, sqrt(q2) - err);
+                }
+                // the reference's minimum includes every vertex distance
+                double um = 1e300;
+                for (int v = 0; v < 4; ++v) {
+                    const double e0 = P.x - bx[kMV + 3 * v], e1 = P.y - bx[kMV + 3 * v + 1], e2 = P.z - bx[kMV + 3 * v + 2];
+                    um = fmin(um, e0 * e0 + e1 * e1 + e2 * e2);
+                }
+                ub = fmin(ub, sqrt(um) * (1.0 + 1e-12) + 1e-300);
             }
         }
         // exclusive prefix minimum of the upper bounds over the list
@@ -715,20 +793,26 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
             ++e;
             continue;
         }
-        // a group of listed nodes whose triangles fit the 64 lanes (two per
-        // K-tetrahedron); this lane's triangle
+        // a group of listed nodes whose usable triangles fit the 64 lanes;
+        // this lane's triangle
         int gend = e, slots = 0;
         long myrec = -1;
         int myu = 0;
         while (gend < nm) {
             const int cg = s_cnt[gend];
             if (cg & 0x80) {
-                const int need = 2 * (cg & 0x7f);
+                const uint64_t pat = s_pat[gend];
+                const int need = (int)__popcll(pat & 0x5555555555555555ull) + 2 * (int)__popcll(pat & 0xaaaaaaaaaaaaaaaaull);
                 if (slots + need > 64) break;
                 if (lane >= slots && lane < slots + need) {
-                    const int q = (lane - slots) >> 1;
-                    myu = (lane - slots) & 1;
-                    if (myu < (int)((s_pat[gend] >> (2 * q)) & 3)) myrec = a.mrec[(long)s_m[gend] * 24 + q];
+                    int idx = lane - slots, q = 0;
+                    for (;; ++q) {
+                        const int nn = (int)((pat >> (2 * q)) & 3);
+                        if (idx < nn) break;
+                        idx -= nn;
+                    }
+                    myu = idx;
+                    myrec = a.mrec[(long)s_m[gend] * 24 + q];
                 }
                 slots += need;
             }
@@ -736,7 +820,7 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
         }
         double myDa = 0.0;
         if (myrec >= 0) myDa = tri_distance(a.trec + (myrec * 2 + myu) * kTS, P);
-        int off = 0;
+        int tix = 0;
         for (int f2 = e; f2 < gend; ++f2) {
             const int cg = s_cnt[f2];
             const int cnt = cg & 0x7f;
@@ -754,7 +838,7 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
             for (int q = 0; q < cnt; ++q) {
                 const int nn = (int)((pat >> (2 * q)) & 3);
                 for (int u = 0; u < nn; ++u) {
-                    const double Da = __shfl(myDa, off + 2 * q + u);
+                    const double Da = __shfl(myDa, tix++);
                     const double ex = (copysign(1.0, D - Da) + 1.0) / 2.0;
                     D = D + ex * (Da - D);
                 }
@@ -763,7 +847,6 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
                 if (iface) SD = sd_update(SD, D, sgn0);
                 else pc = pc_update(pc, D);
             }
-            off += 2 * cnt;
         }
         e = gend;
     }
@@ -881,7 +964,8 @@ int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids,
         const size_t o_nkt = o_mrec + al((size_t)nint * 24 * sizeof(int32_t));
         const size_t o_rad = o_nkt + al((size_t)nint);
         const size_t o_emin = o_rad + al((size_t)nint * sizeof(double));
-        const size_t total = o_emin + al((size_t)nint * sizeof(double));
+        const size_t o_box = o_emin + al((size_t)nint * sizeof(double));
+        const size_t total = o_box + al((size_t)nint * kMB * sizeof(double));
         void *store = nullptr;
         if ((rc = store_take(total, &store))) return rc;
         char *sb = (char *)store;
@@ -889,12 +973,12 @@ int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids,
         double *trec = (double *)(sb + o_trec);
         int32_t *mrec = (int32_t *)(sb + o_mrec);
         int8_t *nkt = (int8_t *)(sb + o_nkt);
-        double *mrad = (double *)(sb + o_rad), *memin = (double *)(sb + o_emin);
+        double *mrad = (double *)(sb + o_rad), *memin = (double *)(sb + o_emin), *mbox = (double *)(sb + o_box);
         geo_iso<<<nblk(ncell), 256, 0, s>>>(g, ncell, ctets, cslot, nuse, trec);
         if ((rc = check_hip(hipGetLastError(), "geo_iso"))) return rc;
-        geo_iface<<<nblk(nint), 256, 0, s>>>(g, nint, ilist, ctets, cslot, nuse, trec, mrec, nkt, mrad, memin);
+        geo_iface<<<nblk(nint), 256, 0, s>>>(g, nint, ilist, ctets, cslot, nuse, trec, mrec, nkt, mrad, memin, mbox);
         if ((rc = check_hip(hipGetLastError(), "geo_iface"))) return rc;
-        DistArgs a{g, nint, nnb, ilist, nlist, mid, nid, mrec, nkt, mrad, memin, nuse, trec, pc};
+        DistArgs a{g, nint, nnb, ilist, nlist, mid, nid, mrec, nkt, mrad, memin, mbox, nuse, trec, pc};
         {
             KTimer kt(LSM_KT_GEOMETRIC, s);
             geo_distance<<<(unsigned)(nint + nnb), 64, 0, s>>>(a);
