@@ -90,6 +90,45 @@ def cpu_baseline(n_ij=256, nk=64, steps=6):
             "seconds": secs, "sample": sample + ", C restatement of the oracle (gcc -O2, 1 thread)"}
 
 
+def cpu_geometric_baseline():
+    """The reference's hybrid (geometric) reinitialisation, compiled Fortran
+    (oracle/_ref), on the host cores: a levelsetmethod call with
+    call_levelsetmethod = .false. (water volume, GetTetrahedraList,
+    GetIsosurfaceAndDistances, NeighboursPhiCorrection, bcond) on a 32 x 32 x 24
+    block of the bench's mapping and level set, one MPI rank per core, each on
+    its own copy; the slowest rank is timed.  The band nodes (interface nodes
+    and their neighbours) of the sample come from the GPU path on the same
+    input.  None when the reference build is absent."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle", "ref"))
+    from lsm_amd import api, inputs
+    try:
+        import refio
+        if not refio.available():
+            return None
+    except Exception:  # noqa: BLE001
+        return None
+    g = inputs.sinusoidal_cube_grid(32, seed=2, nk=24)
+    phi = inputs.spheres_sdf(g, seed=3)
+    z = np.zeros(g.shape)
+    ranks = max(1, min(16, os.cpu_count() or 1))
+    _, secs, _ = refio.run("levelset", g, phi, phi0=phi, u=z, v=z, w=z, hybrid=True, call_levelset=False,
+                           rn_freq=1, timing=True, epslsm=0.05, nprocs=ranks)
+    if not secs:
+        return None
+    blk = api.Block(g)
+    p = torch.as_tensor(phi, device="cuda")
+    pc = torch.where(p < -2.220446049250313e-16, torch.full_like(p, -9.9),
+                     torch.where(p > 2.220446049250313e-16, torch.full_like(p, 9.9), torch.zeros_like(p)))
+    ids = blk.empty(torch.int32)
+    api.geometric_reinitialisation(blk, p, pc, ids)
+    band = int((ids != 0).sum())
+    return {"value": ranks * band / secs, "unit": "band-nodes/s", "cores": ranks, "kind": "reference",
+            "seconds": secs, "sample": f"hybrid levelsetmethod without advection on a 32x32x24 block "
+                                       f"({band} band nodes per rank), {ranks} MPI ranks, one per core"}
+
+
 def _events():
     import torch
     return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -642,6 +681,12 @@ def main():
         t1 = time.time()
         cpu = cpu_baseline()
         log(f"cpu baseline ({cpu['kind']}): {cpu['value']:.3e} cell-updates/s in {time.time() - t1:.1f}s")
+        if "geometric" in extra:
+            t1 = time.time()
+            geo_cpu = cpu_geometric_baseline()
+            if geo_cpu:
+                extra["geometric"]["cpu_reference"] = geo_cpu
+                log(f"geometric cpu baseline: {geo_cpu['value']:.3e} band-nodes/s in {time.time() - t1:.1f}s")
 
     line = {
         "metric": METRIC,
