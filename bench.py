@@ -265,27 +265,40 @@ def bench_batch(steps=10):
     dt = B.cfl_dt(lay, grids, vel)
     hmin = min(inputs.spacing_bounds(gd)[0] for gd in grids)
     hmax = max(inputs.spacing_bounds(gd)[1] for gd in grids)
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4, epsReinitialisation=1e-12)
+    res = {"workload": "64 blocks (4x4x4 lattice, sizes U{32..160}, 3 ghost layers), 32 spheres, "
+                       "16-mode Fourier velocity, FP64", "cells": lay.cells(), "sizes": lay.sizes}
+    # per-block plain launches on per-block arrays with dense rows (each block
+    # as a rank of the reference holds it), one launch per block and kernel
+    dlay = B.lattice_layout(lay.sizes, ghost=lay.ghost, align=0)
+    db = B.Batch(dlay, grids)
+    ducn = db.contra_velocity(*[db.pack([v[c] for v in vel]) for c in range(3)])
+    dp0 = db.halo_fill(db.pack(phi))
+    dout = db.empty()
+    dw_adv, dw_full = db.workspace(), db.workspace(api._lib.LSM_OP_LEVELSET_STEP)
+    db.set_config("perblock")
+    res["perblock_adv_ms"] = round(_time(lambda: db.levelset_step(dp0, ducn, dt, out=dout, work=dw_adv), steps), 4)
+    res["perblock_full_ms"] = round(_time(lambda: db.levelset_step(dp0, ducn, dt, prm, out=dout, work=dw_full),
+                                          steps), 4)
+    del db, ducn, dp0, dout, dw_adv, dw_full
+    torch.cuda.empty_cache()
+    # the batch: rows padded to 128-byte lines, first owned column aligned
     bt = B.Batch(lay, grids)
     ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vel]) for c in range(3)])
     p0 = bt.halo_fill(bt.pack(phi))
     del grids, vel, phi
     log(f"batch inputs: {len(lay.blocks)} blocks, {lay.cells()} cells, {time.time() - t0:.1f}s")
-    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4, epsReinitialisation=1e-12)
     out = bt.empty()
     w_adv, w_full = bt.workspace(), bt.workspace(api._lib.LSM_OP_LEVELSET_STEP)
-    res = {"workload": "64 blocks (4x4x4 lattice, sizes U{32..160}, 3 ghost layers), 32 spheres, "
-                       "16-mode Fourier velocity, FP64", "cells": lay.cells(), "sizes": lay.sizes}
     bt.set_config("perblock")
-    res["perblock_adv_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, out=out, work=w_adv), steps), 4)
-    res["perblock_full_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
+    res["perblock_aligned_adv_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, out=out, work=w_adv), steps), 4)
+    res["perblock_aligned_full_ms"] = round(
+        _time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
     cfg, ms = bt.autotune(p0, ucn, dt, reps=3, force=True, out=out, work=w_adv)
     res["autotuned"] = cfg
-    res["full_ms_by_reinit_tile"] = {}
-    for w in (64, 32, 16):
-        bt.set_reinit_tile(w)
-        res["full_ms_by_reinit_tile"][w] = round(
-            _time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
-    bt.set_reinit_tile(0)
+    best_w, times = bt.autotune_reinit(p0, ucn, dt, prm, reps=steps, out=out, work=w_full)
+    res["full_ms_by_reinit_tile"] = {w: round(t, 4) for w, t in times.items()}
+    res["reinit_tile"] = best_w
     res["batched_adv_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, out=out, work=w_adv), steps), 4)
     res["batched_full_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
     bt.graph_capture(out, p0, ucn, dt, work=w_adv)
@@ -297,6 +310,7 @@ def bench_batch(steps=10):
         res[f"{kind}_cell_updates_per_s_batched"] = lay.cells() / (best / 1e3)
         res[f"{kind}_cell_updates_per_s_perblock"] = lay.cells() / (res[f"perblock_{kind}_ms"] / 1e3)
         res[f"{kind}_speedup_vs_perblock"] = round(res[f"perblock_{kind}_ms"] / best, 3)
+        res[f"{kind}_speedup_vs_perblock_aligned"] = round(res[f"perblock_aligned_{kind}_ms"] / best, 3)
     bt.set_config("queue", 8, 8, 0)
     res["queue_adv_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, out=out, work=w_adv), steps), 4)
     log(f"batch: adv {res['perblock_adv_ms']} -> {res['graph_adv_ms']} ms ({res['adv_speedup_vs_perblock']}x), "

@@ -341,6 +341,8 @@ typedef struct LsmBatchBlock {
     double big_phi;
     double dc, de, dz;
     int64_t offset;                   /* element offset of the block's array in the batch arrays */
+    int32_t pitch;                    /* doubles per row of the block's array, >= ni + 2 ghost
+                                         (0: ni + 2 ghost); planes hold nj + 2 ghost rows */
 } LsmBatchBlock;
 
 typedef struct LsmBatchDesc {

@@ -1093,8 +1093,10 @@ int lsm_batch_create(const LsmBatchDesc *d, LsmBatch **out)
         memset(&Y, 0, sizeof Y);
         Y.off = X.offset;
         Y.ni = X.ni + 2 * g; Y.nj = X.nj + 2 * g; Y.nk = X.nk + 2 * g;
-        Y.sj = Y.ni; Y.sk = (long)Y.ni * Y.nj;
-        if (Y.off < 0 || Y.off + (long)Y.ni * Y.nj * Y.nk > d->total) return bad(LSM_ERR_ARG, "array outside the batch arrays", b);
+        if (X.pitch != 0 && X.pitch < Y.ni) return bad(LSM_ERR_ARG, "row pitch below ni + 2 ghost", b);
+        Y.sj = X.pitch ? X.pitch : Y.ni;
+        Y.sk = Y.sj * Y.nj;
+        if (Y.off < 0 || Y.off + Y.sk * Y.nk > d->total) return bad(LSM_ERR_ARG, "array outside the batch arrays", b);
         Y.r = block_range(X, g);
         for (int ax = 0; ax < 3; ++ax) Y.gs[ax] = conn ? B->lstart[ax][X.lattice[ax]] : 0;
         for (int q = 0; q < 6; ++q) Y.bt[q] = X.btype[q];

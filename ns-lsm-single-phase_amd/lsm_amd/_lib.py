@@ -44,7 +44,8 @@ class LsmReinitParams(C.Structure):
 class LsmBatchBlock(C.Structure):
     _fields_ = [("ni", C.c_int32), ("nj", C.c_int32), ("nk", C.c_int32), ("lattice", C.c_int32 * 3),
                 ("phys_faces", C.c_int32), ("btype", C.c_int32 * 6), ("big_phi", C.c_double),
-                ("dc", C.c_double), ("de", C.c_double), ("dz", C.c_double), ("offset", C.c_int64)]
+                ("dc", C.c_double), ("de", C.c_double), ("dz", C.c_double), ("offset", C.c_int64),
+                ("pitch", C.c_int32)]
 
 
 class LsmBatchDesc(C.Structure):

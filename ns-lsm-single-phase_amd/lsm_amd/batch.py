@@ -9,7 +9,10 @@ unrelated instances without ghost layers.
 
 Batch fields are one flat float64 CUDA tensor of `layout.total` doubles:
 block b's array, shape (nk+2g, nj+2g, ni+2g) i fastest with owned nodes at
-local indices g .. g+n-1, starts at `layout.offsets[b]`.
+local indices g .. g+n-1, starts at `layout.offsets[b]`; its rows are
+`layout.pitch[b]` doubles apart.  By default rows are padded to a multiple of
+16 doubles and every block starts so that its first owned column lies on a
+128-byte line (the stage kernels' wavefronts then read whole lines).
 
 multiblock_case() builds the multi-block workload: per-axis block sizes drawn
 from U{nmin..nmax}, a per-block sinusoidal perturbation of a uniform grid of
@@ -58,11 +61,17 @@ class Layout:
     sizes: tuple = ((), (), ())         # owned sizes of each lattice slab per axis (i, j, k)
     offsets: List[int] = field(default_factory=list)
     total: int = 0
+    pitch: List[int] = field(default_factory=list)   # doubles per row of each block's array
+    align: int = 16
 
     def shape(self, b):
         g = self.ghost
         s = self.blocks[b]
         return (s.nk + 2 * g, s.nj + 2 * g, s.ni + 2 * g)
+
+    def alloc_shape(self, b):
+        nk, nj, _ = self.shape(b)
+        return (nk, nj, self.pitch[b])
 
     def global_shape(self):
         return tuple(int(sum(self.sizes[a])) for a in (2, 1, 0))   # (Nk, Nj, Ni)
@@ -72,20 +81,32 @@ class Layout:
 
 
 def _finish(lay: Layout) -> Layout:
+    """Offsets and row pitches.  align > 0: rows padded to a multiple of align
+    doubles and offset + ghost a multiple of align (first owned column on a
+    line boundary); align = 0: dense rows, 256-byte aligned block arrays."""
     off = 0
+    a = int(lay.align)
+    g = lay.ghost
     lay.offsets = []
+    lay.pitch = []
     for b in range(len(lay.blocks)):
-        lay.offsets.append(off)
         nk, nj, ni = lay.shape(b)
-        n = ni * nj * nk
-        off += (n + 31) // 32 * 32     # 256-byte aligned block arrays
-    lay.total = off
+        if a > 0:
+            pitch = (ni + a - 1) // a * a
+            off += (-(off + g)) % a
+        else:
+            pitch = ni
+            off = (off + 31) // 32 * 32
+        lay.offsets.append(off)
+        lay.pitch.append(pitch)
+        off += pitch * nj * nk
+    lay.total = (off + 31) // 32 * 32
     return lay
 
 
-def lattice_layout(sizes, ghost=3, btype=(1, 1, 1, 1, 1, 1), big_phi=5.0) -> Layout:
+def lattice_layout(sizes, ghost=3, btype=(1, 1, 1, 1, 1, 1), big_phi=5.0, align=16) -> Layout:
     """Connected layout: sizes = (sizes_i, sizes_j, sizes_k), owned nodes of
-    each lattice slab along each axis."""
+    each lattice slab along each axis; align as in _finish."""
     L = tuple(len(s) for s in sizes)
     starts = [np.concatenate([[0], np.cumsum(s)]).astype(int) for s in sizes]
     blocks = []
@@ -102,13 +123,13 @@ def lattice_layout(sizes, ghost=3, btype=(1, 1, 1, 1, 1, 1), big_phi=5.0) -> Lay
                 blocks.append(BlockSpec(int(sizes[0][I]), int(sizes[1][J]), int(sizes[2][K]), (I, J, K),
                                         (int(starts[0][I]), int(starts[1][J]), int(starts[2][K])), f,
                                         tuple(btype), big_phi))
-    return _finish(Layout(blocks, ghost, True, L, tuple(tuple(int(v) for v in s) for s in sizes)))
+    return _finish(Layout(blocks, ghost, True, L, tuple(tuple(int(v) for v in s) for s in sizes), align=align))
 
 
-def independent_layout(shapes, btype=(1, 1, 1, 1, 1, 1), big_phi=5.0) -> Layout:
+def independent_layout(shapes, btype=(1, 1, 1, 1, 1, 1), big_phi=5.0, align=16) -> Layout:
     """Independent instances of owned sizes shapes = [(ni, nj, nk), ...]."""
     blocks = [BlockSpec(int(a), int(b), int(c), btype=tuple(btype), big_phi=big_phi) for a, b, c in shapes]
-    return _finish(Layout(blocks, 0, False))
+    return _finish(Layout(blocks, 0, False, align=align))
 
 
 # ---------------------------------------------------------------------------
@@ -175,13 +196,13 @@ def block_grid(lay: Layout, b: int, amps) -> inputs.Grid:
 
 
 def multiblock_case(lattice=(4, 4, 4), nmin=32, nmax=160, ghost=3, amax=0.04, seed=6, nspheres=32,
-                    seed_phi=7, seed_vel=4, btype=(1, 1, 1, 1, 1, 1)):
+                    seed_phi=7, seed_vel=4, btype=(1, 1, 1, 1, 1, 1), align=16):
     """The multi-block workload: returns (layout, grids, phi, (u, v, w)) with
     per-block numpy arrays (ghost layers included)."""
     rng = np.random.default_rng(seed)
     sizes = tuple(tuple(int(v) for v in rng.integers(nmin, nmax + 1, size=lattice[a])) for a in range(3))
     amps = rng.uniform(0.0, amax, size=int(np.prod(lattice)))
-    lay = lattice_layout(sizes, ghost=ghost, btype=btype)
+    lay = lattice_layout(sizes, ghost=ghost, btype=btype, align=align)
     grids = [block_grid(lay, b, amps) for b in range(len(lay.blocks))]
     phi = [inputs.spheres_sdf(gd, nspheres=nspheres, seed=seed_phi, rmin=0.05, rmax=0.15) for gd in grids]
     modes = inputs.fourier_modes(16, 4, seed_vel)
@@ -252,6 +273,7 @@ class Batch:
             e.big_phi = s.big_phi
             e.dc = e.de = e.dz = 1.0
             e.offset = lay.offsets[b]
+            e.pitch = lay.pitch[b]
         d = LsmBatchDesc()
         d.nblocks = len(lay.blocks)
         d.ghost = lay.ghost
@@ -290,9 +312,12 @@ class Batch:
         import torch
         flat = np.zeros(self.lay.total)
         for b, a in enumerate(per_block):
-            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
-            assert a.size == int(np.prod(self.lay.shape(b)))
-            flat[self.lay.offsets[b]:self.lay.offsets[b] + a.size] = a
+            a = np.asarray(a, dtype=np.float64)
+            shp = self.lay.shape(b)
+            assert a.size == int(np.prod(shp))
+            ash = self.lay.alloc_shape(b)
+            n = int(np.prod(ash))
+            flat[self.lay.offsets[b]:self.lay.offsets[b] + n].reshape(ash)[:, :, :shp[2]] = a.reshape(shp)
         return torch.as_tensor(flat, device=self.device)
 
     def unpack(self, t) -> List[np.ndarray]:
@@ -300,8 +325,9 @@ class Batch:
         out = []
         for b in range(len(self.lay.blocks)):
             shp = self.lay.shape(b)
-            n = int(np.prod(shp))
-            out.append(flat[self.lay.offsets[b]:self.lay.offsets[b] + n].reshape(shp).copy())
+            ash = self.lay.alloc_shape(b)
+            n = int(np.prod(ash))
+            out.append(flat[self.lay.offsets[b]:self.lay.offsets[b] + n].reshape(ash)[:, :, :shp[2]].copy())
         return out
 
     def _check(self, t, name):
@@ -418,6 +444,32 @@ class Batch:
                                              C.c_void_p(work.data_ptr()), work.numel(), int(reps), int(bool(force)),
                                              C.byref(ms), _stream()), "lsm_batch_autotune")
         return self.get_config(), (ms.value if ms.value >= 0 else None)
+
+    def autotune_reinit(self, phi_n, ucn, delti_lsm, params, mode="fast", reps=3, out=None, work=None,
+                        widths=(64, 32, 16)):
+        """Times one full step (levelset_step with params) per reinitialisation
+        tile width, keeps the fastest; returns (width, {width: ms per step}).
+        `out` is overwritten."""
+        import torch
+        if out is None:
+            out = self.empty()
+        if work is None:
+            work = self.workspace(_lib.LSM_OP_LEVELSET_STEP)
+        times = {}
+        for w in widths:
+            self.set_reinit_tile(w)
+            self.levelset_step(phi_n, ucn, delti_lsm, params, mode=mode, out=out, work=work)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                self.levelset_step(phi_n, ucn, delti_lsm, params, mode=mode, out=out, work=work)
+            e1.record()
+            torch.cuda.synchronize()
+            times[w] = e0.elapsed_time(e1) / reps
+        best = min(times, key=times.get)
+        self.set_reinit_tile(best)
+        return best, times
 
     def graph_capture(self, phi_np1, phi_n, ucn, delti_lsm, params=None, mode="fast", work=None):
         """Records one levelset_step with these tensors into a HIP graph kept

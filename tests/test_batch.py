@@ -19,9 +19,9 @@ torch = pytest.importorskip("torch")
 from lsm_amd import _lib, batch as B, inputs  # noqa: E402
 
 
-def _case(lattice=(3, 2, 2), nmin=6, nmax=14, ghost=3, seed=11, btype=(1, 1, 1, 1, 1, 1)):
+def _case(lattice=(3, 2, 2), nmin=6, nmax=14, ghost=3, seed=11, btype=(1, 1, 1, 1, 1, 1), align=16):
     lay, grids, phi, vel, amps = B.multiblock_case(lattice=lattice, nmin=nmin, nmax=nmax, ghost=ghost, seed=seed,
-                                                   nspheres=6, btype=btype)
+                                                   nspheres=6, btype=btype, align=align)
     return lay, grids, phi, vel, amps
 
 
@@ -37,11 +37,19 @@ def test_multiblock_grid_is_one_grid():
     for c in range(3):
         for name in ("csi", "eta", "zet"):
             assert np.array_equal(B.assemble(lay, [getattr(g, name)[c] for g in grids]), getattr(G, name)[c])
-    # per-block arrays carry the ghost layers, offsets are disjoint and aligned
+    # per-block arrays carry the ghost layers; rows padded to 16 doubles, the
+    # first owned column of every row on a 128-byte line, arrays disjoint
+    end = 0
     for b in range(len(lay.blocks)):
-        assert lay.offsets[b] % 32 == 0
         assert grids[b].x.shape == lay.shape(b)
-    assert lay.offsets[-1] + int(np.prod(lay.shape(len(lay.blocks) - 1))) <= lay.total
+        assert lay.pitch[b] % 16 == 0 and lay.pitch[b] >= lay.shape(b)[2]
+        assert (lay.offsets[b] + lay.ghost) % 16 == 0
+        assert lay.offsets[b] >= end
+        end = lay.offsets[b] + int(np.prod(lay.alloc_shape(b)))
+    assert end <= lay.total
+    dense = B.lattice_layout(lay.sizes, ghost=lay.ghost, align=0)
+    for b in range(len(dense.blocks)):
+        assert dense.offsets[b] % 32 == 0 and dense.pitch[b] == dense.shape(b)[2]
 
 
 def _desc(lay, blocks=None, ghost=None, connected=None, lattice=None):
@@ -55,6 +63,7 @@ def _desc(lay, blocks=None, ghost=None, connected=None, lattice=None):
         e.big_phi = 5.0
         e.dc = e.de = e.dz = 1.0
         e.offset = lay.offsets[b]
+        e.pitch = lay.pitch[b]
     if blocks:
         blocks(arr)
     d = _lib.LsmBatchDesc()
@@ -68,7 +77,7 @@ def _desc(lay, blocks=None, ghost=None, connected=None, lattice=None):
     return d, arr
 
 
-@pytest.mark.parametrize("what", ["faces", "sizes", "lattice", "ghost", "overlap", "independent_ghost"])
+@pytest.mark.parametrize("what", ["faces", "sizes", "lattice", "ghost", "overlap", "independent_ghost", "pitch"])
 def test_batch_create_rejects_bad_layouts(what):
     """Descriptor validation (host logic, before any device call)."""
     L = _lib.load()
@@ -86,6 +95,8 @@ def test_batch_create_rejects_bad_layouts(what):
         kw["blocks"] = lambda a: setattr(a[3], "offset", lay.total - 10)
     elif what == "independent_ghost":
         kw["connected"] = 0
+    elif what == "pitch":
+        kw["blocks"] = lambda a: setattr(a[2], "pitch", a[2].ni + 5)   # below ni + 2 ghost
     d, keep = _desc(lay, **kw)
     h = C.c_void_p()
     rc = L.lsm_batch_create(C.byref(d), C.byref(h))
@@ -149,8 +160,9 @@ CONFIGS = [("queue", 4, 8, 0, 64), ("queue", 8, 4, 2, 64), ("grid", 4, 16, 0, 64
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 @pytest.mark.parametrize("btype", [(1, 1, 1, 1, 1, 1), (0, 2, 1, 2, 1, 0)])
-def test_connected_batch_matches_single_block(mode, btype):
-    lay, grids, phi, vel, _ = _case(btype=btype)
+@pytest.mark.parametrize("align", [16, 0])
+def test_connected_batch_matches_single_block(mode, btype, align):
+    lay, grids, phi, vel, _ = _case(btype=btype, align=align)
     dt = B.cfl_dt(lay, grids, vel)
     ref = _single_block_steps(lay, grids, phi, vel, dt, 3, mode, btype)
     bt = B.Batch(lay, grids)
@@ -168,10 +180,11 @@ def test_connected_batch_matches_single_block(mode, btype):
 
 
 @pytest.mark.gpu
-def test_independent_batch_matches_each_instance():
+@pytest.mark.parametrize("align", [16, 0])
+def test_independent_batch_matches_each_instance(align):
     from lsm_amd import api
     shapes = [(12, 9, 10), (17, 6, 8), (9, 14, 7), (33, 5, 6), (8, 8, 21)]
-    lay = B.independent_layout(shapes)
+    lay = B.independent_layout(shapes, align=align)
     grids, phis, vels = [], [], []
     for q, (ni, nj, nk) in enumerate(shapes):
         gd = inputs.sinusoidal_cube_grid(ni, amp=0.03, seed=40 + q, nj=nj, nk=nk)
