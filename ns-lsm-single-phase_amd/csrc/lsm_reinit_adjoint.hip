@@ -106,6 +106,130 @@ __global__ __launch_bounds__(256) void radj_tape_gather(TGatherArgs a)
     else a.out[m] = acc + lin + a.adn[m];
 }
 
+// Tiled form of radj_tape_gather, the same sums in the same order: a
+// workgroup of 64 (i) x GY (j) threads marches up a chunk of k planes; per
+// plane the products lam q_0 (with a 3-node i halo), lam q_1 (3-node j halo)
+// and the branch codes (both halos) sit in LDS, and lam q_2 and the codes of
+// planes k-3 .. k+3 of the thread's own column in a register window.
+constexpr int GX = 64, GY = 8, GH = 3;
+
+// the coefficient of a face node's one-sided difference (biased_coef) at one
+// slot, or of the i = 1, k = 1 edge's fixed second-order stencil; faces only,
+// kept out of line
+__device__ __noinline__ double face_coef(int br, int order_bc, int dir, bool enobc, int slot, bool edge)
+{
+    double e[4];
+    if (edge) { e[0] = -3.0; e[1] = 4.0; e[2] = -1.0; e[3] = 0.0; }
+    else biased_coef(br, order_bc, dir, enobc, e);
+    return slot == 0 ? e[0] : slot == 1 ? e[1] : slot == 2 ? e[2] : e[3];
+}
+
+__global__ __launch_bounds__(GX * GY) void radj_tape_gather_tiled(TGatherArgs a, int tiles_i, int tiles_j, int kc)
+{
+    __shared__ double sW0[GY][GX + 2 * GH];
+    __shared__ double sW1[GY + 2 * GH][GX];
+    __shared__ uint16_t sC[GY + 2 * GH][GX + 2 * GH];
+    __shared__ double s_eno[16][5];   // interior coefficients of every ENO2 branch code
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * GX + tx;
+    if (tid < 16) {
+        double e[5];
+        eno2_coef(tid, e);
+        for (int q = 0; q < 5; ++q) s_eno[tid][q] = e[q];
+    }
+    const int b = blockIdx.x;
+    const int ti = b % tiles_i, t2 = b / tiles_i;
+    const int tj = t2 % tiles_j, tk = t2 / tiles_j;
+    const int i0 = ti * GX, j0 = tj * GY;
+    const int i = i0 + tx, j = j0 + ty;
+    const int kb = tk * kc, ke = min(kb + kc, a.nk) - 1;
+    const bool inb = i < a.ni && j < a.nj;
+    const int ic = min(i, a.ni - 1), jc = min(j, a.nj - 1);
+    const long col = (long)ic + a.sj * jc;
+    const bool active = a.it < *a.iters;
+    const bool j_on = j == 0 || j == a.nj - 1;
+    auto kcl = [&](int kk) { return min(max(kk, 0), a.nk - 1); };
+    // k window: planes k-3 .. k+3 of this column (clamped; entries outside the
+    // block are never used)
+    double w2[7];
+    int c2[7];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+        const long l = col + a.sk * (long)kcl(kb - 3 + q);
+        w2[q] = a.lam[l] * a.q[2][l];
+        c2[q] = a.code[l];
+    }
+    for (int k = kb; k <= ke; ++k) {
+        const long pk = a.sk * (long)k;
+        // i products with the i halo (rows of the tile)
+        for (int c = tid; c < GY * (GX + 2 * GH); c += GX * GY) {
+            const int rr = c / (GX + 2 * GH), cc = c % (GX + 2 * GH);
+            const long l = (long)min(max(i0 - GH + cc, 0), a.ni - 1) + a.sj * min(j0 + rr, a.nj - 1) + pk;
+            sW0[rr][cc] = a.lam[l] * a.q[0][l];
+        }
+        // j products with the j halo (columns of the tile)
+        for (int c = tid; c < (GY + 2 * GH) * GX; c += GX * GY) {
+            const int rr = c / GX, cc = c % GX;
+            const long l = (long)min(i0 + cc, a.ni - 1) + a.sj * min(max(j0 - GH + rr, 0), a.nj - 1) + pk;
+            sW1[rr][cc] = a.lam[l] * a.q[1][l];
+        }
+        // codes with both halos
+        for (int c = tid; c < (GY + 2 * GH) * (GX + 2 * GH); c += GX * GY) {
+            const int rr = c / (GX + 2 * GH), cc = c % (GX + 2 * GH);
+            const long l = (long)min(max(i0 - GH + cc, 0), a.ni - 1) + a.sj * min(max(j0 - GH + rr, 0), a.nj - 1) + pk;
+            sC[rr][cc] = a.code[l];
+        }
+        __syncthreads();
+        if (inb) {
+            const long m = col + pk;
+            const double lam = a.lam[m];
+            const bool upd = active && a.conv[m];
+            double acc = upd ? a.c2 * lam : lam;
+            if (active) {
+                if (sC[ty + GH][tx + GH] & kHasT) a.adj_s[m] += lam * a.T[m];
+                const int idx[3] = {i, j, k};
+                const int nn[3] = {a.ni, a.nj, a.nk};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+#pragma unroll
+                    for (int o = -3; o <= 3; ++o) {
+                        const int ia = idx[d] - o;   // index of the node whose offset-o entry is this node
+                        if (ia < 0 || ia > nn[d] - 1) continue;
+                        int slot, kind;              // kind: 0 inside, 1 low face, 2 high face
+                        if (ia == 0) { slot = o; kind = 1; }
+                        else if (ia == nn[d] - 1) { slot = -o; kind = 2; }
+                        else { slot = o + 2; kind = 0; }
+                        if (slot < 0 || slot > (kind == 0 ? 4 : 3)) continue;
+                        int c;
+                        double w;
+                        if (d == 0) { c = sC[ty + GH][tx + GH - o]; w = sW0[ty][tx + GH - o]; }
+                        else if (d == 1) { c = sC[ty + GH - o][tx + GH]; w = sW1[ty + GH - o][tx]; }
+                        else { c = c2[3 - o]; w = w2[3 - o]; }
+                        if (!(c & kLive)) continue;
+                        const int br = (c >> (4 * d)) & 15;
+                        const double e = kind == 0 ? s_eno[br][slot]
+                                                   : face_coef(br, a.order_bc, kind == 1 ? 1 : -1, a.eno_bc != 0, slot,
+                                                               d == 0 && ia == 0 && k == 0 && !j_on);   // edge i = 1, k = 1
+                        acc += w * e;
+                    }
+                }
+            }
+            const double lin = upd ? a.c1 * lam : 0.0;
+            if (a.stage == 3) { a.adn[m] = lin; a.out[m] = acc; }
+            else if (a.stage == 2) { a.adn[m] += lin; a.out[m] = acc; }
+            else a.out[m] = acc + lin + a.adn[m];
+        }
+        // slide the k window
+#pragma unroll
+        for (int q = 0; q < 6; ++q) { w2[q] = w2[q + 1]; c2[q] = c2[q + 1]; }
+        {
+            const long l = col + a.sk * (long)kcl(k + 4);
+            w2[6] = a.lam[l] * a.q[2][l];
+            c2[6] = a.code[l];
+        }
+        __syncthreads();
+    }
+}
+
 // adj_phi0 = lam + adj_s * s'(phi0)
 __global__ __launch_bounds__(256) void radj_final(long n, const double *lam, const double *adj_s,
                                                   const double *phi0, double eps, double *adj_phi0)
@@ -162,7 +286,10 @@ int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const dou
                            {t.q[it][st][0], t.q[it][st][1], t.q[it][st][2]}, t.code[it][st], t.T[it][st],
                            t.conv[it], t.iters, it, st + 1, p.order_bc, p.eno_bc, C[st][0], C[st][1], other, adn,
                            adj_s};
-            radj_tape_gather<<<nblk(b.n), 256, 0, s>>>(ga);
+            const int tiles_i = (b.e.ni + GX - 1) / GX, tiles_j = (b.e.nj + GY - 1) / GY;
+            const int kc = 8, tiles_k = (b.e.nk + kc - 1) / kc;
+            radj_tape_gather_tiled<<<(unsigned)(tiles_i * tiles_j * tiles_k), dim3(GX, GY, 1), 0, s>>>(
+                ga, tiles_i, tiles_j, kc);
             double *tmp = lam; lam = other; other = tmp;
         }
         if ((rc = check_hip(hipGetLastError(), "reinit adjoint kernels"))) return rc;
