@@ -450,7 +450,7 @@ int launch_bcond_adjoint(const Blk &b, const double *phi, double *adj, hipStream
 int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
                        const double *U2, const double *U3, double dt, int mode, const double *adj_np1,
                        double *adj_n, double *adjU1, double *adjU2, double *adjU3, Carver &w,
-                       hipStream_t s, const double *const *stages)
+                       hipStream_t s, const double *const *stages, bool adj_in_place)
 {
     const size_t fb = (size_t)b.n * sizeof(double);
     const double *F[3];
@@ -458,7 +458,9 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
     double *Fw[3] = {nullptr, nullptr, nullptr};
     if (!stages)
         for (auto &p : Fw) p = (double *)w.take(fb);
-    for (auto &p : L) p = (double *)w.take(fb);
+    // adj_in_place: the caller's adj_np1 is scratch the reverse sweep may overwrite
+    L[0] = adj_in_place ? const_cast<double *>(adj_np1) : (double *)w.take(fb);
+    L[1] = (double *)w.take(fb);
     for (int q = 0; q < 3; ++q) F[q] = stages ? stages[q] : Fw[q];
     if (!F[0] || !F[1] || !F[2] || !L[0] || !L[1]) return fail(LSM_ERR_WORKSPACE, "adjoint workspace too small");
     double *P = (double *)w.take(15 * fb);   // only the two-kernel form uses it
@@ -466,7 +468,7 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
     // forward stages phi^(1), phi^(2), phi^(3)
     if (!stages &&
         (rc = launch_rk3_stages(b, phi_n, mask, U1, U2, U3, dt, mode, Fw[0], Fw[1], Fw[2], s))) return rc;
-    if ((rc = launch_copy(adj_np1, L[0], b.n, s))) return rc;
+    if (!adj_in_place && (rc = launch_copy(adj_np1, L[0], b.n, s))) return rc;
     adj_fill<<<nblk(b.n), 256, 0, s>>>(adj_n, b.n, 0.0);
 
     static const double C[3][3] = {{1.0, 0.0, 1.0},

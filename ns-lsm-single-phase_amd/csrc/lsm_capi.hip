@@ -210,7 +210,7 @@ static int step_reverse(const Blk &b, const LsmReinitParams &p, const double *ph
     if ((rc = launch_reinit_eno2_adjoint(b, p, T.F[2], L, LX, nullptr, r, s, &T.rt))) return rc;
     const double *st[3] = {T.F[0], T.F[1], T.F[2]};
     Carver k = c;   // the reinitialisation adjoint's fields are free again
-    return launch_rk3_adjoint(b, phi_n, nullptr, U1, U2, U3, dt, mode, LX, adj_n, aU1, aU2, aU3, k, s, st);
+    return launch_rk3_adjoint(b, phi_n, nullptr, U1, U2, U3, dt, mode, LX, adj_n, aU1, aU2, aU3, k, s, st, true);
 }
 
 static size_t op_bytes(const Blk &b, int op, int itermax = 0)

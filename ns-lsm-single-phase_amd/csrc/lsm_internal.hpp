@@ -104,7 +104,7 @@ size_t rk3_adjoint_work(const Blk &b);
 int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
                        const double *U2, const double *U3, double dt, int mode, const double *adj_np1,
                        double *adj_n, double *adjU1, double *adjU2, double *adjU3, Carver &w,
-                       hipStream_t s, const double *const *stages = nullptr);
+                       hipStream_t s, const double *const *stages = nullptr, bool adj_in_place = false);
 
 // ---- launchers (lsm_reinit.hip) ----
 size_t reinit_eno2_work(const Blk &b);

@@ -57,57 +57,10 @@ struct TGatherArgs {
     double *adj_s;   // accumulated dJ/ds(phi0)
 };
 
-__global__ __launch_bounds__(256) void radj_tape_gather(TGatherArgs a)
-{
-    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= a.n) return;
-    const bool active = a.it < *a.iters;
-    const double lam = a.lam[m];
-    const bool upd = active && a.conv[m];
-    double acc = upd ? a.c2 * lam : lam;
-    if (active) {
-        if (a.code[m] & kHasT) a.adj_s[m] += lam * a.T[m];
-        const int i = (int)(m % a.ni);
-        const long r = m / a.ni;
-        const int idx[3] = {i, (int)(r % a.nj), (int)(r / a.nj)};
-        const int nn[3] = {a.ni, a.nj, a.nk};
-        const long st[3] = {1, a.sj, a.sk};
-        const bool j_on = idx[1] == 0 || idx[1] == a.nj - 1;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-#pragma unroll
-            for (int o = -3; o <= 3; ++o) {
-                const int ia = idx[d] - o;   // index of the node whose offset-o entry is this node
-                if (ia < 0 || ia > nn[d] - 1) continue;
-                int slot, kind;              // kind: 0 inside, 1 low face, 2 high face
-                if (ia == 0) { slot = o; kind = 1; }
-                else if (ia == nn[d] - 1) { slot = -o; kind = 2; }
-                else { slot = o + 2; kind = 0; }
-                if (slot < 0 || slot > (kind == 0 ? 4 : 3)) continue;
-                const long l = m - (long)o * st[d];
-                const int c = a.code[l];
-                if (!(c & kLive)) continue;
-                const int br = (c >> (4 * d)) & 15;
-                double e[5];
-                if (kind == 0) {
-                    eno2_coef(br, e);
-                } else if (d == 0 && ia == 0 && idx[2] == 0 && !j_on) {
-                    e[0] = -3.0; e[1] = 4.0; e[2] = -1.0; e[3] = 0.0;   // edge i = 1, k = 1
-                } else {
-                    biased_coef(br, a.order_bc, kind == 1 ? 1 : -1, a.eno_bc != 0, e);
-                }
-                acc += (a.lam[l] * a.q[d][l]) * e[slot];
-            }
-        }
-    }
-    const double lin = upd ? a.c1 * lam : 0.0;
-    if (a.stage == 3) { a.adn[m] = lin; a.out[m] = acc; }
-    else if (a.stage == 2) { a.adn[m] += lin; a.out[m] = acc; }
-    else a.out[m] = acc + lin + a.adn[m];
-}
-
-// Tiled form of radj_tape_gather, the same sums in the same order: a
-// workgroup of 64 (i) x GY (j) threads marches up a chunk of k planes; per
+// One reverse stage: at each node the sum over the nodes whose stencil holds
+// it of their adjoint x q_d x the coefficient rebuilt from their branch code,
+// plus the linear parts.  A workgroup of 64 (i) x GY (j) threads marches up a
+// chunk of k planes; per
 // plane the products lam q_0 (with a 3-node i halo), lam q_1 (3-node j halo)
 // and the branch codes (both halos) sit in LDS, and lam q_2 and the codes of
 // planes k-3 .. k+3 of the thread's own column in a register window.
@@ -124,7 +77,7 @@ __device__ __noinline__ double face_coef(int br, int order_bc, int dir, bool eno
     return slot == 0 ? e[0] : slot == 1 ? e[1] : slot == 2 ? e[2] : e[3];
 }
 
-__global__ __launch_bounds__(GX * GY) void radj_tape_gather_tiled(TGatherArgs a, int tiles_i, int tiles_j, int kc)
+__global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherArgs a, int tiles_i, int tiles_j, int kc)
 {
     __shared__ double sW0[GY][GX + 2 * GH];
     __shared__ double sW1[GY + 2 * GH][GX];
@@ -273,24 +226,28 @@ int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const dou
     double *L0 = (double *)w.take(fb), *L1 = (double *)w.take(fb);
     double *adn = (double *)w.take(fb), *adj_s = (double *)w.take(fb);
     if (!L0 || !L1 || !adn || !adj_s) return fail(LSM_ERR_WORKSPACE, "reinit adjoint workspace too small");
-    if ((rc = launch_copy(adj_out, L0, b.n, s))) return rc;
     radj_zero<<<nblk(b.n), 256, 0, s>>>(b.n, adj_s);
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
-    double *lam = L0, *other = L1;
+    // the first stage reads adj_out itself; then the two buffers alternate
+    const double *lam = adj_out;
+    double *bufs[2] = {L0, L1};
+    int wi = 0;
     for (int it = t.nit - 1; it >= 0; --it) {
         for (int st = 2; st >= 0; --st) {
             KTimer kt(LSM_KT_REINIT_ADJOINT, s);
+            double *other = bufs[wi];
             TGatherArgs ga{b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, b.n, lam,
                            {t.q[it][st][0], t.q[it][st][1], t.q[it][st][2]}, t.code[it][st], t.T[it][st],
                            t.conv[it], t.iters, it, st + 1, p.order_bc, p.eno_bc, C[st][0], C[st][1], other, adn,
                            adj_s};
             const int tiles_i = (b.e.ni + GX - 1) / GX, tiles_j = (b.e.nj + GY - 1) / GY;
             const int kc = 8, tiles_k = (b.e.nk + kc - 1) / kc;
-            radj_tape_gather_tiled<<<(unsigned)(tiles_i * tiles_j * tiles_k), dim3(GX, GY, 1), 0, s>>>(
-                ga, tiles_i, tiles_j, kc);
-            double *tmp = lam; lam = other; other = tmp;
+            const unsigned grid = (unsigned)(tiles_i * tiles_j * tiles_k);
+            radj_tape_gather<<<grid, dim3(GX, GY, 1), 0, s>>>(ga, tiles_i, tiles_j, kc);
+            lam = other;
+            wi ^= 1;
         }
         if ((rc = check_hip(hipGetLastError(), "reinit adjoint kernels"))) return rc;
     }
