@@ -323,8 +323,8 @@ __global__ __launch_bounds__(AX * R) void adj_stage_fused(FusedArgs a)
     auto kcl = [&](int kk) -> long { return g.sk * (long)min(max(kk, 0), g.nk - 1); };
 
     __shared__ double sp[R + 4][AX + 4];      // stage input of the plane, two-node halo
-    __shared__ double Pi[5][R][AX];           // i partials (offset -2..2) of each node
-    __shared__ double Pj[5][R][AX];           // j partials
+    __shared__ double Pj[5][R][AX];           // j partials (offset -2..2) of each node; the i
+                                              // partials stay in registers and move by lane shuffles
 
     const double *__restrict__ prev = a.prev;
     double q0 = prev[col + kcl(kb - 4)], q1 = prev[col + kcl(kb - 3)], q2 = prev[col + kcl(kb - 2)];
@@ -353,6 +353,7 @@ __global__ __launch_bounds__(AX * R) void adj_stage_fused(FusedArgs a)
         }
         __syncthreads();
         const bool part = live && inb && inter_ij && p >= 1 && p <= g.nk - 2 && upd;
+        double pi0 = 0.0, pi1 = 0.0, pi2 = 0.0, pi3 = 0.0, pi4 = 0.0;
         if (part) {
             // one direction at a time, each written out before the next is formed
             const double m = lam * (-((a.c3 * ajv) * a.dt));
@@ -362,8 +363,7 @@ __global__ __launch_bounds__(AX * R) void adj_stage_fused(FusedArgs a)
             double gl[5];
             double dph = weno3_dir_grad_fast(sp[cy][cx - 2], sp[cy][cx - 1], sp[cy][cx], sp[cy][cx + 1],
                                              sp[cy][cx + 2], U[0], i == 1, i == g.ni - 2, a.dd[0], gl);
-#pragma unroll
-            for (int o = 0; o < 5; ++o) Pi[o][ty][tx] = m * gl[o];
+            pi0 = m * gl[0]; pi1 = m * gl[1]; pi2 = m * gl[2]; pi3 = m * gl[3]; pi4 = m * gl[4];
             if (own) a.aU1[l] += m * (a.dd[0] * dph);
             __builtin_amdgcn_sched_barrier(0);
             dph = weno3_dir_grad_fast(sp[cy - 2][cx], sp[cy - 1][cx], sp[cy][cx], sp[cy + 1][cx],
@@ -378,15 +378,18 @@ __global__ __launch_bounds__(AX * R) void adj_stage_fused(FusedArgs a)
             if (own) a.aU3[l] += m * (a.dd[2] * dph);
         } else {
 #pragma unroll
-            for (int o = 0; o < 5; ++o) { Pi[o][ty][tx] = 0.0; Pj[o][ty][tx] = 0.0; }
+            for (int o = 0; o < 5; ++o) Pj[o][ty][tx] = 0.0;
         }
+        // i partial o of lane tx - (o - 2) (a wavefront is one tile row)
+        const double gi[5] = {__shfl(pi0, (tx + 2) & (AX - 1)), __shfl(pi1, (tx + 1) & (AX - 1)), pi2,
+                              __shfl(pi3, (tx - 1) & (AX - 1)), __shfl(pi4, (tx - 2) & (AX - 1))};
         __syncthreads();
         double pend2 = 0.0;
         if (outq && live && p >= kb && p <= ke) {
             // partial of node tx - (o - 2) with respect to this node
             double sacc = upd ? a.c2 * lam : lam;
 #pragma unroll
-            for (int o = 0; o < 5; ++o) sacc += Pi[o][ty][tx - (o - 2)] + Pj[o][ty - (o - 2)][tx];
+            for (int o = 0; o < 5; ++o) sacc += gi[o] + Pj[o][ty - (o - 2)][tx];
             pend2 = sacc;
             if (upd && a.prev_out) a.adj_n[col + g.sk * (long)p] += a.c1 * lam;
             else if (upd) pend2 += a.c1 * lam;
