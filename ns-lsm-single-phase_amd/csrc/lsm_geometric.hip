@@ -627,6 +627,16 @@ __device__ __forceinline__ double pc_update(double pc, double D)
     return copysign(1.0, pc) * aux;
 }
 
+// the value a lane holds, the lane index being the same in every lane
+// (v_readlane: no LDS round trip)
+__device__ __forceinline__ double lane_value(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 constexpr int kBoxMax = 729;   // 9 x 9 x 9 search box
 constexpr double kSlack = 2e-14;   // D and |phicorrected| exceed a bound by at most roundings of 10
 
@@ -838,7 +848,7 @@ __global__ __launch_bounds__(64) void geo_distance(DistArgs a)
             for (int q = 0; q < cnt; ++q) {
                 const int nn = (int)((pat >> (2 * q)) & 3);
                 for (int u = 0; u < nn; ++u) {
-                    const double Da = __shfl(myDa, tix++);
+                    const double Da = lane_value(myDa, tix++);
                     const double ex = (copysign(1.0, D - Da) + 1.0) / 2.0;
                     D = D + ex * (Da - D);
                 }
