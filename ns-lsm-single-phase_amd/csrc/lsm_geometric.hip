@@ -16,7 +16,8 @@
 //   geo_cells     per cell: K-tetrahedra and their vertices (pnodes), slot
 //   geo_iso       per cell with K-tetrahedra: their isosurface triangles
 //   geo_nodes     per node: interface node (a vertex of a K-tetrahedron)
-//   geo_mark      per node: neighbour (an interface node within 4 nodes)
+//   geo_dilate,   per node: neighbour (an interface node within 4 nodes),
+//   geo_mark      from the interface flags dilated along i, j, k
 //   geo_distance  per interface node: signed distance to the isosurfaces
 //                 of the K-tetrahedra of the interface nodes within 4 nodes;
 //                 per neighbour node: min(|phicorrected|, that distance)
@@ -398,10 +399,33 @@ __device__ __forceinline__ void box(int c, int n, int &lo, int &hi)
 }
 
 // neighbours: the nodes the search boxes of GetIsosurfaceAndDistances.F90:153-240
-// visit without being interface nodes (InterfaceNodesID = -1), listed
+// visit without being interface nodes (InterfaceNodesID = -1).  The clamped
+// 9^3 box is a product of three index ranges, so "an interface node in the
+// box" is a dilation of the interface flags by 4 nodes along i, then j, then
+// k; the last pass lists the neighbours.
+__global__ __launch_bounds__(256) void geo_dilate(Geom g, long n, const uint8_t *__restrict__ src, int first,
+                                                  uint8_t *__restrict__ dst, int axis)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    int idx[3];
+    node_of(g, l, idx[0], idx[1], idx[2]);
+    const int nn[3] = {g.ni, g.nj, g.nk};
+    const long st[3] = {1, g.sj, g.sk};
+    int lo, hi;
+    box(idx[axis], nn[axis], lo, hi);
+    const long base = l - (long)idx[axis] * st[axis];
+    uint8_t m = 0;
+    for (int c = lo; c <= hi; ++c) {
+        const uint8_t v = src[base + (long)c * st[axis]];
+        if (first ? v == 1 : v != 0) { m = 1; break; }
+    }
+    dst[l] = m;
+}
+
 __global__ __launch_bounds__(256) void geo_mark(Geom g, long n, const int8_t *__restrict__ nid,
-                                                uint8_t *__restrict__ nmark, int32_t *__restrict__ nlist,
-                                                int32_t *__restrict__ count)
+                                                const uint8_t *__restrict__ dij, uint8_t *__restrict__ nmark,
+                                                int32_t *__restrict__ nlist, int32_t *__restrict__ count)
 {
     const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
     bool m = false;
@@ -409,12 +433,11 @@ __global__ __launch_bounds__(256) void geo_mark(Geom g, long n, const int8_t *__
         if (nid[l] == 0) {
             int i, j, k;
             node_of(g, l, i, j, k);
-            int i0, i1, j0, j1, k0, k1;
-            box(i, g.ni, i0, i1); box(j, g.nj, j0, j1); box(k, g.nk, k0, k1);
-            for (int ks = k0; ks <= k1 && !m; ++ks)
-                for (int js = j0; js <= j1 && !m; ++js)
-                    for (int is = i0; is <= i1; ++is)
-                        if (nid[(long)is + g.sj * js + g.sk * ks] == 1) { m = true; break; }
+            int k0, k1;
+            box(k, g.nk, k0, k1);
+            const long base = l - (long)k * g.sk;
+            for (int c = k0; c <= k1; ++c)
+                if (dij[base + (long)c * g.sk]) { m = true; break; }
         }
         nmark[l] = m ? 1 : 0;
     }
@@ -925,7 +948,7 @@ size_t geometric_work(const Blk &b)
     const size_t nc = (size_t)(b.e.ni - 1) * (b.e.nj - 1) * (b.e.nk - 1);
     const size_t n = (size_t)b.n;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    return al(n) + al(n) + al(nc) + al(nc) + al(nc * 4) + al(n * 4) * 3 + 256 + 2048;
+    return al(n) * 3 + al(nc) + al(nc) + al(nc * 4) + al(n * 4) * 3 + 256 + 2048;
 }
 
 int launch_hybrid_init(const double *phi, const double *base, double *pc, long n, hipStream_t s)
@@ -940,12 +963,12 @@ int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids,
     const long n = b.n;
     const long ncell = (long)(b.e.ni - 1) * (b.e.nj - 1) * (b.e.nk - 1);
     int8_t *nid = (int8_t *)c.take(n);
-    uint8_t *nmark = (uint8_t *)c.take(n);
+    uint8_t *nmark = (uint8_t *)c.take(n), *dil = (uint8_t *)c.take(n);
     uint8_t *ctets = (uint8_t *)c.take(ncell), *cpn = (uint8_t *)c.take(ncell);
     int32_t *cslot = (int32_t *)c.take(ncell * 4);
     int32_t *mid = (int32_t *)c.take(n * 4), *ilist = (int32_t *)c.take(n * 4), *nlist = (int32_t *)c.take(n * 4);
     int32_t *count = (int32_t *)c.take(256);
-    if (!nid || !nmark || !ctets || !cpn || !cslot || !mid || !ilist || !nlist || !count)
+    if (!nid || !nmark || !dil || !ctets || !cpn || !cslot || !mid || !ilist || !nlist || !count)
         return fail(LSM_ERR_WORKSPACE, "workspace too small");
     if (n > 0x7fffffffL) return fail(LSM_ERR_ARG, "block of %ld nodes: at most 2^31 - 1", n);
     Geom g{b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, (long)(b.e.ni - 1), (long)(b.e.ni - 1) * (b.e.nj - 1),
@@ -958,7 +981,10 @@ int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids,
     if ((rc = check_hip(hipGetLastError(), "geo_cells"))) return rc;
     geo_nodes<<<nblk(n), 256, 0, s>>>(g, n, cpn, nid, mid, ilist, count + 1);
     if ((rc = check_hip(hipGetLastError(), "geo_nodes"))) return rc;
-    geo_mark<<<nblk(n), 256, 0, s>>>(g, n, nid, nmark, nlist, count + 2);
+    // interface flags dilated along i (into nmark, as scratch), then j (into dil)
+    geo_dilate<<<nblk(n), 256, 0, s>>>(g, n, (const uint8_t *)nid, 1, nmark, 0);
+    geo_dilate<<<nblk(n), 256, 0, s>>>(g, n, nmark, 0, dil, 1);
+    geo_mark<<<nblk(n), 256, 0, s>>>(g, n, nid, dil, nmark, nlist, count + 2);
     if ((rc = check_hip(hipGetLastError(), "geo_mark"))) return rc;
     // the counts size the isosurface store and the distance launch
     int32_t cnt[3] = {0, 0, 0};
