@@ -671,7 +671,7 @@ def main():
         log(f"levelsetmethod (4 reinit iterations): {ls_ms:.2f} ms per call")
 
     if not args.no_extra:
-        extra["geometric"] = bench_geometric(g, bufs[base], blk, vel, dt)
+        extra["geometric"] = bench_geometric(g, torch.as_tensor(phi_np, device="cuda"), blk, vel, dt)
         log(f"geometric: {extra['geometric']}")
     if not args.no_extra:
         extra["consumers"] = bench_consumers(g, bufs[base], blk)
