@@ -165,27 +165,38 @@ def _seeded(kind, shape):
     elif kind == "channel":
         g = inputs.channel_grid(ni, nj, nk)
         phi = inputs.solitary_phi(g, H=0.35)
+    elif kind == "noise":                          # every node an interface node, slivers
+        g = inputs.sinusoidal_cube_grid(ni, amp=0.03, seed=24, nj=nj, nk=nk)
+        phi = np.random.default_rng(25).uniform(-0.5, 0.5, g.shape)
+    elif kind == "steps":                          # values in {-2..2}: exact zeros everywhere
+        g = inputs.uniform_grid(ni, nj, nk, h=0.37)
+        phi = np.random.default_rng(26).integers(-2, 3, g.shape).astype(np.float64)
     else:                                          # zero on node layers and diagonals
         g = inputs.uniform_grid(ni, nj, nk, h=1.0 / 32)
         phi = np.minimum(g.z - 10.0 / 32, g.x - g.y + 3.0 / 32)
     return g, phi
 
 
-GEO = [("spheres", (67, 45, 38)), ("spheres", (64, 64, 64)), ("channel", (150, 9, 41)),
-       ("nodes", (40, 33, 29))]
+GEO = [("spheres", (67, 45, 38), 1.0), ("spheres", (64, 64, 64), 1.0), ("channel", (150, 9, 41), 1.0),
+       ("nodes", (40, 33, 29), 1.0), ("noise", (23, 19, 17), 1.0), ("steps", (21, 20, 19), 1.0),
+       ("spheres", (31, 29, 27), 1.3)]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind,shape", GEO, ids=[f"{k}-{s[0]}x{s[1]}x{s[2]}" for k, s in GEO])
-def test_gpu_geometric_matches_oracle(kind, shape):
+@pytest.mark.parametrize("kind,shape,scale", GEO, ids=[f"{k}-{s[0]}x{s[1]}x{s[2]}-{c}" for k, s, c in GEO])
+def test_gpu_geometric_matches_oracle(kind, shape, scale):
     """geometric_reinitialisation alone on larger blocks: phicorrected and
     InterfaceNodesID equal the oracle bit for bit (the isosurface store grows
-    and is reused across these calls)."""
+    and is reused across these calls).  The distance search skips triangles
+    by error-bounded lower bounds; noise (sliver triangles, every node an
+    interface node), exact zeros everywhere and |phicorrected| above 10
+    (scale 1.3: skips that are valid only below 10 must not happen) are the
+    cases where a loose bound would show."""
     import torch
     from lsm_amd import api
-    print(f"[geometric] oracle parity {kind} {shape}", flush=True)
+    print(f"[geometric] oracle parity {kind} {shape} scale {scale}", flush=True)
     g, phi = _seeded(kind, shape)
-    pc0 = _initial(phi, base=phi)
+    pc0 = _initial(phi, base=phi) * scale
     ref, ref_ids = O.geometric_reinit(O.Block(g), phi, pc0)
     blk = api.Block(g)
     pc = _dev(pc0)
