@@ -112,18 +112,30 @@ def targets(cb: ChannelBatch, H_target, nsteps: int = 60, every: int = 10, mode:
     return out
 
 
-def loss_and_gradients(cb: ChannelBatch, eta_t, nsteps: int = 60, every: int = 10, mode: str = "fast"):
+def loss_and_gradients(cb: ChannelBatch, eta_t, nsteps: int = 60, every: int = 10, mode: str = "fast",
+                       streams: int = 4):
     """J = sum_b mean_columns (eta_b - eta*_b)^2 and, per instance, dJ/dphi0
-    (a field) and dJ/dH (total: through phi0(H) and u(H))."""
-    J = 0.0
-    g_phi, g_H = [], []
+    (a field) and dJ/dH (total: through phi0(H) and u(H)).  The instances'
+    forward and backward passes are issued round-robin on `streams` HIP streams
+    (default priority), so small instances run side by side; the backward of
+    each runs on the stream of its forward."""
+    main = torch.cuda.current_stream()
+    pool = [torch.cuda.Stream() for _ in range(max(1, int(streams)))]
+    for st in pool:
+        st.wait_stream(main)
+    Js, g_phi, Hs = [], [], []
     for b, blk in enumerate(cb.blocks):
-        H = torch.tensor(cb.specs[b].H, dtype=torch.float64, device=blk.device, requires_grad=True)
-        dphi = blk.zeros().requires_grad_(True)
-        eta = elevation(cb, b, instance_forward(cb, b, H, dphi, nsteps, every, mode))
-        Jb = ((eta - eta_t[b]) ** 2).mean()
-        Jb.backward()
-        J += float(Jb.detach())
-        g_phi.append(dphi.grad)
-        g_H.append(float(H.grad))
-    return J, g_phi, g_H
+        with torch.cuda.stream(pool[b % len(pool)]):
+            H = torch.tensor(cb.specs[b].H, dtype=torch.float64, device=blk.device, requires_grad=True)
+            dphi = blk.zeros().requires_grad_(True)
+            eta = elevation(cb, b, instance_forward(cb, b, H, dphi, nsteps, every, mode))
+            Jb = ((eta - eta_t[b]) ** 2).mean()
+            Jb.backward()
+            Js.append(Jb.detach())
+            g_phi.append(dphi.grad)
+            Hs.append(H)
+    for st in pool:
+        main.wait_stream(st)
+    torch.cuda.synchronize()
+    J = float(sum(float(j) for j in Js))
+    return J, g_phi, [float(H.grad) for H in Hs]
