@@ -113,7 +113,7 @@ def targets(cb: ChannelBatch, H_target, nsteps: int = 60, every: int = 10, mode:
 
 
 def loss_and_gradients(cb: ChannelBatch, eta_t, nsteps: int = 60, every: int = 10, mode: str = "fast",
-                       streams: int = 4):
+                       streams: int = 8):
     """J = sum_b mean_columns (eta_b - eta*_b)^2 and, per instance, dJ/dphi0
     (a field) and dJ/dH (total: through phi0(H) and u(H)).  The instances'
     forward and backward passes are issued round-robin on `streams` HIP streams
