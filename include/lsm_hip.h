@@ -21,8 +21,11 @@
  *  - Every call returns LSM_OK (0) or a negative LSM_ERR_* code; the message
  *    of the last error on the calling thread is in lsm_last_error().
  *  - `work` arguments are device scratch of at least lsm_workspace_bytes()
- *    bytes; NULL lets the library use its own per-device buffer (allocated
- *    once, on first use).
+ *    bytes; NULL lets the library use its own buffer for the calling
+ *    (device, stream) pair (allocated on first use, grown on demand, kept
+ *    until exit).  Calls on different streams never share library-owned
+ *    memory, so they may run concurrently; calls on one stream from several
+ *    host threads at once must be serialised by the caller.
  */
 #ifndef LSM_HIP_H
 #define LSM_HIP_H
@@ -201,7 +204,8 @@ int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizer
  * sign of phicorrected; other nodes keep their values.  ids (device int32,
  * may be NULL) = 1 interface node, -1 such a neighbour, 0 other.  Needs the
  * coordinates x, y, z of blk.  Synchronises the stream once (the number of
- * cut cells sizes the isosurface store, which the library keeps per device).
+ * cut cells sizes the isosurface store, which the library keeps per
+ * (device, stream) pair).
  * Workspace: lsm_workspace_bytes(blk, LSM_OP_REINIT_GEOMETRIC). */
 int lsm_reinit_geometric(const LsmBlock *blk, const double *phi0, double *phicorrected, int32_t *ids,
                          void *work, size_t work_bytes, void *stream);
@@ -425,7 +429,10 @@ int lsm_batch_autotune(LsmBatch *batch, double *phi_np1, const double *phi_n, co
 /* Records one lsm_batch_levelset_step (with the current configuration and
  * these arguments, which the graph keeps; p = NULL: advection only) into a
  * HIP graph owned by the batch; lsm_batch_graph_launch replays it on any
- * stream.  Needs a workspace and launch timing switched off. */
+ * stream.  Needs a workspace and launch timing switched off.  Any later
+ * reconfiguration (lsm_batch_set_config with another configuration,
+ * lsm_batch_set_reinit_tile, lsm_batch_autotune) drops the graph: the
+ * launch then fails with LSM_ERR_ARG until the step is captured again. */
 int lsm_batch_graph_capture(LsmBatch *batch, const LsmReinitParams *p, double *phi_np1,
                             const double *phi_n, const double *ucn1, const double *ucn2,
                             const double *ucn3, double delti_lsm, int mode, void *work,

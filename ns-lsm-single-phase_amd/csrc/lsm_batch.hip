@@ -626,9 +626,20 @@ Range block_range(const LsmBatchBlock &d, int g)
     return r;
 }
 
+// A captured step holds the device table pointers, grid sizes and queue
+// bounds of the configuration it was recorded with: any reconfiguration
+// drops it (lsm_batch_graph_launch then fails until the next capture).  The
+// caller has synchronised the device, so no replay is in flight.
+void drop_graph(LsmBatch *B)
+{
+    if (B->gexec) { (void)hipGraphExecDestroy(B->gexec); B->gexec = nullptr; }
+    if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
+}
+
 // (re)builds the tile tables for tiles of w columns, ty rows and kc planes
 int build_tiles(LsmBatch *B, int w, int ty, int kc)
 {
+    drop_graph(B);
     std::vector<int32_t> tb;
     for (int b = 0; b < B->nb; ++b) {
         BBlock &X = B->hb[b];
@@ -783,6 +794,7 @@ int upload(const std::vector<T> &v, T **dst)
 // the bcond_lsm face segments of each axis
 int build_reinit_tables(LsmBatch *B)
 {
+    drop_graph(B);
     const int W = B->rt_w, TY = 256 / W;
     std::vector<int32_t> rt, sh;
     std::vector<HSeg> bs[3];
@@ -1170,6 +1182,10 @@ int lsm_batch_set_config(LsmBatch *B, int form, int tile_cols, int tile_rows, in
         // the device tables may still be read by queued work
         if ((rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))) return rc;
         if ((rc = build_tiles(B, tile_cols, tile_rows, kchunk))) return rc;
+    } else if (B->gexec && (form != B->tune.form || wg_per_cu != B->tune.wg_per_cu)) {
+        // a replay may be in flight
+        if ((rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))) return rc;
+        drop_graph(B);
     }
     B->tune.form = form;
     B->tune.wg_per_cu = wg_per_cu;
@@ -1242,7 +1258,7 @@ int lsm_batch_rk3_step(LsmBatch *B, double *phi_np1, const double *phi_n, const 
     const size_t need = lsm_batch_workspace_bytes(B, LSM_OP_RK3);
     void *scr = work;
     if (!scr) {
-        int rc = default_work(need, &scr);
+        int rc = default_work(need, (hipStream_t)stream, &scr);
         if (rc) return rc;
     } else if (work_bytes < need) {
         return fail(LSM_ERR_WORKSPACE, "workspace of %zu bytes, %zu needed", work_bytes, need);
@@ -1387,9 +1403,10 @@ int lsm_batch_graph_capture(LsmBatch *B, const LsmReinitParams *p, double *phi_n
     if (timing_on()) return fail(LSM_ERR_UNSUPPORTED, "switch launch timing off before a graph capture");
     int rc;
     hipStream_t cs;
+    // a replay of the previous graph may be in flight
+    if (B->gexec && (rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))) return rc;
+    drop_graph(B);
     if ((rc = check_hip(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate"))) return rc;
-    if (B->gexec) { (void)hipGraphExecDestroy(B->gexec); B->gexec = nullptr; }
-    if (B->graph) { (void)hipGraphDestroy(B->graph); B->graph = nullptr; }
     rc = check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     int rc2 = LSM_OK;
     if (!rc) {

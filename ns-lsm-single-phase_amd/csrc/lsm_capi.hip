@@ -4,7 +4,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "lsm_internal.hpp"
@@ -62,36 +64,46 @@ int make_blk(const LsmBlock *b, Blk *o, bool need_metrics, bool need_coords)
     return LSM_OK;
 }
 
-// one cached scratch buffer per device, grown on demand and kept until exit
-static std::mutex g_work_mu;
-static void *g_work[64];
-static size_t g_work_size[64];
+// Library-owned device buffers, one per (device, stream, purpose), grown on
+// demand and kept until exit.  Keying by stream lets calls on different
+// streams run concurrently without sharing scratch; calls on one stream are
+// ordered by the stream itself (several host threads issuing calls on the
+// same stream at once must serialise them).
+static std::mutex g_cache_mu;
+static std::map<std::tuple<int, uintptr_t, int>, std::pair<void *, size_t>> g_cache;
 
-int default_work(size_t bytes, void **ptr)
+int cached_buffer(int purpose, size_t bytes, hipStream_t s, void **ptr)
 {
     int dev = 0;
     int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
     if (rc) return rc;
-    if (dev < 0 || dev >= 64) return fail(LSM_ERR_ARG, "device %d out of range", dev);
-    std::lock_guard<std::mutex> lk(g_work_mu);
-    if (g_work_size[dev] < bytes) {
-        if (g_work[dev]) {
-            // the buffer may still be in use by queued work of earlier calls
-            rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto &e = g_cache[std::make_tuple(dev, (uintptr_t)s, purpose)];
+    if (e.second < bytes) {
+        if (e.first) {
+            // the buffer may still be in use by queued work of earlier calls on this stream
+            rc = check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
             if (rc) return rc;
-            (void)hipFree(g_work[dev]);
-            g_work[dev] = nullptr;
-            g_work_size[dev] = 0;
+            (void)hipFree(e.first);
+            e.first = nullptr;
+            e.second = 0;
         }
-        rc = check_hip(hipMalloc(&g_work[dev], bytes), "hipMalloc(workspace)");
+        const size_t want = purpose == kBufStore ? bytes + bytes / 4 + 4096 : bytes;
+        rc = check_hip(hipMalloc(&e.first, want), purpose == kBufStore ? "hipMalloc(isosurface store)"
+                                                                        : "hipMalloc(workspace)");
         if (rc) return rc;
-        g_work_size[dev] = bytes;
+        e.second = want;
     }
-    *ptr = g_work[dev];
+    *ptr = e.first;
     return LSM_OK;
 }
 
-static int workspace(void *work, size_t work_bytes, size_t need, Carver *c)
+int default_work(size_t bytes, hipStream_t s, void **ptr)
+{
+    return cached_buffer(kBufWork, bytes, s, ptr);
+}
+
+static int workspace(void *work, size_t work_bytes, size_t need, Carver *c, hipStream_t s)
 {
     if (need == 0) need = 256;
     if (work) {
@@ -101,7 +113,7 @@ static int workspace(void *work, size_t work_bytes, size_t need, Carver *c)
         return LSM_OK;
     }
     void *p = nullptr;
-    int rc = default_work(need, &p);
+    int rc = default_work(need, s, &p);
     if (rc) return rc;
     *c = Carver{(char *)p, need, 0};
     return LSM_OK;
@@ -340,7 +352,7 @@ int lsm_rk3_step(const LsmBlock *blk, double *phi_np1, const double *phi_n, cons
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT && mode != LSM_ADVECT_UPWIND1)
         return fail(LSM_ERR_ARG, "bad mode %d", mode);
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3), &c, (hipStream_t)stream);
     if (rc) return rc;
     double *scratch = (double *)c.take(b.n * sizeof(double));
     if (!scratch) return fail(LSM_ERR_WORKSPACE, "workspace too small");
@@ -360,7 +372,7 @@ int lsm_advect_fold(const LsmBlock *blk, const double *ucn1, const double *ucn2,
         return fail(LSM_ERR_UNSUPPORTED, "folded advection needs delti_lsm > 0 and dc, de, dz > 0");
     hipStream_t s = (hipStream_t)stream;
     void *p = nullptr;
-    if ((rc = default_work(256, &p))) return rc;
+    if ((rc = default_work(256, s, &p))) return rc;
     int *bad = (int *)p;
     if ((rc = check_hip(hipMemsetAsync(bad, 0, sizeof(int), s), "hipMemsetAsync"))) return rc;
     if ((rc = launch_fold(b, ucn1, ucn2, ucn3, delti_lsm, v1, v2, v3, bad, s))) return rc;
@@ -383,7 +395,7 @@ int lsm_rk3_step_folded(const LsmBlock *blk, double *phi_np1, const double *phi_
     if (phi_np1 == phi_n) return fail(LSM_ERR_ARG, "phi_np1 and phi_n must not alias");
     if (!b.aj) return fail(LSM_ERR_ARG, "aj is NULL");
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3), &c, (hipStream_t)stream);
     if (rc) return rc;
     double *scratch = (double *)c.take(b.n * sizeof(double));
     if (!scratch) return fail(LSM_ERR_WORKSPACE, "workspace too small");
@@ -408,7 +420,7 @@ int lsm_rk3_step_adjoint(const LsmBlock *blk, const double *phi_n, const int32_t
         return fail(LSM_ERR_UNSUPPORTED, "orderAD = 0 (LSM_ADVECT_UPWIND1): this entry point covers the WENO3 right-hand side only");
     if (mode != LSM_ADVECT_FAST && mode != LSM_ADVECT_EXACT) return fail(LSM_ERR_ARG, "bad mode %d", mode);
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3_ADJOINT), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_RK3_ADJOINT), &c, (hipStream_t)stream);
     if (rc) return rc;
     return launch_rk3_adjoint(b, phi_n, adv_mask, ucn1, ucn2, ucn3, delti_lsm, mode, adj_np1, adj_n,
                               adj_ucn1, adj_ucn2, adj_ucn3, c, (hipStream_t)stream);
@@ -457,7 +469,7 @@ int lsm_reinit_geometric(const LsmBlock *blk, const double *phi0, double *phicor
     if (!phi0 || !phicorrected) return fail(LSM_ERR_ARG, "NULL array");
     if (phi0 == phicorrected) return fail(LSM_ERR_ARG, "phi0 and phicorrected must not alias");
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_GEOMETRIC), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_GEOMETRIC), &c, (hipStream_t)stream);
     if (rc) return rc;
     return launch_geometric(b, phi0, phicorrected, ids, c, (hipStream_t)stream);
 }
@@ -524,7 +536,7 @@ int lsm_reinit_sussman_eno2(const LsmBlock *blk, const LsmReinitParams *p, const
     if (!phi0 || !phi_out) return fail(LSM_ERR_ARG, "NULL array");
     if (phi0 == phi_out) return fail(LSM_ERR_ARG, "phi0 and phi_out must not alias");
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_ENO2), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_ENO2), &c, (hipStream_t)stream);
     if (rc) return rc;
     return launch_reinit_eno2(b, *p, phi0, phi_out, iters_done, c, (hipStream_t)stream);
 }
@@ -545,7 +557,7 @@ int lsm_reinit_sussman_eno2_adjoint(const LsmBlock *blk, const LsmReinitParams *
     if (phi_out && (phi_out == phi0 || phi_out == adj_out || phi_out == adj_phi0))
         return fail(LSM_ERR_ARG, "phi_out must not alias the other arrays");
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_ENO2_ADJOINT, p->rn_itermax), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_ENO2_ADJOINT, p->rn_itermax), &c, (hipStream_t)stream);
     if (rc) return rc;
     return launch_reinit_eno2_adjoint(b, *p, phi0, adj_out, adj_phi0, phi_out, c, (hipStream_t)stream);
 }
@@ -589,7 +601,7 @@ int lsm_levelset_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, con
     if ((rc = check_adj_args(adj_np1, adj_n, adj_ucn1, adj_ucn2, adj_ucn3))) return rc;
     hipStream_t s = (hipStream_t)stream;
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_STEP_ADJOINT, p->rn_itermax), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_STEP_ADJOINT, p->rn_itermax), &c, (hipStream_t)stream);
     if (rc) return rc;
     // forward, recorded into the first part of the workspace, then the reverse sweep
     StepTrace T;
@@ -633,7 +645,7 @@ int lsm_levelset_step_adjoint_recorded(const LsmBlock *blk, const LsmReinitParam
     if (!trace) return fail(LSM_ERR_ARG, "NULL trace");
     if (trace_bytes < step_trace_bytes(b, p->rn_itermax)) return fail(LSM_ERR_WORKSPACE, "trace too small");
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_STEP_ADJOINT_RECORDED, p->rn_itermax), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_STEP_ADJOINT_RECORDED, p->rn_itermax), &c, (hipStream_t)stream);
     if (rc) return rc;
     StepTrace T;
     Carver tr{(char *)const_cast<void *>(trace), trace_bytes, 0};
@@ -681,7 +693,7 @@ int lsm_reinit_godunov(const LsmBlock *blk, const LsmReinitParams *p, const doub
     if (!phi0 || !phi_out) return fail(LSM_ERR_ARG, "NULL array");
     if (phi0 == phi_out) return fail(LSM_ERR_ARG, "phi0 and phi_out must not alias");
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_GODUNOV), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_GODUNOV), &c, (hipStream_t)stream);
     if (rc) return rc;
     return launch_reinit_godunov(b, *p, phi0, phi_out, c, (hipStream_t)stream);
 }
@@ -694,7 +706,7 @@ int lsm_water_volume(const LsmBlock *blk, const double *phi, double *vol_out, in
     if (rc) return rc;
     if (!phi || !vol_out) return fail(LSM_ERR_ARG, "NULL array");
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_WATER_VOLUME), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_WATER_VOLUME), &c, (hipStream_t)stream);
     if (rc) return rc;
     return launch_water_volume(b, phi, vol_out, adv_mask, c, (hipStream_t)stream);
 }
@@ -719,7 +731,7 @@ int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p, double *phi
     if (do_reinit && !hybrid && (rc = check_reinit(p))) return rc;
     hipStream_t s = (hipStream_t)stream;
     Carver c;
-    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_LEVELSET_STEP), &c);
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_LEVELSET_STEP), &c, (hipStream_t)stream);
     if (rc) return rc;
     const size_t fb = b.n * sizeof(double);
     double *U1 = (double *)c.take(fb), *U2 = (double *)c.take(fb), *U3 = (double *)c.take(fb);

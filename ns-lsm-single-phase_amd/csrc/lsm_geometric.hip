@@ -912,31 +912,10 @@ __global__ void hybrid_init(const double *__restrict__ phi, const double *__rest
     pc[l] = v;
 }
 
-// the isosurface store: one per device, grown on demand, kept until exit
-std::mutex g_store_mu;
-void *g_store[64];
-size_t g_store_size[64];
-
-int store_take(size_t bytes, void **p)
+// the isosurface store: one per (device, stream), grown on demand, kept until exit
+int store_take(size_t bytes, hipStream_t s, void **p)
 {
-    int dev = 0;
-    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
-    if (rc) return rc;
-    if (dev < 0 || dev >= 64) return fail(LSM_ERR_ARG, "device %d out of range", dev);
-    std::lock_guard<std::mutex> lk(g_store_mu);
-    if (g_store_size[dev] < bytes) {
-        if (g_store[dev]) {
-            if ((rc = check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize"))) return rc;
-            (void)hipFree(g_store[dev]);
-            g_store[dev] = nullptr;
-            g_store_size[dev] = 0;
-        }
-        const size_t want = bytes + bytes / 4 + 4096;
-        if ((rc = check_hip(hipMalloc(&g_store[dev], want), "hipMalloc(isosurface store)"))) return rc;
-        g_store_size[dev] = want;
-    }
-    *p = g_store[dev];
-    return LSM_OK;
+    return cached_buffer(kBufStore, bytes, s, p);
 }
 
 unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
@@ -1003,7 +982,7 @@ int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids,
         const size_t o_box = o_emin + al((size_t)nint * sizeof(double));
         const size_t total = o_box + al((size_t)nint * kMB * sizeof(double));
         void *store = nullptr;
-        if ((rc = store_take(total, &store))) return rc;
+        if ((rc = store_take(total, s, &store))) return rc;
         char *sb = (char *)store;
         int8_t *nuse = (int8_t *)sb;
         double *trec = (double *)(sb + o_trec);

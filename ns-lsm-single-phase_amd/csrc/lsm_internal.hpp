@@ -30,7 +30,10 @@ int check_hip(hipError_t e, const char *what);
 int make_blk(const LsmBlock *b, Blk *out, bool need_metrics, bool need_coords);
 
 // library-owned per-device scratch (used when the caller passes work = NULL)
-int default_work(size_t bytes, void **ptr);
+// library-owned buffers per (device, stream, purpose)
+enum { kBufWork = 0, kBufStore = 1 };
+int cached_buffer(int purpose, size_t bytes, hipStream_t s, void **ptr);
+int default_work(size_t bytes, hipStream_t s, void **ptr);
 
 // carve `n` bytes, 256-byte aligned, from a workspace cursor
 struct Carver {

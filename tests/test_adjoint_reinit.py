@@ -27,9 +27,11 @@ import oracle as O  # noqa: E402
 import torch_ref as T  # noqa: E402
 from lsm_amd import inputs  # noqa: E402
 
-# (shape, OrderReinitialisationBoundaries, ENOBCReinitialisation, iterations)
+# (shape, OrderReinitialisationBoundaries, ENOBCReinitialisation, iterations);
+# the last two span two or three 64-node i tiles of the tape gather (partial
+# last tile) with more than 8 k planes
 CASES = [((12, 10, 9), 1, False, 4), ((11, 9, 8), 2, True, 3), ((10, 12, 11), 3, False, 4),
-         ((9, 8, 10), 2, False, 2)]
+         ((9, 8, 10), 2, False, 2), ((133, 9, 10), 1, False, 3), ((70, 12, 11), 2, True, 2)]
 
 
 def _field(shape, seed=81):
@@ -186,11 +188,13 @@ def _step_case(shape, seed):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["exact", "fast"])
-def test_step_adjoint_matches_autograd(mode):
+@pytest.mark.parametrize("shape", [(12, 11, 10), (75, 9, 10)])
+def test_step_adjoint_matches_autograd(mode, shape):
     """Two full steps: the HIP adjoint of each against autograd of the
-    re-expression (rk3_step -> reinit -> bcond), for phi and U/J."""
+    re-expression (rk3_step -> reinit -> bcond), for phi and U/J; the second
+    shape spans two i tiles of the adjoint stage and tape-gather kernels."""
     from lsm_amd import autograd
-    g, phi, vel, dt, hmin, hmax = _step_case((12, 11, 10), 131)
+    g, phi, vel, dt, hmin, hmax = _step_case(shape, 131)
     btype = (1, 0, 1, 2, 1, 1)
     api, blk, dev = _gpu_setup(g, btype, 5.0)
     ob = O.Block(g, btype=btype, big_phi=5.0)

@@ -238,6 +238,41 @@ def test_graph_replays_and_autotune():
     assert cfg2 == cfg and ms2 is None
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("reconf", ["reinit_tile", "tiles", "form", "autotune"])
+def test_reconfiguration_drops_the_graph(reconf):
+    """A captured step holds the device tables of its configuration; after a
+    reconfiguration the batch refuses to replay it (the launch fails on the
+    host, no kernel runs) until the step is captured again."""
+    from lsm_amd._lib import LsmError
+    lay, grids, phi, vel, _ = _case(lattice=(2, 2, 1), seed=16)
+    dt = B.cfl_dt(lay, grids, vel)
+    bt = B.Batch(lay, grids)
+    ucn = bt.contra_velocity(*[bt.pack([v[c] for v in vel]) for c in range(3)])
+    p0 = bt.halo_fill(bt.pack(phi))
+    bt.set_config("grid", 4, 8, 0)
+    out = bt.empty()
+    bt.graph_capture(out, p0, ucn, dt)
+    bt.graph_launch()
+    torch.cuda.synchronize()
+    if reconf == "reinit_tile":
+        bt.set_reinit_tile(32)
+    elif reconf == "tiles":
+        bt.set_config("grid", 8, 8, 0)
+    elif reconf == "form":
+        bt.set_config("queue", 4, 8, 0)
+    else:
+        bt.autotune(p0, ucn, dt, reps=1, force=True)
+    with pytest.raises(LsmError):
+        bt.graph_launch()
+    ref = B.assemble(lay, bt.unpack(bt.rk3_step(p0, ucn, dt)))
+    bt.graph_capture(out, p0, ucn, dt)
+    out.fill_(0.0)
+    bt.graph_launch()
+    torch.cuda.synchronize()
+    assert np.array_equal(B.assemble(lay, bt.unpack(out)), ref)
+
+
 def _params(epslsm, dtau, nit=4, eps=1e-9, order=1, eno=False):
     from lsm_amd import api
     return api.ReinitParams(epslsm=epslsm, RNtimestep=dtau, RNitermax=nit, epsReinitialisation=eps,

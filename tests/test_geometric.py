@@ -246,3 +246,56 @@ def test_gpu_hybrid_rejected_where_unsupported():
     with pytest.raises(LsmError, match="hybrid"):
         api.levelset_step_adjoint(blk, _dev(phi), U, 1e-3, prm, _dev(np.ones(g.shape)))
     torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+def test_concurrent_streams_do_not_share_library_memory():
+    """Two geometric reinitialisations (isosurface store kept by the library)
+    and two Sussman reinitialisations with a NULL workspace (library scratch),
+    issued on two streams of default priority at once, give the results of
+    the same calls run one after the other, bit for bit."""
+    import ctypes as C
+    import torch
+    from lsm_amd import _lib, api
+    L = _lib.load()
+    cases = []
+    for q, n in enumerate((40, 46)):
+        g = inputs.sinusoidal_cube_grid(n, amp=0.03, seed=300 + q, nj=n - 3, nk=n - 6)
+        phi = inputs.spheres_sdf(g, nspheres=5, seed=310 + q, rmin=0.1, rmax=0.25)
+        hmin, hmax = inputs.spacing_bounds(g)
+        cases.append((api.Block(g), phi, api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4)))
+
+    def run(blk, phi, prm, stream):
+        p0 = torch.as_tensor(phi, device="cuda")
+        pc = torch.full_like(p0, 9.9) * torch.sign(p0)
+        ids = torch.zeros(blk.shape, dtype=torch.int32, device="cuda")
+        out = blk.empty()
+        it = torch.zeros(1, dtype=torch.int32, device="cuda")
+        s = C.c_void_p(stream.cuda_stream)
+        ws = blk.workspace(_lib.LSM_OP_REINIT_GEOMETRIC)
+        torch.cuda.current_stream().synchronize()   # inputs written on torch's stream
+        _lib.check(L.lsm_reinit_geometric(blk.ref, C.c_void_p(p0.data_ptr()), C.c_void_p(pc.data_ptr()),
+                                          C.c_void_p(ids.data_ptr()), C.c_void_p(ws.data_ptr()), ws.numel(), s),
+                   "lsm_reinit_geometric")
+        c = prm.c()
+        _lib.check(L.lsm_reinit_sussman_eno2(blk.ref, C.byref(c), C.c_void_p(p0.data_ptr()),
+                                             C.c_void_p(out.data_ptr()), C.c_void_p(it.data_ptr()), None, 0, s),
+                   "lsm_reinit_sussman_eno2")
+        return pc, ids, out, it, ws, p0
+
+    s0 = torch.cuda.Stream()
+    s1 = torch.cuda.Stream()
+    ser = []
+    for (blk, phi, prm) in cases:
+        r = run(blk, phi, prm, s0)
+        s0.synchronize()
+        ser.append([t.cpu() for t in r[:4]])
+    torch.cuda.synchronize()
+    for rep in range(3):
+        print(f"[geometric] two streams, repetition {rep}", flush=True)
+        r0 = run(*cases[0], s0)
+        r1 = run(*cases[1], s1)
+        torch.cuda.synchronize()
+        for got, ref in ((r0, ser[0]), (r1, ser[1])):
+            for a, b in zip(got[:4], ref):
+                assert torch.equal(a.cpu(), b), rep
