@@ -437,7 +437,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle-s", type=float, default=2.0,
+                    help="untimed steps run for this many seconds before the W warm-up steps, so the "
+                         "GPU clocks have come up when the timed region starts (reported in extra)")
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--mode", default="fast", choices=["fast", "exact"])
     ap.add_argument("--data-model", default="reference", choices=["folded", "reference"],
@@ -523,6 +526,20 @@ def main():
             return lambda src, dst: api.ad_RKTVD3_folded(blk, src, V, out=dst, work=work)
         return lambda src, dst: api.ad_RKTVD3(blk, src, ucn, dt, out=dst, mode=kind, work=work)
 
+    def settle(kind, seconds):
+        # clock settle: untimed steps until `seconds` have passed (a short
+        # warm-up leaves the first timed steps 10-20 % slow while the clocks
+        # come up)
+        step = make_step(kind)
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < seconds:
+            for _ in range(20):
+                step(bufs[n % 2], bufs[(n + 1) % 2])
+                n += 1
+            torch.cuda.synchronize()
+        return n, time.perf_counter() - t0
+
     def timed(kind, steps, warmup, instrument=True):
         step = make_step(kind)
         for s in range(warmup):
@@ -550,6 +567,7 @@ def main():
     # the K timed steps run without per-kernel events; a second pass of K
     # steps right after, with the library's HIP events around every launch,
     # gives the kernels' average durations
+    settle_steps, settle_s = settle(head_kind, args.settle_s) if args.settle_s > 0 else (0, 0.0)
     ms_total, wall, _, _, _, _, last = timed(head_kind, args.steps, args.warmup, instrument=False)
     ms_inst, _, kms, kn, sms, sn, last = timed(head_kind, args.steps, 0, instrument=True)
     base = last
@@ -576,7 +594,9 @@ def main():
     log(f"{args.steps} steps: {ms_per_step:.3f} ms/step, {value:.3e} cell-updates/s, "
         f"stage kernel {stage_ms:.4f} ms ({achieved:.0f} GB/s alg), shell {sms.value / max(sn.value, 1):.4f} ms")
 
-    extra = {"wall_s_timed_region": round(wall, 4), "shell_avg_ms": round(sms.value / max(sn.value, 1), 5),
+    extra = {"clock_settle": {"seconds": round(settle_s, 2), "untimed_steps": settle_steps,
+                              "note": "untimed steps before the W warm-up steps (GPU clocks up)"},
+             "wall_s_timed_region": round(wall, 4), "shell_avg_ms": round(sms.value / max(sn.value, 1), 5),
              "data_model": head_kind, "instrumented_ms_per_step": round(ms_inst / args.steps, 5)}
     if not args.no_extra:
         # the other arithmetic / data-model variants of the same step
