@@ -297,6 +297,18 @@ int lsm_reinit_godunov(const LsmBlock *blk, const LsmReinitParams *p,
 int lsm_water_volume(const LsmBlock *blk, const double *phi, double *vol_out,
                      int32_t *adv_mask, void *work, size_t work_bytes, void *stream);
 
+/* No counterpart in the reference: lsm_water_volume for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same results as lsm_water_volume; no workspace
+ * argument and no allocation of any kind: its scratch is a fixed 32 MiB device
+ * array of the library (one per GPU, held until the process ends).  A block
+ * whose scratch (24 + 4 bytes per 64 x 4 x 8-cell tile, plus one byte per node
+ * when adv_mask is asked for) exceeds it is rejected with LSM_ERR_WORKSPACE
+ * before any device work.  Every call on one GPU shares that array: calls
+ * that overlap on one GPU (several streams or host threads at once) race. */
+int lsm_water_volume_noalloc(const LsmBlock *blk, const double *phi, double *vol_out,
+                             int32_t *adv_mask, void *stream);
+
 /* levelsetmethod.F90:1-372 (levelsetmethod, non-hybrid Sussman path, no
  * global mass correction): pre-advection water volume (vol_out, device, 3
  * doubles, may be NULL), contravariant velocity from q, numiter_lsm RK3 steps

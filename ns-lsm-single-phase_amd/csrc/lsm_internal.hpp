@@ -29,7 +29,6 @@ int check_hip(hipError_t e, const char *what);
 // device pointers must be present
 int make_blk(const LsmBlock *b, Blk *out, bool need_metrics, bool need_coords);
 
-// library-owned per-device scratch (used when the caller passes work = NULL)
 // library-owned buffers per (device, stream, purpose)
 enum { kBufWork = 0, kBufStore = 1 };
 int cached_buffer(int purpose, size_t bytes, hipStream_t s, void **ptr);
@@ -160,7 +159,8 @@ int launch_bcond_adjoint(const Blk &b, const double *phi, double *adj, hipStream
 
 // ---- launchers (lsm_volume.hip) ----
 size_t water_volume_work(const Blk &b);
+// timed = false: no launch timing events (the allocation-free entry point)
 int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *mask, Carver &w,
-                        hipStream_t s);
+                        hipStream_t s, bool timed = true);
 
 }  // namespace lsm

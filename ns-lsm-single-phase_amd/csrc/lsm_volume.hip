@@ -445,15 +445,22 @@ long vol_blocks(const Blk &b)
 
 }  // namespace
 
-size_t water_volume_work(const Blk &b)
+// scratch of one evaluation: per-workgroup partials and cut flags, and the
+// per-node change flags of the mask when the mask is asked for
+static size_t volume_scratch_bytes(const Blk &b, bool with_mask)
 {
     const size_t nb = (size_t)vol_blocks(b);
     return ((nb * 3 * sizeof(double) + 255) & ~(size_t)255) + ((nb * sizeof(int32_t) + 255) & ~(size_t)255) +
-           (((size_t)b.n + 255) & ~(size_t)255) + 1024;
+           (with_mask ? (((size_t)b.n + 255) & ~(size_t)255) : 0) + 1024;
+}
+
+size_t water_volume_work(const Blk &b)
+{
+    return volume_scratch_bytes(b, true);
 }
 
 int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *mask, Carver &w,
-                        hipStream_t s)
+                        hipStream_t s, bool timed)
 {
     const long nblk = vol_blocks(b);
     double *partial = (double *)w.take((size_t)nblk * 3 * sizeof(double));
@@ -466,8 +473,11 @@ int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *m
     if (vol) {
         VolArgs a{phi, b.x, b.y, b.z, partial, cutflag, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk,
                   (b.e.ni - 1 + VTX - 1) / VTX, (b.e.nj - 1 + VTY - 1) / VTY};
-        {
+        if (timed) {
             KTimer kt(LSM_KT_VOLUME, s);
+            volume_wet<<<(unsigned)nblk, blk, 0, s>>>(a);
+            volume_cut<<<(unsigned)nblk, blk, 0, s>>>(a);
+        } else {
             volume_wet<<<(unsigned)nblk, blk, 0, s>>>(a);
             volume_cut<<<(unsigned)nblk, blk, 0, s>>>(a);
         }
@@ -487,3 +497,61 @@ int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *m
 }
 
 }  // namespace lsm
+
+// ---------------------------------------------------------------------------
+// allocation-free water volume: the scratch of the evaluation lives in a
+// fixed device array of the library, so the call allocates nothing
+// ---------------------------------------------------------------------------
+#include <mutex>
+
+namespace {
+
+constexpr size_t kVolumeScratchBytes = (size_t)32 << 20;   // 32 MiB per device
+
+// The array's name is a host-side stand-in in host code: its device address
+// is taken once per device with hipGetSymbolAddress and only that address is
+// ever passed to kernels.
+static __device__ __align__(256) unsigned char scratch[kVolumeScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_dev[64];
+
+int scratch_address(void **out)
+{
+    int dev = 0;
+    int rc = lsm::check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) return lsm::fail(LSM_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        rc = lsm::check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        if (!p) return lsm::fail(LSM_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return LSM_OK;
+}
+
+}  // namespace
+
+extern "C" int lsm_water_volume_noalloc(const LsmBlock *blk, const double *phi, double *vol_out,
+                                        int32_t *adv_mask, void *stream)
+{
+    using namespace lsm;
+    Blk b;
+    int rc = make_blk(blk, &b, false, true);
+    if (rc) return rc;
+    if (!phi || !vol_out) return fail(LSM_ERR_ARG, "NULL array");
+    const size_t need = volume_scratch_bytes(b, adv_mask != nullptr);
+    if (need > kVolumeScratchBytes)
+        return fail(LSM_ERR_WORKSPACE,
+                    "lsm_water_volume_noalloc: a %dx%dx%d block needs %zu bytes of scratch%s, the "
+                    "library's fixed scratch holds %zu (use lsm_water_volume with a workspace)",
+                    b.e.ni, b.e.nj, b.e.nk, need, adv_mask ? " with the mask" : "", kVolumeScratchBytes);
+    void *p = nullptr;
+    if ((rc = scratch_address(&p))) return rc;
+    Carver c{(char *)p, kVolumeScratchBytes, 0};
+    return launch_water_volume(b, phi, vol_out, adv_mask, c, (hipStream_t)stream, false);
+}

@@ -12,6 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "liblsm_hip.so")
 
 LSM_OK = 0
+LSM_ERR_ARG, LSM_ERR_UNSUPPORTED, LSM_ERR_HIP, LSM_ERR_WORKSPACE = -1, -2, -3, -4
 LSM_FACE_ALL = 0x3F
 LSM_ADVECT_FAST = 0
 LSM_ADVECT_EXACT = 1
@@ -120,6 +121,7 @@ _SIGS = {
     "lsm_reinit_godunov": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp,
                                      C.c_size_t, _vp]),
     "lsm_water_volume": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
+    "lsm_water_volume_noalloc": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp]),
     "lsm_levelset_step": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp, _vp,
                                     _vp, C.c_int64, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, _vp, _vp, C.c_size_t, _vp]),
