@@ -76,7 +76,7 @@ def cpu_baseline(n_ij=256, nk=64, steps=6):
             if secs:
                 return {"value": ranks * cells / secs, "unit": "cell-updates/s", "cores": ranks,
                         "kind": "reference", "seconds": secs,
-                        "sample": sample + f", compiled reference Fortran (amdflang -O2), {ranks} MPI "
+                        "sample": sample + f", compiled reference Fortran (amdflang -O3, the reference's own flag), {ranks} MPI "
                                            "ranks, one per core, each on its own block; slowest rank timed"}
     except Exception as e:  # noqa: BLE001
         log(f"reference harness unavailable ({e}); timing the C oracle instead")
@@ -127,6 +127,44 @@ def cpu_geometric_baseline():
     return {"value": ranks * band / secs, "unit": "band-nodes/s", "cores": ranks, "kind": "reference",
             "seconds": secs, "sample": f"hybrid levelsetmethod without advection on a 32x32x24 block "
                                        f"({band} band nodes per rank), {ranks} MPI ranks, one per core"}
+
+
+def bench_fortran_route(n=128, steps=5):
+    """levelsetmethod called from Fortran on host arrays (the reference
+    harness, one rank, levelsetmethod.F90 called as mg_driver.F90:563-581 does)
+    at n^3 of the bench construction, 4 Sussman iterations per step: the
+    unmodified levelsetmethod.F90 with the callee drop-ins (phi crosses PCIe
+    at every callee), the whole-call drop-in (phi, phi_n and q up once, phi
+    down once per call) and the compiled reference on one core; next to the
+    same step on device-resident tensors.  Harness times include the first
+    call's grid upload."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle", "ref"))
+    from lsm_amd import api, inputs, _lib
+    import refio
+    g, phi, vel, dt = make_case(n)
+    hmin, hmax = inputs.spacing_bounds(g)
+    kw = dict(u=vel[0], v=vel[1], w=vel[2], delti_lsm=dt, nsteps=steps, epslsm=1.5 * hmax,
+              rn_timestep=0.5 * hmin, eps_reinit=1e-12, rn_itermax=4, timing=True, phi0=phi)
+    res = {"workload": f"{n}^3 of the bench construction, {steps} levelsetmethod calls (RK3 + 4 Sussman "
+                       "iterations + volume + bcond), host arrays, one rank"}
+    for tag, h in (("callee_dropins", refio.SHIM_HARNESS), ("whole_call_dropin", refio.RESIDENT_HARNESS),
+                   ("reference_1core", refio.HARNESS)):
+        if not os.path.exists(h):
+            continue
+        _, secs, _ = refio.run("levelset", g, phi, harness=h, **kw)
+        if secs:
+            res[f"{tag}_ms_per_call"] = round(secs * 1e3 / steps, 3)
+    blk = api.Block(g)
+    prm = api.ReinitParams(epslsm=1.5 * hmax, RNtimestep=0.5 * hmin, RNitermax=4, epsReinitialisation=1e-12)
+    uvw = [torch.as_tensor(np.ascontiguousarray(a), device="cuda") for a in vel]
+    p = torch.as_tensor(phi, device="cuda")
+    pn = p.clone()
+    lw = blk.workspace(_lib.LSM_OP_LEVELSET_STEP)
+    res["device_resident_ms_per_call"] = round(_time(lambda: api.levelsetmethod(blk, p, pn, *uvw, prm, dt,
+                                                                                 work=lw), steps), 3)
+    return res
 
 
 def _events():
@@ -454,7 +492,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--workload", default="all",
                     choices=["all", "step-adjoint", "batch", "differentiation", "consumers", "inverse",
-                             "geometric"],
+                             "geometric", "fortran-route"],
                     help="run only one secondary workload and print its figures (for profiling); "
                          "'all' is the benchmark line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -496,6 +534,8 @@ def main():
             res = bench_differentiation()
         elif args.workload == "inverse":
             res = bench_inverse()
+        elif args.workload == "fortran-route":
+            res = bench_fortran_route()
         elif args.workload == "consumers":
             g, phi_np, _, _ = make_case(n)
             res = bench_consumers(g, torch.as_tensor(phi_np, device="cuda"), api.Block(g))
@@ -707,6 +747,13 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_extra and not args.no_diff:
         extra["differentiation"] = bench_differentiation()
+        torch.cuda.empty_cache()
+    if not args.no_extra:
+        try:
+            extra["fortran_route"] = bench_fortran_route()
+            log(f"fortran route: {extra['fortran_route']}")
+        except Exception as e:  # noqa: BLE001
+            log(f"fortran route not timed: {e}")
         torch.cuda.empty_cache()
     if not args.no_extra and not args.no_inverse:
         extra["inverse"] = bench_inverse()

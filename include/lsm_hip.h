@@ -475,15 +475,19 @@ int lsm_levelset_step_adjoint_recorded(const LsmBlock *blk, const LsmReinitParam
  * device mirrors of one block and copies arguments in and results out; every
  * call is synchronous.  Metric vectors are the reference's interleaved
  * csi(1:3,i,j,k) arrays and q is q(1:4,i,j,k) with u, v, w in entries 2..4.
- * lsm_host_setup uploads the grid (again only when the extents or array
- * addresses change) and must precede the other calls; ext = il, iu, jl, ju,
- * kl, ku, igp, jgp, kgp.  The contravariant velocity stays on the device
+ * lsm_host_setup uploads the grid and must precede the other calls; it uploads
+ * again only when the extents, the array addresses or a sampled fingerprint of
+ * the arrays' contents (about a thousand entries of each) change, so grid arrays
+ * rewritten in place are picked up; after editing single entries in place call
+ * lsm_host_invalidate, which makes the next lsm_host_setup upload the grid
+ * again.  ext = il, iu, jl, ju, kl, ku, igp, jgp, kgp.  The contravariant velocity stays on the device
  * between lsm_host_contra_velocity and the advection calls (one
  * levelsetmethod call computes it once, levelsetmethod.F90:227). */
 int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype[6], double dc,
                    double de, double dz, double big_phi, const double *csi, const double *eta,
                    const double *zet, const double *aj, const double *x, const double *y,
                    const double *z);
+int lsm_host_invalidate(void);
 int lsm_host_release(void);
 /* rhs_contra_j.F90:2 */
 int lsm_host_contra_velocity(const double *q, double *ucn_aos);
@@ -494,6 +498,16 @@ int lsm_host_upwind_rhs(const double *phi, double *rhs, const int32_t *adv_mask)
 /* ad_RKTVD3.F90:1 */
 int lsm_host_rk3(double *phi_np1, const double *phi_n, const int32_t *adv_mask, double delti_lsm,
                  int mode);
+/* levelsetmethod.F90:1 (the whole call, non-hybrid or hybrid as p->hybrid
+ * says, GlobalMassCorrection = .false.): phi, phi_n and q(1:4, ...) are
+ * uploaded once, volume, velocity, numiter_lsm RK3 steps, the
+ * reinitialisation when mod(ntime, rn_freq) == 0 and bcond_lsm run on the
+ * device, and phi^(n+1) and the pre-advection volumes vol[0..2] (K-tetrahedra,
+ * second tetrahedra, bulk cells; may be NULL) come back: two field transfers
+ * per call instead of two per callee. */
+int lsm_host_levelset_step(double *phi, const double *phi_n, const double *q, const LsmReinitParams *p,
+                           int numiter_lsm, double delti_lsm, int call_levelset, int call_reinit,
+                           int rn_freq, int ntime, int mode, double vol[3]);
 /* bcond_lsm.F90:2 */
 int lsm_host_bcond(double *phi);
 /* calc_NormGrad_d.F90:1 */

@@ -22,6 +22,7 @@ struct HostState {
     // identity of the uploaded grid
     int ext[9] = {0};
     const void *key[7] = {nullptr};
+    uint64_t sig = 0;                // sampled fingerprint of the grid arrays' contents
     bool ready = false;
     LsmBlock blk{};
     long n = 0;
@@ -32,6 +33,9 @@ struct HostState {
     int32_t *mask = nullptr;
     double *vol = nullptr;
     int32_t *iters = nullptr;
+    double *q = nullptr;             // q(1:4, ...) of a whole-levelsetmethod call
+    void *lswork = nullptr;          // its workspace
+    size_t lswork_bytes = 0;
     std::vector<double> aos;         // host transposition buffer
 };
 
@@ -47,11 +51,35 @@ void release()
     if (g_h.vol) (void)hipFree(g_h.vol);
     if (g_h.iters) (void)hipFree(g_h.iters);
     g_h.mask = nullptr; g_h.vol = nullptr; g_h.iters = nullptr;
+    if (g_h.q) (void)hipFree(g_h.q);
+    if (g_h.lswork) (void)hipFree(g_h.lswork);
+    g_h.q = nullptr; g_h.lswork = nullptr; g_h.lswork_bytes = 0;
     g_h.ready = false;
     g_h.have_ucn = false;
 }
 
 int alloc(void **p, size_t bytes) { return check_hip(hipMalloc(p, bytes), "hipMalloc(host adapter)"); }
+
+// FNV-1a over up to 1024 evenly spaced entries (and the last one) of each grid
+// array: an in-place rewrite of a grid array (new metrics at the same
+// address) changes it; edits of single entries may not, for those the caller
+// calls lsm_host_invalidate
+uint64_t grid_signature(const double *const arr[7], const long len[7])
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const double v) {
+        uint64_t u;
+        memcpy(&u, &v, sizeof u);
+        for (int q = 0; q < 8; ++q) { h ^= (u >> (8 * q)) & 0xff; h *= 1099511628211ull; }
+    };
+    for (int a = 0; a < 7; ++a) {
+        if (!arr[a] || len[a] <= 0) continue;
+        const long stride = len[a] > 1024 ? len[a] / 1024 : 1;
+        for (long q = 0; q < len[a]; q += stride) mix(arr[a][q]);
+        mix(arr[a][len[a] - 1]);
+    }
+    return h;
+}
 
 int up(double *d, const double *h, long n)
 {
@@ -212,8 +240,6 @@ int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype
     std::lock_guard<std::mutex> lk(g_mu);
     if (!ext || !btype || !csi || !eta || !zet || !aj) return fail(LSM_ERR_ARG, "NULL argument");
     const void *key[7] = {csi, eta, zet, aj, x, y, z};
-    bool same = g_h.ready && memcmp(g_h.ext, ext, sizeof g_h.ext) == 0 &&
-                memcmp(g_h.key, key, sizeof key) == 0;
     LsmBlock b{};
     b.il = ext[0]; b.iu = ext[1]; b.jl = ext[2]; b.ju = ext[3]; b.kl = ext[4]; b.ku = ext[5];
     b.igp = ext[6]; b.jgp = ext[7]; b.kgp = ext[8];
@@ -223,6 +249,13 @@ int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype
     Blk chk;
     int rc = make_blk(&b, &chk, false, false);
     if (rc) return rc;
+    const double *arr[7] = {csi, eta, zet, aj, x, y, z};
+    const long len[7] = {3 * chk.n, 3 * chk.n, 3 * chk.n, chk.n, x ? chk.n : 0, y ? chk.n : 0, z ? chk.n : 0};
+    const uint64_t sig = grid_signature(arr, len);
+    // the device mirror is reused while the extents, the array addresses and
+    // the contents' fingerprint are those of the upload
+    const bool same = g_h.ready && memcmp(g_h.ext, ext, sizeof g_h.ext) == 0 &&
+                      memcmp(g_h.key, key, sizeof key) == 0 && g_h.sig == sig;
     if (!same) {
         release();
         const long n = chk.n;
@@ -242,6 +275,7 @@ int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype
         }
         memcpy(g_h.ext, ext, sizeof g_h.ext);
         memcpy(g_h.key, key, sizeof key);
+        g_h.sig = sig;
         g_h.n = n;
         g_h.ready = true;
     }
@@ -255,6 +289,14 @@ int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype
     b.y = (x && y && z) ? g_h.dm[11] : nullptr;
     b.z = (x && y && z) ? g_h.dm[12] : nullptr;
     g_h.blk = b;
+    return LSM_OK;
+}
+
+int lsm_host_invalidate(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_h.ready = false;   // the next lsm_host_setup uploads the grid again
+    g_h.have_ucn = false;
     return LSM_OK;
 }
 
@@ -342,6 +384,37 @@ int lsm_host_rk3(double *phi_np1, const double *phi_n, const int32_t *mask, doub
         return rc;
     if ((rc = sync())) return rc;
     return down(phi_np1, g_h.f[1], n);
+}
+
+int lsm_host_levelset_step(double *phi, const double *phi_n, const double *q, const LsmReinitParams *p,
+                           int numiter_lsm, double delti_lsm, int call_levelset, int call_reinit, int rn_freq,
+                           int ntime, int mode, double vol[3])
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    if (!phi || !phi_n || !p || (call_levelset && !q)) return fail(LSM_ERR_ARG, "NULL argument");
+    const long n = g_h.n;
+    if (!g_h.q && (rc = alloc((void **)&g_h.q, 4 * n * sizeof(double)))) return rc;
+    const size_t need = lsm_workspace_bytes(&g_h.blk, LSM_OP_LEVELSET_STEP);
+    if (need == 0) return fail(LSM_ERR_ARG, "lsm_workspace_bytes failed");
+    if (g_h.lswork_bytes < need) {
+        if (g_h.lswork) (void)hipFree(g_h.lswork);
+        g_h.lswork = nullptr;
+        g_h.lswork_bytes = 0;
+        if ((rc = alloc(&g_h.lswork, need))) return rc;
+        g_h.lswork_bytes = need;
+    }
+    // one upload of phi, phi_n and q; the whole call stays on the device
+    if ((rc = up(g_h.f[0], phi, n)) || (rc = up(g_h.f[1], phi_n, n))) return rc;
+    if (call_levelset && (rc = up(g_h.q, q, 4 * n))) return rc;
+    if ((rc = lsm_levelset_step(&g_h.blk, p, g_h.f[0], g_h.f[1], g_h.q + 1, g_h.q + 2, g_h.q + 3, 4, numiter_lsm,
+                                delti_lsm, call_levelset, call_reinit, rn_freq, ntime, mode, g_h.vol,
+                                g_h.lswork, g_h.lswork_bytes, nullptr)))
+        return rc;
+    if ((rc = sync())) return rc;
+    if (vol && (rc = down(vol, g_h.vol, 3))) return rc;
+    return down(phi, g_h.f[0], n);
 }
 
 int lsm_host_bcond(double *phi)

@@ -36,6 +36,11 @@ module lsm_hip_f
          real(c_double), intent(in)     :: csi(*), eta(*), zet(*), aj(*), x(*), y(*), z(*)
       end function lsm_host_setup
 
+      ! the next lsm_host_setup uploads the grid again (after in-place edits of grid arrays)
+      integer(c_int) function lsm_host_invalidate() bind(C, name="lsm_host_invalidate")
+         import :: c_int
+      end function lsm_host_invalidate
+
       integer(c_int) function lsm_host_contra_velocity(q, ucn) bind(C, name="lsm_host_contra_velocity")
          import :: c_int, c_double
          real(c_double), intent(in)    :: q(*)
@@ -85,6 +90,19 @@ module lsm_hip_f
          type(lsm_reinit_params), intent(in) :: p
          integer(c_int32_t), intent(inout)   :: iters
       end function lsm_host_reinit_eno2
+
+      ! levelsetmethod as a whole, phi resident on the device for the call
+      integer(c_int) function lsm_host_levelset_step(phi, phi_n, q, p, numiter_lsm, delti_lsm,      &
+                                                     call_levelset, call_reinit, rn_freq, ntime, mode, &
+                                                     vol) bind(C, name="lsm_host_levelset_step")
+         import :: c_int, c_double, lsm_reinit_params
+         real(c_double), intent(inout)       :: phi(*)
+         real(c_double), intent(in)          :: phi_n(*), q(*)
+         type(lsm_reinit_params), intent(in) :: p
+         integer(c_int), value               :: numiter_lsm, call_levelset, call_reinit, rn_freq, ntime, mode
+         real(c_double), value               :: delti_lsm
+         real(c_double), intent(inout)       :: vol(3)
+      end function lsm_host_levelset_step
 
       integer(c_int) function lsm_host_sign_ini(phi0, epslsm, m_sphi0, sgn, ng) bind(C, name="lsm_host_sign_ini")
          import :: c_int, c_double

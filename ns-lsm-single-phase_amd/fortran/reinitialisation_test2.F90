@@ -1,5 +1,6 @@
 ! Drop-in for reinitialisation_test2.F90:1-218: Sussman pseudo-time
-! reinitialisation with ENO2 gradients, RK2 pseudo-time stepping and the
+! reinitialisation with ENO2 gradients, SSP-RK3 pseudo-time stepping
+! (reinitialisation_test2.F90:137-184) and the
 ! local mass correction, ended by the |grad d| convergence test or after
 ! RNitermax iterations; on the GPU through liblsm_hip.  The hybrid masking
 ! by InterfaceNodesID (reinitialisation_test2.F90:189-208) is done here.

@@ -131,6 +131,9 @@ _SIGS = {
     "lsm_host_setup": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32), C.c_double,
                                  C.c_double, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lsm_host_release": (C.c_int, []),
+    "lsm_host_invalidate": (C.c_int, []),
+    "lsm_host_levelset_step": (C.c_int, [_vp, _vp, _vp, C.POINTER(LsmReinitParams), C.c_int, C.c_double,
+                                         C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "lsm_host_contra_velocity": (C.c_int, [_vp, _vp]),
     "lsm_host_weno3_rhs": (C.c_int, [_vp, _vp, _vp, C.c_int]),
     "lsm_host_upwind_rhs": (C.c_int, [_vp, _vp, _vp]),

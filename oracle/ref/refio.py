@@ -17,6 +17,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 HARNESS = os.path.join(HERE, "..", "_ref", "lsm_ref_harness")
 SHIM_HARNESS = os.path.join(HERE, "..", "_ref", "lsm_shim_harness")
+# the same harness with levelsetmethod itself replaced by the whole-call drop-in
+RESIDENT_HARNESS = os.path.join(HERE, "..", "_ref", "lsm_resident_harness")
 
 MODES = {
     "advect": 1, "weno_rhs": 2, "bcond": 3, "contra": 4, "normgrad": 5,

@@ -20,6 +20,7 @@ import pytest
 
 from conftest import ROOT
 from fixtures import load
+from lsm_amd import inputs
 
 sys.path.insert(0, os.path.join(ROOT, "oracle", "ref"))
 import refio  # noqa: E402
@@ -128,15 +129,24 @@ def test_dropin_water_volume(name, key):
     assert np.array_equal(outs[1], fx.out(key, 1))
 
 
+HARNESSES = [pytest.param(refio.SHIM_HARNESS, id="callee-dropins",
+                          marks=pytest.mark.skipif(not os.path.exists(refio.SHIM_HARNESS),
+                                                   reason="drop-in harness not built (make -C oracle/ref shim)")),
+             pytest.param(refio.RESIDENT_HARNESS, id="whole-call",
+                          marks=pytest.mark.skipif(not os.path.exists(refio.RESIDENT_HARNESS),
+                                                   reason="resident harness not built (make -C oracle/ref resident)"))]
+
+
 @pytest.mark.parametrize("name", ["curvilinear", "channel"])
+@pytest.mark.parametrize("harness", HARNESSES)
 @pytest.mark.gpu
-@pytest.mark.skipif(not os.path.exists(refio.SHIM_HARNESS),
-                    reason="drop-in harness not built (make -C oracle/ref shim)")
-def test_dropin_levelsetmethod(name):
-    """The unmodified levelsetmethod.F90 driving the GPU routines: volume,
-    contravariant velocity, RK3 advection, Sussman reinitialisation, bcond."""
+def test_dropin_levelsetmethod(name, harness):
+    """levelsetmethod driving the GPU: the unmodified levelsetmethod.F90 with
+    the callee drop-ins, and the whole-call drop-in (phi resident on the
+    device for the call): volume, contravariant velocity, RK3 advection,
+    Sussman reinitialisation, bcond, GlobalVolume.txt."""
     fx = load(name)
-    outs, extra = replay(fx, "levelset", "levelset")
+    outs, extra = replay(fx, "levelset", "levelset", harness=harness)
     assert linf_rel(outs[0], fx.out("levelset")) <= 1e-13
     gv = fx.d["out_levelset_volume"]
     assert np.max(np.abs(extra["global_volume"] - gv)) <= 1e-12 * np.max(gv)
@@ -209,13 +219,13 @@ HYBRID = [("hybrid_cube", "geometric"), ("hybrid_cube", "levelset"), ("hybrid_ch
 
 
 @pytest.mark.parametrize("name,key", HYBRID, ids=[f"{a}-{b}" for a, b in HYBRID])
+@pytest.mark.parametrize("harness", HARNESSES)
 @pytest.mark.gpu
-@pytest.mark.skipif(not os.path.exists(refio.SHIM_HARNESS),
-                    reason="drop-in harness not built (make -C oracle/ref shim)")
-def test_dropin_hybrid_levelsetmethod(name, key):
-    """The unmodified levelsetmethod.F90 with hybrid_reinitialisation =
-    .true.: the geometric_reinitialisation drop-in (and the band-restricted
-    advection) on the GPU, bit for bit."""
+def test_dropin_hybrid_levelsetmethod(name, key, harness):
+    """levelsetmethod with hybrid_reinitialisation = .true. (the unmodified
+    levelsetmethod.F90 with the geometric_reinitialisation drop-in, and the
+    whole-call drop-in): the geometric reinitialisation and the band-restricted
+    advection on the GPU, bit for bit."""
     fx = load(name)
     cl = fx.kw(key, "call_levelset")
     if cl is None:
@@ -224,5 +234,49 @@ def test_dropin_hybrid_levelsetmethod(name, key):
     if name == "hybrid_planes":
         p = fx.inp("plane_" + key.split("_", 1)[1])
         over.update(phi=p, phi0=p)
-    outs, _ = replay(fx, key, "levelset", **over)
+    outs, _ = replay(fx, key, "levelset", harness=harness, **over)
     assert np.array_equal(outs[0], fx.out(key)), linf_rel(outs[0], fx.out(key))
+
+
+@pytest.mark.gpu
+def test_host_setup_follows_grid_rewritten_in_place():
+    """The host adapter re-uploads the grid when arrays at the same addresses
+    are rewritten (sampled fingerprint), and after lsm_host_invalidate when
+    a single entry is edited in place."""
+    import ctypes as C
+    from lsm_amd import _lib
+    L = _lib.load()
+    g = inputs.sinusoidal_cube_grid(12, amp=0.03, seed=5, nj=10, nk=9)
+    n = g.x.size
+    aos = lambda comps: np.ascontiguousarray(np.stack([c.ravel() for c in comps], axis=1))  # noqa: E731
+    csi, eta, zet = aos(g.csi), aos(g.eta), aos(g.zet)
+    aj, x, y, z = (np.ascontiguousarray(a.ravel()) for a in (g.aj, g.x, g.y, g.z))
+    nk, nj, ni = g.shape
+    ext = (C.c_int32 * 9)(1, ni, 1, nj, 1, nk, 0, 0, 0)
+    bt = (C.c_int32 * 6)(*([1] * 6))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    rng = np.random.default_rng(6)
+    q = np.ascontiguousarray(rng.standard_normal((n, 4)))
+
+    def contra():
+        _lib.check(L.lsm_host_setup(ext, _lib.LSM_FACE_ALL, bt, 1.0, 1.0, 1.0, 5.0, dp(csi), dp(eta), dp(zet),
+                                    dp(aj), dp(x), dp(y), dp(z)), "lsm_host_setup")
+        out = np.zeros((n, 3))
+        _lib.check(L.lsm_host_contra_velocity(dp(q), dp(out)), "lsm_host_contra_velocity")
+        return out
+
+    def expect():
+        u, v, w = q[:, 1], q[:, 2], q[:, 3]
+        return np.stack([(1.0 / aj) * ((m[:, 0] * u + m[:, 1] * v) + m[:, 2] * w) for m in (csi, eta, zet)], axis=1)
+
+    try:
+        assert np.array_equal(contra(), expect())
+        csi *= 1.5                                  # whole array rewritten in place
+        aj[:] = aj * 0.75
+        assert np.array_equal(contra(), expect())
+        mid = n // 2 + 3                            # one entry edited: fingerprint may miss it
+        eta[mid, 1] += 0.25
+        _lib.check(L.lsm_host_invalidate(), "lsm_host_invalidate")
+        assert np.array_equal(contra(), expect())
+    finally:
+        L.lsm_host_release()
