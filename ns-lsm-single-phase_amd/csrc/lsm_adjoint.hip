@@ -409,6 +409,176 @@ __global__ __launch_bounds__(AX * R) void adj_stage_fused(FusedArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same stage adjoint with each thread doing only the work its node needs
+// and the loads of the next plane in flight while a plane computes.  Tiles and
+// k march as adj_stage_fused; rows 0, 1, R-2, R-1 of the tile (whole
+// wavefronts) only supply the j partials of their nodes, lanes 0, 1, 62, 63
+// only the i partials, and the two planes below and above the k chunk only
+// the k partials of the tile's own columns.  The i partials of a row are
+// summed across lanes at once (shuffles), the j partials pass through LDS,
+// the k partials stay in the column's register window.
+// ---------------------------------------------------------------------------
+template <int R>
+__global__ __launch_bounds__(AX * R) void adj_stage_lean(FusedArgs a)
+{
+    constexpr int OY = R - 4;
+    constexpr int NCELL = (R + 4) * (AX + 4);
+    const AdjGeo &g = a.g;
+    const int ntile = a.tiles_i * a.tiles_j * ((g.nk + a.kc - 1) / a.kc);
+    // consecutive tiles on one XCD (workgroup b runs on XCD b % 8)
+    const int b = (gridDim.x & 7) == 0 ? (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3))
+                                       : (int)blockIdx.x;
+    if (b >= ntile) return;
+    const int ti = b % a.tiles_i, t2 = b / a.tiles_i;
+    const int tj = t2 % a.tiles_j, tk = t2 / a.tiles_j;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * AX + tx;
+    const int i = ti * AOX - 2 + tx, j = tj * OY - 2 + ty;
+    const int kb = tk * a.kc, ke = min(kb + a.kc - 1, g.nk - 1);
+    const bool hrow = ty < 2 || ty >= R - 2;               // uniform per wavefront
+    const bool hlane = tx < 2 || tx >= 2 + AOX;
+    const bool inb = i >= 0 && i < g.ni && j >= 0 && j < g.nj;
+    const bool own = inb && !hrow && !hlane;
+    const bool inter_ij = i >= 1 && i <= g.ni - 2 && j >= 1 && j <= g.nj - 2;
+    const int ic = min(max(i, 0), g.ni - 1), jc = min(max(j, 0), g.nj - 1);
+    const long col = (long)ic + g.sj * jc;
+    auto kcl = [&](int kk) -> long { return g.sk * (long)min(max(kk, 0), g.nk - 1); };
+    // the two LDS cells of the stage-input plane this thread loads
+    const int c0 = tid, c1 = tid + AX * R;
+    auto cell = [&](int c, int p) -> long {
+        const int rr = c / (AX + 4), cc = c - rr * (AX + 4);
+        const int ii = min(max(ti * AOX - 4 + cc, 0), g.ni - 1);
+        const int jj = min(max(tj * OY - 4 + rr, 0), g.nj - 1);
+        return (long)ii + g.sj * jj + kcl(p);
+    };
+
+    __shared__ double sp[2][R + 4][AX + 4];   // stage input of the plane, two-node halo
+    __shared__ double Pj[5][R][AX];           // j partials (offset -2..2) of each node
+
+    const double *__restrict__ prev = a.prev;
+    const double *__restrict__ lamp = a.lam;
+    // operands of plane p, loaded one plane ahead
+    auto need = [&](int p, bool &ni_, bool &nj_, bool &nk_) {
+        const bool core = p >= kb && p <= ke, live = p >= 0 && p < g.nk;
+        ni_ = live && core && inb && !hrow;
+        nj_ = live && core && inb && !hlane;
+        nk_ = live && own;
+    };
+    // k window of the column: q0..q4 = prev(p-2 .. p+2) (clamped), q4 loaded
+    // one plane ahead
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0;
+    if (own) {
+        q0 = prev[col + kcl(kb - 4)]; q1 = prev[col + kcl(kb - 3)]; q2 = prev[col + kcl(kb - 2)];
+        q3 = prev[col + kcl(kb - 1)];
+    }
+    double nlam = 0.0, naj = 0.0, nu1 = 0.0, nu2 = 0.0, nu3 = 0.0, nq = 0.0, ns0 = 0.0, ns1 = 0.0;
+    int nupd = 1;
+    auto fetch = [&](int p) {
+        bool fi, fj, fk;
+        need(p, fi, fj, fk);
+        const long l = col + g.sk * (long)min(max(p, 0), g.nk - 1);
+        if (fi || fj || fk) {
+            nlam = lamp[l];
+            naj = a.aj[l];
+            nupd = a.mask == nullptr || a.mask[l] > 0;
+        }
+        if (fi) nu1 = a.U1[l];
+        if (fj) nu2 = a.U2[l];
+        if (fk) nu3 = a.U3[l];
+        if (own) nq = prev[col + kcl(p + 2)];
+        if (p >= kb && p <= ke) {
+            ns0 = prev[cell(c0, p)];
+            if (c1 < NCELL) ns1 = prev[cell(c1, p)];
+        }
+    };
+    fetch(kb - 2);
+    double kacc0 = 0.0, kacc1 = 0.0, kacc2 = 0.0, kacc3 = 0.0, kacc4 = 0.0;   // planes p-2 .. p+2
+    double pend0 = 0.0, pend1 = 0.0;                                          // planes p-2, p-1
+    int buf = 0;
+    for (int p = kb - 2; p <= ke + 2; ++p) {
+        const bool core = p >= kb && p <= ke, live = p >= 0 && p < g.nk;
+        bool fi, fj, fk;
+        need(p, fi, fj, fk);
+        const double lam = nlam, ajv = naj, u1 = nu1, u2 = nu2, u3 = nu3;
+        const bool upd = nupd != 0;
+        q4 = nq;
+        if (core) {
+            sp[buf][c0 / (AX + 4)][c0 % (AX + 4)] = ns0;
+            if (c1 < NCELL) sp[buf][c1 / (AX + 4)][c1 % (AX + 4)] = ns1;
+        }
+        if (p < ke + 2) fetch(p + 1);
+        const long l = col + g.sk * (long)p;
+        const bool wr_u = own && core && a.aU1 != nullptr;
+        double o1 = 0.0, o2 = 0.0, o3 = 0.0;
+        if (wr_u) { o1 = a.aU1[l]; o2 = a.aU2[l]; o3 = a.aU3[l]; }
+        const bool part = live && inb && inter_ij && p >= 1 && p <= g.nk - 2 && upd;
+        const double m = lam * (-((a.c3 * ajv) * a.dt));
+        double gsum = 0.0;
+        double d1 = 0.0, d2 = 0.0, d3 = 0.0;
+        if (core) {
+            __syncthreads();
+            const int cx = tx + 2, cy = ty + 2;
+            const double (*P)[AX + 4] = sp[buf];
+            if (!hrow) {
+                double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0, p4 = 0.0;
+                if (part) {
+                    double gl[5];
+                    d1 = weno3_dir_grad_fast(P[cy][cx - 2], P[cy][cx - 1], P[cy][cx], P[cy][cx + 1], P[cy][cx + 2],
+                                             u1, i == 1, i == g.ni - 2, a.dd[0], gl);
+                    p0 = m * gl[0]; p1 = m * gl[1]; p2 = m * gl[2]; p3 = m * gl[3]; p4 = m * gl[4];
+                }
+                // i partial o of lane tx - (o - 2) with respect to this lane's node
+                gsum = (((p2 + __shfl(p0, (tx + 2) & (AX - 1))) + __shfl(p1, (tx + 1) & (AX - 1))) +
+                        __shfl(p3, (tx - 1) & (AX - 1))) + __shfl(p4, (tx - 2) & (AX - 1));
+            }
+            if (!hlane) {
+                double gl[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+                if (part) {
+                    d2 = weno3_dir_grad_fast(P[cy - 2][cx], P[cy - 1][cx], P[cy][cx], P[cy + 1][cx], P[cy + 2][cx],
+                                             u2, j == 1, j == g.nj - 2, a.dd[1], gl);
+                }
+#pragma unroll
+                for (int o = 0; o < 5; ++o) Pj[o][ty][tx] = part ? m * gl[o] : 0.0;
+            }
+        }
+        if (own && part) {
+            double gl[5];
+            d3 = weno3_dir_grad_fast(q0, q1, q2, q3, q4, u3, p == 1, p == g.nk - 2, a.dd[2], gl);
+            kacc0 += m * gl[0]; kacc1 += m * gl[1]; kacc2 += m * gl[2]; kacc3 += m * gl[3];
+            kacc4 += m * gl[4];
+        }
+        double pend2 = 0.0;
+        if (core) {
+            __syncthreads();
+            if (own && live) {
+                double sacc = (upd ? a.c2 * lam : lam) + gsum;
+#pragma unroll
+                for (int o = 0; o < 5; ++o) sacc += Pj[o][ty - (o - 2)][tx];
+                pend2 = sacc;
+                if (upd && a.prev_out) a.adj_n[l] += a.c1 * lam;
+                else if (upd) pend2 += a.c1 * lam;
+            }
+            if (wr_u && part) {
+                a.aU1[l] = o1 + m * (a.dd[0] * d1);
+                a.aU2[l] = o2 + m * (a.dd[1] * d2);
+                a.aU3[l] = o3 + m * (a.dd[2] * d3);
+            }
+        }
+        // plane p - 2 has all its k partials now
+        const int pf = p - 2;
+        if (own && pf >= kb && pf <= ke) {
+            const double tot = pend0 + kacc0;
+            const long lf = col + g.sk * (long)pf;
+            if (a.prev_out) a.prev_out[lf] = tot;
+            else a.adj_n[lf] += tot;
+        }
+        kacc0 = kacc1; kacc1 = kacc2; kacc2 = kacc3; kacc3 = kacc4; kacc4 = 0.0;
+        pend0 = pend1; pend1 = pend2;
+        q0 = q1; q1 = q2; q2 = q3; q3 = q4;
+        buf ^= 1;
+    }
+}
+
 __global__ __launch_bounds__(256) void adj_fill(double *p, long n, double v)
 {
     const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -483,7 +653,9 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
 
     // fused partials + gather (default) or the two-kernel form with the 15
     // partial fields in HBM
-    int fused = 16;
+    // 2: the lean stage kernel (default); 16 / 8: the first fused form with
+    // 16 / 8 rows; 0: the two-kernel form
+    int fused = 2;
     if (const char *e = getenv("LSM_ADJ_FUSED")) fused = atoi(e);
     int kc = 16;
     if (const char *e = getenv("LSM_ADJ_KCHUNK")) { const int v = atoi(e); if (v > 0) kc = v; }
@@ -496,7 +668,15 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
         ba.phi = phi_s;
         ba.lam = lam;
         for (int ax = 2; ax >= 0; --ax) adj_bcond_phase<<<nblk(lines[ax]), 256, 0, s>>>(ba, ax);
-        if (fused == 8 || fused == 16) {
+        if (fused == 2) {
+            FusedArgs fa{g, prev, lam, U1, U2, U3, b.aj, mask, st == 0 ? nullptr : other, adj_n,
+                         adjU1, adjU2, adjU3, C[st][0], C[st][1], C[st][2], dt, {b.dc, b.de, b.dz},
+                         (b.e.ni + AOX - 1) / AOX, (b.e.nj + 11) / 12, kc};
+            const long nt = (long)fa.tiles_i * fa.tiles_j * ((b.e.nk + kc - 1) / kc);
+            const unsigned grid = (unsigned)((nt + 7) / 8 * 8);
+            KTimer kt(LSM_KT_ADJOINT_STAGE, s);
+            adj_stage_lean<16><<<grid, dim3(AX, 16, 1), 0, s>>>(fa);
+        } else if (fused == 8 || fused == 16) {
             FusedArgs fa{g, prev, lam, U1, U2, U3, b.aj, mask, st == 0 ? nullptr : other, adj_n,
                          adjU1, adjU2, adjU3, C[st][0], C[st][1], C[st][2], dt, {b.dc, b.de, b.dz},
                          (b.e.ni + AOX - 1) / AOX, 0, kc};

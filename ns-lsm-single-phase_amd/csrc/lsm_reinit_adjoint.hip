@@ -27,6 +27,8 @@
 // q_d x the coefficient rebuilt from the code, plus the linear parts, and
 // accumulates adjoint x T for dJ/ds(phi0), turned into dJ/dphi0 at the end.
 // Deterministic (no atomics).
+#include <stdlib.h>
+
 #include "lsm_internal.hpp"
 #include "lsm_reinit_common.hpp"
 
@@ -64,7 +66,7 @@ struct TGatherArgs {
 // plane the products lam q_0 (with a 3-node i halo), lam q_1 (3-node j halo)
 // and the branch codes (both halos) sit in LDS, and lam q_2 and the codes of
 // planes k-3 .. k+3 of the thread's own column in a register window.
-constexpr int GX = 64, GY = 8, GH = 3;
+constexpr int GX = 64, GH = 3;
 
 // the coefficient of a face node's one-sided difference (biased_coef) at one
 // slot, or of the i = 1, k = 1 edge's fixed second-order stencil; faces only,
@@ -77,12 +79,19 @@ __device__ __noinline__ double face_coef(int br, int order_bc, int dir, bool eno
     return slot == 0 ? e[0] : slot == 1 ? e[1] : slot == 2 ? e[2] : e[3];
 }
 
+template <int GY>
 __global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherArgs a, int tiles_i, int tiles_j, int kc)
 {
     __shared__ double sW0[GY][GX + 2 * GH];
     __shared__ double sW1[GY + 2 * GH][GX];
     __shared__ uint16_t sC[GY + 2 * GH][GX + 2 * GH];
     __shared__ double s_eno[16][5];   // interior coefficients of every ENO2 branch code
+    // k window of each thread's own column, planes k-3 .. k+3 in a ring of 7
+    // (slot kk mod 7; only the thread itself reads its entries, and plane k+4
+    // replaces plane k-3 after the plane's sums: no barrier needed)
+    __shared__ double sK[7][GY][GX];
+    __shared__ uint16_t sKc[7][GY][GX];
+    auto rs = [](int kk) { return (kk + 70) % 7; };
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * GX + tx;
     if (tid < 16) {
         double e[5];
@@ -103,13 +112,11 @@ __global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherArgs a, int t
     auto kcl = [&](int kk) { return min(max(kk, 0), a.nk - 1); };
     // k window: planes k-3 .. k+3 of this column (clamped; entries outside the
     // block are never used)
-    double w2[7];
-    int c2[7];
-#pragma unroll
     for (int q = 0; q < 7; ++q) {
-        const long l = col + a.sk * (long)kcl(kb - 3 + q);
-        w2[q] = a.lam[l] * a.q[2][l];
-        c2[q] = a.code[l];
+        const int kk = kb - 3 + q;
+        const long l = col + a.sk * (long)kcl(kk);
+        sK[rs(kk)][ty][tx] = a.lam[l] * a.q[2][l];
+        sKc[rs(kk)][ty][tx] = a.code[l];
     }
     for (int k = kb; k <= ke; ++k) {
         const long pk = a.sk * (long)k;
@@ -156,7 +163,7 @@ __global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherArgs a, int t
                         double w;
                         if (d == 0) { c = sC[ty + GH][tx + GH - o]; w = sW0[ty][tx + GH - o]; }
                         else if (d == 1) { c = sC[ty + GH - o][tx + GH]; w = sW1[ty + GH - o][tx]; }
-                        else { c = c2[3 - o]; w = w2[3 - o]; }
+                        else { c = sKc[rs(k - o)][ty][tx]; w = sK[rs(k - o)][ty][tx]; }
                         if (!(c & kLive)) continue;
                         const int br = (c >> (4 * d)) & 15;
                         const double e = kind == 0 ? s_eno[br][slot]
@@ -171,13 +178,11 @@ __global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherArgs a, int t
             else if (a.stage == 2) { a.adn[m] += lin; a.out[m] = acc; }
             else a.out[m] = acc + lin + a.adn[m];
         }
-        // slide the k window
-#pragma unroll
-        for (int q = 0; q < 6; ++q) { w2[q] = w2[q + 1]; c2[q] = c2[q + 1]; }
+        // slide the k window: plane k + 4 replaces plane k - 3
         {
             const long l = col + a.sk * (long)kcl(k + 4);
-            w2[6] = a.lam[l] * a.q[2][l];
-            c2[6] = a.code[l];
+            sK[rs(k + 4)][ty][tx] = a.lam[l] * a.q[2][l];
+            sKc[rs(k + 4)][ty][tx] = a.code[l];
         }
         __syncthreads();
     }
@@ -230,6 +235,10 @@ int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const dou
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    // tile rows (8 or 16) and planes per tile of the gather
+    int gy = 8, kc = 8;
+    if (const char *e = getenv("LSM_RADJ_GY")) gy = atoi(e) == 16 ? 16 : 8;
+    if (const char *e = getenv("LSM_RADJ_KC")) { const int v = atoi(e); if (v > 0 && v <= 64) kc = v; }
     // the first stage reads adj_out itself; then the two buffers alternate
     const double *lam = adj_out;
     double *bufs[2] = {L0, L1};
@@ -242,10 +251,11 @@ int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const dou
                            {t.q[it][st][0], t.q[it][st][1], t.q[it][st][2]}, t.code[it][st], t.T[it][st],
                            t.conv[it], t.iters, it, st + 1, p.order_bc, p.eno_bc, C[st][0], C[st][1], other, adn,
                            adj_s};
-            const int tiles_i = (b.e.ni + GX - 1) / GX, tiles_j = (b.e.nj + GY - 1) / GY;
-            const int kc = 8, tiles_k = (b.e.nk + kc - 1) / kc;
+            const int tiles_i = (b.e.ni + GX - 1) / GX, tiles_j = (b.e.nj + gy - 1) / gy;
+            const int tiles_k = (b.e.nk + kc - 1) / kc;
             const unsigned grid = (unsigned)(tiles_i * tiles_j * tiles_k);
-            radj_tape_gather<<<grid, dim3(GX, GY, 1), 0, s>>>(ga, tiles_i, tiles_j, kc);
+            if (gy == 16) radj_tape_gather<16><<<grid, dim3(GX, 16, 1), 0, s>>>(ga, tiles_i, tiles_j, kc);
+            else radj_tape_gather<8><<<grid, dim3(GX, 8, 1), 0, s>>>(ga, tiles_i, tiles_j, kc);
             lam = other;
             wi ^= 1;
         }

@@ -201,8 +201,9 @@ def test_checkpointed_trajectory_matches_step_chain():
     ((70, 33, 20), (1, 1, 1, 1, 1, 1), False, "16"), ((70, 33, 20), (0, 2, 1, 2, 1, 0), True, "16"),
     ((61, 14, 37), (2, 0, 2, 1, 0, 1), True, "8"), ((9, 12, 10), (1, 2, 0, 1, 1, 0), False, "16")])
 def test_fused_adjoint_matches_two_kernel_form(shape, btype, with_mask, rows, monkeypatch):
-    """The fused adjoint stage kernel (partials kept on chip, several tiles in
-    i, j and k) against the two-kernel form with the partials in HBM."""
+    """The fused adjoint stage kernels (partials kept on chip, several tiles in
+    i, j and k: the first fused form and the lean one) against the two-kernel
+    form with the partials in HBM."""
     from lsm_amd import api
     g, phi, vel, dt = _case(shape, seed=121)
     blk = api.Block(g, btype=btype, big_phi=0.3)
@@ -212,10 +213,11 @@ def test_fused_adjoint_matches_two_kernel_form(shape, btype, with_mask, rows, mo
     ucn = api.rhs_contra_j(blk, *(dev(a) for a in vel))
     lam = dev(rng.standard_normal(g.shape))
     res = {}
-    for form in (rows, "0"):
+    for form in (rows, "2", "0"):
         monkeypatch.setenv("LSM_ADJ_FUSED", form)
         aU = [blk.zeros() for _ in range(3)]
         adj = api.ad_RKTVD3_adjoint(blk, dev(phi), ucn, dt, lam, adv_mask=md, mode="exact", adj_ucn=aU)
         res[form] = [t.cpu().numpy() for t in [adj] + aU]
-    for a, b in zip(res[rows], res["0"]):
-        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b)), np.max(np.abs(a - b)) / np.max(np.abs(b))
+    for form in (rows, "2"):
+        for a, b in zip(res[form], res["0"]):
+            assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b)), (form, np.max(np.abs(a - b)) / np.max(np.abs(b)))
