@@ -33,6 +33,17 @@ ADV_BYTES_PER_CELL_UPDATE = 160.0
 ADV_FOLDED_BYTES_PER_CELL_UPDATE = 136.0
 VOLUME_BYTES_PER_CELL = 32.0
 REINIT_BYTES_PER_CELL_ITER = 304.0
+# algorithmic bytes per node of the reverse-sweep kernels, averaged over the
+# three RK stages of a launch sequence.  Advection adjoint stage: reads the
+# stage input, the adjoint, U/J (3), aj, the accumulated dJ/d(U/J) (3) and
+# dJ/dphi^n; writes dJ/d(U/J) (3), dJ/dphi^n and (stages 3, 2) the adjoint of
+# the stage input: 120, 120, 112 B.  Tape gather: reads the adjoint, the three
+# tape weights, the branch code (2 B) and the update mask (1 B), writes the
+# adjoint of the stage input, and the iteration's dJ/dd^n (written, updated,
+# read over stages 3, 2, 1: 8, 16, 8 B).
+ADJ_STAGE_BYTES_PER_NODE = (120.0 + 120.0 + 112.0) / 3.0
+TAPE_GATHER_BYTES_PER_NODE = 8 + 24 + 2 + 1 + 8 + (8.0 + 16.0 + 8.0) / 3.0
+
 
 
 def log(*a):
@@ -257,8 +268,36 @@ def bench_step_adjoint(g, phi, ucn, dt, blk):
         api.levelset_step_adjoint(blk, phi, ucn, dt, prm, lam, mode="fast", adj_ucn=aU, out=adj, work=wad)
     f_ms, b_ms = _time(fwd, 5), _time(bwd, 5)
     n_int = (g.ni - 2) * (g.nj - 2) * (g.nk - 2)
+    # the reverse-sweep kernels' own durations (library HIP events, one more call)
+    import ctypes as C
+    L = _lib.load()
+    L.lsm_timing_enable(1)
+    bwd()
+    import torch
+    torch.cuda.synchronize()
+    kt = {}
+    for name, kid in (("adj_stage", _lib.LSM_KT_ADJOINT_STAGE), ("tape_gather", _lib.LSM_KT_REINIT_ADJOINT)):
+        ms, cnt = C.c_double(), C.c_int64()
+        L.lsm_timing_get(kid, C.byref(ms), C.byref(cnt))
+        kt[name] = (ms.value / max(cnt.value, 1), int(cnt.value))
+    L.lsm_timing_enable(0)
+    nodes = g.ni * g.nj * g.nk
+    adj_roof = {}
+    for name, per_node, prof in (("adj_stage", ADJ_STAGE_BYTES_PER_NODE, "adj_stage_lean_256"),
+                                 ("tape_gather", TAPE_GATHER_BYTES_PER_NODE, "radj_tape_gather_256")):
+        ms, cnt = kt[name]
+        t = load_traffic(prof)
+        traffic = t["bytes_per_launch"] if t and nodes == 256 ** 3 else None
+        if cnt:
+            alg = per_node * nodes
+            gbs = alg / (ms / 1e3) / 1e9
+            adj_roof[name] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(gbs / HBM_PEAK_GBS, 4), "avg_launch_ms": round(ms, 5), "launches": cnt,
+                              "alg_bytes_per_launch": alg, "alg_bytes_per_node": per_node,
+                              "traffic": traffic}
     return {"step_fwd_ms": round(f_ms, 4), "step_adjoint_ms": round(b_ms, 4),
             "step_adjoint_forward_ratio": round(b_ms / f_ms, 3),
+            "adjoint_roofline": adj_roof,
             "step_adjoint_cell_updates_per_s": n_int / (b_ms / 1e3),
             "step_definition": "ad_RKTVD3 + 4 reinitialisation_test2 iterations + bcond_lsm (levelsetmethod, "
                                "numiter_lsm = 1); adjoint = forward recompute incl. the reinitialisation trace + "

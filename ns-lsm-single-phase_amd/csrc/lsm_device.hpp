@@ -288,25 +288,32 @@ __device__ __forceinline__ void biased_coef(int br, int order, int dir, bool eno
     if (order == 3) { e[0] = -11.0 / 6.0 * as; e[1] = 3.0 * as; e[2] = -1.5 * as; e[3] = 1.0 / 3.0 * as; }
 }
 
+// The library sin / cos, called out of line: they are needed only inside the
+// interface band (|phi| <= eps), and inlined their registers count against
+// every node of the stencil kernels that use them (same functions, same
+// results)
+static __device__ __noinline__ double band_sin(double x) { return sin(x); }
+static __device__ __noinline__ double band_cos(double x) { return cos(x); }
+
 // d sign_phi0 / d phi0
 __device__ __forceinline__ double sign_phi0_deriv(double p, double eps)
 {
     if (fabs(p) > eps) return 0.0;
-    return (1.0 - cos((kPi * p) / eps)) / eps;
+    return (1.0 - band_cos((kPi * p) / eps)) / eps;
 }
 
 // GetSignPhi0, AdvectionMethods.F90:576-587
 __device__ __forceinline__ double sign_phi0(double p, double eps)
 {
     if (fabs(p) > eps) return sign1(p);
-    return p / eps - (1.0 / kPi) * sin((kPi * p) / eps);
+    return p / eps - (1.0 / kPi) * band_sin((kPi * p) / eps);
 }
 
 // deltaf, global_lsm.F90:120-133
 __device__ __forceinline__ double deltaf(double v, double eps)
 {
     if (fabs(v) > eps) return 0.0;
-    return (1.0 / (2.0 * eps)) * (1.0 + cos((kPi * v) / eps));
+    return (1.0 / (2.0 * eps)) * (1.0 + band_cos((kPi * v) / eps));
 }
 
 // GetLambdaMassCorrection, global_lsm.F90:200-219
