@@ -71,7 +71,20 @@ extern "C" {
 #define LSM_OP_STEP_ADJOINT_RECORDED 10 /* sized by lsm_adjoint_workspace_bytes */
 #define LSM_OP_REINIT_GEOMETRIC 11
 
-/* One curvilinear block (the host-associated state of levelsetmethod.F90:1-44). */
+/* One blanking region (global_app.F90:30-43, set up by init.F90:1216-1567;
+ * with one block per rank li_blk_* = le_blk_*): a box of nodes given in the
+ * block's array indices (li_blk_ia..li_blk_kb, inclusive) and blktype(1:4)
+ * of its i and j faces (0: bcond_lsm extrapolates the face from outside).
+ * The box must lie in the block, with ia <= iu-1 and ib >= il+1 (likewise
+ * for j) so that bcond_lsm's writes next to each face stay in the array. */
+#define LSM_MAX_BLANK 4
+typedef struct LsmBlank {
+    int32_t ia, ib, ja, jb, ka, kb;
+    int32_t blktype[4];
+} LsmBlank;
+
+/* One curvilinear block (the host-associated state of levelsetmethod.F90:1-44).
+ * Zero-initialise it: nblk = 0 means no blanking. */
 typedef struct LsmBlock {
     int32_t il, iu, jl, ju, kl, ku;   /* array extents */
     int32_t igp, jgp, kgp;            /* ghost widths */
@@ -84,6 +97,13 @@ typedef struct LsmBlock {
     const double *zet[3];             /* zeta_x, zeta_y, zeta_z */
     const double *aj;                 /* aj = 1/det(dx/dxi) */
     const double *x, *y, *z;          /* node coordinates (water volume only) */
+    int32_t nblk;                     /* blanking regions in blank[] (nblk of global_app.F90:31) */
+    LsmBlank blank[LSM_MAX_BLANK];    /* read by lsm_weno3_rhs, lsm_bcond, lsm_rk3_step,
+                                         lsm_water_volume(_noalloc), lsm_reinit_geometric,
+                                         lsm_levelset_step and the lsm_host_* adapter; the
+                                         entry points whose reference routines do not
+                                         read the blanking arrays ignore it; the adjoints,
+                                         the folded step and the consumers reject nblk != 0 */
 } LsmBlock;
 
 /* Parameters of the pseudo-time reinitialisation (global_lsm.F90:17-58). */
@@ -487,6 +507,11 @@ int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype
                    double de, double dz, double big_phi, const double *csi, const double *eta,
                    const double *zet, const double *aj, const double *x, const double *y,
                    const double *z);
+/* The blanking regions of the bound block (nblk, li_blk_* and blktype of
+ * global_app.F90:30-43): nblk rows of ten values, li_blk_ia, li_blk_ib,
+ * li_blk_ja, li_blk_jb, li_blk_ka, li_blk_kb (array indices) and
+ * blktype(1:4).  lsm_host_setup clears them; call this after it. */
+int lsm_host_blanking(int32_t nblk, const int32_t *boxes);
 int lsm_host_invalidate(void);
 int lsm_host_release(void);
 /* rhs_contra_j.F90:2 */

@@ -89,8 +89,33 @@ struct RhsArgs {
     long sj, sk;
     Range r;
     double dc, de, dz;
+    Blank bl;
 };
 
+// the WENO3 right-hand side at interior node (i, j, k); the stencils are
+// forced one-sided on the physical faces and next to the i and j faces of
+// blanking boxes
+template <bool FAST>
+__device__ __forceinline__ double weno3_node(const RhsArgs &a, const double *p, int i, int j, int k, long l)
+{
+    const long sj = a.sj, sk = a.sk;
+    bool fmb = a.r.fb && i == a.r.ilo, fmf = a.r.ff && i == a.r.ihi;
+    bool fml = a.r.fl && j == a.r.jlo, fmr = a.r.fr && j == a.r.jhi;
+    const bool fmd = a.r.fd && k == a.r.klo, fmu = a.r.fu && k == a.r.khi;
+    if (a.bl.n) a.bl.bias(i, j, fmb, fmf, fml, fmr);
+    // forced stencils do not read their far entry: a dummy replaces it, so
+    // the reads past a physical face are never made
+    const double xm2 = fmb ? 0.0 : p[l - 2], xp2 = fmf ? 0.0 : p[l + 2];
+    const double ym2 = fml ? 0.0 : p[l - 2 * sj], yp2 = fmr ? 0.0 : p[l + 2 * sj];
+    const double zm2 = fmd ? 0.0 : p[l - 2 * sk], zp2 = fmu ? 0.0 : p[l + 2 * sk];
+    const double c = p[l];
+    const double Lc = weno3_dir<FAST>(xm2, p[l - 1], c, p[l + 1], xp2, a.U1[l], fmb, fmf, a.dc);
+    const double Le = weno3_dir<FAST>(ym2, p[l - sj], c, p[l + sj], yp2, a.U2[l], fml, fmr, a.de);
+    const double Lz = weno3_dir<FAST>(zm2, p[l - sk], c, p[l + sk], zp2, a.U3[l], fmd, fmu, a.dz);
+    return (Lc + Le) + Lz;
+}
+
+// interior nodes with AdvectionNodes != 0 outside the blanking boxes
 template <bool FAST>
 __global__ __launch_bounds__(256) void weno3_rhs_kernel(RhsArgs a)
 {
@@ -100,22 +125,99 @@ __global__ __launch_bounds__(256) void weno3_rhs_kernel(RhsArgs a)
     if (i > a.r.ihi || j > a.r.jhi || k > a.r.khi) return;
     const long l = (long)i + a.sj * j + a.sk * k;
     if (a.mask && a.mask[l] == 0) return;
-    const double *p = a.phi;
+    if (a.bl.n && a.bl.inside(i, j, k)) return;
+    a.rhs[l] = weno3_node<FAST>(a, a.phi, i, j, k, l);
+}
+
+// One RK stage of ad_RKTVD3.F90:52-80 on a block with blanking regions,
+// unfused: dst = ((c1 phi^n) + (c2 src)) - (((c3 aj) dt) rh(src)) at every
+// node with AdvectionNodes > 0 (rh = 0 on the shell and inside the boxes),
+// dst = src elsewhere; bcond_lsm follows as its own launches.
+struct BlankStageArgs {
+    RhsArgs r;
+    const double *pn;
+    double *dst;
+    const double *aj;
+    double c1, c2, c3, dt;
+};
+
+template <bool FAST>
+__global__ __launch_bounds__(256) void blank_stage_kernel(BlankStageArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= a.r.ni || j >= a.r.nj) return;
+    const long l = (long)i + a.r.sj * j + a.r.sk * k;
+    const double v = a.r.phi[l];
+    if (a.r.mask && !(a.r.mask[l] > 0)) { a.dst[l] = v; return; }
+    double rh = 0.0;
+    const Range &R = a.r.r;
+    if (i >= R.ilo && i <= R.ihi && j >= R.jlo && j <= R.jhi && k >= R.klo && k <= R.khi &&
+        !a.r.bl.inside(i, j, k))
+        rh = weno3_node<FAST>(a.r, a.r.phi, i, j, k, l);
+    a.dst[l] = (a.c1 * a.pn[l] + a.c2 * v) - ((a.c3 * a.aj[l]) * a.dt) * rh;
+}
+
+// ---------------------------------------------------------------------------
+// the blanking part of bcond_lsm (bcond_lsm.F90:290-562), one box at a time:
+// phase 0 zeroes the nodes two and more inside the box, phases 1-4 the i
+// faces (ia, ib) and j faces (ja, jb), each face node and the node behind it
+// set from the two nodes outside, phase 5 the four i-j corners in the order
+// (ia, ja), (ib, ja), (ib, jb), (ia, jb).  k faces are not treated
+// (commented out in the reference).
+// ---------------------------------------------------------------------------
+struct BlankBcArgs {
+    double *phi;
+    long sj, sk;
+    int i0, i1, j0, j1, k0, k1;   // the thread range of the phase (inclusive)
+    int ia, ib, ja, jb;           // the box's i and j faces
+    int corner[4];                // phase 5: which corners are treated
+    double big;
+};
+
+__device__ __forceinline__ void blank_extrap(double *phi, long lb, long behind, long l1, long l2, double big)
+{
+    const double v = bc_face(1, phi[l1], phi[l2], 0.0, big, false);
+    phi[lb] = v;
+    phi[behind] = v;
+}
+
+__global__ __launch_bounds__(256) void blank_bcond_phase(BlankBcArgs a, int phase)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long n1 = a.i1 - a.i0 + 1, n2 = a.j1 - a.j0 + 1, n3 = a.k1 - a.k0 + 1;
+    if (t >= n1 * n2 * n3) return;
+    const int i = a.i0 + (int)(t % n1), j = a.j0 + (int)((t / n1) % n2), k = a.k0 + (int)(t / (n1 * n2));
     const long sj = a.sj, sk = a.sk;
-    auto at = [&](long q) -> double { return p[q]; };
-    // out-of-range stencil entries are only read when the stencil is not
-    // forced; forced stencils replace them by a dummy, so guard the reads
-    const bool fmb = a.r.fb && i == a.r.ilo, fmf = a.r.ff && i == a.r.ihi;
-    const bool fml = a.r.fl && j == a.r.jlo, fmr = a.r.fr && j == a.r.jhi;
-    const bool fmd = a.r.fd && k == a.r.klo, fmu = a.r.fu && k == a.r.khi;
-    const double xm2 = fmb ? 0.0 : at(l - 2), xp2 = fmf ? 0.0 : at(l + 2);
-    const double ym2 = fml ? 0.0 : at(l - 2 * sj), yp2 = fmr ? 0.0 : at(l + 2 * sj);
-    const double zm2 = fmd ? 0.0 : at(l - 2 * sk), zp2 = fmu ? 0.0 : at(l + 2 * sk);
-    const double c = at(l);
-    const double Lc = weno3_dir<FAST>(xm2, at(l - 1), c, at(l + 1), xp2, a.U1[l], fmb, fmf, a.dc);
-    const double Le = weno3_dir<FAST>(ym2, at(l - sj), c, at(l + sj), yp2, a.U2[l], fml, fmr, a.de);
-    const double Lz = weno3_dir<FAST>(zm2, at(l - sk), c, at(l + sk), zp2, a.U3[l], fmd, fmu, a.dz);
-    a.rhs[l] = (Lc + Le) + Lz;
+    const long l = (long)i + sj * j + sk * k;
+    switch (phase) {
+    case 0: a.phi[l] = 0.0; break;
+    case 1: blank_extrap(a.phi, l, l + 1, l - 1, l - 2, a.big); break;
+    case 2: blank_extrap(a.phi, l, l - 1, l + 1, l + 2, a.big); break;
+    case 3: blank_extrap(a.phi, l, l + sj, l - sj, l - 2 * sj, a.big); break;
+    case 4: blank_extrap(a.phi, l, l - sj, l + sj, l + 2 * sj, a.big); break;
+    default: {
+        // one thread per k plane, the corners in the reference's order
+        const long k0 = sk * k;
+        if (a.corner[0]) {
+            const long c = a.ia + sj * a.ja + k0;
+            blank_extrap(a.phi, c, c + 1 + sj, c - 1 - sj, c - 2 - 2 * sj, a.big);
+        }
+        if (a.corner[1]) {
+            const long c = a.ib + sj * a.ja + k0;
+            blank_extrap(a.phi, c, c - 1 + sj, c + 1 - sj, c + 2 - 2 * sj, a.big);
+        }
+        if (a.corner[2]) {
+            const long c = a.ib + sj * a.jb + k0;
+            blank_extrap(a.phi, c, c - 1 - sj, c + 1 + sj, c + 2 + 2 * sj, a.big);
+        }
+        if (a.corner[3]) {
+            const long c = a.ia + sj * a.jb + k0;
+            blank_extrap(a.phi, c, c + 1 - sj, c - 1 + sj, c - 2 + 2 * sj, a.big);
+        }
+    }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -211,11 +313,18 @@ int launch_contra(const Blk &b, const double *u, const double *v, const double *
     return check_hip(hipGetLastError(), "contra_kernel");
 }
 
+static RhsArgs rhs_args(const Blk &b, const double *phi, const double *U1, const double *U2, const double *U3,
+                        const int32_t *mask, double *rhs)
+{
+    return RhsArgs{phi, U1, U2, U3, mask, rhs, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, make_range(b),
+                   b.dc, b.de, b.dz, b.bl};
+}
+
 int launch_weno3_rhs(const Blk &b, const double *phi, const double *U1, const double *U2,
                      const double *U3, const int32_t *mask, double *rhs, int mode, hipStream_t s)
 {
-    const Range r = make_range(b);
-    RhsArgs a{phi, U1, U2, U3, mask, rhs, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, r, b.dc, b.de, b.dz};
+    const RhsArgs a = rhs_args(b, phi, U1, U2, U3, mask, rhs);
+    const Range &r = a.r;
     const int ni = r.ihi - r.ilo + 1, nj = r.jhi - r.jlo + 1, nk = r.khi - r.klo + 1;
     if (ni <= 0 || nj <= 0 || nk <= 0) return LSM_OK;
     dim3 blk(64, 4, 1), grd(blocks_for(ni, 64), blocks_for(nj, 4), nk);
@@ -233,6 +342,43 @@ int launch_bcond(const Blk &b, double *phi, hipStream_t s)
         bcond_phase<<<blocks_for(2 * per[ax], 256), 256, 0, s>>>(a, ax);
         int rc = check_hip(hipGetLastError(), "bcond_phase");
         if (rc) return rc;
+    }
+    return launch_bcond_blanking(b, phi, s);
+}
+
+// the phases of one box in order; a phase whose guard fails or whose range
+// is empty is not launched
+int launch_bcond_blanking(const Blk &b, double *phi, hipStream_t s)
+{
+    const int ni = b.e.ni, nj = b.e.nj, nk = b.e.nk;
+    for (int q = 0; q < b.bl.n; ++q) {
+        const int *B = b.bl.b[q];
+        const int ia = B[0], ib = B[1], ja = B[2], jb = B[3], ka = B[4], kb = B[5];
+        // bcond_lsm.F90:320, 366, 412, 458: blktype 0 and two nodes between the face and the block edge
+        const bool fi0 = B[6] == 0 && ia > 1, fi1 = B[7] == 0 && ib < ni - 2;
+        const bool fj0 = B[8] == 0 && ja > 1, fj1 = B[9] == 0 && jb < nj - 2;
+        const int k0 = max(0, ka), k1 = min(nk - 1, kb);
+        BlankBcArgs a{};
+        a.phi = phi; a.sj = b.e.sj; a.sk = b.e.sk; a.big = b.big_phi;
+        a.ia = ia; a.ib = ib; a.ja = ja; a.jb = jb;
+        a.corner[0] = fi0 && fj0; a.corner[1] = fi1 && fj0; a.corner[2] = fi1 && fj1; a.corner[3] = fi0 && fj1;
+        auto run = [&](int phase, bool on, int i0, int i1, int j0, int j1, int kk0, int kk1) -> int {
+            if (!on || i0 > i1 || j0 > j1 || kk0 > kk1) return LSM_OK;
+            a.i0 = i0; a.i1 = i1; a.j0 = j0; a.j1 = j1; a.k0 = kk0; a.k1 = kk1;
+            const long n = (long)(i1 - i0 + 1) * (j1 - j0 + 1) * (kk1 - kk0 + 1);
+            blank_bcond_phase<<<blocks_for(n, 256), 256, 0, s>>>(a, phase);
+            return check_hip(hipGetLastError(), "blank_bcond_phase");
+        };
+        const int jlo = max(0, ja), jhi = min(nj - 1, jb), ilo = max(0, ia), ihi = min(ni - 1, ib);
+        int rc;
+        if ((rc = run(0, true, max(0, ia + 2), min(ni - 1, ib - 2), max(0, ja + 2), min(nj - 1, jb - 2),
+                      max(0, ka + 2), min(nk - 1, kb - 2)))) return rc;
+        if ((rc = run(1, fi0, ia, ia, jlo, jhi, k0, k1))) return rc;
+        if ((rc = run(2, fi1, ib, ib, jlo, jhi, k0, k1))) return rc;
+        if ((rc = run(3, fj0, ilo, ihi, ja, ja, k0, k1))) return rc;
+        if ((rc = run(4, fj1, ilo, ihi, jb, jb, k0, k1))) return rc;
+        const bool anyc = a.corner[0] || a.corner[1] || a.corner[2] || a.corner[3];
+        if ((rc = run(5, anyc, 0, 0, 0, 0, k0, k1))) return rc;
     }
     return LSM_OK;
 }
@@ -268,6 +414,24 @@ static int rk3_run(const Blk &b, const double *phi_n, const int32_t *mask, const
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
     const bool fast = mode != LSM_ADVECT_EXACT;
+    if (b.bl.n) {
+        if (folded) return fail(LSM_ERR_UNSUPPORTED, "blanking regions: the folded step is not implemented");
+        if (b.e.nk > 65535) return fail(LSM_ERR_ARG, "nk > 65535");
+        for (int st = 0; st < 3; ++st) {
+            BlankStageArgs a{rhs_args(b, src[st], U1, U2, U3, mask, nullptr), phi_n, dst[st], b.aj,
+                             C[st][0], C[st][1], C[st][2], dt};
+            const dim3 grd(blocks_for(b.e.ni, 64), blocks_for(b.e.nj, 4), b.e.nk);
+            {
+                KTimer kt(LSM_KT_ADVECT_STAGE, s);
+                if (fast) blank_stage_kernel<true><<<grd, dim3(64, 4, 1), 0, s>>>(a);
+                else blank_stage_kernel<false><<<grd, dim3(64, 4, 1), 0, s>>>(a);
+            }
+            int rc = check_hip(hipGetLastError(), "blank_stage_kernel");
+            if (rc) return rc;
+            if ((rc = launch_bcond(b, dst[st], s))) return rc;
+        }
+        return LSM_OK;
+    }
 
     // interior stage kernel, then the boundary shell kernel, per stage
     int TY = 4;

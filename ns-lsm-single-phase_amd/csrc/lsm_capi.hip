@@ -30,7 +30,7 @@ int check_hip(hipError_t e, const char *what)
     return fail(LSM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-int make_blk(const LsmBlock *b, Blk *o, bool need_metrics, bool need_coords)
+int make_blk(const LsmBlock *b, Blk *o, bool need_metrics, bool need_coords, BlankPolicy blanking)
 {
     if (!b) return fail(LSM_ERR_ARG, "LsmBlock is NULL");
     const int ni = b->iu - b->il + 1, nj = b->ju - b->jl + 1, nk = b->ku - b->kl + 1;
@@ -61,6 +61,33 @@ int make_blk(const LsmBlock *b, Blk *o, bool need_metrics, bool need_coords)
         if (!b->aj) return fail(LSM_ERR_ARG, "aj is NULL");
     }
     if (need_coords && (!b->x || !b->y || !b->z)) return fail(LSM_ERR_ARG, "x/y/z is NULL");
+    o->bl.n = 0;
+    if (b->nblk < 0 || b->nblk > LSM_MAX_BLANK)
+        return fail(LSM_ERR_ARG, "nblk = %d (0..%d blanking regions)", b->nblk, LSM_MAX_BLANK);
+    if (b->nblk == 0 || blanking == kBlankIgnore) return LSM_OK;
+    if (blanking == kBlankReject)
+        return fail(LSM_ERR_UNSUPPORTED, "blanking regions (nblk = %d): not implemented by this entry point",
+                    b->nblk);
+    // boxes in 0-based node indices; the extrapolations of bcond_lsm write
+    // the node behind each box face, which must lie in the array
+    const int lo[3] = {b->il, b->jl, b->kl}, ext[3] = {ni, nj, nk};
+    for (int q = 0; q < b->nblk; ++q) {
+        const LsmBlank &B = b->blank[q];
+        const int box[6] = {B.ia, B.ib, B.ja, B.jb, B.ka, B.kb};
+        for (int d = 0; d < 3; ++d) {
+            const int a0 = box[2 * d] - lo[d], a1 = box[2 * d + 1] - lo[d];
+            if (a0 < 0 || a0 > a1 || a1 > ext[d] - 1)
+                return fail(LSM_ERR_ARG, "blanking region %d: %c range %d..%d outside the block", q, "ijk"[d],
+                            box[2 * d], box[2 * d + 1]);
+            if (d < 2 && (a0 > ext[d] - 2 || a1 < 1))
+                return fail(LSM_ERR_ARG, "blanking region %d: %c range %d..%d needs a node behind each face", q,
+                            "ijk"[d], box[2 * d], box[2 * d + 1]);
+            o->bl.b[q][2 * d] = a0;
+            o->bl.b[q][2 * d + 1] = a1;
+        }
+        for (int f = 0; f < 4; ++f) o->bl.b[q][6 + f] = B.blktype[f];
+    }
+    o->bl.n = b->nblk;
     return LSM_OK;
 }
 
@@ -283,14 +310,14 @@ int lsm_timing_get(int id, double *ms_total, int64_t *launches)
 size_t lsm_workspace_bytes(const LsmBlock *blk, int op)
 {
     Blk b;
-    if (make_blk(blk, &b, false, false)) return 0;
+    if (make_blk(blk, &b, false, false, kBlankIgnore)) return 0;
     return op_bytes(b, op);
 }
 
 size_t lsm_adjoint_workspace_bytes(const LsmBlock *blk, int op, int rn_itermax)
 {
     Blk b;
-    if (make_blk(blk, &b, false, false)) return 0;
+    if (make_blk(blk, &b, false, false, kBlankIgnore)) return 0;
     if (rn_itermax < 0 || rn_itermax > kMaxTraceIters) return 0;
     return op_bytes(b, op, rn_itermax);
 }
@@ -299,7 +326,7 @@ int lsm_contra_velocity(const LsmBlock *blk, const double *u, const double *v, c
                         int64_t vel_stride, double *ucn1, double *ucn2, double *ucn3, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, true, false);
+    int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
     if (!u || !v || !w || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL array");
     if (vel_stride < 1) return fail(LSM_ERR_ARG, "vel_stride must be >= 1");
@@ -310,7 +337,7 @@ int lsm_weno3_rhs(const LsmBlock *blk, const double *phi, const double *ucn1, co
                   const double *ucn3, const int32_t *adv_mask, double *rhs, int mode, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, false);
+    int rc = make_blk(blk, &b, false, false, kBlankUse);
     if (rc) return rc;
     if (!phi || !ucn1 || !ucn2 || !ucn3 || !rhs) return fail(LSM_ERR_ARG, "NULL array");
     if (mode == LSM_ADVECT_UPWIND1)
@@ -323,7 +350,7 @@ int lsm_upwind_rhs(const LsmBlock *blk, const double *phi, const double *ucn1, c
                    const double *ucn3, const int32_t *adv_mask, double *rhs, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, false);
+    int rc = make_blk(blk, &b, false, false, kBlankIgnore);
     if (rc) return rc;
     if (!phi || !ucn1 || !ucn2 || !ucn3 || !rhs) return fail(LSM_ERR_ARG, "NULL array");
     if (phi == rhs) return fail(LSM_ERR_ARG, "phi and rhs must not alias");
@@ -333,7 +360,7 @@ int lsm_upwind_rhs(const LsmBlock *blk, const double *phi, const double *ucn1, c
 int lsm_bcond(const LsmBlock *blk, double *phi, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, false);
+    int rc = make_blk(blk, &b, false, false, kBlankUse);
     if (rc) return rc;
     if (!phi) return fail(LSM_ERR_ARG, "NULL array");
     return launch_bcond(b, phi, (hipStream_t)stream);
@@ -344,7 +371,7 @@ int lsm_rk3_step(const LsmBlock *blk, double *phi_np1, const double *phi_n, cons
                  int mode, void *work, size_t work_bytes, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, false);
+    int rc = make_blk(blk, &b, false, false, kBlankUse);
     if (rc) return rc;
     if (!b.aj) return fail(LSM_ERR_ARG, "aj is NULL");
     if (!phi_np1 || !phi_n || !ucn1 || !ucn2 || !ucn3) return fail(LSM_ERR_ARG, "NULL array");
@@ -364,7 +391,7 @@ int lsm_advect_fold(const LsmBlock *blk, const double *ucn1, const double *ucn2,
                     double delti_lsm, double *v1, double *v2, double *v3, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, false);
+    int rc = make_blk(blk, &b, false, false, kBlankIgnore);
     if (rc) return rc;
     if (!b.aj) return fail(LSM_ERR_ARG, "aj is NULL");
     if (!ucn1 || !ucn2 || !ucn3 || !v1 || !v2 || !v3) return fail(LSM_ERR_ARG, "NULL array");
@@ -439,7 +466,7 @@ int lsm_normgrad_eno2(const LsmBlock *blk, const double *d, const double *phizer
                       int eno_bc, double *normgrad, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, true, false);
+    int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
     if (!d || !phizero || !normgrad) return fail(LSM_ERR_ARG, "NULL array");
     if (order_bc < 1 || order_bc > 3) return fail(LSM_ERR_ARG, "order_bc = %d (1, 2 or 3)", order_bc);
@@ -464,7 +491,7 @@ int lsm_reinit_geometric(const LsmBlock *blk, const double *phi0, double *phicor
                          void *work, size_t work_bytes, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, true);
+    int rc = make_blk(blk, &b, false, true, kBlankUse);
     if (rc) return rc;
     if (!phi0 || !phicorrected) return fail(LSM_ERR_ARG, "NULL array");
     if (phi0 == phicorrected) return fail(LSM_ERR_ARG, "phi0 and phicorrected must not alias");
@@ -530,7 +557,7 @@ int lsm_reinit_sussman_eno2(const LsmBlock *blk, const LsmReinitParams *p, const
                             void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, true, false);
+    int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
     if ((rc = check_reinit(p))) return rc;
     if (!phi0 || !phi_out) return fail(LSM_ERR_ARG, "NULL array");
@@ -546,7 +573,7 @@ int lsm_reinit_sussman_eno2_adjoint(const LsmBlock *blk, const LsmReinitParams *
                                     void *work, size_t work_bytes, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, true, false);
+    int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
     if ((rc = check_reinit(p))) return rc;
     if (p->rn_itermax > kMaxTraceIters)
@@ -658,7 +685,7 @@ int lsm_sign_ini(const LsmBlock *blk, const double *phi0, double epslsm, int m_s
                  double *sign_out, double *normgrad_out, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, true, false);
+    int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
     if (!phi0 || !sign_out || !normgrad_out) return fail(LSM_ERR_ARG, "NULL array");
     return launch_sign_ini(b, phi0, epslsm, m_sphi0, sign_out, normgrad_out, (hipStream_t)stream);
@@ -667,7 +694,7 @@ int lsm_sign_ini(const LsmBlock *blk, const double *phi0, double epslsm, int m_s
 int lsm_rh_rn(const LsmBlock *blk, const double *d, const double *sign, double *rhs, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, true, false);
+    int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
     if (!d || !sign || !rhs) return fail(LSM_ERR_ARG, "NULL array");
     return launch_rh_rn(b, d, sign, rhs, (hipStream_t)stream);
@@ -677,7 +704,7 @@ int lsm_lambda_rn(const LsmBlock *blk, const double *phi0, const double *normgra
                   double timestep, double epslsm, double *lambda, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, false);
+    int rc = make_blk(blk, &b, false, false, kBlankIgnore);
     if (rc) return rc;
     if (!phi0 || !normgrad0 || !d || !lambda) return fail(LSM_ERR_ARG, "NULL array");
     return launch_lambda_rn(b, phi0, normgrad0, d, timestep, epslsm, lambda, (hipStream_t)stream);
@@ -687,7 +714,7 @@ int lsm_reinit_godunov(const LsmBlock *blk, const LsmReinitParams *p, const doub
                        double *phi_out, void *work, size_t work_bytes, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, true, false);
+    int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
     if ((rc = check_reinit(p))) return rc;
     if (!phi0 || !phi_out) return fail(LSM_ERR_ARG, "NULL array");
@@ -702,7 +729,7 @@ int lsm_water_volume(const LsmBlock *blk, const double *phi, double *vol_out, in
                      void *work, size_t work_bytes, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, true);
+    int rc = make_blk(blk, &b, false, true, kBlankUse);
     if (rc) return rc;
     if (!phi || !vol_out) return fail(LSM_ERR_ARG, "NULL array");
     Carver c;
@@ -719,7 +746,7 @@ int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p, double *phi
 {
     const bool hybrid = p && p->hybrid && call_reinit;
     Blk b;
-    int rc = make_blk(blk, &b, true, vol_out != nullptr || hybrid);
+    int rc = make_blk(blk, &b, true, vol_out != nullptr || hybrid, kBlankUse);
     if (rc) return rc;
     if (!phi || !phi_n) return fail(LSM_ERR_ARG, "NULL array");
     if (call_levelset && (!u || !v || !w)) return fail(LSM_ERR_ARG, "NULL velocity");

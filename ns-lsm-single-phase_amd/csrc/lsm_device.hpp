@@ -353,4 +353,49 @@ struct Ext {
     __host__ __device__ long idx(int i, int j, int k) const { return (long)i + sj * j + sk * k; }
 };
 
+// Blanking regions (global_app.F90:30-43) passed to kernels by value: n
+// boxes, each (ia, ib, ja, jb, ka, kb) as 0-based node indices followed by
+// blktype(1:4).  The loops run over at most four boxes of kernel-argument
+// memory, uniform across the wavefront.
+struct Blank {
+    int n;
+    int b[4][10];        // LSM_MAX_BLANK boxes
+    // the node lies in a box (faces included)
+    __host__ __device__ bool inside(int i, int j, int k) const
+    {
+        for (int q = 0; q < n; ++q)
+            if (i >= b[q][0] && i <= b[q][1] && j >= b[q][2] && j <= b[q][3] && k >= b[q][4] && k <= b[q][5])
+                return true;
+        return false;
+    }
+    // the node lies strictly inside a box
+    __host__ __device__ bool interior(int i, int j, int k) const
+    {
+        for (int q = 0; q < n; ++q)
+            if (i > b[q][0] && i < b[q][1] && j > b[q][2] && j < b[q][3] && k > b[q][4] && k < b[q][5])
+                return true;
+        return false;
+    }
+    // the cell with lower corner (i, j, k) has that corner in [ia, ib) x [ja, jb) x [ka, kb)
+    __host__ __device__ bool cell(int i, int j, int k) const
+    {
+        for (int q = 0; q < n; ++q)
+            if (i >= b[q][0] && i < b[q][1] && j >= b[q][2] && j < b[q][3] && k >= b[q][4] && k < b[q][5])
+                return true;
+        return false;
+    }
+    // WENO3 stencils biased away from the i and j faces of a box next to the
+    // node, at every k (calc_RH_AD_WENO3.F90:152-188)
+    __host__ __device__ void bias(int i, int j, bool &mb, bool &mf, bool &ml, bool &mr) const
+    {
+        for (int q = 0; q < n; ++q) {
+            const bool jin = j >= b[q][2] && j <= b[q][3], iin = i >= b[q][0] && i <= b[q][1];
+            if (i == b[q][0] - 1 && jin) mf = true;
+            if (i == b[q][1] + 1 && jin) mb = true;
+            if (j == b[q][2] - 1 && iin) mr = true;
+            if (j == b[q][3] + 1 && iin) ml = true;
+        }
+    }
+};
+
 }  // namespace lsm

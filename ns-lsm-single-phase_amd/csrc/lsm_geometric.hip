@@ -53,6 +53,7 @@ struct Geom {
     long sj, sk;
     long csj, csk;                 // cell strides: (ni-1), (ni-1)(nj-1)
     const double *x, *y, *z, *phi0;
+    Blank bl;                      // blanking regions (read by geo_cells only)
 };
 
 // NORM2 as the Fortran runtime evaluates it for REAL(8): the largest |x| so
@@ -216,6 +217,13 @@ __global__ __launch_bounds__(256) void geo_cells(Geom g, long ncell, uint8_t *__
     if (c >= ncell) return;
     int i, j, k;
     cell_of(g, c, i, j, k);
+    if (g.bl.n && g.bl.cell(i, j, k)) {
+        // blanked cells are not searched (GetTetrahedraList.F90:209-227)
+        ctets[c] = 0;
+        cpn[c] = 0;
+        cslot[c] = -1;
+        return;
+    }
     double ph[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q)
@@ -951,7 +959,7 @@ int launch_geometric(const Blk &b, const double *phi0, double *pc, int32_t *ids,
         return fail(LSM_ERR_WORKSPACE, "workspace too small");
     if (n > 0x7fffffffL) return fail(LSM_ERR_ARG, "block of %ld nodes: at most 2^31 - 1", n);
     Geom g{b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, (long)(b.e.ni - 1), (long)(b.e.ni - 1) * (b.e.nj - 1),
-           b.x, b.y, b.z, phi0};
+           b.x, b.y, b.z, phi0, b.bl};
     int rc;
     // counts: cells with K-tetrahedra, interface nodes, neighbours
     geo_zero<<<1, 64, 0, s>>>(count);

@@ -292,6 +292,27 @@ int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype
     return LSM_OK;
 }
 
+int lsm_host_blanking(int32_t nblk, const int32_t *boxes)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    if (nblk < 0 || nblk > LSM_MAX_BLANK) return fail(LSM_ERR_ARG, "nblk = %d (0..%d)", nblk, LSM_MAX_BLANK);
+    if (nblk > 0 && !boxes) return fail(LSM_ERR_ARG, "NULL boxes");
+    LsmBlock b = g_h.blk;
+    b.nblk = nblk;
+    for (int q = 0; q < nblk; ++q) {
+        const int32_t *v = boxes + 10 * q;
+        LsmBlank &B = b.blank[q];
+        B.ia = v[0]; B.ib = v[1]; B.ja = v[2]; B.jb = v[3]; B.ka = v[4]; B.kb = v[5];
+        for (int f = 0; f < 4; ++f) B.blktype[f] = v[6 + f];
+    }
+    Blk chk;
+    if ((rc = make_blk(&b, &chk, false, false, kBlankUse))) return rc;
+    g_h.blk = b;
+    return LSM_OK;
+}
+
 int lsm_host_invalidate(void)
 {
     std::lock_guard<std::mutex> lk(g_mu);

@@ -11,6 +11,8 @@
 
 namespace lsm {
 
+static_assert(LSM_MAX_BLANK == 4, "Blank holds four boxes");
+
 // Everything a kernel needs about one block, by value.
 struct Blk {
     Ext e;
@@ -19,6 +21,7 @@ struct Blk {
     const double *csi0, *csi1, *csi2, *eta0, *eta1, *eta2, *zet0, *zet1, *zet2;
     const double *aj, *x, *y, *z;
     long n;                 // nodes in the block
+    Blank bl;               // blanking regions, 0-based (bl.n = 0: none)
 };
 
 // thread-local last-error text
@@ -26,8 +29,12 @@ int fail(int code, const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
 
 // validates the descriptor; `need_metrics` / `need_coords` select which
-// device pointers must be present
-int make_blk(const LsmBlock *b, Blk *out, bool need_metrics, bool need_coords);
+// device pointers must be present.  Blanking regions (nblk != 0) are
+// rejected unless `blanking` is kBlankUse (the entry point implements them)
+// or kBlankIgnore (its reference routine does not read them)
+enum BlankPolicy { kBlankReject = 0, kBlankUse = 1, kBlankIgnore = 2 };
+int make_blk(const LsmBlock *b, Blk *out, bool need_metrics, bool need_coords,
+             BlankPolicy blanking = kBlankReject);
 
 // library-owned buffers per (device, stream, purpose)
 enum { kBufWork = 0, kBufStore = 1 };
@@ -64,7 +71,10 @@ int launch_contra(const Blk &b, const double *u, const double *v, const double *
                   double *U1, double *U2, double *U3, hipStream_t s);
 int launch_weno3_rhs(const Blk &b, const double *phi, const double *U1, const double *U2,
                      const double *U3, const int32_t *mask, double *rhs, int mode, hipStream_t s);
+// bcond_lsm: the physical faces, then the blanking regions
 int launch_bcond(const Blk &b, double *phi, hipStream_t s);
+// the blanking part of bcond_lsm alone (nothing when b.bl.n == 0)
+int launch_bcond_blanking(const Blk &b, double *phi, hipStream_t s);
 int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t *mask,
                const double *U1, const double *U2, const double *U3, double dt, int mode,
                double *scratch, hipStream_t s, bool folded = false);

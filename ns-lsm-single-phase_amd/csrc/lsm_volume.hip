@@ -221,6 +221,7 @@ struct VolArgs {
     int ni, nj, nk;
     long sj, sk;
     int tiles_i, tiles_j;
+    Blank bl;           // cells whose lower corner lies in a box are not counted
 };
 
 struct CellTile {
@@ -266,7 +267,8 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double *out)
 }
 
 // TotalWaterVolume.F90:136-294, fully wet cells (HexahedronVolume) and the
-// classification of the rest
+// classification of the rest; a cell whose lower corner node lies in a
+// blanking box is skipped (TotalWaterVolume.F90:139-160)
 __global__ __launch_bounds__(VTX * VTY) void volume_wet(VolArgs a)
 {
     const CellTile c = tile_of(a);
@@ -275,6 +277,7 @@ __global__ __launch_bounds__(VTX * VTY) void volume_wet(VolArgs a)
     bool cut = false;
     if (c.ok) {
         for (int k = c.k0; k < c.k1; ++k) {
+            if (a.bl.n && a.bl.inside(c.i, c.j, k)) continue;
             const long l = (long)c.i + a.sj * c.j + a.sk * k;
             bool allp = true, alln = true;
 #pragma unroll
@@ -317,7 +320,7 @@ __global__ __launch_bounds__(VTX * VTY) void volume_cut(VolArgs a)
     int n = 0;
     for (int k = c.k0; k < c.k1; ++k) {
         bool cut = false;
-        if (c.ok) {
+        if (c.ok && !(a.bl.n && a.bl.inside(c.i, c.j, k))) {
             const long l = (long)c.i + a.sj * c.j + a.sk * k;
             bool allp = true, alln = true;
 #pragma unroll
@@ -367,7 +370,9 @@ __global__ __launch_bounds__(VTX * VTY) void volume_cut(VolArgs a)
 // neighbourhood (clipped to the block) has the opposite sign or |phi| <
 // eps_sims; the mask is 1 on the neighbourhood of every such node.  Pass 1
 // writes the per-node flag (min / max / min |.| over the neighbourhood), pass
-// 2 the dilation, so every mask entry is written exactly once.
+// 2 the dilation, so every mask entry is written exactly once.  Nodes in a
+// blanking box are not flagged and nodes strictly inside one are left out of
+// the neighbourhood search (TotalWaterVolume.F90:139-160, 190-207).
 struct MaskArgs {
     const double *phi;
     uint8_t *chg;
@@ -375,6 +380,7 @@ struct MaskArgs {
     int ni, nj, nk;
     long sj, sk;
     int tiles_i, tiles_j;
+    Blank bl;           // nodes in a box are not interface nodes; nodes strictly inside are not searched
 };
 
 __global__ __launch_bounds__(VTX * VTY) void mask_change(MaskArgs a)
@@ -385,7 +391,7 @@ __global__ __launch_bounds__(VTX * VTY) void mask_change(MaskArgs a)
     if (i >= a.ni || j >= a.nj) return;
     const long l = (long)i + a.sj * j + a.sk * k;
     uint8_t flag = 0;
-    if (i <= a.ni - 2 && j <= a.nj - 2 && k <= a.nk - 2) {
+    if (i <= a.ni - 2 && j <= a.nj - 2 && k <= a.nk - 2 && !(a.bl.n && a.bl.inside(i, j, k))) {
         const double p = a.phi[l];
         double mn = p, mx = p, ma = fabs(p);
         const int km = max(k - 1, 0), kp = min(k + 1, a.nk - 1);
@@ -394,6 +400,7 @@ __global__ __launch_bounds__(VTX * VTY) void mask_change(MaskArgs a)
         for (int kk = km; kk <= kp; ++kk)
             for (int jj = jm; jj <= jp; ++jj)
                 for (int ii = im; ii <= ip; ++ii) {
+                    if (a.bl.n && a.bl.interior(ii, jj, kk)) continue;
                     const double q = a.phi[(long)ii + a.sj * jj + a.sk * kk];
                     mn = fmin(mn, q); mx = fmax(mx, q); ma = fmin(ma, fabs(q));
                 }
@@ -472,7 +479,7 @@ int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *m
     // vol == NULL: the AdvectionNodes mask alone
     if (vol) {
         VolArgs a{phi, b.x, b.y, b.z, partial, cutflag, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk,
-                  (b.e.ni - 1 + VTX - 1) / VTX, (b.e.nj - 1 + VTY - 1) / VTY};
+                  (b.e.ni - 1 + VTX - 1) / VTX, (b.e.nj - 1 + VTY - 1) / VTY, b.bl};
         if (timed) {
             KTimer kt(LSM_KT_VOLUME, s);
             volume_wet<<<(unsigned)nblk, blk, 0, s>>>(a);
@@ -487,7 +494,7 @@ int launch_water_volume(const Blk &b, const double *phi, double *vol, int32_t *m
     }
     if (mask) {
         MaskArgs m{phi, chg, mask, b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, (b.e.ni + VTX - 1) / VTX,
-                   (b.e.nj + VTY - 1) / VTY};
+                   (b.e.nj + VTY - 1) / VTY, b.bl};
         const unsigned mb = (unsigned)(m.tiles_i * m.tiles_j * b.e.nk);
         mask_change<<<mb, blk, 0, s>>>(m);
         mask_dilate<<<mb, blk, 0, s>>>(m);
@@ -541,7 +548,7 @@ extern "C" int lsm_water_volume_noalloc(const LsmBlock *blk, const double *phi, 
 {
     using namespace lsm;
     Blk b;
-    int rc = make_blk(blk, &b, false, true);
+    int rc = make_blk(blk, &b, false, true, kBlankUse);
     if (rc) return rc;
     if (!phi || !vol_out) return fail(LSM_ERR_ARG, "NULL array");
     const size_t need = volume_scratch_bytes(b, adv_mask != nullptr);

@@ -36,6 +36,13 @@ module lsm_hip_f
          real(c_double), intent(in)     :: csi(*), eta(*), zet(*), aj(*), x(*), y(*), z(*)
       end function lsm_host_setup
 
+      ! blanking regions of the bound block: nblk rows of (li_blk_ia, ib, ja, jb, ka, kb, blktype(1:4))
+      integer(c_int) function lsm_host_blanking(nblk, boxes) bind(C, name="lsm_host_blanking")
+         import :: c_int, c_int32_t
+         integer(c_int32_t), value      :: nblk
+         integer(c_int32_t), intent(in) :: boxes(*)
+      end function lsm_host_blanking
+
       ! the next lsm_host_setup uploads the grid again (after in-place edits of grid arrays)
       integer(c_int) function lsm_host_invalidate() bind(C, name="lsm_host_invalidate")
          import :: c_int
@@ -228,6 +235,25 @@ contains
       call lsm_f_check(lsm_host_setup(e, faces, bt, dc, de, dz, big_phi, &
                                       csi, eta, zet, aj, x, y, z), 'lsm_host_setup')
    end subroutine lsm_f_bind
+
+   ! Passes the blanking regions of global_app (one block per rank: the
+   ! interior boxes li_blk_* and the exterior boxes le_blk_* coincide) to the
+   ! block bound by lsm_f_bind.
+   subroutine lsm_f_bind_blanking(nblk, nblke, ia, ib, ja, jb, ka, kb, blktype)
+      integer, intent(in) :: nblk, nblke
+      integer, intent(in) :: ia(:,:), ib(:,:), ja(:,:), jb(:,:), ka(:,:), kb(:,:), blktype(:,:)
+      integer(c_int32_t) :: boxes(10, 4)
+      integer :: nb
+      if (nblk /= nblke) call lsm_f_unsupported('level-set drop-in', &
+                                                'blanking regions across rank boundaries (nblk /= nblke)')
+      if (nblk > 4) call lsm_f_unsupported('level-set drop-in', 'more than 4 blanking regions')
+      boxes = 0
+      do nb = 1, nblk
+         boxes(1:6, nb) = int([ia(1, nb), ib(1, nb), ja(1, nb), jb(1, nb), ka(1, nb), kb(1, nb)], c_int32_t)
+         boxes(7:10, nb) = int(blktype(1:4, nb), c_int32_t)
+      end do
+      call lsm_f_check(lsm_host_blanking(int(nblk, c_int32_t), boxes), 'lsm_host_blanking')
+   end subroutine lsm_f_bind_blanking
 
    ! physical-face mask from the neighbour ranks (back, front, left, right,
    ! down, up; a face is physical when its neighbour is proc_null)

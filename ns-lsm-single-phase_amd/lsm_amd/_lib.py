@@ -25,6 +25,14 @@ LSM_OP_REINIT_GEOMETRIC = 11
 _vp = C.c_void_p
 
 
+LSM_MAX_BLANK = 4
+
+
+class LsmBlank(C.Structure):
+    _fields_ = [("ia", C.c_int32), ("ib", C.c_int32), ("ja", C.c_int32), ("jb", C.c_int32),
+                ("ka", C.c_int32), ("kb", C.c_int32), ("blktype", C.c_int32 * 4)]
+
+
 class LsmBlock(C.Structure):
     _fields_ = [("il", C.c_int32), ("iu", C.c_int32), ("jl", C.c_int32), ("ju", C.c_int32),
                 ("kl", C.c_int32), ("ku", C.c_int32),
@@ -33,7 +41,8 @@ class LsmBlock(C.Structure):
                 ("dc", C.c_double), ("de", C.c_double), ("dz", C.c_double),
                 ("big_phi", C.c_double),
                 ("csi", _vp * 3), ("eta", _vp * 3), ("zet", _vp * 3),
-                ("aj", _vp), ("x", _vp), ("y", _vp), ("z", _vp)]
+                ("aj", _vp), ("x", _vp), ("y", _vp), ("z", _vp),
+                ("nblk", C.c_int32), ("blank", LsmBlank * LSM_MAX_BLANK)]
 
 
 class LsmReinitParams(C.Structure):
@@ -132,6 +141,7 @@ _SIGS = {
                                  C.c_double, C.c_double, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "lsm_host_release": (C.c_int, []),
     "lsm_host_invalidate": (C.c_int, []),
+    "lsm_host_blanking": (C.c_int, [C.c_int32, _vp]),
     "lsm_host_levelset_step": (C.c_int, [_vp, _vp, _vp, C.POINTER(LsmReinitParams), C.c_int, C.c_double,
                                          C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),
     "lsm_host_contra_velocity": (C.c_int, [_vp, _vp]),

@@ -51,7 +51,9 @@ class Block:
     face/boundary description (the host-associated state of levelsetmethod)."""
 
     def __init__(self, grid, btype: Sequence[int] = (1, 1, 1, 1, 1, 1), big_phi: float = 5.0,
-                 device="cuda"):
+                 device="cuda", blanking=()):
+        """blanking: up to four regions ((ia, ib, ja, jb, ka, kb), (blktype 1..4)) with
+        0-based node indices of the numpy arrays (the reference's li_blk_* - 1)."""
         def dev(a):
             return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
 
@@ -74,6 +76,14 @@ class Block:
         s.zet[:] = [t.data_ptr() for t in self.zet]
         s.aj = self.aj.data_ptr()
         s.x, s.y, s.z = self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
+        if len(blanking) > _lib.LSM_MAX_BLANK:
+            raise ValueError(f"at most {_lib.LSM_MAX_BLANK} blanking regions")
+        s.nblk = len(blanking)
+        for q, (box, bt) in enumerate(blanking):
+            B = s.blank[q]
+            B.ia, B.ib, B.ja, B.jb, B.ka, B.kb = [int(v) + 1 for v in box]   # array indices (il = 1)
+            B.blktype[:] = [int(v) for v in bt]
+        self.blanking = tuple(blanking)
         self.desc = s
         self.device = self.aj.device
 

@@ -26,6 +26,11 @@ typedef struct {
     const double *csi[3], *eta[3], *zet[3];   /* grad xi, grad eta, grad zeta */
     const double *aj;
     const double *x, *y, *z;
+    /* blanking regions (global_app.F90:30-43; one rank: li_blk = le_blk):
+       nblk boxes, each (ia, ib, ja, jb, ka, kb) as 0-based node indices,
+       then blktype(1:4) of the box */
+    int32_t nblk;
+    int32_t blank[4][10];
 } OBlock;
 
 typedef struct {
@@ -177,7 +182,7 @@ void oracle_contra(const OBlock *b, const double *u, const double *v, const doub
 }
 
 /* ------------------------------------------------------------------------- */
-/* R4: calc_RH_AD_WENO3, calc_RH_AD_WENO3.F90:86-403 (no blanking)            */
+/* R4: calc_RH_AD_WENO3, calc_RH_AD_WENO3.F90:86-403 (blanking: 152-188)    */
 /* writes interior nodes only; the caller zeroes rh (ad_RKTVD3.F90:55)        */
 /* ------------------------------------------------------------------------- */
 void oracle_weno3_rhs(const OBlock *b, const double *phi, const double *U1,
@@ -190,9 +195,21 @@ void oracle_weno3_rhs(const OBlock *b, const double *phi, const double *U1,
     for (int k = 1; k <= nk - 2; ++k) {
         size_t l = IX(b, i, j, k);
         if (mask[l] == 0) continue;
-        const int mb = (i == 1), mf = (i == ni - 2);
-        const int ml = (j == 1), mr = (j == nj - 2);
+        int mb = (i == 1), mf = (i == ni - 2);
+        int ml = (j == 1), mr = (j == nj - 2);
         const int md = (k == 1), mu = (k == nk - 2);
+        /* blanking: nodes inside a box are skipped, the stencils next to a
+           box's i and j faces biased away from it (calc_RH_AD_WENO3.F90:152-188) */
+        int inside = 0;
+        for (int q = 0; q < b->nblk; ++q) {
+            const int32_t *B = b->blank[q];
+            if (i >= B[0] && i <= B[1] && j >= B[2] && j <= B[3] && k >= B[4] && k <= B[5]) inside = 1;
+            if (i == B[0] - 1 && j >= B[2] && j <= B[3]) mf = 1;
+            if (i == B[1] + 1 && j >= B[2] && j <= B[3]) mb = 1;
+            if (j == B[2] - 1 && i >= B[0] && i <= B[1]) mr = 1;
+            if (j == B[3] + 1 && i >= B[0] && i <= B[1]) ml = 1;
+        }
+        if (inside) continue;
         const int bnd[3][2] = {{mb, mf}, {ml, mr}, {md, mu}};
         const size_t st[3] = {si, sj, sk};
         const double *Uc[3] = {U1, U2, U3};
@@ -233,6 +250,69 @@ static void bc_face_pair(double *phi, size_t lb, size_t l1, size_t l2, int bt, d
     }
 }
 
+/* the blanking regions of bcond_lsm.F90:290-562, box by box: the nodes two
+   and more inside the box zeroed, then the box's i faces, j faces and the four
+   i-j corners extrapolated from outside (second order, first order where the
+   farther node is at least BigPhi), each written twice (the face node and
+   the node behind it), where the box's blktype is 0 and the face lies at
+   least two nodes inside the block.  k faces are not treated (commented out
+   in the reference). */
+static void blank_extrap(double *phi, size_t lb, size_t lbehind, size_t l1, size_t l2, double big)
+{
+    const double a2 = 4.0 / 3.0, b2 = -1.0 / 3.0, a1 = 1.0, b1 = 0.0;
+    const double ex = (1.0 + fsign1(fabs(phi[l2]) - big)) / 2.0;
+    const double ta = a2 + ex * (a1 - a2);
+    const double tb = b2 + ex * (b1 - b2);
+    phi[lb] = ta * phi[l1] + tb * phi[l2];
+    phi[lbehind] = phi[lb];
+}
+
+static inline int imax(int a, int c) { return a > c ? a : c; }
+static inline int imin(int a, int c) { return a < c ? a : c; }
+
+void oracle_bcond_blanking(const OBlock *b, double *phi)
+{
+    const int ni = b->ni, nj = b->nj, nk = b->nk;
+    const double big = b->big_phi;
+    for (int q = 0; q < b->nblk; ++q) {
+        const int32_t *B = b->blank[q];
+        const int ia = B[0], ib = B[1], ja = B[2], jb = B[3], ka = B[4], kb = B[5];
+        const int *bt = &B[6];
+        for (int k = imax(0, ka + 2); k <= imin(nk - 1, kb - 2); ++k)
+        for (int j = imax(0, ja + 2); j <= imin(nj - 1, jb - 2); ++j)
+        for (int i = imax(0, ia + 2); i <= imin(ni - 1, ib - 2); ++i) phi[IX(b, i, j, k)] = 0.0;
+        const int k0 = imax(0, ka), k1 = imin(nk - 1, kb);
+        if (bt[0] == 0 && ia > 1)
+            for (int k = k0; k <= k1; ++k) for (int j = imax(0, ja); j <= imin(nj - 1, jb); ++j)
+                blank_extrap(phi, IX(b, ia, j, k), IX(b, ia + 1, j, k), IX(b, ia - 1, j, k), IX(b, ia - 2, j, k), big);
+        if (bt[1] == 0 && ib < ni - 2)
+            for (int k = k0; k <= k1; ++k) for (int j = imax(0, ja); j <= imin(nj - 1, jb); ++j)
+                blank_extrap(phi, IX(b, ib, j, k), IX(b, ib - 1, j, k), IX(b, ib + 1, j, k), IX(b, ib + 2, j, k), big);
+        if (bt[2] == 0 && ja > 1)
+            for (int k = k0; k <= k1; ++k) for (int i = imax(0, ia); i <= imin(ni - 1, ib); ++i)
+                blank_extrap(phi, IX(b, i, ja, k), IX(b, i, ja + 1, k), IX(b, i, ja - 1, k), IX(b, i, ja - 2, k), big);
+        if (bt[3] == 0 && jb < nj - 2)
+            for (int k = k0; k <= k1; ++k) for (int i = imax(0, ia); i <= imin(ni - 1, ib); ++i)
+                blank_extrap(phi, IX(b, i, jb, k), IX(b, i, jb - 1, k), IX(b, i, jb + 1, k), IX(b, i, jb + 2, k), big);
+        if (bt[0] == 0 && ia > 1 && bt[2] == 0 && ja > 1)
+            for (int k = k0; k <= k1; ++k)
+                blank_extrap(phi, IX(b, ia, ja, k), IX(b, ia + 1, ja + 1, k), IX(b, ia - 1, ja - 1, k),
+                             IX(b, ia - 2, ja - 2, k), big);
+        if (bt[1] == 0 && ib < ni - 2 && bt[2] == 0 && ja > 1)
+            for (int k = k0; k <= k1; ++k)
+                blank_extrap(phi, IX(b, ib, ja, k), IX(b, ib - 1, ja + 1, k), IX(b, ib + 1, ja - 1, k),
+                             IX(b, ib + 2, ja - 2, k), big);
+        if (bt[1] == 0 && ib < ni - 2 && bt[3] == 0 && jb < nj - 2)
+            for (int k = k0; k <= k1; ++k)
+                blank_extrap(phi, IX(b, ib, jb, k), IX(b, ib - 1, jb - 1, k), IX(b, ib + 1, jb + 1, k),
+                             IX(b, ib + 2, jb + 2, k), big);
+        if (bt[0] == 0 && ia > 1 && bt[3] == 0 && jb < nj - 2)
+            for (int k = k0; k <= k1; ++k)
+                blank_extrap(phi, IX(b, ia, jb, k), IX(b, ia + 1, jb - 1, k), IX(b, ia - 1, jb + 1, k),
+                             IX(b, ia - 2, jb + 2, k), big);
+    }
+}
+
 void oracle_bcond(const OBlock *b, double *phi)
 {
     const int ni = b->ni, nj = b->nj, nk = b->nk;
@@ -248,6 +328,7 @@ void oracle_bcond(const OBlock *b, double *phi)
         bc_face_pair(phi, IX(b, i, j, 0), IX(b, i, j, 1), IX(b, i, j, 2), b->btype[4], b->big_phi, 0);
     for (int j = 0; j < nj; ++j) for (int i = 0; i < ni; ++i)
         bc_face_pair(phi, IX(b, i, j, nk - 1), IX(b, i, j, nk - 2), IX(b, i, j, nk - 3), b->btype[5], b->big_phi, 0);
+    oracle_bcond_blanking(b, phi);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -752,7 +833,7 @@ static void cell_analysis(const double ph[8], const double V[8][3], double *pc_v
     }
 }
 
-/* TotalWaterVolume, TotalWaterVolume.F90:69-294 (no blanking).  vol[0..2] =
+/* TotalWaterVolume, TotalWaterVolume.F90:69-294 (blanking: 139-207).  vol[0..2] =
    (K-tetrahedra, second tetrahedra, bulk cells); mask = AdvectionNodes. */
 void oracle_water_volume(const OBlock *b, const double *phi, double vol[3], int32_t *mask)
 {
@@ -764,6 +845,14 @@ void oracle_water_volume(const OBlock *b, const double *phi, double vol[3], int3
     for (int k = 0; k <= nk - 2; ++k)
     for (int j = 0; j <= nj - 2; ++j)
     for (int i = 0; i <= ni - 2; ++i) {
+        /* blanking: a node inside a box is neither an interface node nor the
+           corner of a counted cell (TotalWaterVolume.F90:139-160) */
+        int blanked = 0;
+        for (int q = 0; q < b->nblk; ++q) {
+            const int32_t *B = b->blank[q];
+            if (i >= B[0] && i <= B[1] && j >= B[2] && j <= B[3] && k >= B[4] && k <= B[5]) blanked = 1;
+        }
+        if (blanked) continue;
         const int km = k - 1 < 0 ? 0 : k - 1, kp = k + 1 > nk - 1 ? nk - 1 : k + 1;
         const int jm = j - 1 < 0 ? 0 : j - 1, jp = j + 1 > nj - 1 ? nj - 1 : j + 1;
         const int im = i - 1 < 0 ? 0 : i - 1, ip = i + 1 > ni - 1 ? ni - 1 : i + 1;
@@ -774,6 +863,13 @@ void oracle_water_volume(const OBlock *b, const double *phi, double vol[3], int3
             for (int kk = km; kk <= kp && !change; ++kk)
             for (int jj = jm; jj <= jp && !change; ++jj)
             for (int ii = im; ii <= ip; ++ii) {
+                /* nodes strictly inside a box are not searched (TotalWaterVolume.F90:190-207) */
+                int skip = 0;
+                for (int q = 0; q < b->nblk; ++q) {
+                    const int32_t *B = b->blank[q];
+                    if (ii > B[0] && ii < B[1] && jj > B[2] && jj < B[3] && kk > B[4] && kk < B[5]) skip = 1;
+                }
+                if (skip) continue;
                 const double q = phi[IX(b, ii, jj, kk)];
                 if (sgn * q < 0.0 || fabs(q) < EPS_SIMS) { change = 1; break; }
             }
@@ -1201,6 +1297,14 @@ int oracle_geometric_reinit(const OBlock *b, const double *phi0, double *pc, int
         for (int q = 0; q < 8; ++q) ph[q] = phi0[IX(b, i + CELL_VTX[q][0], j + CELL_VTX[q][1], k + CELL_VTX[q][2])];
         int pn;
         const size_t c = (size_t)i + (size_t)(ni - 1) * ((size_t)j + (size_t)(nj - 1) * k);
+        /* blanking: cells whose corner node lies in [ia, ib) x [ja, jb) x [ka, kb)
+           of a box are not searched (GetTetrahedraList.F90:209-227) */
+        int blanked = 0;
+        for (int q = 0; q < b->nblk; ++q) {
+            const int32_t *B = b->blank[q];
+            if (i >= B[0] && i < B[1] && j >= B[2] && j < B[3] && k >= B[4] && k < B[5]) blanked = 1;
+        }
+        if (blanked) { cell_tets[c] = 0; cell_slot[c] = -1; continue; }
         cell_tets[c] = cell_ktets(ph, &pn);
         cell_slot[c] = cell_tets[c] ? nslots++ : -1;
         for (int q = 0; q < 8; ++q)

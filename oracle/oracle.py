@@ -24,7 +24,8 @@ class OBlock(C.Structure):
                 ("dc", C.c_double), ("de", C.c_double), ("dz", C.c_double),
                 ("big_phi", C.c_double),
                 ("csi", _dp * 3), ("eta", _dp * 3), ("zet", _dp * 3),
-                ("aj", _dp), ("x", _dp), ("y", _dp), ("z", _dp)]
+                ("aj", _dp), ("x", _dp), ("y", _dp), ("z", _dp),
+                ("nblk", C.c_int32), ("blank", (C.c_int32 * 10) * 4)]
 
 
 class OReinit(C.Structure):
@@ -92,7 +93,9 @@ def _c(a, dtype=np.float64):
 class Block:
     """Keeps the numpy arrays alive while the C struct points at them."""
 
-    def __init__(self, grid, btype=(1, 1, 1, 1, 1, 1), big_phi=5.0):
+    def __init__(self, grid, btype=(1, 1, 1, 1, 1, 1), big_phi=5.0, blanking=()):
+        """blanking: up to 4 boxes ((ia, ib, ja, jb, ka, kb), (bt1, bt2, bt3, bt4))
+        with 0-based node indices (the reference's li_blk_* - 1) and blktype(1:4)."""
         self.keep = [_c(a) for a in (grid.csi[0], grid.csi[1], grid.csi[2],
                                      grid.eta[0], grid.eta[1], grid.eta[2],
                                      grid.zet[0], grid.zet[1], grid.zet[2],
@@ -108,6 +111,11 @@ class Block:
         s.eta[:] = [_p(k[3]), _p(k[4]), _p(k[5])]
         s.zet[:] = [_p(k[6]), _p(k[7]), _p(k[8])]
         s.aj, s.x, s.y, s.z = _p(k[9]), _p(k[10]), _p(k[11]), _p(k[12])
+        if len(blanking) > 4:
+            raise ValueError("at most 4 blanking boxes")
+        s.nblk = len(blanking)
+        for q, (box, bt) in enumerate(blanking):
+            s.blank[q][:] = [int(v) for v in box] + [int(v) for v in bt]
         self.s = s
         self.shape = grid.shape
 

@@ -36,7 +36,9 @@ program lsm_ref_harness
                                li_ia, li_ib, li_ja, li_jb, li_ka, li_kb
    use global_mpi
    use global_lsm
-   use global_app,      only : nblk, nblke
+   use global_app,      only : nblk, nblke, blktype,                               &
+                               li_blk_ia, li_blk_ib, li_blk_ja, li_blk_jb, li_blk_ka, li_blk_kb, &
+                               le_blk_ia, le_blk_ib, le_blk_ja, le_blk_jb, le_blk_ka, le_blk_kb
    use global_param,    only : ntime, delti
    use global_obstacle, only : is_obstacle, act_obstacle_ad, act_obstacle_rn
    use global_debug,    only : OUTPUTdir
@@ -99,6 +101,24 @@ program lsm_ref_harness
    MPI_REAL_TYPE = MPI_DOUBLE_PRECISION
    nblk  = 0
    nblke = 0
+   ! a blanking region (init.F90:1216-1567 with one rank: the interior and the
+   ! extended boxes coincide; nba_max = 1, global.F90:21): ip(31) = 1, the box
+   ! li_blk ia, ib, ja, jb, ka, kb in ip(32:37), blktype(1:6) in ip(38:43)
+   allocate(blktype(6,1,1))
+   blktype = 0
+   if (ip(31) /= 0) then
+      nblk  = 1
+      nblke = 1
+      allocate(li_blk_ia(1,1), li_blk_ib(1,1), li_blk_ja(1,1), li_blk_jb(1,1), li_blk_ka(1,1), li_blk_kb(1,1))
+      allocate(le_blk_ia(1,1), le_blk_ib(1,1), le_blk_ja(1,1), le_blk_jb(1,1), le_blk_ka(1,1), le_blk_kb(1,1))
+      li_blk_ia(1,1) = ip(32) ; li_blk_ib(1,1) = ip(33)
+      li_blk_ja(1,1) = ip(34) ; li_blk_jb(1,1) = ip(35)
+      li_blk_ka(1,1) = ip(36) ; li_blk_kb(1,1) = ip(37)
+      le_blk_ia = li_blk_ia ; le_blk_ib = li_blk_ib
+      le_blk_ja = li_blk_ja ; le_blk_jb = li_blk_jb
+      le_blk_ka = li_blk_ka ; le_blk_kb = li_blk_kb
+      blktype(1:6,1,1) = ip(38:43)
+   end if
    is_obstacle = 0
    ! the 1-D line communicators of one rank (calc_h reads comm1d_csi(1)%myrank)
    allocate(comm1d_csi(1), comm1d_eta(1), comm1d_zet(1))

@@ -43,7 +43,7 @@ def run(mode, grid, phi, phi0=None, aux=None, mask=None, u=None, v=None, w=None,
         hybrid=False, call_reinit=True, rn_freq=1, ntime0=1,
         sussman_method=0, call_levelset=True, delti_lsm=1e-3, epslsm=0.05,
         rn_timestep=1e-3, eps_reinit=1e-5, big_phi=5.0, lambda_dt=1e-3,
-        timing=False, workdir=None, nprocs=1, harness=None):
+        timing=False, workdir=None, nprocs=1, harness=None, blanking=None):
     """Run one harness mode; returns (list of outputs, seconds or None, extra).
     harness: path of the executable (default: the reference harness)."""
     nk, nj, ni = grid.shape
@@ -66,6 +66,12 @@ def run(mode, grid, phi, phi0=None, aux=None, mask=None, u=None, v=None, w=None,
     ip[22] = sussman_method
     ip[23] = int(bool(call_levelset))
     ip[24] = int(bool(timing))
+    if blanking is not None:
+        # one blanking box: ((ia, ib, ja, jb, ka, kb) 0-based, blktype(1:6))
+        box, bt = blanking
+        ip[30] = 1
+        ip[31:37] = [int(v) + 1 for v in box]
+        ip[37:43] = list(bt) + [0] * (6 - len(bt))
     rp = np.zeros(32, dtype="<f8")
     rp[0:3] = (grid.dc, grid.de, grid.dz)
     rp[3] = delti_lsm

@@ -64,7 +64,7 @@ def case(name, g, phi, phi0, u, v, w, params, runs, extra_inputs=None):
 RUN_KEYS = {"nsteps", "btype", "order_ad", "numiter_lsm", "rn_itermax", "order_reinit_bc",
             "eno_bc", "m_sphi0", "m_ti", "hybrid", "call_reinit", "rn_freq", "ntime0",
             "sussman_method", "call_levelset", "delti_lsm", "epslsm", "rn_timestep",
-            "eps_reinit", "big_phi", "lambda_dt"}
+            "eps_reinit", "big_phi", "lambda_dt", "blanking"}
 
 
 def curvilinear_case():
@@ -288,8 +288,48 @@ def hybrid_case():
          extra_inputs=extra)
 
 
+# blanking regions: ((ia, ib, ja, jb, ka, kb) 0-based, blktype(1:6)); the
+# reference reads blktype(1:4) only
+BLANK_BOXES = {
+    "inner": ((6, 11, 5, 9, 3, 9), (0, 0, 0, 0, 0, 0)),       # every face extrapolated
+    "tall": ((2, 8, 2, 11, 0, 11), (0, 1, 0, 0, 0, 0)),       # all k planes, ib face kept
+    "edge": ((9, 14, 1, 6, 4, 7), (0, 0, 1, 0, 0, 0)),        # ib and ja next to the block faces
+    "thin": ((12, 12, 10, 11, 2, 5), (0, 0, 0, 0, 0, 0)),     # one node thick in i, two in j
+}
+
+
+def blanking_case():
+    """Blanking regions (nblk = 1, global_app.F90:30-43): the biased WENO3
+    stencils next to a box, the box part of bcond_lsm, advection, the water
+    volume and AdvectionNodes, levelsetmethod (Sussman, hybrid geometric and
+    orderAD = 0) on a distorted cube."""
+    g = inputs.sinusoidal_cube_grid(16, amp=0.03, seed=2, nj=14, nk=12)
+    phi = inputs.spheres_sdf(g, nspheres=4, seed=3, rmin=0.15, rmax=0.3)
+    phi = phi * (0.6 + (g.x - 0.4) ** 2)
+    u, v, w = inputs.fourier_velocity(g, seed=4)
+    dt = inputs.cfl_dt(g, u, v, w)
+    hmin, hmax = inputs.spacing_bounds(g)
+    params = dict(delti_lsm=dt, epslsm=1.5 * hmax, rn_timestep=0.5 * hmin, eps_reinit=1e-5,
+                  big_phi=5.0, btype=(1, 1, 1, 1, 1, 1))
+    runs = []
+    for name, bl in BLANK_BOXES.items():
+        runs += [(f"weno_rhs_{name}", "weno_rhs", {"blanking": bl}, {}),
+                 (f"bcond_{name}", "bcond", {"blanking": bl}, {}),
+                 (f"bcond_mixed_{name}", "bcond", {"blanking": bl, "btype": (0, 2, 1, 2, 1, 0),
+                                                   "big_phi": 0.25}, {}),
+                 (f"advect_{name}", "advect", {"blanking": bl, "nsteps": 2}, {}),
+                 (f"volume_{name}", "volume", {"blanking": bl}, {})]
+    bl = BLANK_BOXES["inner"]
+    runs += [("levelset_inner", "levelset", {"blanking": bl, "nsteps": 2, "rn_itermax": 4}, {}),
+             ("hybrid_inner", "levelset", {"blanking": bl, "nsteps": 2, "hybrid": True, "rn_freq": 1}, {}),
+             ("upwind_inner", "advect", {"blanking": bl, "nsteps": 2, "order_ad": 0}, {}),
+             ("levelset_tall", "levelset", {"blanking": BLANK_BOXES["tall"], "nsteps": 1, "rn_itermax": 3}, {})]
+    case("blanking", g, phi, phi, u, v, w, params, runs)
+
+
 CASES = {"curvilinear": curvilinear_case, "channel": channel_case, "boundary": boundary_case,
-         "volume": volume_case, "consumers": consumers_case, "upwind": upwind_case, "hybrid": hybrid_case}
+         "volume": volume_case, "consumers": consumers_case, "upwind": upwind_case, "hybrid": hybrid_case,
+         "blanking": blanking_case}
 
 if __name__ == "__main__":
     if not refio.available():

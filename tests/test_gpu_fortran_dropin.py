@@ -43,6 +43,13 @@ def replay(fx, key, mode, harness=refio.SHIM_HARNESS, **over):
         v = fx.kw(key, k)
         if v is not None:
             args[k] = v
+    bl = fx.d.get(f"kw_{key}_blanking")
+    if bl is not None:
+        args["blanking"] = (tuple(int(t) for t in bl[0]), tuple(int(t) for t in bl[1]))
+    for k in ("order_ad", "hybrid", "rn_freq"):
+        v = fx.kw(key, k)
+        if v is not None:
+            args[k] = v
     phi = over.pop("phi", fx.inp("phi"))
     phi0 = over.pop("phi0", fx.inp("phi0"))
     u, v, w = fx.velocity
@@ -236,6 +243,74 @@ def test_dropin_hybrid_levelsetmethod(name, key, harness):
         over.update(phi=p, phi0=p)
     outs, _ = replay(fx, key, "levelset", harness=harness, **over)
     assert np.array_equal(outs[0], fx.out(key)), linf_rel(outs[0], fx.out(key))
+
+
+# blanking regions (fixture "blanking", one box per run)
+BLANK_EXACT = [(f"{kind}_{box}", mode) for box in ("inner", "tall", "edge", "thin")
+               for kind, mode in (("weno_rhs", "weno_rhs"), ("bcond", "bcond"), ("bcond_mixed", "bcond"),
+                                  ("advect", "advect"))] + [("upwind_inner", "advect")]
+
+
+@pytest.mark.parametrize("key,mode", BLANK_EXACT, ids=[k for k, _ in BLANK_EXACT])
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(refio.SHIM_HARNESS),
+                    reason="drop-in harness not built (make -C oracle/ref shim)")
+def test_dropin_blanking_bitwise(key, mode):
+    """The drop-ins with a blanking region of global_app (nblk = 1): the
+    biased WENO3 stencils, the box part of bcond_lsm and advection, bit for
+    bit; the water volume to 1e-13 with the mask identical."""
+    fx = load("blanking")
+    outs, _ = replay(fx, key, mode)
+    n = 0
+    while f"out_{key}_{n}" in fx.d:
+        assert np.array_equal(outs[n], fx.out(key, n)), (key, n, linf_rel(outs[n], fx.out(key, n)))
+        n += 1
+    assert n == len(outs)
+
+
+@pytest.mark.parametrize("box", ["inner", "tall", "edge", "thin"])
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(refio.SHIM_HARNESS),
+                    reason="drop-in harness not built (make -C oracle/ref shim)")
+def test_dropin_blanking_volume(box):
+    fx = load("blanking")
+    key = f"volume_{box}"
+    outs, _ = replay(fx, key, "volume")
+    ref = fx.out(key, 0)
+    assert np.max(np.abs(outs[0] - ref)) <= 1e-13 * max(np.sum(ref), 1e-30)
+    assert np.array_equal(outs[1], fx.out(key, 1))
+
+
+@pytest.mark.parametrize("key", ["levelset_inner", "levelset_tall", "hybrid_inner"])
+@pytest.mark.parametrize("harness", HARNESSES)
+@pytest.mark.gpu
+def test_dropin_blanking_levelsetmethod(key, harness):
+    """levelsetmethod with a blanking region through the callee drop-ins and
+    the whole-call drop-in: bit for bit on the hybrid path, 1e-13 where the
+    Sussman reinitialisation calls device sin/cos."""
+    fx = load("blanking")
+    outs, extra = replay(fx, key, "levelset", harness=harness)
+    if key.startswith("hybrid"):
+        assert np.array_equal(outs[0], fx.out(key)), linf_rel(outs[0], fx.out(key))
+    else:
+        assert linf_rel(outs[0], fx.out(key)) <= 1e-13
+    gv = fx.d[f"out_{key}_volume"]
+    assert np.max(np.abs(extra["global_volume"] - gv)) <= 1e-12 * np.max(np.abs(gv))
+
+
+@pytest.mark.skipif(not refio.available(), reason="reference harness not built (make -C oracle/ref)")
+@pytest.mark.parametrize("key,mode", BLANK_EXACT + [("levelset_inner", "levelset"), ("hybrid_inner", "levelset")],
+                         ids=[k for k, _ in BLANK_EXACT] + ["levelset_inner", "hybrid_inner"])
+def test_blanking_replay_reproduces_golden_with_reference(key, mode):
+    """CPU check of the blanking replays: the reference harness reproduces
+    the golden outputs from the same reconstruction."""
+    fx = load("blanking")
+    outs, _ = replay(fx, key, mode, harness=refio.HARNESS)
+    n = 0
+    while f"out_{key}_{n}" in fx.d:
+        assert np.array_equal(outs[n], fx.out(key, n)), (key, n)
+        n += 1
+    assert n == len(outs)
 
 
 @pytest.mark.gpu
