@@ -100,10 +100,11 @@ typedef struct LsmBlock {
     int32_t nblk;                     /* blanking regions in blank[] (nblk of global_app.F90:31) */
     LsmBlank blank[LSM_MAX_BLANK];    /* read by lsm_weno3_rhs, lsm_bcond, lsm_rk3_step,
                                          lsm_water_volume(_noalloc), lsm_reinit_geometric,
-                                         lsm_levelset_step and the lsm_host_* adapter; the
-                                         entry points whose reference routines do not
-                                         read the blanking arrays ignore it; the adjoints,
-                                         the folded step and the consumers reject nblk != 0 */
+                                         lsm_levelset_step, the gradients of phi and h,
+                                         lsm_calc_h and the lsm_host_* adapter; the entry
+                                         points whose reference routines do not read the
+                                         blanking arrays ignore it; the adjoints, the folded
+                                         step and lsm_free_surface reject nblk != 0 */
 } LsmBlock;
 
 /* Parameters of the pseudo-time reinitialisation (global_lsm.F90:17-58). */
@@ -512,6 +513,11 @@ int lsm_host_setup(const int32_t ext[9], int32_t phys_faces, const int32_t btype
  * li_blk_ja, li_blk_jb, li_blk_ka, li_blk_kb (array indices) and
  * blktype(1:4).  lsm_host_setup clears them; call this after it. */
 int lsm_host_blanking(int32_t nblk, const int32_t *boxes);
+/* The blanking regions the consumer calls below (lsm_host_phi_gradient_eno2,
+ * lsm_host_h_gradient_eno2, lsm_host_calc_h) apply, in the same rows; they
+ * stay in force until the next call (nblk = 0: none) and are checked against
+ * each call's extents. */
+int lsm_host_consumer_blanking(int32_t nblk, const int32_t *boxes);
 int lsm_host_invalidate(void);
 int lsm_host_release(void);
 /* rhs_contra_j.F90:2 */

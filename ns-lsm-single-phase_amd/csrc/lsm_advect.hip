@@ -174,13 +174,15 @@ struct BlankBcArgs {
     int ia, ib, ja, jb;           // the box's i and j faces
     int corner[4];                // phase 5: which corners are treated
     double big;
+    bool switch_order;            // first order where |phi| at the far node >= big (bcond_lsm)
 };
 
-__device__ __forceinline__ void blank_extrap(double *phi, long lb, long behind, long l1, long l2, double big)
+__device__ __forceinline__ void blank_extrap(const BlankBcArgs &a, long lb, long behind, long l1, long l2)
 {
-    const double v = bc_face(1, phi[l1], phi[l2], 0.0, big, false);
-    phi[lb] = v;
-    phi[behind] = v;
+    const double v = a.switch_order ? bc_face(1, a.phi[l1], a.phi[l2], 0.0, a.big, false)
+                                    : (4.0 / 3.0) * a.phi[l1] + (-1.0 / 3.0) * a.phi[l2];
+    a.phi[lb] = v;
+    a.phi[behind] = v;
 }
 
 __global__ __launch_bounds__(256) void blank_bcond_phase(BlankBcArgs a, int phase)
@@ -193,28 +195,28 @@ __global__ __launch_bounds__(256) void blank_bcond_phase(BlankBcArgs a, int phas
     const long l = (long)i + sj * j + sk * k;
     switch (phase) {
     case 0: a.phi[l] = 0.0; break;
-    case 1: blank_extrap(a.phi, l, l + 1, l - 1, l - 2, a.big); break;
-    case 2: blank_extrap(a.phi, l, l - 1, l + 1, l + 2, a.big); break;
-    case 3: blank_extrap(a.phi, l, l + sj, l - sj, l - 2 * sj, a.big); break;
-    case 4: blank_extrap(a.phi, l, l - sj, l + sj, l + 2 * sj, a.big); break;
+    case 1: blank_extrap(a, l, l + 1, l - 1, l - 2); break;
+    case 2: blank_extrap(a, l, l - 1, l + 1, l + 2); break;
+    case 3: blank_extrap(a, l, l + sj, l - sj, l - 2 * sj); break;
+    case 4: blank_extrap(a, l, l - sj, l + sj, l + 2 * sj); break;
     default: {
         // one thread per k plane, the corners in the reference's order
         const long k0 = sk * k;
         if (a.corner[0]) {
             const long c = a.ia + sj * a.ja + k0;
-            blank_extrap(a.phi, c, c + 1 + sj, c - 1 - sj, c - 2 - 2 * sj, a.big);
+            blank_extrap(a, c, c + 1 + sj, c - 1 - sj, c - 2 - 2 * sj);
         }
         if (a.corner[1]) {
             const long c = a.ib + sj * a.ja + k0;
-            blank_extrap(a.phi, c, c - 1 + sj, c + 1 - sj, c + 2 - 2 * sj, a.big);
+            blank_extrap(a, c, c - 1 + sj, c + 1 - sj, c + 2 - 2 * sj);
         }
         if (a.corner[2]) {
             const long c = a.ib + sj * a.jb + k0;
-            blank_extrap(a.phi, c, c - 1 - sj, c + 1 + sj, c + 2 + 2 * sj, a.big);
+            blank_extrap(a, c, c - 1 - sj, c + 1 + sj, c + 2 + 2 * sj);
         }
         if (a.corner[3]) {
             const long c = a.ia + sj * a.jb + k0;
-            blank_extrap(a.phi, c, c + 1 - sj, c - 1 + sj, c - 2 + 2 * sj, a.big);
+            blank_extrap(a, c, c + 1 - sj, c - 1 + sj, c - 2 + 2 * sj);
         }
     }
     }
@@ -346,20 +348,31 @@ int launch_bcond(const Blk &b, double *phi, hipStream_t s)
     return launch_bcond_blanking(b, phi, s);
 }
 
-// the phases of one box in order; a phase whose guard fails or whose range
-// is empty is not launched
 int launch_bcond_blanking(const Blk &b, double *phi, hipStream_t s)
 {
+    return launch_blank_extrap(b, phi, 0, true, true, false, s);
+}
+
+// The phases of one box after another; a phase whose guard fails or whose
+// range is empty is not launched.  m: the first owned index the guards and
+// the face loops count from (bcond_lsm's i_mysta = il: 0, calc_h's
+// il + 1: 1), so a face is treated where it lies more than m + 1 nodes
+// inside the block, and its loops are clamped to [m, n - 1 - m].
+int launch_blank_extrap(const Blk &b, double *phi, int m, bool zero, bool switch_order, bool full_k,
+                        hipStream_t s)
+{
     const int ni = b.e.ni, nj = b.e.nj, nk = b.e.nk;
+    const int mei = ni - 1 - m, mej = nj - 1 - m;
     for (int q = 0; q < b.bl.n; ++q) {
         const int *B = b.bl.b[q];
         const int ia = B[0], ib = B[1], ja = B[2], jb = B[3], ka = B[4], kb = B[5];
-        // bcond_lsm.F90:320, 366, 412, 458: blktype 0 and two nodes between the face and the block edge
-        const bool fi0 = B[6] == 0 && ia > 1, fi1 = B[7] == 0 && ib < ni - 2;
-        const bool fj0 = B[8] == 0 && ja > 1, fj1 = B[9] == 0 && jb < nj - 2;
-        const int k0 = max(0, ka), k1 = min(nk - 1, kb);
+        // bcond_lsm.F90:324, 358, 386, 410 (calc_h.F90:215, 241, 261, 280): blktype 0 and
+        // two nodes between the face and the first owned node
+        const bool fi0 = B[6] == 0 && ia > m + 1, fi1 = B[7] == 0 && ib < mei - 1;
+        const bool fj0 = B[8] == 0 && ja > m + 1, fj1 = B[9] == 0 && jb < mej - 1;
+        const int k0 = full_k ? 0 : max(0, ka), k1 = full_k ? nk - 1 : min(nk - 1, kb);
         BlankBcArgs a{};
-        a.phi = phi; a.sj = b.e.sj; a.sk = b.e.sk; a.big = b.big_phi;
+        a.phi = phi; a.sj = b.e.sj; a.sk = b.e.sk; a.big = b.big_phi; a.switch_order = switch_order;
         a.ia = ia; a.ib = ib; a.ja = ja; a.jb = jb;
         a.corner[0] = fi0 && fj0; a.corner[1] = fi1 && fj0; a.corner[2] = fi1 && fj1; a.corner[3] = fi0 && fj1;
         auto run = [&](int phase, bool on, int i0, int i1, int j0, int j1, int kk0, int kk1) -> int {
@@ -369,9 +382,9 @@ int launch_bcond_blanking(const Blk &b, double *phi, hipStream_t s)
             blank_bcond_phase<<<blocks_for(n, 256), 256, 0, s>>>(a, phase);
             return check_hip(hipGetLastError(), "blank_bcond_phase");
         };
-        const int jlo = max(0, ja), jhi = min(nj - 1, jb), ilo = max(0, ia), ihi = min(ni - 1, ib);
+        const int jlo = max(m, ja), jhi = min(mej, jb), ilo = max(m, ia), ihi = min(mei, ib);
         int rc;
-        if ((rc = run(0, true, max(0, ia + 2), min(ni - 1, ib - 2), max(0, ja + 2), min(nj - 1, jb - 2),
+        if ((rc = run(0, zero, max(0, ia + 2), min(ni - 1, ib - 2), max(0, ja + 2), min(nj - 1, jb - 2),
                       max(0, ka + 2), min(nk - 1, kb - 2)))) return rc;
         if ((rc = run(1, fi0, ia, ia, jlo, jhi, k0, k1))) return rc;
         if ((rc = run(2, fi1, ib, ib, jlo, jhi, k0, k1))) return rc;

@@ -477,7 +477,7 @@ static int gradient_entry(const LsmBlock *blk, const double *f, int nc, int orde
                           double *g1, double *g2, double *g3, void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, false);
+    int rc = make_blk(blk, &b, false, false, kBlankUse);
     if (rc) return rc;
     for (int q = 0; q < nc; ++q)
         if (!blk->csi[q] || !blk->eta[q] || !blk->zet[q])
@@ -517,7 +517,7 @@ int lsm_calc_h(const LsmBlock *blk, const double *z, const double *phi, double *
                void *stream)
 {
     Blk b;
-    int rc = make_blk(blk, &b, false, false);
+    int rc = make_blk(blk, &b, false, false, kBlankUse);
     if (rc) return rc;
     if (!z || !phi || !h) return fail(LSM_ERR_ARG, "NULL array");
     return launch_free_surface(b, z, phi, h, nullptr, missing, (hipStream_t)stream);

@@ -21,13 +21,16 @@ constexpr int GX = 64, GY = 4;   // gradient tile: 64 i-lanes x 4 j-rows of one 
 // not (the routines have no vertex blocks and their output is intent(inout)).
 // One thread per node; the stencil neighbours come through L1/L2, and the
 // tiles of consecutive planes go to one XCD so its L2 holds the j/k halos.
+// Blanking (interior nodes only, calc_phi_gradient_ENO2.F90:133-158,
+// 216-231): the stencils one node off a box's i or j face are biased away from
+// it and the xi (eta) derivative is zero on the box's i (j) faces, at every k.
 template <int NC>
 __global__ __launch_bounds__(GX *GY) void gradient_eno2_kernel(Geo g, const double *__restrict__ f,
                                                                int order_bc, int eno_bc, int tiles_i,
                                                                int tiles_j, int ntiles,
                                                                double *__restrict__ o0,
                                                                double *__restrict__ o1,
-                                                               double *__restrict__ o2)
+                                                               double *__restrict__ o2, Blank bl)
 {
     const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
     if (t >= ntiles) return;
@@ -45,7 +48,23 @@ __global__ __launch_bounds__(GX *GY) void gradient_eno2_kernel(Geo g, const doub
     if (on[0] && on[1] && on[2]) return;
     const long l = (long)i + g.sj * j + g.sk * k;
     double gr[3];
-    rn::curv_derivs(g, f, f[l], order_bc, eno_bc != 0, i, j, k, l, on, gr);
+    int blo = 0, bhi = 0;
+    bool z0 = false, z1 = false;
+    if (bl.n && !on[0] && !on[1] && !on[2]) {
+        bool mb = false, mf = false, ml = false, mr = false;
+        bl.bias(i, j, mb, mf, ml, mr);
+        blo = (mb ? 1 : 0) | (ml ? 2 : 0);
+        bhi = (mf ? 1 : 0) | (mr ? 2 : 0);
+        for (int q = 0; q < bl.n; ++q) {
+            const int *B = bl.b[q];
+            const bool jin = j >= B[2] && j <= B[3], iin = i >= B[0] && i <= B[1];
+            if ((i == B[0] || i == B[1]) && jin) z0 = true;
+            if ((j == B[2] || j == B[3]) && iin) z1 = true;
+        }
+    }
+    rn::curv_derivs(g, f, f[l], order_bc, eno_bc != 0, i, j, k, l, on, gr, blo, bhi);
+    if (z0) gr[0] = 0.0;
+    if (z1) gr[1] = 0.0;
     o0[l] = (g.m[0][l] * gr[0] + g.m[3][l] * gr[1]) + g.m[6][l] * gr[2];
     o1[l] = (g.m[1][l] * gr[0] + g.m[4][l] * gr[1]) + g.m[7][l] * gr[2];
     if (NC == 3) o2[l] = (g.m[2][l] * gr[0] + g.m[5][l] * gr[1]) + g.m[8][l] * gr[2];
@@ -195,10 +214,10 @@ int launch_gradient_eno2(const Blk &b, const double *f, int nc, int order_bc, in
     const dim3 blk(GX, GY);
     if (nc == 3)
         gradient_eno2_kernel<3><<<grid, blk, 0, s>>>(make_geo(b), f, order_bc, eno_bc, tiles_i, tiles_j,
-                                                     (int)ntiles, o0, o1, o2);
+                                                     (int)ntiles, o0, o1, o2, b.bl);
     else
         gradient_eno2_kernel<2><<<grid, blk, 0, s>>>(make_geo(b), f, order_bc, eno_bc, tiles_i, tiles_j,
-                                                     (int)ntiles, o0, o1, nullptr);
+                                                     (int)ntiles, o0, o1, nullptr, b.bl);
     return check_hip(hipGetLastError(), "gradient_eno2_kernel");
 }
 
@@ -214,7 +233,13 @@ int launch_free_surface(const Blk &b, const double *z, const double *phi, double
     }
     free_surface_kernel<<<(unsigned)nwg, dim3(HX, HS), 0, s>>>(b.e.ni, b.e.nj, b.e.nk, b.e.sj, b.e.sk, z, phi,
                                                                h, zfs, missing);
-    return check_hip(hipGetLastError(), "free_surface_kernel");
+    int rc = check_hip(hipGetLastError(), "free_surface_kernel");
+    if (rc || !h || !b.bl.n) return rc;
+    // calc_h.F90:195-345: h extrapolated onto the blanking boxes' i and j faces
+    // over whole columns.  (The column skip of calc_h.F90:105-118 tests the
+    // loop variable k, which holds kend + 1 from the previous column, so it
+    // never holds for a box inside the block: every column is computed.)
+    return launch_blank_extrap(b, h, 1, false, false, true, s);
 }
 
 int launch_free_surface_adjoint(const Blk &b, const double *z, const double *phi, const double *adj_zfs,

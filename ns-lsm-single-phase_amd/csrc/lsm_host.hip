@@ -143,9 +143,26 @@ struct ConsumerState {
     double *f[4] = {nullptr};
     int32_t *cnt = nullptr;
     std::vector<double> buf;
+    int32_t nblk = 0;              // blanking regions of the consumer calls (lsm_host_consumer_blanking)
+    LsmBlank blank[LSM_MAX_BLANK] = {};
 };
 
 ConsumerState g_c;
+
+// rows of ten values (ia, ib, ja, jb, ka, kb, blktype(1:4)) into the block
+int set_blanking(LsmBlock &b, int32_t nblk, const int32_t *boxes)
+{
+    if (nblk < 0 || nblk > LSM_MAX_BLANK) return fail(LSM_ERR_ARG, "nblk = %d (0..%d)", nblk, LSM_MAX_BLANK);
+    if (nblk > 0 && !boxes) return fail(LSM_ERR_ARG, "NULL boxes");
+    b.nblk = nblk;
+    for (int q = 0; q < nblk; ++q) {
+        const int32_t *v = boxes + 10 * q;
+        LsmBlank &B = b.blank[q];
+        B.ia = v[0]; B.ib = v[1]; B.ja = v[2]; B.jb = v[3]; B.ka = v[4]; B.kb = v[5];
+        for (int f = 0; f < 4; ++f) B.blktype[f] = v[6 + f];
+    }
+    return LSM_OK;
+}
 
 void consumer_release()
 {
@@ -169,8 +186,10 @@ int consumer_block(const int32_t ext[9], int32_t faces, double dc, double de, do
     b->phys_faces = faces;
     for (int q = 0; q < 6; ++q) b->btype[q] = 1;
     b->dc = dc; b->de = de; b->dz = dz;
+    b->nblk = g_c.nblk;
+    for (int q = 0; q < LSM_MAX_BLANK; ++q) b->blank[q] = g_c.blank[q];
     Blk chk;
-    int rc = make_blk(b, &chk, false, false);
+    int rc = make_blk(b, &chk, false, false, kBlankUse);
     if (rc) return rc;
     if (memcmp(g_c.ext, ext, sizeof g_c.ext) != 0 || g_c.faces != faces || !g_c.f[0]) {
         consumer_release();
@@ -300,16 +319,21 @@ int lsm_host_blanking(int32_t nblk, const int32_t *boxes)
     if (nblk < 0 || nblk > LSM_MAX_BLANK) return fail(LSM_ERR_ARG, "nblk = %d (0..%d)", nblk, LSM_MAX_BLANK);
     if (nblk > 0 && !boxes) return fail(LSM_ERR_ARG, "NULL boxes");
     LsmBlock b = g_h.blk;
-    b.nblk = nblk;
-    for (int q = 0; q < nblk; ++q) {
-        const int32_t *v = boxes + 10 * q;
-        LsmBlank &B = b.blank[q];
-        B.ia = v[0]; B.ib = v[1]; B.ja = v[2]; B.jb = v[3]; B.ka = v[4]; B.kb = v[5];
-        for (int f = 0; f < 4; ++f) B.blktype[f] = v[6 + f];
-    }
+    if ((rc = set_blanking(b, nblk, boxes))) return rc;
     Blk chk;
     if ((rc = make_blk(&b, &chk, false, false, kBlankUse))) return rc;
     g_h.blk = b;
+    return LSM_OK;
+}
+
+int lsm_host_consumer_blanking(int32_t nblk, const int32_t *boxes)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    LsmBlock b{};
+    int rc = set_blanking(b, nblk, boxes);
+    if (rc) return rc;
+    g_c.nblk = b.nblk;
+    for (int q = 0; q < LSM_MAX_BLANK; ++q) g_c.blank[q] = b.blank[q];
     return LSM_OK;
 }
 

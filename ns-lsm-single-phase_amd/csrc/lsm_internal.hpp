@@ -75,6 +75,10 @@ int launch_weno3_rhs(const Blk &b, const double *phi, const double *U1, const do
 int launch_bcond(const Blk &b, double *phi, hipStream_t s);
 // the blanking part of bcond_lsm alone (nothing when b.bl.n == 0)
 int launch_bcond_blanking(const Blk &b, double *phi, hipStream_t s);
+// the box-face extrapolations of bcond_lsm (m = 0, zero = switch_order = true,
+// full_k = false) and of calc_h (m = 1, fixed second order, whole columns)
+int launch_blank_extrap(const Blk &b, double *f, int m, bool zero, bool switch_order, bool full_k,
+                        hipStream_t s);
 int launch_rk3(const Blk &b, double *phi_np1, const double *phi_n, const int32_t *mask,
                const double *U1, const double *U2, const double *U3, double dt, int mode,
                double *scratch, hipStream_t s, bool folded = false);

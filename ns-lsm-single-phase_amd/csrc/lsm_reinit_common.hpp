@@ -40,9 +40,11 @@ __device__ __forceinline__ void node_of(const Geo &g, long l, int &i, int &j, in
 // one-sided formula of order order_bc on the face, and the fixed second-order
 // xi formula on the edge i = 1, k = 1 (calc_NormGrad_d.F90:1281-1283; the
 // same in calc_phi_gradient_ENO2.F90:1413-1415).
+// blo / bhi (bit a: axis a) force the interior ENO2 stencil one-sided as the
+// physical faces do (the blanking biasing of the consumers of phi).
 __device__ __forceinline__ void curv_derivs(const Geo &g, const double *__restrict__ d, double p0,
                                             int order_bc, bool eno_bc, int i, int j, int k, long l,
-                                            const bool on[3], double gr[3])
+                                            const bool on[3], double gr[3], int blo = 0, int bhi = 0)
 {
     const int n[3] = {g.ni, g.nj, g.nk};
     const int idx[3] = {i, j, k};
@@ -60,7 +62,7 @@ __device__ __forceinline__ void curv_derivs(const Geo &g, const double *__restri
                 gr[a] = g.dd[a] * biased(d[l], d[l + ss], d[l + 2 * ss], d[l + 3 * ss], order_bc, dir, eno_bc);
             }
         } else {
-            const bool lo = (idx[a] == 1), hi = (idx[a] == n[a] - 2);
+            const bool lo = (idx[a] == 1) || ((blo >> a) & 1), hi = (idx[a] == n[a] - 2) || ((bhi >> a) & 1);
             const double fLL = lo ? 0.0 : d[l - 2 * s];
             const double fRR = hi ? 0.0 : d[l + 2 * s];
             gr[a] = g.dd[a] * eno2(p0, fLL, d[l - s], d[l], d[l + s], fRR, lo, hi);

@@ -2,7 +2,8 @@
 ! external subroutine of the reference, called by mg_driver.F90:127-135 and
 ! init_LSM.F90:458): h = z_fs - z in every column of the block, z_fs at the
 ! first sign change of phi from the bottom, h = 0 (and the reference's
-! message) in a column without one, on the GPU through liblsm_hip.
+! message) in a column without one, then h extrapolated onto the faces of
+! the blanking regions (calc_h.F90:195-345), on the GPU through liblsm_hip.
 subroutine calc_h( il,iu          ,&
                    jl,ju          ,&
                    kl,ku          ,&
@@ -13,7 +14,8 @@ subroutine calc_h( il,iu          ,&
                   )
    use precision
    use global_mpi
-   use global_app, only : nblk, nblke
+   use global_app, only : nblk, nblke, blktype, li_blk_ia, li_blk_ib, li_blk_ja, li_blk_jb, &
+                          li_blk_ka, li_blk_kb
    use lsm_hip_f
    implicit none
    integer, intent(in) :: il,iu,jl,ju,kl,ku
@@ -23,8 +25,12 @@ subroutine calc_h( il,iu          ,&
    real (kind = rdf), dimension(il:iu,jl:ju,kl:ku) , intent(inout) :: h
    integer(c_int32_t) :: missing
    integer :: m
-   if ( nblk /= 0 .or. nblke /= 0 ) &
-      call lsm_f_unsupported( 'calc_h', 'blanking regions (nblk /= 0)' )
+   if ( nblk /= 0 .or. nblke /= 0 ) then
+      call lsm_f_bind_blanking( nblk, nblke, li_blk_ia, li_blk_ib, li_blk_ja, li_blk_jb, &
+                                li_blk_ka, li_blk_kb, blktype(:, :, myzone), consumers = .true. )
+   else
+      call lsm_f_check( lsm_host_consumer_blanking( 0_c_int32_t, [0_c_int32_t] ), 'lsm_host_consumer_blanking' )
+   end if
    missing = 0
    call lsm_f_check( lsm_host_calc_h( int([il, iu, jl, ju, kl, ku, igp, jgp, kgp], c_int32_t),   &
                         lsm_f_faces([myback, myfront, myleft, myright, mydown, myup], mpi_proc_null), &

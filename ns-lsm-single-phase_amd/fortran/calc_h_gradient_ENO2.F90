@@ -1,7 +1,7 @@
 ! Drop-in for calc_h_gradient_ENO2.F90:1-2018 (an external subroutine of the
 ! reference, called by mg_driver.F90:140-150): the x and y components of the
-! ENO2 gradient of h, stencils and vertices as in calc_phi_gradient_ENO2, on
-! the GPU through liblsm_hip.
+! ENO2 gradient of h, stencils, blanking and vertices as in
+! calc_phi_gradient_ENO2, on the GPU through liblsm_hip.
 subroutine calc_h_gradient_ENO2( il,iu          ,&
                                  jl,ju          ,&
                                  kl,ku          ,&
@@ -15,7 +15,8 @@ subroutine calc_h_gradient_ENO2( il,iu          ,&
                                 )
    use precision
    use global_mpi
-   use global_app, only : nblk, nblke
+   use global_app, only : nblk, nblke, blktype, li_blk_ia, li_blk_ib, li_blk_ja, li_blk_jb, &
+                          li_blk_ka, li_blk_kb
    use global_lsm, only : OrderReinitialisationBoundaries , ENOBCReinitialisation
    use lsm_hip_f
    implicit none
@@ -26,8 +27,12 @@ subroutine calc_h_gradient_ENO2( il,iu          ,&
    real (kind = rdf), dimension(il:iu,jl:ju,kl:ku) , intent(in) :: h
    real (kind = rdf), dimension(1:2,il:iu,jl:ju,kl:ku) , intent(inout) :: h_gradient
    integer(c_int) :: eno
-   if ( nblk /= 0 .or. nblke /= 0 ) &
-      call lsm_f_unsupported( 'calc_h_gradient_ENO2', 'blanking regions (nblk /= 0)' )
+   if ( nblk /= 0 .or. nblke /= 0 ) then
+      call lsm_f_bind_blanking( nblk, nblke, li_blk_ia, li_blk_ib, li_blk_ja, li_blk_jb, &
+                                li_blk_ka, li_blk_kb, blktype(:, :, myzone), consumers = .true. )
+   else
+      call lsm_f_check( lsm_host_consumer_blanking( 0_c_int32_t, [0_c_int32_t] ), 'lsm_host_consumer_blanking' )
+   end if
    eno = 0
    if ( ENOBCReinitialisation ) eno = 1
    call lsm_f_check( lsm_host_h_gradient_eno2(                                                  &

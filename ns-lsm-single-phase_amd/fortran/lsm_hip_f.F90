@@ -43,6 +43,13 @@ module lsm_hip_f
          integer(c_int32_t), intent(in) :: boxes(*)
       end function lsm_host_blanking
 
+      ! blanking regions of the consumer calls (lsm_host_phi/h_gradient_eno2, lsm_host_calc_h)
+      integer(c_int) function lsm_host_consumer_blanking(nblk, boxes) bind(C, name="lsm_host_consumer_blanking")
+         import :: c_int, c_int32_t
+         integer(c_int32_t), value      :: nblk
+         integer(c_int32_t), intent(in) :: boxes(*)
+      end function lsm_host_consumer_blanking
+
       ! the next lsm_host_setup uploads the grid again (after in-place edits of grid arrays)
       integer(c_int) function lsm_host_invalidate() bind(C, name="lsm_host_invalidate")
          import :: c_int
@@ -238,10 +245,12 @@ contains
 
    ! Passes the blanking regions of global_app (one block per rank: the
    ! interior boxes li_blk_* and the exterior boxes le_blk_* coincide) to the
-   ! block bound by lsm_f_bind.
-   subroutine lsm_f_bind_blanking(nblk, nblke, ia, ib, ja, jb, ka, kb, blktype)
+   ! block bound by lsm_f_bind, or with consumers = .true. to the consumer
+   ! calls of phi.
+   subroutine lsm_f_bind_blanking(nblk, nblke, ia, ib, ja, jb, ka, kb, blktype, consumers)
       integer, intent(in) :: nblk, nblke
       integer, intent(in) :: ia(:,:), ib(:,:), ja(:,:), jb(:,:), ka(:,:), kb(:,:), blktype(:,:)
+      logical, intent(in), optional :: consumers
       integer(c_int32_t) :: boxes(10, 4)
       integer :: nb
       if (nblk /= nblke) call lsm_f_unsupported('level-set drop-in', &
@@ -252,6 +261,12 @@ contains
          boxes(1:6, nb) = int([ia(1, nb), ib(1, nb), ja(1, nb), jb(1, nb), ka(1, nb), kb(1, nb)], c_int32_t)
          boxes(7:10, nb) = int(blktype(1:4, nb), c_int32_t)
       end do
+      if (present(consumers)) then
+         if (consumers) then
+            call lsm_f_check(lsm_host_consumer_blanking(int(nblk, c_int32_t), boxes), 'lsm_host_consumer_blanking')
+            return
+         end if
+      end if
       call lsm_f_check(lsm_host_blanking(int(nblk, c_int32_t), boxes), 'lsm_host_blanking')
    end subroutine lsm_f_bind_blanking
 

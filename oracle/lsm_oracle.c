@@ -928,6 +928,22 @@ void oracle_gradient_eno2(const OBlock *b, const double *f, double *g0, double *
         for (int a = 0; a < 3; ++a) on[a] = (idx[a] == 0 || idx[a] == n[a] - 1);
         if (on[0] && on[1] && on[2]) continue;
         double g[3];
+        /* blanking, interior loop only (calc_phi_gradient_ENO2.F90:133-158,
+           216-231; calc_h_gradient_ENO2.F90 the same): the stencils one node
+           off a box's i or j face are biased away from it, and the xi (eta)
+           derivative is zero on the box's i (j) faces */
+        int blo[3] = {0, 0, 0}, bhi[3] = {0, 0, 0}, zero_d[2] = {0, 0};
+        if (!on[0] && !on[1] && !on[2])
+            for (int q = 0; q < b->nblk; ++q) {
+                const int32_t *B = b->blank[q];
+                const int jin = j >= B[2] && j <= B[3], iin = i >= B[0] && i <= B[1];
+                if (i == B[0] - 1 && jin) bhi[0] = 1;
+                if (i == B[1] + 1 && jin) blo[0] = 1;
+                if (j == B[2] - 1 && iin) bhi[1] = 1;
+                if (j == B[3] + 1 && iin) blo[1] = 1;
+                if ((i == B[0] || i == B[1]) && jin) zero_d[0] = 1;
+                if ((j == B[2] || j == B[3]) && iin) zero_d[1] = 1;
+            }
         for (int a = 0; a < 3; ++a) {
             const size_t s = st[a];
             if (on[a]) {
@@ -941,12 +957,14 @@ void oracle_gradient_eno2(const OBlock *b, const double *f, double *g0, double *
                                                  order_bc, dir, eno_bc);
                 }
             } else {
-                const int lo = (idx[a] == 1), hi = (idx[a] == n[a] - 2);
+                const int lo = (idx[a] == 1) || blo[a], hi = (idx[a] == n[a] - 2) || bhi[a];
                 const double fLL = lo ? 0.0 : f[l - 2 * s];
                 const double fRR = hi ? 0.0 : f[l + 2 * s];
                 g[a] = dd[a] * oracle_eno2(f[l], fLL, f[l - s], f[l], f[l + s], fRR, lo, hi);
             }
         }
+        if (zero_d[0]) g[0] = 0.0;
+        if (zero_d[1]) g[1] = 0.0;
         for (int c = 0; c < nc; ++c)
             out[c][l] = (b->csi[c][l] * g[0] + b->eta[c][l] * g[1]) + b->zet[c][l] * g[2];
     }
@@ -979,6 +997,11 @@ int oracle_calc_h(const OBlock *b, const double *z, const double *phi, double *h
     const int ni = b->ni, nj = b->nj, nk = b->nk;
     const size_t sk = (size_t)ni * nj;
     int missing = 0;
+    /* calc_h.F90:105-118 skips columns strictly inside a box by testing the
+       loop variable k, which holds kend + 1 from the previous column's h loop
+       (undefined before the first column with a free surface; the first
+       column lies on the block face, outside every box): the test never
+       holds for a box inside the block, so every column is computed */
     for (int j = 0; j < nj; ++j)
     for (int i = 0; i < ni; ++i) {
         const size_t l0 = IX(b, i, j, 0);
@@ -990,6 +1013,47 @@ int oracle_calc_h(const OBlock *b, const double *z, const double *phi, double *h
             for (int k = 0; k < nk; ++k) h[l0 + k * sk] = 0.0;
         }
     }
+    /* calc_h.F90:195-345: per box, h extrapolated (4/3, -1/3) over whole
+       columns onto the box's i faces, j faces and the four i-j corners, the
+       face node and the node behind it; calc_h's own i_mysta = 1 and
+       i_myend = ni - 2 (nodes without the block border) give the guards and
+       clamp the face loops */
+    const double i1 = 4.0 / 3.0, i2 = -1.0 / 3.0;
+    const int ms = 1, mei = ni - 2, mej = nj - 2;
+#define HCOL(dst, behind, a, c)                                                        \
+    for (int k = 0; k < nk; ++k) {                                                     \
+        const size_t o = (size_t)k * sk;                                               \
+        h[(dst) + o] = i1 * h[(a) + o] + i2 * h[(c) + o];                              \
+        h[(behind) + o] = h[(dst) + o];                                                \
+    }
+    for (int q = 0; q < b->nblk; ++q) {
+        const int32_t *B = b->blank[q];
+        const int ia = B[0], ib = B[1], ja = B[2], jb = B[3];
+        const int *bt = &B[6];
+        const int fi0 = bt[0] == 0 && ia > ms + 1, fi1 = bt[1] == 0 && ib < mei - 1;
+        const int fj0 = bt[2] == 0 && ja > ms + 1, fj1 = bt[3] == 0 && jb < mej - 1;
+        if (fi0)
+            for (int j = imax(ms, ja); j <= imin(mej, jb); ++j)
+                HCOL(IX(b, ia, j, 0), IX(b, ia + 1, j, 0), IX(b, ia - 1, j, 0), IX(b, ia - 2, j, 0))
+        if (fi1)
+            for (int j = imax(ms, ja); j <= imin(mej, jb); ++j)
+                HCOL(IX(b, ib, j, 0), IX(b, ib - 1, j, 0), IX(b, ib + 1, j, 0), IX(b, ib + 2, j, 0))
+        if (fj0)
+            for (int i = imax(ms, ia); i <= imin(mei, ib); ++i)
+                HCOL(IX(b, i, ja, 0), IX(b, i, ja + 1, 0), IX(b, i, ja - 1, 0), IX(b, i, ja - 2, 0))
+        if (fj1)
+            for (int i = imax(ms, ia); i <= imin(mei, ib); ++i)
+                HCOL(IX(b, i, jb, 0), IX(b, i, jb - 1, 0), IX(b, i, jb + 1, 0), IX(b, i, jb + 2, 0))
+        if (fi0 && fj0)
+            HCOL(IX(b, ia, ja, 0), IX(b, ia + 1, ja + 1, 0), IX(b, ia - 1, ja - 1, 0), IX(b, ia - 2, ja - 2, 0))
+        if (fi1 && fj0)
+            HCOL(IX(b, ib, ja, 0), IX(b, ib - 1, ja + 1, 0), IX(b, ib + 1, ja - 1, 0), IX(b, ib + 2, ja - 2, 0))
+        if (fi1 && fj1)
+            HCOL(IX(b, ib, jb, 0), IX(b, ib - 1, jb - 1, 0), IX(b, ib + 1, jb + 1, 0), IX(b, ib + 2, jb + 2, 0))
+        if (fi0 && fj1)
+            HCOL(IX(b, ia, jb, 0), IX(b, ia + 1, jb - 1, 0), IX(b, ia - 1, jb + 1, 0), IX(b, ia - 2, jb + 2, 0))
+    }
+#undef HCOL
     return missing;
 }
 

@@ -327,9 +327,37 @@ def blanking_case():
     case("blanking", g, phi, phi, u, v, w, params, runs)
 
 
+CONSUMER_BOXES = {
+    "inner": ((5, 12, 4, 8, 2, 9), (0, 0, 0, 0, 0, 0)),       # every face of calc_h's h extrapolated
+    "guard": ((2, 16, 1, 9, 0, 15), (0, 0, 1, 0, 0, 0)),      # ia, ja at calc_h's guards, ib, jb past them
+    "thin": ((9, 9, 6, 7, 4, 11), (0, 0, 0, 0, 0, 0)),        # one node thick in i
+}
+
+
+def blanking_consumers_case():
+    """The consumers of phi with a blanking region: calc_phi_gradient_ENO2 and
+    calc_h_gradient_ENO2 (biased interior stencils, zero derivative across
+    the box faces), calc_h (h extrapolated onto the box faces), on the
+    solitary-wave channel."""
+    c = inputs.channel_grid(20, 12, 16)
+    phi = inputs.solitary_phi(c, H=0.3)
+    aux = np.random.default_rng(23).uniform(-1.0, 1.0, c.shape)
+    zeros = np.zeros(c.shape)
+    (h,), _, _ = refio.run(15, c, phi, aux=aux)
+    params = dict(btype=(1, 1, 1, 1, 1, 1))
+    runs = []
+    for name, bl in CONSUMER_BOXES.items():
+        runs += [(f"calc_h_{name}", 15, {"blanking": bl}, {"aux": aux}),
+                 (f"phigrad_{name}", 14, {"blanking": bl, "order_reinit_bc": 2, "eno_bc": 1}, {"aux": aux}),
+                 (f"hgrad_{name}", 16, {"blanking": bl, "order_reinit_bc": 1, "eno_bc": 0},
+                  {"phi": h, "aux": aux})]
+    case("blanking_consumers", c, phi, phi, zeros, zeros, zeros, params, runs,
+         extra_inputs={"aux": aux, "h": h})
+
+
 CASES = {"curvilinear": curvilinear_case, "channel": channel_case, "boundary": boundary_case,
          "volume": volume_case, "consumers": consumers_case, "upwind": upwind_case, "hybrid": hybrid_case,
-         "blanking": blanking_case}
+         "blanking": blanking_case, "blanking_consumers": blanking_consumers_case}
 
 if __name__ == "__main__":
     if not refio.available():

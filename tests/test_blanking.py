@@ -142,6 +142,42 @@ def test_oracle_bcond_box_by_hand(fx):
     assert np.array_equal(out[ka, ja + 2:jb - 1, ia + 2:ib - 1], phi[ka, ja + 2:jb - 1, ia + 2:ib - 1])
 
 
+CBOXES = ["inner", "guard", "thin"]
+
+
+@pytest.fixture(scope="module")
+def cfx():
+    return load("blanking_consumers")
+
+
+def _init(aux, nc):
+    """the outputs' values before the call (the harness sets component c to aux + c + 1)"""
+    return [aux + (c + 1.0) for c in range(nc)]
+
+
+@pytest.mark.parametrize("box", CBOXES)
+def test_oracle_consumers(cfx, box):
+    """calc_phi_gradient_ENO2, calc_h_gradient_ENO2 and calc_h with a box,
+    bit for bit against the reference."""
+    blk = O.Block(cfx.grid, blanking=_blanking(cfx, f"calc_h_{box}"))
+    h, _ = O.calc_h(blk, cfx.grid.z, cfx.inp("phi"))
+    _eq(h, cfx.out(f"calc_h_{box}"))
+    g = O.gradient_eno2(blk, cfx.inp("phi"), _init(cfx.inp("aux"), 3), 3, 2, 1)
+    for c in range(3):
+        _eq(g[c], cfx.out(f"phigrad_{box}", c))
+    g = O.gradient_eno2(blk, cfx.inp("h"), _init(cfx.inp("aux"), 2), 2, 1, 0)
+    for c in range(2):
+        _eq(g[c], cfx.out(f"hgrad_{box}", c))
+
+
+def test_oracle_consumer_blanking_changes_the_outputs(cfx):
+    plain = O.Block(cfx.grid)
+    h, _ = O.calc_h(plain, cfx.grid.z, cfx.inp("phi"))
+    assert not np.array_equal(h, cfx.out("calc_h_inner"))
+    g = O.gradient_eno2(plain, cfx.inp("phi"), _init(cfx.inp("aux"), 3), 3, 2, 1)
+    assert not np.array_equal(g[0], cfx.out("phigrad_inner", 0))
+
+
 # ---------------------------------------------------------------------------
 # the GPU against the reference and the oracle
 # ---------------------------------------------------------------------------
@@ -315,6 +351,56 @@ def test_gpu_blanking_geometric_matches_oracle():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("box", CBOXES)
+def test_gpu_blanking_consumers_golden(cfx, box):
+    """calc_phi_gradient_ENO2, calc_h_gradient_ENO2 and calc_h with a box on
+    the GPU, bit for bit against the reference."""
+    torch, api = _gpu()
+    print(f"[blanking] consumers, box {box}", flush=True)
+    blk = api.Block(cfx.grid, blanking=_blanking(cfx, f"calc_h_{box}"))
+    aux = cfx.inp("aux")
+    h, _ = api.calc_h(blk, _dev(torch, cfx.grid.z), _dev(torch, cfx.inp("phi")), out=_dev(torch, aux))
+    _eq(_host(torch, h), cfx.out(f"calc_h_{box}"))
+    g3 = api.calc_phi_gradient_ENO2(blk, _dev(torch, cfx.inp("phi")), 2, 1,
+                                    out=[_dev(torch, a) for a in _init(aux, 3)])
+    for c in range(3):
+        _eq(_host(torch, g3[c]), cfx.out(f"phigrad_{box}", c))
+    g2 = api.calc_h_gradient_ENO2(blk, _dev(torch, cfx.inp("h")), 1, 0,
+                                  out=[_dev(torch, a) for a in _init(aux, 2)])
+    for c in range(2):
+        _eq(_host(torch, g2[c]), cfx.out(f"hgrad_{box}", c))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(3))
+def test_gpu_blanking_consumers_match_oracle_random_boxes(seed):
+    """Gradients and calc_h with one to four random boxes on the channel,
+    bit for bit against the oracle."""
+    torch, api = _gpu()
+    from lsm_amd import inputs
+    rng = np.random.default_rng(200 + seed)
+    ni, nj, nk = int(rng.integers(12, 70)), int(rng.integers(8, 20)), int(rng.integers(8, 24))
+    print(f"[blanking] consumers random boxes seed {seed} shape {(nk, nj, ni)}", flush=True)
+    g = inputs.channel_grid(ni, nj, nk)
+    phi = inputs.solitary_phi(g, H=0.3)
+    boxes = _random_boxes(rng, g.shape, int(rng.integers(1, 5)))
+    ob, gb = O.Block(g, blanking=boxes), api.Block(g, blanking=boxes)
+    aux = rng.uniform(-1.0, 1.0, g.shape)
+    href, _ = O.calc_h(ob, g.z, phi)
+    h, _ = api.calc_h(gb, _dev(torch, g.z), _dev(torch, phi))
+    _eq(_host(torch, h), href)
+    for order, eno in ((1, 0), (2, 1), (3, 1)):
+        ref = O.gradient_eno2(ob, phi, _init(aux, 3), 3, order, eno)
+        got = api.calc_phi_gradient_ENO2(gb, _dev(torch, phi), order, eno, out=[_dev(torch, a) for a in _init(aux, 3)])
+        for c in range(3):
+            _eq(_host(torch, got[c]), ref[c])
+        ref = O.gradient_eno2(ob, href, _init(aux, 2), 2, order, eno)
+        got = api.calc_h_gradient_ENO2(gb, h, order, eno, out=[_dev(torch, a) for a in _init(aux, 2)])
+        for c in range(2):
+            _eq(_host(torch, got[c]), ref[c])
+
+
+@pytest.mark.gpu
 def test_gpu_blanking_rejected_where_unsupported():
     """Entry points whose blanking form is not implemented refuse nblk != 0."""
     torch, api = _gpu()
@@ -327,6 +413,9 @@ def test_gpu_blanking_rejected_where_unsupported():
     with pytest.raises(LsmError, match="blanking"):
         api.ad_RKTVD3_adjoint(gb, phi, ucn, 1e-3, torch.ones_like(phi))
     with pytest.raises(LsmError, match="blanking"):
-        api.calc_phi_gradient_ENO2(gb, phi)
+        api.free_surface(gb, _dev(torch, g.z), phi)
     with pytest.raises(LsmError, match="blanking"):
         api.ad_RKTVD3_folded(gb, phi, api.fold_velocity(gb, ucn, 1e-3))
+    # ia = ni - 1: no node behind the i face
+    with pytest.raises(LsmError, match="blanking region"):
+        api.bcond_lsm(api.Block(g, blanking=[((11, 11, 3, 6, 2, 6), (0, 0, 0, 0))]), phi.clone())

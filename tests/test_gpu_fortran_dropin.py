@@ -298,6 +298,48 @@ def test_dropin_blanking_levelsetmethod(key, harness):
     assert np.max(np.abs(extra["global_volume"] - gv)) <= 1e-12 * np.max(np.abs(gv))
 
 
+BLANK_CONSUMERS = [(f"{kind}_{box}", mode, phi_key) for box in ("inner", "guard", "thin")
+                   for kind, mode, phi_key in (("calc_h", "calc_h", "phi"), ("phigrad", "phi_gradient", "phi"),
+                                               ("hgrad", "h_gradient", "h"))]
+
+
+def replay_blank_consumer(fx, key, mode, phi_key, harness):
+    bl = fx.d[f"kw_{key}_blanking"]
+    print(f"[dropin] {fx.name}/{key} mode={mode} {os.path.basename(harness)}", flush=True)
+    outs, _, _ = refio.run(mode, fx.grid, fx.inp(phi_key), aux=fx.inp("aux"),
+                           order_reinit_bc=fx.kw(key, "order_reinit_bc", 1), eno_bc=fx.kw(key, "eno_bc", 0),
+                           blanking=(tuple(int(t) for t in bl[0]), tuple(int(t) for t in bl[1])),
+                           harness=harness)
+    return outs
+
+
+@pytest.mark.parametrize("key,mode,phi_key", BLANK_CONSUMERS, ids=[k for k, _, _ in BLANK_CONSUMERS])
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(refio.SHIM_HARNESS),
+                    reason="drop-in harness not built (make -C oracle/ref shim)")
+def test_dropin_blanking_consumers_bitwise(key, mode, phi_key):
+    """The consumer drop-ins with a blanking region, bit for bit."""
+    fx = load("blanking_consumers")
+    outs = replay_blank_consumer(fx, key, mode, phi_key, refio.SHIM_HARNESS)
+    n = 0
+    while f"out_{key}_{n}" in fx.d:
+        assert np.array_equal(outs[n], fx.out(key, n)), (key, n, linf_rel(outs[n], fx.out(key, n)))
+        n += 1
+    assert n == len(outs)
+
+
+@pytest.mark.skipif(not refio.available(), reason="reference harness not built (make -C oracle/ref)")
+@pytest.mark.parametrize("key,mode,phi_key", BLANK_CONSUMERS, ids=[k for k, _, _ in BLANK_CONSUMERS])
+def test_blanking_consumer_replay_reproduces_golden_with_reference(key, mode, phi_key):
+    fx = load("blanking_consumers")
+    outs = replay_blank_consumer(fx, key, mode, phi_key, refio.HARNESS)
+    n = 0
+    while f"out_{key}_{n}" in fx.d:
+        assert np.array_equal(outs[n], fx.out(key, n)), (key, n)
+        n += 1
+    assert n == len(outs)
+
+
 @pytest.mark.skipif(not refio.available(), reason="reference harness not built (make -C oracle/ref)")
 @pytest.mark.parametrize("key,mode", BLANK_EXACT + [("levelset_inner", "levelset"), ("hybrid_inner", "levelset")],
                          ids=[k for k, _ in BLANK_EXACT] + ["levelset_inner", "hybrid_inner"])
