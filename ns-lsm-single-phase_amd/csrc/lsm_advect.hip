@@ -470,7 +470,7 @@ static int rk3_run(const Blk &b, const double *phi_n, const int32_t *mask, const
     for (int st = 0; st < 3; ++st) {
         StageArgs a{src[st], phi_n, dst[st], U1, U2, U3, b.aj, mask, b.e.ni, b.e.nj, b.e.nk,
                     b.e.sj, b.e.sk, r, C[st][0], C[st][1], C[st][2], dt, b.dc, b.de, b.dz,
-                    tiles_i, tiles_j, kc, ntiles, 0};
+                    tiles_i, tiles_j, kc, ntiles, 0, 0};
         if (ntiles > 0) {
             KTimer kt(LSM_KT_ADVECT_STAGE, s);
             dim3 blk(TX, TY, 1);

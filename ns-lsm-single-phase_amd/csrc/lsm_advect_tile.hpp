@@ -57,6 +57,7 @@ struct StageArgs {
     double c1, c2, c3, dt, dc, de, dz;
     int tiles_i, tiles_j, kc, ntiles;
     int jb0;                // j of the first tile row
+    int ib0;                // i of the first tile column (a 128-byte line boundary of the rows)
 };
 
 // streamed operands are used once per stage: non-temporal loads/stores keep
@@ -90,7 +91,7 @@ __device__ __forceinline__ void advect_tile(const StageArgs &a, int tile)
     int tx, ty;
     if constexpr (W == TX) { tx = threadIdx.x; ty = threadIdx.y; }
     else { tx = threadIdx.x & (W - 1); ty = threadIdx.y * (TX / W) + (threadIdx.x / W); }
-    const int i = ti * W + tx, j = a.jb0 + tj * TY + ty;
+    const int i = a.ib0 + ti * W + tx, j = a.jb0 + tj * TY + ty;
     const int kb = a.r.klo + tk * a.kc;
     const int ke = min(kb + a.kc - 1, a.r.khi);
 
@@ -111,7 +112,7 @@ __device__ __forceinline__ void advect_tile(const StageArgs &a, int tile)
     const bool has_ih = lin < 4 * TY;                       // 16 i-halo cells
     const int hr = (lin >> 2) % TY, hc4 = lin & 3;
     const int hcol = (hc4 < 2) ? hc4 : hc4 + W;
-    const int hi_i = min(max(ti * W - HALO + hcol, 0), a.ni - 1);
+    const int hi_i = min(max(a.ib0 + ti * W - HALO + hcol, 0), a.ni - 1);
     const int hi_j = min(a.jb0 + tj * TY + hr, a.nj - 1);
     const long col_ih = (long)hi_i + a.sj * hi_j;
     auto kcl = [&](int kk) -> long { return a.sk * (long)min(max(kk, 0), a.nk - 1); };

@@ -45,11 +45,12 @@ struct BBlock {
     int gs[3];                // global index of the owned node (ia, ja, ka)
     int bt[6];
     double big, dc, de, dz;
-    int tiles_i, tiles_j, ntiles, tile0, kc, jb0;
+    int tiles_i, tiles_j, ntiles, tile0, kc, jb0, ib0;
     long shell0;              // first shell node of the block in the batch enumeration
     long nI, nJ, nK;
     int cnt_i, cnt_j;
     int rt_tiles_i, rt_tiles_j, rt_ntiles, rt_tile0, rt_kc;   // reinitialisation tiles (W x 256/W x rt_kc)
+    int rt_ib0;               // i of the first reinitialisation tile column (on a 128-byte line)
     int sh_wg0;               // first shell workgroup of the block (reinitialisation)
 };
 
@@ -68,7 +69,7 @@ __device__ __forceinline__ StageArgs stage_args(const BBlock &B, const BFields &
     a.ni = B.ni; a.nj = B.nj; a.nk = B.nk; a.sj = B.sj; a.sk = B.sk;
     a.r = B.r;
     a.c1 = f.c1; a.c2 = f.c2; a.c3 = f.c3; a.dt = f.dt; a.dc = B.dc; a.de = B.de; a.dz = B.dz;
-    a.tiles_i = B.tiles_i; a.tiles_j = B.tiles_j; a.kc = B.kc; a.ntiles = B.ntiles; a.jb0 = B.jb0;
+    a.tiles_i = B.tiles_i; a.tiles_j = B.tiles_j; a.kc = B.kc; a.ntiles = B.ntiles; a.jb0 = B.jb0; a.ib0 = B.ib0;
     return a;
 }
 
@@ -396,7 +397,7 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
     const int tj = t2 % B.rt_tiles_j, tk = t2 / B.rt_tiles_j;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const Range &r = B.r;
-    const int i0 = r.ilo + ti * W;
+    const int i0 = B.rt_ib0 + ti * W;
     const int i = i0 + tx, j = r.jlo + tj * TY + ty;
     const int kb = r.klo + tk * B.rt_kc;
     const int ke = live ? min(kb + B.rt_kc - 1, r.khi) : kb - 1;
@@ -644,7 +645,11 @@ int build_tiles(LsmBatch *B, int w, int ty, int kc)
     for (int b = 0; b < B->nb; ++b) {
         BBlock &X = B->hb[b];
         const Range &r = X.r;
-        X.tiles_i = (r.ihi + 1 + w - 1) / w;
+        // tiles start at the last column at or before the first owned one
+        // that lies on a 128-byte line (in the first row; the padded layouts
+        // align every row): ghost columns left of it are never computed
+        X.ib0 = max(0, r.ia - (int)((X.off + r.ia) % 16));
+        X.tiles_i = (r.ihi - X.ib0 + 1 + w - 1) / w;
         X.jb0 = r.jlo;
         const int nj_int = r.jhi - r.jlo + 1, nk_int = r.khi - r.klo + 1;
         X.tiles_j = nj_int > 0 ? (nj_int + ty - 1) / ty : 0;
@@ -806,7 +811,9 @@ int build_reinit_tables(LsmBatch *B)
         const int kc = 8;
         X.rt_kc = kc;
         const int ni_int = r.ihi - r.ilo + 1, nj_int = r.jhi - r.jlo + 1, nk_int = r.khi - r.klo + 1;
-        X.rt_tiles_i = ni_int > 0 ? (ni_int + W - 1) / W : 0;
+        // first tile column on a 128-byte line at or before the first interior column
+        X.rt_ib0 = max(0, r.ilo - (int)((X.off + r.ilo) % 16));
+        X.rt_tiles_i = ni_int > 0 ? (r.ihi - X.rt_ib0 + 1 + W - 1) / W : 0;
         X.rt_tiles_j = nj_int > 0 ? (nj_int + TY - 1) / TY : 0;
         const int tk = nk_int > 0 ? (nk_int + kc - 1) / kc : 0;
         X.rt_ntiles = (r.ihi >= r.ilo) ? X.rt_tiles_i * X.rt_tiles_j * tk : 0;
@@ -918,6 +925,7 @@ int batch_rk3(LsmBatch *B, double *phi_np1, const double *phi_n, const double *U
                 a.ni = X.ni; a.nj = X.nj; a.nk = X.nk; a.sj = X.sj; a.sk = X.sk; a.r = X.r;
                 a.c1 = f.c1; a.c2 = f.c2; a.c3 = f.c3; a.dt = dt; a.dc = X.dc; a.de = X.de; a.dz = X.dz;
                 a.tiles_i = X.tiles_i; a.tiles_j = X.tiles_j; a.kc = X.kc; a.ntiles = X.ntiles; a.jb0 = X.jb0;
+                a.ib0 = X.ib0;
                 if (X.ntiles > 0 && (rc = launch_stage_block(a, B->tune.ty, fast, s))) return rc;
                 ShellArgs sh{};
                 sh.out_ro = f.out + X.off; sh.out = f.out + X.off; sh.ps = f.ps + X.off; sh.pn = f.pn + X.off;
