@@ -355,7 +355,7 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
         // ---- owned nodes on physical faces: one node per thread ----
         const int w = bid - a.grid_tiles + a.sh_base;
         const int b = a.sh_block[w];
-        const BBlock &B = a.blocks[b];
+        const BBlock B = a.blocks[b];
         RCtl *ctl = a.ctl + b;
         if (!ctl->active) return;
         const long e = (long)(w - B.sh_wg0) * 256 + tid;
@@ -389,7 +389,10 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
     const bool live = tile < a.ntiles;
     tile += a.tile_base;
     const int b = a.rtile_block[live ? tile : a.tile_base];
-    const BBlock &B = a.blocks[b];
+    // a copy: the descriptor then lives in scalar registers for the whole
+    // tile instead of being re-read after every store (no alias analysis
+    // separates it from the output arrays)
+    const BBlock B = a.blocks[b];
     RCtl *ctl = a.ctl + b;
     if (!ctl->active) return;
     const int t = live ? tile - B.rt_tile0 : 0;
