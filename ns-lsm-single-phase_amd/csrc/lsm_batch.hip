@@ -343,7 +343,7 @@ __device__ __forceinline__ void rn_reduce(double vmax, double vmin, int tid, int
 // is not a multiple of 64, but more partly used cache lines per load: on the
 // 64-block workload 64 columns are fastest, 30.3 / 30.7 / 34.1 ms per full
 // step for 64 / 32 / 16).
-template <int STAGE, int W>
+template <int STAGE, int W, bool RED>
 __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
 {
     constexpr int TY = 256 / W;
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
             }
             vmax = ngv; vmin = ngv;
         }
-        if ((STAGE == 3 || STAGE == 0) && a.reduce) rn_reduce(vmax, vmin, tid, 256, ctl);
+        if constexpr (RED) rn_reduce(vmax, vmin, tid, 256, ctl);
         return;
     }
     // ---- interior tiles: W (i) x TY (j) columns marching up kc planes ----
@@ -473,12 +473,12 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
                 else upd = cv != 0;
                 __builtin_nontemporal_store(rn_update(a.p, ngv, q2, dnv, p0, upd), a.dout + B.off + l);
             }
-            if (STAGE == 3 || STAGE == 0) { vmax = fmax(vmax, ngv); vmin = fmin(vmin, ngv); }
+            if constexpr (RED) { vmax = fmax(vmax, ngv); vmin = fmin(vmin, ngv); }
         }
         q0 = q1; q1 = q2; q2 = q3; q3 = q4;
         buf ^= 1;
     }
-    if ((STAGE == 3 || STAGE == 0) && a.reduce) rn_reduce(vmax, vmin, tid, 256, ctl);
+    if constexpr (RED) rn_reduce(vmax, vmin, tid, 256, ctl);
 }
 
 // per-block exit-test state: a block with ghost layers counts |grad d| = 0
@@ -811,7 +811,8 @@ int build_reinit_tables(LsmBatch *B)
     for (int b = 0; b < B->nb; ++b) {
         BBlock &X = B->hb[b];
         const Range &r = X.r;
-        const int kc = 8;
+        int kc = 8;
+        if (const char *e = getenv("LSM_BATCH_REINIT_KCHUNK")) { int v = atoi(e); if (v > 0) kc = v; }
         X.rt_kc = kc;
         const int ni_int = r.ihi - r.ilo + 1, nj_int = r.jhi - r.jlo + 1, nk_int = r.khi - r.klo + 1;
         // first tile column on a 128-byte line at or before the first interior column
@@ -888,9 +889,18 @@ int reinit_stage_launch(LsmBatch *B, BRArgs a, int b0, int nb, hipStream_t s)
     a.grid_tiles = (a.ntiles + 7) / 8 * 8;
     const unsigned grid = (unsigned)(a.grid_tiles + (w1 - w0));
     if (grid == 0) return LSM_OK;
-    if (B->rt_w == 16) batch_reinit_stage<STAGE, 16><<<grid, dim3(16, 16, 1), 0, s>>>(a);
-    else if (B->rt_w == 32) batch_reinit_stage<STAGE, 32><<<grid, dim3(32, 8, 1), 0, s>>>(a);
-    else batch_reinit_stage<STAGE, 64><<<grid, dim3(64, 4, 1), 0, s>>>(a);
+    // the max / min reduction of the exit test is compiled in only where it is used
+    constexpr bool CAN = STAGE == 3 || STAGE == 0;
+    const bool red = CAN && a.reduce;
+    if (red) {
+        if (B->rt_w == 16) batch_reinit_stage<STAGE, 16, CAN><<<grid, dim3(16, 16, 1), 0, s>>>(a);
+        else if (B->rt_w == 32) batch_reinit_stage<STAGE, 32, CAN><<<grid, dim3(32, 8, 1), 0, s>>>(a);
+        else batch_reinit_stage<STAGE, 64, CAN><<<grid, dim3(64, 4, 1), 0, s>>>(a);
+    } else {
+        if (B->rt_w == 16) batch_reinit_stage<STAGE, 16, false><<<grid, dim3(16, 16, 1), 0, s>>>(a);
+        else if (B->rt_w == 32) batch_reinit_stage<STAGE, 32, false><<<grid, dim3(32, 8, 1), 0, s>>>(a);
+        else batch_reinit_stage<STAGE, 64, false><<<grid, dim3(64, 4, 1), 0, s>>>(a);
+    }
     return check_hip(hipGetLastError(), "batch_reinit_stage");
 }
 
