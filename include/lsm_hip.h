@@ -491,6 +491,27 @@ int lsm_levelset_step_adjoint_recorded(const LsmBlock *blk, const LsmReinitParam
                                        double *adj_ucn2, double *adj_ucn3, void *work, size_t work_bytes,
                                        void *stream);
 
+/* ---- multi-instance execution -------------------------------------------
+ * No reference counterpart: the reference advances one block per MPI rank.
+ * Between lsm_multi_begin and lsm_multi_end (on the calling thread) the
+ * entry points lsm_contra_velocity, lsm_rk3_step, lsm_bcond,
+ * lsm_rk3_step_adjoint, lsm_bcond_adjoint, lsm_reinit_sussman_eno2(_adjoint),
+ * lsm_levelset_step (without volume or hybrid reinitialisation),
+ * lsm_levelset_step_record, lsm_levelset_step_adjoint(_recorded) launch
+ * nothing: they validate their arguments and record their kernel launches
+ * for the current instance (lsm_multi_instance starts the next one; their
+ * `stream` arguments are ignored).  Every other entry point fails while
+ * recording, and recorded calls need explicit workspaces (one per instance:
+ * the instances run at the same time).  lsm_multi_end(stream) requires all
+ * instances to have recorded the same kernel sequence (same options, any
+ * block shapes) and issues it as one launch per kernel over all instances,
+ * in order, on `stream`; lsm_multi_cancel drops a recording.  The results
+ * are those of the calls made one after another. */
+int lsm_multi_begin(void);
+int lsm_multi_instance(void);
+int lsm_multi_end(void *stream);
+int lsm_multi_cancel(void);
+
 /* ---- host-memory adapter (for Fortran ISO_C_BINDING shims) -------------
  * Same operations on HOST arrays in the reference's layout: the adapter keeps
  * device mirrors of one block and copies arguments in and results out; every

@@ -17,6 +17,7 @@
 #include <stdlib.h>
 
 #include "lsm_internal.hpp"
+#include "lsm_multi.hpp"
 
 namespace lsm {
 
@@ -194,9 +195,30 @@ __device__ __forceinline__ void bc_coefs(int bt, double p2, double big, bool is_
     a = 1.0; c = 0.0;   // not reached: type 2 leaves other faces alone
 }
 
+__device__ __forceinline__ void adj_bcond_phase_run(const BcAdjArgs &a, int axis, int bid);
+
+struct BcAdjLaunch {
+    BcAdjArgs a;
+    int axis;
+};
+
+struct KBcAdj {
+    using Args = BcAdjLaunch;
+    static constexpr int kThreads = 256;
+    __device__ __forceinline__ static void run(const Args &x, int bid, int) { adj_bcond_phase_run(x.a, x.axis, bid); }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &x);
+};
+
 __global__ __launch_bounds__(256) void adj_bcond_phase(BcAdjArgs a, int axis)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    adj_bcond_phase_run(a, axis, (int)blockIdx.x);
+}
+
+void KBcAdj::one(unsigned g, dim3 b, hipStream_t s, const Args &x) { adj_bcond_phase<<<g, b, 0, s>>>(x.a, x.axis); }
+
+__device__ __forceinline__ void adj_bcond_phase_run(const BcAdjArgs &a, int axis, int bid)
+{
+    const long t = (long)bid * 256 + threadIdx.x;
     const AdjGeo &g = a.g;
     int n1, n2, nax;
     long s, s1, s2;
@@ -420,15 +442,34 @@ __global__ __launch_bounds__(AX * R) void adj_stage_fused(FusedArgs a)
 // the k partials stay in the column's register window.
 // ---------------------------------------------------------------------------
 template <int R>
+struct KAdjLean {
+    using Args = FusedArgs;
+    static constexpr int kThreads = AX * R;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int nb);
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+template <int R>
 __global__ __launch_bounds__(AX * R) void adj_stage_lean(FusedArgs a)
+{
+    KAdjLean<R>::run(a, (int)blockIdx.x, (int)gridDim.x);
+}
+
+template <int R>
+void KAdjLean<R>::one(unsigned g, dim3 b, hipStream_t s, const Args &a)
+{
+    adj_stage_lean<R><<<g, b, 0, s>>>(a);
+}
+
+template <int R>
+__device__ __forceinline__ void KAdjLean<R>::run(const FusedArgs &a, int bid, int nb)
 {
     constexpr int OY = R - 4;
     constexpr int NCELL = (R + 4) * (AX + 4);
     const AdjGeo &g = a.g;
     const int ntile = a.tiles_i * a.tiles_j * ((g.nk + a.kc - 1) / a.kc);
     // consecutive tiles on one XCD (workgroup b runs on XCD b % 8)
-    const int b = (gridDim.x & 7) == 0 ? (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3))
-                                       : (int)blockIdx.x;
+    const int b = (nb & 7) == 0 ? (int)((bid & 7) * (nb >> 3) + (bid >> 3)) : bid;
     if (b >= ntile) return;
     const int ti = b % a.tiles_i, t2 = b / a.tiles_i;
     const int tj = t2 % a.tiles_j, tk = t2 / a.tiles_j;
@@ -579,11 +620,26 @@ __global__ __launch_bounds__(AX * R) void adj_stage_lean(FusedArgs a)
     }
 }
 
-__global__ __launch_bounds__(256) void adj_fill(double *p, long n, double v)
-{
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < n) p[l] = v;
-}
+struct FillArgs {
+    double *p;
+    long n;
+    double v;
+};
+
+struct KFill {
+    using Args = FillArgs;
+    static constexpr int kThreads = 256;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int)
+    {
+        const long l = (long)bid * 256 + threadIdx.x;
+        if (l < a.n) a.p[l] = a.v;
+    }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+__global__ __launch_bounds__(256) void adj_fill(double *p, long n, double v) { KFill::run(FillArgs{p, n, v}, (int)blockIdx.x, (int)gridDim.x); }
+
+void KFill::one(unsigned g, dim3 b, hipStream_t s, const Args &a) { adj_fill<<<g, b, 0, s>>>(a.p, a.n, a.v); }
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 
@@ -616,8 +672,11 @@ int launch_bcond_adjoint(const Blk &b, const double *phi, double *adj, hipStream
     ba.phi = phi;
     ba.lam = adj;
     const long lines[3] = {(long)b.e.nj * b.e.nk, (long)b.e.ni * b.e.nk, (long)b.e.ni * b.e.nj};
-    for (int ax = 2; ax >= 0; --ax) adj_bcond_phase<<<nblk(lines[ax]), 256, 0, s>>>(ba, ax);
-    return check_hip(hipGetLastError(), "adj_bcond_phase");
+    for (int ax = 2; ax >= 0; --ax) {
+        const int rc = klaunch<KBcAdj>(nblk(lines[ax]), dim3(256), s, BcAdjLaunch{ba, ax}, "adj_bcond_phase");
+        if (rc) return rc;
+    }
+    return LSM_OK;
 }
 
 int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, const double *U1,
@@ -642,7 +701,7 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
     if (!stages &&
         (rc = launch_rk3_stages(b, phi_n, mask, U1, U2, U3, dt, mode, Fw[0], Fw[1], Fw[2], s))) return rc;
     if (!adj_in_place && (rc = launch_copy(adj_np1, L[0], b.n, s))) return rc;
-    adj_fill<<<nblk(b.n), 256, 0, s>>>(adj_n, b.n, 0.0);
+    if ((rc = klaunch<KFill>(nblk(b.n), dim3(256), s, FillArgs{adj_n, b.n, 0.0}, "adj_fill"))) return rc;
 
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
@@ -659,6 +718,8 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
     if (const char *e = getenv("LSM_ADJ_FUSED")) fused = atoi(e);
     int kc = 16;
     if (const char *e = getenv("LSM_ADJ_KCHUNK")) { const int v = atoi(e); if (v > 0) kc = v; }
+    if (fused != 2 && recording())
+        return fail(LSM_ERR_UNSUPPORTED, "only the default advection adjoint kernel can be recorded");
 
     double *lam = L[0], *other = L[1];
     for (int st = 2; st >= 0; --st) {
@@ -667,7 +728,9 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
         // (1) bcond_lsm transposed
         ba.phi = phi_s;
         ba.lam = lam;
-        for (int ax = 2; ax >= 0; --ax) adj_bcond_phase<<<nblk(lines[ax]), 256, 0, s>>>(ba, ax);
+        for (int ax = 2; ax >= 0; --ax)
+            if ((rc = klaunch<KBcAdj>(nblk(lines[ax]), dim3(256), s, BcAdjLaunch{ba, ax}, "adj_bcond_phase")))
+                return rc;
         if (fused == 2) {
             FusedArgs fa{g, prev, lam, U1, U2, U3, b.aj, mask, st == 0 ? nullptr : other, adj_n,
                          adjU1, adjU2, adjU3, C[st][0], C[st][1], C[st][2], dt, {b.dc, b.de, b.dz},
@@ -675,7 +738,7 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
             const long nt = (long)fa.tiles_i * fa.tiles_j * ((b.e.nk + kc - 1) / kc);
             const unsigned grid = (unsigned)((nt + 7) / 8 * 8);
             KTimer kt(LSM_KT_ADJOINT_STAGE, s);
-            adj_stage_lean<16><<<grid, dim3(AX, 16, 1), 0, s>>>(fa);
+            if ((rc = klaunch<KAdjLean<16>>(grid, dim3(AX, 16, 1), s, fa, "adj_stage_lean"))) return rc;
         } else if (fused == 8 || fused == 16) {
             FusedArgs fa{g, prev, lam, U1, U2, U3, b.aj, mask, st == 0 ? nullptr : other, adj_n,
                          adjU1, adjU2, adjU3, C[st][0], C[st][1], C[st][2], dt, {b.dc, b.de, b.dz},

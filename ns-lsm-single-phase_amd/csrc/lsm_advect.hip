@@ -16,27 +16,62 @@
 
 #include "lsm_internal.hpp"
 #include "lsm_advect_tile.hpp"
+#include "lsm_multi.hpp"
 
 namespace lsm {
 
 namespace {
 
 template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false>
+struct KAdvect {
+    using Args = StageArgs;
+    static constexpr int kThreads = TX * TY;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int nb)
+    {
+        const int tile = xcd_tile(bid, nb);
+        if (tile >= a.ntiles) return;
+        advect_tile<FAST, MASK, TY, NT, FOLD>(a, tile);
+    }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false>
 __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
 {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    if (tile >= a.ntiles) return;
-    advect_tile<FAST, MASK, TY, NT, FOLD>(a, tile);
+    KAdvect<FAST, MASK, TY, NT, FOLD>::run(a, (int)blockIdx.x, (int)gridDim.x);
 }
+
+template <bool FAST, bool MASK, int TY, bool NT, bool FOLD>
+void KAdvect<FAST, MASK, TY, NT, FOLD>::one(unsigned g, dim3 b, hipStream_t s, const Args &a)
+{
+    advect_stage<FAST, MASK, TY, NT, FOLD><<<g, b, 0, s>>>(a);
+}
+
+template <bool MASK>
+struct KShell {
+    using Args = ShellArgs;
+    static constexpr int kThreads = 256;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int)
+    {
+        const long t = (long)bid * 256 + threadIdx.x;
+        int i, j, k;
+        if (!shell_node(a, t, i, j, k)) return;
+        const double v = sh_f3<MASK>(a, i, j, k);
+        a.out[(long)i + a.sj * j + a.sk * k] = v;
+    }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
 
 template <bool MASK>
 __global__ __launch_bounds__(256) void stage_shell(ShellArgs a)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    int i, j, k;
-    if (!shell_node(a, t, i, j, k)) return;
-    const double v = sh_f3<MASK>(a, i, j, k);
-    a.out[(long)i + a.sj * j + a.sk * k] = v;
+    KShell<MASK>::run(a, (int)blockIdx.x, (int)gridDim.x);
+}
+
+template <bool MASK>
+void KShell<MASK>::one(unsigned g, dim3 b, hipStream_t s, const Args &a)
+{
+    stage_shell<MASK><<<g, b, 0, s>>>(a);
 }
 
 // ---------------------------------------------------------------------------
@@ -51,9 +86,9 @@ struct BcArgs {
     double big;
 };
 
-__global__ __launch_bounds__(256) void bcond_phase(BcArgs a, int axis)
+__device__ __forceinline__ void bcond_phase_run(const BcArgs &a, int axis, int bid)
 {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = (long)bid * 256 + threadIdx.x;
     int n1, n2;
     long s, s1, s2;
     bool f_lo, f_hi;
@@ -77,6 +112,25 @@ __global__ __launch_bounds__(256) void bcond_phase(BcArgs a, int axis)
         a.phi[l] = bc_face(bt_hi, a.phi[l - s], a.phi[l - 2 * s], a.phi[l], a.big, axis == 0);
     }
 }
+
+struct BcPhaseArgs {
+    BcArgs a;
+    int axis;
+};
+
+struct KBcPhase {
+    using Args = BcPhaseArgs;
+    static constexpr int kThreads = 256;
+    __device__ __forceinline__ static void run(const Args &x, int bid, int) { bcond_phase_run(x.a, x.axis, bid); }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &x);
+};
+
+__global__ __launch_bounds__(256) void bcond_phase(BcArgs a, int axis)
+{
+    bcond_phase_run(a, axis, (int)blockIdx.x);
+}
+
+void KBcPhase::one(unsigned g, dim3 b, hipStream_t s, const Args &x) { bcond_phase<<<g, b, 0, s>>>(x.a, x.axis); }
 
 // ---------------------------------------------------------------------------
 // standalone WENO3 right-hand side (calc_RH_AD_WENO3) on interior nodes
@@ -233,9 +287,20 @@ struct ContraArgs {
     long n;
 };
 
-__global__ __launch_bounds__(256) void contra_kernel(ContraArgs a)
+struct KContra {
+    using Args = ContraArgs;
+    static constexpr int kThreads = 256;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int);
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+__global__ __launch_bounds__(256) void contra_kernel(ContraArgs a) { KContra::run(a, (int)blockIdx.x, (int)gridDim.x); }
+
+void KContra::one(unsigned g, dim3 b, hipStream_t s, const Args &a) { contra_kernel<<<g, b, 0, s>>>(a); }
+
+__device__ __forceinline__ void KContra::run(const Args &a, int bid, int)
 {
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long l = (long)bid * 256 + threadIdx.x;
     if (l >= a.n) return;
     const double u = a.u[l * a.stride], v = a.v[l * a.stride], w = a.w[l * a.stride];
     const double tmp = 1.0 / a.aj[l];
@@ -265,11 +330,29 @@ __global__ __launch_bounds__(256) void fold_kernel(long n, const double *__restr
     V3[l] = (s * dz) * U3[l];
 }
 
+struct CopyArgs {
+    const double *src;
+    double *dst;
+    long n;
+};
+
+struct KCopy {
+    using Args = CopyArgs;
+    static constexpr int kThreads = 256;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int)
+    {
+        const long l = (long)bid * 256 + threadIdx.x;
+        if (l < a.n) a.dst[l] = a.src[l];
+    }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
 __global__ __launch_bounds__(256) void copy_kernel(const double *__restrict__ s, double *__restrict__ d, long n)
 {
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < n) d[l] = s[l];
+    KCopy::run(CopyArgs{s, d, n}, (int)blockIdx.x, (int)gridDim.x);
 }
+
+void KCopy::one(unsigned g, dim3 b, hipStream_t s, const Args &a) { copy_kernel<<<g, b, 0, s>>>(a.src, a.dst, a.n); }
 
 inline unsigned blocks_for(long n, int per) { return (unsigned)((n + per - 1) / per); }
 
@@ -302,8 +385,7 @@ int launch_shell_block(const ShellArgs &sh, hipStream_t s)
 int launch_copy(const double *src, double *dst, long n, hipStream_t s)
 {
     if (n <= 0) return LSM_OK;
-    copy_kernel<<<blocks_for(n, 256), 256, 0, s>>>(src, dst, n);
-    return check_hip(hipGetLastError(), "copy_kernel");
+    return klaunch<KCopy>(blocks_for(n, 256), dim3(256), s, CopyArgs{src, dst, n}, "copy_kernel");
 }
 
 int launch_contra(const Blk &b, const double *u, const double *v, const double *w, long stride,
@@ -311,8 +393,7 @@ int launch_contra(const Blk &b, const double *u, const double *v, const double *
 {
     ContraArgs a{u, v, w, stride, b.csi0, b.csi1, b.csi2, b.eta0, b.eta1, b.eta2,
                  b.zet0, b.zet1, b.zet2, b.aj, U1, U2, U3, b.n};
-    contra_kernel<<<blocks_for(b.n, 256), 256, 0, s>>>(a);
-    return check_hip(hipGetLastError(), "contra_kernel");
+    return klaunch<KContra>(blocks_for(b.n, 256), dim3(256), s, a, "contra_kernel");
 }
 
 static RhsArgs rhs_args(const Blk &b, const double *phi, const double *U1, const double *U2, const double *U3,
@@ -341,10 +422,10 @@ int launch_bcond(const Blk &b, double *phi, hipStream_t s)
     for (int q = 0; q < 6; ++q) a.bt[q] = b.btype[q];
     const long per[3] = {(long)b.e.nj * b.e.nk, (long)b.e.ni * b.e.nk, (long)b.e.ni * b.e.nj};
     for (int ax = 0; ax < 3; ++ax) {
-        bcond_phase<<<blocks_for(2 * per[ax], 256), 256, 0, s>>>(a, ax);
-        int rc = check_hip(hipGetLastError(), "bcond_phase");
+        int rc = klaunch<KBcPhase>(blocks_for(2 * per[ax], 256), dim3(256), s, BcPhaseArgs{a, ax}, "bcond_phase");
         if (rc) return rc;
     }
+    if (b.bl.n && recording()) return fail(LSM_ERR_UNSUPPORTED, "blanking regions cannot be recorded");
     return launch_bcond_blanking(b, phi, s);
 }
 
@@ -421,6 +502,8 @@ static int rk3_run(const Blk &b, const double *phi_n, const int32_t *mask, const
                    const double *U2, const double *U3, double dt, int mode, const double *const src[3],
                    double *const dst[3], hipStream_t s, bool folded)
 {
+    if (recording() && (mode == LSM_ADVECT_UPWIND1 || b.bl.n))
+        return fail(LSM_ERR_UNSUPPORTED, "orderAD = 0 and blanking regions cannot be recorded");
     if (mode == LSM_ADVECT_UPWIND1) return launch_upwind_rk3(b, phi_n, mask, U1, U2, U3, dt, src, dst, s);
     const Range r = make_range(b);
     static const double C[3][3] = {{1.0, 0.0, 1.0},
@@ -474,31 +557,30 @@ static int rk3_run(const Blk &b, const double *phi_n, const int32_t *mask, const
         if (ntiles > 0) {
             KTimer kt(LSM_KT_ADVECT_STAGE, s);
             dim3 blk(TX, TY, 1);
+            int rc;
             if (folded) {
-                if (mask) advect_stage<true, true, 4, true, true><<<grid, blk, 0, s>>>(a);
-                else advect_stage<true, false, 4, true, true><<<grid, blk, 0, s>>>(a);
+                if (mask) rc = klaunch<KAdvect<true, true, 4, true, true>>(grid, blk, s, a, "advect_stage");
+                else rc = klaunch<KAdvect<true, false, 4, true, true>>(grid, blk, s, a, "advect_stage");
             } else if (TY == 8) {
-                if (fast && mask) advect_stage<true, true, 8><<<grid, blk, 0, s>>>(a);
-                else if (fast) advect_stage<true, false, 8><<<grid, blk, 0, s>>>(a);
-                else if (mask) advect_stage<false, true, 8><<<grid, blk, 0, s>>>(a);
-                else advect_stage<false, false, 8><<<grid, blk, 0, s>>>(a);
+                if (fast && mask) rc = klaunch<KAdvect<true, true, 8>>(grid, blk, s, a, "advect_stage");
+                else if (fast) rc = klaunch<KAdvect<true, false, 8>>(grid, blk, s, a, "advect_stage");
+                else if (mask) rc = klaunch<KAdvect<false, true, 8>>(grid, blk, s, a, "advect_stage");
+                else rc = klaunch<KAdvect<false, false, 8>>(grid, blk, s, a, "advect_stage");
             } else {
-                if (fast && mask) advect_stage<true, true, 4><<<grid, blk, 0, s>>>(a);
-                else if (fast && nt_off) advect_stage<true, false, 4, false><<<grid, blk, 0, s>>>(a);
-                else if (fast) advect_stage<true, false, 4><<<grid, blk, 0, s>>>(a);
-                else if (mask) advect_stage<false, true, 4><<<grid, blk, 0, s>>>(a);
-                else advect_stage<false, false, 4><<<grid, blk, 0, s>>>(a);
+                if (fast && mask) rc = klaunch<KAdvect<true, true, 4>>(grid, blk, s, a, "advect_stage");
+                else if (fast && nt_off) rc = klaunch<KAdvect<true, false, 4, false>>(grid, blk, s, a, "advect_stage");
+                else if (fast) rc = klaunch<KAdvect<true, false, 4>>(grid, blk, s, a, "advect_stage");
+                else if (mask) rc = klaunch<KAdvect<false, true, 4>>(grid, blk, s, a, "advect_stage");
+                else rc = klaunch<KAdvect<false, false, 4>>(grid, blk, s, a, "advect_stage");
             }
-            int rc = check_hip(hipGetLastError(), "advect_stage");
             if (rc) return rc;
         }
         sh.out_ro = dst[st]; sh.out = dst[st]; sh.ps = src[st]; sh.pn = phi_n;
         sh.c1 = C[st][0]; sh.c2 = C[st][1]; sh.c3 = C[st][2];
         if (nshell > 0) {
             KTimer kt(LSM_KT_ADVECT_SHELL, s);
-            if (mask) stage_shell<true><<<blocks_for(nshell, 256), 256, 0, s>>>(sh);
-            else stage_shell<false><<<blocks_for(nshell, 256), 256, 0, s>>>(sh);
-            int rc = check_hip(hipGetLastError(), "stage_shell");
+            int rc = mask ? klaunch<KShell<true>>(blocks_for(nshell, 256), dim3(256), s, sh, "stage_shell")
+                          : klaunch<KShell<false>>(blocks_for(nshell, 256), dim3(256), s, sh, "stage_shell");
             if (rc) return rc;
         }
     }

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "lsm_internal.hpp"
+#include "lsm_multi.hpp"
 
 namespace lsm {
 
@@ -32,6 +33,8 @@ int check_hip(hipError_t e, const char *what)
 
 int make_blk(const LsmBlock *b, Blk *o, bool need_metrics, bool need_coords, BlankPolicy blanking)
 {
+    if (recording() && !recordable_here())
+        return fail(LSM_ERR_UNSUPPORTED, "this entry point cannot be recorded (lsm_multi_begin .. lsm_multi_end)");
     if (!b) return fail(LSM_ERR_ARG, "LsmBlock is NULL");
     const int ni = b->iu - b->il + 1, nj = b->ju - b->jl + 1, nk = b->ku - b->kl + 1;
     if (ni < 4 || nj < 4 || nk < 4)
@@ -139,6 +142,8 @@ static int workspace(void *work, size_t work_bytes, size_t need, Carver *c, hipS
         *c = Carver{(char *)work, work_bytes, 0};
         return LSM_OK;
     }
+    // recorded instances run together: one library buffer per stream would be shared by all of them
+    if (recording()) return fail(LSM_ERR_ARG, "recorded calls (lsm_multi_begin .. lsm_multi_end) need their own workspace");
     void *p = nullptr;
     int rc = default_work(need, s, &p);
     if (rc) return rc;
@@ -158,7 +163,7 @@ bool timing_on() { return g_t_on; }
 
 KTimer::KTimer(int kid, hipStream_t st) : id(kid), s(st)
 {
-    if (!g_t_on) return;
+    if (!g_t_on || recording()) return;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
     (void)hipEventRecord(a, s);
 }
@@ -309,6 +314,7 @@ int lsm_timing_get(int id, double *ms_total, int64_t *launches)
 
 size_t lsm_workspace_bytes(const LsmBlock *blk, int op)
 {
+    Recordable rec;
     Blk b;
     if (make_blk(blk, &b, false, false, kBlankIgnore)) return 0;
     return op_bytes(b, op);
@@ -316,6 +322,7 @@ size_t lsm_workspace_bytes(const LsmBlock *blk, int op)
 
 size_t lsm_adjoint_workspace_bytes(const LsmBlock *blk, int op, int rn_itermax)
 {
+    Recordable rec;
     Blk b;
     if (make_blk(blk, &b, false, false, kBlankIgnore)) return 0;
     if (rn_itermax < 0 || rn_itermax > kMaxTraceIters) return 0;
@@ -325,6 +332,7 @@ size_t lsm_adjoint_workspace_bytes(const LsmBlock *blk, int op, int rn_itermax)
 int lsm_contra_velocity(const LsmBlock *blk, const double *u, const double *v, const double *w,
                         int64_t vel_stride, double *ucn1, double *ucn2, double *ucn3, void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
@@ -359,6 +367,7 @@ int lsm_upwind_rhs(const LsmBlock *blk, const double *phi, const double *ucn1, c
 
 int lsm_bcond(const LsmBlock *blk, double *phi, void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = make_blk(blk, &b, false, false, kBlankUse);
     if (rc) return rc;
@@ -370,6 +379,7 @@ int lsm_rk3_step(const LsmBlock *blk, double *phi_np1, const double *phi_n, cons
                  const double *ucn1, const double *ucn2, const double *ucn3, double delti_lsm,
                  int mode, void *work, size_t work_bytes, void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = make_blk(blk, &b, false, false, kBlankUse);
     if (rc) return rc;
@@ -435,6 +445,7 @@ int lsm_rk3_step_adjoint(const LsmBlock *blk, const double *phi_n, const int32_t
                          int mode, const double *adj_np1, double *adj_n, double *adj_ucn1,
                          double *adj_ucn2, double *adj_ucn3, void *work, size_t work_bytes, void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = make_blk(blk, &b, false, false);
     if (rc) return rc;
@@ -455,6 +466,7 @@ int lsm_rk3_step_adjoint(const LsmBlock *blk, const double *phi_n, const int32_t
 
 int lsm_bcond_adjoint(const LsmBlock *blk, const double *phi, double *adj, void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = make_blk(blk, &b, false, false);
     if (rc) return rc;
@@ -556,6 +568,7 @@ int lsm_reinit_sussman_eno2(const LsmBlock *blk, const LsmReinitParams *p, const
                             double *phi_out, int32_t *iters_done, void *work, size_t work_bytes,
                             void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
@@ -572,6 +585,7 @@ int lsm_reinit_sussman_eno2_adjoint(const LsmBlock *blk, const LsmReinitParams *
                                     const double *adj_out, double *adj_phi0, double *phi_out,
                                     void *work, size_t work_bytes, void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = make_blk(blk, &b, true, false, kBlankIgnore);
     if (rc) return rc;
@@ -622,6 +636,7 @@ int lsm_levelset_step_adjoint(const LsmBlock *blk, const LsmReinitParams *p, con
                               double *adj_ucn1, double *adj_ucn2, double *adj_ucn3, double *phi_np1,
                               void *work, size_t work_bytes, void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = check_step_adjoint(blk, p, phi_n, ucn1, ucn2, ucn3, mode, b);
     if (rc) return rc;
@@ -645,6 +660,7 @@ int lsm_levelset_step_record(const LsmBlock *blk, const LsmReinitParams *p, cons
                              const double *ucn1, const double *ucn2, const double *ucn3, double delti_lsm,
                              int mode, double *phi_np1, void *trace, size_t trace_bytes, void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = check_step_adjoint(blk, p, phi_n, ucn1, ucn2, ucn3, mode, b);
     if (rc) return rc;
@@ -665,6 +681,7 @@ int lsm_levelset_step_adjoint_recorded(const LsmBlock *blk, const LsmReinitParam
                                        const double *adj_np1, double *adj_n, double *adj_ucn1, double *adj_ucn2,
                                        double *adj_ucn3, void *work, size_t work_bytes, void *stream)
 {
+    Recordable rec;
     Blk b;
     int rc = check_step_adjoint(blk, p, phi_n, ucn1, ucn2, ucn3, mode, b);
     if (rc) return rc;
@@ -744,7 +761,10 @@ int lsm_levelset_step(const LsmBlock *blk, const LsmReinitParams *p, double *phi
                       int call_reinit, int rn_freq, int ntime, int mode, double *vol_out,
                       void *work, size_t work_bytes, void *stream)
 {
+    Recordable rec;
     const bool hybrid = p && p->hybrid && call_reinit;
+    if (recording() && (vol_out || hybrid))
+        return fail(LSM_ERR_UNSUPPORTED, "the water volume and the hybrid reinitialisation cannot be recorded");
     Blk b;
     int rc = make_blk(blk, &b, true, vol_out != nullptr || hybrid, kBlankUse);
     if (rc) return rc;

@@ -37,7 +37,7 @@ int make_blk(const LsmBlock *b, Blk *out, bool need_metrics, bool need_coords,
              BlankPolicy blanking = kBlankReject);
 
 // library-owned buffers per (device, stream, purpose)
-enum { kBufWork = 0, kBufStore = 1 };
+enum { kBufWork = 0, kBufStore = 1, kBufMulti = 2 };
 int cached_buffer(int purpose, size_t bytes, hipStream_t s, void **ptr);
 int default_work(size_t bytes, hipStream_t s, void **ptr);
 

@@ -21,6 +21,7 @@
 
 #include "lsm_internal.hpp"
 #include "lsm_reinit_common.hpp"
+#include "lsm_multi.hpp"
 
 namespace lsm {
 
@@ -51,28 +52,53 @@ __global__ __launch_bounds__(256) void normgrad_kernel(Geo g, const double *__re
     ng[l] = normgrad_node(g, d, phi0, p0s, order_bc, eno_bc != 0, i, j, k, l);
 }
 
-__global__ void reinit_init_ctl(Ctl *ctl, int32_t *iters_out)
-{
-    ctl->maxbits = 0ull;
-    ctl->minbits = kInfBits;
-    ctl->active = 1;
-    ctl->iters = 0;
-    if (iters_out) *iters_out = 0;
-}
+struct CtlArgs {
+    Ctl *ctl;
+    double eps;
+    int32_t *iters_out;
+};
+
+struct KInitCtl {
+    using Args = CtlArgs;
+    static constexpr int kThreads = 1;
+    __device__ __forceinline__ static void run(const Args &a, int, int)
+    {
+        a.ctl->maxbits = 0ull;
+        a.ctl->minbits = kInfBits;
+        a.ctl->active = 1;
+        a.ctl->iters = 0;
+        if (a.iters_out) *a.iters_out = 0;
+    }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+__global__ void reinit_init_ctl(Ctl *ctl, int32_t *iters_out) { KInitCtl::run(CtlArgs{ctl, 0.0, iters_out}, 0, 1); }
+
+void KInitCtl::one(unsigned g, dim3 b, hipStream_t s, const Args &a) { reinit_init_ctl<<<g, b, 0, s>>>(a.ctl, a.iters_out); }
 
 // exit test at the top of a pseudo-time iteration (reinitialisation_test2.F90:100-107)
-__global__ void reinit_check(Ctl *ctl, double eps, int32_t *iters_out)
-{
-    if (ctl->active) {
-        const double mx = __longlong_as_double((long long)ctl->maxbits);
-        const double mn = __longlong_as_double((long long)ctl->minbits);
-        if (fabs(mx - 1.0) < eps && fabs(mn - 1.0) < eps) ctl->active = 0;
-        else ctl->iters += 1;
+struct KCheck {
+    using Args = CtlArgs;
+    static constexpr int kThreads = 1;
+    __device__ __forceinline__ static void run(const Args &a, int, int)
+    {
+        Ctl *ctl = a.ctl;
+        if (ctl->active) {
+            const double mx = __longlong_as_double((long long)ctl->maxbits);
+            const double mn = __longlong_as_double((long long)ctl->minbits);
+            if (fabs(mx - 1.0) < a.eps && fabs(mn - 1.0) < a.eps) ctl->active = 0;
+            else ctl->iters += 1;
+        }
+        ctl->maxbits = 0ull;
+        ctl->minbits = kInfBits;
+        if (a.iters_out) *a.iters_out = ctl->iters;
     }
-    ctl->maxbits = 0ull;
-    ctl->minbits = kInfBits;
-    if (iters_out) *iters_out = ctl->iters;
-}
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+__global__ void reinit_check(Ctl *ctl, double eps, int32_t *iters_out) { KCheck::run(CtlArgs{ctl, eps, iters_out}, 0, 1); }
+
+void KCheck::one(unsigned g, dim3 b, hipStream_t s, const Args &a) { reinit_check<<<g, b, 0, s>>>(a.ctl, a.eps, a.iters_out); }
 
 struct StageP {
     double c1, c2, c3, dt, epslsm, eps_r;
@@ -168,14 +194,33 @@ __device__ __forceinline__ void tape_write(const RTArgs &a, long l, bool upd, do
 
 // STAGE 0: only the max/min of |grad din| (reinitialisation_test2.F90:96-101)
 template <int STAGE, bool TAPE = false>
+struct KReinit {
+    using Args = RTArgs;
+    static constexpr int kThreads = RTX * RTY;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int nb);
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+template <int STAGE, bool TAPE = false>
 __global__ __launch_bounds__(RTX * RTY) void reinit_stage_tiled(RTArgs a)
+{
+    KReinit<STAGE, TAPE>::run(a, (int)blockIdx.x, (int)gridDim.x);
+}
+
+template <int STAGE, bool TAPE>
+void KReinit<STAGE, TAPE>::one(unsigned g, dim3 b, hipStream_t s, const Args &a)
+{
+    reinit_stage_tiled<STAGE, TAPE><<<g, b, 0, s>>>(a);
+}
+
+template <int STAGE, bool TAPE>
+__device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int bid, int)
 {
     if (!a.ctl->active) return;
     const Geo &g = a.g;
     const int tid = threadIdx.y * RTX + threadIdx.x;
     const double inf = __longlong_as_double((long long)kInfBits);
     double vmax = 0.0, vmin = inf;
-    const int bid = blockIdx.x;
 
     if (bid >= a.grid_tiles) {
         // ---- boundary shell: one node per thread, generic formulas ----
@@ -515,24 +560,26 @@ RTShape rt_shape(const Blk &b, const LsmReinitParams &p, const double *phi0, uin
     return r;
 }
 
-void rt_launch(RTShape &r, int stage, hipStream_t s, bool tape = false)
+int rt_launch(RTShape &r, int stage, hipStream_t s, bool tape = false)
 {
     const dim3 rblk(RTX, RTY, 1);
+    const char *w = "reinit_stage_tiled";
     switch (stage) {
-    case 0: reinit_stage_tiled<0><<<r.grid, rblk, 0, s>>>(r.a); break;
-    case 1:
-        if (tape) reinit_stage_tiled<1, true><<<r.grid, rblk, 0, s>>>(r.a);
-        else reinit_stage_tiled<1><<<r.grid, rblk, 0, s>>>(r.a);
-        break;
-    case 2:
-        if (tape) reinit_stage_tiled<2, true><<<r.grid, rblk, 0, s>>>(r.a);
-        else reinit_stage_tiled<2><<<r.grid, rblk, 0, s>>>(r.a);
-        break;
-    default:
-        if (tape) reinit_stage_tiled<3, true><<<r.grid, rblk, 0, s>>>(r.a);
-        else reinit_stage_tiled<3><<<r.grid, rblk, 0, s>>>(r.a);
-        break;
+    case 0: return klaunch<KReinit<0>>(r.grid, rblk, s, r.a, w);
+    case 1: return tape ? klaunch<KReinit<1, true>>(r.grid, rblk, s, r.a, w) : klaunch<KReinit<1>>(r.grid, rblk, s, r.a, w);
+    case 2: return tape ? klaunch<KReinit<2, true>>(r.grid, rblk, s, r.a, w) : klaunch<KReinit<2>>(r.grid, rblk, s, r.a, w);
+    default: return tape ? klaunch<KReinit<3, true>>(r.grid, rblk, s, r.a, w) : klaunch<KReinit<3>>(r.grid, rblk, s, r.a, w);
     }
+}
+
+int init_ctl(Ctl *ctl, int32_t *iters, hipStream_t s)
+{
+    return klaunch<KInitCtl>(1, dim3(1), s, CtlArgs{ctl, 0.0, iters}, "reinit_init_ctl");
+}
+
+int check_ctl(Ctl *ctl, double eps, int32_t *iters, hipStream_t s)
+{
+    return klaunch<KCheck>(1, dim3(1), s, CtlArgs{ctl, eps, iters}, "reinit_check");
 }
 
 void rt_stage_params(RTShape &r, const LsmReinitParams &p, int st)
@@ -553,26 +600,23 @@ int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi
     if (!A || !B || !conv || !ctl) return fail(LSM_ERR_WORKSPACE, "reinit workspace too small");
     int rc;
     if ((rc = launch_copy(phi0, out, b.n, s))) return rc;
-    reinit_init_ctl<<<1, 1, 0, s>>>(ctl, iters);
-    if ((rc = check_hip(hipGetLastError(), "reinit_init_ctl"))) return rc;
+    if ((rc = init_ctl(ctl, iters, s))) return rc;
     if (p.rn_itermax <= 0) return LSM_OK;
     RTShape r = rt_shape(b, p, phi0, conv, ctl);
     // max/min of |grad phi0| before the first iteration (reinitialisation_test2.F90:96-101)
     r.a.din = phi0; r.a.dn = phi0; r.a.dout = nullptr;
-    rt_launch(r, 0, s);
-    if ((rc = check_hip(hipGetLastError(), "reinit_stage_tiled"))) return rc;
+    if ((rc = rt_launch(r, 0, s))) return rc;
     const double *sin_[3] = {out, A, B};
     const double *sdn[3] = {out, out, out};
     double *sout[3] = {A, B, out};
     for (int it = 0; it < p.rn_itermax; ++it) {
-        reinit_check<<<1, 1, 0, s>>>(ctl, p.eps_reinit, iters);
+        if ((rc = check_ctl(ctl, p.eps_reinit, iters, s))) return rc;
         for (int st = 0; st < 3; ++st) {
             KTimer kt(LSM_KT_REINIT_STAGE, s);
             rt_stage_params(r, p, st);
             r.a.din = sin_[st]; r.a.dn = sdn[st]; r.a.dout = sout[st];
-            rt_launch(r, st + 1, s);
+            if ((rc = rt_launch(r, st + 1, s))) return rc;
         }
-        if ((rc = check_hip(hipGetLastError(), "reinit_stage"))) return rc;
     }
     return LSM_OK;
 }
@@ -622,16 +666,15 @@ int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const doubl
     if (!out || out == phi0) return fail(LSM_ERR_ARG, "reinit trace output missing or aliasing phi0");
     Ctl *ctl = (Ctl *)cv;
     if ((rc = launch_copy(phi0, out, b.n, s))) return rc;
-    reinit_init_ctl<<<1, 1, 0, s>>>(ctl, t.iters);
-    if ((rc = check_hip(hipGetLastError(), "reinit_init_ctl"))) return rc;
+    if ((rc = init_ctl(ctl, t.iters, s))) return rc;
     if (t.nit == 0) return LSM_OK;
     RTShape r = rt_shape(b, p, phi0, nullptr, ctl);
     r.a.din = phi0; r.a.dn = phi0; r.a.dout = nullptr;
-    rt_launch(r, 0, s);
+    if ((rc = rt_launch(r, 0, s))) return rc;
     const double *sin_[3] = {out, A, B};
     double *sout[3] = {A, B, out};
     for (int it = 0; it < t.nit; ++it) {
-        reinit_check<<<1, 1, 0, s>>>(ctl, p.eps_reinit, t.iters);
+        if ((rc = check_ctl(ctl, p.eps_reinit, t.iters, s))) return rc;
         r.a.conv = t.conv[it];
         for (int st = 0; st < 3; ++st) {
             rt_stage_params(r, p, st);
@@ -639,10 +682,10 @@ int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const doubl
             for (int d = 0; d < 3; ++d) r.a.tq[d] = t.q[it][st][d];
             r.a.tcode = t.code[it][st];
             r.a.tT = t.T[it][st];
-            rt_launch(r, st + 1, s, true);
+            if ((rc = rt_launch(r, st + 1, s, true))) return rc;
         }
     }
-    return check_hip(hipGetLastError(), "reinit trace");
+    return LSM_OK;
 }
 
 int launch_sign_ini(const Blk &b, const double *phi0, double eps, int m_sphi0, double *sgn, double *ng,

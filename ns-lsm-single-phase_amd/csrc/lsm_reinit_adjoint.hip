@@ -31,6 +31,7 @@
 
 #include "lsm_internal.hpp"
 #include "lsm_reinit_common.hpp"
+#include "lsm_multi.hpp"
 
 namespace lsm {
 
@@ -79,8 +80,39 @@ __device__ __noinline__ double face_coef(int br, int order_bc, int dir, bool eno
     return slot == 0 ? e[0] : slot == 1 ? e[1] : slot == 2 ? e[2] : e[3];
 }
 
+struct TGatherLaunch {
+    TGatherArgs a;
+    int tiles_i, tiles_j, kc;
+};
+
+template <int GY>
+__device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int tiles_i, int tiles_j, int kc, int bid);
+
+template <int GY>
+struct KTGather {
+    using Args = TGatherLaunch;
+    static constexpr int kThreads = GX * GY;
+    __device__ __forceinline__ static void run(const Args &x, int bid, int)
+    {
+        radj_tape_gather_run<GY>(x.a, x.tiles_i, x.tiles_j, x.kc, bid);
+    }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &x);
+};
+
 template <int GY>
 __global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherArgs a, int tiles_i, int tiles_j, int kc)
+{
+    radj_tape_gather_run<GY>(a, tiles_i, tiles_j, kc, (int)blockIdx.x);
+}
+
+template <int GY>
+void KTGather<GY>::one(unsigned g, dim3 b, hipStream_t s, const Args &x)
+{
+    radj_tape_gather<GY><<<g, b, 0, s>>>(x.a, x.tiles_i, x.tiles_j, x.kc);
+}
+
+template <int GY>
+__device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int tiles_i, int tiles_j, int kc, int bid)
 {
     __shared__ double sW0[GY][GX + 2 * GH];
     __shared__ double sW1[GY + 2 * GH][GX];
@@ -98,7 +130,7 @@ __global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherArgs a, int t
         eno2_coef(tid, e);
         for (int q = 0; q < 5; ++q) s_eno[tid][q] = e[q];
     }
-    const int b = blockIdx.x;
+    const int b = bid;
     const int ti = b % tiles_i, t2 = b / tiles_i;
     const int tj = t2 % tiles_j, tk = t2 / tiles_j;
     const int i0 = ti * GX, j0 = tj * GY;
@@ -189,19 +221,48 @@ __global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherArgs a, int t
 }
 
 // adj_phi0 = lam + adj_s * s'(phi0)
-__global__ __launch_bounds__(256) void radj_final(long n, const double *lam, const double *adj_s,
-                                                  const double *phi0, double eps, double *adj_phi0)
-{
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
-    adj_phi0[l] = lam[l] + adj_s[l] * sign_phi0_deriv(phi0[l], eps);
-}
+struct FinalArgs {
+    long n;
+    const double *lam, *adj_s, *phi0;
+    double eps;
+    double *adj_phi0;
+};
 
-__global__ __launch_bounds__(256) void radj_zero(long n, double *p)
-{
-    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < n) p[l] = 0.0;
-}
+struct KFinal {
+    using Args = FinalArgs;
+    static constexpr int kThreads = 256;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int)
+    {
+        const long l = (long)bid * 256 + threadIdx.x;
+        if (l >= a.n) return;
+        a.adj_phi0[l] = a.lam[l] + a.adj_s[l] * sign_phi0_deriv(a.phi0[l], a.eps);
+    }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+__global__ __launch_bounds__(256) void radj_final(FinalArgs a) { KFinal::run(a, (int)blockIdx.x, (int)gridDim.x); }
+
+void KFinal::one(unsigned g, dim3 b, hipStream_t s, const Args &a) { radj_final<<<g, b, 0, s>>>(a); }
+
+struct ZeroArgs {
+    long n;
+    double *p;
+};
+
+struct KZero {
+    using Args = ZeroArgs;
+    static constexpr int kThreads = 256;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int)
+    {
+        const long l = (long)bid * 256 + threadIdx.x;
+        if (l < a.n) a.p[l] = 0.0;
+    }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+__global__ __launch_bounds__(256) void radj_zero(long n, double *p) { KZero::run(ZeroArgs{n, p}, (int)blockIdx.x, (int)gridDim.x); }
+
+void KZero::one(unsigned g, dim3 b, hipStream_t s, const Args &a) { radj_zero<<<g, b, 0, s>>>(a.n, a.p); }
 
 inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 size_t fbytes(const Blk &b) { return ((size_t)b.n * sizeof(double) + 255) & ~(size_t)255; }
@@ -231,7 +292,7 @@ int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const dou
     double *L0 = (double *)w.take(fb), *L1 = (double *)w.take(fb);
     double *adn = (double *)w.take(fb), *adj_s = (double *)w.take(fb);
     if (!L0 || !L1 || !adn || !adj_s) return fail(LSM_ERR_WORKSPACE, "reinit adjoint workspace too small");
-    radj_zero<<<nblk(b.n), 256, 0, s>>>(b.n, adj_s);
+    if ((rc = klaunch<KZero>(nblk(b.n), dim3(256), s, ZeroArgs{b.n, adj_s}, "radj_zero"))) return rc;
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
@@ -254,15 +315,16 @@ int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const dou
             const int tiles_i = (b.e.ni + GX - 1) / GX, tiles_j = (b.e.nj + gy - 1) / gy;
             const int tiles_k = (b.e.nk + kc - 1) / kc;
             const unsigned grid = (unsigned)(tiles_i * tiles_j * tiles_k);
-            if (gy == 16) radj_tape_gather<16><<<grid, dim3(GX, 16, 1), 0, s>>>(ga, tiles_i, tiles_j, kc);
-            else radj_tape_gather<8><<<grid, dim3(GX, 8, 1), 0, s>>>(ga, tiles_i, tiles_j, kc);
+            const TGatherLaunch gl{ga, tiles_i, tiles_j, kc};
+            if (gy == 16) rc = klaunch<KTGather<16>>(grid, dim3(GX, 16, 1), s, gl, "radj_tape_gather");
+            else rc = klaunch<KTGather<8>>(grid, dim3(GX, 8, 1), s, gl, "radj_tape_gather");
+            if (rc) return rc;
             lam = other;
             wi ^= 1;
         }
-        if ((rc = check_hip(hipGetLastError(), "reinit adjoint kernels"))) return rc;
     }
-    radj_final<<<nblk(b.n), 256, 0, s>>>(b.n, lam, adj_s, phi0, p.epslsm, adj_phi0);
-    return check_hip(hipGetLastError(), "radj_final");
+    return klaunch<KFinal>(nblk(b.n), dim3(256), s, FinalArgs{b.n, lam, adj_s, phi0, p.epslsm, adj_phi0},
+                           "radj_final");
 }
 
 }  // namespace lsm

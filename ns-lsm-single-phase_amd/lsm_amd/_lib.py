@@ -142,6 +142,10 @@ _SIGS = {
     "lsm_host_release": (C.c_int, []),
     "lsm_host_invalidate": (C.c_int, []),
     "lsm_host_blanking": (C.c_int, [C.c_int32, _vp]),
+    "lsm_multi_begin": (C.c_int, []),
+    "lsm_multi_instance": (C.c_int, []),
+    "lsm_multi_end": (C.c_int, [_vp]),
+    "lsm_multi_cancel": (C.c_int, []),
     "lsm_host_consumer_blanking": (C.c_int, [C.c_int32, _vp]),
     "lsm_host_levelset_step": (C.c_int, [_vp, _vp, _vp, C.POINTER(LsmReinitParams), C.c_int, C.c_double,
                                          C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp]),

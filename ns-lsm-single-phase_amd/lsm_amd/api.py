@@ -9,6 +9,7 @@ torch's current stream.
 from __future__ import annotations
 
 import ctypes as C
+from contextlib import contextmanager
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -44,6 +45,52 @@ def _i32(t: Optional[torch.Tensor], shape, name: str):
     if tuple(t.shape) != tuple(shape):
         raise ValueError(f"{name}: shape {tuple(t.shape)} != block shape {tuple(shape)}")
     return C.c_void_p(t.data_ptr())
+
+
+# tensors allocated by this module inside a multi-instance recording, kept
+# alive until the recorded launches have been issued (lsm_multi_end)
+_recording: list = []
+
+
+def _keep(t):
+    if _recording:
+        _recording[-1].append(t)
+    return t
+
+
+class MultiRecorder:
+    def __init__(self, L):
+        self._L = L
+        self.count = 0
+
+    def instance(self):
+        """Starts the next instance: the library calls that follow belong to it."""
+        check(self._L.lsm_multi_instance(), "lsm_multi_instance")
+        self.count += 1
+
+
+@contextmanager
+def multi():
+    """Multi-instance execution (lsm_multi_begin .. lsm_multi_end): inside the
+    block, call rec.instance() before each instance's library calls; those
+    calls record their launches instead of issuing them, and at the end of
+    the block every kernel runs as one launch over all instances, on the
+    current stream.  Only library calls may touch the recorded outputs inside
+    the block (anything else would run before the recorded kernels)."""
+    L = _lib.load()
+    check(L.lsm_multi_begin(), "lsm_multi_begin")
+    keep: list = []
+    _recording.append(keep)
+    rec = MultiRecorder(L)
+    try:
+        yield rec
+    except BaseException:
+        L.lsm_multi_cancel()
+        raise
+    finally:
+        _recording.pop()
+    check(L.lsm_multi_end(_stream()), "lsm_multi_end")
+    del keep
 
 
 class Block:
@@ -92,16 +139,16 @@ class Block:
         return C.byref(self.desc)
 
     def empty(self, dtype=torch.float64):
-        return torch.empty(self.shape, dtype=dtype, device=self.device)
+        return _keep(torch.empty(self.shape, dtype=dtype, device=self.device))
 
     def zeros(self, dtype=torch.float64):
-        return torch.zeros(self.shape, dtype=dtype, device=self.device)
+        return _keep(torch.zeros(self.shape, dtype=dtype, device=self.device))
 
     def workspace(self, op: int) -> torch.Tensor:
         n = _lib.load().lsm_workspace_bytes(self.ref, op)
         if n == 0:
             raise LsmError("lsm_workspace_bytes failed: " + _lib.load().lsm_last_error().decode())
-        return torch.empty(n, dtype=torch.uint8, device=self.device)
+        return _keep(torch.empty(n, dtype=torch.uint8, device=self.device))
 
     def adjoint_workspace(self, op: int, rn_itermax: int) -> torch.Tensor:
         """Scratch of the adjoints that keep every reinitialisation iteration."""
@@ -109,7 +156,7 @@ class Block:
         if n == 0:
             raise LsmError(f"lsm_adjoint_workspace_bytes failed (rn_itermax = {rn_itermax}): "
                            + _lib.load().lsm_last_error().decode())
-        return torch.empty(n, dtype=torch.uint8, device=self.device)
+        return _keep(torch.empty(n, dtype=torch.uint8, device=self.device))
 
 
 @dataclass
@@ -334,7 +381,7 @@ def reinitialisation_test2(blk: Block, phi0, params: ReinitParams, out=None, wor
     L = _lib.load()
     if out is None:
         out = blk.empty()
-    iters = torch.zeros(1, dtype=torch.int32, device=blk.device)
+    iters = _keep(torch.zeros(1, dtype=torch.int32, device=blk.device))
     wp, wn, keep = _ws(blk, _lib.LSM_OP_REINIT_ENO2, work)
     p = params.c()
     check(L.lsm_reinit_sussman_eno2(blk.ref, C.byref(p), _f64(phi0, blk.shape, "phi0"),

@@ -18,7 +18,7 @@ INCLUDE = os.path.join(REPO, "include")
 LIB = os.path.join(PKG, "liblsm_hip.so")
 SOURCES = ["lsm_capi.hip", "lsm_advect.hip", "lsm_reinit.hip", "lsm_volume.hip", "lsm_host.hip",
            "lsm_adjoint.hip", "lsm_reinit_adjoint.hip", "lsm_batch.hip", "lsm_consumers.hip",
-           "lsm_upwind.hip", "lsm_geometric.hip"]
+           "lsm_upwind.hip", "lsm_geometric.hip", "lsm_multi.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LSM_OFFLOAD_ARCH", "gfx950")
