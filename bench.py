@@ -405,7 +405,9 @@ def bench_inverse(nsteps=60, every=10, ninst=32):
     perturbed amplitude (seed 9); gradients with respect to each instance's
     initial phi and amplitude H, checkpoint every 10 steps.  The forward runs
     as one independent batch (one launch per stage over all instances); the
-    gradient runs instance by instance."""
+    gradient's forward, replay and reverse sweeps run as multi-instance
+    executions (one launch per kernel over all instances), and the same
+    gradient issued instance by instance on 8 streams is timed beside it."""
     import torch
     from lsm_amd import _lib, batch as B, channels, inputs
     t0 = time.time()
@@ -432,9 +434,17 @@ def bench_inverse(nsteps=60, every=10, ninst=32):
     eta_t = channels.targets(cb, Ht, nsteps, every)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    J, g_phi, g_H = channels.loss_and_gradients(cb, eta_t, nsteps, every)
+    J, g_phi, g_H = channels.loss_and_gradients_batched(cb, eta_t, nsteps, every)
     torch.cuda.synchronize()
     grad_ms = (time.perf_counter() - t1) * 1e3
+    del g_phi
+    torch.cuda.empty_cache()
+    # the same gradient with the instances issued one by one on 8 streams
+    t1 = time.perf_counter()
+    J_s, _, g_H_s = channels.loss_and_gradients(cb, eta_t, nsteps, every)
+    torch.cuda.synchronize()
+    streams_ms = (time.perf_counter() - t1) * 1e3
+    same = J_s == J and g_H_s == g_H
     return {"workload": f"{ninst} solitary-wave channels (nx U{{128..512}}, ny U{{16..64}}, nz U{{48..128}}, "
                         f"H U[0.1,0.5]), {nsteps} steps x (RK3 + 4 reinitialisation iterations + bcond), "
                         f"surface-elevation misfit, gradients w.r.t. phi0 and H, checkpoint every {every}",
@@ -443,7 +453,11 @@ def bench_inverse(nsteps=60, every=10, ninst=32):
             "batched_forward_cell_updates_per_s": cells * nsteps / (fwd_ms / 1e3),
             "loss": J, "dJ_dH_first": g_H[:4],
             "forward_plus_gradient_ms": round(grad_ms, 1),
-            "gradient_over_batched_forward": round(grad_ms / fwd_ms, 3)}
+            "gradient_over_batched_forward": round(grad_ms / fwd_ms, 3),
+            "gradient_form": "multi-instance: one launch per kernel over all instances, forward, replay and "
+                             "reverse sweep",
+            "per_instance_streams_forward_plus_gradient_ms": round(streams_ms, 1),
+            "per_instance_streams_same_result": bool(same)}
 
 
 def bench_differentiation(n=192, nsteps=20, every=5):

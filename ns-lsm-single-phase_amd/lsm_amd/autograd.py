@@ -15,7 +15,8 @@
 * levelset_step(blk, phi_n, ucn, dt, params): one full step
   bcond_lsm(reinitialisation_test2(ad_RKTVD3(phi^n))) with its adjoint
   (lsm_levelset_step_adjoint); levelset_trajectory(...) runs nsteps of them
-  with checkpointing every k steps.
+  with checkpointing every k steps; multi_trajectory(...) does the same for
+  n independent blocks at once, one launch per kernel over all of them.
 
 * free_surface(blk, phi): the free-surface height z_fs of every column
   (find_z_free_surface, as calc_h obtains it) with its adjoint
@@ -242,6 +243,94 @@ def levelset_trajectory(blk, phi0, ucn: Sequence[torch.Tensor], dt: float, param
     every = max(1, int(checkpoint_every))
     return _StepTrajectory.apply(phi0, ucn[0], ucn[1], ucn[2], blk, float(dt), params, int(nsteps), mode,
                                  every)
+
+
+class _MultiTrajectory(torch.autograd.Function):
+    """levelset_trajectory of n independent blocks advanced together: every
+    step (forward, replay with recording, reverse sweep) is one multi-instance
+    execution (api.multi), i.e. one launch per kernel over all blocks."""
+
+    @staticmethod
+    def forward(ctx, blks, dt, params, nsteps, mode, every, *t):
+        n = len(blks)
+        phi0, U = t[:n], [tuple(t[n + 3 * b + c].contiguous() for c in range(3)) for b in range(n)]
+        cur = [p.contiguous().clone() for p in phi0]
+        nxt = [blk.empty() for blk in blks]
+        tmp = [blk.empty() for blk in blks]
+        wrk = [blk.workspace(api._lib.LSM_OP_RK3) for blk in blks]
+        wrn = [blk.workspace(api._lib.LSM_OP_REINIT_ENO2) for blk in blks]
+        ckpts = []
+        for s in range(nsteps):
+            if s % every == 0:
+                ckpts.append([c.clone() for c in cur])
+            with api.multi() as m:
+                for b, blk in enumerate(blks):
+                    m.instance()
+                    x = api.ad_RKTVD3(blk, cur[b], U[b], dt, mode=mode, out=tmp[b], work=wrk[b])
+                    api.reinitialisation_test2(blk, x, params, out=nxt[b], work=wrn[b])
+                    api.bcond_lsm(blk, nxt[b])
+            cur, nxt = nxt, cur
+        flat_u = [u for b in range(n) for u in U[b]]
+        ctx.save_for_backward(*flat_u, *[c for seg in ckpts for c in seg])
+        ctx.blks, ctx.dt, ctx.params, ctx.nsteps, ctx.mode, ctx.every = blks, dt, params, nsteps, mode, every
+        ctx.nseg = len(ckpts)
+        return tuple(cur)
+
+    @staticmethod
+    def backward(ctx, *g):
+        blks, dt, params, nsteps, mode, every = ctx.blks, ctx.dt, ctx.params, ctx.nsteps, ctx.mode, ctx.every
+        n = len(blks)
+        saved = ctx.saved_tensors
+        U = [tuple(saved[3 * b + c] for c in range(3)) for b in range(n)]
+        ck = saved[3 * n:]
+        ckpts = [ck[q * n:(q + 1) * n] for q in range(ctx.nseg)]
+        want_u = any(ctx.needs_input_grad[6 + n:])
+        aU = [[torch.zeros_like(U[b][0]) for _ in range(3)] for b in range(n)] if want_u else [None] * n
+        wad = [blk.adjoint_workspace(api._lib.LSM_OP_STEP_ADJOINT_RECORDED, params.RNitermax) for blk in blks]
+        lam = [(gb if gb is not None else torch.zeros_like(U[b][0])).contiguous().clone() for b, gb in enumerate(g)]
+        nxt = [blk.empty() for blk in blks]
+        traces = []
+        for sidx in range(ctx.nseg - 1, -1, -1):
+            s0 = sidx * every
+            s1 = min(s0 + every, nsteps)
+            # replay the segment once for all blocks, recording every step
+            states = [list(ckpts[sidx])]
+            for q in range(s1 - s0):
+                if len(traces) <= q:
+                    traces.append([blk.adjoint_workspace(api._lib.LSM_OP_STEP_TRACE, params.RNitermax)
+                                   for blk in blks])
+                last = q == s1 - s0 - 1
+                nstate = None if last else [blk.empty() for blk in blks]
+                with api.multi() as m:
+                    for b, blk in enumerate(blks):
+                        m.instance()
+                        api.levelset_step_record(blk, states[-1][b], U[b], dt, params, mode=mode,
+                                                 phi_np1=None if last else nstate[b], trace=traces[q][b])
+                if not last:
+                    states.append(nstate)
+            for q in range(s1 - s0 - 1, -1, -1):
+                with api.multi() as m:
+                    for b, blk in enumerate(blks):
+                        m.instance()
+                        api.levelset_step_adjoint_recorded(blk, states[q][b], U[b], dt, params, traces[q][b],
+                                                           lam[b], mode=mode, adj_ucn=aU[b], out=nxt[b],
+                                                           work=wad[b])
+                lam, nxt = nxt, lam
+        gu = [aU[b][c] if want_u else None for b in range(n) for c in range(3)]
+        return (None, None, None, None, None, None, *lam, *gu)
+
+
+def multi_trajectory(blks, phi0s, ucns, dt: float, params: "api.ReinitParams", nsteps: int,
+                     mode: str = "exact", checkpoint_every: int = 5):
+    """levelset_trajectory for n independent blocks (phi0s[b], ucns[b] = (U1,
+    U2, U3) of block b), advanced together: each step, replayed step and
+    reverse sweep runs as one launch per kernel over all blocks.  Returns the
+    n final fields."""
+    if nsteps < 1:
+        raise ValueError("nsteps must be >= 1")
+    flat = [u for U in ucns for u in U]
+    return _MultiTrajectory.apply(list(blks), float(dt), params, int(nsteps), mode,
+                                  max(1, int(checkpoint_every)), *phi0s, *flat)
 
 
 def heaviside(phi, eps: float):

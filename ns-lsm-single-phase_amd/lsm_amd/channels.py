@@ -139,3 +139,26 @@ def loss_and_gradients(cb: ChannelBatch, eta_t, nsteps: int = 60, every: int = 1
     torch.cuda.synchronize()
     J = float(sum(float(j) for j in Js))
     return J, g_phi, [float(H.grad) for H in Hs]
+
+
+def loss_and_gradients_batched(cb: ChannelBatch, eta_t, nsteps: int = 60, every: int = 10, mode: str = "fast"):
+    """loss_and_gradients with all instances advanced together: the forward
+    steps, the replayed steps and the reverse sweeps of the checkpointed
+    trajectory each run as one launch per kernel over every instance
+    (autograd.multi_trajectory); phi0(H), u(H) and the surface elevation stay
+    per-instance torch operations."""
+    Hs, dphis, phi0s, ucns = [], [], [], []
+    for b, blk in enumerate(cb.blocks):
+        H = torch.tensor(cb.specs[b].H, dtype=torch.float64, device=blk.device, requires_grad=True)
+        dphi = blk.zeros().requires_grad_(True)
+        Hs.append(H)
+        dphis.append(dphi)
+        phi0s.append(autograd.solitary_phi(blk, H, d=cb.depth) + dphi)
+        ucns.append(autograd.contravariant(blk, *autograd.solitary_velocity(blk, H, d=cb.depth)))
+    phiT = autograd.multi_trajectory(cb.blocks, phi0s, ucns, cb.dt, cb.params, nsteps, mode=mode,
+                                     checkpoint_every=every)
+    Js = [((elevation(cb, b, phiT[b]) - eta_t[b]) ** 2).mean() for b in range(len(cb.blocks))]
+    J = torch.stack(Js).sum()
+    J.backward()
+    torch.cuda.synchronize()
+    return float(J), [d.grad for d in dphis], [float(H.grad) for H in Hs]

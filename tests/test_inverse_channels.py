@@ -166,3 +166,22 @@ def test_gpu_batched_forward_equals_instances(small_batch):
             for _ in range(NSTEPS):
                 p = autograd.levelset_step(blk, p, U, cb.dt, cb.params, mode="exact")
         assert np.array_equal(B.owned(lay, b, got[b]), _host(p)), b
+
+
+@pytest.mark.gpu
+def test_gpu_inverse_multi_instance_gradients_equal_per_instance(small_batch):
+    """The gradients with every step run as one launch per kernel over all
+    channels (loss_and_gradients_batched) equal those of the channels issued
+    one by one on streams, bit for bit."""
+    import torch
+    cb = small_batch
+    print(f"[inverse] multi-instance gradients, {len(cb.specs)} channels", flush=True)
+    Ht = channels.perturbed_amplitudes(cb.specs, seed=9, rel=0.1)
+    eta_t = channels.targets(cb, Ht, NSTEPS, EVERY, mode="exact")
+    J, g_phi, g_H = channels.loss_and_gradients(cb, eta_t, NSTEPS, EVERY, mode="exact")
+    Jm, g_phi_m, g_H_m = channels.loss_and_gradients_batched(cb, eta_t, NSTEPS, EVERY, mode="exact")
+    assert Jm == J
+    assert g_H_m == g_H
+    for a, b in zip(g_phi_m, g_phi):
+        assert np.array_equal(_host(a), _host(b))
+    torch.cuda.synchronize()

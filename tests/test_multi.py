@@ -148,3 +148,39 @@ def test_multi_rejects_misuse_and_recovers():
         m.instance()
         y = api.ad_RKTVD3(b0, p0, u0, dt0, mode="fast")
     _eq(x, y)
+
+
+@pytest.mark.gpu
+def test_multi_trajectory_matches_per_block_trajectories():
+    """Five checkpointed steps of three blocks advanced together
+    (multi_trajectory) against levelset_trajectory per block: the final
+    fields and the gradients with respect to phi0 and U/J bit for bit."""
+    import torch
+    from lsm_amd import autograd
+    print("[multi] trajectory", flush=True)
+    cases = _cases()
+    prm = _params()
+    dt = min(c[3] for c in cases)
+    rng = np.random.default_rng(11)
+    ws = [torch.as_tensor(rng.standard_normal(blk.shape), device="cuda") for blk, _, _, _ in cases]
+
+    def inputs():
+        phis = [p.clone().requires_grad_(True) for _, p, _, _ in cases]
+        us = [[u.clone().requires_grad_(True) for u in ucn] for _, _, ucn, _ in cases]
+        return phis, us
+
+    phis, us = inputs()
+    ref = []
+    for (blk, _, _, _), p, u, w in zip(cases, phis, us, ws):
+        out = autograd.levelset_trajectory(blk, p, u, dt, prm, 5, mode="fast", checkpoint_every=2)
+        (out * w).sum().backward()
+        ref.append((out.detach().clone(), p.grad.clone(), [t.grad.clone() for t in u]))
+    phis, us = inputs()
+    outs = autograd.multi_trajectory([c[0] for c in cases], phis, us, dt, prm, 5, mode="fast",
+                                     checkpoint_every=2)
+    sum((o * w).sum() for o, w in zip(outs, ws)).backward()
+    for b in range(len(cases)):
+        _eq(outs[b].detach(), ref[b][0])
+        _eq(phis[b].grad, ref[b][1])
+        for c in range(3):
+            _eq(us[b][c].grad, ref[b][2][c])
