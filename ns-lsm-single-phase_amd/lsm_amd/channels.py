@@ -158,7 +158,7 @@ def loss_and_gradients_batched(cb: ChannelBatch, eta_t, nsteps: int = 60, every:
     phiT = autograd.multi_trajectory(cb.blocks, phi0s, ucns, cb.dt, cb.params, nsteps, mode=mode,
                                      checkpoint_every=every)
     Js = [((elevation(cb, b, phiT[b]) - eta_t[b]) ** 2).mean() for b in range(len(cb.blocks))]
-    J = torch.stack(Js).sum()
-    J.backward()
+    torch.stack(Js).sum().backward()
     torch.cuda.synchronize()
-    return float(J), [d.grad for d in dphis], [float(H.grad) for H in Hs]
+    # the loss summed in instance order, as loss_and_gradients does
+    return float(sum(float(j.detach()) for j in Js)), [d.grad for d in dphis], [float(H.grad) for H in Hs]
