@@ -124,6 +124,9 @@ __device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int t
     __shared__ double sK[7][GY][GX];
     __shared__ uint16_t sKc[7][GY][GX];
     auto rs = [](int kk) { return (kk + 70) % 7; };
+    constexpr int NT = GX * GY;
+    constexpr int N0 = GY * (GX + 2 * GH), N1 = (GY + 2 * GH) * GX, NC = (GY + 2 * GH) * (GX + 2 * GH);
+    constexpr int R0 = (N0 + NT - 1) / NT, R1 = (N1 + NT - 1) / NT, RC = (NC + NT - 1) / NT;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * GX + tx;
     if (tid < 16) {
         double e[5];
@@ -142,6 +145,23 @@ __device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int t
     const bool active = a.it < *a.iters;
     const bool j_on = j == 0 || j == a.nj - 1;
     auto kcl = [&](int kk) { return min(max(kk, 0), a.nk - 1); };
+    // in-plane offsets of this thread's halo-fill entries (the same on every plane)
+    int o0[R0], o1[R1], oc[RC];
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+        const int c = min(tid + r * NT, N0 - 1), rr = c / (GX + 2 * GH), cc = c % (GX + 2 * GH);
+        o0[r] = min(max(i0 - GH + cc, 0), a.ni - 1) + (int)a.sj * min(j0 + rr, a.nj - 1);
+    }
+#pragma unroll
+    for (int r = 0; r < R1; ++r) {
+        const int c = min(tid + r * NT, N1 - 1), rr = c / GX, cc = c % GX;
+        o1[r] = min(i0 + cc, a.ni - 1) + (int)a.sj * min(max(j0 - GH + rr, 0), a.nj - 1);
+    }
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+        const int c = min(tid + r * NT, NC - 1), rr = c / (GX + 2 * GH), cc = c % (GX + 2 * GH);
+        oc[r] = min(max(i0 - GH + cc, 0), a.ni - 1) + (int)a.sj * min(max(j0 - GH + rr, 0), a.nj - 1);
+    }
     // k window: planes k-3 .. k+3 of this column (clamped; entries outside the
     // block are never used)
     for (int q = 0; q < 7; ++q) {
@@ -150,31 +170,47 @@ __device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int t
         sK[rs(kk)][ty][tx] = a.lam[l] * a.q[2][l];
         sKc[rs(kk)][ty][tx] = a.code[l];
     }
+    // the halo planes of the next plane are loaded into registers while the
+    // current plane's sums run, and stored to LDS after the plane's barrier
+    double l0[R0], q0[R0], l1[R1], q1[R1];
+    uint16_t cv[RC];
+    auto fetch = [&](int k) {
+        const double *lk = a.lam + a.sk * (long)k, *q0k = a.q[0] + a.sk * (long)k, *q1k = a.q[1] + a.sk * (long)k;
+        const uint16_t *ck = a.code + a.sk * (long)k;
+#pragma unroll
+        for (int r = 0; r < R0; ++r) { l0[r] = lk[o0[r]]; q0[r] = q0k[o0[r]]; }
+#pragma unroll
+        for (int r = 0; r < R1; ++r) { l1[r] = lk[o1[r]]; q1[r] = q1k[o1[r]]; }
+#pragma unroll
+        for (int r = 0; r < RC; ++r) cv[r] = ck[oc[r]];
+    };
+    fetch(kb);
     for (int k = kb; k <= ke; ++k) {
         const long pk = a.sk * (long)k;
-        // i products with the i halo (rows of the tile)
-        for (int c = tid; c < GY * (GX + 2 * GH); c += GX * GY) {
-            const int rr = c / (GX + 2 * GH), cc = c % (GX + 2 * GH);
-            const long l = (long)min(max(i0 - GH + cc, 0), a.ni - 1) + a.sj * min(j0 + rr, a.nj - 1) + pk;
-            sW0[rr][cc] = a.lam[l] * a.q[0][l];
-        }
-        // j products with the j halo (columns of the tile)
-        for (int c = tid; c < (GY + 2 * GH) * GX; c += GX * GY) {
-            const int rr = c / GX, cc = c % GX;
-            const long l = (long)min(i0 + cc, a.ni - 1) + a.sj * min(max(j0 - GH + rr, 0), a.nj - 1) + pk;
-            sW1[rr][cc] = a.lam[l] * a.q[1][l];
-        }
-        // codes with both halos
-        for (int c = tid; c < (GY + 2 * GH) * (GX + 2 * GH); c += GX * GY) {
-            const int rr = c / (GX + 2 * GH), cc = c % (GX + 2 * GH);
-            const long l = (long)min(max(i0 - GH + cc, 0), a.ni - 1) + a.sj * min(max(j0 - GH + rr, 0), a.nj - 1) + pk;
-            sC[rr][cc] = a.code[l];
-        }
+        double *w0 = &sW0[0][0], *w1 = &sW1[0][0];
+        uint16_t *wc = &sC[0][0];
+#pragma unroll
+        for (int r = 0; r < R0; ++r)
+            if (tid + r * NT < N0) w0[tid + r * NT] = l0[r] * q0[r];
+#pragma unroll
+        for (int r = 0; r < R1; ++r)
+            if (tid + r * NT < N1) w1[tid + r * NT] = l1[r] * q1[r];
+#pragma unroll
+        for (int r = 0; r < RC; ++r)
+            if (tid + r * NT < NC) wc[tid + r * NT] = cv[r];
         __syncthreads();
+        const long m = col + pk;
+        const double lam = a.lam[m];
+        const uint8_t cnv = a.conv[m];
+        double adn0 = 0.0;
+        if (a.stage != 3) adn0 = a.adn[m];
+        // the next plane of the k window and of the halos
+        const long lk = col + a.sk * (long)kcl(k + 4);
+        const double nlk = a.lam[lk], nqk = a.q[2][lk];
+        const uint16_t nck = a.code[lk];
+        if (k < ke) fetch(k + 1);
         if (inb) {
-            const long m = col + pk;
-            const double lam = a.lam[m];
-            const bool upd = active && a.conv[m];
+            const bool upd = active && cnv;
             double acc = upd ? a.c2 * lam : lam;
             if (active) {
                 if (sC[ty + GH][tx + GH] & kHasT) a.adj_s[m] += lam * a.T[m];
@@ -207,15 +243,12 @@ __device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int t
             }
             const double lin = upd ? a.c1 * lam : 0.0;
             if (a.stage == 3) { a.adn[m] = lin; a.out[m] = acc; }
-            else if (a.stage == 2) { a.adn[m] += lin; a.out[m] = acc; }
-            else a.out[m] = acc + lin + a.adn[m];
+            else if (a.stage == 2) { a.adn[m] = adn0 + lin; a.out[m] = acc; }
+            else a.out[m] = acc + lin + adn0;
         }
         // slide the k window: plane k + 4 replaces plane k - 3
-        {
-            const long l = col + a.sk * (long)kcl(k + 4);
-            sK[rs(k + 4)][ty][tx] = a.lam[l] * a.q[2][l];
-            sKc[rs(k + 4)][ty][tx] = a.code[l];
-        }
+        sK[rs(k + 4)][ty][tx] = nlk * nqk;
+        sKc[rs(k + 4)][ty][tx] = nck;
         __syncthreads();
     }
 }
