@@ -266,7 +266,13 @@ def bench_step_adjoint(g, phi, ucn, dt, blk):
 
     def bwd():
         api.levelset_step_adjoint(blk, phi, ucn, dt, prm, lam, mode="fast", adj_ucn=aU, out=adj, work=wad)
-    f_ms, b_ms = _time(fwd, 5), _time(bwd, 5)
+    # forward and adjoint timed in alternating rounds (the median of each), so
+    # that both see the same clock and thermal state
+    f_runs, b_runs = [], []
+    for _ in range(3):
+        f_runs.append(_time(fwd, 5))
+        b_runs.append(_time(bwd, 3))
+    f_ms, b_ms = sorted(f_runs)[1], sorted(b_runs)[1]
     n_int = (g.ni - 2) * (g.nj - 2) * (g.nk - 2)
     # the reverse-sweep kernels' own durations (library HIP events, one more call)
     import ctypes as C

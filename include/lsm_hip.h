@@ -105,6 +105,12 @@ typedef struct LsmBlock {
                                          points whose reference routines do not read the
                                          blanking arrays ignore it; the adjoints, the folded
                                          step and lsm_free_surface reject nblk != 0 */
+    const double *gten;               /* optional (NULL: none): the metric tensor of the
+                                         Sussman reinitialisation, 6 x n doubles written by
+                                         lsm_metric_tensor from csi, eta, zet; with it the
+                                         reinitialisation stages read 6 fields per node
+                                         instead of 9 and skip the per-call tensor pass.
+                                         The caller keeps it in step with the metrics. */
 } LsmBlock;
 
 /* Parameters of the pseudo-time reinitialisation (global_lsm.F90:17-58). */
@@ -274,10 +280,23 @@ int lsm_free_surface_adjoint(const LsmBlock *blk, const double *z, const double 
 
 /* reinitialisation_test2.F90:1-218 (reinitialisation_test2, Sussman path,
  * non-hybrid): phi_out = reinitialised phi0.  iters_done (device int32, may be
- * NULL) receives the number of pseudo-time iterations that ran. */
+ * NULL) receives the number of pseudo-time iterations that ran.  At nodes off
+ * the block faces |grad d| is evaluated as sqrt(g^T G g) from the metric
+ * tensor G_ab = grad xi_a . grad xi_b (lsm_metric_tensor) and the ENO2
+ * derivatives g: the reference's |csi g_xi + eta g_eta + zet g_zeta|
+ * (calc_NormGrad_d.F90:145-159) up to rounding. */
 int lsm_reinit_sussman_eno2(const LsmBlock *blk, const LsmReinitParams *p,
                             const double *phi0, double *phi_out, int32_t *iters_done,
                             void *work, size_t work_bytes, void *stream);
+
+/* No reference counterpart: the metric tensor the reinitialisation stages
+ * read, gten[c * n + l] for node l (n = the block's node count) and
+ * c = 0..5: G_xx, G_xe, G_xz, G_ee, G_ez, G_zz of G_ab = grad xi_a . grad xi_b
+ * (xi_a = xi, eta, zeta; grad xi = csi, grad eta = eta, grad zeta = zet).
+ * Computed once per grid and passed as LsmBlock.gten it saves three of the
+ * nine metric reads per node and stage; without it every reinitialisation
+ * call computes it into its workspace. */
+int lsm_metric_tensor(const LsmBlock *blk, double *gten, void *stream);
 
 /* Adjoint of lsm_reinit_sussman_eno2 (reverse mode of
  * reinitialisation_test2.F90:42-210 with calc_NormGrad_d.F90:39-2446 and

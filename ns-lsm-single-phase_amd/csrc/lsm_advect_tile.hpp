@@ -36,16 +36,6 @@ inline Range make_range(const Blk &b)
     return r;
 }
 
-// workgroup -> tile, keeping consecutive tiles on one XCD: the dispatcher
-// deals workgroups round-robin over the 8 XCDs, so workgroup b runs on XCD
-// b % 8; giving each XCD a contiguous run of tiles keeps the j/k halo reuse
-// inside one L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-    if ((nb & 7) != 0) return b;
-    return (b & 7) * (nb >> 3) + (b >> 3);
-}
-
 struct StageArgs {
     const double *ps, *pn;
     double *out;

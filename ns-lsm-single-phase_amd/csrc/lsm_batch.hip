@@ -30,6 +30,7 @@
 
 #include "lsm_advect_tile.hpp"
 #include "lsm_internal.hpp"
+#include "lsm_reinit_common.hpp"
 
 namespace lsm {
 
@@ -249,6 +250,8 @@ struct BRArgs {
     int tile_base, sh_base;       // first tile / shell workgroup of this launch
     int ntiles, grid_tiles;       // tiles; tile workgroups (padded to 8), shell workgroups follow
     const double *m[9];           // batch metric arrays (csi, eta, zet)
+    const double *gt;             // metric tensor of every node: component c at gt + c * gs
+    long gs;
     const double *din, *dn, *phi0;
     double *dout;
     uint8_t *conv;
@@ -307,16 +310,8 @@ __device__ double normgrad_node_r(const double *__restrict__ d, const double *__
 // the stage update at one node (reinitialisation_test2.F90:137-181)
 __device__ __forceinline__ double rn_update(const RStage &p, double ngv, double dv, double dnv, double p0, bool upd)
 {
-    double v = dv;
-    if (upd) {
-        const double s = sign_phi0(p0, p.eps);
-        double rhs = s * (ngv - 1.0);
-        const double lam = lambda_mass(dv, s, ngv, p.eps);
-        const double aux = deltaf(dv, p.eps) * ngv;
-        rhs = rhs - lam * aux;
-        v = (p.c1 * dnv + p.c2 * dv) - (p.c3 * p.dt) * rhs;
-    }
-    return v;
+    if (!upd) return dv;
+    return sussman_update(p.c1, p.c2, p.c3 * p.dt, ngv, dv, dnv, sign_phi0(p0, p.eps), deltaf(dv, p.eps));
 }
 
 __device__ __forceinline__ void rn_reduce(double vmax, double vmin, int tid, int nthreads, RCtl *ctl)
@@ -449,9 +444,9 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
         if (act) {
             const long l = col + sk * (long)k;
             const double p0 = phi0[l];
-            double m[9];
+            double G[6];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) m[c] = __builtin_nontemporal_load(a.m[c] + B.off + l);
+            for (int c = 0; c < 6; ++c) G[c] = __builtin_nontemporal_load(a.gt + c * a.gs + B.off + l);
             double dnv = q2;
             uint8_t cv = 0;
             if (STAGE > 1) { dnv = __builtin_nontemporal_load(a.dn + B.off + l); cv = a.conv[B.off + l]; }
@@ -463,10 +458,8 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
                                          jhi ? 0.0 : pl[cy + 2][cx], jlo, jhi);
             const bool klo = r.fd && k == r.klo, khi = r.fu && k == r.khi;
             const double g2 = dd2 * eno2(p0, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi);
-            const double gx = (m[0] * g0 + m[3] * g1) + m[6] * g2;
-            const double gy = (m[1] * g0 + m[4] * g1) + m[7] * g2;
-            const double gz = (m[2] * g0 + m[5] * g1) + m[8] * g2;
-            const double ngv = sqrt((gx * gx + gy * gy) + gz * gz);
+            double t[3];
+            const double ngv = rn::tensor_norm(G, g0, g1, g2, t);
             if (STAGE > 0) {
                 bool upd;
                 if (STAGE == 1) { upd = ngv - 1.0 > a.p.eps_r; a.conv[B.off + l] = upd ? 1 : 0; }
@@ -587,6 +580,7 @@ struct LsmBatch {
     int32_t *d_tile_block = nullptr;
     int32_t *d_block_of = nullptr;
     const double *met[9] = {nullptr};  // batch metric arrays (csi, eta, zet), may be NULL
+    double *gten = nullptr;            // metric tensor of every node (6 x total), with the metrics
     int32_t *d_rtile_block = nullptr;  // reinitialisation tiles
     int rt_ntiles = 0;
     int rt_w = 64;                     // reinitialisation tile columns (64, 32 or 16; rows 256 / rt_w)
@@ -984,6 +978,7 @@ int batch_reinit(LsmBatch *B, const LsmReinitParams &p, const double *phi0, doub
     BRArgs a{};
     a.blocks = B->db; a.rtile_block = B->d_rtile_block; a.sh_block = B->d_sh_block;
     for (int c = 0; c < 9; ++c) a.m[c] = B->met[c];
+    a.gt = B->gten; a.gs = B->total;
     a.phi0 = phi0; a.conv = conv; a.ctl = ctl;
     a.p = RStage{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
     const bool per = B->tune.form == LSM_BATCH_PERBLOCK;
@@ -1162,6 +1157,18 @@ int lsm_batch_create(const LsmBatchDesc *d, LsmBatch **out)
     if ((rc = build_tiles(B, 64, 4, 8))) { lsm_batch_destroy(B); return rc; }
     if ((rc = build_halo(B))) { lsm_batch_destroy(B); return rc; }
     if ((rc = build_reinit_tables(B))) { lsm_batch_destroy(B); return rc; }
+    // the reinitialisation reads the metric tensor (computed here once: the
+    // batch's grid is fixed for its lifetime)
+    bool have_met = true;
+    for (int c = 0; c < 9; ++c) have_met = have_met && B->met[c];
+    if (have_met) {
+        if ((rc = check_hip(hipMalloc(&B->gten, 6 * (size_t)B->total * sizeof(double)), "hipMalloc")) ||
+            (rc = launch_metric_tensor(B->met, B->total, B->gten, nullptr)) ||
+            (rc = check_hip(hipStreamSynchronize(nullptr), "hipStreamSynchronize"))) {
+            lsm_batch_destroy(B);
+            return rc;
+        }
+    }
     *out = B;
     return LSM_OK;
 }
@@ -1177,6 +1184,7 @@ int lsm_batch_destroy(LsmBatch *B)
     if (B->d_segs) (void)hipFree(B->d_segs);
     if (B->d_rtile_block) (void)hipFree(B->d_rtile_block);
     if (B->d_sh_block) (void)hipFree(B->d_sh_block);
+    if (B->gten) (void)hipFree(B->gten);
     for (int ax = 0; ax < 3; ++ax) {
         if (B->d_bseg[ax]) (void)hipFree(B->d_bseg[ax]);
         if (B->d_bwg[ax]) (void)hipFree(B->d_bwg[ax]);

@@ -57,6 +57,7 @@ int make_blk(const LsmBlock *b, Blk *o, bool need_metrics, bool need_coords, Bla
     o->zet0 = b->zet[0]; o->zet1 = b->zet[1]; o->zet2 = b->zet[2];
     o->aj = b->aj; o->x = b->x; o->y = b->y; o->z = b->z;
     o->n = (long)ni * nj * nk;
+    o->gten = b->gten;
     if (need_metrics) {
         for (int q = 0; q < 3; ++q)
             if (!b->csi[q] || !b->eta[q] || !b->zet[q])
@@ -579,6 +580,15 @@ int lsm_reinit_sussman_eno2(const LsmBlock *blk, const LsmReinitParams *p, const
     rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_REINIT_ENO2), &c, (hipStream_t)stream);
     if (rc) return rc;
     return launch_reinit_eno2(b, *p, phi0, phi_out, iters_done, c, (hipStream_t)stream);
+}
+
+int lsm_metric_tensor(const LsmBlock *blk, double *gten, void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false, kBlankIgnore);
+    if (rc) return rc;
+    if (!gten) return fail(LSM_ERR_ARG, "gten is NULL");
+    return launch_metric_tensor(b, gten, (hipStream_t)stream);
 }
 
 int lsm_reinit_sussman_eno2_adjoint(const LsmBlock *blk, const LsmReinitParams *p, const double *phi0,

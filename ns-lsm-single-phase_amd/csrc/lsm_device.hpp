@@ -14,6 +14,17 @@
 
 namespace lsm {
 
+// workgroup -> tile, keeping consecutive tiles on one XCD: the dispatcher
+// deals workgroups round-robin over the 8 XCDs, so workgroup b runs on XCD
+// b % 8; giving each XCD a contiguous run of tiles keeps the j/k halo reuse
+// inside one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    if ((nb & 7) != 0) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+
 constexpr double kEpsSims = 2.220446049250313e-16;   // epsilon(1._rdf), precision.F90:62
 constexpr double kPi = 3.141592653589793;            // precision.F90:51
 constexpr double kTiny = 2.2250738585072014e-308;    // tiny(eps_sims), TetrahedronMethods.F90:177
@@ -316,14 +327,27 @@ __device__ __forceinline__ double deltaf(double v, double eps)
     return (1.0 / (2.0 * eps)) * (1.0 + band_cos((kPi * v) / eps));
 }
 
-// GetLambdaMassCorrection, global_lsm.F90:200-219
-__device__ __forceinline__ double lambda_mass(double p, double s, double g, double eps)
+// GetLambdaMassCorrection, global_lsm.F90:200-219, given del = deltaf(d)
+__device__ __forceinline__ double lambda_mass_del(double del, double s, double g)
 {
-    const double del = deltaf(p, eps);
     const double num = -((del * s) * (1.0 - g));
     const double den = (del * del) * g;
     if (fabs(den) < kEpsSims) return 0.0;
     return num / den;
+}
+
+// The update of one node of a Sussman stage (reinitialisation_test2.F90:137-181):
+// (c1 d^n + c2 d) - (c3 dtau) rhs, rhs = s (|grad d| - 1) - lambda deltaf(d) |grad d|.
+// s = sign_phi0(phi0) and del = deltaf(d) are the caller's (each evaluated once
+// per node: the band sin / cos are out-of-line calls the compiler cannot merge).
+__device__ __forceinline__ double sussman_update(double c1, double c2, double c3dt, double ngv, double dv,
+                                                 double dnv, double s, double del)
+{
+    double rhs = s * (ngv - 1.0);
+    const double lam = lambda_mass_del(del, s, ngv);
+    const double aux = del * ngv;
+    rhs = rhs - lam * aux;
+    return (c1 * dnv + c2 * dv) - c3dt * rhs;
 }
 
 // One face of bcond_lsm.F90:53-287.  bt: 0 copy the neighbour, 1 second-order

@@ -22,6 +22,7 @@ struct Blk {
     const double *aj, *x, *y, *z;
     long n;                 // nodes in the block
     Blank bl;               // blanking regions, 0-based (bl.n = 0: none)
+    const double *gten;     // the caller's metric tensor (6 x n), or NULL
 };
 
 // thread-local last-error text
@@ -124,6 +125,9 @@ int launch_rk3_adjoint(const Blk &b, const double *phi_n, const int32_t *mask, c
 
 // ---- launchers (lsm_reinit.hip) ----
 size_t reinit_eno2_work(const Blk &b);
+// the reinitialisation's metric tensor (6 x n doubles, component c at gt + c * n)
+int launch_metric_tensor(const Blk &b, double *gt, hipStream_t s);
+int launch_metric_tensor(const double *const m[9], long n, double *gt, hipStream_t s);
 size_t reinit_godunov_work(const Blk &b);
 int launch_normgrad(const Blk &b, const double *d, const double *phi0, int order_bc, int eno_bc,
                     double *ng, hipStream_t s);

@@ -100,6 +100,34 @@ __global__ void reinit_check(Ctl *ctl, double eps, int32_t *iters_out) { KCheck:
 
 void KCheck::one(unsigned g, dim3 b, hipStream_t s, const Args &a) { reinit_check<<<g, b, 0, s>>>(a.ctl, a.eps, a.iters_out); }
 
+// the metric tensor of every node (rn::metric_tensor), component c at gt + c * n
+struct MetricArgs {
+    const double *m[9];
+    long n;
+    double *gt;
+};
+
+struct KMetric {
+    using Args = MetricArgs;
+    static constexpr int kThreads = 256;
+    __device__ __forceinline__ static void run(const Args &a, int bid, int)
+    {
+        const long l = (long)bid * 256 + threadIdx.x;
+        if (l >= a.n) return;
+        double m[9], G[6];
+#pragma unroll
+        for (int c = 0; c < 9; ++c) m[c] = __builtin_nontemporal_load(a.m[c] + l);
+        rn::metric_tensor(m, G);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a.gt[c * a.n + l] = G[c];
+    }
+    static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
+};
+
+__global__ __launch_bounds__(256) void metric_tensor_kernel(MetricArgs a) { KMetric::run(a, (int)blockIdx.x, (int)gridDim.x); }
+
+void KMetric::one(unsigned g, dim3 b, hipStream_t s, const Args &a) { metric_tensor_kernel<<<g, b, 0, s>>>(a); }
+
 struct StageP {
     double c1, c2, c3, dt, epslsm, eps_r;
     int order_bc, eno_bc;
@@ -123,6 +151,8 @@ struct RTArgs {
     const double *din, *dn;
     double *dout;
     const double *phi0;
+    const double *gt;   // metric tensor of the interior nodes: component c at gt + c * gs
+    long gs;
     uint8_t *conv;
     Ctl *ctl;
     // tape of the stage for the adjoint (TAPE kernels): per node the
@@ -135,21 +165,18 @@ struct RTArgs {
     long nI, nJ, nK;                                 // shell nodes on i, j and k faces
 };
 
-// the stage update at one node, given |grad d| (reinitialisation_test2.F90:137-181)
+// the stage update at one node, given |grad d| (reinitialisation_test2.F90:137-181);
+// s = sign_phi0(phi0) and del = deltaf(d) are returned for the tape
 template <int STAGE>
 __device__ __forceinline__ double reinit_update(const StageP &p, double ngv, double dv, double dnv,
-                                                double p0, bool upd)
+                                                double p0, bool upd, double &s, double &del)
 {
-    double v = dv;
-    if (upd) {
-        const double s = sign_phi0(p0, p.epslsm);
-        double rhs = s * (ngv - 1.0);
-        const double lam = lambda_mass(dv, s, ngv, p.epslsm);
-        const double aux = deltaf(dv, p.epslsm) * ngv;
-        rhs = rhs - lam * aux;
-        v = (p.c1 * dnv + p.c2 * dv) - (p.c3 * p.dt) * rhs;
-    }
-    return v;
+    s = 0.0;
+    del = 0.0;
+    if (!upd) return dv;
+    s = sign_phi0(p0, p.epslsm);
+    del = deltaf(dv, p.epslsm);
+    return sussman_update(p.c1, p.c2, p.c3 * p.dt, ngv, dv, dnv, s, del);
 }
 
 // block-wide max/min (256 threads, linear id tid) into ctl
@@ -173,16 +200,15 @@ __device__ __forceinline__ void reduce_maxmin_t(double vmax, double vmin, int ti
 
 // the tape entry of one node: w are the direction weights already divided by
 // |grad d| and scaled by the spacing, br the branch codes
-__device__ __forceinline__ void tape_write(const RTArgs &a, long l, bool upd, double ngv, double dv, double p0,
-                                           const double w[3], int br)
+__device__ __forceinline__ void tape_write(const RTArgs &a, long l, bool upd, double ngv, double p0, double s,
+                                           double del, const double w[3], int br)
 {
     uint16_t code = 0;
     if (upd) {
-        const double del = deltaf(dv, a.p.epslsm);
         if (fabs((del * del) * ngv) < kEpsSims) {   // lambda = 0: rhs = s (|grad d| - 1)
             const double c3dt = a.p.c3 * a.p.dt;
             if (fabs(p0) <= a.p.epslsm) { a.tT[l] = -c3dt * (ngv - 1.0); code |= rn::kHasT; }
-            const double f = -c3dt * sign_phi0(p0, a.p.epslsm);
+            const double f = -c3dt * s;
             if (f != 0.0 && ngv > 0.0) {
                 a.tq[0][l] = f * w[0]; a.tq[1][l] = f * w[1]; a.tq[2][l] = f * w[2];
                 code |= (uint16_t)(rn::kLive | br);
@@ -256,8 +282,9 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int b
                 else upd = a.conv[l] != 0;
                 const double dv = a.din[l];
                 const double dnv = (STAGE == 1) ? dv : a.dn[l];
-                a.dout[l] = reinit_update<STAGE>(a.p, ngv, dv, dnv, a.phi0[l], upd);
-                if constexpr (TAPE) tape_write(a, l, upd, ngv, dv, a.phi0[l], tw, tbr);
+                double sv, del;
+                a.dout[l] = reinit_update<STAGE>(a.p, ngv, dv, dnv, a.phi0[l], upd, sv, del);
+                if constexpr (TAPE) tape_write(a, l, upd, ngv, a.phi0[l], sv, del, tw, tbr);
             }
             vmax = ngv; vmin = ngv;
         }
@@ -320,9 +347,9 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int b
         if (act) {
             const long l = col + g.sk * (long)k;
             const double p0 = a.phi0[l];
-            double m[9];
+            double G[6];
 #pragma unroll
-            for (int c = 0; c < 9; ++c) m[c] = __builtin_nontemporal_load(g.m[c] + l);
+            for (int c = 0; c < 6; ++c) G[c] = __builtin_nontemporal_load(a.gt + c * a.gs + l);
             double dnv = q2;
             uint8_t cv = 0;
             if (STAGE > 1) { dnv = __builtin_nontemporal_load(a.dn + l); cv = a.conv[l]; }
@@ -335,15 +362,14 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int b
                                              pl[cy + 1][cx], jhi ? 0.0 : pl[cy + 2][cx], jlo, jhi);
             const bool klo = k == 1, khi = k == g.nk - 2;
             const double g2 = g.dd[2] * eno2(p0, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi);
-            const double gx = (m[0] * g0 + m[3] * g1) + m[6] * g2;
-            const double gy = (m[1] * g0 + m[4] * g1) + m[7] * g2;
-            const double gz = (m[2] * g0 + m[5] * g1) + m[8] * g2;
-            const double ngv = sqrt((gx * gx + gy * gy) + gz * gz);
+            double t[3];
+            const double ngv = rn::tensor_norm(G, g0, g1, g2, t);
             if (STAGE > 0) {
                 bool upd;
                 if (STAGE == 1) { upd = ngv - 1.0 > a.p.eps_r; a.conv[l] = upd ? 1 : 0; }
                 else upd = cv != 0;
-                __builtin_nontemporal_store(reinit_update<STAGE>(a.p, ngv, q2, dnv, p0, upd), a.dout + l);
+                double sv, del;
+                __builtin_nontemporal_store(reinit_update<STAGE>(a.p, ngv, q2, dnv, p0, upd, sv, del), a.dout + l);
                 if constexpr (TAPE) {
                     const int br = eno2_branch(p0, ilo ? 0.0 : pl[cy][cx - 2], pl[cy][cx - 1], q2, pl[cy][cx + 1],
                                                ihi ? 0.0 : pl[cy][cx + 2], ilo, ihi) |
@@ -353,9 +379,8 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int b
                     const double inv = ngv > 0.0 ? 1.0 / ngv : 0.0;
                     double tw[3];
 #pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        tw[d] = ((gx * m[3 * d] + gy * m[3 * d + 1]) + gz * m[3 * d + 2]) * inv * g.dd[d];
-                    tape_write(a, l, upd, ngv, q2, p0, tw, br);
+                    for (int d = 0; d < 3; ++d) tw[d] = t[d] * inv * g.dd[d];
+                    tape_write(a, l, upd, ngv, p0, sv, del, tw, br);
                 }
             }
             if (STAGE == 3 || STAGE == 0) { vmax = fmax(vmax, ngv); vmin = fmin(vmin, ngv); }
@@ -512,9 +537,37 @@ size_t fb(const Blk &b) { return ((size_t)b.n * sizeof(double) + 255) & ~(size_t
 
 }  // namespace
 
-size_t reinit_eno2_work(const Blk &b) { return 2 * fb(b) + (((size_t)b.n + 255) & ~(size_t)255) + 1024; }
-
 size_t reinit_godunov_work(const Blk &b) { return 4 * fb(b) + 1024; }
+
+int launch_metric_tensor(const double *const m[9], long n, double *gt, hipStream_t s)
+{
+    MetricArgs a{};
+    for (int c = 0; c < 9; ++c) a.m[c] = m[c];
+    a.n = n;
+    a.gt = gt;
+    return klaunch<KMetric>(nb(n), dim3(256), s, a, "metric_tensor_kernel");
+}
+
+int launch_metric_tensor(const Blk &b, double *gt, hipStream_t s)
+{
+    const double *m[9] = {b.csi0, b.csi1, b.csi2, b.eta0, b.eta1, b.eta2, b.zet0, b.zet1, b.zet2};
+    return launch_metric_tensor(m, b.n, gt, s);
+}
+
+// the block's metric tensor: the caller's, or computed into the workspace
+static int block_tensor(const Blk &b, Carver &w, hipStream_t s, const double **gt)
+{
+    if (b.gten) { *gt = b.gten; return LSM_OK; }
+    double *t = (double *)w.take(6 * (size_t)b.n * sizeof(double));
+    if (!t) return fail(LSM_ERR_WORKSPACE, "reinit workspace too small (metric tensor)");
+    *gt = t;
+    return launch_metric_tensor(b, t, s);
+}
+
+// workspace of block_tensor
+static size_t tensor_work(const Blk &b) { return b.gten ? 0 : ((6 * (size_t)b.n * sizeof(double) + 255) & ~(size_t)255); }
+
+size_t reinit_eno2_work(const Blk &b) { return 2 * fb(b) + (((size_t)b.n + 255) & ~(size_t)255) + 1024 + tensor_work(b); }
 
 int launch_normgrad(const Blk &b, const double *d, const double *phi0, int order_bc, int eno_bc,
                     double *ng, hipStream_t s)
@@ -536,11 +589,13 @@ struct RTShape {
     unsigned grid;
 };
 
-RTShape rt_shape(const Blk &b, const LsmReinitParams &p, const double *phi0, uint8_t *conv, Ctl *ctl)
+RTShape rt_shape(const Blk &b, const LsmReinitParams &p, const double *phi0, uint8_t *conv, Ctl *ctl,
+                 const double *gt)
 {
     RTShape r{};
     RTArgs &ra = r.a;
     ra.g = make_geo(b); ra.phi0 = phi0; ra.conv = conv; ra.ctl = ctl;
+    ra.gt = gt; ra.gs = b.n;
     ra.tiles_i = (b.e.ni + RTX - 1) / RTX;
     ra.tiles_j = (b.e.nj + RTY - 1) / RTY;
     const int nk_int = b.e.nk - 2;
@@ -602,7 +657,9 @@ int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi
     if ((rc = launch_copy(phi0, out, b.n, s))) return rc;
     if ((rc = init_ctl(ctl, iters, s))) return rc;
     if (p.rn_itermax <= 0) return LSM_OK;
-    RTShape r = rt_shape(b, p, phi0, conv, ctl);
+    const double *gt = nullptr;
+    if ((rc = block_tensor(b, w, s, &gt))) return rc;
+    RTShape r = rt_shape(b, p, phi0, conv, ctl, gt);
     // max/min of |grad phi0| before the first iteration (reinitialisation_test2.F90:96-101)
     r.a.din = phi0; r.a.dn = phi0; r.a.dout = nullptr;
     if ((rc = rt_launch(r, 0, s))) return rc;
@@ -625,7 +682,7 @@ size_t reinit_trace_work(const Blk &b, int itermax)
 {
     const size_t u8 = ((size_t)b.n + 255) & ~(size_t)255;
     const size_t u16 = ((size_t)b.n * sizeof(uint16_t) + 255) & ~(size_t)255;
-    return (size_t)itermax * (u8 + 3 * (4 * fb(b) + u16)) + 2 * fb(b) + 2048;
+    return (size_t)itermax * (u8 + 3 * (4 * fb(b) + u16)) + 2 * fb(b) + 2048 + tensor_work(b);
 }
 
 int carve_reinit_trace(const Blk &b, const LsmReinitParams &p, ReinitTape &t, Carver &w, void **ctl_out,
@@ -668,7 +725,9 @@ int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const doubl
     if ((rc = launch_copy(phi0, out, b.n, s))) return rc;
     if ((rc = init_ctl(ctl, t.iters, s))) return rc;
     if (t.nit == 0) return LSM_OK;
-    RTShape r = rt_shape(b, p, phi0, nullptr, ctl);
+    const double *gt = nullptr;
+    if ((rc = block_tensor(b, w, s, &gt))) return rc;
+    RTShape r = rt_shape(b, p, phi0, nullptr, ctl, gt);
     r.a.din = phi0; r.a.dn = phi0; r.a.dout = nullptr;
     if ((rc = rt_launch(r, 0, s))) return rc;
     const double *sin_[3] = {out, A, B};

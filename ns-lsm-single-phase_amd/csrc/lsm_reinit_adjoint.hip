@@ -29,6 +29,8 @@
 // Deterministic (no atomics).
 #include <stdlib.h>
 
+#include <algorithm>
+
 #include "lsm_internal.hpp"
 #include "lsm_reinit_common.hpp"
 #include "lsm_multi.hpp"
@@ -100,15 +102,15 @@ struct KTGather {
 };
 
 template <int GY>
-__global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherArgs a, int tiles_i, int tiles_j, int kc)
+__global__ __launch_bounds__(GX * GY) void radj_tape_gather(TGatherLaunch x)
 {
-    radj_tape_gather_run<GY>(a, tiles_i, tiles_j, kc, (int)blockIdx.x);
+    KTGather<GY>::run(x, (int)blockIdx.x, (int)gridDim.x);
 }
 
 template <int GY>
 void KTGather<GY>::one(unsigned g, dim3 b, hipStream_t s, const Args &x)
 {
-    radj_tape_gather<GY><<<g, b, 0, s>>>(x.a, x.tiles_i, x.tiles_j, x.kc);
+    radj_tape_gather<GY><<<g, b, 0, s>>>(x);
 }
 
 template <int GY>
@@ -330,7 +332,11 @@ int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const dou
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
     // tile rows (8 or 16) and planes per tile of the gather
-    int gy = 8, kc = 8;
+    // (planes per tile: up to 32 while the grid keeps about four workgroups
+    // per CU; the k window's 7 halo planes are then read once per 32)
+    int gy = 8;
+    int kc = (int)std::min<long>(32, std::max<long>(8, (long)b.e.nk * ((b.e.ni + GX - 1) / GX) *
+                                                            ((b.e.nj + 7) / 8) / 1024));
     if (const char *e = getenv("LSM_RADJ_GY")) gy = atoi(e) == 16 ? 16 : 8;
     if (const char *e = getenv("LSM_RADJ_KC")) { const int v = atoi(e); if (v > 0 && v <= 64) kc = v; }
     // the first stage reads adj_out itself; then the two buffers alternate

@@ -34,6 +34,31 @@ __device__ __forceinline__ void node_of(const Geo &g, long l, int &i, int &j, in
     k = (int)(r / g.nj);
 }
 
+// The metric tensor G_ab = grad xi_a . grad xi_b of one node from its nine
+// metrics m (csi, eta, zet: the gradients of xi, eta, zeta), stored in the
+// order G_xx, G_xe, G_xz, G_ee, G_ez, G_zz.  |grad d|^2 = g^T G g for the
+// computational derivatives g: 6 fields per node instead of 9.
+__device__ __forceinline__ void metric_tensor(const double m[9], double G[6])
+{
+    G[0] = (m[0] * m[0] + m[1] * m[1]) + m[2] * m[2];
+    G[1] = (m[0] * m[3] + m[1] * m[4]) + m[2] * m[5];
+    G[2] = (m[0] * m[6] + m[1] * m[7]) + m[2] * m[8];
+    G[3] = (m[3] * m[3] + m[4] * m[4]) + m[5] * m[5];
+    G[4] = (m[3] * m[6] + m[4] * m[7]) + m[5] * m[8];
+    G[5] = (m[6] * m[6] + m[7] * m[7]) + m[8] * m[8];
+}
+
+// |grad d| from the tensor and the computational derivatives g0, g1, g2;
+// t = G g (t_a / |grad d| = d|grad d| / d g_a).  The quadratic form of a
+// positive definite G is clamped at 0 against rounding.
+__device__ __forceinline__ double tensor_norm(const double G[6], double g0, double g1, double g2, double t[3])
+{
+    t[0] = (G[0] * g0 + G[1] * g1) + G[2] * g2;
+    t[1] = (G[1] * g0 + G[3] * g1) + G[4] * g2;
+    t[2] = (G[2] * g0 + G[4] * g1) + G[5] * g2;
+    return sqrt(fmax((g0 * t[0] + g1 * t[1]) + g2 * t[2], 0.0));
+}
+
 // Derivatives of d along xi, eta, zeta at node (i,j,k) (not a vertex of a
 // six-face block) with the stencils of calc_NormGrad_d.F90:39-2446: ENO2 with
 // upwind sign p0 where the direction is not normal to a physical face, the

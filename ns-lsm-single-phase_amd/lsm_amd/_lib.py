@@ -42,7 +42,7 @@ class LsmBlock(C.Structure):
                 ("big_phi", C.c_double),
                 ("csi", _vp * 3), ("eta", _vp * 3), ("zet", _vp * 3),
                 ("aj", _vp), ("x", _vp), ("y", _vp), ("z", _vp),
-                ("nblk", C.c_int32), ("blank", LsmBlank * LSM_MAX_BLANK)]
+                ("nblk", C.c_int32), ("blank", LsmBlank * LSM_MAX_BLANK), ("gten", _vp)]
 
 
 class LsmReinitParams(C.Structure):
@@ -121,6 +121,7 @@ _SIGS = {
     "lsm_calc_h": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp]),
     "lsm_free_surface": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp]),
     "lsm_free_surface_adjoint": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, _vp]),
+    "lsm_metric_tensor": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp]),
     "lsm_reinit_sussman_eno2": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp,
                                           _vp, _vp, C.c_size_t, _vp]),
     "lsm_sign_ini": (C.c_int, [C.POINTER(LsmBlock), _vp, C.c_double, C.c_int, _vp, _vp, _vp]),

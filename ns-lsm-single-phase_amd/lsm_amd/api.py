@@ -133,6 +133,11 @@ class Block:
         self.blanking = tuple(blanking)
         self.desc = s
         self.device = self.aj.device
+        # the reinitialisation's metric tensor, computed once for the block's grid
+        self.gten = torch.empty(6 * ni * nj * nk, dtype=torch.float64, device=device)
+        s.gten = self.gten.data_ptr()
+        check(_lib.load().lsm_metric_tensor(self.ref, C.c_void_p(self.gten.data_ptr()), _stream()),
+              "lsm_metric_tensor")
 
     @property
     def ref(self):
