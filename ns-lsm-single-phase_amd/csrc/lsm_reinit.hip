@@ -199,22 +199,26 @@ __device__ __forceinline__ void reduce_maxmin_t(double vmax, double vmin, int ti
 }
 
 // the tape entry of one node: w are the direction weights already divided by
-// |grad d| and scaled by the spacing, br the branch codes
+// |grad d| and scaled by the spacing, br the branch codes.  The weights are
+// written at every node (zero where the node's update does not depend on its
+// stencil), so every wavefront stores whole 128-byte lines.
 __device__ __forceinline__ void tape_write(const RTArgs &a, long l, bool upd, double ngv, double p0, double s,
                                            double del, const double w[3], int br)
 {
     uint16_t code = 0;
+    double q0 = 0.0, q1 = 0.0, q2 = 0.0;
     if (upd) {
         if (fabs((del * del) * ngv) < kEpsSims) {   // lambda = 0: rhs = s (|grad d| - 1)
             const double c3dt = a.p.c3 * a.p.dt;
             if (fabs(p0) <= a.p.epslsm) { a.tT[l] = -c3dt * (ngv - 1.0); code |= rn::kHasT; }
             const double f = -c3dt * s;
             if (f != 0.0 && ngv > 0.0) {
-                a.tq[0][l] = f * w[0]; a.tq[1][l] = f * w[1]; a.tq[2][l] = f * w[2];
+                q0 = f * w[0]; q1 = f * w[1]; q2 = f * w[2];
                 code |= (uint16_t)(rn::kLive | br);
             }
         }
     }
+    a.tq[0][l] = q0; a.tq[1][l] = q1; a.tq[2][l] = q2;
     a.tcode[l] = code;
 }
 

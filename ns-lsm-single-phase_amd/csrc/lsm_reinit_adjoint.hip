@@ -88,15 +88,16 @@ struct TGatherLaunch {
 };
 
 template <int GY>
-__device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int tiles_i, int tiles_j, int kc, int bid);
+__device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int tiles_i, int tiles_j, int kc, int bid,
+                                                     int nb);
 
 template <int GY>
 struct KTGather {
     using Args = TGatherLaunch;
     static constexpr int kThreads = GX * GY;
-    __device__ __forceinline__ static void run(const Args &x, int bid, int)
+    __device__ __forceinline__ static void run(const Args &x, int bid, int nb)
     {
-        radj_tape_gather_run<GY>(x.a, x.tiles_i, x.tiles_j, x.kc, bid);
+        radj_tape_gather_run<GY>(x.a, x.tiles_i, x.tiles_j, x.kc, bid, nb);
     }
     static void one(unsigned g, dim3 b, hipStream_t s, const Args &x);
 };
@@ -114,144 +115,212 @@ void KTGather<GY>::one(unsigned g, dim3 b, hipStream_t s, const Args &x)
 }
 
 template <int GY>
-__device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int tiles_i, int tiles_j, int kc, int bid)
+__device__ __forceinline__ void radj_tape_gather_run(const TGatherArgs &a, int tiles_i, int tiles_j, int kc, int bid,
+                                                     int nb)
 {
-    __shared__ double sW0[GY][GX + 2 * GH];
-    __shared__ double sW1[GY + 2 * GH][GX];
-    __shared__ uint16_t sC[GY + 2 * GH][GX + 2 * GH];
-    __shared__ double s_eno[16][5];   // interior coefficients of every ENO2 branch code
-    // k window of each thread's own column, planes k-3 .. k+3 in a ring of 7
-    // (slot kk mod 7; only the thread itself reads its entries, and plane k+4
-    // replaces plane k-3 after the plane's sums: no barrier needed)
-    __shared__ double sK[7][GY][GX];
-    __shared__ uint16_t sKc[7][GY][GX];
-    auto rs = [](int kk) { return (kk + 70) % 7; };
-    constexpr int NT = GX * GY;
-    constexpr int N0 = GY * (GX + 2 * GH), N1 = (GY + 2 * GH) * GX, NC = (GY + 2 * GH) * (GX + 2 * GH);
-    constexpr int R0 = (N0 + NT - 1) / NT, R1 = (N1 + NT - 1) / NT, RC = (NC + NT - 1) / NT;
+    // j halo cells (2 GH rows x GX columns) and i halo cells (GY rows x 2 GH
+    // columns) are loaded by one thread each
+    constexpr int NT = GX * GY, NJH = 2 * GH * GX, NIH = 2 * GH * GY;
+    static_assert(NJH + NIH <= NT, "halo cells exceed the workgroup");
+    __shared__ double sW0[2][GY][GX + 2 * GH];     // lam q_0 of the plane, i halo
+    __shared__ double sW1[2][GY + 2 * GH][GX];     // lam q_1 of the plane, j halo
+    __shared__ uint16_t sCi[2][GY][GX + 2 * GH];   // branch codes, i halo
+    __shared__ uint16_t sCj[2][GY + 2 * GH][GX];   // branch codes, j halo
+    __shared__ double s_eno[16][5];                // interior coefficients of every ENO2 branch code
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * GX + tx;
     if (tid < 16) {
         double e[5];
         eno2_coef(tid, e);
         for (int q = 0; q < 5; ++q) s_eno[tid][q] = e[q];
     }
-    const int b = bid;
+    // consecutive tiles on one XCD (workgroup bid runs on XCD bid % 8), so the
+    // halos a tile shares with its neighbours are read from one L2
+    const int ntile = tiles_i * tiles_j * ((a.nk + kc - 1) / kc);
+    const int b = (nb & 7) == 0 ? (bid & 7) * (nb >> 3) + (bid >> 3) : bid;
+    if (b >= ntile) return;
     const int ti = b % tiles_i, t2 = b / tiles_i;
     const int tj = t2 % tiles_j, tk = t2 / tiles_j;
     const int i0 = ti * GX, j0 = tj * GY;
     const int i = i0 + tx, j = j0 + ty;
     const int kb = tk * kc, ke = min(kb + kc, a.nk) - 1;
     const bool inb = i < a.ni && j < a.nj;
-    const int ic = min(i, a.ni - 1), jc = min(j, a.nj - 1);
-    const long col = (long)ic + a.sj * jc;
-    const bool active = a.it < *a.iters;
-    const bool j_on = j == 0 || j == a.nj - 1;
-    auto kcl = [&](int kk) { return min(max(kk, 0), a.nk - 1); };
-    // in-plane offsets of this thread's halo-fill entries (the same on every plane)
-    int o0[R0], o1[R1], oc[RC];
-#pragma unroll
-    for (int r = 0; r < R0; ++r) {
-        const int c = min(tid + r * NT, N0 - 1), rr = c / (GX + 2 * GH), cc = c % (GX + 2 * GH);
-        o0[r] = min(max(i0 - GH + cc, 0), a.ni - 1) + (int)a.sj * min(j0 + rr, a.nj - 1);
+    const long col = (long)min(i, a.ni - 1) + a.sj * min(j, a.nj - 1);
+    auto kcl = [&](int kk) { return a.sk * (long)min(max(kk, 0), a.nk - 1); };
+
+    if (!(a.it < *a.iters)) {
+        // an iteration the forward did not run: the adjoint passes through
+        if (!inb) return;
+        for (int k = kb; k <= ke; ++k) {
+            const long m = col + a.sk * (long)k;
+            const double lam = a.lam[m];
+            const double lin = 0.0;
+            const double adn0 = a.stage != 3 ? a.adn[m] : 0.0;
+            if (a.stage == 3) { a.adn[m] = lin; a.out[m] = lam; }
+            else if (a.stage == 2) { a.adn[m] = adn0 + lin; a.out[m] = lam; }
+            else a.out[m] = lam + lin + adn0;
+        }
+        return;
     }
-#pragma unroll
-    for (int r = 0; r < R1; ++r) {
-        const int c = min(tid + r * NT, N1 - 1), rr = c / GX, cc = c % GX;
-        o1[r] = min(i0 + cc, a.ni - 1) + (int)a.sj * min(max(j0 - GH + rr, 0), a.nj - 1);
+
+    // this thread's halo cell: hkind 1 j halo, 2 i halo, 0 none
+    int hkind = 0, hoff = 0, hrow = 0, hcol = 0;
+    if (tid < NJH) {
+        const int r = tid / GX, c = tid % GX;
+        const int jj = r < GH ? j0 - GH + r : j0 + GY + (r - GH);
+        hkind = 1; hrow = r < GH ? r : GY + r; hcol = c;
+        hoff = min(i0 + c, a.ni - 1) + (int)a.sj * min(max(jj, 0), a.nj - 1);
+    } else if (tid < NJH + NIH) {
+        const int t = tid - NJH, r = t / (2 * GH), c = t % (2 * GH);
+        const int ii = c < GH ? i0 - GH + c : i0 + GX + (c - GH);
+        hkind = 2; hrow = r; hcol = c < GH ? c : GX + c;
+        hoff = min(max(ii, 0), a.ni - 1) + (int)a.sj * min(j0 + r, a.nj - 1);
     }
-#pragma unroll
-    for (int r = 0; r < RC; ++r) {
-        const int c = min(tid + r * NT, NC - 1), rr = c / (GX + 2 * GH), cc = c % (GX + 2 * GH);
-        oc[r] = min(max(i0 - GH + cc, 0), a.ni - 1) + (int)a.sj * min(max(j0 - GH + rr, 0), a.nj - 1);
-    }
-    // k window: planes k-3 .. k+3 of this column (clamped; entries outside the
-    // block are never used)
-    for (int q = 0; q < 7; ++q) {
-        const int kk = kb - 3 + q;
-        const long l = col + a.sk * (long)kcl(kk);
-        sK[rs(kk)][ty][tx] = a.lam[l] * a.q[2][l];
-        sKc[rs(kk)][ty][tx] = a.code[l];
-    }
-    // the halo planes of the next plane are loaded into registers while the
-    // current plane's sums run, and stored to LDS after the plane's barrier
-    double l0[R0], q0[R0], l1[R1], q1[R1];
-    uint16_t cv[RC];
-    auto fetch = [&](int k) {
-        const double *lk = a.lam + a.sk * (long)k, *q0k = a.q[0] + a.sk * (long)k, *q1k = a.q[1] + a.sk * (long)k;
-        const uint16_t *ck = a.code + a.sk * (long)k;
-#pragma unroll
-        for (int r = 0; r < R0; ++r) { l0[r] = lk[o0[r]]; q0[r] = q0k[o0[r]]; }
-#pragma unroll
-        for (int r = 0; r < R1; ++r) { l1[r] = lk[o1[r]]; q1[r] = q1k[o1[r]]; }
-#pragma unroll
-        for (int r = 0; r < RC; ++r) cv[r] = ck[oc[r]];
+    const double *__restrict__ hq = hkind == 2 ? a.q[0] : a.q[1];
+
+    // operands of plane t, loaded one plane ahead: the column's lam, q, code
+    // (q_0, q_1 and the halo cell only on the tile's own planes), and the
+    // output-time operands (conv, adn) of plane t - 3
+    double nl = 0.0, nq0 = 0.0, nq1 = 0.0, nq2 = 0.0, nhl = 0.0, nhq = 0.0, nadn = 0.0;
+    int nc = 0, nhc = 0, ncv = 0;
+    auto fetch = [&](int t) {
+        const long l = col + kcl(t);
+        nl = a.lam[l]; nq2 = a.q[2][l]; nc = a.code[l];
+        if (t >= kb && t <= ke) {
+            nq0 = a.q[0][l]; nq1 = a.q[1][l];
+            if (hkind) {
+                const long h = hoff + a.sk * (long)t;
+                nhl = a.lam[h]; nhq = hq[h]; nhc = a.code[h];
+            }
+        }
+        const int m = t - 3;
+        if (m >= kb && m <= ke) {
+            const long lm = col + a.sk * (long)m;
+            ncv = __builtin_nontemporal_load(a.conv + lm);
+            if (a.stage != 3) nadn = __builtin_nontemporal_load(a.adn + lm);
+        }
     };
-    fetch(kb);
-    for (int k = kb; k <= ke; ++k) {
-        const long pk = a.sk * (long)k;
-        double *w0 = &sW0[0][0], *w1 = &sW1[0][0];
-        uint16_t *wc = &sC[0][0];
+
+    // rings over the planes: lam q_2 and codes of planes t-6 .. t, lam and the
+    // i/j sums of planes t-3 .. t
+    double kq[7], lr[4], sr[4];
+    int kcd[7];
 #pragma unroll
-        for (int r = 0; r < R0; ++r)
-            if (tid + r * NT < N0) w0[tid + r * NT] = l0[r] * q0[r];
+    for (int q = 0; q < 7; ++q) { kq[q] = 0.0; kcd[q] = 0; }
 #pragma unroll
-        for (int r = 0; r < R1; ++r)
-            if (tid + r * NT < N1) w1[tid + r * NT] = l1[r] * q1[r];
+    for (int q = 0; q < 4; ++q) { lr[q] = 0.0; sr[q] = 0.0; }
+
+    const bool j_on = j == 0 || j == a.nj - 1;
+    // waves whose row or lanes reach within 3 nodes of a face take the face terms
+    const bool near_i = i0 <= 3 || i0 + GX - 1 >= a.ni - 4;
+    const bool near_j = j <= 3 || j >= a.nj - 4;
+    fetch(kb - 3);
+    for (int t = kb - 3; t <= ke + 3; ++t) {
+        const double lam = nl, q0 = nq0, q1 = nq1, q2 = nq2, hl = nhl, hqv = nhq;
+        const int code = nc, hc = nhc, cv = ncv;
+        const double adn0 = nadn;
+        if (t < ke + 3) fetch(t + 1);
+        // shift the rings
 #pragma unroll
-        for (int r = 0; r < RC; ++r)
-            if (tid + r * NT < NC) wc[tid + r * NT] = cv[r];
-        __syncthreads();
-        const long m = col + pk;
-        const double lam = a.lam[m];
-        const uint8_t cnv = a.conv[m];
-        double adn0 = 0.0;
-        if (a.stage != 3) adn0 = a.adn[m];
-        // the next plane of the k window and of the halos
-        const long lk = col + a.sk * (long)kcl(k + 4);
-        const double nlk = a.lam[lk], nqk = a.q[2][lk];
-        const uint16_t nck = a.code[lk];
-        if (k < ke) fetch(k + 1);
-        if (inb) {
-            const bool upd = active && cnv;
-            double acc = upd ? a.c2 * lam : lam;
-            if (active) {
-                if (sC[ty + GH][tx + GH] & kHasT) a.adj_s[m] += lam * a.T[m];
-                const int idx[3] = {i, j, k};
-                const int nn[3] = {a.ni, a.nj, a.nk};
+        for (int q = 0; q < 6; ++q) { kq[q] = kq[q + 1]; kcd[q] = kcd[q + 1]; }
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
+        for (int q = 0; q < 3; ++q) { lr[q] = lr[q + 1]; sr[q] = sr[q + 1]; }
+        kq[6] = lam * q2; kcd[6] = code; lr[3] = lam; sr[3] = 0.0;
+        if (t >= kb && t <= ke) {
+            const int buf = t & 1;
+            sW0[buf][ty][tx + GH] = lam * q0;
+            sCi[buf][ty][tx + GH] = (uint16_t)code;
+            sW1[buf][ty + GH][tx] = lam * q1;
+            sCj[buf][ty + GH][tx] = (uint16_t)code;
+            if (hkind == 1) { sW1[buf][hrow][hcol] = hl * hqv; sCj[buf][hrow][hcol] = (uint16_t)hc; }
+            else if (hkind == 2) { sW0[buf][hrow][hcol] = hl * hqv; sCi[buf][hrow][hcol] = (uint16_t)hc; }
+            __syncthreads();
+            // i and j terms of this plane: the nodes whose stencil holds (i, j, t)
+            double s = 0.0;
 #pragma unroll
-                    for (int o = -3; o <= 3; ++o) {
-                        const int ia = idx[d] - o;   // index of the node whose offset-o entry is this node
-                        if (ia < 0 || ia > nn[d] - 1) continue;
-                        int slot, kind;              // kind: 0 inside, 1 low face, 2 high face
-                        if (ia == 0) { slot = o; kind = 1; }
-                        else if (ia == nn[d] - 1) { slot = -o; kind = 2; }
-                        else { slot = o + 2; kind = 0; }
-                        if (slot < 0 || slot > (kind == 0 ? 4 : 3)) continue;
-                        int c;
-                        double w;
-                        if (d == 0) { c = sC[ty + GH][tx + GH - o]; w = sW0[ty][tx + GH - o]; }
-                        else if (d == 1) { c = sC[ty + GH - o][tx + GH]; w = sW1[ty + GH - o][tx]; }
-                        else { c = sKc[rs(k - o)][ty][tx]; w = sK[rs(k - o)][ty][tx]; }
-                        if (!(c & kLive)) continue;
-                        const int br = (c >> (4 * d)) & 15;
-                        const double e = kind == 0 ? s_eno[br][slot]
-                                                   : face_coef(br, a.order_bc, kind == 1 ? 1 : -1, a.eno_bc != 0, slot,
-                                                               d == 0 && ia == 0 && k == 0 && !j_on);   // edge i = 1, k = 1
-                        acc += w * e;
-                    }
+            for (int o = -2; o <= 2; ++o) {
+                const int ia = i - o;
+                const int c = sCi[buf][ty][tx + GH - o];
+                const double e = s_eno[c & 15][o + 2];
+                const bool ok = (c & kLive) && ia >= 1 && ia <= a.ni - 2;
+                s += ok ? sW0[buf][ty][tx + GH - o] * e : 0.0;
+            }
+            if (near_i) {
+                // one-sided differences of the face nodes i = 0 (slots 0..3 at
+                // i = 0..3) and i = ni-1 (slots 0..3 at i = ni-1 .. ni-4)
+                if (i <= 3) {
+                    const int c = sCi[buf][ty][GH - i0];
+                    if (c & kLive)
+                        s += sW0[buf][ty][GH - i0] *
+                             face_coef(c & 15, a.order_bc, 1, a.eno_bc != 0, i, t == 0 && !j_on);
+                }
+                if (i >= a.ni - 4 && i <= a.ni - 1) {
+                    const int c = sCi[buf][ty][GH + a.ni - 1 - i0];
+                    if (c & kLive)
+                        s += sW0[buf][ty][GH + a.ni - 1 - i0] *
+                             face_coef(c & 15, a.order_bc, -1, a.eno_bc != 0, a.ni - 1 - i, false);
                 }
             }
-            const double lin = upd ? a.c1 * lam : 0.0;
-            if (a.stage == 3) { a.adn[m] = lin; a.out[m] = acc; }
-            else if (a.stage == 2) { a.adn[m] = adn0 + lin; a.out[m] = acc; }
-            else a.out[m] = acc + lin + adn0;
+#pragma unroll
+            for (int o = -2; o <= 2; ++o) {
+                const int ja = j - o;
+                const int c = sCj[buf][ty + GH - o][tx];
+                const double e = s_eno[(c >> 4) & 15][o + 2];
+                const bool ok = (c & kLive) && ja >= 1 && ja <= a.nj - 2;
+                s += ok ? sW1[buf][ty + GH - o][tx] * e : 0.0;
+            }
+            if (near_j) {
+                if (j <= 3) {
+                    const int c = sCj[buf][GH - j0][tx];
+                    if (c & kLive)
+                        s += sW1[buf][GH - j0][tx] * face_coef((c >> 4) & 15, a.order_bc, 1, a.eno_bc != 0, j, false);
+                }
+                if (j >= a.nj - 4) {
+                    const int c = sCj[buf][GH + a.nj - 1 - j0][tx];
+                    if (c & kLive)
+                        s += sW1[buf][GH + a.nj - 1 - j0][tx] *
+                             face_coef((c >> 4) & 15, a.order_bc, -1, a.eno_bc != 0, a.nj - 1 - j, false);
+                }
+            }
+            sr[3] = s;
         }
-        // slide the k window: plane k + 4 replaces plane k - 3
-        sK[rs(k + 4)][ty][tx] = nlk * nqk;
-        sKc[rs(k + 4)][ty][tx] = nck;
-        __syncthreads();
+        // plane m = t - 3 has all its k neighbours (planes m-3 .. m+3) in the ring
+        const int m = t - 3;
+        if (m >= kb && m <= ke && inb) {
+            double s = 0.0;
+#pragma unroll
+            for (int o = -2; o <= 2; ++o) {
+                const int ka = m - o;
+                const int c = kcd[3 - o];
+                const double e = s_eno[(c >> 8) & 15][o + 2];
+                const bool ok = (c & kLive) && ka >= 1 && ka <= a.nk - 2;
+                s += ok ? kq[3 - o] * e : 0.0;
+            }
+            // the face planes 0 (ring slot 3 - m) and nk-1 (ring slot 3 + nk-1-m),
+            // picked by unrolled selects so the rings stay in registers
+            if (m <= 3 || m >= a.nk - 4) {
+                double wlo = 0.0, whi = 0.0;
+                int clo = 0, chi = 0;
+#pragma unroll
+                for (int r = 0; r < 7; ++r) {
+                    if (r == 3 - m) { wlo = kq[r]; clo = kcd[r]; }
+                    if (r == 3 + (a.nk - 1 - m)) { whi = kq[r]; chi = kcd[r]; }
+                }
+                if (m <= 3 && (clo & kLive))
+                    s += wlo * face_coef((clo >> 8) & 15, a.order_bc, 1, a.eno_bc != 0, m, false);
+                if (m >= a.nk - 4 && (chi & kLive))
+                    s += whi * face_coef((chi >> 8) & 15, a.order_bc, -1, a.eno_bc != 0, a.nk - 1 - m, false);
+            }
+            const long lm = col + a.sk * (long)m;
+            const double lm_ = lr[0];
+            const bool upd = cv != 0;
+            double acc = upd ? a.c2 * lm_ : lm_;
+            acc += sr[0] + s;
+            if (kcd[3] & kHasT) a.adj_s[lm] += lm_ * a.T[lm];
+            const double lin = upd ? a.c1 * lm_ : 0.0;
+            if (a.stage == 3) { a.adn[lm] = lin; a.out[lm] = acc; }
+            else if (a.stage == 2) { a.adn[lm] = adn0 + lin; a.out[lm] = acc; }
+            else a.out[lm] = acc + lin + adn0;
+        }
     }
 }
 
@@ -353,7 +422,7 @@ int launch_reinit_eno2_adjoint(const Blk &b, const LsmReinitParams &p, const dou
                            adj_s};
             const int tiles_i = (b.e.ni + GX - 1) / GX, tiles_j = (b.e.nj + gy - 1) / gy;
             const int tiles_k = (b.e.nk + kc - 1) / kc;
-            const unsigned grid = (unsigned)(tiles_i * tiles_j * tiles_k);
+            const unsigned grid = (unsigned)((tiles_i * tiles_j * tiles_k + 7) / 8 * 8);
             const TGatherLaunch gl{ga, tiles_i, tiles_j, kc};
             if (gy == 16) rc = klaunch<KTGather<16>>(grid, dim3(GX, 16, 1), s, gl, "radj_tape_gather");
             else rc = klaunch<KTGather<8>>(grid, dim3(GX, 8, 1), s, gl, "radj_tape_gather");
