@@ -218,8 +218,11 @@ __device__ __forceinline__ void tape_write(const RTArgs &a, long l, bool upd, do
             }
         }
     }
-    a.tq[0][l] = q0; a.tq[1][l] = q1; a.tq[2][l] = q2;
-    a.tcode[l] = code;
+    // streamed: the tape is read back only by the reverse sweep
+    __builtin_nontemporal_store(q0, a.tq[0] + l);
+    __builtin_nontemporal_store(q1, a.tq[1] + l);
+    __builtin_nontemporal_store(q2, a.tq[2] + l);
+    __builtin_nontemporal_store(code, a.tcode + l);
 }
 
 // STAGE 0: only the max/min of |grad din| (reinitialisation_test2.F90:96-101)
@@ -380,7 +383,11 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int b
                                    (eno2_branch(p0, jlo ? 0.0 : pl[cy - 2][cx], pl[cy - 1][cx], q2, pl[cy + 1][cx],
                                                 jhi ? 0.0 : pl[cy + 2][cx], jlo, jhi) << 4) |
                                    (eno2_branch(p0, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi) << 8);
-                    const double inv = ngv > 0.0 ? 1.0 / ngv : 0.0;
+                    // 1 / |grad d|: hardware reciprocal and two Newton steps
+                    double inv = __builtin_amdgcn_rcp(ngv);
+                    inv = fma(inv, fma(-ngv, inv, 1.0), inv);
+                    inv = fma(inv, fma(-ngv, inv, 1.0), inv);
+                    inv = ngv > 0.0 ? inv : 0.0;
                     double tw[3];
 #pragma unroll
                     for (int d = 0; d < 3; ++d) tw[d] = t[d] * inv * g.dd[d];
@@ -604,7 +611,9 @@ RTShape rt_shape(const Blk &b, const LsmReinitParams &p, const double *phi0, uin
     ra.tiles_j = (b.e.nj + RTY - 1) / RTY;
     const int nk_int = b.e.nk - 2;
     const int tij = ra.tiles_i * ra.tiles_j;
-    int kc = (nk_int + (8192 + tij - 1) / tij - 1) / ((8192 + tij - 1) / tij);
+    // about 4096 tiles (16 planes per tile at 256^3: the k halo of a tile is
+    // 4 planes of d per chunk)
+    int kc = (nk_int + (4096 + tij - 1) / tij - 1) / ((4096 + tij - 1) / tij);
     if (kc < 8) kc = 8;
     if (const char *e = getenv("LSM_REINIT_KCHUNK")) { int v = atoi(e); if (v > 0) kc = v; }
     ra.kc = kc;
