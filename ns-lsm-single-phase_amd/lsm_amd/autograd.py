@@ -184,19 +184,32 @@ def levelset_step(blk, phi_n, ucn: Sequence[torch.Tensor], dt: float, params: "a
 
 class _StepTrajectory(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, phi0, U1, U2, U3, blk, dt, params, nsteps, mode, every):
+    def forward(ctx, phi0, U1, U2, U3, blk, dt, params, nsteps, mode, every, record):
         U = tuple(t.contiguous() for t in (U1, U2, U3))
         ckpts = []
         cur = phi0.contiguous().clone()
         wrk = blk.workspace(api._lib.LSM_OP_RK3)
         wrn = blk.workspace(api._lib.LSM_OP_REINIT_ENO2)
         tmp = blk.empty()
+        # the steps of the last segment are recorded now (same results as the
+        # plain steps), so the backward pass need not replay them
+        last0 = ((nsteps - 1) // every) * every if record else nsteps
+        rec_states, rec_traces = [], []
         for s in range(nsteps):
             if s % every == 0:
                 ckpts.append(cur)
-            cur = _step_forward(blk, cur, U, dt, params, mode, wrk, wrn, tmp).clone()
+            if s >= last0:
+                tr = blk.adjoint_workspace(api._lib.LSM_OP_STEP_TRACE, params.RNitermax)
+                nxt = blk.empty()
+                api.levelset_step_record(blk, cur, U, dt, params, mode=mode, phi_np1=nxt, trace=tr)
+                rec_states.append(cur)
+                rec_traces.append(tr)
+                cur = nxt
+            else:
+                cur = _step_forward(blk, cur, U, dt, params, mode, wrk, wrn, tmp).clone()
         ctx.save_for_backward(*U, *ckpts)
         ctx.blk, ctx.dt, ctx.params, ctx.nsteps, ctx.mode, ctx.every = blk, dt, params, nsteps, mode, every
+        ctx.rec = (rec_states, rec_traces)
         return cur
 
     @staticmethod
@@ -209,26 +222,35 @@ class _StepTrajectory(torch.autograd.Function):
         wad = blk.adjoint_workspace(api._lib.LSM_OP_STEP_ADJOINT_RECORDED, params.RNitermax)
         lam = g.contiguous().clone()
         nxt = blk.empty()
-        traces = []
+        rec_states, traces = ctx.rec
+        ctx.rec = None
         for sidx in range(len(ckpts) - 1, -1, -1):
             s0 = sidx * every
             s1 = min(s0 + every, nsteps)
-            # replay the segment once, recording every step for its reverse sweep
-            states = [ckpts[sidx]]
-            for q in range(s1 - s0):
-                if len(traces) <= q:
-                    traces.append(blk.adjoint_workspace(api._lib.LSM_OP_STEP_TRACE, params.RNitermax))
-                nxt_state = blk.empty() if q < s1 - s0 - 1 else None
-                api.levelset_step_record(blk, states[-1], U, dt, params, mode=mode, phi_np1=nxt_state,
-                                         trace=traces[q])
-                if nxt_state is not None:
-                    states.append(nxt_state)
+            if sidx == len(ckpts) - 1 and rec_states:
+                # recorded by the forward pass
+                states = rec_states
+            else:
+                # replay the segment once, recording every step for its reverse sweep
+                states = [ckpts[sidx]]
+                for q in range(s1 - s0):
+                    if len(traces) <= q:
+                        traces.append(blk.adjoint_workspace(api._lib.LSM_OP_STEP_TRACE, params.RNitermax))
+                    nxt_state = blk.empty() if q < s1 - s0 - 1 else None
+                    api.levelset_step_record(blk, states[-1], U, dt, params, mode=mode, phi_np1=nxt_state,
+                                             trace=traces[q])
+                    if nxt_state is not None:
+                        states.append(nxt_state)
             for q in range(s1 - s0 - 1, -1, -1):
                 api.levelset_step_adjoint_recorded(blk, states[q], U, dt, params, traces[q], lam, mode=mode,
                                                    adj_ucn=aU, out=nxt, work=wad)
                 lam, nxt = nxt, lam
         gu = aU if want_u else [None, None, None]
-        return (lam, *gu, None, None, None, None, None, None)
+        return (lam, *gu, None, None, None, None, None, None, None)
+
+
+def _wants_grad(*ts) -> bool:
+    return torch.is_grad_enabled() and any(isinstance(t, torch.Tensor) and t.requires_grad for t in ts)
 
 
 def levelset_trajectory(blk, phi0, ucn: Sequence[torch.Tensor], dt: float, params: "api.ReinitParams",
@@ -237,12 +259,13 @@ def levelset_trajectory(blk, phi0, ucn: Sequence[torch.Tensor], dt: float, param
     every k steps: the forward keeps nsteps/k checkpoints; the backward replays
     one segment at a time, recording each of its k steps (lsm_levelset_step_
     record) and then running their reverse sweeps from the records, so each
-    step's forward is computed twice in all (forward pass, replay)."""
+    step's forward is computed twice in all (forward pass, replay), except the
+    last segment's, which the forward pass records when a gradient is wanted."""
     if nsteps < 1:
         raise ValueError("nsteps must be >= 1")
     every = max(1, int(checkpoint_every))
     return _StepTrajectory.apply(phi0, ucn[0], ucn[1], ucn[2], blk, float(dt), params, int(nsteps), mode,
-                                 every)
+                                 every, _wants_grad(phi0, *ucn))
 
 
 class _MultiTrajectory(torch.autograd.Function):
@@ -251,7 +274,7 @@ class _MultiTrajectory(torch.autograd.Function):
     execution (api.multi), i.e. one launch per kernel over all blocks."""
 
     @staticmethod
-    def forward(ctx, blks, dt, params, nsteps, mode, every, *t):
+    def forward(ctx, blks, dt, params, nsteps, mode, every, record, *t):
         n = len(blks)
         phi0, U = t[:n], [tuple(t[n + 3 * b + c].contiguous() for c in range(3)) for b in range(n)]
         cur = [p.contiguous().clone() for p in phi0]
@@ -260,9 +283,25 @@ class _MultiTrajectory(torch.autograd.Function):
         wrk = [blk.workspace(api._lib.LSM_OP_RK3) for blk in blks]
         wrn = [blk.workspace(api._lib.LSM_OP_REINIT_ENO2) for blk in blks]
         ckpts = []
+        # the steps of the last segment are recorded now (same results as the
+        # plain steps), so the backward pass need not replay them
+        last0 = ((nsteps - 1) // every) * every if record else nsteps
+        rec_states, rec_traces = [], []
         for s in range(nsteps):
             if s % every == 0:
                 ckpts.append([c.clone() for c in cur])
+            if s >= last0:
+                trs = [blk.adjoint_workspace(api._lib.LSM_OP_STEP_TRACE, params.RNitermax) for blk in blks]
+                new = [blk.empty() for blk in blks]
+                with api.multi() as m:
+                    for b, blk in enumerate(blks):
+                        m.instance()
+                        api.levelset_step_record(blk, cur[b], U[b], dt, params, mode=mode, phi_np1=new[b],
+                                                 trace=trs[b])
+                rec_states.append(cur)
+                rec_traces.append(trs)
+                cur = new
+                continue
             with api.multi() as m:
                 for b, blk in enumerate(blks):
                     m.instance()
@@ -274,6 +313,7 @@ class _MultiTrajectory(torch.autograd.Function):
         ctx.save_for_backward(*flat_u, *[c for seg in ckpts for c in seg])
         ctx.blks, ctx.dt, ctx.params, ctx.nsteps, ctx.mode, ctx.every = blks, dt, params, nsteps, mode, every
         ctx.nseg = len(ckpts)
+        ctx.rec = (rec_states, rec_traces)
         return tuple(cur)
 
     @staticmethod
@@ -284,18 +324,22 @@ class _MultiTrajectory(torch.autograd.Function):
         U = [tuple(saved[3 * b + c] for c in range(3)) for b in range(n)]
         ck = saved[3 * n:]
         ckpts = [ck[q * n:(q + 1) * n] for q in range(ctx.nseg)]
-        want_u = any(ctx.needs_input_grad[6 + n:])
+        want_u = any(ctx.needs_input_grad[7 + n:])
         aU = [[torch.zeros_like(U[b][0]) for _ in range(3)] for b in range(n)] if want_u else [None] * n
         wad = [blk.adjoint_workspace(api._lib.LSM_OP_STEP_ADJOINT_RECORDED, params.RNitermax) for blk in blks]
         lam = [(gb if gb is not None else torch.zeros_like(U[b][0])).contiguous().clone() for b, gb in enumerate(g)]
         nxt = [blk.empty() for blk in blks]
-        traces = []
+        rec_states, traces = ctx.rec
+        ctx.rec = None
         for sidx in range(ctx.nseg - 1, -1, -1):
             s0 = sidx * every
             s1 = min(s0 + every, nsteps)
             # replay the segment once for all blocks, recording every step
+            # (the last segment was recorded by the forward pass)
             states = [list(ckpts[sidx])]
-            for q in range(s1 - s0):
+            if sidx == ctx.nseg - 1 and rec_states:
+                states = rec_states
+            for q in range(s1 - s0 if states is not rec_states else 0):
                 if len(traces) <= q:
                     traces.append([blk.adjoint_workspace(api._lib.LSM_OP_STEP_TRACE, params.RNitermax)
                                    for blk in blks])
@@ -317,7 +361,7 @@ class _MultiTrajectory(torch.autograd.Function):
                                                            work=wad[b])
                 lam, nxt = nxt, lam
         gu = [aU[b][c] if want_u else None for b in range(n) for c in range(3)]
-        return (None, None, None, None, None, None, *lam, *gu)
+        return (None, None, None, None, None, None, None, *lam, *gu)
 
 
 def multi_trajectory(blks, phi0s, ucns, dt: float, params: "api.ReinitParams", nsteps: int,
@@ -330,7 +374,7 @@ def multi_trajectory(blks, phi0s, ucns, dt: float, params: "api.ReinitParams", n
         raise ValueError("nsteps must be >= 1")
     flat = [u for U in ucns for u in U]
     return _MultiTrajectory.apply(list(blks), float(dt), params, int(nsteps), mode,
-                                  max(1, int(checkpoint_every)), *phi0s, *flat)
+                                  max(1, int(checkpoint_every)), _wants_grad(*phi0s, *flat), *phi0s, *flat)
 
 
 def heaviside(phi, eps: float):
