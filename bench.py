@@ -32,7 +32,11 @@ HBM_PEAK_GBS = 8000.0
 ADV_BYTES_PER_CELL_UPDATE = 160.0
 ADV_FOLDED_BYTES_PER_CELL_UPDATE = 136.0
 VOLUME_BYTES_PER_CELL = 32.0
-REINIT_BYTES_PER_CELL_ITER = 304.0
+# Sussman iteration on the metric-tensor data model the kernels use: stage 1
+# reads d, phi0 and the six tensor components and writes d and the update
+# mask (73 B); stages 2-3 also read d^n and the mask (81 B).  (The reference
+# data model, nine metric arrays per node, is 304 B.)
+REINIT_BYTES_PER_CELL_ITER = 235.0
 # algorithmic bytes per node of the reverse-sweep kernels, averaged over the
 # three RK stages of a launch sequence.  Advection adjoint stage: reads the
 # stage input, the adjoint, U/J (3), aj, the accumulated dJ/d(U/J) (3) and
