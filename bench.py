@@ -365,8 +365,13 @@ def bench_batch(steps=10):
     dw_adv, dw_full = db.workspace(), db.workspace(api._lib.LSM_OP_LEVELSET_STEP)
     db.set_config("perblock")
     res["perblock_adv_ms"] = round(_time(lambda: db.levelset_step(dp0, ducn, dt, out=dout, work=dw_adv), steps), 4)
-    res["perblock_full_ms"] = round(_time(lambda: db.levelset_step(dp0, ducn, dt, prm, out=dout, work=dw_full),
-                                          steps), 4)
+    # the per-block form at its best reinitialisation tile width
+    pb = {}
+    for w in (64, 32, 16):
+        db.set_reinit_tile(w)
+        pb[w] = round(_time(lambda: db.levelset_step(dp0, ducn, dt, prm, out=dout, work=dw_full), steps), 4)
+    res["perblock_full_ms_by_reinit_tile"] = pb
+    res["perblock_full_ms"] = min(pb.values())
     del db, ducn, dp0, dout, dw_adv, dw_full
     torch.cuda.empty_cache()
     # the batch: rows padded to 128-byte lines, first owned column aligned
@@ -379,8 +384,12 @@ def bench_batch(steps=10):
     w_adv, w_full = bt.workspace(), bt.workspace(api._lib.LSM_OP_LEVELSET_STEP)
     bt.set_config("perblock")
     res["perblock_aligned_adv_ms"] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, out=out, work=w_adv), steps), 4)
-    res["perblock_aligned_full_ms"] = round(
-        _time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
+    pb = {}
+    for w in (64, 32, 16):
+        bt.set_reinit_tile(w)
+        pb[w] = round(_time(lambda: bt.levelset_step(p0, ucn, dt, prm, out=out, work=w_full), steps), 4)
+    res["perblock_aligned_full_ms"] = min(pb.values())
+    bt.set_reinit_tile(0)
     cfg, ms = bt.autotune(p0, ucn, dt, reps=3, force=True, out=out, work=w_adv)
     res["autotuned"] = cfg
     best_w, times = bt.autotune_reinit(p0, ucn, dt, prm, reps=steps, out=out, work=w_full)
