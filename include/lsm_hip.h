@@ -70,6 +70,7 @@ extern "C" {
 #define LSM_OP_STEP_TRACE     9        /* sized by lsm_adjoint_workspace_bytes */
 #define LSM_OP_STEP_ADJOINT_RECORDED 10 /* sized by lsm_adjoint_workspace_bytes */
 #define LSM_OP_REINIT_GEOMETRIC 11
+#define LSM_OP_LAMBDA_RN_SIMPSON 12
 
 /* One blanking region (global_app.F90:30-43, set up by init.F90:1216-1567;
  * with one block per rank li_blk_* = le_blk_*): a box of nodes given in the
@@ -125,6 +126,8 @@ typedef struct LsmReinitParams {
     int32_t m_ti;                     /* Godunov path: 1 SSP-RK3, else forward Euler */
     int32_t hybrid;                   /* hybrid_reinitialisation: geometric instead of Sussman
                                          in lsm_levelset_step (GlobalMassCorrection = .false.) */
+    int32_t lambda_method;            /* sussman_correction_method (Godunov path): 0 the pointwise
+                                         lambda, 1 the filtered one (lsm_lambda_rn_simpson) */
 } LsmReinitParams;
 
 /* Library version string. */
@@ -325,8 +328,19 @@ int lsm_lambda_rn(const LsmBlock *blk, const double *phi0, const double *normgra
                   const double *d, double timestep, double epslsm, double *lambda,
                   void *stream);
 
+/* calc_lambda_RN.F90:65-120 (calc_lambda_RN, sussman_correction_method = 1):
+ * the numerator deltaf(phi0) (d - phi0) / timestep and the denominator
+ * deltaf(phi0)^2 |grad phi0| of every node filtered by testfilter.F90:1-124
+ * (27-node average, weights 64 / 16 / 4 / 1 divided by aj, at the nodes off the
+ * block's faces), lambda = -num / den, 0 where |den| <= 1E-10.  Workspace:
+ * lsm_workspace_bytes(blk, LSM_OP_LAMBDA_RN_SIMPSON). */
+int lsm_lambda_rn_simpson(const LsmBlock *blk, const double *phi0, const double *normgrad0,
+                          const double *d, double timestep, double epslsm, double *lambda,
+                          void *work, size_t work_bytes, void *stream);
+
 /* reinitialization_benchmark.F90:1-228 (reinitialization_benchmark, M_Grad = 0,
- * no obstacles, non-hybrid): Godunov pseudo-time reinitialisation. */
+ * no obstacles, non-hybrid): Godunov pseudo-time reinitialisation; the lambda
+ * correction by p->lambda_method (calc_lambda_RN's sussman_correction_method). */
 int lsm_reinit_godunov(const LsmBlock *blk, const LsmReinitParams *p,
                        const double *phi0, double *phi_out,
                        void *work, size_t work_bytes, void *stream);
@@ -598,6 +612,9 @@ int lsm_host_rh_rn(const double *d, const double *sign, double *rhs);
 /* calc_lambda_RN.F90:1 */
 int lsm_host_lambda_rn(const double *phi0, const double *normgrad0, const double *d,
                        double timestep, double epslsm, double *lambda);
+/* calc_lambda_RN.F90:1 with sussman_correction_method = 1 */
+int lsm_host_lambda_rn_simpson(const double *phi0, const double *normgrad0, const double *d,
+                               double timestep, double epslsm, double *lambda);
 /* reinitialization_benchmark.F90:1 */
 int lsm_host_reinit_godunov(const double *phi0, double *phi_out, const LsmReinitParams *p);
 /* TotalWaterVolume.F90:1 */

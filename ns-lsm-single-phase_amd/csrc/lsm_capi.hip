@@ -271,6 +271,7 @@ static size_t op_bytes(const Blk &b, int op, int itermax = 0)
     case LSM_OP_STEP_ADJOINT_RECORDED: return step_reverse_bytes(b, itermax);
     case LSM_OP_STEP_ADJOINT: return step_trace_bytes(b, itermax) + step_reverse_bytes(b, itermax);
     case LSM_OP_REINIT_GEOMETRIC: return geometric_work(b);
+    case LSM_OP_LAMBDA_RN_SIMPSON: return lambda_simpson_work(b);
     case LSM_OP_LEVELSET_STEP: {
         // ucn, phi copy, rk3 scratch, AdvectionNodes mask
         size_t a = 3 * field_bytes(b) + field_bytes(b) + field_bytes(b) + (b.n * 4 + 256) + 1024;
@@ -562,6 +563,8 @@ static int check_reinit(const LsmReinitParams *p)
     if (p->order_bc < 1 || p->order_bc > 3) return fail(LSM_ERR_ARG, "order_bc = %d (1, 2 or 3)", p->order_bc);
     if (p->rn_itermax < 0) return fail(LSM_ERR_ARG, "rn_itermax < 0");
     if (!(p->epslsm > 0.0)) return fail(LSM_ERR_ARG, "epslsm must be > 0");
+    if (p->lambda_method != 0 && p->lambda_method != 1)
+        return fail(LSM_ERR_ARG, "lambda_method = %d (0 or 1)", p->lambda_method);
     return LSM_OK;
 }
 
@@ -735,6 +738,22 @@ int lsm_lambda_rn(const LsmBlock *blk, const double *phi0, const double *normgra
     if (rc) return rc;
     if (!phi0 || !normgrad0 || !d || !lambda) return fail(LSM_ERR_ARG, "NULL array");
     return launch_lambda_rn(b, phi0, normgrad0, d, timestep, epslsm, lambda, (hipStream_t)stream);
+}
+
+int lsm_lambda_rn_simpson(const LsmBlock *blk, const double *phi0, const double *normgrad0, const double *d,
+                          double timestep, double epslsm, double *lambda, void *work, size_t work_bytes,
+                          void *stream)
+{
+    Blk b;
+    int rc = make_blk(blk, &b, true, false, kBlankIgnore);
+    if (rc) return rc;
+    if (!phi0 || !normgrad0 || !d || !lambda) return fail(LSM_ERR_ARG, "NULL array");
+    if (lambda == phi0 || lambda == normgrad0 || lambda == d)
+        return fail(LSM_ERR_ARG, "lambda must not alias an input (the filter reads neighbours)");
+    Carver c;
+    rc = workspace(work, work_bytes, op_bytes(b, LSM_OP_LAMBDA_RN_SIMPSON), &c, (hipStream_t)stream);
+    if (rc) return rc;
+    return launch_lambda_rn_simpson(b, phi0, normgrad0, d, timestep, epslsm, lambda, c, (hipStream_t)stream);
 }
 
 int lsm_reinit_godunov(const LsmBlock *blk, const LsmReinitParams *p, const double *phi0,

@@ -558,6 +558,22 @@ int lsm_host_lambda_rn(const double *phi0, const double *ng0, const double *d, d
     return down(lam, g_h.f[3], n);
 }
 
+int lsm_host_lambda_rn_simpson(const double *phi0, const double *ng0, const double *d, double dt, double eps,
+                               double *lam)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = need_ready())) return rc;
+    const long n = g_h.n;
+    if ((rc = up(g_h.f[0], phi0, n)) || (rc = up(g_h.f[1], ng0, n)) || (rc = up(g_h.f[2], d, n)))
+        return rc;
+    if ((rc = lsm_lambda_rn_simpson(&g_h.blk, g_h.f[0], g_h.f[1], g_h.f[2], dt, eps, g_h.f[3], nullptr, 0,
+                                    nullptr)))
+        return rc;
+    if ((rc = sync())) return rc;
+    return down(lam, g_h.f[3], n);
+}
+
 int lsm_host_reinit_godunov(const double *phi0, double *phi_out, const LsmReinitParams *p)
 {
     std::lock_guard<std::mutex> lk(g_mu);

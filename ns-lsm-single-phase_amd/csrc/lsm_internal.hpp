@@ -129,6 +129,7 @@ size_t reinit_eno2_work(const Blk &b);
 int launch_metric_tensor(const Blk &b, double *gt, hipStream_t s);
 int launch_metric_tensor(const double *const m[9], long n, double *gt, hipStream_t s);
 size_t reinit_godunov_work(const Blk &b);
+size_t lambda_simpson_work(const Blk &b);
 int launch_normgrad(const Blk &b, const double *d, const double *phi0, int order_bc, int eno_bc,
                     double *ng, hipStream_t s);
 int launch_reinit_eno2(const Blk &b, const LsmReinitParams &p, const double *phi0, double *out,
@@ -138,6 +139,8 @@ int launch_sign_ini(const Blk &b, const double *phi0, double eps, int m_sphi0, d
 int launch_rh_rn(const Blk &b, const double *d, const double *sgn, double *rh, hipStream_t s);
 int launch_lambda_rn(const Blk &b, const double *phi0, const double *ng0, const double *d,
                      double dt, double eps, double *lam, hipStream_t s);
+int launch_lambda_rn_simpson(const Blk &b, const double *phi0, const double *ng0, const double *d, double dt,
+                             double eps, double *lam, Carver &w, hipStream_t s);
 int launch_reinit_godunov(const Blk &b, const LsmReinitParams &p, const double *phi0,
                           double *out, Carver &w, hipStream_t s);
 

@@ -516,6 +516,75 @@ __global__ __launch_bounds__(256) void lambda_kernel(long n, const double *__res
     lam[l] = lambda_node(phi0[l], ng0[l], d[l], dt, eps, dc, de, dz);
 }
 
+// calc_lambda_RN.F90:65-120 (sussman_correction_method = 1): the pointwise
+// numerator and denominator of every node, filtered by testfilter below
+__global__ __launch_bounds__(256) void lambda_parts_kernel(long n, const double *__restrict__ phi0,
+                                                           const double *__restrict__ ng0,
+                                                           const double *__restrict__ d, double dt, double eps,
+                                                           double *__restrict__ num, double *__restrict__ den)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const double del = deltaf(phi0[l], eps);
+    num[l] = del * ((1.0 / dt) * (d[l] - phi0[l]));
+    den[l] = (del * del) * ng0[l];
+}
+
+// testfilter.F90:46-121 on the numerator and the denominator, then the
+// quotient of calc_lambda_RN.F90:94-118.  At the nodes off the block's faces
+// the 27-node average with weights 64 / 16 / 4 / 1 (the node, face, edge and
+// corner neighbours) divided by aj there, in the reference's order (both
+// filters share the weights and their sum); the face nodes keep the
+// pointwise values.  lambda = 0 where the filtered denominator is within 1E-10
+// (a default-real literal in the reference) of 0.
+__global__ __launch_bounds__(256) void lambda_filter_kernel(Geo g, const double *__restrict__ aj,
+                                                            const double *__restrict__ num,
+                                                            const double *__restrict__ den,
+                                                            double *__restrict__ lam)
+{
+    const long n = (long)g.ni * g.nj * g.nk;
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    int i, j, k;
+    node_of(g, l, i, j, k);
+    double fn = num[l], fd = den[l];
+    if (i > 0 && i < g.ni - 1 && j > 0 && j < g.nj - 1 && k > 0 && k < g.nk - 1) {
+        // testfilter.F90:58-84 order: node, +-i, +-j, +-k, the i-j, i-k and
+        // j-k edges, then the corners
+        constexpr signed char off[27][3] = {
+            {0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
+            {1, 1, 0},   {1, -1, 0},  {-1, 1, 0},  {-1, -1, 0}, {1, 0, 1},   {1, 0, -1},  {-1, 0, 1},
+            {-1, 0, -1}, {0, 1, 1},   {0, 1, -1},  {0, -1, 1},  {0, -1, -1}, {1, 1, 1},   {1, 1, -1},
+            {1, -1, 1},  {1, -1, -1}, {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+        double sn = 0.0, sd = 0.0, sv = 0.0;
+#pragma unroll
+        for (int m = 0; m < 27; ++m) {
+            const int nz = (off[m][0] != 0) + (off[m][1] != 0) + (off[m][2] != 0);
+            const double c = nz == 0 ? 64.0 : nz == 1 ? 16.0 : nz == 2 ? 4.0 : 1.0;
+            const long q = l + off[m][0] + g.sj * off[m][1] + g.sk * off[m][2];
+            const double w = c / aj[q];
+            sn = sn + w * num[q];
+            sd = sd + w * den[q];
+            sv = sv + w;
+        }
+        fn = sn / sv;
+        fd = sd / sv;
+    }
+    lam[l] = (fabs(fd) <= (double)1e-10f) ? 0.0 : -(fn / fd);
+}
+
+// the correction of reinitialization_benchmark.F90:166-181 with a lambda field
+__global__ __launch_bounds__(256) void lambda_apply_kernel(long n, const double *__restrict__ v,
+                                                           const double *__restrict__ lam,
+                                                           const double *__restrict__ phi0,
+                                                           const double *__restrict__ ng0, double dt, double eps,
+                                                           double *__restrict__ out)
+{
+    const long l = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    out[l] = v[l] + ((dt * lam[l]) * deltaf(phi0[l], eps)) * ng0[l];
+}
+
 // one RK stage (or the Euler step) of reinitialization_benchmark.F90:96-158;
 // LAMBDA fuses the correction of reinitialization_benchmark.F90:166-181
 template <bool LAMBDA>
@@ -548,7 +617,9 @@ size_t fb(const Blk &b) { return ((size_t)b.n * sizeof(double) + 255) & ~(size_t
 
 }  // namespace
 
-size_t reinit_godunov_work(const Blk &b) { return 4 * fb(b) + 1024; }
+size_t reinit_godunov_work(const Blk &b) { return 7 * fb(b) + 1024; }
+
+size_t lambda_simpson_work(const Blk &b) { return 2 * fb(b) + 512; }
 
 int launch_metric_tensor(const double *const m[9], long n, double *gt, hipStream_t s)
 {
@@ -780,6 +851,19 @@ int launch_lambda_rn(const Blk &b, const double *phi0, const double *ng0, const 
     return check_hip(hipGetLastError(), "lambda_kernel");
 }
 
+int launch_lambda_rn_simpson(const Blk &b, const double *phi0, const double *ng0, const double *d, double dt,
+                             double eps, double *lam, Carver &w, hipStream_t s)
+{
+    const size_t f = b.n * sizeof(double);
+    double *num = (double *)w.take(f), *den = (double *)w.take(f);
+    if (!num || !den) return fail(LSM_ERR_WORKSPACE, "lambda workspace too small");
+    lambda_parts_kernel<<<nb(b.n), 256, 0, s>>>(b.n, phi0, ng0, d, dt, eps, num, den);
+    int rc = check_hip(hipGetLastError(), "lambda_parts_kernel");
+    if (rc) return rc;
+    lambda_filter_kernel<<<nb(b.n), 256, 0, s>>>(make_geo(b), b.aj, num, den, lam);
+    return check_hip(hipGetLastError(), "lambda_filter_kernel");
+}
+
 int launch_reinit_godunov(const Blk &b, const LsmReinitParams &p, const double *phi0, double *out,
                           Carver &w, hipStream_t s)
 {
@@ -795,6 +879,27 @@ int launch_reinit_godunov(const Blk &b, const LsmReinitParams &p, const double *
     static const double C[3][3] = {{1.0, 0.0, 1.0},
                                    {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0},
                                    {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    if (p.lambda_method == 1) {
+        // filtered lambda (calc_lambda_RN.F90:65-120): the stages without the
+        // fused correction, then lambda over the whole field and the correction
+        double *lam = (double *)w.take(f);
+        if (!lam) return fail(LSM_ERR_WORKSPACE, "reinit workspace too small");
+        for (int it = 0; it < p.rn_itermax; ++it) {
+            if (p.m_ti == 1) {
+                godunov_stage<false><<<nb(b.n), 256, 0, s>>>(g, out, out, A, sgn, phi0, ng0, C[0][0], C[0][1], C[0][2], dt, eps);
+                godunov_stage<false><<<nb(b.n), 256, 0, s>>>(g, A, out, B, sgn, phi0, ng0, C[1][0], C[1][1], C[1][2], dt, eps);
+                godunov_stage<false><<<nb(b.n), 256, 0, s>>>(g, B, out, A, sgn, phi0, ng0, C[2][0], C[2][1], C[2][2], dt, eps);
+            } else {
+                godunov_stage<false><<<nb(b.n), 256, 0, s>>>(g, out, out, A, sgn, phi0, ng0, 0.0, 0.0, -1.0, dt, eps);
+            }
+            if ((rc = check_hip(hipGetLastError(), "godunov_stage"))) return rc;
+            Carver lw = w;
+            if ((rc = launch_lambda_rn_simpson(b, phi0, ng0, A, dt, eps, lam, lw, s))) return rc;
+            lambda_apply_kernel<<<nb(b.n), 256, 0, s>>>(b.n, A, lam, phi0, ng0, dt, eps, out);
+            if ((rc = check_hip(hipGetLastError(), "lambda_apply_kernel"))) return rc;
+        }
+        return LSM_OK;
+    }
     for (int it = 0; it < p.rn_itermax; ++it) {
         if (p.m_ti == 1) {
             godunov_stage<false><<<nb(b.n), 256, 0, s>>>(g, out, out, A, sgn, phi0, ng0, C[0][0], C[0][1], C[0][2], dt, eps);

@@ -22,6 +22,7 @@ module lsm_hip_f
       real(c_double)     :: epslsm, eps_reinit, rn_timestep
       integer(c_int32_t) :: rn_itermax, order_bc, eno_bc, m_sphi0, m_ti
       integer(c_int32_t) :: hybrid = 0
+      integer(c_int32_t) :: lambda_method = 0
    end type lsm_reinit_params
 
    interface
@@ -138,6 +139,14 @@ module lsm_hip_f
          real(c_double), value         :: dt, eps
          real(c_double), intent(inout) :: lam(*)
       end function lsm_host_lambda_rn
+
+      integer(c_int) function lsm_host_lambda_rn_simpson(phi0, ng0, d, dt, eps, lam) &
+                              bind(C, name="lsm_host_lambda_rn_simpson")
+         import :: c_int, c_double
+         real(c_double), intent(in)    :: phi0(*), ng0(*), d(*)
+         real(c_double), value         :: dt, eps
+         real(c_double), intent(inout) :: lam(*)
+      end function lsm_host_lambda_rn_simpson
 
       integer(c_int) function lsm_host_reinit_godunov(phi0, phi_out, p) bind(C, name="lsm_host_reinit_godunov")
          import :: c_int, c_double, lsm_reinit_params

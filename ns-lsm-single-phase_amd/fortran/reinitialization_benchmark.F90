@@ -21,8 +21,8 @@ subroutine reinitialization_benchmark(  phizero, phicorregido,                  
    include 'lsm_shim_bind.inc'
    if ( M_Grad /= 0 ) call lsm_f_unsupported( 'reinitialization_benchmark', 'M_Grad /= 0' )
    if ( is_obstacle /= 0 ) call lsm_f_unsupported( 'reinitialization_benchmark', 'obstacles' )
-   if ( sussman_correction_method /= 0 ) &
-      call lsm_f_unsupported( 'reinitialization_benchmark', 'sussman_correction_method /= 0' )
+   if ( sussman_correction_method /= 0 .and. sussman_correction_method /= 1 ) &
+      call lsm_f_unsupported( 'reinitialization_benchmark', 'sussman_correction_method not 0 or 1' )
    p%epslsm      = epslsm
    p%eps_reinit  = 0.0_rdf
    p%rn_timestep = RNtimestep
@@ -31,6 +31,7 @@ subroutine reinitialization_benchmark(  phizero, phicorregido,                  
    p%eno_bc      = 0
    p%m_sphi0     = M_Sphi0
    p%m_ti        = M_TI
+   p%lambda_method = sussman_correction_method
    allocate( sgndf(il:iu,jl:ju,kl:ku) )
    call lsm_f_check( lsm_host_reinit_godunov( phizero, sgndf, p ), 'reinitialization_benchmark' )
    if ( hybrid_reinitialisation ) then

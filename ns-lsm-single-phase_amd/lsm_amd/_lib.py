@@ -21,6 +21,7 @@ LSM_OP_RK3, LSM_OP_REINIT_ENO2, LSM_OP_REINIT_GODUNOV, LSM_OP_WATER_VOLUME, LSM_
 LSM_OP_RK3_ADJOINT = 6
 LSM_OP_REINIT_ENO2_ADJOINT, LSM_OP_STEP_ADJOINT, LSM_OP_STEP_TRACE, LSM_OP_STEP_ADJOINT_RECORDED = 7, 8, 9, 10
 LSM_OP_REINIT_GEOMETRIC = 11
+LSM_OP_LAMBDA_RN_SIMPSON = 12
 
 _vp = C.c_void_p
 
@@ -48,7 +49,8 @@ class LsmBlock(C.Structure):
 class LsmReinitParams(C.Structure):
     _fields_ = [("epslsm", C.c_double), ("eps_reinit", C.c_double), ("rn_timestep", C.c_double),
                 ("rn_itermax", C.c_int32), ("order_bc", C.c_int32), ("eno_bc", C.c_int32),
-                ("m_sphi0", C.c_int32), ("m_ti", C.c_int32), ("hybrid", C.c_int32)]
+                ("m_sphi0", C.c_int32), ("m_ti", C.c_int32), ("hybrid", C.c_int32),
+                ("lambda_method", C.c_int32)]
 
 
 class LsmBatchBlock(C.Structure):
@@ -127,6 +129,8 @@ _SIGS = {
     "lsm_sign_ini": (C.c_int, [C.POINTER(LsmBlock), _vp, C.c_double, C.c_int, _vp, _vp, _vp]),
     "lsm_rh_rn": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp]),
     "lsm_lambda_rn": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp]),
+    "lsm_lambda_rn_simpson": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, C.c_double, C.c_double, _vp, _vp,
+                                        C.c_size_t, _vp]),
     "lsm_reinit_geometric": (C.c_int, [C.POINTER(LsmBlock), _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "lsm_reinit_godunov": (C.c_int, [C.POINTER(LsmBlock), C.POINTER(LsmReinitParams), _vp, _vp, _vp,
                                      C.c_size_t, _vp]),
@@ -160,6 +164,7 @@ _SIGS = {
     "lsm_host_sign_ini": (C.c_int, [_vp, C.c_double, C.c_int, _vp, _vp]),
     "lsm_host_rh_rn": (C.c_int, [_vp, _vp, _vp]),
     "lsm_host_lambda_rn": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp]),
+    "lsm_host_lambda_rn_simpson": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp]),
     "lsm_host_reinit_godunov": (C.c_int, [_vp, _vp, C.POINTER(LsmReinitParams)]),
     "lsm_host_water_volume": (C.c_int, [_vp, _vp, _vp]),
     "lsm_host_phi_gradient_eno2": (C.c_int, [_vp, C.c_int32, C.c_double, C.c_double, C.c_double, _vp, _vp, _vp,

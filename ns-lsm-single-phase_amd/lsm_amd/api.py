@@ -176,6 +176,7 @@ class ReinitParams:
     M_Sphi0: int = 1
     M_TI: int = 1
     hybrid_reinitialisation: bool = False   # levelsetmethod only: geometric reinitialisation
+    sussman_correction_method: int = 0      # Godunov path: 0 pointwise lambda, 1 filtered (testfilter)
 
     def c(self) -> LsmReinitParams:
         p = LsmReinitParams()
@@ -188,6 +189,7 @@ class ReinitParams:
         p.m_sphi0 = int(self.M_Sphi0)
         p.m_ti = int(self.M_TI)
         p.hybrid = int(bool(self.hybrid_reinitialisation))
+        p.lambda_method = int(self.sussman_correction_method)
         return p
 
 
@@ -454,12 +456,21 @@ def calc_RH_RN(blk: Block, d, sign):
     return rh
 
 
-def calc_lambda_RN(blk: Block, phi0, normgrad0, d, timestep: float, epslsm: float):
-    """calc_lambda_RN.F90:1-122 (point-constant method)."""
+def calc_lambda_RN(blk: Block, phi0, normgrad0, d, timestep: float, epslsm: float,
+                   sussman_correction_method: int = 0, work=None):
+    """calc_lambda_RN.F90:1-122: the point-constant lambda (method 0) or the
+    one filtered by testfilter (method 1)."""
     lam = blk.empty()
-    check(_lib.load().lsm_lambda_rn(blk.ref, _f64(phi0, blk.shape, "phi0"), _f64(normgrad0, blk.shape, "ng0"),
-                                    _f64(d, blk.shape, "d"), float(timestep), float(epslsm),
-                                    _f64(lam, blk.shape, "lambda"), _stream()), "lsm_lambda_rn")
+    L = _lib.load()
+    args = (blk.ref, _f64(phi0, blk.shape, "phi0"), _f64(normgrad0, blk.shape, "ng0"), _f64(d, blk.shape, "d"),
+            float(timestep), float(epslsm), _f64(lam, blk.shape, "lambda"))
+    if sussman_correction_method == 1:
+        wp, wn, keep = _ws(blk, _lib.LSM_OP_LAMBDA_RN_SIMPSON, work)
+        check(L.lsm_lambda_rn_simpson(*args, wp, wn, _stream()), "lsm_lambda_rn_simpson")
+    elif sussman_correction_method == 0:
+        check(L.lsm_lambda_rn(*args, _stream()), "lsm_lambda_rn")
+    else:
+        raise ValueError(f"sussman_correction_method = {sussman_correction_method} (0 or 1)")
     return lam
 
 

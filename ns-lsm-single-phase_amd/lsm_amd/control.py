@@ -205,8 +205,8 @@ class LsmParams:
         bad = []
         if self.orderAD not in (0, 1):
             bad.append(f"orderAD = {self.orderAD} (0: first-order upwind, 1: WENO3)")
-        if self.sussman_correction_method != 0:
-            bad.append("sussman_correction_method = 1 (Simpson filter)")
+        if self.sussman_correction_method not in (0, 1):
+            bad.append(f"sussman_correction_method = {self.sussman_correction_method} (0: point, 1: filtered)")
         if self.hybrid_reinitialisation and self.call_reinitialisation and self.GlobalMassCorrection:
             bad.append("hybrid reinitialisation with GlobalMassCorrection (global mass correction)")
         if self.nobstacles != 0:
@@ -227,7 +227,8 @@ class LsmParams:
                                 OrderReinitialisationBoundaries=self.OrderReinitialisationBoundaries,
                                 ENOBCReinitialisation=self.ENOBCReinitialisation,
                                 M_Sphi0=M_Sphi0, M_TI=M_TI,
-                                hybrid_reinitialisation=self.hybrid_reinitialisation)
+                                hybrid_reinitialisation=self.hybrid_reinitialisation,
+                                sussman_correction_method=self.sussman_correction_method)
 
 
 def read_ind3dmg_lsm(src: str, nzone: int = 1, delti: Optional[float] = None) -> LsmParams:

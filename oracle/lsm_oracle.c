@@ -40,6 +40,7 @@ typedef struct {
     int32_t rn_itermax;         /* RNitermax */
     int32_t order_bc;           /* OrderReinitialisationBoundaries */
     int32_t eno_bc;             /* ENOBCReinitialisation */
+    int32_t lambda_method;      /* sussman_correction_method (Godunov path): 0 point, 1 filtered */
 } OReinit;
 
 #define EPS_SIMS 2.220446049250313e-16   /* epsilon(1._rdf), precision.F90:62 */
@@ -589,6 +590,58 @@ void oracle_lambda_rn(const OBlock *b, const double *phi0, const double *ng0, co
     }
 }
 
+/* testfilter, testfilter.F90:1-124: at the nodes off the block's faces the
+   average of the 27-node neighbourhood with weights 64 (the node), 16 (the six
+   face neighbours), 4 (the twelve edge neighbours) and 1 (the eight corners),
+   each divided by aj there, summed in the reference's order; the face nodes
+   keep var */
+static void testfilter(const OBlock *b, const double *var, double *varf)
+{
+    static const int off[27][3] = {
+        {0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
+        {1, 1, 0},   {1, -1, 0},  {-1, 1, 0},  {-1, -1, 0}, {1, 0, 1},   {1, 0, -1},  {-1, 0, 1},
+        {-1, 0, -1}, {0, 1, 1},   {0, 1, -1},  {0, -1, 1},  {0, -1, -1}, {1, 1, 1},   {1, 1, -1},
+        {1, -1, 1},  {1, -1, -1}, {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+    const size_t n = (size_t)b->ni * b->nj * b->nk;
+    memcpy(varf, var, n * sizeof(double));
+    for (int k = 1; k < b->nk - 1; ++k)
+    for (int j = 1; j < b->nj - 1; ++j)
+    for (int i = 1; i < b->ni - 1; ++i) {
+        double avgvar = 0.0, avgvol = 0.0;
+        for (int m = 0; m < 27; ++m) {
+            const int nz = (off[m][0] != 0) + (off[m][1] != 0) + (off[m][2] != 0);
+            const double c = nz == 0 ? 64.0 : nz == 1 ? 16.0 : nz == 2 ? 4.0 : 1.0;
+            const size_t q = IX(b, i + off[m][0], j + off[m][1], k + off[m][2]);
+            const double w = c / b->aj[q];
+            avgvar = avgvar + w * var[q];
+            avgvol = avgvol + w;
+        }
+        varf[IX(b, i, j, k)] = avgvar / avgvol;
+    }
+}
+
+/* R15: calc_lambda_RN, method 1 (the filtered quotient), calc_lambda_RN.F90:65-120:
+   numerator and denominator filtered by testfilter, lambda = 0 where the
+   filtered denominator is within 1E-10 (a default-real literal) of 0 */
+void oracle_lambda_rn_simpson(const OBlock *b, const double *phi0, const double *ng0, const double *d,
+                              double *lam, double dt, double epslsm)
+{
+    const size_t n = (size_t)b->ni * b->nj * b->nk;
+    if (n == 0) return;
+    double *num = (double *)malloc(n * sizeof(double)), *den = (double *)malloc(n * sizeof(double));
+    double *fnum = (double *)malloc(n * sizeof(double)), *fden = (double *)malloc(n * sizeof(double));
+    for (size_t l = 0; l < n; ++l) {
+        const double del = deltaf(phi0[l], epslsm);
+        num[l] = del * ((1.0 / dt) * (d[l] - phi0[l]));
+        den[l] = (del * del) * ng0[l];
+    }
+    testfilter(b, num, fnum);
+    testfilter(b, den, fden);
+    const double tiny = (double)1e-10f;
+    for (size_t l = 0; l < n; ++l) lam[l] = (fabs(fden[l]) <= tiny) ? 0.0 : -(fnum[l] / fden[l]);
+    free(num); free(den); free(fnum); free(fden);
+}
+
 /* R12: reinitialization_benchmark, reinitialization_benchmark.F90:45-222    */
 /* (M_Grad = 0, no obstacles, non-hybrid)                                    */
 void oracle_reinit_godunov(const OBlock *b, const OReinit *p, int m_sphi0, int m_ti,
@@ -618,7 +671,8 @@ void oracle_reinit_godunov(const OBlock *b, const OReinit *p, int m_sphi0, int m
             oracle_rh_rn(b, out, sg, rh);
             for (size_t l = 0; l < n; ++l) out[l] = sn[l] - dt * rh[l];
         }
-        oracle_lambda_rn(b, phi0, ng0, out, lam, dt, p->epslsm);
+        if (p->lambda_method == 1) oracle_lambda_rn_simpson(b, phi0, ng0, out, lam, dt, p->epslsm);
+        else oracle_lambda_rn(b, phi0, ng0, out, lam, dt, p->epslsm);
         for (size_t l = 0; l < n; ++l)
             out[l] = out[l] + ((dt * lam[l]) * deltaf(phi0[l], p->epslsm)) * ng0[l];
         memcpy(sn, out, n * sizeof(double));

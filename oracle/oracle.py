@@ -31,7 +31,7 @@ class OBlock(C.Structure):
 class OReinit(C.Structure):
     _fields_ = [("epslsm", C.c_double), ("eps_reinit", C.c_double),
                 ("rn_timestep", C.c_double), ("rn_itermax", C.c_int32),
-                ("order_bc", C.c_int32), ("eno_bc", C.c_int32)]
+                ("order_bc", C.c_int32), ("eno_bc", C.c_int32), ("lambda_method", C.c_int32)]
 
 
 _lib = None
@@ -57,6 +57,7 @@ def lib():
         L.oracle_sign_ini.argtypes = [bp, _dp, _dp, _dp, C.c_double, C.c_int]
         L.oracle_rh_rn.argtypes = [bp, _dp, _dp, _dp]
         L.oracle_lambda_rn.argtypes = [bp, _dp, _dp, _dp, _dp, C.c_double, C.c_double]
+        L.oracle_lambda_rn_simpson.argtypes = [bp, _dp, _dp, _dp, _dp, C.c_double, C.c_double]
         L.oracle_reinit_godunov.argtypes = [bp, rp, C.c_int, C.c_int, _dp, _dp]
         L.oracle_water_volume.argtypes = [bp, _dp, _dp, _ip]
         L.oracle_levelset_step.argtypes = [bp, rp, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
@@ -236,17 +237,21 @@ def rh_rn(blk, d, sgn):
     return rh
 
 
-def lambda_rn(blk, phi0, ng0, d, dt, epslsm):
+def lambda_rn(blk, phi0, ng0, d, dt, epslsm, method=0):
+    """calc_lambda_RN: method 0 the pointwise quotient, 1 the filtered one
+    (sussman_correction_method)."""
     phi0, ng0, d = _c(phi0), _c(ng0), _c(d)
     lam = np.empty(blk.shape)
-    lib().oracle_lambda_rn(blk.ref, _p(phi0), _p(ng0), _p(d), _p(lam), float(dt), float(epslsm))
+    f = lib().oracle_lambda_rn_simpson if method == 1 else lib().oracle_lambda_rn
+    f(blk.ref, _p(phi0), _p(ng0), _p(d), _p(lam), float(dt), float(epslsm))
     return lam
 
 
-def reinit_godunov(blk, phi0, epslsm, rn_timestep, rn_itermax, m_sphi0=1, m_ti=1):
+def reinit_godunov(blk, phi0, epslsm, rn_timestep, rn_itermax, m_sphi0=1, m_ti=1, lambda_method=0):
     phi0 = _c(phi0)
     out = np.empty(blk.shape)
     r = _reinit(epslsm, 0.0, rn_timestep, rn_itermax)
+    r.lambda_method = int(lambda_method)
     lib().oracle_reinit_godunov(blk.ref, C.byref(r), int(m_sphi0), int(m_ti), _p(phi0), _p(out))
     return out
 

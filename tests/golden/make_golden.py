@@ -107,6 +107,29 @@ def curvilinear_case():
          extra_inputs={"sign": sg, "ng0": ng0})
 
 
+def simpson_case():
+    """calc_lambda_RN with sussman_correction_method = 1 (the testfilter-filtered
+    quotient) alone and inside reinitialization_benchmark, on a distorted cube
+    whose Jacobian varies node to node (the filter weights divide by aj)."""
+    g = inputs.sinusoidal_cube_grid(12, amp=0.05, seed=21, nj=11, nk=10)
+    phi = inputs.spheres_sdf(g, nspheres=3, seed=22, rmin=0.2, rmax=0.3)
+    u, v, w = inputs.fourier_velocity(g, seed=23)
+    hmin, hmax = inputs.spacing_bounds(g)
+    eps = 1.5 * hmax
+    dtau = 0.5 * hmin
+    phi0 = phi * (0.1 + (g.x - 0.5) ** 2)
+    params = dict(delti_lsm=1e-3, epslsm=eps, rn_timestep=dtau, eps_reinit=1e-5,
+                  big_phi=5.0, btype=(1, 1, 1, 1, 1, 1), sussman_method=1)
+    (sg, ng0), _, _ = refio.run("sign_ini", g, phi, phi0=phi0, m_sphi0=1, epslsm=eps)
+    runs = [("lambda_rn", "lambda_rn", {"lambda_dt": dtau}, {"aux": ng0}),
+            ("lambda_rn_m0", "lambda_rn", {"lambda_dt": dtau, "sussman_method": 0}, {"aux": ng0})]
+    for ms in (0, 1):
+        for mt in (0, 1):
+            runs.append((f"reinit_godunov_m{ms}_t{mt}", "reinit_godunov",
+                         {"m_sphi0": ms, "m_ti": mt, "rn_itermax": 4}, {"phi": phi0}))
+    case("simpson", g, phi, phi0, u, v, w, params, runs, extra_inputs={"sign": sg, "ng0": ng0})
+
+
 def channel_case():
     """Solitary-wave channel (the 96x24x48 configuration, coarsened)."""
     g = inputs.channel_grid(32, 6, 16)
@@ -357,7 +380,7 @@ def blanking_consumers_case():
 
 CASES = {"curvilinear": curvilinear_case, "channel": channel_case, "boundary": boundary_case,
          "volume": volume_case, "consumers": consumers_case, "upwind": upwind_case, "hybrid": hybrid_case,
-         "blanking": blanking_case, "blanking_consumers": blanking_consumers_case}
+         "blanking": blanking_case, "blanking_consumers": blanking_consumers_case, "simpson": simpson_case}
 
 if __name__ == "__main__":
     if not refio.available():

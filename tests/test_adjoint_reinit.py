@@ -28,10 +28,12 @@ import torch_ref as T  # noqa: E402
 from lsm_amd import inputs  # noqa: E402
 
 # (shape, OrderReinitialisationBoundaries, ENOBCReinitialisation, iterations);
-# the last two span two or three 64-node i tiles of the tape gather (partial
-# last tile) with more than 8 k planes
+# (133, ...) and (70, ...) span two or three 64-node i tiles of the tape gather
+# (partial last tile) with more than 8 k planes; in the last two every axis has
+# at most 7 nodes, so nodes receive the one-sided terms of both faces of an axis
 CASES = [((12, 10, 9), 1, False, 4), ((11, 9, 8), 2, True, 3), ((10, 12, 11), 3, False, 4),
-         ((9, 8, 10), 2, False, 2), ((133, 9, 10), 1, False, 3), ((70, 12, 11), 2, True, 2)]
+         ((9, 8, 10), 2, False, 2), ((133, 9, 10), 1, False, 3), ((70, 12, 11), 2, True, 2),
+         ((7, 6, 7), 2, True, 3), ((6, 7, 5), 3, False, 2)]
 
 
 def _field(shape, seed=81):

@@ -153,6 +153,13 @@ def test_unsupported_options_are_rejected(ctl):
     prm.orderAD = 2
     with pytest.raises(NotImplementedError, match="orderAD"):
         prm.check_supported()
+    prm.orderAD = 1
+    prm.sussman_correction_method = 1  # the filtered lambda (Godunov path) is supported
+    prm.check_supported()
+    assert prm.reinit_params().c().lambda_method == 1
+    prm.sussman_correction_method = 2
+    with pytest.raises(NotImplementedError, match="sussman_correction_method"):
+        prm.check_supported()
 
 
 REF_CONTROL = "/root/reference/control.dat"

@@ -38,8 +38,10 @@ def replay(fx, key, mode, harness=refio.SHIM_HARNESS, **over):
     args = {k: fx.par(k) for k in PARAM_KEYS}
     if "p_order_ad" in fx.d:
         args["order_ad"] = fx.par("order_ad")
+    if "p_sussman_method" in fx.d:
+        args["sussman_method"] = fx.par("sussman_method")
     for k in ("nsteps", "numiter_lsm", "rn_itermax", "order_reinit_bc", "eno_bc", "m_sphi0",
-              "m_ti", "lambda_dt", "btype", "big_phi"):
+              "m_ti", "lambda_dt", "btype", "big_phi", "sussman_method"):
         v = fx.kw(key, k)
         if v is not None:
             args[k] = v
@@ -102,6 +104,10 @@ CLOSE = [("curvilinear", f"reinit_eno2_o{o}", "reinit_eno2", {"phi": "phi0"}) fo
         [("curvilinear", f"sign_ini_m{m}", "sign_ini", {}) for m in (0, 1)] + \
         [("curvilinear", "lambda_rn", "lambda_rn", {"aux": "ng0"})] + \
         [("curvilinear", f"reinit_godunov_m{m}_t{t}", "reinit_godunov", {"phi": "phi0"})
+         for m in (0, 1) for t in (0, 1)] + \
+        [("simpson", "lambda_rn", "lambda_rn", {"aux": "ng0"}),
+         ("simpson", "lambda_rn_m0", "lambda_rn", {"aux": "ng0"})] + \
+        [("simpson", f"reinit_godunov_m{m}_t{t}", "reinit_godunov", {"phi": "phi0"})
          for m in (0, 1) for t in (0, 1)]
 
 

@@ -118,6 +118,29 @@ def test_rh_rn_and_lambda(curv):
     assert linf_rel(lam, fx.out("lambda_rn")) <= 1e-13
 
 
+@pytest.mark.parametrize("key,method", [("lambda_rn", 1), ("lambda_rn_m0", 0)])
+def test_lambda_filtered(key, method):
+    """calc_lambda_RN with sussman_correction_method = 1 (testfilter) against
+    the compiled reference; 1e-13 as the pointwise lambda (device cos in
+    deltaf)."""
+    fx = load("simpson")
+    blk = api.Block(fx.grid, btype=fx.par("btype"), big_phi=fx.par("big_phi"))
+    lam = host(api.calc_lambda_RN(blk, dev(fx.inp("phi0")), dev(fx.inp("ng0")), dev(fx.inp("phi")),
+                                  fx.kw(key, "lambda_dt"), fx.par("epslsm"), sussman_correction_method=method))
+    assert linf_rel(lam, fx.out(key)) <= 1e-13
+
+
+@pytest.mark.parametrize("m", [0, 1])
+@pytest.mark.parametrize("t", [0, 1])
+def test_reinit_godunov_filtered_lambda(m, t):
+    fx = load("simpson")
+    blk = api.Block(fx.grid, btype=fx.par("btype"), big_phi=fx.par("big_phi"))
+    prm = api.ReinitParams(epslsm=fx.par("epslsm"), RNtimestep=fx.par("rn_timestep"), RNitermax=4,
+                           M_Sphi0=m, M_TI=t, sussman_correction_method=1)
+    out = host(api.reinitialization_benchmark(blk, dev(fx.inp("phi0")), prm))
+    assert linf_rel(out, fx.out(f"reinit_godunov_m{m}_t{t}")) <= 1e-13
+
+
 @pytest.mark.parametrize("m", [0, 1])
 @pytest.mark.parametrize("t", [0, 1])
 def test_reinit_godunov(curv, m, t):

@@ -152,3 +152,73 @@ def test_volume_cases(plane):
     _eq(vol, fx.out(f"volume_{plane}", 0))
     _eq(mask, fx.out(f"volume_{plane}", 1))
     _eq(O.normgrad(blk, p, p), fx.out(f"normgrad_{plane}"))
+
+
+# ---- calc_lambda_RN with sussman_correction_method = 1 (testfilter.F90) ----
+
+@pytest.fixture(scope="module")
+def simpson():
+    return load("simpson")
+
+
+@pytest.mark.parametrize("key,method", [("lambda_rn", 1), ("lambda_rn_m0", 0)])
+def test_lambda_rn_filtered(simpson, key, method):
+    lam = O.lambda_rn(_blk(simpson), simpson.inp("phi0"), simpson.inp("ng0"), simpson.inp("phi"),
+                      simpson.kw(key, "lambda_dt"), simpson.par("epslsm"), method=method)
+    _eq(lam, simpson.out(key))
+
+
+@pytest.mark.parametrize("m", [0, 1])
+@pytest.mark.parametrize("t", [0, 1])
+def test_reinit_godunov_filtered_lambda(simpson, m, t):
+    out = O.reinit_godunov(_blk(simpson), simpson.inp("phi0"), simpson.par("epslsm"),
+                           simpson.par("rn_timestep"), 4, m, t, lambda_method=1)
+    _eq(out, simpson.out(f"reinit_godunov_m{m}_t{t}"))
+
+
+def test_testfilter_closed_forms(simpson):
+    """The filtered quotient against what the mathematics fixes: with a
+    uniform Jacobian the filter is the 64/16/4/1 average, so a field linear in
+    i, j, k is reproduced exactly at interior nodes; with num = c * den at
+    every node lambda is -c wherever the filtered denominator is not tiny; and
+    the face nodes keep the pointwise quotient."""
+    g = simpson.grid
+    blk = O.Block(g)
+    n = g.shape
+    rng = np.random.default_rng(5)
+    phi0 = (rng.uniform(-1, 1, n)) * 0.02            # every node in the band
+    ng0 = rng.uniform(0.5, 2.0, n)
+    eps = 0.05
+    dt = 0.01
+    # d - phi0 = -c * dt * deltaf(phi0) * ng0 makes num = -c * den pointwise
+    delta = np.where(np.abs(phi0) > eps, 0.0, (1.0 / (2.0 * eps)) * (1.0 + np.cos(np.pi * phi0 / eps)))
+    c = 0.37
+    d = phi0 - c * dt * delta * ng0
+    lam = O.lambda_rn(blk, phi0, ng0, d, dt, eps, method=1)
+    assert np.max(np.abs(lam - c)) <= 1e-12
+    lam0 = O.lambda_rn(blk, phi0, ng0, d, dt, eps, method=0)
+    faces = np.zeros(n, bool)
+    faces[0, :, :] = faces[-1, :, :] = faces[:, 0, :] = faces[:, -1, :] = faces[:, :, 0] = faces[:, :, -1] = True
+    np.testing.assert_array_equal(lam[faces], lam0[faces])
+
+
+def test_testfilter_weights_on_a_quadratic():
+    """On a uniform grid (constant Jacobian) the 64/16/4/1 filter maps a field
+    quadratic in the i index, f = i^2, to f + 72/216 = f + 1/3 at the interior
+    nodes (the sum of weight x offset^2 over the 27 nodes is 72, of the weights
+    216): a wrong weight moves the 1/3.  Driven through the filtered lambda
+    with a uniform denominator (phi0 = 0, |grad phi0| = eps^2: den = 1)."""
+    from lsm_amd import inputs
+    g = inputs.sinusoidal_cube_grid(9, amp=0.0, seed=1, nj=8, nk=7)
+    assert np.ptp(g.aj) <= 1e-12 * np.max(np.abs(g.aj))
+    blk = O.Block(g)
+    nk, nj, ni = g.shape
+    eps, dt = 0.1, 0.01
+    phi0 = np.zeros(g.shape)
+    ng0 = np.full(g.shape, eps * eps)
+    f = np.broadcast_to((np.arange(ni, dtype=float) ** 2)[None, None, :], g.shape)
+    d = eps * dt * f                       # num = (1/eps) (1/dt) d = f
+    lam = O.lambda_rn(blk, phi0, ng0, d, dt, eps, method=1)
+    inner = (slice(1, -1),) * 3
+    np.testing.assert_allclose(lam[inner], -(f[inner] + 1.0 / 3.0), rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(lam[0], -f[0], rtol=1e-13, atol=1e-13)
