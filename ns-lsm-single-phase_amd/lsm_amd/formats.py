@@ -12,6 +12,8 @@ is negated when one precedes), the convention of gfortran and flang.
 * phi_ini   per zone: phi, then phi_n, one record each (init_LSM.F90:350-357)
 * phi.NNN   restart of one rank: phi, then phi_n over the rank's owned nodes
             (main.F90:404-455)
+* debugfiles/<name>.NN  a debug dump of one rank: one record of a field over
+            the rank's nodes without ghost layers (global_debug.F90:15-99)
 
 Arrays are returned shaped (nk, nj, ni) (C order, i fastest), the layout the
 rest of this package uses.
@@ -129,3 +131,30 @@ def grid_from_file(path: str, shape: Tuple[int, int, int], dc: float = 1.0, de: 
     g = inputs.Grid(x, y, z)
     g.dc, g.de, g.dz = dc, de, dz
     return inputs.compute_metrics(g)
+
+
+def debug_path(outdir: str, name: str, rank: int = 0) -> str:
+    """global_debug.F90:44-47: trim(OUTPUTdir) // 'debugfiles/' // name // '.' //
+    the rank as i2.2 (two digits, '**' from 100 on, as the Fortran edit
+    descriptor writes it); OUTPUTdir is concatenated as given."""
+    nn = f"{rank:02d}" if 0 <= rank <= 99 else "**"
+    return f"{outdir.rstrip()}debugfiles/{name}.{nn}"
+
+
+def write_debug(path: str, field: np.ndarray, ghost: Tuple[int, int, int] = (0, 0, 0)) -> None:
+    """outputD3_real (global_debug.F90:62-99, through outputD1_real): the
+    field of one rank, shaped (nk, nj, ni) with `ghost` = (igp, jgp, kgp)
+    ghost layers on each side, written without them as one record."""
+    igp, jgp, kgp = ghost
+    a = np.asarray(field)
+    nk, nj, ni = a.shape
+    inner = a[kgp:nk - kgp, jgp:nj - jgp, igp:ni - igp]
+    write_records(path, [_bytes(inner)])
+
+
+def read_debug(path: str, shape: Tuple[int, int, int]) -> np.ndarray:
+    """A debug dump of `shape` = (nk, nj, ni) nodes (without ghost layers)."""
+    recs = read_records(path)
+    if len(recs) != 1:
+        raise ValueError(f"{path}: {len(recs)} records, expected 1")
+    return _field(recs[0], shape)

@@ -109,3 +109,55 @@ def test_metrics_match_compiled_mg_metrics(name, tmp_path):
             h.zet[0], h.zet[1], h.zet[2], h.aj]
     for n, a in enumerate(ours):
         assert np.array_equal(a, fx.out("metrics", n)), n
+
+
+DEBUG_FORTRAN = r"""
+program dbg
+  implicit none
+  integer, parameter :: ni = 5, nj = 4, nk = 3, g = 2
+  real(8) :: v(1-g:ni+g, 1-g:nj+g, 1-g:nk+g), p(ni, nj, nk)
+  integer :: i, j, k
+  v = -1.0d0
+  do k = 1, nk
+  do j = 1, nj
+  do i = 1, ni
+     v(i,j,k) = i + 10.0d0 * j + 100.0d0 * k
+  end do
+  end do
+  end do
+  ! outputD1_real's record (global_debug.F90:49-55): the field without ghost points
+  open(43, file='debugfiles/phi.07', form='unformatted')
+  write(43) v(1:ni, 1:nj, 1:nk)
+  close(43)
+  open(44, file='debugfiles/py.07', form='unformatted')
+  read(44) p
+  close(44)
+  print '(es25.16)', sum(p * v(1:ni, 1:nj, 1:nk))
+end program dbg
+"""
+
+
+def test_debug_path():
+    assert F.debug_path("out/", "phicorrected", 3) == "out/debugfiles/phicorrected.03"
+    assert F.debug_path("out/", "phi", 123).endswith("phi.**")
+
+
+@pytest.mark.skipif(shutil.which("amdflang") is None, reason="amdflang not available")
+def test_debug_dump_fortran_compatibility(tmp_path):
+    """A dump written by Fortran (one unformatted record of the nodes without
+    ghost layers) reads back, and one written here reads in Fortran."""
+    (tmp_path / "debugfiles").mkdir()
+    (tmp_path / "dbg.f90").write_text(DEBUG_FORTRAN)
+    subprocess.run(["amdflang", "dbg.f90", "-o", "dbg"], cwd=tmp_path, check=True)
+    shape = (3, 4, 5)
+    kk, jj, ii = np.meshgrid(np.arange(1, 4), np.arange(1, 5), np.arange(1, 6), indexing="ij")
+    ref = ii + 10.0 * jj + 100.0 * kk
+    rng = np.random.default_rng(7)
+    mine = rng.standard_normal(shape)
+    padded = np.full((3 + 4, 4 + 4, 5 + 4), -1.0)
+    padded[2:-2, 2:-2, 2:-2] = mine
+    F.write_debug(F.debug_path(str(tmp_path) + "/", "py", 7), padded, ghost=(2, 2, 2))
+    out = subprocess.run(["./dbg"], cwd=tmp_path, check=True, capture_output=True, text=True).stdout
+    assert abs(float(out) - np.sum(mine * ref)) <= 1e-12 * np.sum(np.abs(mine * ref))
+    got = F.read_debug(F.debug_path(str(tmp_path) + "/", "phi", 7), shape)
+    assert np.array_equal(got, ref)
