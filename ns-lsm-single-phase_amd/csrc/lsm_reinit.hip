@@ -179,20 +179,21 @@ __device__ __forceinline__ double reinit_update(const StageP &p, double ngv, dou
     return sussman_update(p.c1, p.c2, p.c3 * p.dt, ngv, dv, dnv, s, del);
 }
 
-// block-wide max/min (256 threads, linear id tid) into ctl
+// block-wide max/min (NW wavefronts, linear id tid) into ctl
+template <int NW>
 __device__ __forceinline__ void reduce_maxmin_t(double vmax, double vmin, int tid, Ctl *ctl)
 {
     for (int off = 32; off > 0; off >>= 1) {
         vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
         vmin = fmin(vmin, __shfl_xor(vmin, off, 64));
     }
-    __shared__ double smax[4], smin[4];
+    __shared__ double smax[NW], smin[NW];
     const int lane = tid & 63, wv = tid >> 6;
     if (lane == 0) { smax[wv] = vmax; smin[wv] = vmin; }
     __syncthreads();
     if (tid == 0) {
         double a = smax[0], b = smin[0];
-        for (int q = 1; q < 4; ++q) { a = fmax(a, smax[q]); b = fmin(b, smin[q]); }
+        for (int q = 1; q < NW; ++q) { a = fmax(a, smax[q]); b = fmin(b, smin[q]); }
         atomicMax(&ctl->maxbits, (unsigned long long)__double_as_longlong(a));
         atomicMin(&ctl->minbits, (unsigned long long)__double_as_longlong(b));
     }
@@ -225,29 +226,30 @@ __device__ __forceinline__ void tape_write(const RTArgs &a, long l, bool upd, do
     __builtin_nontemporal_store(code, a.tcode + l);
 }
 
-// STAGE 0: only the max/min of |grad din| (reinitialisation_test2.F90:96-101)
-template <int STAGE, bool TAPE = false>
+// STAGE 0: only the max/min of |grad din| (reinitialisation_test2.F90:96-101).
+// TY: tile rows (one wavefront per row)
+template <int STAGE, bool TAPE = false, int TY = RTY>
 struct KReinit {
     using Args = RTArgs;
-    static constexpr int kThreads = RTX * RTY;
+    static constexpr int kThreads = RTX * TY;
     __device__ __forceinline__ static void run(const Args &a, int bid, int nb);
     static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
 };
 
-template <int STAGE, bool TAPE = false>
-__global__ __launch_bounds__(RTX * RTY) void reinit_stage_tiled(RTArgs a)
+template <int STAGE, bool TAPE = false, int TY = RTY>
+__global__ __launch_bounds__(RTX * TY) void reinit_stage_tiled(RTArgs a)
 {
-    KReinit<STAGE, TAPE>::run(a, (int)blockIdx.x, (int)gridDim.x);
+    KReinit<STAGE, TAPE, TY>::run(a, (int)blockIdx.x, (int)gridDim.x);
 }
 
-template <int STAGE, bool TAPE>
-void KReinit<STAGE, TAPE>::one(unsigned g, dim3 b, hipStream_t s, const Args &a)
+template <int STAGE, bool TAPE, int TY>
+void KReinit<STAGE, TAPE, TY>::one(unsigned g, dim3 b, hipStream_t s, const Args &a)
 {
-    reinit_stage_tiled<STAGE, TAPE><<<g, b, 0, s>>>(a);
+    reinit_stage_tiled<STAGE, TAPE, TY><<<g, b, 0, s>>>(a);
 }
 
-template <int STAGE, bool TAPE>
-__device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int bid, int)
+template <int STAGE, bool TAPE, int TY>
+__device__ __forceinline__ void KReinit<STAGE, TAPE, TY>::run(const RTArgs &a, int bid, int)
 {
     if (!a.ctl->active) return;
     const Geo &g = a.g;
@@ -257,7 +259,7 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int b
 
     if (bid >= a.grid_tiles) {
         // ---- boundary shell: one node per thread, generic formulas ----
-        const long t = (long)(bid - a.grid_tiles) * (RTX * RTY) + tid;
+        const long t = (long)(bid - a.grid_tiles) * (RTX * TY) + tid;
         int i = 0, j = 0, k = 0;
         bool ok = true;
         if (t < a.nI) {
@@ -295,7 +297,7 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int b
             }
             vmax = ngv; vmin = ngv;
         }
-        if (STAGE == 3 || STAGE == 0) reduce_maxmin_t(vmax, vmin, tid, a.ctl);
+        if (STAGE == 3 || STAGE == 0) reduce_maxmin_t<TY>(vmax, vmin, tid, a.ctl);
         return;
     }
 
@@ -309,24 +311,24 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int b
     const int tj = t2 % a.tiles_j;
     const int tk = t2 / a.tiles_j;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = ti * RTX + tx, j = tj * RTY + ty;
+    const int i = ti * RTX + tx, j = tj * TY + ty;
     const int kb = 1 + tk * a.kc;
     const int ke = live ? min(kb + a.kc - 1, g.nk - 2) : kb - 1;
 
-    __shared__ double sm[2][RTY + 2 * RH][RTX + 2 * RH];
+    __shared__ double sm[2][TY + 2 * RH][RTX + 2 * RH];
 
     const int ic = min(i, g.ni - 1), jc = min(j, g.nj - 1);
     const long col = (long)ic + g.sj * jc;
     const bool has_jh = ty < 4;
-    const int rr = (ty < 2) ? ty : ty - 2 + RTY + RH;
-    const int jh = min(max(tj * RTY - RH + rr, 0), g.nj - 1);
+    const int rr = (ty < 2) ? ty : ty - 2 + TY + RH;
+    const int jh = min(max(tj * TY - RH + rr, 0), g.nj - 1);
     const long col_jh = (long)ic + g.sj * jh;
     const int lin = tid;
-    const bool has_ih = lin < 4 * RTY;
-    const int hr = (lin >> 2) % RTY, hc4 = lin & 3;
+    const bool has_ih = lin < 4 * TY;
+    const int hr = (lin >> 2) % TY, hc4 = lin & 3;
     const int hcol = (hc4 < 2) ? hc4 : hc4 + RTX;
     const int hi_i = min(max(ti * RTX - RH + hcol, 0), g.ni - 1);
-    const int hi_j = min(tj * RTY + hr, g.nj - 1);
+    const int hi_j = min(tj * TY + hr, g.nj - 1);
     const long col_ih = (long)hi_i + g.sj * hi_j;
     auto kcl = [&](int kk) -> long { return g.sk * (long)min(max(kk, 0), g.nk - 1); };
     const bool act = live && i >= 1 && i <= g.ni - 2 && j >= 1 && j <= g.nj - 2;
@@ -399,7 +401,7 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE>::run(const RTArgs &a, int b
         q0 = q1; q1 = q2; q2 = q3; q3 = q4;
         buf ^= 1;
     }
-    if (STAGE == 3 || STAGE == 0) reduce_maxmin_t(vmax, vmin, tid, a.ctl);
+    if (STAGE == 3 || STAGE == 0) reduce_maxmin_t<TY>(vmax, vmin, tid, a.ctl);
 }
 
 // ---------------------------------------------------------------------------
@@ -669,17 +671,22 @@ const double kRkC[3][3] = {{1.0, 0.0, 1.0},
 struct RTShape {
     RTArgs a;
     unsigned grid;
+    int ty;   // tile rows
 };
 
+// ty: tile rows, 4 for the plain stages, 8 for the taped ones (whose wider
+// writes make the taller tile's smaller j halo pay: 1.6 % on the step adjoint
+// at 256^3, 0.5 % slower for the plain stages)
 RTShape rt_shape(const Blk &b, const LsmReinitParams &p, const double *phi0, uint8_t *conv, Ctl *ctl,
-                 const double *gt)
+                 const double *gt, int ty = RTY)
 {
     RTShape r{};
     RTArgs &ra = r.a;
     ra.g = make_geo(b); ra.phi0 = phi0; ra.conv = conv; ra.ctl = ctl;
     ra.gt = gt; ra.gs = b.n;
+    r.ty = ty == 8 ? 8 : RTY;
     ra.tiles_i = (b.e.ni + RTX - 1) / RTX;
-    ra.tiles_j = (b.e.nj + RTY - 1) / RTY;
+    ra.tiles_j = (b.e.nj + r.ty - 1) / r.ty;
     const int nk_int = b.e.nk - 2;
     const int tij = ra.tiles_i * ra.tiles_j;
     // about 4096 tiles (16 planes per tile at 256^3: the k halo of a tile is
@@ -693,22 +700,32 @@ RTShape rt_shape(const Blk &b, const LsmReinitParams &p, const double *phi0, uin
     ra.nI = 2L * b.e.nj * b.e.nk;
     ra.nJ = 2L * (b.e.ni - 2) * b.e.nk;
     ra.nK = 2L * (b.e.ni - 2) * (b.e.nj - 2);
-    const unsigned shell_blocks = (unsigned)((ra.nI + ra.nJ + ra.nK + 255) / 256);
+    const long wg = (long)RTX * r.ty;
+    const unsigned shell_blocks = (unsigned)((ra.nI + ra.nJ + ra.nK + wg - 1) / wg);
     r.grid = (unsigned)ra.grid_tiles + shell_blocks;
     ra.p = StageP{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};
     return r;
 }
 
-int rt_launch(RTShape &r, int stage, hipStream_t s, bool tape = false)
+template <int TY>
+int rt_launch_ty(RTShape &r, int stage, hipStream_t s, bool tape)
 {
-    const dim3 rblk(RTX, RTY, 1);
+    const dim3 rblk(RTX, TY, 1);
     const char *w = "reinit_stage_tiled";
     switch (stage) {
-    case 0: return klaunch<KReinit<0>>(r.grid, rblk, s, r.a, w);
-    case 1: return tape ? klaunch<KReinit<1, true>>(r.grid, rblk, s, r.a, w) : klaunch<KReinit<1>>(r.grid, rblk, s, r.a, w);
-    case 2: return tape ? klaunch<KReinit<2, true>>(r.grid, rblk, s, r.a, w) : klaunch<KReinit<2>>(r.grid, rblk, s, r.a, w);
-    default: return tape ? klaunch<KReinit<3, true>>(r.grid, rblk, s, r.a, w) : klaunch<KReinit<3>>(r.grid, rblk, s, r.a, w);
+    case 0: return klaunch<KReinit<0, false, TY>>(r.grid, rblk, s, r.a, w);
+    case 1: return tape ? klaunch<KReinit<1, true, TY>>(r.grid, rblk, s, r.a, w)
+                        : klaunch<KReinit<1, false, TY>>(r.grid, rblk, s, r.a, w);
+    case 2: return tape ? klaunch<KReinit<2, true, TY>>(r.grid, rblk, s, r.a, w)
+                        : klaunch<KReinit<2, false, TY>>(r.grid, rblk, s, r.a, w);
+    default: return tape ? klaunch<KReinit<3, true, TY>>(r.grid, rblk, s, r.a, w)
+                         : klaunch<KReinit<3, false, TY>>(r.grid, rblk, s, r.a, w);
     }
+}
+
+int rt_launch(RTShape &r, int stage, hipStream_t s, bool tape = false)
+{
+    return r.ty == 8 ? rt_launch_ty<8>(r, stage, s, tape) : rt_launch_ty<RTY>(r, stage, s, tape);
 }
 
 int init_ctl(Ctl *ctl, int32_t *iters, hipStream_t s)
@@ -811,7 +828,7 @@ int launch_reinit_eno2_trace(const Blk &b, const LsmReinitParams &p, const doubl
     if (t.nit == 0) return LSM_OK;
     const double *gt = nullptr;
     if ((rc = block_tensor(b, w, s, &gt))) return rc;
-    RTShape r = rt_shape(b, p, phi0, nullptr, ctl, gt);
+    RTShape r = rt_shape(b, p, phi0, nullptr, ctl, gt, 8);
     r.a.din = phi0; r.a.dn = phi0; r.a.dout = nullptr;
     if ((rc = rt_launch(r, 0, s))) return rc;
     const double *sin_[3] = {out, A, B};
