@@ -249,7 +249,6 @@ struct BRArgs {
     const int32_t *sh_block;      // block of every shell workgroup
     int tile_base, sh_base;       // first tile / shell workgroup of this launch
     int ntiles, grid_tiles;       // tiles; tile workgroups (padded to 8), shell workgroups follow
-    const double *m[9];           // batch metric arrays (csi, eta, zet)
     const double *gt;             // metric tensor of every node: component c at gt + c * gs
     long gs;
     const double *din, *dn, *phi0;
@@ -263,9 +262,10 @@ struct BRArgs {
 // |grad d| at an owned node (calc_NormGrad_d.F90:39-2446 on one rank): one-sided
 // formulas on the block's physical faces, ENO2 elsewhere (ghost values in the
 // stencil at internal faces); phi0s is phizero at the block's last interior
-// node, which the i-face formulas use for their upwind sign
+// node, which the i-face formulas use for their upwind sign.  |grad d| from
+// the metric tensor gt (component c at gt + c * gs), as in the interior.
 __device__ double normgrad_node_r(const double *__restrict__ d, const double *__restrict__ phi0,
-                                  const double *const *m, long sj, long sk, const Range &r, const double dd[3],
+                                  const double *gt, long gs, long sj, long sk, const Range &r, const double dd[3],
                                   double phi0s, int order_bc, bool eno_bc, int i, int j, int k, long l)
 {
     const int idx[3] = {i, j, k};
@@ -301,10 +301,10 @@ __device__ double normgrad_node_r(const double *__restrict__ d, const double *__
             gr[a] = dd[a] * eno2(p0, fLL, d[l - s], d[l], d[l + s], fRR, lo_o, hi_o);
         }
     }
-    const double gx = (m[0][l] * gr[0] + m[3][l] * gr[1]) + m[6][l] * gr[2];
-    const double gy = (m[1][l] * gr[0] + m[4][l] * gr[1]) + m[7][l] * gr[2];
-    const double gz = (m[2][l] * gr[0] + m[5][l] * gr[1]) + m[8][l] * gr[2];
-    return sqrt((gx * gx + gy * gy) + gz * gz);
+    double G[6], t[3];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) G[c] = gt[c * gs + l];
+    return rn::tensor_norm(G, gr[0], gr[1], gr[2], t);
 }
 
 // the stage update at one node (reinitialisation_test2.F90:137-181)
@@ -360,11 +360,10 @@ __global__ __launch_bounds__(256) void batch_reinit_stage(BRArgs a)
         if (shell_node(sh, e, i, j, k)) {
             const long l = (long)i + B.sj * j + B.sk * k;
             const double *din = a.din + B.off, *phi0 = a.phi0 + B.off;
-            const double *mm[9];
-            for (int c = 0; c < 9; ++c) mm[c] = a.m[c] + B.off;
             const double dd[3] = {B.dc, B.de, B.dz};
             const double p0s = phi0[(long)B.r.ihi + B.sj * B.r.jhi + B.sk * B.r.khi];
-            const double ngv = normgrad_node_r(din, phi0, mm, B.sj, B.sk, B.r, dd, p0s, a.p.order_bc, a.p.eno_bc != 0,
+            const double ngv = normgrad_node_r(din, phi0, a.gt + B.off, a.gs, B.sj, B.sk, B.r, dd, p0s, a.p.order_bc,
+                                               a.p.eno_bc != 0,
                                                i, j, k, l);
             if (STAGE > 0) {
                 bool upd;
@@ -977,7 +976,6 @@ int batch_reinit(LsmBatch *B, const LsmReinitParams &p, const double *phi0, doub
     if (p.rn_itermax <= 0) return LSM_OK;
     BRArgs a{};
     a.blocks = B->db; a.rtile_block = B->d_rtile_block; a.sh_block = B->d_sh_block;
-    for (int c = 0; c < 9; ++c) a.m[c] = B->met[c];
     a.gt = B->gten; a.gs = B->total;
     a.phi0 = phi0; a.conv = conv; a.ctl = ctl;
     a.p = RStage{0, 0, 0, p.rn_timestep, p.epslsm, p.eps_reinit, p.order_bc, p.eno_bc};

@@ -139,9 +139,11 @@ struct StageP {
 // current plane with its two-node i/j halo sits in LDS, d(k-2..k+2) of the
 // thread's own column in registers, so d is read from HBM once per stage.
 // The boundary shell (every node with an index on a face, whose one-sided
-// and edge/vertex formulas live in normgrad_node) is handled by extra
-// workgroups of the same launch, one node per thread; the two parts write
-// disjoint nodes and read only din, so they need no ordering.
+// and edge/vertex formulas live in normgrad_node_g) is handled by extra
+// workgroups of the same launch, one node per thread, except the i-face
+// nodes off the edges, which the tiles take; the two parts write disjoint
+// nodes and read only din, so they need no ordering.  |grad d| comes from the
+// metric tensor everywhere.
 // ---------------------------------------------------------------------------
 constexpr int RTX = 64, RTY = 4, RH = 2;
 
@@ -163,6 +165,7 @@ struct RTArgs {
     double *tT;
     int tiles_i, tiles_j, kc, ntiles, grid_tiles;   // interior tiles (grid_tiles: padded to 8)
     long nI, nJ, nK;                                 // shell nodes on i, j and k faces
+    int iface;   // the i-face nodes off the edges run in the tiles (the shell keeps the i-face edges)
 };
 
 // the stage update at one node, given |grad d| (reinitialisation_test2.F90:137-181);
@@ -236,8 +239,10 @@ struct KReinit {
     static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
 };
 
+// the plain first stage is held to 80 VGPRs (6 waves per SIMD)
 template <int STAGE, bool TAPE = false, int TY = RTY>
-__global__ __launch_bounds__(RTX * TY) void reinit_stage_tiled(RTArgs a)
+__global__ __launch_bounds__(RTX * TY) __attribute__((amdgpu_waves_per_eu(STAGE == 1 && !TAPE ? 6 : 1)))
+void reinit_stage_tiled(RTArgs a)
 {
     KReinit<STAGE, TAPE, TY>::run(a, (int)blockIdx.x, (int)gridDim.x);
 }
@@ -262,7 +267,13 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE, TY>::run(const RTArgs &a, i
         const long t = (long)(bid - a.grid_tiles) * (RTX * TY) + tid;
         int i = 0, j = 0, k = 0;
         bool ok = true;
-        if (t < a.nI) {
+        if (t < a.nI && a.iface) {
+            // the edges of the two i faces: j = 1 or nj (every k), then k = 1 or nk
+            const long per = 2L * g.nk + 2L * (g.nj - 2), u = t % per;
+            i = (t < per) ? 0 : g.ni - 1;
+            if (u < 2L * g.nk) { j = (u < g.nk) ? 0 : g.nj - 1; k = (int)(u % g.nk); }
+            else { const long v = u - 2L * g.nk; j = 1 + (int)(v % (g.nj - 2)); k = (v < g.nj - 2) ? 0 : g.nk - 1; }
+        } else if (t < a.nI) {
             const long per = (long)g.nj * g.nk;
             j = (int)((t % per) % g.nj); k = (int)((t % per) / g.nj);
             i = (t < per) ? 0 : g.ni - 1;
@@ -282,9 +293,10 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE, TY>::run(const RTArgs &a, i
             const double p0s = a.phi0[(long)(g.ni - 2) + g.sj * (g.nj - 2) + g.sk * (g.nk - 2)];
             double tw[3];
             int tbr = 0;
-            const double ngv = TAPE ? rn::normgrad_branch(g, a.din, a.phi0, p0s, a.p.order_bc, a.p.eno_bc != 0, i, j,
-                                                          k, l, tw, tbr)
-                                    : normgrad_node(g, a.din, a.phi0, p0s, a.p.order_bc, a.p.eno_bc != 0, i, j, k, l);
+            const double ngv = TAPE ? rn::normgrad_branch(g, a.din, a.phi0, a.gt, a.gs, p0s, a.p.order_bc,
+                                                          a.p.eno_bc != 0, i, j, k, l, tw, tbr)
+                                    : rn::normgrad_node_g(g, a.din, a.phi0, a.gt, a.gs, p0s, a.p.order_bc,
+                                                          a.p.eno_bc != 0, i, j, k, l);
             if (STAGE > 0) {
                 bool upd;
                 if (STAGE == 1) { upd = ngv - 1.0 > a.p.eps_r; a.conv[l] = upd ? 1 : 0; }
@@ -331,8 +343,16 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE, TY>::run(const RTArgs &a, i
     const int hi_j = min(tj * TY + hr, g.nj - 1);
     const long col_ih = (long)hi_i + g.sj * hi_j;
     auto kcl = [&](int kk) -> long { return g.sk * (long)min(max(kk, 0), g.nk - 1); };
-    const bool act = live && i >= 1 && i <= g.ni - 2 && j >= 1 && j <= g.nj - 2;
+    // Nodes on an i face off its edges run in the tiles as well, with the
+    // one-sided i formula of the shell (calc_NormGrad_d.F90:187-367) from the
+    // tile's row in LDS and the ENO2 upwind sign of phizero at the last
+    // interior node: their cache lines are the ones the tile loads anyway,
+    // where one node per thread in the shell took a line per node and field.
+    const bool fi = a.iface && live && (i == 0 || i == g.ni - 1) && j >= 1 && j <= g.nj - 2;
+    const bool act = (live && i >= 1 && i <= g.ni - 2 && j >= 1 && j <= g.nj - 2) || fi;
     const bool ilo = i == 1, ihi = i == g.ni - 2, jlo = j == 1, jhi = j == g.nj - 2;
+    const int fdir = (i == 0) ? 1 : -1;
+    const double p0s = a.iface ? a.phi0[(long)(g.ni - 2) + g.sj * (g.nj - 2) + g.sk * (g.nk - 2)] : 0.0;
     const double *__restrict__ d = a.din;
 
     double q0 = d[col + kcl(kb - 2)], q1 = d[col + kcl(kb - 1)];
@@ -356,6 +376,7 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE, TY>::run(const RTArgs &a, i
         if (act) {
             const long l = col + g.sk * (long)k;
             const double p0 = a.phi0[l];
+            const double pe = fi ? p0s : p0;   // ENO2 upwind sign
             double G[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) G[c] = __builtin_nontemporal_load(a.gt + c * a.gs + l);
@@ -365,12 +386,15 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE, TY>::run(const RTArgs &a, i
             const double (*pl)[RTX + 2 * RH] = sm[buf];
             const int cx = tx + RH, cy = ty + RH;
             // calc_NormGrad_d.F90:70-163 (ENO2 in each direction, upwind side by phi0)
-            const double g0 = g.dd[0] * eno2(p0, ilo ? 0.0 : pl[cy][cx - 2], pl[cy][cx - 1], q2,
-                                             pl[cy][cx + 1], ihi ? 0.0 : pl[cy][cx + 2], ilo, ihi);
-            const double g1 = g.dd[1] * eno2(p0, jlo ? 0.0 : pl[cy - 2][cx], pl[cy - 1][cx], q2,
+            double g0;
+            if (fi) g0 = g.dd[0] * biased(q2, pl[cy][cx + fdir], pl[cy][cx + 2 * fdir], pl[cy][cx + 3 * fdir],
+                                          a.p.order_bc, fdir, a.p.eno_bc != 0);
+            else g0 = g.dd[0] * eno2(pe, ilo ? 0.0 : pl[cy][cx - 2], pl[cy][cx - 1], q2, pl[cy][cx + 1],
+                                     ihi ? 0.0 : pl[cy][cx + 2], ilo, ihi);
+            const double g1 = g.dd[1] * eno2(pe, jlo ? 0.0 : pl[cy - 2][cx], pl[cy - 1][cx], q2,
                                              pl[cy + 1][cx], jhi ? 0.0 : pl[cy + 2][cx], jlo, jhi);
             const bool klo = k == 1, khi = k == g.nk - 2;
-            const double g2 = g.dd[2] * eno2(p0, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi);
+            const double g2 = g.dd[2] * eno2(pe, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi);
             double t[3];
             const double ngv = rn::tensor_norm(G, g0, g1, g2, t);
             if (STAGE > 0) {
@@ -380,11 +404,15 @@ __device__ __forceinline__ void KReinit<STAGE, TAPE, TY>::run(const RTArgs &a, i
                 double sv, del;
                 __builtin_nontemporal_store(reinit_update<STAGE>(a.p, ngv, q2, dnv, p0, upd, sv, del), a.dout + l);
                 if constexpr (TAPE) {
-                    const int br = eno2_branch(p0, ilo ? 0.0 : pl[cy][cx - 2], pl[cy][cx - 1], q2, pl[cy][cx + 1],
-                                               ihi ? 0.0 : pl[cy][cx + 2], ilo, ihi) |
-                                   (eno2_branch(p0, jlo ? 0.0 : pl[cy - 2][cx], pl[cy - 1][cx], q2, pl[cy + 1][cx],
+                    int b0 = 0;
+                    if (!fi) b0 = eno2_branch(pe, ilo ? 0.0 : pl[cy][cx - 2], pl[cy][cx - 1], q2, pl[cy][cx + 1],
+                                              ihi ? 0.0 : pl[cy][cx + 2], ilo, ihi);
+                    else if (a.p.order_bc == 2 && a.p.eno_bc)
+                        b0 = eno2_boundary_branch(q2, pl[cy][cx + fdir], pl[cy][cx + 2 * fdir], fdir);
+                    const int br = b0 |
+                                   (eno2_branch(pe, jlo ? 0.0 : pl[cy - 2][cx], pl[cy - 1][cx], q2, pl[cy + 1][cx],
                                                 jhi ? 0.0 : pl[cy + 2][cx], jlo, jhi) << 4) |
-                                   (eno2_branch(p0, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi) << 8);
+                                   (eno2_branch(pe, klo ? 0.0 : q0, q1, q2, q3, khi ? 0.0 : q4, klo, khi) << 8);
                     // 1 / |grad d|: hardware reciprocal and two Newton steps
                     double inv = __builtin_amdgcn_rcp(ngv);
                     inv = fma(inv, fma(-ngv, inv, 1.0), inv);
@@ -697,7 +725,10 @@ RTShape rt_shape(const Blk &b, const LsmReinitParams &p, const double *phi0, uin
     ra.kc = kc;
     ra.ntiles = tij * ((nk_int + kc - 1) / kc);
     ra.grid_tiles = (ra.ntiles + 7) / 8 * 8;
-    ra.nI = 2L * b.e.nj * b.e.nk;
+    // i-face nodes off the edges in the tiles where the one-sided i stencil of
+    // i = ni stays inside its tile's row and halo (not in lane 0 of a tile)
+    ra.iface = b.e.ni >= 5 && b.e.nj >= 3 && b.e.nk >= 3 && (b.e.ni - 1) % RTX != 0;
+    ra.nI = ra.iface ? 2L * (2L * b.e.nk + 2L * (b.e.nj - 2)) : 2L * b.e.nj * b.e.nk;
     ra.nJ = 2L * (b.e.ni - 2) * b.e.nk;
     ra.nK = 2L * (b.e.ni - 2) * (b.e.nj - 2);
     const long wg = (long)RTX * r.ty;

@@ -116,18 +116,36 @@ __device__ inline double normgrad_node(const Geo &g, const double *__restrict__ 
     return sqrt((gx * gx + gy * gy) + gz * gz);
 }
 
+// normgrad_node with |grad d|^2 = g^T G g from the metric tensor gt of the
+// node (component c at gt + c * gs), the form of the reinitialisation stages
+__device__ inline double normgrad_node_g(const Geo &g, const double *__restrict__ d,
+                                         const double *__restrict__ phi0, const double *gt, long gs, double phi0s,
+                                         int order_bc, bool eno_bc, int i, int j, int k, long l)
+{
+    bool on[3];
+    on[0] = (i == 0 || i == g.ni - 1);
+    on[1] = (j == 0 || j == g.nj - 1);
+    on[2] = (k == 0 || k == g.nk - 1);
+    const double p0 = (on[0] && !on[1] && !on[2]) ? phi0s : phi0[l];
+    double gr[3], G[6], t[3];
+    curv_derivs(g, d, p0, order_bc, eno_bc, i, j, k, l, on, gr);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) G[c] = gt[c * gs + l];
+    return tensor_norm(G, gr[0], gr[1], gr[2], t);
+}
+
 // branch codes of the adjoint: bits 0-3, 4-7, 8-11 the difference formula of
 // each direction, kLive: the node's update depends on its stencil, kHasT: it
 // depends on s(phi0) (inside the interface band)
 constexpr uint16_t kLive = 0x8000;
 constexpr uint16_t kHasT = 0x4000;
 
-// |grad d| at node (i,j,k) (the formula of normgrad_node), the direction
+// |grad d| at node (i,j,k) (the formula of normgrad_node_g), the direction
 // weights w_d = (grad d . grad xi_d / |grad d|) x spacing and the branch codes
 // of the three difference formulas (4 bits each)
 __device__ inline double normgrad_branch(const Geo &g, const double *__restrict__ d, const double *__restrict__ phi0,
-                                  double phi0s, int order_bc, bool eno_bc, int i, int j, int k, long l,
-                                  double w[3], int &code)
+                                  const double *gt, long gs, double phi0s, int order_bc, bool eno_bc, int i, int j,
+                                  int k, long l, double w[3], int &code)
 {
     const int n[3] = {g.ni, g.nj, g.nk};
     const int idx[3] = {i, j, k};
@@ -164,14 +182,13 @@ __device__ inline double normgrad_branch(const Geo &g, const double *__restrict_
         }
         code |= br << (4 * a);
     }
-    const double gx = (g.m[0][l] * gr[0] + g.m[3][l] * gr[1]) + g.m[6][l] * gr[2];
-    const double gy = (g.m[1][l] * gr[0] + g.m[4][l] * gr[1]) + g.m[7][l] * gr[2];
-    const double gz = (g.m[2][l] * gr[0] + g.m[5][l] * gr[1]) + g.m[8][l] * gr[2];
-    const double ng = sqrt((gx * gx + gy * gy) + gz * gz);
+    double G[6], t[3];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) G[c] = gt[c * gs + l];
+    const double ng = tensor_norm(G, gr[0], gr[1], gr[2], t);   // t_a = grad d . grad xi_a
     const double inv = ng > 0.0 ? 1.0 / ng : 0.0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-        w[a] = ((gx * g.m[3 * a][l] + gy * g.m[3 * a + 1][l]) + gz * g.m[3 * a + 2][l]) * inv * scale[a];
+    for (int a = 0; a < 3; ++a) w[a] = t[a] * inv * scale[a];
     return ng;
 }
 

@@ -33,7 +33,7 @@ from lsm_amd import inputs  # noqa: E402
 # at most 7 nodes, so nodes receive the one-sided terms of both faces of an axis
 CASES = [((12, 10, 9), 1, False, 4), ((11, 9, 8), 2, True, 3), ((10, 12, 11), 3, False, 4),
          ((9, 8, 10), 2, False, 2), ((133, 9, 10), 1, False, 3), ((70, 12, 11), 2, True, 2),
-         ((7, 6, 7), 2, True, 3), ((6, 7, 5), 3, False, 2)]
+         ((7, 6, 7), 2, True, 3), ((6, 7, 5), 3, False, 2), ((66, 7, 6), 2, True, 3), ((65, 8, 7), 1, False, 2)]
 
 
 def _field(shape, seed=81):

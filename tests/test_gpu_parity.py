@@ -101,6 +101,27 @@ def test_reinit_eno2(curv, order):
     assert int(host(it)[0]) == it_ref
 
 
+@pytest.mark.parametrize("shape,order,eno", [((65, 9, 8), 2, True), ((66, 7, 9), 1, False),
+                                              ((67, 6, 7), 2, True), ((130, 5, 6), 3, False),
+                                              ((4, 9, 7), 2, False)])
+def test_reinit_ragged_extents(shape, order, eno):
+    """Extents where the i = ni face node falls in lane 0, 1 or 2 of a tile (the
+    first one keeps the i faces in the boundary shell; the other two take the
+    one-sided i stencil from the tile's halo columns), a two-tile row and a
+    block too narrow for the tiles' i faces match the oracle's
+    reinitialisation for every boundary order (1e-13: device sin/cos)."""
+    ni, nj, nk = shape
+    g = inputs.sinusoidal_cube_grid(ni, amp=0.03, seed=91, nj=nj, nk=nk)
+    phi = inputs.spheres_sdf(g, nspheres=3, seed=92, rmin=0.2, rmax=0.35) * (0.5 + (g.x - 0.3) ** 2)
+    lo, hi = inputs.spacing_bounds(g)
+    prm = api.ReinitParams(epslsm=1.5 * hi, RNtimestep=0.5 * lo, RNitermax=4, epsReinitialisation=1e-12,
+                           OrderReinitialisationBoundaries=order, ENOBCReinitialisation=eno)
+    out, it = api.reinitialisation_test2(api.Block(g), dev(phi), prm)
+    ref, it_ref = O.reinit_eno2(O.Block(g), phi, 1.5 * hi, 1e-12, 0.5 * lo, 4, order, eno)
+    assert int(host(it)[0]) == it_ref
+    assert linf_rel(host(out), ref) <= 1e-13, (shape, linf_rel(host(out), ref))
+
+
 @pytest.mark.parametrize("m", [0, 1])
 def test_sign_ini(curv, m):
     fx, blk, _ = curv
