@@ -163,7 +163,13 @@ __device__ __forceinline__ void advect_tile(const StageArgs &a, int tile)
                 if (FOLD) v = (a.c1 * pnv + a.c2 * q2) - a.c3 * rh;
                 else v = (a.c1 * pnv + a.c2 * q2) - ((a.c3 * ajv) * a.dt) * rh;
             }
-            stnt(a.out + col + a.sk * (long)k, v);
+            // the two layers next to a physical face are read back by the
+            // boundary shell right after the stage: cached stores there
+            const bool nearf = (a.r.fb && i <= a.r.ilo + 1) || (a.r.ff && i >= a.r.ihi - 1) ||
+                               (a.r.fl && j <= a.r.jlo + 1) || (a.r.fr && j >= a.r.jhi - 1) ||
+                               (a.r.fd && k <= a.r.klo + 1) || (a.r.fu && k >= a.r.khi - 1);
+            if (nearf) a.out[col + a.sk * (long)k] = v;
+            else stnt(a.out + col + a.sk * (long)k, v);
         }
         q0 = q1; q1 = q2; q2 = q3; q3 = q4;
         buf ^= 1;
