@@ -22,7 +22,9 @@ namespace lsm {
 
 namespace {
 
-template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false>
+// NEARC: cached stores next to the physical faces (single blocks; the
+// per-block form of a batch launches the variant without them)
+template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false, bool NEARC = true>
 struct KAdvect {
     using Args = StageArgs;
     static constexpr int kThreads = TX * TY;
@@ -30,21 +32,21 @@ struct KAdvect {
     {
         const int tile = xcd_tile(bid, nb);
         if (tile >= a.ntiles) return;
-        advect_tile<FAST, MASK, TY, NT, FOLD>(a, tile);
+        advect_tile<FAST, MASK, TY, NT, FOLD, TX, NEARC>(a, tile);
     }
     static void one(unsigned g, dim3 b, hipStream_t s, const Args &a);
 };
 
-template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false>
+template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false, bool NEARC = true>
 __global__ __launch_bounds__(TX * TY) void advect_stage(StageArgs a)
 {
-    KAdvect<FAST, MASK, TY, NT, FOLD>::run(a, (int)blockIdx.x, (int)gridDim.x);
+    KAdvect<FAST, MASK, TY, NT, FOLD, NEARC>::run(a, (int)blockIdx.x, (int)gridDim.x);
 }
 
-template <bool FAST, bool MASK, int TY, bool NT, bool FOLD>
-void KAdvect<FAST, MASK, TY, NT, FOLD>::one(unsigned g, dim3 b, hipStream_t s, const Args &a)
+template <bool FAST, bool MASK, int TY, bool NT, bool FOLD, bool NEARC>
+void KAdvect<FAST, MASK, TY, NT, FOLD, NEARC>::one(unsigned g, dim3 b, hipStream_t s, const Args &a)
 {
-    advect_stage<FAST, MASK, TY, NT, FOLD><<<g, b, 0, s>>>(a);
+    advect_stage<FAST, MASK, TY, NT, FOLD, NEARC><<<g, b, 0, s>>>(a);
 }
 
 template <bool MASK>
@@ -365,11 +367,11 @@ int launch_stage_block(const StageArgs &a, int ty, bool fast, hipStream_t s)
     const unsigned grid = (unsigned)((a.ntiles + 7) / 8 * 8);
     const dim3 blk(TX, ty, 1);
     if (ty == 8) {
-        if (fast) advect_stage<true, false, 8><<<grid, blk, 0, s>>>(a);
-        else advect_stage<false, false, 8><<<grid, blk, 0, s>>>(a);
+        if (fast) advect_stage<true, false, 8, true, false, false><<<grid, blk, 0, s>>>(a);
+        else advect_stage<false, false, 8, true, false, false><<<grid, blk, 0, s>>>(a);
     } else {
-        if (fast) advect_stage<true, false, 4><<<grid, blk, 0, s>>>(a);
-        else advect_stage<false, false, 4><<<grid, blk, 0, s>>>(a);
+        if (fast) advect_stage<true, false, 4, true, false, false><<<grid, blk, 0, s>>>(a);
+        else advect_stage<false, false, 4, true, false, false><<<grid, blk, 0, s>>>(a);
     }
     return check_hip(hipGetLastError(), "advect_stage");
 }

@@ -71,7 +71,10 @@ __device__ __forceinline__ void stnt_(double *p, double v)
 // up a chunk of kc planes.  W = 64: one wavefront per j row (blockDim 64 x
 // TY); W = 32: each wavefront covers two half rows (blockDim 64 x TY/2), for
 // blocks whose rows are not much longer than a multiple of 32
-template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false, int W = TX>
+// NEARC: cached stores for the two node layers next to a physical face (single
+// blocks, whose boundary shell reads them right after the stage; in the
+// batched forms they displaced the halo lines the tiles reuse)
+template <bool FAST, bool MASK, int TY, bool NT = true, bool FOLD = false, int W = TX, bool NEARC = false>
 __device__ __forceinline__ void advect_tile(const StageArgs &a, int tile)
 {
     const int ti = tile % a.tiles_i;
@@ -165,9 +168,9 @@ __device__ __forceinline__ void advect_tile(const StageArgs &a, int tile)
             }
             // the two layers next to a physical face are read back by the
             // boundary shell right after the stage: cached stores there
-            const bool nearf = (a.r.fb && i <= a.r.ilo + 1) || (a.r.ff && i >= a.r.ihi - 1) ||
+            const bool nearf = NEARC && ((a.r.fb && i <= a.r.ilo + 1) || (a.r.ff && i >= a.r.ihi - 1) ||
                                (a.r.fl && j <= a.r.jlo + 1) || (a.r.fr && j >= a.r.jhi - 1) ||
-                               (a.r.fd && k <= a.r.klo + 1) || (a.r.fu && k >= a.r.khi - 1);
+                               (a.r.fd && k <= a.r.klo + 1) || (a.r.fu && k >= a.r.khi - 1));
             if (nearf) a.out[col + a.sk * (long)k] = v;
             else stnt(a.out + col + a.sk * (long)k, v);
         }
